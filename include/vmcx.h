@@ -1,0 +1,133 @@
+/*
+ * vmcx.h — C ABI of libvmcx.so, the MI355X (gfx950) kernels of the VMC inner
+ * loop for the spinful Fermi-Hubbard model.
+ *
+ * Conventions
+ *   - Every pointer argument is a DEVICE pointer owned by the caller (for
+ *     example the data_ptr() of a torch tensor), unless stated otherwise.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Entry points only enqueue work; nothing synchronises the host.
+ *   - Return value: VMCX_OK (0) or an error code; vmcx_last_error() returns
+ *     the message of the last failure on the calling thread.
+ *   - Configurations use the quimb site code {0 empty, 1 down, 2 up, 3 both},
+ *     int64, row-major [B][n_sites]; site index x*Ly + y.
+ *   - Orbitals use the netket order: up orbital of site i = i, down = N + i.
+ *   - A Slater walker state is: configs, the inverse of the occupied-row
+ *     matrix A = M[occ] (ainv, [B][NFP][NFP], column k of A^-1 stored
+ *     contiguously at ainv + (b*NFP + k)*NFP), slot ([B][2N] uint8: the row
+ *     of A that holds orbital i, 0xFF if i is empty), sign and ln|psi|
+ *     ([B] float64) and the number of accepted moves since the last exact
+ *     refresh ([B] int32).  NFP = vmcx_slater_nfp(n_f).
+ *   - Limits: n_sites <= 64, n_f <= 64, 2*n_sites*NFP*8 bytes of M fit LDS.
+ */
+#ifndef VMCX_H
+#define VMCX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VMCX_OK 0
+#define VMCX_EINVAL 1
+#define VMCX_EHIP 2
+#define VMCX_ELIB 3
+
+int vmcx_abi_version(void);
+const char *vmcx_last_error(void);
+
+/* Padded width (8, 16, 32 or 64) used for A^-1 storage; -1 if n_f > 64. */
+int vmcx_slater_nfp(int n_f);
+
+/*
+ * Exact (re)build of the Slater walker state from configs and M, by an
+ * in-register Gauss-Jordan inversion with partial pivoting (FP64).
+ *   ln|psi| and sign of psi(x) = det(M[occ ascending, :]).
+ *   ainv / slot / since_refresh may be NULL: then only sign and logabs are
+ *   written (batched log-amplitude of arbitrary configurations).
+ *   only_if_since >= 0: rebuild only walkers with since_refresh >= it.
+ * Replaces: SlaterDeterminant_GPU.amplitude + _base.log_amplitude
+ *   (GPU/models/slater.py:36-61, GPU/models/_base.py:101-109) and the
+ *   forward_log calls of GPU/sampler.py:239 and GPU/vmc_utils.py:407-432.
+ */
+int vmcx_slater_refresh(int64_t B, int n_sites, int n_f, const double *M,
+                        const int64_t *configs, double *ainv, uint8_t *slot,
+                        double *sign, double *logabs, int32_t *since_refresh,
+                        int only_if_since, void *stream);
+
+/*
+ * n_sweeps Metropolis sweeps over the edge list (exchange / hop proposals,
+ * Sherman-Morrison rank-1 / rank-2 ratio and inverse updates).
+ *   edges: [n_edges][2] int32, sweep order (row edges then column edges).
+ *   Draws: if u_choice is NULL, Philox4x32-10 with key = seed and counter
+ *   (edge, sweep_offset + sweep, walker, 0); else injected arrays of shape
+ *   [n_sweeps][n_edges][B]: u_choice, u_acc (float64), bits (uint8,
+ *   bit0 = the (0,3)/(3,0) choice, bit1 = the (1,2)/(2,1) choice).
+ *   n_accept ([B] int64, may be NULL) is incremented per accepted move.
+ *   refresh_every > 0: after each sweep, walkers with since_refresh >=
+ *   refresh_every are rebuilt exactly (vmcx_slater_refresh semantics).
+ * Replaces: MetropolisExchangeSpinfulSamplerGPU.step / burn_in
+ *   (GPU/sampler.py:175-197, 216-388) with propose_exchange_or_hopping_vec
+ *   (GPU/sampler.py:47-137).
+ */
+int vmcx_slater_sweep(int64_t B, int n_sites, int n_f, const double *M,
+                      int n_edges, const int32_t *edges, double hopping_rate,
+                      int n_sweeps, uint64_t seed, int64_t sweep_offset,
+                      const double *u_choice, const uint8_t *bits,
+                      const double *u_acc, int refresh_every,
+                      int64_t *configs, double *ainv, uint8_t *slot,
+                      double *sign, double *logabs, int32_t *since_refresh,
+                      int64_t *n_accept, void *stream);
+
+/*
+ * Hubbard local energy of every walker from its Slater state:
+ *   E_loc = U * n_double + sum_{hops} (-t) * s_JW * psi(eta)/psi(x),
+ *   s_JW = the symmray-order Jordan-Wigner sign times the netket row
+ *   reordering sign, both popcount parities; psi(eta)/psi(x) from A^-1.
+ *   edges: [n_edges][2] int32 (any order).
+ * Replaces: _evaluate_energy_impl (GPU/vmc_utils.py:322-476) with
+ *   get_conn_batch_gpu (hamiltonian_torch.py:586-655).
+ */
+int vmcx_slater_local_energy(int64_t B, int n_sites, int n_f, const double *M,
+                             int n_edges, const int32_t *edges, double t,
+                             double U, const int64_t *configs,
+                             const double *ainv, const uint8_t *slot,
+                             double *eloc, void *stream);
+
+/*
+ * Per-sample log-derivatives O[b][i*n_f + j] = d ln|psi| / d M[i][j]
+ * = A^-1[j][slot(i)] for occupied i, else 0.  o_dtype: 0 = float64,
+ * 1 = float32.  ld = row stride of O in elements (>= 2*n_sites*n_f).
+ * Replaces: compute_grads_gpu with use_log_amp=True for the Slater model
+ *   (GPU/vmc_utils.py:1389-1634).
+ */
+int vmcx_slater_logderiv(int64_t B, int n_sites, int n_f, const double *ainv,
+                         const uint8_t *slot, void *O, int o_dtype, int64_t ld,
+                         void *stream);
+
+/*
+ * Batched connected configurations of the Hubbard Hamiltonian in the
+ * reference's term-major order: terms = for each edge (+1 spin, -1 spin)
+ * then one on-site term per site.
+ *   Pass 1 (vmcx_hubbard_conn_count): counts[n_terms] int64.
+ *   Pass 2 (vmcx_hubbard_conn_fill): offsets[n_terms] int64 = exclusive
+ *   scan of counts; writes etas [T][n_sites] int64, coeffs [T] float64,
+ *   batch_ids [T] int64.
+ * Replaces: SpinfulFermionGPUMixin.get_conn_batch_gpu
+ *   (hamiltonian_torch.py:586-655).
+ */
+int vmcx_hubbard_conn_count(int64_t B, int n_sites, int n_edges,
+                            const int32_t *edges, const int64_t *configs,
+                            int64_t *counts, void *stream);
+int vmcx_hubbard_conn_fill(int64_t B, int n_sites, int n_edges,
+                           const int32_t *edges, double t, double U,
+                           const int64_t *configs, const int64_t *offsets,
+                           int64_t *etas, double *coeffs, int64_t *batch_ids,
+                           void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VMCX_H */

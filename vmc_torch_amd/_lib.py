@@ -1,0 +1,98 @@
+"""ctypes binding of libvmcx.so (the HIP kernels) and tensor-argument checks.
+
+The library is loaded from this package directory only.  There is no CPU
+fallback: if the library is missing, or a tensor is not on a HIP device,
+every entry point raises.
+"""
+import ctypes
+import os
+import re
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvmcx.so")
+HEADER = os.path.join(_HERE, "..", "include", "vmcx.h")
+
+_c = ctypes
+_P = _c.c_void_p
+_I64 = _c.c_int64
+_I = _c.c_int
+_D = _c.c_double
+_U64 = _c.c_uint64
+
+_SIGNATURES = {
+    "vmcx_abi_version": (_I, []),
+    "vmcx_last_error": (_c.c_char_p, []),
+    "vmcx_slater_nfp": (_I, [_I]),
+    "vmcx_slater_refresh": (_I, [_I64, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P]),
+    "vmcx_slater_sweep": (_I, [_I64, _I, _I, _P, _I, _P, _D, _I, _U64, _I64,
+                               _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "vmcx_slater_local_energy": (_I, [_I64, _I, _I, _P, _I, _P, _D, _D, _P, _P,
+                                      _P, _P, _P]),
+    "vmcx_slater_logderiv": (_I, [_I64, _I, _I, _P, _P, _P, _I, _I64, _P]),
+    "vmcx_hubbard_conn_count": (_I, [_I64, _I, _I, _P, _P, _P, _P]),
+    "vmcx_hubbard_conn_fill": (_I, [_I64, _I, _I, _P, _D, _D, _P, _P, _P, _P,
+                                    _P, _P]),
+}
+
+_LIB = None
+
+
+class VmcxError(RuntimeError):
+    """An entry point of libvmcx returned an error status."""
+
+
+def header_functions():
+    """Names of the functions declared in include/vmcx.h."""
+    with open(HEADER) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(vmcx_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    """Load libvmcx.so once (raises ImportError when it is not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is not built; run `python -m vmc_torch_amd.build`")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().vmcx_last_error().decode(errors="replace")
+        raise VmcxError(f"libvmcx status {rc}: {msg}")
+
+
+def stream_ptr(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def require(t, dtype, name, shape=None, device=None):
+    """Fail loudly unless t is a contiguous HIP tensor of the given dtype."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name} must live on a HIP device (got {t.device}); "
+                         "libvmcx has no CPU path")
+    if device is not None and t.device != device:
+        raise ValueError(f"{name} is on {t.device}, expected {device}")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype} (got {t.dtype})")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+    return t

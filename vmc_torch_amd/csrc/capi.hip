@@ -1,0 +1,120 @@
+// extern "C" entry points of libvmcx.so (declared in include/vmcx.h).
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vmcx.h"
+#include "kernels.h"
+
+namespace vmcx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+size_t max_lds_per_block() {
+  static std::mutex mu;
+  static std::vector<size_t> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 65536;
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+  if (cache[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                              dev) != hipSuccess || v <= 0)
+      v = 65536;
+    cache[dev] = (size_t)v;
+  }
+  return cache[dev];
+}
+
+}  // namespace vmcx
+
+static inline int status_of(int rc) {
+  return rc == 0 ? VMCX_OK : (rc == 1 ? VMCX_EINVAL : VMCX_EHIP);
+}
+
+extern "C" {
+
+int vmcx_abi_version(void) { return 1; }
+
+const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
+
+int vmcx_slater_nfp(int n_f) { return (n_f >= 1) ? vmcx::nfp_of(n_f) : -1; }
+
+int vmcx_slater_refresh(int64_t B, int n_sites, int n_f, const double *M,
+                        const int64_t *configs, double *ainv, uint8_t *slot,
+                        double *sign, double *logabs, int32_t *since_refresh,
+                        int only_if_since, void *stream) {
+  return status_of(vmcx::slater_refresh(B, n_sites, n_f, M, configs, ainv, slot,
+                                        sign, logabs, since_refresh,
+                                        only_if_since, (hipStream_t)stream));
+}
+
+int vmcx_slater_sweep(int64_t B, int n_sites, int n_f, const double *M,
+                      int n_edges, const int32_t *edges, double hopping_rate,
+                      int n_sweeps, uint64_t seed, int64_t sweep_offset,
+                      const double *u_choice, const uint8_t *bits,
+                      const double *u_acc, int refresh_every, int64_t *configs,
+                      double *ainv, uint8_t *slot, double *sign, double *logabs,
+                      int32_t *since_refresh, int64_t *n_accept, void *stream) {
+  vmcx::SweepArgs a;
+  a.B = B;
+  a.n_sites = n_sites;
+  a.n_f = n_f;
+  a.n_edges = n_edges;
+  a.n_sweeps = n_sweeps;
+  a.M = M;
+  a.edges = edges;
+  a.keep_rate = 1.0 - hopping_rate;
+  a.seed = seed;
+  a.sweep_offset = sweep_offset;
+  a.u_choice = u_choice;
+  a.bits = bits;
+  a.u_acc = u_acc;
+  a.configs = configs;
+  a.ainv = ainv;
+  a.slot = slot;
+  a.sign = sign;
+  a.logabs = logabs;
+  a.since_refresh = since_refresh;
+  a.n_accept = n_accept;
+  return status_of(vmcx::slater_sweep(a, refresh_every, (hipStream_t)stream));
+}
+
+int vmcx_slater_local_energy(int64_t B, int n_sites, int n_f, const double *M,
+                             int n_edges, const int32_t *edges, double t,
+                             double U, const int64_t *configs,
+                             const double *ainv, const uint8_t *slot,
+                             double *eloc, void *stream) {
+  return status_of(vmcx::slater_local_energy(B, n_sites, n_f, M, n_edges, edges,
+                                             t, U, configs, ainv, slot, eloc,
+                                             (hipStream_t)stream));
+}
+
+int vmcx_slater_logderiv(int64_t B, int n_sites, int n_f, const double *ainv,
+                         const uint8_t *slot, void *O, int o_dtype, int64_t ld,
+                         void *stream) {
+  return status_of(vmcx::slater_logderiv(B, n_sites, n_f, ainv, slot, O,
+                                         o_dtype, ld, (hipStream_t)stream));
+}
+
+int vmcx_hubbard_conn_count(int64_t B, int n_sites, int n_edges,
+                            const int32_t *edges, const int64_t *configs,
+                            int64_t *counts, void *stream) {
+  return status_of(vmcx::hubbard_conn_count(B, n_sites, n_edges, edges, configs,
+                                            counts, (hipStream_t)stream));
+}
+
+int vmcx_hubbard_conn_fill(int64_t B, int n_sites, int n_edges,
+                           const int32_t *edges, double t, double U,
+                           const int64_t *configs, const int64_t *offsets,
+                           int64_t *etas, double *coeffs, int64_t *batch_ids,
+                           void *stream) {
+  return status_of(vmcx::hubbard_conn_fill(B, n_sites, n_edges, edges, t, U,
+                                           configs, offsets, etas, coeffs,
+                                           batch_ids, (hipStream_t)stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,47 @@
+// Internal (C++) entry points of the kernel files, called by capi.hip.
+#pragma once
+
+#include "common.h"
+
+namespace vmcx {
+
+struct SweepArgs {
+  int64_t B;
+  int n_sites, n_f, n_edges, n_sweeps;
+  const double *M;
+  const int32_t *edges;
+  double keep_rate;  // 1 - hopping_rate
+  uint64_t seed;
+  int64_t sweep_offset;
+  const double *u_choice;
+  const uint8_t *bits;
+  const double *u_acc;
+  int64_t *configs;
+  double *ainv;
+  uint8_t *slot;
+  double *sign, *logabs;
+  int32_t *since_refresh;
+  int64_t *n_accept;
+};
+
+int slater_refresh(int64_t B, int n_sites, int n_f, const double *M,
+                   const int64_t *configs, double *ainv, uint8_t *slot,
+                   double *sign, double *logabs, int32_t *since,
+                   int only_if_since, hipStream_t st);
+int slater_sweep(const SweepArgs &a, int refresh_every, hipStream_t st);
+int slater_local_energy(int64_t B, int n_sites, int n_f, const double *M,
+                        int n_edges, const int32_t *edges, double t, double U,
+                        const int64_t *configs, const double *ainv,
+                        const uint8_t *slot, double *eloc, hipStream_t st);
+int slater_logderiv(int64_t B, int n_sites, int n_f, const double *ainv,
+                    const uint8_t *slot, void *O, int o_dtype, int64_t ld,
+                    hipStream_t st);
+int hubbard_conn_count(int64_t B, int n_sites, int n_edges,
+                       const int32_t *edges, const int64_t *configs,
+                       int64_t *counts, hipStream_t st);
+int hubbard_conn_fill(int64_t B, int n_sites, int n_edges, const int32_t *edges,
+                      double t, double U, const int64_t *configs,
+                      const int64_t *offsets, int64_t *etas, double *coeffs,
+                      int64_t *batch_ids, hipStream_t st);
+
+}  // namespace vmcx

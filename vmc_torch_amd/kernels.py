@@ -1,0 +1,162 @@
+"""Tensor-level wrappers of the libvmcx entry points (one call = one C-ABI
+call, plus argument checks).  The reference-shaped classes in sampler.py,
+models.py, vmc_utils.py and hamiltonian.py are built on these."""
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr, require, stream_ptr
+
+F64, I64, I32, U8 = torch.float64, torch.int64, torch.int32, torch.uint8
+
+
+def slater_nfp(n_f):
+    nfp = lib().vmcx_slater_nfp(int(n_f))
+    if nfp < 0:
+        raise ValueError(f"n_f = {n_f} exceeds the 64-fermion limit of the Slater kernels")
+    return nfp
+
+
+@dataclass
+class SlaterWalkerState:
+    """Device-resident Slater state of a batch of walkers (see vmcx.h)."""
+    ainv: torch.Tensor        # [B, NFP, NFP] float64, column k of A^-1 contiguous
+    slot: torch.Tensor        # [B, 2N] uint8
+    sign: torch.Tensor        # [B] float64
+    logabs: torch.Tensor      # [B] float64
+    since_refresh: torch.Tensor  # [B] int32
+    n_accept: torch.Tensor    # [B] int64
+
+    @classmethod
+    def empty(cls, B, n_sites, n_f, device):
+        nfp = slater_nfp(n_f)
+        return cls(
+            ainv=torch.empty(B, nfp, nfp, dtype=F64, device=device),
+            slot=torch.empty(B, 2 * n_sites, dtype=U8, device=device),
+            sign=torch.empty(B, dtype=F64, device=device),
+            logabs=torch.empty(B, dtype=F64, device=device),
+            since_refresh=torch.zeros(B, dtype=I32, device=device),
+            n_accept=torch.zeros(B, dtype=I64, device=device),
+        )
+
+
+def _check_m(M, n_sites):
+    require(M, F64, "M")
+    if M.dim() != 2 or M.shape[0] != 2 * n_sites:
+        raise ValueError(f"M must be (2*n_sites, n_f) = ({2 * n_sites}, n_f), got {tuple(M.shape)}")
+    return M.shape[1]
+
+
+def _check_configs(configs, device):
+    require(configs, I64, "configs", device=device)
+    if configs.dim() != 2:
+        raise ValueError("configs must be (B, n_sites)")
+    return configs.shape
+
+
+def slater_refresh(M, configs, state=None, only_if_since=-1, sign=None, logabs=None):
+    """Exact rebuild of a walker state (or, with state=None, just the
+    batched (sign, ln|psi|) of arbitrary configurations)."""
+    B, N = _check_configs(configs, M.device)
+    n_f = _check_m(M, N)
+    if state is None:
+        sign = torch.empty(B, dtype=F64, device=M.device) if sign is None else sign
+        logabs = torch.empty(B, dtype=F64, device=M.device) if logabs is None else logabs
+        check(lib().vmcx_slater_refresh(B, N, n_f, ptr(M), ptr(configs), None, None,
+                                        ptr(sign), ptr(logabs), None, -1,
+                                        stream_ptr(M.device)))
+        return sign, logabs
+    nfp = slater_nfp(n_f)
+    require(state.ainv, F64, "ainv", (B, nfp, nfp), M.device)
+    require(state.slot, U8, "slot", (B, 2 * N), M.device)
+    require(state.since_refresh, I32, "since_refresh", (B,), M.device)
+    check(lib().vmcx_slater_refresh(B, N, n_f, ptr(M), ptr(configs), ptr(state.ainv),
+                                    ptr(state.slot), ptr(state.sign), ptr(state.logabs),
+                                    ptr(state.since_refresh), int(only_if_since),
+                                    stream_ptr(M.device)))
+    return state.sign, state.logabs
+
+
+def slater_sweep(M, edges, configs, state, n_sweeps=1, seed=0, sweep_offset=0,
+                 hopping_rate=0.25, refresh_every=64, draws=None):
+    """n_sweeps Metropolis sweeps in place.  draws = (u_choice, bits, u_acc),
+    each (n_sweeps, n_edges, B), replaces the Philox stream."""
+    B, N = _check_configs(configs, M.device)
+    n_f = _check_m(M, N)
+    nfp = slater_nfp(n_f)
+    require(edges, I32, "edges", device=M.device)
+    E = edges.shape[0]
+    for name, t, dt, shp in (("ainv", state.ainv, F64, (B, nfp, nfp)),
+                             ("slot", state.slot, U8, (B, 2 * N)),
+                             ("sign", state.sign, F64, (B,)),
+                             ("logabs", state.logabs, F64, (B,)),
+                             ("since_refresh", state.since_refresh, I32, (B,)),
+                             ("n_accept", state.n_accept, I64, (B,))):
+        require(t, dt, name, shp, M.device)
+    uc = bits = ua = None
+    if draws is not None:
+        uc, bits, ua = draws
+        require(uc, F64, "u_choice", (n_sweeps, E, B), M.device)
+        require(bits, U8, "bits", (n_sweeps, E, B), M.device)
+        require(ua, F64, "u_acc", (n_sweeps, E, B), M.device)
+    check(lib().vmcx_slater_sweep(B, N, n_f, ptr(M), E, ptr(edges), float(hopping_rate),
+                                  int(n_sweeps), int(seed) & (2 ** 64 - 1), int(sweep_offset),
+                                  ptr(uc), ptr(bits), ptr(ua), int(refresh_every),
+                                  ptr(configs), ptr(state.ainv), ptr(state.slot),
+                                  ptr(state.sign), ptr(state.logabs),
+                                  ptr(state.since_refresh), ptr(state.n_accept),
+                                  stream_ptr(M.device)))
+
+
+def slater_local_energy(M, edges, t, U, configs, state, out=None):
+    B, N = _check_configs(configs, M.device)
+    n_f = _check_m(M, N)
+    require(edges, I32, "edges", device=M.device)
+    out = torch.empty(B, dtype=F64, device=M.device) if out is None else require(out, F64, "eloc", (B,))
+    check(lib().vmcx_slater_local_energy(B, N, n_f, ptr(M), edges.shape[0], ptr(edges),
+                                         float(t), float(U), ptr(configs), ptr(state.ainv),
+                                         ptr(state.slot), ptr(out), stream_ptr(M.device)))
+    return out
+
+
+def slater_logderiv(state, n_sites, n_f, dtype=F64, out=None):
+    B = state.ainv.shape[0]
+    np_ = 2 * n_sites * n_f
+    if out is None:
+        out = torch.empty(B, np_, dtype=dtype, device=state.ainv.device)
+    require(out, dtype, "O")
+    if out.dim() != 2 or out.shape[0] != B or out.shape[1] < np_:
+        raise ValueError("O must be (B, >= 2*n_sites*n_f)")
+    code = {torch.float64: 0, torch.float32: 1}[dtype]
+    check(lib().vmcx_slater_logderiv(B, n_sites, n_f, ptr(state.ainv), ptr(state.slot),
+                                     ptr(out), code, out.stride(0),
+                                     stream_ptr(state.ainv.device)))
+    return out
+
+
+def hubbard_get_conn(configs, edges, t, U):
+    """(conn_etas [T, N] int64, coeffs [T] float64, batch_ids [T] int64),
+    term-major as the reference's get_conn_batch_gpu."""
+    B, N = _check_configs(configs, configs.device)
+    require(edges, I32, "edges", device=configs.device)
+    E = edges.shape[0]
+    n_terms = 2 * E + N
+    dev = configs.device
+    counts = torch.empty(n_terms, dtype=I64, device=dev)
+    s = stream_ptr(dev)
+    check(lib().vmcx_hubbard_conn_count(B, N, E, ptr(edges), ptr(configs), ptr(counts), s))
+    offsets = torch.cumsum(counts, 0) - counts
+    total = int(counts.sum().item())
+    etas = torch.empty(total, N, dtype=I64, device=dev)
+    coeffs = torch.empty(total, dtype=F64, device=dev)
+    bids = torch.empty(total, dtype=I64, device=dev)
+    if total:
+        check(lib().vmcx_hubbard_conn_fill(B, N, E, ptr(edges), float(t), float(U),
+                                           ptr(configs), ptr(offsets.contiguous()),
+                                           ptr(etas), ptr(coeffs), ptr(bids), s))
+    return etas, coeffs, bids
+
+
+__all__ = ["SlaterWalkerState", "slater_nfp", "slater_refresh", "slater_sweep",
+           "slater_local_energy", "slater_logderiv", "hubbard_get_conn", "_lib"]

@@ -1,0 +1,69 @@
+"""Walker initialisation, local energy and per-sample log-derivatives with
+the reference's interface (GPU/vmc_utils.py, GPU/vmc_setup.py)."""
+import torch
+
+from . import kernels
+
+
+def random_initial_config(N_f, N_sites, seed=None):
+    """GPU/vmc_utils.py:22-39: pattern [1, 2] * (N/2), the first N - N_f
+    sites emptied, then a random permutation (torch CPU generator)."""
+    if seed is not None:
+        torch.manual_seed(seed)
+    cfg = torch.tensor([1, 2] * (N_sites // 2), dtype=torch.int64)
+    cfg[: N_sites - N_f] = 0
+    cfg = cfg[torch.randperm(N_sites)]
+    n1, n2 = int((cfg == 1).sum()), int((cfg == 2).sum())
+    if n1 != N_f // 2 or n2 != N_f // 2:
+        raise AssertionError(f"Number of spin up and spin down fermions should be {N_f // 2}, "
+                             f"but got {n1} and {n2}")
+    return cfg
+
+
+def initialize_walkers(init_fn, batch_size, seed=42, rank=0, device=None):
+    """GPU/vmc_setup.py:139-159: walker i uses seed + rank*B + i."""
+    rows = [torch.as_tensor(init_fn(seed=seed + rank * batch_size + i), dtype=torch.int64)
+            for i in range(batch_size)]
+    return torch.stack(rows).to(device)
+
+
+def _slater_like(model):
+    return hasattr(model, "walker_state")
+
+
+@torch.inference_mode()
+def evaluate_energy(fxs, model, H, current_amps=None, verbose=False, use_log_amp=False, **kwargs):
+    """GPU/vmc_utils.py:479-489: returns (E_mean, E_loc[B]).
+
+    The local energy E_loc(x) = sum_eta H[eta, x] psi(eta)/psi(x) is formed
+    in one HIP launch from the walker state; current_amps is accepted for
+    interface compatibility (the state already carries sign and ln|psi|).
+    """
+    if not _slater_like(model):
+        raise TypeError(f"{type(model).__name__} has no HIP local-energy path")
+    st = model.walker_state(fxs)
+    edges = H.graph.edge_tensor(fxs.device, "graph")
+    eloc = kernels.slater_local_energy(model.M.detach(), edges, H.t, H.U, fxs, st)
+    return eloc.mean(), eloc
+
+
+def compute_grads_gpu(fxs, model, batch_size=None, verbose=False, offload_to_cpu=False,
+                      use_log_amp=False, out_dtype=torch.float64, **kwargs):
+    """GPU/vmc_utils.py:1389-1634: per-sample gradients in the flat
+    parameter order.  With use_log_amp, returns (O = d ln|psi|/d theta,
+    (signs, log_abs)); otherwise (d psi/d theta, psi) as the reference, whose
+    caller divides by psi."""
+    if not _slater_like(model):
+        raise TypeError(f"{type(model).__name__} has no HIP log-derivative path")
+    with torch.inference_mode():
+        st = model.walker_state(fxs)
+        O = kernels.slater_logderiv(st, model.n_sites, model.n_fermions, dtype=out_dtype)
+        if use_log_amp:
+            out = O, (st.sign.clone(), st.logabs.clone())
+        else:
+            amps = st.sign * torch.exp(st.logabs)
+            out = O * amps.unsqueeze(1).to(O.dtype), amps
+    if offload_to_cpu:
+        O_cpu = out[0].cpu()
+        return (O_cpu, out[1])
+    return out
