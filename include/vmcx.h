@@ -126,6 +126,32 @@ int vmcx_hubbard_conn_fill(int64_t B, int n_sites, int n_edges,
                            int64_t *etas, double *coeffs, int64_t *batch_ids,
                            void *stream);
 
+/*
+ * Stochastic reconfiguration in sample space (minSR):
+ *   Ot = (O - mean_s O)/sqrt(Ns), eps = (E - E_mean)/sqrt(Ns),
+ *   T = Ot Ot^T + diag_shift I, alpha = T^-1 eps, dp = Ot^T alpha.
+ *   O: [Ns][ldo] float64 row-major (per-sample log-derivatives), E: [Ns].
+ *   energy_mean: the mean used to centre E; NaN = the sample mean.
+ *   gram_dtype: 0 = FP64 Gram (SYRK); 1 = FP32, 2 = BF16 operands with FP32
+ *   accumulation.  T is factorised by FP64 Cholesky; `refine` rounds of
+ *   iterative refinement against the FP64 operator follow.
+ *   workspace: >= vmcx_sr_workspace_bytes(Ns, Np, gram_dtype) device bytes.
+ *   stats (may be NULL): [4] float64 = mean E, var E, ||T alpha - eps||, ||g||
+ *   with g = Ot^T eps the energy gradient.  info (may be NULL): Cholesky
+ *   info (0 = success, k > 0 = leading minor k not positive).
+ * Replaces: minSR_solver_gpu (GPU/optimizer.py:471-614; MinSRGPU.solve
+ *   :873-895), whose LU solve becomes a Cholesky solve of the same SPD system.
+ */
+int64_t vmcx_sr_workspace_bytes(int64_t Ns, int64_t Np, int gram_dtype);
+int vmcx_sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
+                  const double *E, double energy_mean, double diag_shift,
+                  int gram_dtype, int refine, void *workspace, int64_t ws_bytes,
+                  double *dp, double *stats, int32_t *info, void *stream);
+
+/* params[i] -= lr * dp[i].  Replaces: SGDGPU.step (GPU/optimizer.py:144-186). */
+int vmcx_sgd_update(int64_t n, double *params, const double *dp, double lr,
+                    void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvmcx.so")
-SOURCES = ["capi.hip", "slater.hip", "hubbard.hip"]
+SOURCES = ["capi.hip", "slater.hip", "hubbard.hip", "sr.hip", "chol.hip"]
 ARCH = os.environ.get("VMCX_OFFLOAD_ARCH", "gfx950")
 
 
@@ -48,7 +48,7 @@ def build(verbose=True):
         objs.append(obj)
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp,
-           "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
+           "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -117,4 +117,23 @@ int vmcx_hubbard_conn_fill(int64_t B, int n_sites, int n_edges,
                                            batch_ids, (hipStream_t)stream));
 }
 
+int64_t vmcx_sr_workspace_bytes(int64_t Ns, int64_t Np, int gram_dtype) {
+  return vmcx::sr_workspace_bytes(Ns, Np, gram_dtype);
+}
+
+int vmcx_sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
+                  const double *E, double energy_mean, double diag_shift,
+                  int gram_dtype, int refine, void *workspace, int64_t ws_bytes,
+                  double *dp, double *stats, int32_t *info, void *stream) {
+  const int rc = vmcx::sr_minsr(Ns, Np, O, ldo, E, energy_mean, diag_shift, gram_dtype,
+                                refine, workspace, ws_bytes, dp, stats, info,
+                                (hipStream_t)stream);
+  return rc == 3 ? VMCX_ELIB : status_of(rc);
+}
+
+int vmcx_sgd_update(int64_t n, double *params, const double *dp, double lr,
+                    void *stream) {
+  return status_of(vmcx::sgd_update(n, params, dp, lr, (hipStream_t)stream));
+}
+
 }  // extern "C"

@@ -44,4 +44,18 @@ int hubbard_conn_fill(int64_t B, int n_sites, int n_edges, const int32_t *edges,
                       const int64_t *offsets, int64_t *etas, double *coeffs,
                       int64_t *batch_ids, hipStream_t st);
 
+int64_t sr_workspace_bytes(int64_t Ns, int64_t Np, int gram);
+int sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
+             const double *E, double energy_mean, double shift, int gram,
+             int refine, void *ws, int64_t ws_bytes, double *dp, double *stats,
+             int32_t *info, hipStream_t st);
+int sgd_update(int64_t n, double *params, const double *dp, double lr,
+               hipStream_t st);
+int64_t chol_dinv_bytes(int64_t n);
+int chol_factor(void *rocblas_handle_, int64_t n, double *A, int64_t lda,
+                double *dinv, int32_t *info, hipStream_t st);
+int chol_solve(int64_t n, const double *L, int64_t lda, const double *dinv,
+               const double *b, double *x, double *tmp, double *tmp2,
+               hipStream_t st);
+
 }  // namespace vmcx

@@ -160,3 +160,49 @@ def hubbard_get_conn(configs, edges, t, U):
 
 __all__ = ["SlaterWalkerState", "slater_nfp", "slater_refresh", "slater_sweep",
            "slater_local_energy", "slater_logderiv", "hubbard_get_conn", "_lib"]
+
+
+GRAM_CODES = {"fp64": 0, "fp32": 1, "bf16": 2}
+
+
+class SRWorkspace:
+    """Device workspace of vmcx_sr_minsr, grown on demand and reused."""
+
+    def __init__(self):
+        self.buf = None
+
+    def get(self, Ns, Np, gram, device):
+        need = int(lib().vmcx_sr_workspace_bytes(int(Ns), int(Np), GRAM_CODES[gram]))
+        if need < 0:
+            raise ValueError("bad SR sizes")
+        if self.buf is None or self.buf.numel() < need or self.buf.device != device:
+            self.buf = torch.empty(need, dtype=torch.uint8, device=device)
+        return self.buf
+
+
+def sr_minsr(O, E, energy_mean, diag_shift, gram="fp64", refine=0, workspace=None,
+             dp=None, with_stats=True):
+    """Sample-space SR on the device.  Returns (dp [Np] float64, stats [4]
+    float64 device tensor or None, info [1] int32 device tensor)."""
+    require(O, F64, "O")
+    if O.dim() != 2 or O.stride(1) != 1:
+        raise ValueError("O must be a row-major (Ns, Np) float64 matrix")
+    Ns, Np = O.shape
+    require(E, F64, "local_energies", (Ns,), O.device)
+    ws = (workspace or SRWorkspace()).get(Ns, Np, gram, O.device)
+    dp = torch.empty(Np, dtype=F64, device=O.device) if dp is None else require(dp, F64, "dp", (Np,))
+    stats = torch.empty(4, dtype=F64, device=O.device) if with_stats else None
+    info = torch.empty(1, dtype=I32, device=O.device)
+    em = float("nan") if energy_mean is None else float(energy_mean)
+    check(lib().vmcx_sr_minsr(Ns, Np, ptr(O), O.stride(0), ptr(E), em, float(diag_shift),
+                              GRAM_CODES[gram], int(refine), ptr(ws), ws.numel(), ptr(dp),
+                              ptr(stats), ptr(info), stream_ptr(O.device)))
+    return dp, stats, info
+
+
+def sgd_update(param, dp, lr):
+    """param -= lr * dp (in place, flat view)."""
+    require(param, F64, "param")
+    require(dp, F64, "dp", (param.numel(),), param.device)
+    check(lib().vmcx_sgd_update(param.numel(), ptr(param), ptr(dp), float(lr),
+                                stream_ptr(param.device)))

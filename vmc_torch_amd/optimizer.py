@@ -1,0 +1,110 @@
+"""SR preconditioners and parameter updates with the reference's interface
+(GPU/optimizer.py: PreconditionerGPU.solve, OptimizerGPU.step)."""
+import time
+from typing import Any, Optional, Tuple
+
+import torch
+
+from . import kernels
+
+
+class Scheduler:
+    """GPU/optimizer.py:26-127 (callable step -> learning rate)."""
+
+    def __call__(self, step):
+        raise NotImplementedError
+
+
+class TrivialScheduler(Scheduler):
+    def __init__(self, init_lr):
+        self.init_lr = init_lr
+
+    def __call__(self, step):
+        return self.init_lr
+
+
+class DecayScheduler(Scheduler):
+    """lr = init_lr * decay_rate ** (step // patience), floored at min_lr."""
+
+    def __init__(self, init_lr, decay_rate=0.9, patience=50, min_lr=1e-4):
+        self.init_lr, self.decay_rate = init_lr, decay_rate
+        self.patience, self.min_lr = patience, min_lr
+
+    def __call__(self, step):
+        return max(self.init_lr * self.decay_rate ** (step // self.patience), self.min_lr)
+
+
+class OptimizerGPU:
+    """GPU/optimizer.py:130-175: params updated in place, in parameters() order."""
+
+    def __init__(self, learning_rate: float = 1e-3):
+        self.lr = learning_rate
+
+    def step(self, model, direction, device=None, learning_rate: Optional[float] = None):
+        raise NotImplementedError
+
+    def reset(self):
+        pass
+
+
+class SGDGPU(OptimizerGPU):
+    """p <- p - lr * dp (GPU/optimizer.py:178-186) as one HIP axpy per
+    parameter tensor."""
+
+    @torch.no_grad()
+    def step(self, model, direction, device=None, learning_rate=None):
+        lr = self.lr if learning_rate is None else learning_rate
+        params = list(model.parameters())
+        dev = params[0].device
+        d = torch.as_tensor(direction, device=dev, dtype=torch.float64).contiguous()
+        off = 0
+        for p in params:
+            n = p.numel()
+            kernels.sgd_update(p.data.view(-1), d[off:off + n], lr)
+            off += n
+        if off != d.numel():
+            raise ValueError(f"direction has {d.numel()} entries, parameters {off}")
+        if hasattr(model, "invalidate"):
+            model.invalidate()
+
+
+class PreconditionerGPU:
+    """GPU/optimizer.py:236-251."""
+
+    def solve(self, *, local_o, local_energies, energy_mean: float, total_samples: int,
+              n_params: int, diag_shift: float, device, run_sr: bool) -> Tuple[Any, float, Any]:
+        raise NotImplementedError
+
+
+class MinSRGPU(PreconditionerGPU):
+    """Sample-space SR (GPU/optimizer.py:471-614, MinSRGPU at :873-895).
+
+    gram: "fp64" (plain form), "fp32" or "bf16" (Gram operands in reduced
+    precision with FP32 accumulation; FP64 Cholesky, then `refine` rounds of
+    iterative refinement against the FP64 operator).
+    Returns (dp device tensor, seconds, info) with info = 0 on success, or
+    the Cholesky failure index (the reference returns 1 after an lstsq
+    fallback).
+    """
+
+    def __init__(self, gram="fp64", refine=None, sync_info=True):
+        self.gram = gram
+        self.refine = (0 if gram == "fp64" else 3) if refine is None else refine
+        self.sync_info = sync_info
+        self.workspace = kernels.SRWorkspace()
+        self.last_stats = None
+
+    def solve(self, *, local_o, local_energies, energy_mean, total_samples, n_params,
+              diag_shift, device, run_sr=True):
+        t0 = time.time()
+        if not run_sr:
+            raise NotImplementedError("run_sr=False: use the energy gradient from MinSRGPU stats")
+        O = local_o.to(device=device, dtype=torch.float64).contiguous()
+        E = local_energies.to(device=device, dtype=torch.float64).contiguous()
+        if O.shape[0] != total_samples or O.shape[1] != n_params:
+            raise ValueError("local_o must be (total_samples, n_params) on one GPU")
+        dp, stats, info = kernels.sr_minsr(O, E, energy_mean, diag_shift, self.gram,
+                                           self.refine, self.workspace)
+        self.last_stats = stats
+        info_v = int(info.item()) if self.sync_info else info
+        return dp, time.time() - t0, info_v
