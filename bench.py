@@ -183,6 +183,7 @@ def main():
     names = ["refresh", "sweep", "local_energy", "logderiv", "sr", "update"]
     ev = {k: [] for k in names}
     E_hist = []
+    sr_phases = []
 
     def step(record):
         nonlocal sweep_no
@@ -202,12 +203,16 @@ def main():
         m3 = mark()
         kernels.slater_logderiv(st, N, nf, out=O)
         m4 = mark()
+        ph = [] if record == "phases" else None
         _, stats, info = kernels.sr_minsr(O, E, None, a.diag_shift, a.gram,
-                                          0 if a.gram == "fp64" else 3, ws, dp=dp)
+                                          0 if a.gram == "fp64" else 3, ws, dp=dp,
+                                          phase_ms=ph)
+        if ph is not None:
+            sr_phases.append(ph)
         m5 = mark()
         kernels.sgd_update(M.view(-1), dp, a.lr)
         m6 = mark()
-        if record:
+        if record is True:
             for k, (x0, x1) in zip(names, [(m0, m1), (m1, m2), (m2, m3), (m3, m4), (m4, m5),
                                            (m5, m6)]):
                 ev[k].append((x0, x1))
@@ -227,6 +232,14 @@ def main():
     ms_step = 1e3 * wall / a.steps
     info_v = int(info.item())
     st_v = stats.cpu().tolist()
+    # SR phase split, measured on extra (untimed) steps: the phase timer
+    # synchronises the host once per SR call
+    for _ in range(3):
+        step("phases")
+    torch.cuda.synchronize()
+    phase_names = ["center", "gram", "factor", "solve", "dp_stats"]
+    phase_ms = {k: sum(p[i] for p in sr_phases) / len(sr_phases)
+                for i, k in enumerate(phase_names)}
     kt = {k: sum(x0.elapsed_time(x1) for x0, x1 in ev[k]) / a.steps for k in names}  # ms
     acc_rate = float(torch.stack(accs).double().mean()) / e_sweep.shape[0]
     E_steps = torch.stack(E_hist)                        # (steps, B)
@@ -262,15 +275,29 @@ def main():
         kern[k] = {"ms": kt[k], "bound": bound, "achieved": nbytes / secs / 1e9,
                    "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                    "frac": nbytes / secs / PEAK_HBM}
+    peak_sr = {"fp64": PEAK_FP64, "fp32": PEAK_FP32_MFMA, "bf16": PEAK_BF16}[a.gram]
     gram_flops = float(Ns) * Ns * Np                     # SYRK: lower half of Ot Ot^T
     chol_flops = float(Ns) ** 3 / 3
-    sr_flops = gram_flops + chol_flops
-    peak_sr = {"fp64": PEAK_FP64, "fp32": PEAK_FP32_MFMA, "bf16": PEAK_BF16}[a.gram]
-    kern["sr"] = {"ms": kt["sr"], "bound": "mfma",
-                  "achieved": sr_flops / (kt["sr"] * 1e-3) / 1e12,
-                  "peak": peak_sr / 1e12, "unit": "TFLOP/s",
-                  "frac": sr_flops / (kt["sr"] * 1e-3) / peak_sr,
-                  "note": "Gram (Ns^2 Np, SYRK half) + Cholesky (Ns^3/3) over the whole SR call"}
+    kern["sr"] = {"ms": kt["sr"], "phases_ms": phase_ms}
+
+    def flop_roof(name, flops, ms, peak, note):
+        return {"ms": ms, "bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12,
+                "peak": peak / 1e12, "unit": "TFLOP/s", "frac": flops / (ms * 1e-3) / peak,
+                "note": note}
+    kern["sr.gram"] = flop_roof("gram", gram_flops, phase_ms["gram"], peak_sr,
+                                "Ns^2 Np (SYRK lower half), one rocBLAS call")
+    kern["sr.factor"] = flop_roof("factor", chol_flops, phase_ms["factor"], PEAK_FP64,
+                                  "Ns^3/3: blocked Cholesky (LDS diagonal kernel + TRSM/SYRK)")
+    cbytes = 2.0 * Ns * Np * 8 + Np * 8 * 64             # read O twice, write X
+    kern["sr.center"] = {"ms": phase_ms["center"], "bound": "hbm",
+                         "achieved": cbytes / (phase_ms["center"] * 1e-3) / 1e9,
+                         "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                         "frac": cbytes / (phase_ms["center"] * 1e-3) / PEAK_HBM}
+    sbytes = 2.0 * Ns * Ns / 2 * 8 + 2 * Ns * Np * 8     # read L twice, O^T alpha
+    kern["sr.solve"] = {"ms": phase_ms["solve"], "bound": "hbm",
+                        "achieved": sbytes / (phase_ms["solve"] * 1e-3) / 1e9,
+                        "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                        "frac": sbytes / (phase_ms["solve"] * 1e-3) / PEAK_HBM}
     kern["update"] = {"ms": kt["update"]}
     dom = max((k for k in kern if "frac" in kern[k]), key=lambda k: kern[k]["ms"])
     roof = {kk: kern[dom][kk] for kk in ("bound", "achieved", "peak", "unit", "frac")}

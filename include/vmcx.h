@@ -139,6 +139,10 @@ int vmcx_hubbard_conn_fill(int64_t B, int n_sites, int n_edges,
  *   stats (may be NULL): [4] float64 = mean E, var E, ||T alpha - eps||, ||g||
  *   with g = Ot^T eps the energy gradient.  info (may be NULL): Cholesky
  *   info (0 = success, k > 0 = leading minor k not positive).
+ *   phase_ms (HOST pointer, may be NULL): [5] float = milliseconds of the
+ *   centring, Gram, factorisation, solve + refinement and dp / statistics
+ *   phases, timed with HIP events on `stream`; when given, the call waits
+ *   for the stream before returning.
  * Replaces: minSR_solver_gpu (GPU/optimizer.py:471-614; MinSRGPU.solve
  *   :873-895), whose LU solve becomes a Cholesky solve of the same SPD system.
  */
@@ -146,7 +150,8 @@ int64_t vmcx_sr_workspace_bytes(int64_t Ns, int64_t Np, int gram_dtype);
 int vmcx_sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
                   const double *E, double energy_mean, double diag_shift,
                   int gram_dtype, int refine, void *workspace, int64_t ws_bytes,
-                  double *dp, double *stats, int32_t *info, void *stream);
+                  double *dp, double *stats, int32_t *info, float *phase_ms,
+                  void *stream);
 
 /* params[i] -= lr * dp[i].  Replaces: SGDGPU.step (GPU/optimizer.py:144-186). */
 int vmcx_sgd_update(int64_t n, double *params, const double *dp, double lr,

@@ -36,7 +36,7 @@ _SIGNATURES = {
                                     _P, _P]),
     "vmcx_sr_workspace_bytes": (_I64, [_I64, _I64, _I]),
     "vmcx_sr_minsr": (_I, [_I64, _I64, _P, _I64, _P, _D, _D, _I, _I, _P, _I64,
-                           _P, _P, _P, _P]),
+                           _P, _P, _P, _P, _P]),
     "vmcx_sgd_update": (_I, [_I64, _P, _P, _D, _P]),
 }
 

@@ -124,9 +124,10 @@ int64_t vmcx_sr_workspace_bytes(int64_t Ns, int64_t Np, int gram_dtype) {
 int vmcx_sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
                   const double *E, double energy_mean, double diag_shift,
                   int gram_dtype, int refine, void *workspace, int64_t ws_bytes,
-                  double *dp, double *stats, int32_t *info, void *stream) {
+                  double *dp, double *stats, int32_t *info, float *phase_ms,
+                  void *stream) {
   const int rc = vmcx::sr_minsr(Ns, Np, O, ldo, E, energy_mean, diag_shift, gram_dtype,
-                                refine, workspace, ws_bytes, dp, stats, info,
+                                refine, workspace, ws_bytes, dp, stats, info, phase_ms,
                                 (hipStream_t)stream);
   return rc == 3 ? VMCX_ELIB : status_of(rc);
 }

@@ -15,6 +15,10 @@
 //         backward x_k = Dinv_k^T (y_k) then y_head -= L_k,head^T x_k.
 #include <rocblas/rocblas.h>
 
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
 #include "kernels.h"
 
 namespace vmcx {
@@ -265,14 +269,56 @@ int64_t chol_dinv_bytes(int64_t n) {
     }                                                                        \
   } while (0)
 
+// Side stream, rocBLAS handle and event pool for the look-ahead (per device).
+struct LookAhead {
+  hipStream_t side = nullptr;
+  rocblas_handle h2 = nullptr;
+  std::vector<hipEvent_t> ev;
+};
+
+static LookAhead *lookahead_for_device(int nev) {
+  static std::mutex mu;
+  static std::unordered_map<int, LookAhead *> per_dev;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  LookAhead *la = per_dev[dev];
+  if (la == nullptr) {
+    la = new LookAhead();
+    if (hipStreamCreateWithFlags(&la->side, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    if (rocblas_create_handle(&la->h2) != rocblas_status_success) return nullptr;
+    rocblas_set_stream(la->h2, la->side);
+    rocblas_set_pointer_mode(la->h2, rocblas_pointer_mode_host);
+    per_dev[dev] = la;
+  }
+  while ((int)la->ev.size() < nev) {
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    la->ev.push_back(e);
+  }
+  return la;
+}
+
+// Right-looking blocked Cholesky with one-block look-ahead:
+//   main stream : diag_k -> trsm_k -> [wait syrk_{k-1}] -> next-column GEMM_k
+//   side stream : [wait trsm_k] -> SYRK of the trailing matrix minus the next
+//                 block column
+// so the large trailing updates overlap the latency-bound diagonal
+// factorisation and panel solve of the next block.
 int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
                 int32_t *info, hipStream_t st) {
   rocblas_handle h = (rocblas_handle)handle;
   const int64_t nblk = (n + CHOL_NB - 1) / CHOL_NB;
   double *Dinv = dinv, *DinvT = dinv + nblk * CHOL_NB * CHOL_NB;
+  LookAhead *la = lookahead_for_device((int)(2 * nblk + 2));
+  VMCX_CHECK_ARG(la != nullptr, "look-ahead stream / handle creation failed");
   hipLaunchKernelGGL(zero_i32, dim3(1), dim3(1), 0, st, info);
   VMCX_LAUNCHED();
   const double one = 1.0, mone = -1.0;
+  // side stream starts after everything already queued on the main stream
+  VMCX_HIP(hipEventRecord(la->ev[0], st));
+  VMCX_HIP(hipStreamWaitEvent(la->side, la->ev[0], 0));
+  hipEvent_t syrk_done = nullptr;  // SYRK of the previous step (side stream)
   for (int64_t kb = 0; kb < nblk; ++kb) {
     const int64_t k0 = kb * CHOL_NB;
     const int nbk = (int)((n - k0 < CHOL_NB) ? n - k0 : CHOL_NB);
@@ -282,15 +328,35 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
     if (rest <= 0) break;
     double *Lkk = A + k0 * lda + k0;
     double *P = A + k0 * lda + k0 + nbk;
-    double *A22 = A + (k0 + nbk) * lda + k0 + nbk;
     VMCX_BLAS2(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower,
                              rocblas_operation_transpose, rocblas_diagonal_non_unit,
                              (rocblas_int)rest, nbk, &one, Lkk, (rocblas_int)lda, P,
                              (rocblas_int)lda));
-    VMCX_BLAS2(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none,
-                             (rocblas_int)rest, nbk, &mone, P, (rocblas_int)lda, &one, A22,
-                             (rocblas_int)lda));
+    hipEvent_t panel_done = la->ev[1 + 2 * kb];
+    VMCX_HIP(hipEventRecord(panel_done, st));
+    // next block column: A[k1:, k1:k1+nb1] -= P P[0:nb1]^T (main stream)
+    const int64_t k1 = k0 + nbk;
+    const int nb1 = (int)((n - k1 < CHOL_NB) ? n - k1 : CHOL_NB);
+    if (syrk_done) VMCX_HIP(hipStreamWaitEvent(st, syrk_done, 0));
+    VMCX_BLAS2(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose,
+                             (rocblas_int)rest, nb1, nbk, &mone, P, (rocblas_int)lda, P,
+                             (rocblas_int)lda, &one, A + k1 * lda + k1, (rocblas_int)lda));
+    // the rest of the trailing matrix on the side stream
+    const int64_t rest2 = rest - nb1;
+    if (rest2 > 0) {
+      VMCX_HIP(hipStreamWaitEvent(la->side, panel_done, 0));
+      double *P2 = P + nb1;
+      double *A22 = A + (k1 + nb1) * lda + k1 + nb1;
+      VMCX_BLAS2(rocblas_dsyrk(la->h2, rocblas_fill_lower, rocblas_operation_none,
+                               (rocblas_int)rest2, nbk, &mone, P2, (rocblas_int)lda, &one,
+                               A22, (rocblas_int)lda));
+      syrk_done = la->ev[2 + 2 * kb];
+      VMCX_HIP(hipEventRecord(syrk_done, la->side));
+    } else {
+      syrk_done = nullptr;
+    }
   }
+  if (syrk_done) VMCX_HIP(hipStreamWaitEvent(st, syrk_done, 0));
   hipLaunchKernelGGL(chol_diag_inverse_kernel, dim3((unsigned)nblk), dim3(256), 0, st, A, lda,
                      n, Dinv, DinvT);
   VMCX_LAUNCHED();

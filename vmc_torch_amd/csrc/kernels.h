@@ -48,7 +48,7 @@ int64_t sr_workspace_bytes(int64_t Ns, int64_t Np, int gram);
 int sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
              const double *E, double energy_mean, double shift, int gram,
              int refine, void *ws, int64_t ws_bytes, double *dp, double *stats,
-             int32_t *info, hipStream_t st);
+             int32_t *info, float *phase_ms, hipStream_t st);
 int sgd_update(int64_t n, double *params, const double *dp, double lr,
                hipStream_t st);
 int64_t chol_dinv_bytes(int64_t n);

@@ -241,10 +241,35 @@ int64_t sr_workspace_bytes(int64_t Ns, int64_t Np, int gram) {
   return (int64_t)sr_layout(Ns, Np, gram).total;
 }
 
+// Optional phase timing: HIP events on the launch stream; read back with one
+// host synchronisation at the end of the call (only when phase_ms != NULL).
+struct PhaseTimer {
+  hipEvent_t ev[6];
+  int n = 0;
+  bool on;
+  hipStream_t st;
+  PhaseTimer(bool enable, hipStream_t s) : on(enable), st(s) {
+    if (on)
+      for (auto &e : ev) (void)hipEventCreate(&e);
+  }
+  void mark() {
+    if (on && n < 6) (void)hipEventRecord(ev[n++], st);
+  }
+  void read(float *out) {
+    if (!on) return;
+    (void)hipEventSynchronize(ev[n - 1]);
+    for (int i = 0; i + 1 < n; ++i) (void)hipEventElapsedTime(&out[i], ev[i], ev[i + 1]);
+  }
+  ~PhaseTimer() {
+    if (on)
+      for (auto &e : ev) (void)hipEventDestroy(e);
+  }
+};
+
 int sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
              const double *E, double energy_mean, double shift, int gram,
              int refine, void *ws, int64_t ws_bytes, double *dp, double *stats,
-             int32_t *info, hipStream_t st) {
+             int32_t *info, float *phase_ms, hipStream_t st) {
   VMCX_CHECK_ARG(Ns >= 1 && Np >= 1, "Ns and Np must be positive");
   VMCX_CHECK_ARG(Ns <= 0x7fffffff && Np <= 0x7fffffff, "Ns / Np exceed 32-bit BLAS");
   VMCX_CHECK_ARG(ldo >= Np, "ldo < Np");
@@ -266,6 +291,8 @@ int sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
 
   rocblas_handle h = blas_handle_for(st);
   VMCX_CHECK_ARG(h != nullptr, "rocblas_create_handle failed");
+  PhaseTimer timer(phase_ms != nullptr, st);
+  timer.mark();
 
   // column means (two-level compensated sum) and centring
   const int R = 64;
@@ -292,6 +319,7 @@ int sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
                      use_given, scale, eps, stw);
   VMCX_LAUNCHED();
 
+  timer.mark();
   // Gram
   const double one = 1.0, zero = 0.0;
   if (gram == 0) {
@@ -320,6 +348,7 @@ int sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
   double *t1 = (double *)(w + L.t1), *t2 = (double *)(w + L.t2);
   double *corr = (double *)(w + L.corr), *dinv = (double *)(w + L.dinv);
   int32_t *iinfo = (int32_t *)dinfo;
+  timer.mark();
   if (use_rocsolver) {
     VMCX_BLAS(rocsolver_dpotrf(h, rocblas_fill_lower, nsi, T, nsi, dinfo));
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((Ns + 255) / 256)), dim3(256), 0, st,
@@ -329,9 +358,11 @@ int sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
   } else {
     int rc = chol_factor((void *)h, Ns, T, Ns, dinv, iinfo, st);
     if (rc) return rc;
+    timer.mark();
     rc = chol_solve(Ns, T, Ns, dinv, eps, alpha, t1, t2, st);
     if (rc) return rc;
   }
+  if (use_rocsolver) timer.mark();
   // iterative refinement against the FP64 operator  Ot Ot^T + shift I
   for (int it = 0; it < refine; ++it) {
     VMCX_BLAS(rocblas_dgemv(h, rocblas_operation_none, npi, nsi, &one, X, npi, alpha, 1,
@@ -353,6 +384,7 @@ int sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
     }
     VMCX_LAUNCHED();
   }
+  timer.mark();
   // dp = Ot^T alpha
   VMCX_BLAS(rocblas_dgemv(h, rocblas_operation_none, npi, nsi, &one, X, npi, alpha, 1, &zero,
                           dp, 1));
@@ -372,6 +404,8 @@ int sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
     VMCX_HIP(hipMemcpyAsync(stats, stw, 4 * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
   if (info) VMCX_HIP(hipMemcpyAsync(info, dinfo, sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+  timer.mark();
+  timer.read(phase_ms);
   return 0;
 }
 

@@ -181,9 +181,11 @@ class SRWorkspace:
 
 
 def sr_minsr(O, E, energy_mean, diag_shift, gram="fp64", refine=0, workspace=None,
-             dp=None, with_stats=True):
+             dp=None, with_stats=True, phase_ms=None):
     """Sample-space SR on the device.  Returns (dp [Np] float64, stats [4]
-    float64 device tensor or None, info [1] int32 device tensor)."""
+    float64 device tensor or None, info [1] int32 device tensor).
+    phase_ms: optional list, filled with the 5 phase durations (ms; the call
+    then synchronises the stream)."""
     require(O, F64, "O")
     if O.dim() != 2 or O.stride(1) != 1:
         raise ValueError("O must be a row-major (Ns, Np) float64 matrix")
@@ -194,9 +196,15 @@ def sr_minsr(O, E, energy_mean, diag_shift, gram="fp64", refine=0, workspace=Non
     stats = torch.empty(4, dtype=F64, device=O.device) if with_stats else None
     info = torch.empty(1, dtype=I32, device=O.device)
     em = float("nan") if energy_mean is None else float(energy_mean)
+    import ctypes
+    ph = (ctypes.c_float * 5)() if phase_ms is not None else None
     check(lib().vmcx_sr_minsr(Ns, Np, ptr(O), O.stride(0), ptr(E), em, float(diag_shift),
                               GRAM_CODES[gram], int(refine), ptr(ws), ws.numel(), ptr(dp),
-                              ptr(stats), ptr(info), stream_ptr(O.device)))
+                              ptr(stats), ptr(info),
+                              ctypes.cast(ph, ctypes.c_void_p) if ph is not None else None,
+                              stream_ptr(O.device)))
+    if ph is not None:
+        phase_ms[:] = list(ph)
     return dp, stats, info
 
 
