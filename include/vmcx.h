@@ -153,6 +153,73 @@ int vmcx_sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
                   double *dp, double *stats, int32_t *info, float *phase_ms,
                   void *stream);
 
+/*
+ * Neural-network backflow (NNBF) model: two hidden layers of width `hidden`.
+ *   psi(x) = det((M + bf_scale * reshape(Wout act(W2 act(W1 n + b1) + b2)
+ *                 + bout, (2N, n_f)))[occupied rows ascending])
+ * All weights are FP32 device copies in the reference's layouts (row-major
+ * [out][in]); with gemm = 1 the two dense layers use BF16 copies of W2 and
+ * Wout (and BF16 activations) with FP32 accumulation.
+ * Replaces: NNBF_GPU (GPU/models/NNBF.py:36-162).
+ */
+typedef struct vmcx_nnbf_model {
+  int32_t n_sites;  /* N <= 64 */
+  int32_t n_f;      /* electrons <= 64 */
+  int32_t hidden;   /* width of both hidden layers */
+  int32_t act;      /* 0 = ReLU, 1 = tanh */
+  int32_t gemm;     /* 0 = FP32 GEMMs, 1 = BF16 operands, FP32 accumulation */
+  int32_t reserved;
+  double bf_scale;
+  const float *M, *W1, *b1, *W2, *b2, *Wout, *bout;
+  const void *W2_bf16, *Wout_bf16;
+} vmcx_nnbf_model;
+
+/* Workspace bytes for B walkers, forward chunks of `chunk` configurations
+ * and n_edges lattice edges (local energy); sweeps need chunk >= B. */
+int64_t vmcx_nnbf_workspace_bytes(const vmcx_nnbf_model *m, int64_t B, int64_t chunk,
+                                  int n_edges);
+
+/* Batched (sign, ln|psi|) of B configurations; z1_out (may be NULL) receives
+ * the first-layer pre-activations [B][hidden] (the sweep's walker state).
+ * Replaces: NNBF_GPU.forward_log via _base.vamp_log (GPU/models/_base.py:111-130). */
+int vmcx_nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *configs,
+                      void *workspace, int64_t ws_bytes, int64_t chunk, double *sign,
+                      double *logabs, float *z1_out, void *stream);
+
+/* Metropolis sweeps for NNBF walkers: per edge, all walkers propose (same
+ * proposal table and draws as vmcx_slater_sweep), the dense layers and the
+ * determinant are evaluated for the proposed configurations, and accepted
+ * moves update configs, z1 ([B][hidden] FP32), sign and logabs in place.
+ * edges_host: HOST array [n_edges][2] int32.
+ * Replaces: MetropolisExchangeSpinfulSamplerGPU.step with an NNBF model
+ * (GPU/sampler.py:216-388). */
+int vmcx_nnbf_sweep(const vmcx_nnbf_model *m, int64_t B, int n_edges,
+                    const int32_t *edges_host, double hopping_rate, int n_sweeps,
+                    uint64_t seed, int64_t sweep_offset, const double *u_choice,
+                    const uint8_t *bits, const double *u_acc, int64_t *configs, float *z1,
+                    double *sign, double *logabs, int64_t *n_accept, void *workspace,
+                    int64_t ws_bytes, void *stream);
+
+/* Local energies: walker-major connected configurations, their NNBF
+ * amplitudes in chunks, and E_loc = U n_double + sum coeff psi'/psi.
+ * Synchronises the stream once (to size the connected set).
+ * Replaces: _evaluate_energy_impl (GPU/vmc_utils.py:322-476) for NNBF. */
+int vmcx_nnbf_local_energy(const vmcx_nnbf_model *m, int64_t B, int n_edges,
+                           const int32_t *edges, double t, double U, const int64_t *configs,
+                           const double *sign, const double *logabs, void *workspace,
+                           int64_t ws_bytes, int64_t chunk, double *eloc, void *stream);
+
+/* Per-sample d ln|psi| / d theta, parameter order [M, W1, b1, W2, b2, Wout,
+ * bout] (NNBF.py:44-48); O row stride ld; o_dtype 0 = float64, 1 = float32.
+ * Also writes sign / logabs.  Replaces: compute_grads_gpu for NNBF
+ * (GPU/vmc_utils.py:1389-1634). */
+int vmcx_nnbf_logderiv(const vmcx_nnbf_model *m, int64_t B, const int64_t *configs,
+                       void *workspace, int64_t ws_bytes, int64_t chunk, void *O, int o_dtype,
+                       int64_t ld, double *sign, double *logabs, void *stream);
+
+/* y (bf16) = x (fp32), n elements (weight copies for gemm = 1). */
+int vmcx_to_bf16(int64_t n, const float *x, void *y, void *stream);
+
 /* params[i] -= lr * dp[i].  Replaces: SGDGPU.step (GPU/optimizer.py:144-186). */
 int vmcx_sgd_update(int64_t n, double *params, const double *dp, double lr,
                     void *stream);

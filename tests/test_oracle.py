@@ -272,3 +272,50 @@ def test_tau_int_ar1():
     tau = stats.tau_int(x[:, 100:])
     assert tau == pytest.approx((1 + phi) / (1 - phi), rel=0.05)
     assert stats.tau_int(rng.normal(size=(100, 500))) == pytest.approx(1.0, abs=0.1)
+
+
+# ------------------------------------------------------------------ NNBF
+def _nnbf_small(seed=9, scale=0.3):
+    from oracle import nnbf
+    rng = np.random.default_rng(seed)
+    H = hubbard.Hubbard(2, 2, 1.0, 8.0, 2, 2)
+    params = nnbf.init_params(rng, 4, 4, hidden=6, n_layers=2, m_std=1.0)
+    params = [p + 0.1 * rng.normal(size=p.shape) for p in params]   # nonzero biases
+    return nnbf, H, params, scale
+
+
+def test_nnbf_zero_scale_is_slater():
+    nnbf, H, params, _ = _nnbf_small()
+    x = H.all_states()
+    s, l, _ = nnbf.forward(params, x, 2, 0.0)
+    s0, l0 = slater.slogdet(params[0], x)
+    assert np.array_equal(s, s0) and np.allclose(l, l0, atol=1e-13)
+
+
+def test_nnbf_log_derivative_finite_difference():
+    nnbf, H, params, scale = _nnbf_small()
+    x = H.all_states()[[1, 9, 30]]
+    O = nnbf.log_derivative(params, x, 2, scale)
+    flat = np.concatenate([p.ravel() for p in params])
+    sizes = [p.size for p in params]
+
+    def unflat(v):
+        out, off = [], 0
+        for p, n in zip(params, sizes):
+            out.append(v[off:off + n].reshape(p.shape))
+            off += n
+        return out
+    h = 1e-6
+    rng = np.random.default_rng(0)
+    for k in rng.choice(flat.size, 60, replace=False):
+        vp, vm = flat.copy(), flat.copy()
+        vp[k] += h
+        vm[k] -= h
+        fd = (nnbf.forward(unflat(vp), x, 2, scale)[1] - nnbf.forward(unflat(vm), x, 2, scale)[1]) / (2 * h)
+        assert np.allclose(O[:, k], fd, atol=2e-6), k
+
+
+def test_nnbf_param_count_config3():
+    from oracle import nnbf
+    p = nnbf.init_params(np.random.default_rng(0), 36, 32, hidden=256, n_layers=2)
+    assert sum(a.size for a in p) == 678912

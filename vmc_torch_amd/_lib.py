@@ -38,7 +38,25 @@ _SIGNATURES = {
     "vmcx_sr_minsr": (_I, [_I64, _I64, _P, _I64, _P, _D, _D, _I, _I, _P, _I64,
                            _P, _P, _P, _P, _P]),
     "vmcx_sgd_update": (_I, [_I64, _P, _P, _D, _P]),
+    "vmcx_nnbf_workspace_bytes": (_I64, [_P, _I64, _I64, _I]),
+    "vmcx_nnbf_forward": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _P, _P, _P]),
+    "vmcx_nnbf_sweep": (_I, [_P, _I64, _I, _P, _D, _I, _U64, _I64, _P, _P, _P, _P, _P,
+                             _P, _P, _P, _P, _I64, _P]),
+    "vmcx_nnbf_local_energy": (_I, [_P, _I64, _I, _P, _D, _D, _P, _P, _P, _P, _I64, _I64,
+                                    _P, _P]),
+    "vmcx_nnbf_logderiv": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _I, _I64, _P, _P, _P]),
+    "vmcx_to_bf16": (_I, [_I64, _P, _P, _P]),
 }
+
+
+class NnbfModelStruct(ctypes.Structure):
+    """vmcx_nnbf_model (include/vmcx.h)."""
+    _fields_ = [("n_sites", ctypes.c_int32), ("n_f", ctypes.c_int32),
+                ("hidden", ctypes.c_int32), ("act", ctypes.c_int32),
+                ("gemm", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("bf_scale", ctypes.c_double),
+                ("M", _P), ("W1", _P), ("b1", _P), ("W2", _P), ("b2", _P),
+                ("Wout", _P), ("bout", _P), ("W2_bf16", _P), ("Wout_bf16", _P)]
 
 _LIB = None
 

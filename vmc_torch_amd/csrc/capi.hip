@@ -132,6 +132,51 @@ int vmcx_sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
   return rc == 3 ? VMCX_ELIB : status_of(rc);
 }
 
+int64_t vmcx_nnbf_workspace_bytes(const vmcx_nnbf_model *m, int64_t B, int64_t chunk,
+                                  int n_edges) {
+  return vmcx::nnbf_workspace_bytes(m, B, chunk, n_edges);
+}
+
+static inline int status3(int rc) { return rc == 3 ? VMCX_ELIB : status_of(rc); }
+
+int vmcx_nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *configs,
+                      void *workspace, int64_t ws_bytes, int64_t chunk, double *sign,
+                      double *logabs, float *z1_out, void *stream) {
+  return status3(vmcx::nnbf_forward(m, B, configs, workspace, ws_bytes, chunk, sign, logabs,
+                                    z1_out, (hipStream_t)stream));
+}
+
+int vmcx_nnbf_sweep(const vmcx_nnbf_model *m, int64_t B, int n_edges,
+                    const int32_t *edges_host, double hopping_rate, int n_sweeps,
+                    uint64_t seed, int64_t sweep_offset, const double *u_choice,
+                    const uint8_t *bits, const double *u_acc, int64_t *configs, float *z1,
+                    double *sign, double *logabs, int64_t *n_accept, void *workspace,
+                    int64_t ws_bytes, void *stream) {
+  return status3(vmcx::nnbf_sweep(m, B, n_edges, edges_host, hopping_rate, n_sweeps, seed,
+                                  sweep_offset, u_choice, bits, u_acc, configs, z1, sign,
+                                  logabs, n_accept, workspace, ws_bytes, (hipStream_t)stream));
+}
+
+int vmcx_nnbf_local_energy(const vmcx_nnbf_model *m, int64_t B, int n_edges,
+                           const int32_t *edges, double t, double U, const int64_t *configs,
+                           const double *sign, const double *logabs, void *workspace,
+                           int64_t ws_bytes, int64_t chunk, double *eloc, void *stream) {
+  return status3(vmcx::nnbf_local_energy(m, B, n_edges, edges, t, U, configs, sign, logabs,
+                                         workspace, ws_bytes, chunk, eloc,
+                                         (hipStream_t)stream));
+}
+
+int vmcx_nnbf_logderiv(const vmcx_nnbf_model *m, int64_t B, const int64_t *configs,
+                       void *workspace, int64_t ws_bytes, int64_t chunk, void *O, int o_dtype,
+                       int64_t ld, double *sign, double *logabs, void *stream) {
+  return status3(vmcx::nnbf_logderiv(m, B, configs, workspace, ws_bytes, chunk, O, o_dtype, ld,
+                                     sign, logabs, (hipStream_t)stream));
+}
+
+int vmcx_to_bf16(int64_t n, const float *x, void *y, void *stream) {
+  return status_of(vmcx::to_bf16(n, x, y, (hipStream_t)stream));
+}
+
 int vmcx_sgd_update(int64_t n, double *params, const double *dp, double lr,
                     void *stream) {
   return status_of(vmcx::sgd_update(n, params, dp, lr, (hipStream_t)stream));

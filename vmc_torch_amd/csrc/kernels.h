@@ -3,6 +3,8 @@
 
 #include "common.h"
 
+typedef struct _rocblas_handle *rocblas_handle;
+
 namespace vmcx {
 
 struct SweepArgs {
@@ -51,6 +53,7 @@ int sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
              int32_t *info, float *phase_ms, hipStream_t st);
 int sgd_update(int64_t n, double *params, const double *dp, double lr,
                hipStream_t st);
+rocblas_handle blas_handle_for(hipStream_t st);
 int64_t chol_dinv_bytes(int64_t n);
 int chol_factor(void *rocblas_handle_, int64_t n, double *A, int64_t lda,
                 double *dinv, int32_t *info, hipStream_t st);
@@ -58,4 +61,26 @@ int chol_solve(int64_t n, const double *L, int64_t lda, const double *dinv,
                const double *b, double *x, double *tmp, double *tmp2,
                hipStream_t st);
 
+}  // namespace vmcx
+
+#include "../../include/vmcx.h"
+
+namespace vmcx {
+int64_t nnbf_workspace_bytes(const vmcx_nnbf_model *m, int64_t B, int64_t chunk, int n_edges);
+int nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
+                 int64_t ws_bytes, int64_t chunk, double *sign, double *logabs, float *z1_out,
+                 hipStream_t st);
+int nnbf_sweep(const vmcx_nnbf_model *m, int64_t B, int n_edges, const int32_t *edges_host,
+               double hopping_rate, int n_sweeps, uint64_t seed, int64_t sweep_offset,
+               const double *u_choice, const uint8_t *bits, const double *u_acc,
+               int64_t *cfg, float *z1, double *sign, double *logabs, int64_t *n_accept,
+               void *ws, int64_t ws_bytes, hipStream_t st);
+int nnbf_local_energy(const vmcx_nnbf_model *m, int64_t B, int n_edges, const int32_t *edges,
+                      double t, double U, const int64_t *cfg, const double *sign,
+                      const double *logabs, void *ws, int64_t ws_bytes, int64_t chunk,
+                      double *eloc, hipStream_t st);
+int nnbf_logderiv(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
+                  int64_t ws_bytes, int64_t chunk, void *O, int o_dtype, int64_t ld,
+                  double *sign, double *logabs, hipStream_t st);
+int to_bf16(int64_t n, const float *x, void *y, hipStream_t st);
 }  // namespace vmcx

@@ -1,0 +1,823 @@
+// Neural-network backflow (NNBF) walkers on gfx950.
+//
+//   n   = occupation [up | down] (2N), one-hot input
+//   z1  = W1 n + b1          (gather-sum of the W1 columns of occupied orbitals)
+//   h1  = act(z1)
+//   z2  = W2 h1 + b2         GEMM, batch of walkers / configurations
+//   h2  = act(z2)
+//   D   = Wout h2 + bout     GEMM  (2N*n_f outputs)
+//   psi = det((M + s D)[occupied rows])   FP64 Gauss-Jordan in registers
+//
+// GEMMs run on MFMA through rocBLAS gemm_ex: FP32 operands, or BF16 operands
+// with FP32 accumulation.  The sweep proposes one edge for all walkers at a
+// time; z1 is carried per walker and updated by the W1 columns of the moved
+// electrons, so only the two dense layers and the determinant are recomputed
+// per proposal.
+#include <hip/hip_bf16.h>
+#include <rocblas/rocblas.h>
+
+#include "../../include/vmcx.h"
+#include "gj.h"
+#include "kernels.h"
+
+namespace vmcx {
+
+__device__ __forceinline__ float act_f(float z, int act) {
+  return act == 0 ? fmaxf(z, 0.0f) : tanhf(z);
+}
+
+template <typename T>
+__device__ __forceinline__ T to_t(float v);
+template <>
+__device__ __forceinline__ float to_t<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ __hip_bfloat16 to_t<__hip_bfloat16>(float v) {
+  return __float2bfloat16(v);
+}
+
+// ------------------------------------------------------------ layer one
+// one block per configuration; z1[u] = b1[u] + sum over occupied orbitals
+template <typename TI>
+__global__ void __launch_bounds__(256)
+    nnbf_layer1_kernel(int64_t Bc, int N, int H, int act,
+                       const int64_t *__restrict__ cfg, const float *__restrict__ W1,
+                       const float *__restrict__ b1, float *z1, TI *h1) {
+  __shared__ int occ[128];
+  __shared__ int nocc;
+  const int64_t w = blockIdx.x;
+  if (threadIdx.x == 0) {
+    int k = 0;
+    const int64_t *c = cfg + w * N;
+    for (int s = 0; s < N; ++s) {
+      const int64_t x = c[s];
+      if (x & 2) occ[k++] = s;
+      if (x & 1) occ[k++] = N + s;
+    }
+    nocc = k;
+  }
+  __syncthreads();
+  const int n_orb = 2 * N;
+  for (int u = threadIdx.x; u < H; u += blockDim.x) {
+    float z = b1[u];
+    for (int k = 0; k < nocc; ++k) z += W1[(int64_t)u * n_orb + occ[k]];
+    z1[w * H + u] = z;
+    h1[w * H + u] = to_t<TI>(act_f(z, act));
+  }
+}
+
+// z += b (in place), h = act(z)
+template <typename TI>
+__global__ void __launch_bounds__(256)
+    bias_act_kernel(int64_t rows, int H, int act, float *z, const float *__restrict__ b,
+                    TI *h) {
+  const int64_t n = rows * H;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = z[i] + b[i % H];
+    z[i] = v;
+    h[i] = to_t<TI>(act_f(v, act));
+  }
+}
+
+// ------------------------------------------------------------ determinant
+// Builds A^T row-per-lane: a[j] = A[j][col] = M[orb_j][col] + s (D[orb_j*nf+col]
+// + bout[...]) for orb_j the j-th occupied orbital (ascending), with the
+// occupation masks given explicitly (up, dn).
+template <int NFP>
+__device__ __forceinline__ void nnbf_rows(double (&a)[NFP], uint64_t up, uint64_t dn, int N,
+                                          int nf, int col, const float *__restrict__ M,
+                                          const float *__restrict__ Drow,
+                                          const float *__restrict__ bout, double s) {
+  uint64_t mu = up, md = dn;
+#pragma unroll
+  for (int j = 0; j < NFP; ++j) {
+    int orb = -1;
+    if (mu) {
+      orb = __ffsll((long long)mu) - 1;
+      mu &= mu - 1;
+    } else if (md) {
+      orb = N + __ffsll((long long)md) - 1;
+      md &= md - 1;
+    }
+    if (j < nf && col < nf) {
+      if (orb >= 0) {
+        const int64_t e = (int64_t)orb * nf + col;
+        a[j] = (double)M[e] + s * ((double)Drow[e] + (double)bout[e]);
+      } else {
+        a[j] = 0.0;
+      }
+    } else {
+      a[j] = (j == col) ? 1.0 : 0.0;
+    }
+  }
+}
+
+template <int NFP>
+__device__ __forceinline__ void group_masks(const int64_t *cfg, int N, int col, uint64_t &up,
+                                            uint64_t &dn) {
+  uint64_t u = 0, d = 0;
+  for (int s = col; s < N; s += NFP) {
+    const int64_t x = cfg[s];
+    if (x & 2) u |= 1ull << s;
+    if (x & 1) d |= 1ull << s;
+  }
+#pragma unroll
+  for (int o = NFP / 2; o > 0; o >>= 1) {
+    u |= ((uint64_t)__shfl_xor((uint32_t)(u >> 32), o, 64) << 32) |
+         __shfl_xor((uint32_t)u, o, 64);
+    d |= ((uint64_t)__shfl_xor((uint32_t)(d >> 32), o, 64) << 32) |
+         __shfl_xor((uint32_t)d, o, 64);
+  }
+  up = u;
+  dn = d;
+}
+
+template <int NFP>
+__global__ void __launch_bounds__(256)
+    nnbf_det_kernel(int64_t Bc, int N, int nf, const int64_t *__restrict__ cfg,
+                    const float *__restrict__ D, const float *__restrict__ M,
+                    const float *__restrict__ bout, double s, double *sign,
+                    double *logabs, double *ainv) {
+  constexpr int WPW = 64 / NFP;
+  extern __shared__ __align__(16) char smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int grp = lane / NFP, col = lane % NFP;
+  const int wl = wave * WPW + grp;
+  const int per_block = (blockDim.x / 64) * WPW;
+  const int64_t w = (int64_t)blockIdx.x * per_block + wl;
+  if (w >= Bc) return;
+  double *prow = reinterpret_cast<double *>(smem) + wl * NFP;
+  uint8_t *qbuf = reinterpret_cast<uint8_t *>(smem) + (size_t)per_block * NFP * 8 + wl * NFP;
+  uint64_t up, dn;
+  group_masks<NFP>(cfg + w * N, N, col, up, dn);
+  const bool count_ok = (__popcll(up) + __popcll(dn)) == nf;
+  double a[NFP];
+  nnbf_rows<NFP>(a, up, dn, N, nf, col, M, D + w * (int64_t)(2 * N * nf), bout, s);
+  const GJResult r = gj_invert<NFP>(a, col, grp * NFP, prow, qbuf);
+  const bool singular = r.singular || !count_ok;
+  if (col == 0) {
+    sign[w] = singular ? 0.0 : ((r.parity & 1) ? -1.0 : 1.0);
+    logabs[w] = singular ? (count_ok ? log(1e-45) : __builtin_nan(""))
+                         : fmax(r.logdet, log(1e-45));
+  }
+  if (ainv != nullptr && r.my_step >= 0) {
+    double *dst = ainv + ((size_t)w * NFP + r.my_step) * NFP;
+#pragma unroll
+    for (int c = 0; c < NFP; ++c) dst[qbuf[c]] = singular ? 0.0 : a[c];
+  }
+}
+
+// ------------------------------------------------------------ sweep pieces
+// prop[w] = flag | ni << 8 | nj << 12 | nmoves << 16 ; moves[w] = a1 b1 a2 b2 (bytes)
+__global__ void __launch_bounds__(256)
+    nnbf_propose_kernel(int64_t B, int N, int i, int j, int e, int64_t sweep,
+                        uint64_t seed, double keep_rate, const double *u_choice,
+                        const uint8_t *bits, const double *u_acc_in,
+                        const int64_t *__restrict__ cfg, int32_t *prop, uint32_t *moves,
+                        double *u_acc) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= B) return;
+  const int si = (int)cfg[w * N + i], sj = (int)cfg[w * N + j];
+  if (si == sj) {
+    prop[w] = 0;
+    return;
+  }
+  double uc, ua;
+  int b0, b2;
+  if (u_choice == nullptr) {
+    const u32x4 o = philox4x32_10((uint32_t)e, (uint32_t)sweep, (uint32_t)w, 0u,
+                                  (uint32_t)seed, (uint32_t)(seed >> 32));
+    uc = u53(o.x, o.y);
+    ua = u53(o.z, o.w);
+    b0 = o.y & 1u;
+    b2 = (o.y >> 1) & 1u;
+  } else {
+    uc = u_choice[w];
+    ua = u_acc_in[w];
+    b0 = bits[w] & 1;
+    b2 = (bits[w] >> 1) & 1;
+  }
+  const int nmi = (si == 0) ? 0 : ((si == 3) ? 2 : 1);
+  const int nmj = (sj == 0) ? 0 : ((sj == 3) ? 2 : 1);
+  const int dnum = nmi > nmj ? nmi - nmj : nmj - nmi;
+  int ni, nj;
+  if (uc < keep_rate || dnum == 1) {
+    ni = sj;
+    nj = si;
+  } else if (dnum == 0) {
+    ni = b0 ? 3 : 0;
+    nj = b0 ? 0 : 3;
+  } else {
+    ni = b2 ? 2 : 1;
+    nj = b2 ? 1 : 2;
+  }
+  uint32_t mv = 0;
+  int nm = 0;
+  if ((si ^ ni) & 2) {
+    const int from = (si & 2) ? i : j, to = (si & 2) ? j : i;
+    mv |= (uint32_t)(from | (to << 8)) << (16 * nm);
+    ++nm;
+  }
+  if ((si ^ ni) & 1) {
+    const int from = (si & 1) ? i : j, to = (si & 1) ? j : i;
+    mv |= (uint32_t)((N + from) | ((N + to) << 8)) << (16 * nm);
+    ++nm;
+  }
+  prop[w] = 1 | (ni << 8) | (nj << 12) | (nm << 16);
+  moves[w] = mv;
+  u_acc[w] = ua;
+}
+
+// h1p = act(z1 + sum_moves (W1[:, to] - W1[:, from]))
+template <typename TI>
+__global__ void __launch_bounds__(256)
+    nnbf_h1_proposed_kernel(int64_t B, int N, int H, int act, const float *__restrict__ W1,
+                            const float *__restrict__ z1, const int32_t *__restrict__ prop,
+                            const uint32_t *__restrict__ moves, TI *h1p) {
+  const int64_t n = B * H;
+  const int n_orb = 2 * N;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t w = idx / H;
+    const int u = (int)(idx - w * H);
+    float z = z1[idx];
+    const int p = prop[w];
+    if (p & 1) {
+      const int nm = (p >> 16) & 3;
+      const uint32_t mv = moves[w];
+      for (int m = 0; m < nm; ++m) {
+        const int from = (mv >> (16 * m)) & 0xFF, to = (mv >> (16 * m + 8)) & 0xFF;
+        z += W1[(int64_t)u * n_orb + to] - W1[(int64_t)u * n_orb + from];
+      }
+    }
+    h1p[idx] = to_t<TI>(act_f(z, act));
+  }
+}
+
+// determinant of the proposed configuration, Metropolis test, commit
+template <int NFP>
+__global__ void __launch_bounds__(256)
+    nnbf_accept_kernel(int64_t B, int N, int nf, int H, int i, int j, int64_t *cfg,
+                       const int32_t *__restrict__ prop, const uint32_t *__restrict__ moves,
+                       const double *__restrict__ u_acc, const float *__restrict__ D,
+                       const float *__restrict__ M, const float *__restrict__ bout, double s,
+                       const float *__restrict__ W1, float *z1, double *sign, double *logabs,
+                       int64_t *n_accept) {
+  constexpr int WPW = 64 / NFP;
+  extern __shared__ __align__(16) char smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int grp = lane / NFP, col = lane % NFP;
+  const int wl = wave * WPW + grp;
+  const int per_block = (blockDim.x / 64) * WPW;
+  const int64_t w = (int64_t)blockIdx.x * per_block + wl;
+  if (w >= B) return;
+  const int p = prop[w];
+  if (!(p & 1)) return;
+  double *prow = reinterpret_cast<double *>(smem) + wl * NFP;
+  uint8_t *qbuf = reinterpret_cast<uint8_t *>(smem) + (size_t)per_block * NFP * 8 + wl * NFP;
+  const int ni = (p >> 8) & 0xF, nj = (p >> 12) & 0xF;
+  uint64_t up, dn;
+  group_masks<NFP>(cfg + w * N, N, col, up, dn);
+  // proposed occupation
+  up = (up & ~((1ull << i) | (1ull << j))) | ((uint64_t)((ni >> 1) & 1) << i) |
+       ((uint64_t)((nj >> 1) & 1) << j);
+  dn = (dn & ~((1ull << i) | (1ull << j))) | ((uint64_t)(ni & 1) << i) |
+       ((uint64_t)(nj & 1) << j);
+  double a[NFP];
+  nnbf_rows<NFP>(a, up, dn, N, nf, col, M, D + w * (int64_t)(2 * N * nf), bout, s);
+  const GJResult r = gj_invert<NFP>(a, col, grp * NFP, prow, qbuf);
+  const double lp = r.singular ? log(1e-45) : fmax(r.logdet, log(1e-45));
+  const double ratio = exp(2.0 * (lp - logabs[w]));
+  if (!(u_acc[w] < ratio)) return;
+  // commit (group-uniform branch)
+  if (col == 0) {
+    cfg[w * N + i] = ni;
+    cfg[w * N + j] = nj;
+    sign[w] = r.singular ? 0.0 : ((r.parity & 1) ? -1.0 : 1.0);
+    logabs[w] = lp;
+    if (n_accept) n_accept[w] += 1;
+  }
+  const int nm = (p >> 16) & 3;
+  const uint32_t mv = moves[w];
+  const int n_orb = 2 * N;
+  for (int u = col; u < H; u += NFP) {
+    float dz = 0.0f;
+    for (int m = 0; m < nm; ++m) {
+      const int from = (mv >> (16 * m)) & 0xFF, to = (mv >> (16 * m + 8)) & 0xFF;
+      dz += W1[(int64_t)u * n_orb + to] - W1[(int64_t)u * n_orb + from];
+    }
+    z1[w * H + u] += dz;
+  }
+}
+
+// ------------------------------------------------------------ local energy
+// walker-major hop enumeration: counts, then etas / coefficients
+__global__ void __launch_bounds__(256)
+    conn_count_walker_kernel(int64_t B, int N, int E, const int32_t *__restrict__ edges,
+                             const int64_t *__restrict__ cfg, int64_t *counts) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= B) return;
+  const int64_t *c = cfg + w * N;
+  int64_t k = 0;
+  for (int e = 0; e < E; ++e) {
+    const int i = edges[2 * e], j = edges[2 * e + 1];
+    if ((unsigned)i >= (unsigned)N || (unsigned)j >= (unsigned)N || i == j) continue;
+    k += (((c[i] >> 1) ^ (c[j] >> 1)) & 1) + ((c[i] ^ c[j]) & 1);
+  }
+  counts[w] = k;
+}
+
+// single-block exclusive scan (B up to millions; sequential chunks of 1024)
+__global__ void __launch_bounds__(1024)
+    exclusive_scan_kernel(int64_t n, const int64_t *in, int64_t *out, int64_t *total) {
+  __shared__ int64_t sh[1024];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < n; base += blockDim.x) {
+    const int64_t i = base + threadIdx.x;
+    const int64_t v = (i < n) ? in[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+      const int64_t t = (threadIdx.x >= (unsigned)o) ? sh[threadIdx.x - o] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < n) out[i] = carry + sh[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) carry += sh[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total) total[0] = carry;
+}
+
+__global__ void __launch_bounds__(256)
+    conn_fill_walker_kernel(int64_t B, int N, int E, const int32_t *__restrict__ edges,
+                            double t, const int64_t *__restrict__ cfg,
+                            const int64_t *__restrict__ offsets, int64_t *etas,
+                            double *coeffs) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= B) return;
+  const int64_t *c = cfg + w * N;
+  int64_t pos = offsets[w];
+  for (int e = 0; e < E; ++e) {
+    const int i = edges[2 * e], j = edges[2 * e + 1];
+    if ((unsigned)i >= (unsigned)N || (unsigned)j >= (unsigned)N || i == j) continue;
+    for (int sp = 0; sp < 2; ++sp) {
+      const int bit = sp == 0 ? 2 : 1;
+      if (((c[i] & bit) != 0) == ((c[j] & bit) != 0)) continue;
+      const int lo = i < j ? i : j, hi = i < j ? j : i;
+      int par = 0;
+      if (bit == 2) {
+        for (int s = lo + 1; s < hi; ++s) par += (int)((c[s] >> 1) & 1);
+        for (int s = lo + 1; s <= hi; ++s) par += (int)(c[s] & 1);
+      } else {
+        for (int s = lo; s < hi; ++s) par += (int)((c[s] >> 1) & 1);
+        for (int s = lo + 1; s < hi; ++s) par += (int)(c[s] & 1);
+      }
+      int64_t *eta = etas + pos * N;
+      for (int s = 0; s < N; ++s) eta[s] = c[s];
+      eta[i] = c[i] ^ bit;
+      eta[j] = c[j] ^ bit;
+      coeffs[pos] = (par & 1) ? t : -t;
+      ++pos;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    eloc_segment_kernel(int64_t B, int N, double U, const int64_t *__restrict__ cfg,
+                        const int64_t *__restrict__ offsets,
+                        const int64_t *__restrict__ counts, const double *__restrict__ coeffs,
+                        const double *__restrict__ csign, const double *__restrict__ clog,
+                        const double *__restrict__ sign, const double *__restrict__ logabs,
+                        double *eloc) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= B) return;
+  const int64_t *c = cfg + w * N;
+  int nd = 0;
+  for (int s = 0; s < N; ++s) nd += (c[s] == 3);
+  double acc = U * nd;
+  const int64_t o = offsets[w], n = counts[w];
+  for (int64_t k = 0; k < n; ++k)
+    acc += coeffs[o + k] * csign[o + k] * sign[w] * exp(clog[o + k] - logabs[w]);
+  eloc[w] = acc;
+}
+
+// ------------------------------------------------------------ backward
+// One block per sample: O = d ln|psi| / d theta in the order
+// [M, W1, b1, W2, b2, Wout, bout].
+template <typename TO>
+__global__ void __launch_bounds__(256)
+    nnbf_backward_kernel(int64_t Bc, int N, int nf, int H, int act, int nfp,
+                         const int64_t *__restrict__ cfg, const float *__restrict__ z1,
+                         const float *__restrict__ z2, const double *__restrict__ ainv,
+                         const float *__restrict__ W2, const float *__restrict__ Wout,
+                         double s, TO *O, int64_t ld) {
+  extern __shared__ __align__(16) float fsm[];
+  const int n_orb = 2 * N, nD = n_orb * nf;
+  float *gD = fsm;                 // [nD]
+  float *h1 = gD + nD;             // [H]
+  float *h2 = h1 + H;              // [H]
+  float *dz2 = h2 + H;             // [H]
+  float *dz1 = dz2 + H;            // [H]
+  float *nvec = dz1 + H;           // [n_orb]
+  __shared__ int slot_of[128];
+  const int64_t w = blockIdx.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (tid == 0) {
+    int k = 0;
+    const int64_t *c = cfg + w * N;
+    for (int o = 0; o < n_orb; ++o) slot_of[o] = -1;
+    for (int x = 0; x < N; ++x)
+      if (c[x] & 2) slot_of[x] = k++;
+    for (int x = 0; x < N; ++x)
+      if (c[x] & 1) slot_of[N + x] = k++;
+  }
+  __syncthreads();
+  for (int o = tid; o < n_orb; o += nt) nvec[o] = slot_of[o] >= 0 ? 1.0f : 0.0f;
+  const double *Ai = ainv + (size_t)w * nfp * nfp;
+  for (int idx = tid; idx < nD; idx += nt) {
+    const int o = idx / nf, c = idx - o * nf;
+    const int r = slot_of[o];
+    // d ln|det A| / d A[r][c] = (A^-1)[c][r] = ainv[r][c]
+    gD[idx] = (r >= 0) ? (float)Ai[(size_t)r * nfp + c] : 0.0f;
+  }
+  for (int u = tid; u < H; u += nt) {
+    h1[u] = act_f(z1[w * H + u], act);
+    h2[u] = act_f(z2[w * H + u], act);
+  }
+  __syncthreads();
+  TO *row = O + w * ld;
+  // M block: gM (unscaled)
+  for (int idx = tid; idx < nD; idx += nt) row[idx] = (TO)gD[idx];
+  __syncthreads();
+  const float sf = (float)s;
+  for (int idx = tid; idx < nD; idx += nt) gD[idx] *= sf;
+  __syncthreads();
+  // dh2[k] = sum_i gD[i] Wout[i][k] over occupied rows only
+  for (int k = tid; k < H; k += nt) {
+    float acc = 0.0f;
+    for (int o = 0; o < n_orb; ++o) {
+      if (slot_of[o] < 0) continue;
+      const float *wr = Wout + (int64_t)o * nf * H + k;
+      const float *g = gD + o * nf;
+      for (int c = 0; c < nf; ++c) acc += g[c] * wr[(int64_t)c * H];
+    }
+    const float z = z2[w * H + k];
+    dz2[k] = act == 0 ? (z > 0.0f ? acc : 0.0f) : acc * (1.0f - tanhf(z) * tanhf(z));
+  }
+  __syncthreads();
+  // dh1[i] = sum_u dz2[u] W2[u][i]
+  for (int i = tid; i < H; i += nt) {
+    float acc = 0.0f;
+    for (int u = 0; u < H; ++u) acc += dz2[u] * W2[(int64_t)u * H + i];
+    const float z = z1[w * H + i];
+    dz1[i] = act == 0 ? (z > 0.0f ? acc : 0.0f) : acc * (1.0f - tanhf(z) * tanhf(z));
+  }
+  __syncthreads();
+  int64_t off = nD;
+  for (int64_t idx = tid; idx < (int64_t)H * n_orb; idx += nt) {   // W1
+    const int u = (int)(idx / n_orb), jj = (int)(idx - (int64_t)u * n_orb);
+    row[off + idx] = (TO)(dz1[u] * nvec[jj]);
+  }
+  off += (int64_t)H * n_orb;
+  for (int u = tid; u < H; u += nt) row[off + u] = (TO)dz1[u];       // b1
+  off += H;
+  for (int64_t idx = tid; idx < (int64_t)H * H; idx += nt) {         // W2
+    const int u = (int)(idx / H), i = (int)(idx - (int64_t)u * H);
+    row[off + idx] = (TO)(dz2[u] * h1[i]);
+  }
+  off += (int64_t)H * H;
+  for (int u = tid; u < H; u += nt) row[off + u] = (TO)dz2[u];       // b2
+  off += H;
+  for (int64_t idx = tid; idx < (int64_t)nD * H; idx += nt) {        // Wout
+    const int i = (int)(idx / H), k = (int)(idx - (int64_t)i * H);
+    row[off + idx] = (TO)(gD[i] * h2[k]);
+  }
+  off += (int64_t)nD * H;
+  for (int idx = tid; idx < nD; idx += nt) row[off + idx] = (TO)gD[idx];   // bout
+}
+
+// ============================================================ host side
+struct NnbfWs {
+  // per-chunk forward buffers
+  float *z1, *z2, *D;
+  void *h1, *h2;
+  double *ainv;
+  // sweep buffers (per walker)
+  int32_t *prop;
+  uint32_t *moves;
+  double *uacc;
+  // local-energy buffers
+  int64_t *counts, *offsets, *total, *etas;
+  double *coeffs, *csign, *clog;
+  size_t bytes;
+};
+
+static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static NnbfWs nnbf_layout(const vmcx_nnbf_model &m, int64_t B, int64_t chunk, int n_edges,
+                          char *base) {
+  NnbfWs L{};
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    const size_t o = off;
+    off += al(b);
+    return base ? base + o : nullptr;
+  };
+  const int H = m.hidden, nD = 2 * m.n_sites * m.n_f, nfp = nfp_of(m.n_f);
+  const size_t ti = m.gemm == 1 ? 2 : 4;
+  L.z1 = (float *)take((size_t)chunk * H * 4);
+  L.z2 = (float *)take((size_t)chunk * H * 4);
+  L.D = (float *)take((size_t)chunk * nD * 4);
+  L.h1 = take((size_t)chunk * H * ti);
+  L.h2 = take((size_t)chunk * H * ti);
+  L.ainv = (double *)take((size_t)chunk * nfp * nfp * 8);
+  L.prop = (int32_t *)take((size_t)B * 4);
+  L.moves = (uint32_t *)take((size_t)B * 4);
+  L.uacc = (double *)take((size_t)B * 8);
+  const int64_t Tmax = B * 2 * (int64_t)n_edges;
+  L.counts = (int64_t *)take((size_t)B * 8);
+  L.offsets = (int64_t *)take((size_t)B * 8);
+  L.total = (int64_t *)take(8);
+  L.etas = (int64_t *)take((size_t)Tmax * m.n_sites * 8);
+  L.coeffs = (double *)take((size_t)Tmax * 8);
+  L.csign = (double *)take((size_t)Tmax * 8);
+  L.clog = (double *)take((size_t)Tmax * 8);
+  L.bytes = off;
+  return L;
+}
+
+static int check_model(const vmcx_nnbf_model *m) {
+  VMCX_CHECK_ARG(m != nullptr, "null model");
+  VMCX_CHECK_ARG(m->n_sites >= 1 && m->n_sites <= 64, "n_sites must be in [1, 64]");
+  VMCX_CHECK_ARG(m->n_f >= 1 && m->n_f <= 64 && m->n_f <= 2 * m->n_sites, "bad n_f");
+  VMCX_CHECK_ARG(m->hidden >= 1 && m->hidden <= 4096, "hidden must be in [1, 4096]");
+  VMCX_CHECK_ARG(m->act == 0 || m->act == 1, "act must be 0 (ReLU) or 1 (tanh)");
+  VMCX_CHECK_ARG(m->gemm == 0 || m->gemm == 1, "gemm must be 0 (FP32) or 1 (BF16)");
+  VMCX_CHECK_ARG(m->M && m->W1 && m->b1 && m->W2 && m->b2 && m->Wout && m->bout,
+                 "null weight pointer");
+  VMCX_CHECK_ARG(m->gemm == 0 || (m->W2_bf16 && m->Wout_bf16), "BF16 mode needs BF16 weights");
+  return 0;
+}
+
+int64_t nnbf_workspace_bytes(const vmcx_nnbf_model *m, int64_t B, int64_t chunk, int n_edges) {
+  if (!m || B < 0 || chunk < 1 || n_edges < 0) return -1;
+  return (int64_t)nnbf_layout(*m, B, chunk, n_edges, nullptr).bytes;
+}
+
+#define NN_BLAS(call)                                                         \
+  do {                                                                       \
+    rocblas_status s_ = (call);                                              \
+    if (s_ != rocblas_status_success) {                                      \
+      ::vmcx::set_error(std::string(__func__) + ": " #call " -> " +          \
+                        rocblas_status_to_string(s_));                       \
+      return 3;                                                              \
+    }                                                                        \
+  } while (0)
+
+// z2 = W2 h1 ; D = Wout act(z2 + b2)  for `rows` rows already holding h1
+static int dense_layers(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
+                        hipStream_t st) {
+  rocblas_handle h = blas_handle_for(st);
+  VMCX_CHECK_ARG(h != nullptr, "rocblas handle");
+  const int H = m.hidden, nD = 2 * m.n_sites * m.n_f;
+  const float one = 1.0f, zero = 0.0f;
+  const rocblas_datatype ti = m.gemm == 1 ? rocblas_datatype_bf16_r : rocblas_datatype_f32_r;
+  const void *W2 = m.gemm == 1 ? m.W2_bf16 : (const void *)m.W2;
+  const void *Wo = m.gemm == 1 ? m.Wout_bf16 : (const void *)m.Wout;
+  // column-major view: z2 (H x rows) = W2 (H x H, row-major = W2^T col-major) * h1 (H x rows)
+  NN_BLAS(rocblas_gemm_ex(h, rocblas_operation_transpose, rocblas_operation_none, H,
+                          (rocblas_int)rows, H, &one, W2, ti, H, L.h1, ti, H, &zero, L.z2,
+                          rocblas_datatype_f32_r, H, L.z2, rocblas_datatype_f32_r, H,
+                          rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0));
+  const unsigned g = (unsigned)(((rows * H) + 255) / 256 < 8192 ? ((rows * H) + 255) / 256 : 8192);
+  if (m.gemm == 1)
+    hipLaunchKernelGGL(bias_act_kernel<__hip_bfloat16>, dim3(g), dim3(256), 0, st, rows, H,
+                       m.act, L.z2, m.b2, (__hip_bfloat16 *)L.h2);
+  else
+    hipLaunchKernelGGL(bias_act_kernel<float>, dim3(g), dim3(256), 0, st, rows, H, m.act,
+                       L.z2, m.b2, (float *)L.h2);
+  VMCX_LAUNCHED();
+  NN_BLAS(rocblas_gemm_ex(h, rocblas_operation_transpose, rocblas_operation_none, nD,
+                          (rocblas_int)rows, H, &one, Wo, ti, H, L.h2, ti, H, &zero, L.D,
+                          rocblas_datatype_f32_r, nD, L.D, rocblas_datatype_f32_r, nD,
+                          rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0));
+  return 0;
+}
+
+template <int NFP>
+static void launch_det(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
+                       const int64_t *cfg, double *sign, double *logabs, double *ainv,
+                       hipStream_t st) {
+  constexpr int WPW = 64 / NFP;
+  const int64_t per_block = 4 * WPW;
+  const int64_t nblk = (rows + per_block - 1) / per_block;
+  hipLaunchKernelGGL(nnbf_det_kernel<NFP>, dim3((unsigned)nblk), dim3(256),
+                     (size_t)per_block * NFP * 9, st, rows, m.n_sites, m.n_f, cfg, L.D, m.M,
+                     m.bout, m.bf_scale, sign, logabs, ainv);
+}
+
+// forward of `rows` configurations (rows <= chunk); ainv optional
+static int forward_rows(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
+                        const int64_t *cfg, double *sign, double *logabs, double *ainv,
+                        hipStream_t st) {
+  if (rows == 0) return 0;
+  if (m.gemm == 1)
+    hipLaunchKernelGGL(nnbf_layer1_kernel<__hip_bfloat16>, dim3((unsigned)rows), dim3(256), 0,
+                       st, rows, m.n_sites, m.hidden, m.act, cfg, m.W1, m.b1, L.z1,
+                       (__hip_bfloat16 *)L.h1);
+  else
+    hipLaunchKernelGGL(nnbf_layer1_kernel<float>, dim3((unsigned)rows), dim3(256), 0, st, rows,
+                       m.n_sites, m.hidden, m.act, cfg, m.W1, m.b1, L.z1, (float *)L.h1);
+  VMCX_LAUNCHED();
+  int rc = dense_layers(m, L, rows, st);
+  if (rc) return rc;
+  switch (nfp_of(m.n_f)) {
+    case 8: launch_det<8>(m, L, rows, cfg, sign, logabs, ainv, st); break;
+    case 16: launch_det<16>(m, L, rows, cfg, sign, logabs, ainv, st); break;
+    case 32: launch_det<32>(m, L, rows, cfg, sign, logabs, ainv, st); break;
+    default: launch_det<64>(m, L, rows, cfg, sign, logabs, ainv, st); break;
+  }
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+int nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
+                 int64_t ws_bytes, int64_t chunk, double *sign, double *logabs, float *z1_out,
+                 hipStream_t st) {
+  if (check_model(m)) return 1;
+  VMCX_CHECK_ARG(B >= 0 && chunk >= 1, "bad sizes");
+  VMCX_CHECK_ARG(B == 0 || (cfg && sign && logabs && ws), "null pointer");
+  const NnbfWs L = nnbf_layout(*m, B, chunk, 0, (char *)ws);
+  VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
+  for (int64_t r0 = 0; r0 < B; r0 += chunk) {
+    const int64_t rows = (B - r0 < chunk) ? B - r0 : chunk;
+    const int rc = forward_rows(*m, L, rows, cfg + r0 * m->n_sites, sign + r0, logabs + r0,
+                                nullptr, st);
+    if (rc) return rc;
+    if (z1_out)
+      VMCX_HIP(hipMemcpyAsync(z1_out + r0 * m->hidden, L.z1, (size_t)rows * m->hidden * 4,
+                              hipMemcpyDeviceToDevice, st));
+  }
+  return 0;
+}
+
+template <int NFP>
+static void launch_accept(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t B, int i, int j,
+                          int64_t *cfg, float *z1, double *sign, double *logabs,
+                          int64_t *n_accept, hipStream_t st) {
+  constexpr int WPW = 64 / NFP;
+  const int64_t per_block = 4 * WPW;
+  const int64_t nblk = (B + per_block - 1) / per_block;
+  hipLaunchKernelGGL(nnbf_accept_kernel<NFP>, dim3((unsigned)nblk), dim3(256),
+                     (size_t)per_block * NFP * 9, st, B, m.n_sites, m.n_f, m.hidden, i, j, cfg,
+                     L.prop, L.moves, L.uacc, L.D, m.M, m.bout, m.bf_scale, m.W1, z1, sign,
+                     logabs, n_accept);
+}
+
+int nnbf_sweep(const vmcx_nnbf_model *m, int64_t B, int n_edges, const int32_t *edges_host,
+               double hopping_rate, int n_sweeps, uint64_t seed, int64_t sweep_offset,
+               const double *u_choice, const uint8_t *bits, const double *u_acc,
+               int64_t *cfg, float *z1, double *sign, double *logabs, int64_t *n_accept,
+               void *ws, int64_t ws_bytes, hipStream_t st) {
+  if (check_model(m)) return 1;
+  VMCX_CHECK_ARG(B >= 0 && n_edges >= 0 && n_sweeps >= 0, "negative size");
+  VMCX_CHECK_ARG(B == 0 || (cfg && z1 && sign && logabs && ws), "null pointer");
+  VMCX_CHECK_ARG(n_edges == 0 || edges_host, "null edges");
+  VMCX_CHECK_ARG((u_choice == nullptr) == (u_acc == nullptr) &&
+                     (u_choice == nullptr) == (bits == nullptr),
+                 "injected draws need u_choice, bits and u_acc together");
+  if (B == 0) return 0;
+  const NnbfWs L = nnbf_layout(*m, B, B, 0, (char *)ws);
+  VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small (chunk must be >= B)");
+  const int N = m->n_sites;
+  for (int e = 0; e < n_edges; ++e) {
+    const int i = edges_host[2 * e], j = edges_host[2 * e + 1];
+    VMCX_CHECK_ARG(i >= 0 && i < N && j >= 0 && j < N && i != j, "edge out of range");
+  }
+  const unsigned gw = (unsigned)((B + 255) / 256);
+  const int64_t nh = B * m->hidden;
+  const unsigned gh = (unsigned)((nh + 255) / 256 < 16384 ? (nh + 255) / 256 : 16384);
+  for (int sw = 0; sw < n_sweeps; ++sw) {
+    for (int e = 0; e < n_edges; ++e) {
+      const int i = edges_host[2 * e], j = edges_host[2 * e + 1];
+      const size_t di = ((size_t)sw * n_edges + e) * B;
+      hipLaunchKernelGGL(nnbf_propose_kernel, dim3(gw), dim3(256), 0, st, B, N, i, j, e,
+                         sweep_offset + sw, seed, 1.0 - hopping_rate,
+                         u_choice ? u_choice + di : nullptr, bits ? bits + di : nullptr,
+                         u_acc ? u_acc + di : nullptr, cfg, L.prop, L.moves, L.uacc);
+      VMCX_LAUNCHED();
+      if (m->gemm == 1)
+        hipLaunchKernelGGL(nnbf_h1_proposed_kernel<__hip_bfloat16>, dim3(gh), dim3(256), 0, st,
+                           B, N, m->hidden, m->act, m->W1, z1, L.prop, L.moves,
+                           (__hip_bfloat16 *)L.h1);
+      else
+        hipLaunchKernelGGL(nnbf_h1_proposed_kernel<float>, dim3(gh), dim3(256), 0, st, B, N,
+                           m->hidden, m->act, m->W1, z1, L.prop, L.moves, (float *)L.h1);
+      VMCX_LAUNCHED();
+      int rc = dense_layers(*m, L, B, st);
+      if (rc) return rc;
+      switch (nfp_of(m->n_f)) {
+        case 8: launch_accept<8>(*m, L, B, i, j, cfg, z1, sign, logabs, n_accept, st); break;
+        case 16: launch_accept<16>(*m, L, B, i, j, cfg, z1, sign, logabs, n_accept, st); break;
+        case 32: launch_accept<32>(*m, L, B, i, j, cfg, z1, sign, logabs, n_accept, st); break;
+        default: launch_accept<64>(*m, L, B, i, j, cfg, z1, sign, logabs, n_accept, st); break;
+      }
+      VMCX_LAUNCHED();
+    }
+  }
+  return 0;
+}
+
+int nnbf_local_energy(const vmcx_nnbf_model *m, int64_t B, int n_edges, const int32_t *edges,
+                      double t, double U, const int64_t *cfg, const double *sign,
+                      const double *logabs, void *ws, int64_t ws_bytes, int64_t chunk,
+                      double *eloc, hipStream_t st) {
+  if (check_model(m)) return 1;
+  VMCX_CHECK_ARG(B >= 0 && n_edges >= 0 && chunk >= 1, "bad sizes");
+  if (B == 0) return 0;
+  VMCX_CHECK_ARG(cfg && sign && logabs && ws && eloc && (n_edges == 0 || edges),
+                 "null pointer");
+  const NnbfWs L = nnbf_layout(*m, B, chunk, n_edges, (char *)ws);
+  VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
+  const int N = m->n_sites;
+  const unsigned gw = (unsigned)((B + 255) / 256);
+  hipLaunchKernelGGL(conn_count_walker_kernel, dim3(gw), dim3(256), 0, st, B, N, n_edges,
+                     edges, cfg, L.counts);
+  VMCX_LAUNCHED();
+  hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, st, B, L.counts,
+                     L.offsets, L.total);
+  VMCX_LAUNCHED();
+  hipLaunchKernelGGL(conn_fill_walker_kernel, dim3(gw), dim3(256), 0, st, B, N, n_edges,
+                     edges, t, cfg, L.offsets, L.etas, L.coeffs);
+  VMCX_LAUNCHED();
+  int64_t T = 0;
+  VMCX_HIP(hipMemcpyAsync(&T, L.total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  VMCX_HIP(hipStreamSynchronize(st));
+  for (int64_t r0 = 0; r0 < T; r0 += chunk) {
+    const int64_t rows = (T - r0 < chunk) ? T - r0 : chunk;
+    const int rc = forward_rows(*m, L, rows, L.etas + r0 * N, L.csign + r0, L.clog + r0,
+                                nullptr, st);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(eloc_segment_kernel, dim3(gw), dim3(256), 0, st, B, N, U, cfg, L.offsets,
+                     L.counts, L.coeffs, L.csign, L.clog, sign, logabs, eloc);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+int nnbf_logderiv(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
+                  int64_t ws_bytes, int64_t chunk, void *O, int o_dtype, int64_t ld,
+                  double *sign, double *logabs, hipStream_t st) {
+  if (check_model(m)) return 1;
+  VMCX_CHECK_ARG(B >= 0 && chunk >= 1, "bad sizes");
+  VMCX_CHECK_ARG(o_dtype == 0 || o_dtype == 1, "o_dtype must be 0 (f64) or 1 (f32)");
+  if (B == 0) return 0;
+  VMCX_CHECK_ARG(cfg && ws && O && sign && logabs, "null pointer");
+  const int N = m->n_sites, H = m->hidden, nf = m->n_f, nD = 2 * N * nf;
+  const int64_t np = (int64_t)nD + (int64_t)H * 2 * N + H + (int64_t)H * H + H +
+                     (int64_t)nD * H + nD;
+  VMCX_CHECK_ARG(ld >= np, "ld smaller than the parameter count");
+  const NnbfWs L = nnbf_layout(*m, B, chunk, 0, (char *)ws);
+  VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
+  const size_t lds = (size_t)(nD + 4 * H + 2 * N) * sizeof(float);
+  VMCX_CHECK_ARG(lds <= 65536, "backward LDS exceeds 64 KB");
+  for (int64_t r0 = 0; r0 < B; r0 += chunk) {
+    const int64_t rows = (B - r0 < chunk) ? B - r0 : chunk;
+    int rc = forward_rows(*m, L, rows, cfg + r0 * N, sign + r0, logabs + r0, L.ainv, st);
+    if (rc) return rc;
+    if (o_dtype == 0)
+      hipLaunchKernelGGL(nnbf_backward_kernel<double>, dim3((unsigned)rows), dim3(256), lds,
+                         st, rows, N, nf, H, m->act, nfp_of(nf), cfg + r0 * N, L.z1, L.z2,
+                         L.ainv, m->W2, m->Wout, m->bf_scale, (double *)O + r0 * ld, ld);
+    else
+      hipLaunchKernelGGL(nnbf_backward_kernel<float>, dim3((unsigned)rows), dim3(256), lds, st,
+                         rows, N, nf, H, m->act, nfp_of(nf), cfg + r0 * N, L.z1, L.z2, L.ainv,
+                         m->W2, m->Wout, m->bf_scale, (float *)O + r0 * ld, ld);
+    VMCX_LAUNCHED();
+  }
+  return 0;
+}
+
+// FP32 -> BF16 conversion of weight copies
+__global__ void f32_to_bf16_kernel(int64_t n, const float *x, __hip_bfloat16 *y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = __float2bfloat16(x[i]);
+}
+
+int to_bf16(int64_t n, const float *x, void *y, hipStream_t st) {
+  VMCX_CHECK_ARG(n >= 0, "n < 0");
+  if (n == 0) return 0;
+  VMCX_CHECK_ARG(x && y, "null pointer");
+  const unsigned g = (unsigned)((n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384);
+  hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(g), dim3(256), 0, st, n, x, (__hip_bfloat16 *)y);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+}  // namespace vmcx

@@ -10,15 +10,10 @@
 //     per lane;
 //   * the log-derivative row of an occupied orbital is a contiguous column
 //     of A^-1 in HBM (column k stored at ainv + (b*NFP + k)*NFP).
+#include "gj.h"
 #include "kernels.h"
 
 namespace vmcx {
-
-__device__ __forceinline__ void wave_lds_fence() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <int NFP>
 __device__ __forceinline__ double group_sum(double v) {
@@ -108,63 +103,12 @@ __global__ void __launch_bounds__(256)
     }
   }
 
-  double logdet = 0.0;
-  int parity = 0;      // sign flips: negative pivots + permutation inversions
-  uint64_t used = 0;   // group lanes already used as pivot rows
-  int my_step = -1;
   bool singular = !count_ok;
-  const int gbase = grp * NFP;
-
-#pragma unroll 1
-  for (int p = 0; p < NFP; ++p) {
-    // group argmax of |a[0]| over unused lanes (ties -> lowest lane)
-    // NaN / Inf entries rank as 0 so that the order stays total and every
-    // lane is chosen exactly once over the NFP steps
-    const double mag = fabs(a[0]);
-    double cand = ((used >> col) & 1ull) ? -1.0 : ((mag <= 1e300) ? mag : 0.0);
-    int cidx = col;
-#pragma unroll
-    for (int o = NFP / 2; o > 0; o >>= 1) {
-      const double oc = __shfl_xor(cand, o, 64);
-      const int oi = __shfl_xor(cidx, o, 64);
-      if (oc > cand || (oc == cand && oi < cidx)) {
-        cand = oc;
-        cidx = oi;
-      }
-    }
-    const int q = cidx;
-    if (!(cand > 0.0)) singular = true;
-    const double piv = __shfl(a[0], gbase + q, 64);
-    const double f = a[0];
-    if (col == q) {
-      const double inv = (piv != 0.0) ? 1.0 / piv : 0.0;
-#pragma unroll
-      for (int j = 1; j < NFP; ++j) a[j] *= inv;
-      a[0] = inv;
-#pragma unroll
-      for (int j = 0; j < NFP; ++j) prow[j] = a[j];
-      qbuf[p] = (uint8_t)q;
-      my_step = p;
-    }
-    wave_lds_fence();
-    if (col != q) {
-      a[0] = 0.0;
-#pragma unroll
-      for (int j = 0; j < NFP; ++j) a[j] = fma(-f, prow[j], a[j]);
-    }
-    wave_lds_fence();
-    logdet += (piv != 0.0) ? log(fabs(piv)) : 0.0;
-    parity += (piv < 0.0) ? 1 : 0;
-    parity += __popcll(used & ~((2ull << q) - 1ull));
-    used |= 1ull << q;
-    // rotate registers left by one
-    const double a0 = a[0];
-#pragma unroll
-    for (int j = 0; j < NFP - 1; ++j) a[j] = a[j + 1];
-    a[NFP - 1] = a0;
-  }
-  wave_lds_fence();
-
+  const GJResult r = gj_invert<NFP>(a, col, grp * NFP, prow, qbuf);
+  singular = singular || r.singular;
+  const double logdet = r.logdet;
+  const int parity = r.parity;
+  const int my_step = r.my_step;
   if (col == 0) {
     if (singular) {
       sign[w] = 0.0;

@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(1024)
 }
 
 // --------------------------------------------------------------- handles
-static rocblas_handle blas_handle_for(hipStream_t st) {
+rocblas_handle blas_handle_for(hipStream_t st) {
   static std::mutex mu;
   static std::unordered_map<int, rocblas_handle> handles;
   int dev = 0;

@@ -214,3 +214,38 @@ def sgd_update(param, dp, lr):
     require(dp, F64, "dp", (param.numel(),), param.device)
     check(lib().vmcx_sgd_update(param.numel(), ptr(param), ptr(dp), float(lr),
                                 stream_ptr(param.device)))
+
+
+# ------------------------------------------------------------------ NNBF
+def nnbf_struct(n_sites, n_f, hidden, act, gemm, bf_scale, w32, wbf):
+    """Fill the vmcx_nnbf_model struct from FP32 (and BF16) device copies.
+    w32: dict M, W1, b1, W2, b2, Wout, bout; wbf: dict W2, Wout or None."""
+    s = _lib.NnbfModelStruct()
+    s.n_sites, s.n_f, s.hidden = n_sites, n_f, hidden
+    s.act, s.gemm, s.reserved = act, gemm, 0
+    s.bf_scale = float(bf_scale)
+    for k in ("M", "W1", "b1", "W2", "b2", "Wout", "bout"):
+        t = w32[k]
+        require(t, torch.float32, k)
+        setattr(s, k, t.data_ptr())
+    s.W2_bf16 = wbf["W2"].data_ptr() if wbf else None
+    s.Wout_bf16 = wbf["Wout"].data_ptr() if wbf else None
+    return s
+
+
+def nnbf_workspace(struct, B, chunk, n_edges, device, holder):
+    import ctypes
+    need = int(lib().vmcx_nnbf_workspace_bytes(ctypes.byref(struct), int(B), int(chunk),
+                                                int(n_edges)))
+    if need < 0:
+        raise ValueError("bad NNBF workspace sizes")
+    buf = holder.get("ws")
+    if buf is None or buf.numel() < need or buf.device != device:
+        buf = torch.empty(need, dtype=torch.uint8, device=device)
+        holder["ws"] = buf
+    return buf
+
+
+def to_bf16(x, out):
+    check(lib().vmcx_to_bf16(x.numel(), ptr(x), ptr(out), stream_ptr(x.device)))
+    return out

@@ -1,12 +1,17 @@
 """Wavefunction models with the reference's GPU interface
-(vmc_torch/GPU/models/_base.py, slater.py).
+(vmc_torch/GPU/models/_base.py, slater.py, NNBF.py).
 
-A model keeps the parameters as an nn.ParameterList (the order defines the
-flat O_k / dp layout, _base.py:41-44) and exposes forward / forward_log.
-Evaluation runs in libvmcx; the model additionally caches the device-side
-walker state (A^-1, slot map, sign, ln|psi|) that the sampler, the local
-energy and the log-derivative kernels share.
+A model keeps its parameters as an nn.ParameterList whose order defines the
+flat O_k / dp layout (_base.py:41-44) and exposes forward / forward_log.
+Beyond the reference interface every model implements the three hot-path
+operations on a walker batch, all in libvmcx:
+    mcmc_sweep(fxs, edges, n_sweeps, seed, sweep_offset, hopping_rate, draws)
+    local_energy(fxs, H) -> E_loc [B]
+    log_derivative(fxs, dtype) -> (O [B, Np], (sign, log|psi|))
+and caches the device-side walker state they share.
 """
+import ctypes
+
 import torch
 import torch.nn as nn
 
@@ -21,6 +26,9 @@ class WavefunctionModel_GPU(nn.Module):
         super().__init__()
         self.params = nn.ParameterList([
             p if isinstance(p, nn.Parameter) else nn.Parameter(p) for p in params_list])
+        self._snapshot = None
+        self._state = None
+        self._state_key = None
 
     def forward_log(self, x):
         raise NotImplementedError
@@ -33,14 +41,34 @@ class WavefunctionModel_GPU(nn.Module):
     def n_params(self):
         return sum(p.numel() for p in self.params)
 
+    # ----------------------------------------------- parameter change tracking
+    def _flat_params(self):
+        return torch.cat([p.detach().reshape(-1) for p in self.params])
+
+    def params_changed(self):
+        """True when the parameters differ (by value) from the last call that
+        built derived device data; optimizers may write through .data."""
+        flat = self._flat_params()
+        if self._snapshot is None or self._snapshot.shape != flat.shape or \
+                self._snapshot.device != flat.device or not torch.equal(self._snapshot, flat):
+            self._snapshot = flat.clone()
+            return True
+        return False
+
+    def invalidate(self):
+        self._state_key = None
+        self._snapshot = None
+
+    def _walker_key(self, fxs):
+        return (fxs.data_ptr(), tuple(fxs.shape), fxs._version)
+
 
 class SlaterDeterminant_GPU(WavefunctionModel_GPU):
     """psi(x) = det(M[occupied, :]), occupied = netket-ordered occupied
     orbitals ascending (GPU/models/slater.py:15-61; CPU model.py:7437-7457).
 
     M: (n_orbitals = 2*N_sites, n_fermions) float64.  `init_std` scales the
-    N(0, 1) draw (the reference uses std 1; the benchmark configurations use
-    0.1).
+    N(0, 1) draw (the reference uses std 1; the benchmark shapes use 0.1).
     """
 
     def __init__(self, n_orbitals, n_fermions, dtype=torch.float64, init_std=1.0,
@@ -52,9 +80,6 @@ class SlaterDeterminant_GPU(WavefunctionModel_GPU):
         self.n_orbitals = n_orbitals
         self.n_fermions = n_fermions
         self.n_sites = n_orbitals // 2
-        self._state = None
-        self._state_key = None
-        self._m_snapshot = None
 
     @property
     def M(self):
@@ -63,31 +88,193 @@ class SlaterDeterminant_GPU(WavefunctionModel_GPU):
     def forward_log(self, x):
         return kernels.slater_refresh(self.M.detach(), x.contiguous())
 
-    # ---------------------------------------------------- walker state cache
-    def _key(self, fxs):
-        return (fxs.data_ptr(), tuple(fxs.shape), fxs._version,
-                self.M.data_ptr(), self.M._version)
-
-    def _params_unchanged(self):
-        snap = self._m_snapshot
-        return snap is not None and torch.equal(snap, self.M.detach())
-
     def walker_state(self, fxs, refresh=False):
         """State for the walker batch `fxs`, rebuilt exactly when fxs or M
-        changed since it was last built (M is compared by value, since
-        optimizers may write it through .data)."""
-        fxs_ok = self._state is not None and self._state_key == self._key(fxs)
-        if not refresh and fxs_ok and self._params_unchanged():
+        changed since it was last built."""
+        changed = self.params_changed()
+        if not refresh and not changed and self._state is not None and \
+                self._state_key == self._walker_key(fxs):
             return self._state
         B = fxs.shape[0]
         st = self._state
         if st is None or st.ainv.shape[0] != B or st.ainv.device != fxs.device:
             st = kernels.SlaterWalkerState.empty(B, self.n_sites, self.n_fermions, fxs.device)
         kernels.slater_refresh(self.M.detach(), fxs, st)
-        self._state = st
-        self._state_key = self._key(fxs)
-        self._m_snapshot = self.M.detach().clone()
+        self._state, self._state_key = st, self._walker_key(fxs)
         return st
 
-    def invalidate(self):
-        self._state_key = None
+    def mcmc_sweep(self, fxs, edges, n_sweeps, seed, sweep_offset, hopping_rate=0.25,
+                   refresh_every=64, draws=None):
+        st = self.walker_state(fxs)
+        kernels.slater_sweep(self.M.detach(), edges, fxs, st, n_sweeps=n_sweeps, seed=seed,
+                             sweep_offset=sweep_offset, hopping_rate=hopping_rate,
+                             refresh_every=refresh_every, draws=draws)
+        return st.sign, st.logabs, st.n_accept
+
+    def local_energy(self, fxs, H):
+        st = self.walker_state(fxs)
+        edges = H.graph.edge_tensor(fxs.device, "graph")
+        return kernels.slater_local_energy(self.M.detach(), edges, H.t, H.U, fxs, st)
+
+    def log_derivative(self, fxs, dtype=torch.float64):
+        st = self.walker_state(fxs)
+        O = kernels.slater_logderiv(st, self.n_sites, self.n_fermions, dtype=dtype)
+        return O, (st.sign.clone(), st.logabs.clone())
+
+
+class NNBF_GPU(WavefunctionModel_GPU):
+    """Neural-network backflow (GPU/models/NNBF.py:36-162):
+    psi(x) = det((M + bf_scale * reshape(W_out h + b_out))[occupied]), with
+    h the output of `n_layers` = 2 hidden layers of width `hidden_dim`.
+
+    Parameter order [M, W1, b1, W2, b2, W_out, b_out] (NNBF.py:44-48).
+    init="reference": M ~ N(0,1), kaiming-uniform W, zero b, W_out = b_out = 0
+    (NNBF.py:86-111); init="xavier": M ~ N(0, m_std^2), Xavier-uniform for
+    every W including W_out, zero biases (the benchmark configuration).
+    gemm: "fp32" or "bf16" for the two dense layers (FP32 accumulation).
+    """
+
+    ACTS = {"relu": 0, "tanh": 1}
+
+    def __init__(self, n_sites, n_fermions, hidden_dim=64, n_layers=2, activation="tanh",
+                 bf_scale=0.01, dtype=torch.float64, gemm="fp32", init="reference",
+                 m_std=1.0, generator=None, chunk=65536):
+        if n_layers != 2:
+            raise NotImplementedError("the NNBF kernels take exactly two hidden layers")
+        if activation not in self.ACTS:
+            raise ValueError(f"activation must be one of {list(self.ACTS)}")
+        n_orb = 2 * n_sites
+        g = generator
+        if init == "reference":
+            M = torch.randn(n_orb, n_fermions, dtype=dtype, generator=g)
+            W1 = torch.empty(hidden_dim, n_orb, dtype=dtype)
+            W2 = torch.empty(hidden_dim, hidden_dim, dtype=dtype)
+            for w in (W1, W2):
+                bound = (3.0 / w.shape[1]) ** 0.5 * (5.0 / 3.0)   # kaiming_uniform, tanh gain
+                w.uniform_(-bound, bound, generator=g)
+            Wo = torch.zeros(n_orb * n_fermions, hidden_dim, dtype=dtype)
+        else:
+            M = torch.randn(n_orb, n_fermions, dtype=dtype, generator=g) * m_std
+
+            def xav(fo, fi):
+                a = (6.0 / (fi + fo)) ** 0.5
+                return torch.empty(fo, fi, dtype=dtype).uniform_(-a, a, generator=g)
+            W1, W2, Wo = xav(hidden_dim, n_orb), xav(hidden_dim, hidden_dim), \
+                xav(n_orb * n_fermions, hidden_dim)
+        params = [M, W1, torch.zeros(hidden_dim, dtype=dtype), W2,
+                  torch.zeros(hidden_dim, dtype=dtype), Wo,
+                  torch.zeros(n_orb * n_fermions, dtype=dtype)]
+        super().__init__(params)
+        self.n_sites, self.n_fermions, self.hidden = n_sites, n_fermions, hidden_dim
+        self.n_layers, self.activation, self.bf_scale = n_layers, activation, bf_scale
+        self.gemm = gemm
+        self.chunk = chunk
+        self._w32 = None
+        self._wbf = None
+        self._ws = {}
+        self._struct_obj = None
+
+    # ------------------------------------------------------ device weights
+    def _sync_weights(self):
+        if self.params_changed() or self._w32 is None:
+            names = ["M", "W1", "b1", "W2", "b2", "Wout", "bout"]
+            self._w32 = {k: p.detach().to(torch.float32).contiguous()
+                         for k, p in zip(names, self.params)}
+            if self.gemm == "bf16":
+                self._wbf = {k: kernels.to_bf16(self._w32[k], torch.empty_like(
+                    self._w32[k], dtype=torch.bfloat16)) for k in ("W2", "Wout")}
+            else:
+                self._wbf = None
+            self._struct_obj = kernels.nnbf_struct(
+                self.n_sites, self.n_fermions, self.hidden, self.ACTS[self.activation],
+                1 if self.gemm == "bf16" else 0, self.bf_scale, self._w32, self._wbf)
+            self._state_key = None
+            return True
+        return False
+
+    def _ws_for(self, B, chunk, n_edges, device):
+        return kernels.nnbf_workspace(self._struct_obj, B, chunk, n_edges, device, self._ws)
+
+    def forward_log(self, x):
+        x = x.contiguous()
+        kernels.require(x, torch.int64, "configs")
+        self._sync_weights()
+        B = x.shape[0]
+        sign = torch.empty(B, dtype=torch.float64, device=x.device)
+        logabs = torch.empty(B, dtype=torch.float64, device=x.device)
+        chunk = min(self.chunk, max(B, 1))
+        ws = self._ws_for(B, chunk, 0, x.device)
+        kernels.check(kernels.lib().vmcx_nnbf_forward(
+            ctypes.byref(self._struct_obj), B, kernels.ptr(x), kernels.ptr(ws), ws.numel(),
+            chunk, kernels.ptr(sign), kernels.ptr(logabs), None, kernels.stream_ptr(x.device)))
+        return sign, logabs
+
+    # --------------------------------------------------------- walker state
+    def walker_state(self, fxs, refresh=False):
+        changed = self._sync_weights()
+        if not refresh and not changed and self._state is not None and \
+                self._state_key == self._walker_key(fxs):
+            return self._state
+        B = fxs.shape[0]
+        st = {"z1": torch.empty(B, self.hidden, dtype=torch.float32, device=fxs.device),
+              "sign": torch.empty(B, dtype=torch.float64, device=fxs.device),
+              "logabs": torch.empty(B, dtype=torch.float64, device=fxs.device),
+              "n_accept": (self._state["n_accept"] if self._state is not None and
+                           self._state["n_accept"].shape[0] == B
+                           else torch.zeros(B, dtype=torch.int64, device=fxs.device))}
+        chunk = min(self.chunk, max(B, 1))
+        ws = self._ws_for(B, chunk, 0, fxs.device)
+        kernels.check(kernels.lib().vmcx_nnbf_forward(
+            ctypes.byref(self._struct_obj), B, kernels.ptr(fxs), kernels.ptr(ws), ws.numel(),
+            chunk, kernels.ptr(st["sign"]), kernels.ptr(st["logabs"]), kernels.ptr(st["z1"]),
+            kernels.stream_ptr(fxs.device)))
+        self._state, self._state_key = st, self._walker_key(fxs)
+        return st
+
+    def mcmc_sweep(self, fxs, edges, n_sweeps, seed, sweep_offset, hopping_rate=0.25,
+                   refresh_every=64, draws=None):
+        st = self.walker_state(fxs)
+        B = fxs.shape[0]
+        edges_host = edges.detach().to("cpu", torch.int32).contiguous()
+        uc = bits = ua = None
+        if draws is not None:
+            uc, bits, ua = draws
+        ws = self._ws_for(B, B, 0, fxs.device)
+        kernels.check(kernels.lib().vmcx_nnbf_sweep(
+            ctypes.byref(self._struct_obj), B, edges_host.shape[0],
+            ctypes.c_void_p(edges_host.data_ptr()), float(hopping_rate), int(n_sweeps),
+            int(seed) & (2 ** 64 - 1), int(sweep_offset), kernels.ptr(uc), kernels.ptr(bits),
+            kernels.ptr(ua), kernels.ptr(fxs), kernels.ptr(st["z1"]), kernels.ptr(st["sign"]),
+            kernels.ptr(st["logabs"]), kernels.ptr(st["n_accept"]), kernels.ptr(ws), ws.numel(),
+            kernels.stream_ptr(fxs.device)))
+        return st["sign"], st["logabs"], st["n_accept"]
+
+    def local_energy(self, fxs, H):
+        st = self.walker_state(fxs)
+        B = fxs.shape[0]
+        edges = H.graph.edge_tensor(fxs.device, "graph")
+        chunk = self.chunk
+        ws = self._ws_for(B, chunk, edges.shape[0], fxs.device)
+        eloc = torch.empty(B, dtype=torch.float64, device=fxs.device)
+        kernels.check(kernels.lib().vmcx_nnbf_local_energy(
+            ctypes.byref(self._struct_obj), B, edges.shape[0], kernels.ptr(edges), float(H.t),
+            float(H.U), kernels.ptr(fxs), kernels.ptr(st["sign"]), kernels.ptr(st["logabs"]),
+            kernels.ptr(ws), ws.numel(), chunk, kernels.ptr(eloc),
+            kernels.stream_ptr(fxs.device)))
+        return eloc
+
+    def log_derivative(self, fxs, dtype=torch.float32, out=None, chunk=4096):
+        self._sync_weights()
+        B = fxs.shape[0]
+        Np = self.n_params
+        O = torch.empty(B, Np, dtype=dtype, device=fxs.device) if out is None else out
+        sign = torch.empty(B, dtype=torch.float64, device=fxs.device)
+        logabs = torch.empty(B, dtype=torch.float64, device=fxs.device)
+        chunk = min(chunk, max(B, 1))
+        ws = self._ws_for(B, chunk, 0, fxs.device)
+        code = {torch.float64: 0, torch.float32: 1}[dtype]
+        kernels.check(kernels.lib().vmcx_nnbf_logderiv(
+            ctypes.byref(self._struct_obj), B, kernels.ptr(fxs), kernels.ptr(ws), ws.numel(),
+            chunk, kernels.ptr(O), code, O.stride(0), kernels.ptr(sign), kernels.ptr(logabs),
+            kernels.stream_ptr(fxs.device)))
+        return O, (sign, logabs)

@@ -11,7 +11,6 @@ from typing import Tuple
 
 import torch
 
-from . import kernels
 
 
 class SamplerGPU:
@@ -45,28 +44,29 @@ class MetropolisExchangeSpinfulSamplerGPU(SamplerGPU):
 
     @torch.inference_mode()
     def sweeps(self, fxs, model, graph, n_sweeps=1, draws=None):
-        """n_sweeps sweeps in place; returns the walker state."""
-        st = model.walker_state(fxs)
+        """n_sweeps sweeps in place (model.mcmc_sweep runs them in libvmcx);
+        returns (sign, log|psi|) of the walkers afterwards."""
         edges = graph.edge_tensor(fxs.device, "sweep")
-        acc0 = st.n_accept.clone() if n_sweeps else None
-        kernels.slater_sweep(model.M.detach(), edges, fxs, st, n_sweeps=n_sweeps,
-                             seed=self.seed, sweep_offset=self.sweep_counter,
-                             hopping_rate=self.hopping_rate,
-                             refresh_every=self.refresh_every, draws=draws)
+        sign, logabs, n_acc = model.mcmc_sweep(fxs, edges, 0, self.seed, self.sweep_counter,
+                                               self.hopping_rate, self.refresh_every)
+        acc0 = n_acc.clone()
+        sign, logabs, n_acc = model.mcmc_sweep(fxs, edges, n_sweeps, self.seed,
+                                               self.sweep_counter, self.hopping_rate,
+                                               self.refresh_every, draws=draws)
         self.sweep_counter += n_sweeps
         if n_sweeps:
-            self.last_acceptance = (st.n_accept - acc0, n_sweeps * edges.shape[0])
-        return st
+            self.last_acceptance = (n_acc - acc0, n_sweeps * edges.shape[0])
+        return sign, logabs
 
     @torch.inference_mode()
     def step(self, fxs, model, graph, compile=False, verbose=False, use_log_amp=False,
              draws=None, **kwargs):
         """One sweep over all edges.  Returns (fxs, amps) or, with
         use_log_amp, (fxs, (signs, log_abs)); fxs is updated in place."""
-        st = self.sweeps(fxs, model, graph, 1, draws=draws)
+        sign, logabs = self.sweeps(fxs, model, graph, 1, draws=draws)
         if use_log_amp:
-            return fxs, (st.sign.clone(), st.logabs.clone())
-        return fxs, st.sign * torch.exp(st.logabs)
+            return fxs, (sign.clone(), logabs.clone())
+        return fxs, sign * torch.exp(logabs)
 
     def burn_in(self, fxs, model, graph, n_steps, **kwargs):
         self.sweeps(fxs, model, graph, int(n_steps))

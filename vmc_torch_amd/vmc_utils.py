@@ -2,7 +2,6 @@
 the reference's interface (GPU/vmc_utils.py, GPU/vmc_setup.py)."""
 import torch
 
-from . import kernels
 
 
 def random_initial_config(N_f, N_sites, seed=None):
@@ -27,43 +26,41 @@ def initialize_walkers(init_fn, batch_size, seed=42, rank=0, device=None):
     return torch.stack(rows).to(device)
 
 
-def _slater_like(model):
-    return hasattr(model, "walker_state")
+def _check_model(model, what):
+    if not hasattr(model, what):
+        raise TypeError(f"{type(model).__name__} has no HIP {what} path")
 
 
 @torch.inference_mode()
 def evaluate_energy(fxs, model, H, current_amps=None, verbose=False, use_log_amp=False, **kwargs):
     """GPU/vmc_utils.py:479-489: returns (E_mean, E_loc[B]).
 
-    The local energy E_loc(x) = sum_eta H[eta, x] psi(eta)/psi(x) is formed
-    in one HIP launch from the walker state; current_amps is accepted for
-    interface compatibility (the state already carries sign and ln|psi|).
+    E_loc(x) = sum_eta H[eta, x] psi(eta)/psi(x) is formed in libvmcx from the
+    model's walker state; current_amps is accepted for interface
+    compatibility (the state already carries sign and ln|psi|).
     """
-    if not _slater_like(model):
-        raise TypeError(f"{type(model).__name__} has no HIP local-energy path")
-    st = model.walker_state(fxs)
-    edges = H.graph.edge_tensor(fxs.device, "graph")
-    eloc = kernels.slater_local_energy(model.M.detach(), edges, H.t, H.U, fxs, st)
+    _check_model(model, "local_energy")
+    eloc = model.local_energy(fxs, H)
     return eloc.mean(), eloc
 
 
 def compute_grads_gpu(fxs, model, batch_size=None, verbose=False, offload_to_cpu=False,
-                      use_log_amp=False, out_dtype=torch.float64, **kwargs):
+                      use_log_amp=False, out_dtype=None, **kwargs):
     """GPU/vmc_utils.py:1389-1634: per-sample gradients in the flat
     parameter order.  With use_log_amp, returns (O = d ln|psi|/d theta,
     (signs, log_abs)); otherwise (d psi/d theta, psi) as the reference, whose
     caller divides by psi."""
-    if not _slater_like(model):
-        raise TypeError(f"{type(model).__name__} has no HIP log-derivative path")
+    _check_model(model, "log_derivative")
     with torch.inference_mode():
-        st = model.walker_state(fxs)
-        O = kernels.slater_logderiv(st, model.n_sites, model.n_fermions, dtype=out_dtype)
-        if use_log_amp:
-            out = O, (st.sign.clone(), st.logabs.clone())
+        if out_dtype is None:
+            O, (sign, logabs) = model.log_derivative(fxs)
         else:
-            amps = st.sign * torch.exp(st.logabs)
+            O, (sign, logabs) = model.log_derivative(fxs, dtype=out_dtype)
+        if use_log_amp:
+            out = O, (sign, logabs)
+        else:
+            amps = sign * torch.exp(logabs)
             out = O * amps.unsqueeze(1).to(O.dtype), amps
     if offload_to_cpu:
-        O_cpu = out[0].cpu()
-        return (O_cpu, out[1])
+        return (out[0].cpu(), out[1])
     return out
