@@ -205,7 +205,7 @@ def main():
         m4 = mark()
         ph = [] if record == "phases" else None
         _, stats, info = kernels.sr_minsr(O, E, None, a.diag_shift, a.gram,
-                                          0 if a.gram == "fp64" else 3, ws, dp=dp,
+                                          0 if a.gram == "fp64" else 20, 1e-8, ws, dp=dp,
                                           phase_ms=ph)
         if ph is not None:
             sr_phases.append(ph)
@@ -322,7 +322,8 @@ def main():
         "sweep_only_walker_sweeps_per_s": B / (kt["sweep"] * 1e-3),
         "tau_int": tau, "acceptance": acc_rate,
         "energy_per_site_first_last": [e_site[0], e_site[-1]],
-        "sr_solver": {"info": info_v, "residual": st_v[2], "grad_norm": st_v[3]},
+        "sr_solver": {"info": info_v, "rel_residual": st_v[2], "grad_norm": st_v[3],
+                      "pcg_iters": st_v[4]},
         "roofline": roof,
         "kernels": kern,
     }

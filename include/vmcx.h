@@ -130,28 +130,33 @@ int vmcx_hubbard_conn_fill(int64_t B, int n_sites, int n_edges,
  * Stochastic reconfiguration in sample space (minSR):
  *   Ot = (O - mean_s O)/sqrt(Ns), eps = (E - E_mean)/sqrt(Ns),
  *   T = Ot Ot^T + diag_shift I, alpha = T^-1 eps, dp = Ot^T alpha.
- *   O: [Ns][ldo] float64 row-major (per-sample log-derivatives), E: [Ns].
+ *   O: [Ns][ldo] row-major per-sample log-derivatives, float64 (o_dtype 0)
+ *   or float32 (o_dtype 1).  flags bit 0: centre O in place (O is
+ *   overwritten by Ot, saving an Ns x Np copy).  E: [Ns] float64.
  *   energy_mean: the mean used to centre E; NaN = the sample mean.
- *   gram_dtype: 0 = FP64 Gram (SYRK); 1 = FP32, 2 = BF16 operands with FP32
- *   accumulation.  T is factorised by FP64 Cholesky; `refine` rounds of
- *   iterative refinement against the FP64 operator follow.
- *   workspace: >= vmcx_sr_workspace_bytes(Ns, Np, gram_dtype) device bytes.
- *   stats (may be NULL): [4] float64 = mean E, var E, ||T alpha - eps||, ||g||
- *   with g = Ot^T eps the energy gradient.  info (may be NULL): Cholesky
- *   info (0 = success, k > 0 = leading minor k not positive).
+ *   gram_dtype: 0 = FP64 Gram (SYRK; needs FP64 O); 1 = FP32 and 2 = BF16
+ *   operands with FP32 accumulation.  The shifted Gram is factorised by a
+ *   blocked FP64 Cholesky.  max_iter > 0: preconditioned conjugate gradients
+ *   on the exact operator Ot Ot^T + shift (FP64 accumulation), with that
+ *   factor as the preconditioner, until ||res|| <= tol ||eps||.
+ *   workspace: >= vmcx_sr_workspace_bytes(Ns, Np, o_dtype, gram_dtype, flags).
+ *   stats (may be NULL): [5] float64 = mean E, var E, ||T alpha - eps|| /
+ *   ||eps||, ||g|| (g = Ot^T eps, the energy gradient), PCG iterations.
+ *   info (may be NULL): Cholesky info (0 = success, k > 0 = leading minor k
+ *   not positive).
  *   phase_ms (HOST pointer, may be NULL): [5] float = milliseconds of the
- *   centring, Gram, factorisation, solve + refinement and dp / statistics
- *   phases, timed with HIP events on `stream`; when given, the call waits
- *   for the stream before returning.
+ *   centring, Gram, factorisation, solve + PCG and dp / statistics phases,
+ *   timed with HIP events on `stream`; when given, the call waits for the
+ *   stream before returning.
  * Replaces: minSR_solver_gpu (GPU/optimizer.py:471-614; MinSRGPU.solve
  *   :873-895), whose LU solve becomes a Cholesky solve of the same SPD system.
  */
-int64_t vmcx_sr_workspace_bytes(int64_t Ns, int64_t Np, int gram_dtype);
-int vmcx_sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
-                  const double *E, double energy_mean, double diag_shift,
-                  int gram_dtype, int refine, void *workspace, int64_t ws_bytes,
-                  double *dp, double *stats, int32_t *info, float *phase_ms,
-                  void *stream);
+int64_t vmcx_sr_workspace_bytes(int64_t Ns, int64_t Np, int o_dtype, int gram_dtype,
+                                int flags);
+int vmcx_sr_minsr(int64_t Ns, int64_t Np, void *O, int o_dtype, int64_t ldo, int flags,
+                  const double *E, double energy_mean, double diag_shift, int gram_dtype,
+                  int max_iter, double tol, void *workspace, int64_t ws_bytes, double *dp,
+                  double *stats, int32_t *info, float *phase_ms, void *stream);
 
 /*
  * Neural-network backflow (NNBF) model: two hidden layers of width `hidden`.

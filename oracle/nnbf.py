@@ -21,19 +21,30 @@ def unpack(params, n_layers):
     return M, layers, Wo, bo
 
 
-def forward(params, x, n_layers, bf_scale, act="relu"):
-    """Returns (sign, ln|psi|, cache) for configs x (B, N)."""
+def bf16_round(a):
+    """Round to bfloat16 (round to nearest, ties to even), returned as
+    float64: the operand rounding of the BF16 GEMM mode."""
+    b = np.ascontiguousarray(np.asarray(a, dtype=np.float32)).view(np.uint32)
+    b = (b + (((b >> 16) & 1) + 0x7FFF)) & np.uint32(0xFFFF0000)
+    return b.view(np.float32).astype(np.float64)
+
+
+def forward(params, x, n_layers, bf_scale, act="relu", bf16_gemm=False):
+    """Returns (sign, ln|psi|, cache) for configs x (B, N).  bf16_gemm: the
+    operands of the dense layers after the first (activations and weights)
+    are rounded to bfloat16, as in the BF16 GEMM mode."""
     M, layers, Wo, bo = unpack(params, n_layers)
     x = np.atleast_2d(x)
     n = quimb_to_netket(x).astype(np.float64)              # (B, 2N)
+    rnd = bf16_round if bf16_gemm else (lambda v: v)
     hs, zs = [n], []
     h = n
-    for W, b in layers:
-        z = h @ W.T + b
+    for k, (W, b) in enumerate(layers):
+        z = (h @ W.T + b) if k == 0 else (rnd(h) @ rnd(W).T + b)
         zs.append(z)
         h = np.maximum(z, 0.0) if act == "relu" else np.tanh(z)
         hs.append(h)
-    D = h @ Wo.T + bo                                       # (B, 2N*Nf)
+    D = rnd(h) @ rnd(Wo).T + bo                             # (B, 2N*Nf)
     n_orb, nf = M.shape
     Mx = M[None] + bf_scale * D.reshape(-1, n_orb, nf)
     occ = np.array([np.nonzero(r)[0] for r in quimb_to_netket(x)])

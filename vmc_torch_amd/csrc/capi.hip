@@ -37,7 +37,7 @@ static inline int status_of(int rc) {
 
 extern "C" {
 
-int vmcx_abi_version(void) { return 1; }
+int vmcx_abi_version(void) { return 2; }
 
 const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
 
@@ -117,18 +117,18 @@ int vmcx_hubbard_conn_fill(int64_t B, int n_sites, int n_edges,
                                            batch_ids, (hipStream_t)stream));
 }
 
-int64_t vmcx_sr_workspace_bytes(int64_t Ns, int64_t Np, int gram_dtype) {
-  return vmcx::sr_workspace_bytes(Ns, Np, gram_dtype);
+int64_t vmcx_sr_workspace_bytes(int64_t Ns, int64_t Np, int o_dtype, int gram_dtype,
+                                int flags) {
+  return vmcx::sr_workspace_bytes(Ns, Np, o_dtype, gram_dtype, flags);
 }
 
-int vmcx_sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
-                  const double *E, double energy_mean, double diag_shift,
-                  int gram_dtype, int refine, void *workspace, int64_t ws_bytes,
-                  double *dp, double *stats, int32_t *info, float *phase_ms,
-                  void *stream) {
-  const int rc = vmcx::sr_minsr(Ns, Np, O, ldo, E, energy_mean, diag_shift, gram_dtype,
-                                refine, workspace, ws_bytes, dp, stats, info, phase_ms,
-                                (hipStream_t)stream);
+int vmcx_sr_minsr(int64_t Ns, int64_t Np, void *O, int o_dtype, int64_t ldo, int flags,
+                  const double *E, double energy_mean, double diag_shift, int gram_dtype,
+                  int max_iter, double tol, void *workspace, int64_t ws_bytes, double *dp,
+                  double *stats, int32_t *info, float *phase_ms, void *stream) {
+  const int rc = vmcx::sr_minsr(Ns, Np, O, o_dtype, ldo, flags, E, energy_mean, diag_shift,
+                                gram_dtype, max_iter, tol, workspace, ws_bytes, dp, stats, info,
+                                phase_ms, (hipStream_t)stream);
   return rc == 3 ? VMCX_ELIB : status_of(rc);
 }
 

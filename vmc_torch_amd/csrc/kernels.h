@@ -46,11 +46,11 @@ int hubbard_conn_fill(int64_t B, int n_sites, int n_edges, const int32_t *edges,
                       const int64_t *offsets, int64_t *etas, double *coeffs,
                       int64_t *batch_ids, hipStream_t st);
 
-int64_t sr_workspace_bytes(int64_t Ns, int64_t Np, int gram);
-int sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
-             const double *E, double energy_mean, double shift, int gram,
-             int refine, void *ws, int64_t ws_bytes, double *dp, double *stats,
-             int32_t *info, float *phase_ms, hipStream_t st);
+int64_t sr_workspace_bytes(int64_t Ns, int64_t Np, int o_dtype, int gram, int flags);
+int sr_minsr(int64_t Ns, int64_t Np, void *O, int o_dtype, int64_t ldo, int flags,
+             const double *E, double energy_mean, double shift, int gram, int max_iter,
+             double tol, void *ws, int64_t ws_bytes, double *dp, double *stats, int32_t *info,
+             float *phase_ms, hipStream_t st);
 int sgd_update(int64_t n, double *params, const double *dp, double lr,
                hipStream_t st);
 rocblas_handle blas_handle_for(hipStream_t st);

@@ -36,41 +36,62 @@ struct Neu {
   __device__ __forceinline__ double value() const { return s + c; }
 };
 
-// partial[r][p] = sum over rows in chunk r of O[s][p]   (grid: (ceil(Np/256), R))
+// partial[r][p] = sum over rows in chunk r of w[s] O[s][p] (w = NULL: 1)
+template <typename T>
 __global__ void __launch_bounds__(256)
-    colsum_partial_kernel(int64_t Ns, int64_t Np, const double *__restrict__ O,
-                          int64_t ldo, int64_t rows_per_chunk, double *partial) {
+    colsum_partial_kernel(int64_t Ns, int64_t Np, const T *__restrict__ O, int64_t ldo,
+                          int64_t rows_per_chunk, const double *__restrict__ w,
+                          double *partial) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= Np) return;
   const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
   const int64_t r1 = (r0 + rows_per_chunk < Ns) ? r0 + rows_per_chunk : Ns;
   Neu acc;
-  for (int64_t s = r0; s < r1; ++s) acc.add(O[s * ldo + p]);
+  for (int64_t s = r0; s < r1; ++s) {
+    const double v = (double)O[s * ldo + p];
+    acc.add(w ? v * w[s] : v);
+  }
   partial[(int64_t)blockIdx.y * Np + p] = acc.value();
 }
 
 __global__ void __launch_bounds__(256)
     colsum_final_kernel(int64_t Np, int R, const double *__restrict__ partial,
-                        double inv_n, double *mean) {
+                        double scale, double *out) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= Np) return;
   Neu acc;
   for (int r = 0; r < R; ++r) acc.add(partial[(int64_t)r * Np + p]);
-  mean[p] = acc.value() * inv_n;
+  out[p] = acc.value() * scale;
 }
 
-// X[s][p] = (O[s][p] - mean[p]) * scale ; optional low-precision copy
-template <typename TL>
+// v[s] = sum_p X[s][p] u[p]   (one block per row, FP64 accumulation)
+template <typename T>
 __global__ void __launch_bounds__(256)
-    center_kernel(int64_t Ns, int64_t Np, const double *__restrict__ O,
-                  int64_t ldo, const double *__restrict__ mean, double scale,
-                  double *X, TL *Xlo) {
+    gemv_rows_kernel(int64_t Ns, int64_t Np, const T *__restrict__ X, int64_t ldx,
+                     const double *__restrict__ u, double *v) {
+  __shared__ double sh[4];
+  const int64_t s = blockIdx.x;
+  const T *row = X + s * ldx;
+  double acc = 0.0;
+  for (int64_t p = threadIdx.x; p < Np; p += blockDim.x) acc += (double)row[p] * u[p];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) v[s] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// X[s][p] = (O[s][p] - mean[p]) * scale (X may alias O); optional low copy
+template <typename T, typename TL>
+__global__ void __launch_bounds__(256)
+    center_kernel(int64_t Ns, int64_t Np, const T *O, int64_t ldo,
+                  const double *__restrict__ mean, double scale, T *X, int64_t ldx,
+                  TL *Xlo) {
   const int64_t n = Ns * Np;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t s = idx / Np, p = idx - s * Np;
-    const double v = (O[s * ldo + p] - mean[p]) * scale;
-    X[idx] = v;
+    const double v = ((double)O[s * ldo + p] - mean[p]) * scale;
+    X[s * ldx + p] = (T)v;
     if (Xlo) Xlo[idx] = (TL)v;
   }
 }
@@ -115,6 +136,15 @@ __global__ void __launch_bounds__(1024)
   __syncthreads();
   const double em = mean_sh;
   for (int64_t s = threadIdx.x; s < Ns; s += blockDim.x) eps[s] = (E[s] - em) * scale;
+}
+
+// stats <- [mean E, var E, ||res|| / ||eps||, ||g||, PCG iterations]
+__global__ void finish_stats(const double *sc, double *stats) {
+  stats[0] = sc[8];
+  stats[1] = sc[9];
+  stats[2] = sc[11] > 0.0 ? sc[10] / sc[11] : sc[10];
+  stats[3] = sc[12];
+  stats[4] = sc[6];
 }
 
 // T (column-major Ns x Ns, lower used) <- Tin (+ shift on the diagonal)
@@ -202,11 +232,11 @@ rocblas_handle blas_handle_for(hipStream_t st) {
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct SrLayout {
-  size_t X, Xlo, T, Tlo, partial, mean, eps, alpha, r, u, v, stats, info, dinv, t1, t2,
-      corr, total;
+  size_t X, Xlo, T, Tlo, partial, mean, eps, alpha, r, z, pv, q, u, stw, info, dinv, t1, t2,
+      total;
 };
 
-static SrLayout sr_layout(int64_t Ns, int64_t Np, int gram) {
+static SrLayout sr_layout(int64_t Ns, int64_t Np, int o_dtype, int gram, int inplace) {
   SrLayout L{};
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -215,8 +245,10 @@ static SrLayout sr_layout(int64_t Ns, int64_t Np, int gram) {
     return o;
   };
   const int R = 64;
-  L.X = take((size_t)Ns * Np * 8);
-  L.Xlo = take(gram == 1 ? (size_t)Ns * Np * 4 : (gram == 2 ? (size_t)Ns * Np * 2 : 0));
+  const size_t osz = o_dtype == 0 ? 8 : 4;
+  L.X = take(inplace ? 0 : (size_t)Ns * Np * osz);
+  const bool need_lo = (gram == 2) || (gram == 1 && o_dtype == 0);
+  L.Xlo = take(need_lo ? (size_t)Ns * Np * (gram == 2 ? 2 : 4) : 0);
   L.T = take((size_t)Ns * Ns * 8);
   L.Tlo = take(gram ? (size_t)Ns * Ns * 4 : 0);
   L.partial = take((size_t)R * Np * 8);
@@ -224,21 +256,116 @@ static SrLayout sr_layout(int64_t Ns, int64_t Np, int gram) {
   L.eps = take((size_t)Ns * 8);
   L.alpha = take((size_t)Ns * 8);
   L.r = take((size_t)Ns * 8);
+  L.z = take((size_t)Ns * 8);
+  L.pv = take((size_t)Ns * 8);
+  L.q = take((size_t)Ns * 8);
   L.u = take((size_t)Np * 8);
-  L.v = take((size_t)Ns * 8);
-  L.stats = take(8 * 8);
+  L.stw = take(16 * 8);
   L.info = take(64);
   L.dinv = take((size_t)chol_dinv_bytes(Ns));
   L.t1 = take((size_t)Ns * 8);
   L.t2 = take((size_t)Ns * 8);
-  L.corr = take((size_t)Ns * 8);
   L.total = off;
   return L;
 }
 
-int64_t sr_workspace_bytes(int64_t Ns, int64_t Np, int gram) {
-  if (Ns <= 0 || Np <= 0 || gram < 0 || gram > 2) return -1;
-  return (int64_t)sr_layout(Ns, Np, gram).total;
+int64_t sr_workspace_bytes(int64_t Ns, int64_t Np, int o_dtype, int gram, int flags) {
+  if (Ns <= 0 || Np <= 0 || gram < 0 || gram > 2 || o_dtype < 0 || o_dtype > 1) return -1;
+  return (int64_t)sr_layout(Ns, Np, o_dtype, gram, flags & 1).total;
+}
+
+// ---- small device-side scalar kernels for PCG (no host round trips)
+// dot(a, b) -> out[0] (one block, Neumaier)
+__global__ void __launch_bounds__(1024)
+    dot_kernel(int64_t n, const double *a, const double *b, double *out) {
+  __shared__ double sh[16];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += a[i] * b[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Neu acc;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) acc.add(sh[k]);
+    out[0] = acc.value();
+  }
+}
+
+// scalars: [0] rz, [1] pq, [2] rz_new, [3] |r|^2, [4] |eps|^2, [5] done, [6] iters
+// x += a p ; r -= a q  with a = rz / pq   (skipped once done)
+__global__ void pcg_update_xr(int64_t n, double *x, double *r, const double *p,
+                              const double *q, const double *sc) {
+  if (sc[5] != 0.0) return;
+  const double a = sc[0] / sc[1];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    x[i] += a * p[i];
+    r[i] -= a * q[i];
+  }
+}
+
+// check convergence after the x/r update: done if |r| <= tol |eps|
+__global__ void pcg_check(double tol, double *sc) {
+  if (sc[5] != 0.0) return;
+  sc[6] += 1.0;
+  if (sc[3] <= tol * tol * sc[4]) sc[5] = 1.0;
+}
+
+// p = z + (rz_new / rz) p ; rz = rz_new
+__global__ void pcg_update_p(int64_t n, double *p, const double *z, double *sc) {
+  if (sc[5] != 0.0) return;
+  const double b = sc[2] / sc[0];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + b * p[i];
+}
+
+__global__ void pcg_rotate(double *sc) {
+  if (sc[5] != 0.0) return;
+  sc[0] = sc[2];
+}
+
+// r = eps - v - shift * x
+__global__ void residual2_kernel(int64_t n, const double *eps, const double *v,
+                                 const double *x, double shift, double *r) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = eps[i] - v[i] - shift * x[i];
+}
+
+// q = v + shift p
+__global__ void addshift_kernel(int64_t n, const double *v, const double *p, double shift,
+                                double *q) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) q[i] = v[i] + shift * p[i];
+}
+
+__global__ void set_scalars(double *sc) {
+  for (int i = 0; i < 16; ++i) sc[i] = 0.0;
+}
+
+__global__ void copy_scalar(const double *src, double *dst) { dst[0] = src[0]; }
+
+template <typename T>
+static int xt_gemv_rows(int64_t Ns, int64_t Np, const T *X, int64_t ldx, const double *u,
+                        double *v, hipStream_t st) {
+  hipLaunchKernelGGL(gemv_rows_kernel<T>, dim3((unsigned)Ns), dim3(256), 0, st, Ns, Np, X, ldx,
+                     u, v);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+// out[p] = scale * sum_s X[s][p] w[s]
+template <typename T>
+static int xt_gemv_cols(int64_t Ns, int64_t Np, const T *X, int64_t ldx, const double *w,
+                        double scale, double *partial, double *out, hipStream_t st) {
+  const int R = 64;
+  const int64_t rows = (Ns + R - 1) / R;
+  hipLaunchKernelGGL(colsum_partial_kernel<T>, dim3((unsigned)((Np + 255) / 256), R),
+                     dim3(256), 0, st, Ns, Np, X, ldx, rows, w, partial);
+  VMCX_LAUNCHED();
+  hipLaunchKernelGGL(colsum_final_kernel, dim3((unsigned)((Np + 255) / 256)), dim3(256), 0, st,
+                     Np, R, partial, scale, out);
+  VMCX_LAUNCHED();
+  return 0;
 }
 
 // Optional phase timing: HIP events on the launch stream; read back with one
@@ -266,147 +393,159 @@ struct PhaseTimer {
   }
 };
 
-int sr_minsr(int64_t Ns, int64_t Np, const double *O, int64_t ldo,
-             const double *E, double energy_mean, double shift, int gram,
-             int refine, void *ws, int64_t ws_bytes, double *dp, double *stats,
-             int32_t *info, float *phase_ms, hipStream_t st) {
-  VMCX_CHECK_ARG(Ns >= 1 && Np >= 1, "Ns and Np must be positive");
-  VMCX_CHECK_ARG(Ns <= 0x7fffffff && Np <= 0x7fffffff, "Ns / Np exceed 32-bit BLAS");
-  VMCX_CHECK_ARG(ldo >= Np, "ldo < Np");
-  VMCX_CHECK_ARG(gram >= 0 && gram <= 2, "gram must be 0 (f64), 1 (f32) or 2 (bf16)");
-  VMCX_CHECK_ARG(refine >= 0 && refine <= 32, "refine must be in [0, 32]");
-  VMCX_CHECK_ARG(O && E && ws && dp, "null pointer");
-  const SrLayout L = sr_layout(Ns, Np, gram);
-  VMCX_CHECK_ARG(ws_bytes >= (int64_t)L.total, "workspace too small");
-  char *w = (char *)ws;
-  double *X = (double *)(w + L.X);
-  double *T = (double *)(w + L.T);
+template <typename T>
+static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, const double *E,
+                      double energy_mean, double shift, int gram, int max_iter, double tol,
+                      char *w, const SrLayout &L, double *dp, double *stats, int32_t *info,
+                      float *phase_ms, hipStream_t st) {
+  const bool inplace = flags & 1;
+  T *X = inplace ? O : (T *)(w + L.X);
+  const int64_t ldx = inplace ? ldo : Np;
+  double *T64 = (double *)(w + L.T);
   double *partial = (double *)(w + L.partial), *mean = (double *)(w + L.mean);
   double *eps = (double *)(w + L.eps), *alpha = (double *)(w + L.alpha);
-  double *r = (double *)(w + L.r), *u = (double *)(w + L.u), *v = (double *)(w + L.v);
-  double *stw = (double *)(w + L.stats);
-  rocblas_int *dinfo = (rocblas_int *)(w + L.info);
+  double *r = (double *)(w + L.r), *z = (double *)(w + L.z), *pv = (double *)(w + L.pv);
+  double *q = (double *)(w + L.q), *u = (double *)(w + L.u), *sc = (double *)(w + L.stw);
+  double *dinv = (double *)(w + L.dinv), *t1 = (double *)(w + L.t1), *t2 = (double *)(w + L.t2);
+  int32_t *iinfo = (int32_t *)(w + L.info);
   const double scale = 1.0 / std::sqrt((double)Ns);
   const int nsi = (int)Ns, npi = (int)Np;
-
   rocblas_handle h = blas_handle_for(st);
   VMCX_CHECK_ARG(h != nullptr, "rocblas_create_handle failed");
   PhaseTimer timer(phase_ms != nullptr, st);
   timer.mark();
+  int rc;
 
-  // column means (two-level compensated sum) and centring
-  const int R = 64;
-  const int64_t rows = (Ns + R - 1) / R;
-  hipLaunchKernelGGL(colsum_partial_kernel, dim3((unsigned)((Np + 255) / 256), R),
-                     dim3(256), 0, st, Ns, Np, O, ldo, rows, partial);
-  VMCX_LAUNCHED();
-  hipLaunchKernelGGL(colsum_final_kernel, dim3((unsigned)((Np + 255) / 256)), dim3(256),
-                     0, st, Np, R, partial, 1.0 / (double)Ns, mean);
-  VMCX_LAUNCHED();
-  const unsigned cgrid = 4096;
-  if (gram == 1)
-    hipLaunchKernelGGL(center_kernel<float>, dim3(cgrid), dim3(256), 0, st, Ns, Np, O,
-                       ldo, mean, scale, X, (float *)(w + L.Xlo));
-  else if (gram == 2)
-    hipLaunchKernelGGL(center_kernel<__hip_bfloat16>, dim3(cgrid), dim3(256), 0, st, Ns,
-                       Np, O, ldo, mean, scale, X, (__hip_bfloat16 *)(w + L.Xlo));
+  // column means (two-level compensated sums) and centring
+  rc = xt_gemv_cols<T>(Ns, Np, O, ldo, nullptr, 1.0 / (double)Ns, partial, mean, st);
+  if (rc) return rc;
+  const unsigned cgrid = 8192;
+  if (gram == 2)
+    hipLaunchKernelGGL((center_kernel<T, __hip_bfloat16>), dim3(cgrid), dim3(256), 0, st, Ns,
+                       Np, O, ldo, mean, scale, X, ldx, (__hip_bfloat16 *)(w + L.Xlo));
+  else if (gram == 1 && sizeof(T) == 8)
+    hipLaunchKernelGGL((center_kernel<T, float>), dim3(cgrid), dim3(256), 0, st, Ns, Np, O, ldo,
+                       mean, scale, X, ldx, (float *)(w + L.Xlo));
   else
-    hipLaunchKernelGGL(center_kernel<float>, dim3(cgrid), dim3(256), 0, st, Ns, Np, O,
-                       ldo, mean, scale, X, (float *)nullptr);
+    hipLaunchKernelGGL((center_kernel<T, float>), dim3(cgrid), dim3(256), 0, st, Ns, Np, O, ldo,
+                       mean, scale, X, ldx, (float *)nullptr);
   VMCX_LAUNCHED();
   const int use_given = std::isfinite(energy_mean) ? 1 : 0;
-  hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(1024), 0, st, Ns, E, energy_mean,
-                     use_given, scale, eps, stw);
+  hipLaunchKernelGGL(set_scalars, dim3(1), dim3(1), 0, st, sc);
   VMCX_LAUNCHED();
-
+  hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(1024), 0, st, Ns, E, energy_mean, use_given,
+                     scale, eps, sc + 8);
+  VMCX_LAUNCHED();
   timer.mark();
+
   // Gram
   const double one = 1.0, zero = 0.0;
   if (gram == 0) {
-    VMCX_BLAS(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, nsi, npi,
-                            &one, X, npi, &zero, T, nsi));
-    hipLaunchKernelGGL(diag_shift_kernel, dim3((unsigned)((Ns + 255) / 256)), dim3(256), 0,
-                       st, Ns, T, shift);
+    VMCX_BLAS(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, nsi, npi, &one,
+                            (const double *)X, (int)ldx, &zero, T64, nsi));
+    hipLaunchKernelGGL(diag_shift_kernel, dim3((unsigned)((Ns + 255) / 256)), dim3(256), 0, st,
+                       Ns, T64, shift);
     VMCX_LAUNCHED();
   } else {
     const float onef = 1.0f, zerof = 0.0f;
+    const bool direct32 = (gram == 1 && sizeof(T) == 4);
     const rocblas_datatype ab = (gram == 1) ? rocblas_datatype_f32_r : rocblas_datatype_bf16_r;
-    void *Xlo = (void *)(w + L.Xlo);
+    const void *Xg = direct32 ? (const void *)X : (const void *)(w + L.Xlo);
+    const int ldg = direct32 ? (int)ldx : npi;
     float *Tlo = (float *)(w + L.Tlo);
-    VMCX_BLAS(rocblas_gemm_ex(h, rocblas_operation_transpose, rocblas_operation_none, nsi,
-                              nsi, npi, &onef, Xlo, ab, npi, Xlo, ab, npi, &zerof, Tlo,
-                              rocblas_datatype_f32_r, nsi, Tlo, rocblas_datatype_f32_r,
-                              nsi, rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0,
-                              0));
-    hipLaunchKernelGGL(shift_kernel<float>, dim3(4096), dim3(256), 0, st, Ns, Tlo, T,
-                       shift);
+    VMCX_BLAS(rocblas_gemm_ex(h, rocblas_operation_transpose, rocblas_operation_none, nsi, nsi,
+                              npi, &onef, Xg, ab, ldg, Xg, ab, ldg, &zerof, Tlo,
+                              rocblas_datatype_f32_r, nsi, Tlo, rocblas_datatype_f32_r, nsi,
+                              rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0));
+    hipLaunchKernelGGL(shift_kernel<float>, dim3(4096), dim3(256), 0, st, Ns, Tlo, T64, shift);
     VMCX_LAUNCHED();
   }
-  // FP64 Cholesky and solve (blocked factorisation of chol.hip; the
-  // rocSOLVER path is kept for cross-checks: VMCX_SR_ROCSOLVER=1)
-  static const bool use_rocsolver = getenv("VMCX_SR_ROCSOLVER") != nullptr;
-  double *t1 = (double *)(w + L.t1), *t2 = (double *)(w + L.t2);
-  double *corr = (double *)(w + L.corr), *dinv = (double *)(w + L.dinv);
-  int32_t *iinfo = (int32_t *)dinfo;
   timer.mark();
-  if (use_rocsolver) {
-    VMCX_BLAS(rocsolver_dpotrf(h, rocblas_fill_lower, nsi, T, nsi, dinfo));
-    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((Ns + 255) / 256)), dim3(256), 0, st,
-                       Ns, eps, alpha);
-    VMCX_LAUNCHED();
-    VMCX_BLAS(rocsolver_dpotrs(h, rocblas_fill_lower, nsi, 1, T, nsi, alpha, nsi));
-  } else {
-    int rc = chol_factor((void *)h, Ns, T, Ns, dinv, iinfo, st);
+  // FP64 Cholesky of the (shifted) Gram
+  rc = chol_factor((void *)h, Ns, T64, Ns, dinv, iinfo, st);
+  if (rc) return rc;
+  timer.mark();
+  rc = chol_solve(Ns, T64, Ns, dinv, eps, alpha, t1, t2, st);
+  if (rc) return rc;
+  // preconditioned CG on (Ot Ot^T + shift) alpha = eps, preconditioner = the factor
+  const unsigned gv = (unsigned)((Ns + 255) / 256);
+  if (max_iter > 0) {
+    rc = xt_gemv_cols<T>(Ns, Np, X, ldx, alpha, 1.0, partial, u, st);
     if (rc) return rc;
-    timer.mark();
-    rc = chol_solve(Ns, T, Ns, dinv, eps, alpha, t1, t2, st);
+    rc = xt_gemv_rows<T>(Ns, Np, X, ldx, u, q, st);
     if (rc) return rc;
-  }
-  if (use_rocsolver) timer.mark();
-  // iterative refinement against the FP64 operator  Ot Ot^T + shift I
-  for (int it = 0; it < refine; ++it) {
-    VMCX_BLAS(rocblas_dgemv(h, rocblas_operation_none, npi, nsi, &one, X, npi, alpha, 1,
-                            &zero, u, 1));
-    VMCX_BLAS(rocblas_dgemv(h, rocblas_operation_transpose, npi, nsi, &one, X, npi, u, 1,
-                            &zero, v, 1));
-    hipLaunchKernelGGL(residual_kernel, dim3((unsigned)((Ns + 255) / 256)), dim3(256), 0,
-                       st, Ns, eps, v, alpha, shift, r);
+    hipLaunchKernelGGL(residual2_kernel, dim3(gv), dim3(256), 0, st, Ns, eps, q, alpha, shift, r);
     VMCX_LAUNCHED();
-    if (use_rocsolver) {
-      VMCX_BLAS(rocsolver_dpotrs(h, rocblas_fill_lower, nsi, 1, T, nsi, r, nsi));
-      hipLaunchKernelGGL(axpy_kernel, dim3((unsigned)((Ns + 255) / 256)), dim3(256), 0, st,
-                         Ns, 1.0, r, alpha);
-    } else {
-      const int rc = chol_solve(Ns, T, Ns, dinv, r, corr, t1, t2, st);
+    rc = chol_solve(Ns, T64, Ns, dinv, r, z, t1, t2, st);
+    if (rc) return rc;
+    VMCX_HIP(hipMemcpyAsync(pv, z, Ns * 8, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, r, z, sc + 0);
+    hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, eps, eps, sc + 4);
+    VMCX_LAUNCHED();
+    for (int it = 0; it < max_iter; ++it) {
+      rc = xt_gemv_cols<T>(Ns, Np, X, ldx, pv, 1.0, partial, u, st);
       if (rc) return rc;
-      hipLaunchKernelGGL(axpy_kernel, dim3((unsigned)((Ns + 255) / 256)), dim3(256), 0, st,
-                         Ns, 1.0, corr, alpha);
+      rc = xt_gemv_rows<T>(Ns, Np, X, ldx, u, t1, st);
+      if (rc) return rc;
+      hipLaunchKernelGGL(addshift_kernel, dim3(gv), dim3(256), 0, st, Ns, t1, pv, shift, q);
+      hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, pv, q, sc + 1);
+      hipLaunchKernelGGL(pcg_update_xr, dim3(gv), dim3(256), 0, st, Ns, alpha, r, pv, q, sc);
+      hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, r, r, sc + 3);
+      hipLaunchKernelGGL(pcg_check, dim3(1), dim3(1), 0, st, tol, sc);
+      VMCX_LAUNCHED();
+      if (it + 1 == max_iter) break;
+      rc = chol_solve(Ns, T64, Ns, dinv, r, z, t1, t2, st);
+      if (rc) return rc;
+      hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, r, z, sc + 2);
+      hipLaunchKernelGGL(pcg_update_p, dim3(gv), dim3(256), 0, st, Ns, pv, z, sc);
+      hipLaunchKernelGGL(pcg_rotate, dim3(1), dim3(1), 0, st, sc);
+      VMCX_LAUNCHED();
     }
-    VMCX_LAUNCHED();
   }
   timer.mark();
-  // dp = Ot^T alpha
-  VMCX_BLAS(rocblas_dgemv(h, rocblas_operation_none, npi, nsi, &one, X, npi, alpha, 1, &zero,
-                          dp, 1));
+  // dp = Ot^T alpha ; stats
+  rc = xt_gemv_cols<T>(Ns, Np, X, ldx, alpha, 1.0, partial, dp, st);
+  if (rc) return rc;
   if (stats) {
-    // final residual norm of (Ot Ot^T + shift) alpha - eps and the gradient norm
-    VMCX_BLAS(rocblas_dgemv(h, rocblas_operation_transpose, npi, nsi, &one, X, npi, dp, 1,
-                            &zero, v, 1));
-    hipLaunchKernelGGL(residual_kernel, dim3((unsigned)((Ns + 255) / 256)), dim3(256), 0,
-                       st, Ns, eps, v, alpha, shift, r);
+    rc = xt_gemv_rows<T>(Ns, Np, X, ldx, dp, q, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(residual2_kernel, dim3(gv), dim3(256), 0, st, Ns, eps, q, alpha, shift, r);
+    hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, st, Ns, r, sc + 10);
+    hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, st, Ns, eps, sc + 11);
     VMCX_LAUNCHED();
-    hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, st, Ns, r, stw + 2);
+    rc = xt_gemv_cols<T>(Ns, Np, X, ldx, eps, 1.0, partial, u, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, st, Np, u, sc + 12);
     VMCX_LAUNCHED();
-    VMCX_BLAS(rocblas_dgemv(h, rocblas_operation_none, npi, nsi, &one, X, npi, eps, 1, &zero,
-                            u, 1));
-    hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, st, Np, u, stw + 3);
+    // stats: mean, var, ||res||/||eps||, ||g||, iterations
+    hipLaunchKernelGGL(finish_stats, dim3(1), dim3(1), 0, st, sc, stats);
     VMCX_LAUNCHED();
-    VMCX_HIP(hipMemcpyAsync(stats, stw, 4 * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
-  if (info) VMCX_HIP(hipMemcpyAsync(info, dinfo, sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+  if (info) VMCX_HIP(hipMemcpyAsync(info, iinfo, sizeof(int32_t), hipMemcpyDeviceToDevice, st));
   timer.mark();
   timer.read(phase_ms);
   return 0;
+}
+
+int sr_minsr(int64_t Ns, int64_t Np, void *O, int o_dtype, int64_t ldo, int flags,
+             const double *E, double energy_mean, double shift, int gram, int max_iter,
+             double tol, void *ws, int64_t ws_bytes, double *dp, double *stats, int32_t *info,
+             float *phase_ms, hipStream_t st) {
+  VMCX_CHECK_ARG(Ns >= 1 && Np >= 1, "Ns and Np must be positive");
+  VMCX_CHECK_ARG(Ns <= 0x7fffffff && Np <= 0x7fffffff, "Ns / Np exceed 32-bit BLAS");
+  VMCX_CHECK_ARG(ldo >= Np, "ldo < Np");
+  VMCX_CHECK_ARG(o_dtype == 0 || o_dtype == 1, "o_dtype must be 0 (f64) or 1 (f32)");
+  VMCX_CHECK_ARG(gram >= 0 && gram <= 2, "gram must be 0 (f64), 1 (f32) or 2 (bf16)");
+  VMCX_CHECK_ARG(!(gram == 0 && o_dtype == 1), "an FP64 Gram needs FP64 O");
+  VMCX_CHECK_ARG(max_iter >= 0 && max_iter <= 1000, "max_iter must be in [0, 1000]");
+  VMCX_CHECK_ARG(O && E && ws && dp, "null pointer");
+  const SrLayout L = sr_layout(Ns, Np, o_dtype, gram, flags & 1);
+  VMCX_CHECK_ARG(ws_bytes >= (int64_t)L.total, "workspace too small");
+  if (o_dtype == 0)
+    return sr_minsr_t<double>(Ns, Np, (double *)O, ldo, flags, E, energy_mean, shift, gram,
+                              max_iter, tol, (char *)ws, L, dp, stats, info, phase_ms, st);
+  return sr_minsr_t<float>(Ns, Np, (float *)O, ldo, flags, E, energy_mean, shift, gram,
+                           max_iter, tol, (char *)ws, L, dp, stats, info, phase_ms, st);
 }
 
 // p <- p - lr * dp

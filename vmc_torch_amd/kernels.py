@@ -171,36 +171,43 @@ class SRWorkspace:
     def __init__(self):
         self.buf = None
 
-    def get(self, Ns, Np, gram, device):
-        need = int(lib().vmcx_sr_workspace_bytes(int(Ns), int(Np), GRAM_CODES[gram]))
+    def get(self, Ns, Np, o_dtype, gram, flags, device):
+        need = int(lib().vmcx_sr_workspace_bytes(int(Ns), int(Np), int(o_dtype),
+                                                  GRAM_CODES[gram], int(flags)))
         if need < 0:
             raise ValueError("bad SR sizes")
         if self.buf is None or self.buf.numel() < need or self.buf.device != device:
+            self.buf = None
             self.buf = torch.empty(need, dtype=torch.uint8, device=device)
         return self.buf
 
 
-def sr_minsr(O, E, energy_mean, diag_shift, gram="fp64", refine=0, workspace=None,
-             dp=None, with_stats=True, phase_ms=None):
-    """Sample-space SR on the device.  Returns (dp [Np] float64, stats [4]
-    float64 device tensor or None, info [1] int32 device tensor).
-    phase_ms: optional list, filled with the 5 phase durations (ms; the call
-    then synchronises the stream)."""
-    require(O, F64, "O")
+def sr_minsr(O, E, energy_mean, diag_shift, gram="fp64", max_iter=0, tol=1e-10,
+             workspace=None, dp=None, with_stats=True, phase_ms=None, inplace=False):
+    """Sample-space SR on the device.  O: (Ns, Np) float64 or float32,
+    row-major (overwritten by the centred matrix when inplace=True).
+    Returns (dp [Np] float64, stats [5] float64 device tensor or None, info
+    [1] int32 device tensor).  phase_ms: optional list, filled with the 5
+    phase durations (ms; the call then synchronises the stream)."""
+    import ctypes
+    if O.dtype not in (torch.float64, torch.float32):
+        raise TypeError("O must be float64 or float32")
+    require(O, O.dtype, "O")
     if O.dim() != 2 or O.stride(1) != 1:
-        raise ValueError("O must be a row-major (Ns, Np) float64 matrix")
+        raise ValueError("O must be a row-major (Ns, Np) matrix")
     Ns, Np = O.shape
+    od = 0 if O.dtype == torch.float64 else 1
     require(E, F64, "local_energies", (Ns,), O.device)
-    ws = (workspace or SRWorkspace()).get(Ns, Np, gram, O.device)
+    flags = 1 if inplace else 0
+    ws = (workspace or SRWorkspace()).get(Ns, Np, od, gram, flags, O.device)
     dp = torch.empty(Np, dtype=F64, device=O.device) if dp is None else require(dp, F64, "dp", (Np,))
-    stats = torch.empty(4, dtype=F64, device=O.device) if with_stats else None
+    stats = torch.empty(5, dtype=F64, device=O.device) if with_stats else None
     info = torch.empty(1, dtype=I32, device=O.device)
     em = float("nan") if energy_mean is None else float(energy_mean)
-    import ctypes
     ph = (ctypes.c_float * 5)() if phase_ms is not None else None
-    check(lib().vmcx_sr_minsr(Ns, Np, ptr(O), O.stride(0), ptr(E), em, float(diag_shift),
-                              GRAM_CODES[gram], int(refine), ptr(ws), ws.numel(), ptr(dp),
-                              ptr(stats), ptr(info),
+    check(lib().vmcx_sr_minsr(Ns, Np, ptr(O), od, O.stride(0), flags, ptr(E), em,
+                              float(diag_shift), GRAM_CODES[gram], int(max_iter), float(tol),
+                              ptr(ws), ws.numel(), ptr(dp), ptr(stats), ptr(info),
                               ctypes.cast(ph, ctypes.c_void_p) if ph is not None else None,
                               stream_ptr(O.device)))
     if ph is not None:

@@ -80,17 +80,21 @@ class MinSRGPU(PreconditionerGPU):
     """Sample-space SR (GPU/optimizer.py:471-614, MinSRGPU at :873-895).
 
     gram: "fp64" (plain form), "fp32" or "bf16" (Gram operands in reduced
-    precision with FP32 accumulation; FP64 Cholesky, then `refine` rounds of
-    iterative refinement against the FP64 operator).
+    precision with FP32 accumulation; the FP64 Cholesky factor of that Gram
+    then preconditions conjugate gradients on the exact operator, at most
+    `max_iter` iterations to relative residual `tol`).
+    inplace: centre O in place (O is overwritten; saves an Ns x Np copy).
     Returns (dp device tensor, seconds, info) with info = 0 on success, or
     the Cholesky failure index (the reference returns 1 after an lstsq
     fallback).
     """
 
-    def __init__(self, gram="fp64", refine=None, sync_info=True):
+    def __init__(self, gram="fp64", max_iter=None, tol=1e-8, sync_info=True, inplace=False):
         self.gram = gram
-        self.refine = (0 if gram == "fp64" else 3) if refine is None else refine
+        self.max_iter = (0 if gram == "fp64" else 20) if max_iter is None else max_iter
+        self.tol = tol
         self.sync_info = sync_info
+        self.inplace = inplace
         self.workspace = kernels.SRWorkspace()
         self.last_stats = None
 
@@ -99,12 +103,16 @@ class MinSRGPU(PreconditionerGPU):
         t0 = time.time()
         if not run_sr:
             raise NotImplementedError("run_sr=False: use the energy gradient from MinSRGPU stats")
-        O = local_o.to(device=device, dtype=torch.float64).contiguous()
+        dt = local_o.dtype if local_o.dtype in (torch.float32, torch.float64) else torch.float64
+        if self.gram == "fp64":
+            dt = torch.float64
+        O = local_o.to(device=device, dtype=dt).contiguous()
         E = local_energies.to(device=device, dtype=torch.float64).contiguous()
         if O.shape[0] != total_samples or O.shape[1] != n_params:
             raise ValueError("local_o must be (total_samples, n_params) on one GPU")
         dp, stats, info = kernels.sr_minsr(O, E, energy_mean, diag_shift, self.gram,
-                                           self.refine, self.workspace)
+                                           self.max_iter, self.tol, self.workspace,
+                                           inplace=self.inplace)
         self.last_stats = stats
         info_v = int(info.item()) if self.sync_info else info
         return dp, time.time() - t0, info_v
