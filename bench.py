@@ -1,25 +1,31 @@
 #!/usr/bin/env python
 """Benchmark of the VMC inner loop on one MI355X.
 
-Workload (default): 8x8 open-boundary Fermi-Hubbard, t=1, U=8, 32 up + 32
-down; Slater determinant with a 128x64 FP64 orbital matrix drawn N(0, 0.1)
-from seed 0; 8192 walkers in one launch; 200 burn-in sweeps; Sherman-
-Morrison updates with an exact FP64 refresh every 64 accepted moves; SR in
-sample space with diagonal shift 1e-4, learning rate 0.05.
+Default workload (hubbard8x8_slater): 8x8 open-boundary Fermi-Hubbard, t=1,
+U=8, 32 up + 32 down; Slater determinant with a 128x64 FP64 orbital matrix
+drawn N(0, 0.1) from seed 0; 8192 walkers in one launch; 200 burn-in sweeps;
+Sherman-Morrison updates with an exact FP64 refresh every 64 accepted moves;
+SR in sample space with diagonal shift 1e-4, learning rate 0.05.
+
+hubbard6x6_nnbf: 6x6, 16 up + 16 down; neural-network backflow 72 -> 256 ->
+256 (ReLU) -> 72x32 correction (bf_scale 0.01) on an N(0, 0.1) base,
+Xavier weights from seed 0, 678,912 parameters; BF16 GEMMs with FP32
+accumulation and an FP64 determinant; 4096 walkers; O in FP32 (centred in
+place); SR with a BF16 Gram preconditioning conjugate gradients on the exact
+operator, shift 1e-3.
 
 One timed step = one SR iteration: one sweep of every walker (one sample per
-walker), the local energies, the per-sample log-derivatives O (Ns x Np =
-8192 x 8192), the minSR solve and the SGD update, all on the device.
+walker), the local energies, the per-sample log-derivatives O, the minSR solve
+and the SGD update, all on the device.
 
-Printed: one JSON line (rank 0) with effective samples/s as `value`, SR
-iterations/s, walker-sweeps/s, the roofline of the dominant kernel (HIP-event
-timing on the launch stream), per-kernel timings, the CPU baseline (the FP64
-oracle restating the reference algorithm, timed on this host) and the
-|dE/site| of the GPU local energies against the FP64 oracle.
+Printed: one JSON line with effective samples/s as `value`, SR iterations/s,
+walker-sweeps/s, the roofline of the dominant component (HIP-event timing on
+the launch stream), per-component timings, the CPU baseline (the FP64 oracle
+restating the reference algorithm, timed on this host) and the |dE/site| of
+the GPU local energies against the FP64 oracle.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -33,11 +39,19 @@ PEAK_FP32_MFMA = 157.3e12
 PEAK_BF16 = 2.5e15
 
 WORKLOADS = {
-    # name: Lx, Ly, n_up, n_dn, walkers, burn-in, description
-    "hubbard8x8_slater": (8, 8, 32, 32, 8192, 200,
-                          "8x8 OBC Hubbard t=1 U=8 32u+32d, Slater 128x64 N(0,0.1), 8192 walkers"),
-    "hubbard4x2_slater": (4, 2, 4, 4, 1024, 100,
-                          "4x2 OBC Hubbard t=1 U=8 4u+4d, Slater 16x8 N(0,0.1), 1024 walkers"),
+    "hubbard8x8_slater": dict(
+        kind="slater", Lx=8, Ly=8, nu=32, nd=32, B=8192, burn=200, shift=1e-4, lr=0.05,
+        gram="fp64",
+        desc="8x8 OBC Hubbard t=1 U=8 32u+32d, Slater 128x64 N(0,0.1), 8192 walkers"),
+    "hubbard4x2_slater": dict(
+        kind="slater", Lx=4, Ly=2, nu=4, nd=4, B=1024, burn=100, shift=1e-4, lr=0.05,
+        gram="fp64",
+        desc="4x2 OBC Hubbard t=1 U=8 4u+4d, Slater 16x8 N(0,0.1), 1024 walkers"),
+    "hubbard6x6_nnbf": dict(
+        kind="nnbf", Lx=6, Ly=6, nu=16, nd=16, B=4096, burn=200, shift=1e-3, lr=0.05,
+        gram="bf16", hidden=256, bf_scale=0.01, mlp="bf16",
+        desc=("6x6 OBC Hubbard t=1 U=8 16u+16d, NNBF 72-256-256-2304 ReLU, bf_scale 0.01 "
+              "(678,912 params), BF16 GEMMs, 4096 walkers")),
 }
 
 
@@ -47,42 +61,59 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="hubbard8x8_slater", choices=sorted(WORKLOADS))
-    p.add_argument("--gram", default="fp64", choices=["fp64", "fp32", "bf16"])
+    p.add_argument("--gram", default=None, choices=["fp64", "fp32", "bf16"],
+                   help="Gram precision (default: the workload's)")
     p.add_argument("--refresh-every", type=int, default=64)
-    p.add_argument("--diag-shift", type=float, default=1e-4)
-    p.add_argument("--lr", type=float, default=0.05)
     p.add_argument("--tau-sweeps", type=int, default=64)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     return p.parse_args()
 
 
+def log(msg):
+    """Progress line on stderr (stdout carries only the JSON result)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 # ------------------------------------------------------------- CPU baseline
 def _cpu_worker(args):
-    """Reference algorithm on one core: batched full-determinant sweep,
-    E_loc by get_conn + determinants, O_k, for a chunk of walkers."""
+    """Reference algorithm on one core: batched full-amplitude sweep, E_loc by
+    get_conn + full amplitudes, O_k, for a chunk of walkers."""
     os.environ["OMP_NUM_THREADS"] = "1"
-    import numpy as np
-
     from oracle import hubbard as ohub
+    from oracle import nnbf as onn
     from oracle import sampler as osamp
     from oracle import slater as osl
-    Lx, Ly, nu, nd, M, x, seed, budget = args
+    kind, Lx, Ly, nu, nd, params, bf, x, seed, budget = args
     H = ohub.Hubbard(Lx, Ly, 1.0, 8.0, nu, nd)
     edges = H.graph.sweep_edges()
+    if kind == "slater":
+        M = params[0]
+
+        def logamp(c):
+            return osl.slogdet(M, c)
+
+        def work(xs, s, l):
+            osl.local_energy(H, M, xs, logamp=(s, l))
+            osl.log_derivative(M, xs)
+    else:
+        def logamp(c):
+            return onn.forward(params, c, 2, bf)[:2]
+
+        def work(xs, s, l):
+            onn.local_energy(H, params, xs, 2, bf)
+            onn.log_derivative(params, xs, 2, bf)
     done, t0 = 0, time.perf_counter()
     chunk = 4
     while time.perf_counter() - t0 < budget:
         xs = x[(done % len(x)):(done % len(x)) + chunk]
-        xs, s, l, _ = osamp.sweep(lambda c: osl.slogdet(M, c), xs, edges,
-                                  osamp.PhiloxDraws(seed, done), 1)
-        osl.local_energy(H, M, xs, logamp=(s, l))
-        osl.log_derivative(M, xs)
+        xs, s, l, _ = osamp.sweep(logamp, xs, edges, osamp.PhiloxDraws(seed, done), 1)
+        work(xs, s, l)
         done += xs.shape[0]
     return done, time.perf_counter() - t0
 
 
-def cpu_baseline(Lx, Ly, nu, nd, M, x, n_samples_step, tau, seconds):
+def cpu_baseline(kind, Lx, Ly, nu, nd, params, bf, x, n_samples_step, tau, shift, seconds):
     import multiprocessing as mp
 
     import numpy as np
@@ -92,23 +123,24 @@ def cpu_baseline(Lx, Ly, nu, nd, M, x, n_samples_step, tau, seconds):
     ctx = mp.get_context("spawn")
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     per = max(1, len(x) // cores)
-    jobs = [(Lx, Ly, nu, nd, M, x[i * per:(i + 1) * per], 1000 + i, seconds * 0.6)
-            for i in range(cores)]
+    jobs = [(kind, Lx, Ly, nu, nd, params, bf, x[i * per:(i + 1) * per], 1000 + i,
+             seconds * 0.6) for i in range(cores)]
     with ctx.Pool(cores) as pool:
         res = pool.map(_cpu_worker, jobs)
     samples = sum(r[0] for r in res)
     wall = max(r[1] for r in res)
     samples_per_s = samples / wall
-    # SR on the host at Ns_cpu samples, extrapolated to the step's Ns by the
-    # Gram cost Ns^2 Np (dominant term)
-    Np = M.size
-    ns_cpu = 512
+    # SR on the host at a reduced sample count, extrapolated to the step's Ns
+    # by the Gram cost Ns^2 Np (the dominant term at these shapes)
+    Np = int(sum(p.size for p in params))
+    ns_cpu = 512 if Np < 100000 else 128
     rng = np.random.default_rng(0)
     O = rng.normal(size=(ns_cpu, Np))
     E = rng.normal(size=ns_cpu)
     t0 = time.perf_counter()
-    osr.minsr(O, E, E.mean(), 1e-4)
+    osr.minsr(O, E, E.mean(), shift)
     t_sr_small = time.perf_counter() - t0
+    del O
     t_sr = t_sr_small * (n_samples_step / ns_cpu) ** 2
     t_iter = n_samples_step / samples_per_s + t_sr
     return {
@@ -116,7 +148,7 @@ def cpu_baseline(Lx, Ly, nu, nd, M, x, n_samples_step, tau, seconds):
         "unit": "effective samples/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"FP64 oracle restating the reference algorithm (full determinant per "
+        "sample": (f"FP64 oracle restating the reference algorithm (full amplitude per "
                    f"proposal and per connected configuration), {samples} walker-samples "
                    f"(sweep + E_loc + O_k) on {cores} processes in {wall:.1f} s; minSR "
                    f"timed at Ns={ns_cpu}, Np={Np} ({t_sr_small:.2f} s) and extrapolated by "
@@ -137,6 +169,166 @@ def _cpu_model():
     return "unknown"
 
 
+# ------------------------------------------------------------- workloads
+class SlaterRun:
+    """Slater walkers driven through the kernel wrappers (the C-ABI calls the
+    reference-shaped model class makes, without its parameter-change checks)."""
+
+    names = ["refresh", "sweep", "local_energy", "logderiv", "sr", "update"]
+
+    def __init__(self, w, a, dev, H, fxs):
+        import torch
+
+        from vmc_torch_amd import kernels
+        self.k, self.a, self.w, self.H, self.fxs = kernels, a, w, H, fxs
+        N, nf, B = w["Lx"] * w["Ly"], w["nu"] + w["nd"], w["B"]
+        self.N, self.nf, self.B, self.Np = N, nf, B, 2 * N * nf
+        g = torch.Generator().manual_seed(0)
+        self.M = (torch.randn(2 * N, nf, generator=g, dtype=torch.float64) * 0.1).to(dev)
+        self.e_sweep = H.graph.edge_tensor(dev, "sweep")
+        self.e_graph = H.graph.edge_tensor(dev, "graph")
+        self.st = kernels.SlaterWalkerState.empty(B, N, nf, dev)
+        self.O = torch.empty(B, self.Np, dtype=torch.float64, device=dev)
+        self.dp = torch.empty(self.Np, dtype=torch.float64, device=dev)
+        self.ws = kernels.SRWorkspace()
+
+    def _sweep(self, n, seed, off):
+        self.k.slater_sweep(self.M, self.e_sweep, self.fxs, self.st, n_sweeps=n, seed=seed,
+                            sweep_offset=off, refresh_every=self.a.refresh_every)
+
+    def burn(self, n, seed, off):
+        self.k.slater_refresh(self.M, self.fxs, self.st)
+        self._sweep(n, seed, off)
+
+    def local_energy(self):
+        return self.k.slater_local_energy(self.M, self.e_graph, 1.0, 8.0, self.fxs, self.st)
+
+    def step(self, mark, seed, off, gram, phases):
+        m = [mark()]
+        self.k.slater_refresh(self.M, self.fxs, self.st)     # M changed in the last update
+        m.append(mark())
+        acc0 = self.st.n_accept.clone()
+        self._sweep(1, seed, off)
+        m.append(mark())
+        E = self.local_energy()
+        m.append(mark())
+        self.k.slater_logderiv(self.st, self.N, self.nf, out=self.O)
+        m.append(mark())
+        _, stats, info = self.k.sr_minsr(self.O, E, None, self.w["shift"], gram,
+                                         0 if gram == "fp64" else 20, 1e-8, self.ws,
+                                         dp=self.dp, phase_ms=phases)
+        m.append(mark())
+        self.k.sgd_update(self.M.view(-1), self.dp, self.w["lr"])
+        m.append(mark())
+        return m, E, self.st.n_accept - acc0, stats, info
+
+    def pre_tau(self):
+        self.k.slater_refresh(self.M, self.fxs, self.st)
+
+    def tau_sample(self, seed, off):
+        self._sweep(1, seed, off)
+        return self.local_energy()
+
+    def roofline_items(self):
+        """Algorithmic HBM bytes per step: A^-1 (B nfp^2 FP64) read and, for
+        the sweep, written once; configs and the per-walker rows."""
+        B, N, nf = self.B, self.N, self.nf
+        nfp = self.k.slater_nfp(nf)
+        return {
+            "sweep": ("hbm", B * (2 * nfp * nfp * 8 + 2 * 2 * N + 2 * 8 * N)),
+            "local_energy": ("hbm", B * (nfp * nfp * 8 + 2 * N + 8 * N + 8)),
+            "logderiv": ("hbm", B * (nf * nfp * 8 + 2 * N + self.Np * 8)),
+            "refresh": ("hbm", B * (nfp * nfp * 8 + 2 * N + 8 * N)),
+        }
+
+    def oracle_params(self):
+        return [self.M.cpu().numpy()], 0.0
+
+
+class NnbfRun:
+    """NNBF walkers through the reference-shaped model class NNBF_GPU."""
+
+    names = ["sweep", "local_energy", "logderiv", "sr", "update"]
+
+    def __init__(self, w, a, dev, H, fxs):
+        import torch
+
+        from vmc_torch_amd import NNBF_GPU, kernels
+        self.k, self.a, self.w, self.H, self.fxs = kernels, a, w, H, fxs
+        N, nf, B = w["Lx"] * w["Ly"], w["nu"] + w["nd"], w["B"]
+        self.N, self.nf, self.B = N, nf, B
+        self.model = NNBF_GPU(N, nf, hidden_dim=w["hidden"], activation="relu",
+                              bf_scale=w["bf_scale"], gemm=w["mlp"], init="xavier", m_std=0.1,
+                              generator=torch.Generator().manual_seed(0)).to(dev)
+        self.Np = self.model.n_params
+        self.e_sweep = H.graph.edge_tensor(dev, "sweep")
+        self.e_graph = H.graph.edge_tensor(dev, "graph")
+        self.O = torch.empty(B, self.Np, dtype=torch.float32, device=dev)
+        self.dp = torch.empty(self.Np, dtype=torch.float64, device=dev)
+        self.ws = self.k.SRWorkspace()
+
+    def burn(self, n, seed, off):
+        self.model.mcmc_sweep(self.fxs, self.e_sweep, n, seed, off)
+
+    def local_energy(self):
+        return self.model.local_energy(self.fxs, self.H)
+
+    def step(self, mark, seed, off, gram, phases):
+        m = [mark()]
+        acc0 = self.model.walker_state(self.fxs)["n_accept"].clone()
+        _, _, nacc = self.model.mcmc_sweep(self.fxs, self.e_sweep, 1, seed, off)
+        m.append(mark())
+        E = self.local_energy()
+        m.append(mark())
+        self.model.log_derivative(self.fxs, dtype=self.O.dtype, out=self.O)
+        m.append(mark())
+        _, stats, info = self.k.sr_minsr(self.O, E, None, self.w["shift"], gram,
+                                         0 if gram == "fp64" else 20, 1e-8, self.ws,
+                                         dp=self.dp, phase_ms=phases, inplace=True)
+        m.append(mark())
+        off_p = 0
+        for p in self.model.params:
+            n = p.numel()
+            self.k.sgd_update(p.data.view(-1), self.dp[off_p:off_p + n], self.w["lr"])
+            off_p += n
+        self.model.invalidate()
+        m.append(mark())
+        return m, E, nacc - acc0, stats, info
+
+    def pre_tau(self):
+        pass
+
+    def tau_sample(self, seed, off):
+        self.model.mcmc_sweep(self.fxs, self.e_sweep, 1, seed, off)
+        return self.local_energy()
+
+    def roofline_items(self):
+        """Sweep and local energy: dense-layer flops (layer 2 and the output
+        layer; the first layer is a gather) per amplitude evaluated, one per
+        proposal and one per connected configuration.  logderiv: the O write."""
+        H, nD = self.w["hidden"], 2 * self.N * self.nf
+        mlp = 2.0 * (H * H + H * nD)
+        n_prop = self.e_sweep.shape[0] * self.B
+        x = self.fxs
+        e = self.e_graph.long()
+        n_conn = 0
+        for bit in (1, 2):                       # down, up occupation bits
+            occ = (x & bit) != 0
+            n_conn += int((occ[:, e[:, 0]] != occ[:, e[:, 1]]).sum().item())
+        self.n_conn = n_conn
+        return {
+            "sweep": ("mfma_bf16", n_prop * mlp),
+            "local_energy": ("mfma_bf16", n_conn * mlp),
+            "logderiv": ("hbm", self.B * self.Np * 4.0),
+        }
+
+    def oracle_params(self):
+        import torch
+        # the device computes with FP32 copies of the FP64 master parameters
+        return [p.detach().to(torch.float32).to(torch.float64).cpu().numpy()
+                for p in self.model.params], self.w["bf_scale"]
+
+
 # --------------------------------------------------------------------- main
 def main():
     a = parse()
@@ -148,162 +340,134 @@ def main():
     import numpy as np
     import torch
 
-    from vmc_torch_amd import kernels
     from vmc_torch_amd.hamiltonian import spinful_Fermi_Hubbard_square_lattice_torch
     from vmc_torch_amd.stats import tau_int
 
-    Lx, Ly, nu, nd, B, burn, desc = WORKLOADS[a.workload]
+    w = WORKLOADS[a.workload]
+    gram = a.gram or w["gram"]
+    Lx, Ly, nu, nd, B = w["Lx"], w["Ly"], w["nu"], w["nd"], w["B"]
     N, nf = Lx * Ly, nu + nd
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    H = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, nf, n_fermions_per_spin=(nu, nd))
+    H = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, nf,
+                                                   n_fermions_per_spin=(nu, nd))
     H.precompute_hops_gpu(dev)
-    g = torch.Generator().manual_seed(0)
-    M = (torch.randn(2 * N, nf, generator=g, dtype=torch.float64) * 0.1).to(dev)
     # walkers: the reference's initial pattern, permuted per walker (seeded)
     base = torch.tensor([1, 2] * (N // 2), dtype=torch.int64)
     base[: N - nf] = 0
     perms = torch.argsort(torch.rand(B, N, generator=torch.Generator().manual_seed(42)), dim=1)
     fxs = base[perms].contiguous().to(dev)
-    e_sweep = H.graph.edge_tensor(dev, "sweep")
-    e_graph = H.graph.edge_tensor(dev, "graph")
-    st = kernels.SlaterWalkerState.empty(B, N, nf, dev)
-    O = torch.empty(B, 2 * N * nf, dtype=torch.float64, device=dev)
-    dp = torch.empty(2 * N * nf, dtype=torch.float64, device=dev)
-    ws = kernels.SRWorkspace()
-    seed = 0x5EED
-    sweep_no = 0
-
-    kernels.slater_refresh(M, fxs, st)
-    kernels.slater_sweep(M, e_sweep, fxs, st, n_sweeps=burn, seed=seed, sweep_offset=0,
-                         refresh_every=a.refresh_every)
-    sweep_no += burn
+    run = (SlaterRun if w["kind"] == "slater" else NnbfRun)(w, a, dev, H, fxs)
+    seed, sweep_no = 0x5EED, 0
+    log(f"{a.workload}: {B} walkers, {run.Np} parameters; burn-in {w['burn']} sweeps")
+    run.burn(w["burn"], seed, 0)
+    sweep_no += w["burn"]
     torch.cuda.synchronize()
+    log("burn-in done")
 
-    names = ["refresh", "sweep", "local_energy", "logderiv", "sr", "update"]
+    def mark():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    names = run.names
     ev = {k: [] for k in names}
-    E_hist = []
-    sr_phases = []
-
-    def step(record):
-        nonlocal sweep_no
-        def mark():
-            e = torch.cuda.Event(enable_timing=True)
-            e.record()
-            return e
-        m0 = mark()
-        kernels.slater_refresh(M, fxs, st)           # M changed in the last update
-        m1 = mark()
-        acc0 = st.n_accept.clone()
-        kernels.slater_sweep(M, e_sweep, fxs, st, n_sweeps=1, seed=seed,
-                             sweep_offset=sweep_no, refresh_every=a.refresh_every)
-        sweep_no += 1
-        m2 = mark()
-        E = kernels.slater_local_energy(M, e_graph, 1.0, 8.0, fxs, st)
-        m3 = mark()
-        kernels.slater_logderiv(st, N, nf, out=O)
-        m4 = mark()
-        ph = [] if record == "phases" else None
-        _, stats, info = kernels.sr_minsr(O, E, None, a.diag_shift, a.gram,
-                                          0 if a.gram == "fp64" else 20, 1e-8, ws, dp=dp,
-                                          phase_ms=ph)
-        if ph is not None:
-            sr_phases.append(ph)
-        m5 = mark()
-        kernels.sgd_update(M.view(-1), dp, a.lr)
-        m6 = mark()
-        if record is True:
-            for k, (x0, x1) in zip(names, [(m0, m1), (m1, m2), (m2, m3), (m3, m4), (m4, m5),
-                                           (m5, m6)]):
-                ev[k].append((x0, x1))
-            E_hist.append(E.clone())
-        return st.n_accept - acc0, stats, info
-
+    E_hist, accs = [], []
     for _ in range(a.warmup):
-        step(False)
+        run.step(mark, seed, sweep_no, gram, None)
+        sweep_no += 1
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    accs = []
     for _ in range(a.steps):
-        acc, stats, info = step(True)
+        m, E, acc, stats, info = run.step(mark, seed, sweep_no, gram, None)
+        sweep_no += 1
+        for k, x0, x1 in zip(names, m[:-1], m[1:]):
+            ev[k].append((x0, x1))
+        E_hist.append(E.clone())
         accs.append(acc)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / a.steps
+    log(f"{a.steps} timed SR iterations: {ms_step:.2f} ms each")
     info_v = int(info.item())
     st_v = stats.cpu().tolist()
+    kt = {k: sum(x0.elapsed_time(x1) for x0, x1 in ev[k]) / a.steps for k in names}  # ms
     # SR phase split, measured on extra (untimed) steps: the phase timer
     # synchronises the host once per SR call
+    sr_phases = []
     for _ in range(3):
-        step("phases")
+        ph = []
+        run.step(mark, seed, sweep_no, gram, ph)
+        sweep_no += 1
+        sr_phases.append(ph)
     torch.cuda.synchronize()
     phase_names = ["center", "gram", "factor", "solve", "dp_stats"]
     phase_ms = {k: sum(p[i] for p in sr_phases) / len(sr_phases)
                 for i, k in enumerate(phase_names)}
-    kt = {k: sum(x0.elapsed_time(x1) for x0, x1 in ev[k]) / a.steps for k in names}  # ms
-    acc_rate = float(torch.stack(accs).double().mean()) / e_sweep.shape[0]
-    E_steps = torch.stack(E_hist)                        # (steps, B)
-    e_site = [float(v) / N for v in E_steps.mean(dim=1)]
+    acc_rate = float(torch.stack(accs).double().mean()) / run.e_sweep.shape[0]
+    e_site = [float(v) / N for v in torch.stack(E_hist).mean(dim=1)]
 
     # tau_int of E_loc at fixed parameters, one sample per sweep (as in the loop)
-    kernels.slater_refresh(M, fxs, st)
+    run.pre_tau()
     series = []
     for _ in range(a.tau_sweeps):
-        kernels.slater_sweep(M, e_sweep, fxs, st, n_sweeps=1, seed=seed, sweep_offset=sweep_no,
-                             refresh_every=a.refresh_every)
+        series.append(run.tau_sample(seed, sweep_no).clone())
         sweep_no += 1
-        series.append(kernels.slater_local_energy(M, e_graph, 1.0, 8.0, fxs, st).clone())
     torch.cuda.synchronize()
-    ser = torch.stack(series, dim=1)                     # (B, T)
-    tau = tau_int(ser)
+    tau = tau_int(torch.stack(series, dim=1))
+    log(f"SR phases and tau_int ({a.tau_sweeps} sweeps) done: tau_int = {tau:.3f}")
     sr_iters = a.steps / wall
     samples_per_s = B * sr_iters
     ess_per_s = samples_per_s / tau
 
-    # roofline of every kernel: algorithmic work per launch / mean duration
-    Np, Ns = 2 * N * nf, B
-    nfp = kernels.slater_nfp(nf)
+    # roofline of every component: algorithmic work per step / mean duration
+    Np, Ns = run.Np, B
     kern = {}
-    by = {
-        "sweep": ("hbm", B * (2 * nfp * nfp * 8 + 2 * 2 * N + 2 * 8 * N)),
-        "local_energy": ("hbm", B * (nfp * nfp * 8 + 2 * N + 8 * N + 8)),
-        "logderiv": ("hbm", B * (nf * nfp * 8 + 2 * N + Np * 8)),
-        "refresh": ("hbm", B * (nfp * nfp * 8 + 2 * N + 8 * N)),
-    }
-    for k, (bound, nbytes) in by.items():
+    for k, (bound, work) in run.roofline_items().items():
         secs = kt[k] * 1e-3
-        kern[k] = {"ms": kt[k], "bound": bound, "achieved": nbytes / secs / 1e9,
-                   "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                   "frac": nbytes / secs / PEAK_HBM}
-    peak_sr = {"fp64": PEAK_FP64, "fp32": PEAK_FP32_MFMA, "bf16": PEAK_BF16}[a.gram]
-    gram_flops = float(Ns) * Ns * Np                     # SYRK: lower half of Ot Ot^T
+        if bound == "hbm":
+            kern[k] = {"ms": kt[k], "bound": "hbm", "achieved": work / secs / 1e9,
+                       "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": work / secs / PEAK_HBM}
+        else:
+            kern[k] = {"ms": kt[k], "bound": "mfma", "achieved": work / secs / 1e12,
+                       "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                       "frac": work / secs / PEAK_BF16,
+                       "note": "dense-layer flops of every amplitude evaluated, BF16 operands"}
+    peak_sr = {"fp64": PEAK_FP64, "fp32": PEAK_FP32_MFMA, "bf16": PEAK_BF16}[gram]
+    gram_flops = float(Ns) * Ns * Np * (1 if gram == "fp64" else 2)
     chol_flops = float(Ns) ** 3 / 3
     kern["sr"] = {"ms": kt["sr"], "phases_ms": phase_ms}
+    kern["update"] = {"ms": kt["update"]}
 
-    def flop_roof(name, flops, ms, peak, note):
+    def flop_roof(flops, ms, peak, note):
         return {"ms": ms, "bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12,
                 "peak": peak / 1e12, "unit": "TFLOP/s", "frac": flops / (ms * 1e-3) / peak,
                 "note": note}
-    kern["sr.gram"] = flop_roof("gram", gram_flops, phase_ms["gram"], peak_sr,
-                                "Ns^2 Np (SYRK lower half), one rocBLAS call")
-    kern["sr.factor"] = flop_roof("factor", chol_flops, phase_ms["factor"], PEAK_FP64,
-                                  "Ns^3/3: blocked Cholesky (LDS diagonal kernel + TRSM/SYRK)")
-    cbytes = 2.0 * Ns * Np * 8 + Np * 8 * 64             # read O twice, write X
-    kern["sr.center"] = {"ms": phase_ms["center"], "bound": "hbm",
-                         "achieved": cbytes / (phase_ms["center"] * 1e-3) / 1e9,
-                         "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                         "frac": cbytes / (phase_ms["center"] * 1e-3) / PEAK_HBM}
-    sbytes = 2.0 * Ns * Ns / 2 * 8 + 2 * Ns * Np * 8     # read L twice, O^T alpha
-    kern["sr.solve"] = {"ms": phase_ms["solve"], "bound": "hbm",
-                        "achieved": sbytes / (phase_ms["solve"] * 1e-3) / 1e9,
-                        "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                        "frac": sbytes / (phase_ms["solve"] * 1e-3) / PEAK_HBM}
-    kern["update"] = {"ms": kt["update"]}
+
+    def hbm_roof(nbytes, ms):
+        return {"ms": ms, "bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9,
+                "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / PEAK_HBM}
+    kern["sr.gram"] = flop_roof(gram_flops, phase_ms["gram"], peak_sr,
+                                "Ns^2 Np (FP64: SYRK lower half; reduced precision: full "
+                                "GEMM), one rocBLAS call")
+    kern["sr.factor"] = flop_roof(chol_flops, phase_ms["factor"], PEAK_FP64,
+                                  "Ns^3/3: blocked FP64 Cholesky (LDS diagonal kernel + "
+                                  "TRSM/SYRK)")
+    osz = run.O.element_size()
+    if gram == "fp64":
+        cbytes = 2.0 * Ns * Np * osz + Ns * Np * osz          # read O twice, write X
+    else:
+        cbytes = 2.0 * Ns * Np * osz + Ns * Np * (osz + 2)    # + in-place O and BF16/FP32 copy
+    kern["sr.center"] = hbm_roof(cbytes, phase_ms["center"])
     dom = max((k for k in kern if "frac" in kern[k]), key=lambda k: kern[k]["ms"])
     roof = {kk: kern[dom][kk] for kk in ("bound", "achieved", "peak", "unit", "frac")}
     roof["kernel"] = dom
     roof["traffic"] = None
 
+    if w["kind"] == "slater":
+        dtype = "f64" if gram == "fp64" else f"f64+{gram}-gram"
+    else:
+        dtype = f"{w['mlp']}-gemm/f64-det/f32-O/{gram}-gram/f64-cholesky"
     out = {
         "metric": "effective samples/s + SR iters/s at 8x8 Hubbard; |dE/site| vs FP64 oracle",
         "value": ess_per_s,
@@ -311,12 +475,12 @@ def main():
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if a.gram == "fp64" else f"f64+{a.gram}-gram",
-        "data": "synthetic (seeded N(0,0.1) orbitals, seeded initial walkers)",
-        "config": {"workload": a.workload, "description": desc, "walkers": B,
-                   "samples_per_sr_iter": Ns, "n_params": Np, "burn_in_sweeps": burn,
-                   "refresh_every": a.refresh_every, "gram": a.gram,
-                   "diag_shift": a.diag_shift, "lr": a.lr, "parallelism": "walkers on 1 GPU"},
+        "dtype": dtype,
+        "data": "synthetic (seeded parameters, seeded initial walkers)",
+        "config": {"workload": a.workload, "description": w["desc"], "walkers": B,
+                   "samples_per_sr_iter": Ns, "n_params": Np, "burn_in_sweeps": w["burn"],
+                   "gram": gram, "diag_shift": w["shift"], "lr": w["lr"],
+                   "parallelism": "walkers on 1 GPU"},
         "sr_iters_per_s": sr_iters,
         "walker_sweeps_per_s": samples_per_s,
         "sweep_only_walker_sweeps_per_s": B / (kt["sweep"] * 1e-3),
@@ -327,20 +491,30 @@ def main():
         "roofline": roof,
         "kernels": kern,
     }
+    if w["kind"] == "slater":
+        out["config"]["refresh_every"] = a.refresh_every
 
     if not a.no_cpu_baseline:
         # FP64 oracle on a subset of the final walkers: |dE/site|, then the timed baseline
         from oracle import hubbard as ohub
+        from oracle import nnbf as onn
         from oracle import slater as osl
-        Mh = M.cpu().numpy()
-        xs = fxs[:128].cpu().numpy()
-        Eg = kernels.slater_local_energy(M, e_graph, 1.0, 8.0, fxs, st)[:128].cpu().numpy()
-        Eo = osl.local_energy(ohub.Hubbard(Lx, Ly, 1.0, 8.0, nu, nd), Mh, xs)
+        params, bf = run.oracle_params()
+        n_chk = 128 if w["kind"] == "slater" else 32
+        xs = fxs[:n_chk].cpu().numpy()
+        Eg = run.local_energy()[:n_chk].cpu().numpy()
+        Ho = ohub.Hubbard(Lx, Ly, 1.0, 8.0, nu, nd)
+        if w["kind"] == "slater":
+            Eo = osl.local_energy(Ho, params[0], xs)
+        else:
+            Eo = onn.local_energy(Ho, params, xs, 2, bf)
         out["dE_per_site_vs_fp64_oracle"] = {
             "mean": float(abs(Eg.mean() - Eo.mean()) / N),
             "max_per_sample": float(np.abs(Eg - Eo).max() / N), "samples": int(len(xs))}
-        out["cpu_baseline"] = cpu_baseline(Lx, Ly, nu, nd, Mh, fxs[:4096].cpu().numpy(), Ns,
-                                           tau, a.cpu_seconds)
+        log("FP64 oracle check done; CPU baseline starts")
+        out["cpu_baseline"] = cpu_baseline(w["kind"], Lx, Ly, nu, nd, params, bf,
+                                           fxs[:4096].cpu().numpy(), Ns, tau, w["shift"],
+                                           a.cpu_seconds)
     print(json.dumps(out))
 
 

@@ -29,96 +29,230 @@ constexpr int CHOL_NB = 128;
 // Dinv_k (column-major NB x NB): Dinv[m*NB + c] = (L_kk^-1)[c][m].
 // DinvT_k (column-major):       DinvT[c*NB + m] = (L_kk^-1)[c][m].
 //
-// The 128 x 128 diagonal block is factorised in LDS in 16-column steps:
-//   leaf   : one wave factorises the 16 x 16 diagonal tile in registers
-//            (lane i holds row i; columns broadcast with shuffles)
-//   panel  : one thread per row solves x L_tile^T = a by substitution
-//   update : rank-16 update of the trailing lower triangle
-// and its inverse is formed in place column-block by column-block from the
-// right (blocked LAPACK trtri order), with 16 x 16 leaf inverses.
+// The 128 x 128 diagonal block is factorised in LDS in 16-column steps t:
+//   leaf     : one wave factorises the 16 x 16 diagonal tile in registers
+//              (lane i holds row i; columns broadcast with DPP row_newbcast)
+//              and inverts it there; L_tt stays in the lower triangle of the
+//              tile, the strictly lower part of X_tt = L_tt^-1 goes
+//              transposed into its (unused) upper triangle, and the
+//              reciprocal diagonal 1 / L_ii (= X_ii) into rdiag
+//   panel    : L_I,t = A_I,t X_tt^T for every tile I below (matrix cores)
+//   trailing : C_IJ -= L_I,t L_J,t^T for t < J <= I (matrix cores); the wave
+//              that updates tile (t+1, t+1) factorises it at once, while the
+//              other waves finish the rest of the trailing update
+// The inverse X = L^-1 of the whole block is then formed in place, tile
+// column jt from the right:  X_T,jt = -(sum_{jt < K <= T} X_TK L_K,jt) X_jt,jt.
+//
+// v_mfma_f64_16x16x4f64 operand layout (lane l of the wave, k-step of 4):
+//   A (16 x 4): lane l holds A[l % 16][l / 16]
+//   B (4 x 16): lane l holds B[l / 16][l % 16]
+//   D (16 x 16, four FP64 registers r): lane l, r holds D[4 r + l / 16][l % 16]
 constexpr int LEAF = 16;
+constexpr int DIAG_THREADS = 512;
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(1024)
-    chol_diag_kernel(double *A, int64_t lda, int64_t k0, int nbk, int32_t *info) {
+// 1/d and 1/sqrt(d) for d > 0: hardware estimate and two Newton steps
+__device__ __forceinline__ double rcp_f64(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-d, y, 1.0);
+  return fma(y, e, y);
+}
+
+__device__ __forceinline__ double rsqrt_f64(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const double h = d * y;
+    const double e = fma(-h, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+
+// value of v in lane S of each row of 16 lanes (DPP row_newbcast)
+template <int S>
+__device__ __forceinline__ double bcast16_c(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, 0x150 + S, 0xF, 0xF, false);
+  const int hi =
+      __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x150 + S, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// s must fold to a constant (fully unrolled loops)
+__device__ __forceinline__ double bcast16(double v, int s) {
+  switch (s) {
+    case 0: return bcast16_c<0>(v);
+    case 1: return bcast16_c<1>(v);
+    case 2: return bcast16_c<2>(v);
+    case 3: return bcast16_c<3>(v);
+    case 4: return bcast16_c<4>(v);
+    case 5: return bcast16_c<5>(v);
+    case 6: return bcast16_c<6>(v);
+    case 7: return bcast16_c<7>(v);
+    case 8: return bcast16_c<8>(v);
+    case 9: return bcast16_c<9>(v);
+    case 10: return bcast16_c<10>(v);
+    case 11: return bcast16_c<11>(v);
+    case 12: return bcast16_c<12>(v);
+    case 13: return bcast16_c<13>(v);
+    case 14: return bcast16_c<14>(v);
+    default: return bcast16_c<15>(v);
+  }
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Leaf of step kb (wave 0): factor and invert the tile at (kb, kb).
+__device__ __forceinline__ void chol_leaf(double (*L)[CHOL_NB + 1], double *rdiag_all, int kb,
+                                          int lane, int *bad) {
+  const int i = lane & 15;
+  const bool act = lane < LEAF;
+  double r[LEAF];
+#pragma unroll
+  for (int c = 0; c < LEAF; ++c) r[c] = (c <= i) ? L[kb + i][kb + c] : 0.0;
+  double myrs = 1.0;
+  int badj = 0;
+#pragma unroll
+  for (int j = 0; j < LEAF; ++j) {
+    // unscaled column j first (L_ij L_lj = a_ij a_lj / a_jj), so that the
+    // broadcasts do not wait for the square root
+    const double djj = bcast16(r[j], j);
+    double col[LEAF];
+#pragma unroll
+    for (int l = j + 1; l < LEAF; ++l) col[l] = bcast16(r[j], l);
+    const bool ok = djj > 0.0;
+    if (!ok && badj == 0) badj = j + 1;
+    const double inv = ok ? rcp_f64(djj) : 1.0;
+    const double rs = ok ? rsqrt_f64(djj) : 1.0;
+    const double t = r[j] * inv;
+#pragma unroll
+    for (int l = j + 1; l < LEAF; ++l) r[l] = (i >= l) ? fma(-t, col[l], r[l]) : r[l];
+    r[j] = (i == j) ? (ok ? djj * rs : 1.0) : ((i > j) ? r[j] * rs : r[j]);
+    if (i == j) myrs = rs;
+  }
+  // X = L_tt^-1 by rows: X_ii = 1 / L_ii, X_ic = -X_ii sum_{c <= q < i} L_iq X_qc
+  double acc[LEAF];
+#pragma unroll
+  for (int c = 0; c < LEAF; ++c) acc[c] = 0.0;
+#pragma unroll
+  for (int q = 0; q < LEAF - 1; ++q) {
+#pragma unroll
+    for (int c = 0; c <= q; ++c) {
+      const double v = (c == q) ? myrs : -myrs * acc[c];  // X[q][c] in lane q
+      const double xb = bcast16(v, q);
+      acc[c] = (i > q) ? fma(r[q], xb, acc[c]) : acc[c];
+    }
+  }
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < LEAF; ++c) {
+      if (c <= i) L[kb + i][kb + c] = r[c];
+      if (c < i) L[kb + c][kb + i] = -myrs * acc[c];
+    }
+    rdiag_all[kb + i] = myrs;
+    if (lane == 0 && badj && *bad == 0) *bad = kb + badj;
+  }
+}
+
+// X_tt (lower, the leaf inverse of tile t at kb) as a matrix-core B operand:
+// element [k][j] of X_tt^T (transpose = 0) or of X_tt (transpose = 1)
+__device__ __forceinline__ double leaf_inv(const double (*L)[CHOL_NB + 1],
+                                          const double *rdiag_all, int kb, int row, int col) {
+  // X_tt[row][col]
+  if (row == col) return rdiag_all[kb + row];
+  return (row > col) ? L[kb + col][kb + row] : 0.0;
+}
+
+__global__ void __launch_bounds__(DIAG_THREADS)
+    chol_diag_kernel(double *A, int64_t lda, int64_t k0, int nbk, int32_t *info,
+                     double *Dinv, double *DinvT) {
   __shared__ double L[CHOL_NB][CHOL_NB + 1];  // L[i][j], row i, column j
-  __shared__ double rdiag[LEAF];              // 1 / L[kb+c][kb+c] of the current tile
+  __shared__ double W[CHOL_NB][LEAF + 1];     // inverse phase: W_T tiles
+  __shared__ double rdiag_all[CHOL_NB];       // 1 / L[i][i]
   __shared__ int bad;
-  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  constexpr int NWAVE = DIAG_THREADS / 64;
+  constexpr int NT = CHOL_NB / LEAF;
   double *Akk = A + k0 * lda + k0;
-  for (int idx = tid; idx < CHOL_NB * CHOL_NB; idx += nt) {
-    const int j = idx / CHOL_NB, i = idx - j * CHOL_NB;
-    double v = 0.0;
-    if (i < nbk && j < nbk) v = (i >= j) ? Akk[(int64_t)j * lda + i] : 0.0;
-    else if (i == j) v = 1.0;  // identity padding keeps the leaves regular
-    L[i][j] = v;
+  // all loads issued before the first LDS store (16 per thread)
+  {
+    double v[CHOL_NB * CHOL_NB / DIAG_THREADS];
+#pragma unroll
+    for (int q = 0; q < CHOL_NB * CHOL_NB / DIAG_THREADS; ++q) {
+      const int idx = tid + q * DIAG_THREADS;
+      const int j = idx / CHOL_NB, i = idx - j * CHOL_NB;
+      double x = 0.0;
+      if (i < nbk && j < nbk) x = (i >= j) ? Akk[(int64_t)j * lda + i] : 0.0;
+      else if (i == j) x = 1.0;  // identity padding keeps the leaves regular
+      v[q] = x;
+    }
+#pragma unroll
+    for (int q = 0; q < CHOL_NB * CHOL_NB / DIAG_THREADS; ++q) {
+      const int idx = tid + q * DIAG_THREADS;
+      const int j = idx / CHOL_NB, i = idx - j * CHOL_NB;
+      L[i][j] = v[q];
+    }
   }
   if (tid == 0) bad = 0;
   __syncthreads();
+  if (wave == 0) chol_leaf(L, rdiag_all, 0, lane, &bad);
+  __syncthreads();
 
-  for (int kb = 0; kb < CHOL_NB; kb += LEAF) {
-    // ---- leaf: rows kb..kb+15 by lanes 0..15 of wave 0
-    if (wave == 0) {
-      const int i = lane;  // lanes >= LEAF carry dummies
-      double r[LEAF];
+  for (int t = 0; t < NT; ++t) {
+    const int kb = t * LEAF;
+    const int nt = NT - 1 - t;  // tiles below the leaf
+    if (nt == 0) break;
+    // ---- panel: L_I,t = A_I,t X_tt^T, one wave per tile
+    if (wave < nt) {
+      const int r0 = kb + LEAF * (wave + 1);
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int c = 0; c < LEAF; ++c)
-        r[c] = (i < LEAF && c <= i) ? L[kb + i][kb + c] : 0.0;
-#pragma unroll
-      for (int j = 0; j < LEAF; ++j) {
-        const double djj = __shfl(r[j], j, 64);
-        const bool ok = djj > 0.0;
-        if (!ok && lane == 0 && bad == 0) bad = kb + j + 1;
-        const double sd = ok ? sqrt(djj) : 1.0;
-        const double rs = 1.0 / sd;
-        if (i == j) r[j] = sd;
-        if (i > j && i < LEAF) r[j] *= rs;
-        if (lane == j) rdiag[j] = rs;
-#pragma unroll
-        for (int l = j + 1; l < LEAF; ++l) {
-          const double llj = __shfl(r[j], l, 64);
-          if (i >= l && i < LEAF) r[l] -= r[j] * llj;
-        }
+      for (int s4 = 0; s4 < LEAF; s4 += 4) {
+        const int kk = s4 + lk;
+        const double a = L[r0 + li][kb + kk];
+        const double b = leaf_inv(L, rdiag_all, kb, li, kk);  // X_tt^T[kk][li]
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
       }
-      if (i < LEAF) {
+      wave_lds_sync();
 #pragma unroll
-        for (int c = 0; c < LEAF; ++c)
-          if (c <= i) L[kb + i][kb + c] = r[c];
-      }
+      for (int r = 0; r < 4; ++r) L[r0 + 4 * r + lk][kb + li] = acc[r];
     }
     __syncthreads();
-    // ---- panel: rows below the tile, x L_tile^T = a (one thread per row)
-    for (int row = kb + LEAF + tid; row < CHOL_NB; row += nt) {
-      double x[LEAF];
-#pragma unroll
-      for (int c = 0; c < LEAF; ++c) {
-        double v = L[row][kb + c];
-#pragma unroll
-        for (int m = 0; m < c; ++m) v -= x[m] * L[kb + c][kb + m];
-        x[c] = v * rdiag[c];
-      }
-#pragma unroll
-      for (int c = 0; c < LEAF; ++c) L[row][kb + c] = x[c];
-    }
-    __syncthreads();
-    // ---- trailing update: L[i][l] -= P_i . P_l for kb+16 <= l <= i,
-    //      8 threads per row, columns strided by 8
+    // ---- trailing update C_IJ -= P_I P_J^T (1 <= J <= I <= nt, relative
+    //      to the leaf); wave 0 takes tile (1, 1) and then factorises it
     {
-      const int base = kb + LEAF;
-      const int i = base + (tid >> 3);
-      if (i < CHOL_NB) {
-        double pi[LEAF];
+      const int ntiles = nt * (nt + 1) / 2;
+      // tile 0 = (1, 1) to wave 0 alone, the others to waves 1..15
+      const int stride = (wave == 0) ? ntiles : NWAVE - 1;
+      for (int q = wave; q < ntiles; q += stride) {
+        int I = 1, rem = q;
+        while (rem >= I) { rem -= I; ++I; }
+        const int J = rem + 1;
+        const int r0 = kb + LEAF * I, c0 = kb + LEAF * J;
+        dbl4 acc;
 #pragma unroll
-        for (int m = 0; m < LEAF; ++m) pi[m] = L[i][kb + m];
-        for (int l = base + (tid & 7); l <= i; l += 8) {
-          double d0 = 0.0, d1 = 0.0;
+        for (int r = 0; r < 4; ++r) acc[r] = L[r0 + 4 * r + lk][c0 + li];
 #pragma unroll
-          for (int m = 0; m < LEAF; m += 2) {
-            d0 += pi[m] * L[l][kb + m];
-            d1 += pi[m + 1] * L[l][kb + m + 1];
-          }
-          L[i][l] -= d0 + d1;
+        for (int s4 = 0; s4 < LEAF; s4 += 4) {
+          const double a = -L[r0 + li][kb + s4 + lk];
+          const double b = L[c0 + li][kb + s4 + lk];
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
         }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) L[r0 + 4 * r + lk][c0 + li] = acc[r];
+      }
+      if (wave == 0) {
+        wave_lds_sync();
+        chol_leaf(L, rdiag_all, kb + LEAF, lane, &bad);
       }
     }
     __syncthreads();
@@ -127,84 +261,72 @@ __global__ void __launch_bounds__(1024)
     if (tid == 0 && info && info[0] == 0 && bad <= nbk) info[0] = (int32_t)(k0 + bad);
     return;
   }
-  for (int idx = tid; idx < nbk * nbk; idx += nt) {
+  for (int idx = tid; idx < nbk * nbk; idx += DIAG_THREADS) {
     const int j = idx / nbk, i = idx - j * nbk;
     if (i >= j) Akk[(int64_t)j * lda + i] = L[i][j];
   }
-}
-
-// Inverse of every factorised diagonal block (one workgroup each, all in
-// parallel after the factorisation), in place in LDS, column blocks from the
-// right:  X_jj = inv(L_jj);  X_{below,j} = -X_{below,below} L_{below,j} X_jj
-__global__ void __launch_bounds__(256)
-    chol_diag_inverse_kernel(const double *A, int64_t lda, int64_t n, double *Dinv,
-                             double *DinvT) {
-  __shared__ double L[CHOL_NB][CHOL_NB + 1];
-  __shared__ double W[CHOL_NB][LEAF + 1];
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int64_t kb = blockIdx.x, k0 = kb * CHOL_NB;
-  const int nbk = (int)((n - k0 < CHOL_NB) ? n - k0 : CHOL_NB);
-  const double *Akk = A + k0 * lda + k0;
-  for (int idx = tid; idx < CHOL_NB * CHOL_NB; idx += nt) {
-    const int j = idx / CHOL_NB, i = idx - j * CHOL_NB;
-    double v = 0.0;
-    if (i < nbk && j < nbk) v = (i >= j) ? Akk[(int64_t)j * lda + i] : 0.0;
-    else if (i == j) v = 1.0;
-    L[i][j] = v;
-  }
   __syncthreads();
-  for (int jb = CHOL_NB - LEAF; jb >= 0; jb -= LEAF) {
-    const int base = jb + LEAF, rows = CHOL_NB - base;
-    // W = X_{below,below} (already inverted, lower) * L_{below,j}
-    for (int idx = tid; idx < rows * LEAF; idx += nt) {
-      const int ii = idx / LEAF, c = idx - ii * LEAF;
-      const int i = base + ii;
-      double s = 0.0;
-      for (int q = base; q <= i; ++q) s += L[i][q] * L[q][jb + c];
-      W[ii][c] = s;
-    }
-    // leaf inverse of the 16 x 16 tile by 16 lanes (column c each)
-    if (wave == 0) {
-      const int c = lane;
-      double x[LEAF];
+  // ---- X = L^-1 in place (off-diagonal tiles; the diagonal tiles already
+  //      hold their inverses), tile column jt from the right
+  for (int jt = NT - 2; jt >= 0; --jt) {
+    const int jb = jt * LEAF, nt = NT - 1 - jt;
+    if (wave < nt) {
+      const int T = jt + 1 + wave;
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+      for (int K = jt + 1; K <= T; ++K) {
 #pragma unroll
-      for (int i = 0; i < LEAF; ++i) {
-        double v = (i == c) ? 1.0 : 0.0;
-#pragma unroll
-        for (int q = 0; q < i; ++q) v -= L[jb + i][jb + q] * x[q];
-        x[i] = (c < LEAF && i >= c) ? v / L[jb + i][jb + i] : 0.0;
+        for (int s4 = 0; s4 < LEAF; s4 += 4) {
+          const int kk = s4 + lk;
+          const double a = (K == T) ? leaf_inv(L, rdiag_all, LEAF * T, li, kk)
+                                    : L[LEAF * T + li][LEAF * K + kk];
+          const double b = L[LEAF * K + kk][jb + li];
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
       }
-      __builtin_amdgcn_wave_barrier();
-      if (c < LEAF) {
 #pragma unroll
-        for (int i = 0; i < LEAF; ++i)
-          if (i >= c) L[jb + i][jb + c] = x[i];
-      }
+      for (int r = 0; r < 4; ++r) W[LEAF * wave + 4 * r + lk][li] = acc[r];
     }
     __syncthreads();
-    // X_{below,j} = -W * X_jj
-    for (int idx = tid; idx < rows * LEAF; idx += nt) {
-      const int ii = idx / LEAF, c = idx - ii * LEAF;
-      double s = 0.0;
-      for (int q = c; q < LEAF; ++q) s += W[ii][q] * L[jb + q][jb + c];
-      L[base + ii][jb + c] = -s;
+    if (wave < nt) {
+      const int T = jt + 1 + wave;
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s4 = 0; s4 < LEAF; s4 += 4) {
+        const int kk = s4 + lk;
+        const double a = -W[LEAF * wave + li][kk];
+        const double b = leaf_inv(L, rdiag_all, jb, kk, li);  // X_jj[kk][li]
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) L[LEAF * T + 4 * r + lk][jb + li] = acc[r];
     }
     __syncthreads();
   }
-  for (int idx = tid; idx < CHOL_NB * CHOL_NB; idx += nt) {
+  // Dinv[a*NB + b] = X[b][a];  DinvT[a*NB + b] = X[a][b]
+  for (int idx = tid; idx < CHOL_NB * CHOL_NB; idx += DIAG_THREADS) {
     const int a = idx / CHOL_NB, b = idx - a * CHOL_NB;
-    // Dinv[m*NB + c] = X[c][m] (m = a, c = b); DinvT[c*NB + m] = X[c][m] (c = a, m = b)
-    Dinv[kb * CHOL_NB * CHOL_NB + idx] = (a < nbk && b < nbk && b >= a) ? L[b][a] : 0.0;
-    DinvT[kb * CHOL_NB * CHOL_NB + idx] = (a < nbk && b < nbk && a >= b) ? L[a][b] : 0.0;
+    double xba = 0.0, xab = 0.0;
+    if (a < nbk && b < nbk) {
+      if ((a >> 4) == (b >> 4)) {
+        const int tb = a & ~15;
+        xba = leaf_inv(L, rdiag_all, tb, b - tb, a - tb);
+        xab = leaf_inv(L, rdiag_all, tb, a - tb, b - tb);
+      } else if (b > a) {
+        xba = L[b][a];
+      } else {
+        xab = L[a][b];
+      }
+    }
+    Dinv[idx] = xba;
+    DinvT[idx] = xab;
   }
 }
-
 
 // forward block step: y_k = Dinv_k b_k ; b[r] -= sum_c L[r][k0+c] y_k[c] for r >= k0+nbk
 __global__ void __launch_bounds__(256)
     trsv_fwd_kernel(const double *L, int64_t lda, int64_t n, int64_t k0, int nbk,
-                    const double *Dinv, double *b, double *y) {
+                    const double *Dinv, double *b, double *y, const double *skip) {
+  if (skip && skip[0] != 0.0) return;
   __shared__ double z[CHOL_NB];
   const int tid = threadIdx.x;
   if (tid < CHOL_NB) {
@@ -229,7 +351,8 @@ __global__ void __launch_bounds__(256)
 // one wave per output column c'
 __global__ void __launch_bounds__(256)
     trsv_bwd_kernel(const double *L, int64_t lda, int64_t k0, int nbk,
-                    const double *DinvT, double *yv, double *x) {
+                    const double *DinvT, double *yv, double *x, const double *skip) {
+  if (skip && skip[0] != 0.0) return;
   __shared__ double z[CHOL_NB];
   const int tid = threadIdx.x;
   if (tid < CHOL_NB) {
@@ -254,9 +377,10 @@ __global__ void __launch_bounds__(256)
 
 __global__ void zero_i32(int32_t *p) { p[0] = 0; }
 
+// [Dinv blocks | DinvT blocks | two n x NB panel buffers]
 int64_t chol_dinv_bytes(int64_t n) {
   const int64_t nblk = (n + CHOL_NB - 1) / CHOL_NB;
-  return 2 * nblk * CHOL_NB * CHOL_NB * (int64_t)sizeof(double);
+  return (2 * nblk * CHOL_NB * CHOL_NB + 2 * n * CHOL_NB) * (int64_t)sizeof(double);
 }
 
 #define VMCX_BLAS2(call)                                                      \
@@ -300,72 +424,78 @@ static LookAhead *lookahead_for_device(int nev) {
 }
 
 // Right-looking blocked Cholesky with one-block look-ahead:
-//   main stream : diag_k -> trsm_k -> [wait syrk_{k-1}] -> next-column GEMM_k
-//   side stream : [wait trsm_k] -> SYRK of the trailing matrix minus the next
-//                 block column
+//   main stream : diag_k (factor + inverse) -> Pw = A_k,rest Dinv_k^T (one GEMM)
+//                 -> [wait side stream of step k-1] -> next-column GEMM_k
+//   side stream : [wait Pw] -> SYRK of the trailing matrix minus the next block
+//                 column -> copy Pw into the factor
 // so the large trailing updates overlap the latency-bound diagonal
-// factorisation and panel solve of the next block.
+// factorisation and panel of the next block.  The panel buffer alternates
+// between two copies: step k+1 writes the one the side stream of step k-1
+// read, which the main stream has waited for at step k.
 int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
                 int32_t *info, hipStream_t st) {
   rocblas_handle h = (rocblas_handle)handle;
   const int64_t nblk = (n + CHOL_NB - 1) / CHOL_NB;
   double *Dinv = dinv, *DinvT = dinv + nblk * CHOL_NB * CHOL_NB;
+  double *Pbuf[2] = {dinv + 2 * nblk * CHOL_NB * CHOL_NB,
+                     dinv + 2 * nblk * CHOL_NB * CHOL_NB + n * CHOL_NB};
+  const int ldp = (int)n;
   LookAhead *la = lookahead_for_device((int)(2 * nblk + 2));
   VMCX_CHECK_ARG(la != nullptr, "look-ahead stream / handle creation failed");
   hipLaunchKernelGGL(zero_i32, dim3(1), dim3(1), 0, st, info);
   VMCX_LAUNCHED();
-  const double one = 1.0, mone = -1.0;
+  const double one = 1.0, mone = -1.0, zero = 0.0;
   // side stream starts after everything already queued on the main stream
   VMCX_HIP(hipEventRecord(la->ev[0], st));
   VMCX_HIP(hipStreamWaitEvent(la->side, la->ev[0], 0));
-  hipEvent_t syrk_done = nullptr;  // SYRK of the previous step (side stream)
+  hipEvent_t side_done = nullptr;  // side-stream work of the previous step
   for (int64_t kb = 0; kb < nblk; ++kb) {
     const int64_t k0 = kb * CHOL_NB;
     const int nbk = (int)((n - k0 < CHOL_NB) ? n - k0 : CHOL_NB);
-    hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(1024), 0, st, A, lda, k0, nbk, info);
+    double *Dk = Dinv + kb * CHOL_NB * CHOL_NB;
+    hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(DIAG_THREADS), 0, st, A, lda, k0, nbk,
+                       info, Dk, DinvT + kb * CHOL_NB * CHOL_NB);
     VMCX_LAUNCHED();
     const int64_t rest = n - k0 - nbk;
     if (rest <= 0) break;
-    double *Lkk = A + k0 * lda + k0;
     double *P = A + k0 * lda + k0 + nbk;
-    VMCX_BLAS2(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower,
-                             rocblas_operation_transpose, rocblas_diagonal_non_unit,
-                             (rocblas_int)rest, nbk, &one, Lkk, (rocblas_int)lda, P,
-                             (rocblas_int)lda));
+    double *Pw = Pbuf[kb & 1];
+    // Pw = P L_kk^-T  (Dk = L_kk^-1, column-major NB x NB, zero above the diagonal)
+    VMCX_BLAS2(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose,
+                             (rocblas_int)rest, nbk, nbk, &one, P, (rocblas_int)lda, Dk,
+                             CHOL_NB, &zero, Pw, ldp));
     hipEvent_t panel_done = la->ev[1 + 2 * kb];
     VMCX_HIP(hipEventRecord(panel_done, st));
-    // next block column: A[k1:, k1:k1+nb1] -= P P[0:nb1]^T (main stream)
+    // next block column: A[k1:, k1:k1+nb1] -= Pw Pw[0:nb1]^T (main stream)
     const int64_t k1 = k0 + nbk;
     const int nb1 = (int)((n - k1 < CHOL_NB) ? n - k1 : CHOL_NB);
-    if (syrk_done) VMCX_HIP(hipStreamWaitEvent(st, syrk_done, 0));
+    if (side_done) VMCX_HIP(hipStreamWaitEvent(st, side_done, 0));
     VMCX_BLAS2(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose,
-                             (rocblas_int)rest, nb1, nbk, &mone, P, (rocblas_int)lda, P,
-                             (rocblas_int)lda, &one, A + k1 * lda + k1, (rocblas_int)lda));
-    // the rest of the trailing matrix on the side stream
+                             (rocblas_int)rest, nb1, nbk, &mone, Pw, ldp, Pw, ldp, &one,
+                             A + k1 * lda + k1, (rocblas_int)lda));
+    // side stream: the rest of the trailing matrix, then the panel into the factor
+    VMCX_HIP(hipStreamWaitEvent(la->side, panel_done, 0));
     const int64_t rest2 = rest - nb1;
     if (rest2 > 0) {
-      VMCX_HIP(hipStreamWaitEvent(la->side, panel_done, 0));
-      double *P2 = P + nb1;
       double *A22 = A + (k1 + nb1) * lda + k1 + nb1;
       VMCX_BLAS2(rocblas_dsyrk(la->h2, rocblas_fill_lower, rocblas_operation_none,
-                               (rocblas_int)rest2, nbk, &mone, P2, (rocblas_int)lda, &one,
-                               A22, (rocblas_int)lda));
-      syrk_done = la->ev[2 + 2 * kb];
-      VMCX_HIP(hipEventRecord(syrk_done, la->side));
-    } else {
-      syrk_done = nullptr;
+                               (rocblas_int)rest2, nbk, &mone, Pw + nb1, ldp, &one, A22,
+                               (rocblas_int)lda));
     }
+    VMCX_HIP(hipMemcpy2DAsync(P, (size_t)lda * sizeof(double), Pw, (size_t)ldp * sizeof(double),
+                              (size_t)rest * sizeof(double), (size_t)nbk,
+                              hipMemcpyDeviceToDevice, la->side));
+    side_done = la->ev[2 + 2 * kb];
+    VMCX_HIP(hipEventRecord(side_done, la->side));
   }
-  if (syrk_done) VMCX_HIP(hipStreamWaitEvent(st, syrk_done, 0));
-  hipLaunchKernelGGL(chol_diag_inverse_kernel, dim3((unsigned)nblk), dim3(256), 0, st, A, lda,
-                     n, Dinv, DinvT);
-  VMCX_LAUNCHED();
+  if (side_done) VMCX_HIP(hipStreamWaitEvent(st, side_done, 0));
   return 0;
 }
 
 // x = (L L^T)^-1 b ; tmp: n doubles; b is preserved
 int chol_solve(int64_t n, const double *L, int64_t lda, const double *dinv,
-               const double *b, double *x, double *tmp, double *tmp2, hipStream_t st) {
+               const double *b, double *x, double *tmp, double *tmp2, hipStream_t st,
+               const double *skip) {
   const int64_t nblk = (n + CHOL_NB - 1) / CHOL_NB;
   const double *Dinv = dinv, *DinvT = dinv + nblk * CHOL_NB * CHOL_NB;
   VMCX_HIP(hipMemcpyAsync(tmp, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -376,7 +506,7 @@ int chol_solve(int64_t n, const double *L, int64_t lda, const double *dinv,
     const int64_t rest = n - k0 - nbk;
     const unsigned grid = (unsigned)((rest + 255) / 256 > 0 ? (rest + 255) / 256 : 1);
     hipLaunchKernelGGL(trsv_fwd_kernel, dim3(grid), dim3(256), 0, st, L, lda, n, k0, nbk,
-                       Dinv + kb * CHOL_NB * CHOL_NB, tmp, tmp2);
+                       Dinv + kb * CHOL_NB * CHOL_NB, tmp, tmp2, skip);
     VMCX_LAUNCHED();
   }
   // backward: x = L^-T tmp2
@@ -385,7 +515,7 @@ int chol_solve(int64_t n, const double *L, int64_t lda, const double *dinv,
     const int nbk = (int)((n - k0 < CHOL_NB) ? n - k0 : CHOL_NB);
     const unsigned grid = (unsigned)((k0 + 3) / 4 > 0 ? (k0 + 3) / 4 : 1);
     hipLaunchKernelGGL(trsv_bwd_kernel, dim3(grid), dim3(256), 0, st, L, lda, k0, nbk,
-                       DinvT + kb * CHOL_NB * CHOL_NB, tmp2, x);
+                       DinvT + kb * CHOL_NB * CHOL_NB, tmp2, x, skip);
     VMCX_LAUNCHED();
   }
   return 0;

@@ -57,9 +57,10 @@ rocblas_handle blas_handle_for(hipStream_t st);
 int64_t chol_dinv_bytes(int64_t n);
 int chol_factor(void *rocblas_handle_, int64_t n, double *A, int64_t lda,
                 double *dinv, int32_t *info, hipStream_t st);
+// skip != NULL and *skip != 0 (a device flag): every launch returns at once
 int chol_solve(int64_t n, const double *L, int64_t lda, const double *dinv,
                const double *b, double *x, double *tmp, double *tmp2,
-               hipStream_t st);
+               hipStream_t st, const double *skip = nullptr);
 
 }  // namespace vmcx
 

@@ -26,6 +26,8 @@
 namespace vmcx {
 
 // ------------------------------------------------------------ reductions
+constexpr int kColsumChunks = 64;   // row chunks of the column sums
+
 struct Neu {
   double s = 0.0, c = 0.0;
   __device__ __forceinline__ void add(double x) {
@@ -36,27 +38,42 @@ struct Neu {
   __device__ __forceinline__ double value() const { return s + c; }
 };
 
-// partial[r][p] = sum over rows in chunk r of w[s] O[s][p] (w = NULL: 1)
+// partial[r][p] = sum over rows in chunk r of w[s] O[s][p] (w = NULL: 1);
+// with w2 != NULL also partial2[r][p] = sum of w2[s] O[s][p] from the same
+// read of O.  skip != NULL and *skip != 0: the launch does nothing (the
+// device-side convergence flag of the conjugate-gradient loop).
 template <typename T>
 __global__ void __launch_bounds__(256)
     colsum_partial_kernel(int64_t Ns, int64_t Np, const T *__restrict__ O, int64_t ldo,
                           int64_t rows_per_chunk, const double *__restrict__ w,
-                          double *partial) {
+                          double *partial, const double *__restrict__ w2, double *partial2,
+                          const double *skip) {
+  if (skip && skip[0] != 0.0) return;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= Np) return;
   const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
   const int64_t r1 = (r0 + rows_per_chunk < Ns) ? r0 + rows_per_chunk : Ns;
-  Neu acc;
-  for (int64_t s = r0; s < r1; ++s) {
-    const double v = (double)O[s * ldo + p];
-    acc.add(w ? v * w[s] : v);
+  Neu acc, acc2;
+  if (w2) {
+    for (int64_t s = r0; s < r1; ++s) {
+      const double v = (double)O[s * ldo + p];
+      acc.add(w ? v * w[s] : v);
+      acc2.add(v * w2[s]);
+    }
+    partial2[(int64_t)blockIdx.y * Np + p] = acc2.value();
+  } else {
+    for (int64_t s = r0; s < r1; ++s) {
+      const double v = (double)O[s * ldo + p];
+      acc.add(w ? v * w[s] : v);
+    }
   }
   partial[(int64_t)blockIdx.y * Np + p] = acc.value();
 }
 
 __global__ void __launch_bounds__(256)
     colsum_final_kernel(int64_t Np, int R, const double *__restrict__ partial,
-                        double scale, double *out) {
+                        double scale, double *out, const double *skip) {
+  if (skip && skip[0] != 0.0) return;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= Np) return;
   Neu acc;
@@ -68,7 +85,8 @@ __global__ void __launch_bounds__(256)
 template <typename T>
 __global__ void __launch_bounds__(256)
     gemv_rows_kernel(int64_t Ns, int64_t Np, const T *__restrict__ X, int64_t ldx,
-                     const double *__restrict__ u, double *v) {
+                     const double *__restrict__ u, double *v, const double *skip) {
+  if (skip && skip[0] != 0.0) return;
   __shared__ double sh[4];
   const int64_t s = blockIdx.x;
   const T *row = X + s * ldx;
@@ -244,14 +262,14 @@ static SrLayout sr_layout(int64_t Ns, int64_t Np, int o_dtype, int gram, int inp
     off += align256(bytes);
     return o;
   };
-  const int R = 64;
+  const int R = kColsumChunks;
   const size_t osz = o_dtype == 0 ? 8 : 4;
   L.X = take(inplace ? 0 : (size_t)Ns * Np * osz);
   const bool need_lo = (gram == 2) || (gram == 1 && o_dtype == 0);
   L.Xlo = take(need_lo ? (size_t)Ns * Np * (gram == 2 ? 2 : 4) : 0);
   L.T = take((size_t)Ns * Ns * 8);
   L.Tlo = take(gram ? (size_t)Ns * Ns * 4 : 0);
-  L.partial = take((size_t)R * Np * 8);
+  L.partial = take((size_t)2 * R * Np * 8);
   L.mean = take((size_t)Np * 8);
   L.eps = take((size_t)Ns * 8);
   L.alpha = take((size_t)Ns * 8);
@@ -311,6 +329,19 @@ __global__ void pcg_check(double tol, double *sc) {
   if (sc[3] <= tol * tol * sc[4]) sc[5] = 1.0;
 }
 
+// dp += a Ot^T p  (u = Ot^T p), same a as pcg_update_xr; skipped once done
+__global__ void pcg_update_d(int64_t n, double *d, const double *u, const double *sc) {
+  if (sc[5] != 0.0) return;
+  const double a = sc[0] / sc[1];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] += a * u[i];
+}
+
+// the direct solve may already meet the tolerance: no iteration then
+__global__ void pcg_check0(double tol, double *sc) {
+  if (sc[3] <= tol * tol * sc[4]) sc[5] = 1.0;
+}
+
 // p = z + (rz_new / rz) p ; rz = rz_new
 __global__ void pcg_update_p(int64_t n, double *p, const double *z, double *sc) {
   if (sc[5] != 0.0) return;
@@ -346,25 +377,35 @@ __global__ void copy_scalar(const double *src, double *dst) { dst[0] = src[0]; }
 
 template <typename T>
 static int xt_gemv_rows(int64_t Ns, int64_t Np, const T *X, int64_t ldx, const double *u,
-                        double *v, hipStream_t st) {
+                        double *v, hipStream_t st, const double *skip = nullptr) {
   hipLaunchKernelGGL(gemv_rows_kernel<T>, dim3((unsigned)Ns), dim3(256), 0, st, Ns, Np, X, ldx,
-                     u, v);
+                     u, v, skip);
   VMCX_LAUNCHED();
   return 0;
 }
 
-// out[p] = scale * sum_s X[s][p] w[s]
+// out[p] = scale * sum_s X[s][p] w[s]; with w2: out2[p] = scale * sum_s X[s][p] w2[s]
+// (partial holds 2 * kColsumChunks * Np doubles)
 template <typename T>
 static int xt_gemv_cols(int64_t Ns, int64_t Np, const T *X, int64_t ldx, const double *w,
-                        double scale, double *partial, double *out, hipStream_t st) {
-  const int R = 64;
+                        double scale, double *partial, double *out, hipStream_t st,
+                        const double *skip = nullptr, const double *w2 = nullptr,
+                        double *out2 = nullptr) {
+  const int R = kColsumChunks;
   const int64_t rows = (Ns + R - 1) / R;
-  hipLaunchKernelGGL(colsum_partial_kernel<T>, dim3((unsigned)((Np + 255) / 256), R),
-                     dim3(256), 0, st, Ns, Np, X, ldx, rows, w, partial);
+  double *partial2 = partial + (int64_t)R * Np;
+  const unsigned gx = (unsigned)((Np + 255) / 256);
+  hipLaunchKernelGGL(colsum_partial_kernel<T>, dim3(gx, R), dim3(256), 0, st, Ns, Np, X, ldx,
+                     rows, w, partial, w2, partial2, skip);
   VMCX_LAUNCHED();
-  hipLaunchKernelGGL(colsum_final_kernel, dim3((unsigned)((Np + 255) / 256)), dim3(256), 0, st,
-                     Np, R, partial, scale, out);
+  hipLaunchKernelGGL(colsum_final_kernel, dim3(gx), dim3(256), 0, st, Np, R, partial, scale, out,
+                     skip);
   VMCX_LAUNCHED();
+  if (w2) {
+    hipLaunchKernelGGL(colsum_final_kernel, dim3(gx), dim3(256), 0, st, Np, R, partial2, scale,
+                       out2, skip);
+    VMCX_LAUNCHED();
+  }
   return 0;
 }
 
@@ -467,12 +508,21 @@ static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, cons
   timer.mark();
   rc = chol_solve(Ns, T64, Ns, dinv, eps, alpha, t1, t2, st);
   if (rc) return rc;
-  // preconditioned CG on (Ot Ot^T + shift) alpha = eps, preconditioner = the factor
+  // dp = Ot^T alpha and u = Ot^T eps (= the energy gradient, for ||g||) from
+  // one read of Ot
+  rc = xt_gemv_cols<T>(Ns, Np, X, ldx, alpha, 1.0, partial, dp, st, nullptr, eps, u);
+  if (rc) return rc;
+  hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, st, Np, u, sc + 12);
+  VMCX_LAUNCHED();
+  // preconditioned CG on (Ot Ot^T + shift) alpha = eps, preconditioner = the
+  // factor.  dp = Ot^T alpha is carried along (dp += a Ot^T p), so no pass over
+  // Ot is spent on it after the loop.  Once converged (sc[5] set on the
+  // device) every remaining launch returns at once.
   const unsigned gv = (unsigned)((Ns + 255) / 256);
+  const unsigned gp = (unsigned)((Np + 255) / 256);
+  const double *done = sc + 5;
   if (max_iter > 0) {
-    rc = xt_gemv_cols<T>(Ns, Np, X, ldx, alpha, 1.0, partial, u, st);
-    if (rc) return rc;
-    rc = xt_gemv_rows<T>(Ns, Np, X, ldx, u, q, st);
+    rc = xt_gemv_rows<T>(Ns, Np, X, ldx, dp, q, st);
     if (rc) return rc;
     hipLaunchKernelGGL(residual2_kernel, dim3(gv), dim3(256), 0, st, Ns, eps, q, alpha, shift, r);
     VMCX_LAUNCHED();
@@ -481,20 +531,23 @@ static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, cons
     VMCX_HIP(hipMemcpyAsync(pv, z, Ns * 8, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, r, z, sc + 0);
     hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, eps, eps, sc + 4);
+    hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, r, r, sc + 3);
+    hipLaunchKernelGGL(pcg_check0, dim3(1), dim3(1), 0, st, tol, sc);
     VMCX_LAUNCHED();
     for (int it = 0; it < max_iter; ++it) {
-      rc = xt_gemv_cols<T>(Ns, Np, X, ldx, pv, 1.0, partial, u, st);
+      rc = xt_gemv_cols<T>(Ns, Np, X, ldx, pv, 1.0, partial, u, st, done);
       if (rc) return rc;
-      rc = xt_gemv_rows<T>(Ns, Np, X, ldx, u, t1, st);
+      rc = xt_gemv_rows<T>(Ns, Np, X, ldx, u, t1, st, done);
       if (rc) return rc;
       hipLaunchKernelGGL(addshift_kernel, dim3(gv), dim3(256), 0, st, Ns, t1, pv, shift, q);
       hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, pv, q, sc + 1);
       hipLaunchKernelGGL(pcg_update_xr, dim3(gv), dim3(256), 0, st, Ns, alpha, r, pv, q, sc);
+      hipLaunchKernelGGL(pcg_update_d, dim3(gp), dim3(256), 0, st, Np, dp, u, sc);
       hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, r, r, sc + 3);
       hipLaunchKernelGGL(pcg_check, dim3(1), dim3(1), 0, st, tol, sc);
       VMCX_LAUNCHED();
       if (it + 1 == max_iter) break;
-      rc = chol_solve(Ns, T64, Ns, dinv, r, z, t1, t2, st);
+      rc = chol_solve(Ns, T64, Ns, dinv, r, z, t1, t2, st, done);
       if (rc) return rc;
       hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, r, z, sc + 2);
       hipLaunchKernelGGL(pcg_update_p, dim3(gv), dim3(256), 0, st, Ns, pv, z, sc);
@@ -503,19 +556,13 @@ static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, cons
     }
   }
   timer.mark();
-  // dp = Ot^T alpha ; stats
-  rc = xt_gemv_cols<T>(Ns, Np, X, ldx, alpha, 1.0, partial, dp, st);
-  if (rc) return rc;
   if (stats) {
+    // explicit residual of the final alpha: eps - Ot dp - shift alpha
     rc = xt_gemv_rows<T>(Ns, Np, X, ldx, dp, q, st);
     if (rc) return rc;
     hipLaunchKernelGGL(residual2_kernel, dim3(gv), dim3(256), 0, st, Ns, eps, q, alpha, shift, r);
     hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, st, Ns, r, sc + 10);
     hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, st, Ns, eps, sc + 11);
-    VMCX_LAUNCHED();
-    rc = xt_gemv_cols<T>(Ns, Np, X, ldx, eps, 1.0, partial, u, st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, st, Np, u, sc + 12);
     VMCX_LAUNCHED();
     // stats: mean, var, ||res||/||eps||, ||g||, iterations
     hipLaunchKernelGGL(finish_stats, dim3(1), dim3(1), 0, st, sc, stats);
