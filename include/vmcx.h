@@ -159,6 +159,31 @@ int vmcx_sr_minsr(int64_t Ns, int64_t Np, void *O, int o_dtype, int64_t ldo, int
                   double *stats, int32_t *info, float *phase_ms, void *stream);
 
 /*
+ * Stochastic reconfiguration in parameter space by MINRES:
+ *   (S + diag_shift I) dp = g,  S x = O^T (O x) / Ns - (Obar . x) Obar,
+ *   g = (1/Ns) sum_s (E_s - energy_mean) O_s,  Obar = mean_s O_s,
+ * S is never formed (two passes over O per iteration).  Paige-Saunders
+ * MINRES with the convergence test |r| < rtol ||A|| ||dp|| (running
+ * estimates, as scipy.sparse.linalg.minres); at most maxiter iterations.
+ * The whole loop runs on `stream` without host synchronisation.
+ *   O: [Ns][ldo] float64 (o_dtype 0) or float32 (o_dtype 1); E: [Ns].
+ *   energy_mean: NaN = the sample mean.
+ *   workspace: >= vmcx_sr_minres_workspace_bytes(Ns, Np).
+ *   dp: [Np] float64 out; maxiter = 0 returns dp = g (the reference's
+ *   run_SR=False).  stats (may be NULL): [5] = mean E, var E, residual
+ *   estimate |phibar|, ||g||, iterations.  info (may be NULL): 0 =
+ *   converged (or g = 0), maxiter = not converged.
+ * Replaces: distributed_minres_solver_gpu + torch_minres
+ *   (GPU/optimizer.py:257-465, GPU/torch_utils.py:747-857;
+ *   DistributedSRMinresGPU.solve :846-882) on one device.
+ */
+int64_t vmcx_sr_minres_workspace_bytes(int64_t Ns, int64_t Np);
+int vmcx_sr_minres(int64_t Ns, int64_t Np, const void *O, int o_dtype, int64_t ldo,
+                   const double *E, double energy_mean, double diag_shift, double rtol,
+                   int maxiter, void *workspace, int64_t ws_bytes, double *dp, double *stats,
+                   int32_t *info, void *stream);
+
+/*
  * Neural-network backflow (NNBF) model: two hidden layers of width `hidden`.
  *   psi(x) = det((M + bf_scale * reshape(Wout act(W2 act(W1 n + b1) + b2)
  *                 + bout, (2N, n_f)))[occupied rows ascending])

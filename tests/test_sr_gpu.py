@@ -115,3 +115,75 @@ def test_sgd_update(device):
     kernels.sgd_update(p, d, 0.05)
     # one fused multiply-add per entry vs torch's multiply then subtract
     assert torch.allclose(p, want, rtol=1e-15, atol=1e-15)
+
+
+# ------------------------------------------------------------ MINRES (parameter space)
+@pytest.mark.parametrize("Ns,Np", [(1, 5), (37, 53), (200, 129), (513, 1100)])
+def test_minres_matches_oracle(device, Ns, Np):
+    """Same Lanczos recurrence in FP64: the converged solutions agree far
+    below the solver tolerance (bound 1e-8 relative at rtol 1e-10)."""
+    O, E = _problem(Ns, Np, 100 + Ns)
+    want, winfo = osr.sr_minres(O, E, E.mean(), 1e-3, rtol=1e-10, maxiter=600)
+    dp, stats, info = kernels.sr_minres(torch.tensor(O, device=device),
+                                        torch.tensor(E, device=device), None, 1e-3,
+                                        rtol=1e-10, maxiter=600)
+    assert int(info.item()) == winfo == 0
+    assert _rel(dp.cpu().numpy(), want) < 1e-8
+    s = stats.cpu().numpy()
+    assert s[0] == pytest.approx(E.mean(), rel=1e-13)
+    g = osr.energy_gradient(O, E, E.mean())
+    assert s[3] == pytest.approx(np.linalg.norm(g), rel=1e-10)
+    # the converged MINRES solution solves the dense SR system
+    dense = osr.sr_dense(O, E, E.mean(), 1e-3)
+    assert _rel(dp.cpu().numpy(), dense) < 1e-6
+
+
+def test_minres_truncated_iterates_match(device):
+    """Stopped after 4 iterations (not converged): info = maxiter and the
+    4th iterate equals the oracle's 4th iterate."""
+    O, E = _problem(300, 250, 5)
+    want, winfo = osr.sr_minres(O, E, 2.5, 1e-4, rtol=1e-14, maxiter=4)
+    dp, stats, info = kernels.sr_minres(torch.tensor(O, device=device),
+                                        torch.tensor(E, device=device), 2.5, 1e-4,
+                                        rtol=1e-14, maxiter=4)
+    assert winfo == 4 and int(info.item()) == 4
+    assert stats[4].item() == 4
+    assert _rel(dp.cpu().numpy(), want) < 1e-11
+
+
+def test_minres_fp32_o_and_gradient_only(device):
+    O, E = _problem(400, 300, 11)
+    O32 = O.astype(np.float32)
+    want, _ = osr.sr_minres(O32.astype(np.float64), E, E.mean(), 1e-2, rtol=1e-10,
+                            maxiter=400)
+    dp, _, info = kernels.sr_minres(torch.tensor(O32, device=device),
+                                    torch.tensor(E, device=device), None, 1e-2, rtol=1e-10,
+                                    maxiter=400)
+    assert int(info.item()) == 0
+    assert _rel(dp.cpu().numpy(), want) < 1e-8
+    g, _, _ = kernels.sr_minres(torch.tensor(O, device=device), torch.tensor(E, device=device),
+                                None, 1e-2, maxiter=0)
+    assert _rel(g.cpu().numpy(), osr.energy_gradient(O, E, E.mean())) < 1e-13
+
+
+def test_minres_zero_rhs(device):
+    O, _ = _problem(50, 40, 3)
+    E = np.full(50, 1.25)
+    dp, stats, info = kernels.sr_minres(torch.tensor(O, device=device),
+                                        torch.tensor(E, device=device), None, 1e-3)
+    assert int(info.item()) == 0
+    assert float(dp.abs().max()) == 0.0
+    assert stats[4].item() == 0
+
+
+def test_minres_preconditioner_interface(device):
+    from vmc_torch_amd.optimizer import DistributedSRMinresGPU
+    O, E = _problem(256, 180, 9)
+    want, _ = osr.sr_minres(O, E, E.mean(), 1e-3, rtol=5e-5, maxiter=100)
+    pre = DistributedSRMinresGPU(rtol=5e-5, maxiter=100)
+    dp, secs, info = pre.solve(local_o=torch.tensor(O, device=device),
+                               local_energies=torch.tensor(E, device=device),
+                               energy_mean=float(E.mean()), total_samples=256, n_params=180,
+                               diag_shift=1e-3, device=device, run_sr=True)
+    assert info == 0 and secs >= 0
+    assert _rel(dp.cpu().numpy(), want) < 1e-8

@@ -37,7 +37,7 @@ static inline int status_of(int rc) {
 
 extern "C" {
 
-int vmcx_abi_version(void) { return 2; }
+int vmcx_abi_version(void) { return 3; }
 
 const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
 
@@ -130,6 +130,19 @@ int vmcx_sr_minsr(int64_t Ns, int64_t Np, void *O, int o_dtype, int64_t ldo, int
                                 gram_dtype, max_iter, tol, workspace, ws_bytes, dp, stats, info,
                                 phase_ms, (hipStream_t)stream);
   return rc == 3 ? VMCX_ELIB : status_of(rc);
+}
+
+int64_t vmcx_sr_minres_workspace_bytes(int64_t Ns, int64_t Np) {
+  return vmcx::sr_minres_workspace_bytes(Ns, Np);
+}
+
+int vmcx_sr_minres(int64_t Ns, int64_t Np, const void *O, int o_dtype, int64_t ldo,
+                   const double *E, double energy_mean, double diag_shift, double rtol,
+                   int maxiter, void *workspace, int64_t ws_bytes, double *dp, double *stats,
+                   int32_t *info, void *stream) {
+  return status_of(vmcx::sr_minres(Ns, Np, O, o_dtype, ldo, E, energy_mean, diag_shift, rtol,
+                                   maxiter, workspace, ws_bytes, dp, stats, info,
+                                   (hipStream_t)stream));
 }
 
 int64_t vmcx_nnbf_workspace_bytes(const vmcx_nnbf_model *m, int64_t B, int64_t chunk,

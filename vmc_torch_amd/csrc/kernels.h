@@ -54,6 +54,10 @@ int sr_minsr(int64_t Ns, int64_t Np, void *O, int o_dtype, int64_t ldo, int flag
 int sgd_update(int64_t n, double *params, const double *dp, double lr,
                hipStream_t st);
 rocblas_handle blas_handle_for(hipStream_t st);
+int64_t sr_minres_workspace_bytes(int64_t Ns, int64_t Np);
+int sr_minres(int64_t Ns, int64_t Np, const void *O, int o_dtype, int64_t ldo, const double *E,
+              double energy_mean, double diag_shift, double rtol, int maxiter, void *ws,
+              int64_t ws_bytes, double *dp, double *stats, int32_t *info, hipStream_t st);
 int64_t chol_dinv_bytes(int64_t n);
 int chol_factor(void *rocblas_handle_, int64_t n, double *A, int64_t lda,
                 double *dinv, int32_t *info, hipStream_t st);

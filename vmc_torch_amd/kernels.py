@@ -215,6 +215,47 @@ def sr_minsr(O, E, energy_mean, diag_shift, gram="fp64", max_iter=0, tol=1e-10,
     return dp, stats, info
 
 
+class MinresWorkspace:
+    """Device workspace of vmcx_sr_minres, grown on demand and reused."""
+
+    def __init__(self):
+        self.buf = None
+
+    def get(self, Ns, Np, device):
+        need = int(lib().vmcx_sr_minres_workspace_bytes(int(Ns), int(Np)))
+        if need < 0:
+            raise ValueError("bad SR sizes")
+        if self.buf is None or self.buf.numel() < need or self.buf.device != device:
+            self.buf = None
+            self.buf = torch.empty(need, dtype=torch.uint8, device=device)
+        return self.buf
+
+
+def sr_minres(O, E, energy_mean, diag_shift, rtol=5e-5, maxiter=100, workspace=None,
+              dp=None, with_stats=True):
+    """Parameter-space SR by MINRES on the device (S never formed).
+    O: (Ns, Np) float64 or float32, row-major, not centred.  Returns
+    (dp [Np] float64, stats [5] float64 device tensor or None, info [1]
+    int32 device tensor: 0 converged, maxiter not)."""
+    if O.dtype not in (torch.float64, torch.float32):
+        raise TypeError("O must be float64 or float32")
+    require(O, O.dtype, "O")
+    if O.dim() != 2 or O.stride(1) != 1:
+        raise ValueError("O must be a row-major (Ns, Np) matrix")
+    Ns, Np = O.shape
+    od = 0 if O.dtype == torch.float64 else 1
+    require(E, F64, "local_energies", (Ns,), O.device)
+    ws = (workspace or MinresWorkspace()).get(Ns, Np, O.device)
+    dp = torch.empty(Np, dtype=F64, device=O.device) if dp is None else require(dp, F64, "dp", (Np,))
+    stats = torch.empty(5, dtype=F64, device=O.device) if with_stats else None
+    info = torch.empty(1, dtype=I32, device=O.device)
+    em = float("nan") if energy_mean is None else float(energy_mean)
+    check(lib().vmcx_sr_minres(Ns, Np, ptr(O), od, O.stride(0), ptr(E), em, float(diag_shift),
+                               float(rtol), int(maxiter), ptr(ws), ws.numel(), ptr(dp),
+                               ptr(stats), ptr(info), stream_ptr(O.device)))
+    return dp, stats, info
+
+
 def sgd_update(param, dp, lr):
     """param -= lr * dp (in place, flat view)."""
     require(param, F64, "param")

@@ -76,6 +76,44 @@ class PreconditionerGPU:
         raise NotImplementedError
 
 
+class DistributedSRMinresGPU(PreconditionerGPU):
+    """Parameter-space SR by MINRES (GPU/optimizer.py:846-882, solver
+    :257-465, torch_minres GPU/torch_utils.py:747-857), on one device.
+
+    The matvec O^T(O x)/Ns - (Obar.x) Obar + shift x runs in libvmcx with
+    FP64 accumulation; the Lanczos scalars stay on the device, so the solve
+    does not synchronise the host per iteration.  run_sr=False returns the
+    energy gradient g = <E O> - <E><O>.  use_scipy is accepted and ignored
+    (there is no host path).  Returns (dp, seconds, info) with info 0 when
+    converged and maxiter otherwise.
+    """
+
+    def __init__(self, rtol: float = 5e-5, maxiter: int = 100, use_scipy: bool = False,
+                 sync_info: bool = True):
+        self.rtol = rtol
+        self.maxiter = maxiter
+        self.use_scipy = use_scipy
+        self.sync_info = sync_info
+        self.workspace = kernels.MinresWorkspace()
+        self.last_stats = None
+
+    def solve(self, *, local_o, local_energies, energy_mean, total_samples, n_params,
+              diag_shift, device, run_sr=True):
+        t0 = time.time()
+        dt = local_o.dtype if local_o.dtype in (torch.float32, torch.float64) else torch.float64
+        O = local_o.to(device=device, dtype=dt).contiguous()
+        E = local_energies.to(device=device, dtype=torch.float64).contiguous()
+        if O.shape[0] != total_samples or O.shape[1] != n_params:
+            raise ValueError("local_o must be (total_samples, n_params) on one GPU")
+        dp, stats, info = kernels.sr_minres(O, E, energy_mean, diag_shift, self.rtol,
+                                            self.maxiter if run_sr else 0, self.workspace)
+        self.last_stats = stats
+        if not run_sr:
+            return dp, time.time() - t0, None
+        info_v = int(info.item()) if self.sync_info else info
+        return dp, time.time() - t0, info_v
+
+
 class MinSRGPU(PreconditionerGPU):
     """Sample-space SR (GPU/optimizer.py:471-614, MinSRGPU at :873-895).
 
