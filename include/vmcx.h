@@ -254,6 +254,75 @@ int vmcx_to_bf16(int64_t n, const float *x, void *y, void *stream);
 int vmcx_sgd_update(int64_t n, double *params, const double *dp, double lr,
                     void *stream);
 
+/*
+ * Z2-symmetric fermionic PEPS (fPEPS), optionally with the NN-fTNS backflow
+ * theta(x) = theta + eta * MLP(n(x)) (hidden > 0).
+ *   Site tensors T_s[p][l][u][d][r] (p = 0, 1 down, 2 up, 3 both; l/u/d/r of
+ *   dimension D, 1 on the open boundary; virtual index k has parity
+ *   k >= D/2), Z2-even: only entries with even total parity are parameters.
+ *   The flat parameter vector theta lists the allowed entries site by site
+ *   (site i = x*Ly + y) in row-major (p, l, u, d, r) order; dense_map
+ *   [sum_s 4 dl du dd dr] (device, int32) maps each dense position, in the
+ *   same order, to its theta index or -1.
+ *   Amplitude, in the Hamiltonian's Jordan-Wigner order (sites row-major,
+ *   down before up):  psi(x) = (-1)^{C(P,2)} contract_s S_s with
+ *   S_s[l,u,d,r] = T_s[x_s,l,u,d,r] (-1)^{q(d) (q(r) + P_right(s))}
+ *   (P: singly occupied sites; P_right(s): those right of s in its row), the
+ *   Grassmann tensor network with leg order (l, u, d, r).  Contracted
+ *   exactly (2 <= Lx <= 4, Ly <= 8, D <= 4: boundary bonds up to D^2).
+ *   MLP (hidden > 0): n = [up | down] occupations (2N), two hidden layers
+ *   act(W1 n + b1), act(W2 h + b2), output Wo h + bo (n_params), FP64,
+ *   row-major [out][in] weights.
+ *   O layout (vmcx_peps_logderiv): [theta (n_params) | W1 | b1 | W2 | b2 | Wo
+ *   | bo] (the MLP part only with hidden > 0).
+ * Replaces: fPEPS_Model_GPU (GPU/models/pureTNS.py:24-156) and
+ *   NNfTNS_Model_GPU._apply_backflow / _contract_tn (GPU/models/NNfTNS.py:
+ *   280-955) for exact contraction; the quimb/symmray block layout is
+ *   replaced by the dense-map layout above.
+ */
+typedef struct vmcx_peps_model {
+  int32_t Lx, Ly, D, n_params;
+  const int32_t *dense_map;
+  const double *theta;
+  int32_t hidden; /* 0 = plain fPEPS */
+  int32_t act;    /* 0 = ReLU, 1 = tanh */
+  double eta;
+  const double *W1, *b1, *W2, *b2, *Wo, *bo;
+} vmcx_peps_model;
+
+/* Number of dense tensor positions (length of dense_map); -1 on bad sizes. */
+int64_t vmcx_peps_dense_size(const vmcx_peps_model *m);
+/* Workspace bytes for B walkers, chunks of `chunk` configurations and
+ * n_edges edges (local energy); sweeps need chunk >= B. */
+int64_t vmcx_peps_workspace_bytes(const vmcx_peps_model *m, int64_t B, int64_t chunk,
+                                  int n_edges);
+/* (sign, ln|psi|) of B configurations.
+ * Replaces: fPEPS_Model_GPU.log_amplitude / vamp_log (pureTNS.py:109-142). */
+int vmcx_peps_forward(const vmcx_peps_model *m, int64_t B, const int64_t *configs,
+                      void *workspace, int64_t ws_bytes, int64_t chunk, double *sign,
+                      double *logabs, void *stream);
+/* Metropolis sweeps (proposal table, draws and acceptance as vmcx_slater_sweep),
+ * full amplitude per proposal.  edges_host: HOST [n_edges][2] int32.
+ * Replaces: MetropolisExchangeSpinfulSamplerGPU.step with an fPEPS / NN-fTNS
+ * model (GPU/sampler.py:216-388). */
+int vmcx_peps_sweep(const vmcx_peps_model *m, int64_t B, int n_edges, const int32_t *edges_host,
+                    double hopping_rate, int n_sweeps, uint64_t seed, int64_t sweep_offset,
+                    const double *u_choice, const uint8_t *bits, const double *u_acc,
+                    int64_t *configs, double *sign, double *logabs, int64_t *n_accept,
+                    void *workspace, int64_t ws_bytes, void *stream);
+/* E_loc of B walkers (walker-major connected configurations, amplitudes in
+ * chunks).  Replaces: _evaluate_energy_impl (GPU/vmc_utils.py:322-476). */
+int vmcx_peps_local_energy(const vmcx_peps_model *m, int64_t B, int n_edges,
+                           const int32_t *edges, double t, double U, const int64_t *configs,
+                           const double *sign, const double *logabs, void *workspace,
+                           int64_t ws_bytes, int64_t chunk, double *eloc, void *stream);
+/* O[B][ld] = d ln|psi| / d theta in the layout above (o_dtype 0 f64, 1 f32),
+ * with the environments of the exact contraction (and the MLP backward).
+ * Replaces: compute_grads_gpu for fPEPS / NN-fTNS (GPU/vmc_utils.py:1389-1634). */
+int vmcx_peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *configs,
+                       void *workspace, int64_t ws_bytes, int64_t chunk, void *O, int o_dtype,
+                       int64_t ld, double *sign, double *logabs, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import random_configs
+from helpers import random_configs, tie_free_draws
 from oracle import hubbard as ohub
 from oracle import nnbf as onn
 from oracle import sampler as osamp
@@ -97,37 +97,6 @@ def test_nnbf_local_energy_fp32(device, case):
     assert np.allclose(E, Eo, rtol=1e-3, atol=1e-3 * np.abs(Eo).max())
 
 
-def _tie_free_draws(logamp, x, edges, n_sw, rng, margin=1e-3):
-    """Oracle trajectory and injected draws with every accept uniform at
-    least `margin` (relative) away from its acceptance ratio."""
-    B, E = x.shape[0], len(edges)
-    uc = rng.random((n_sw, E, B))
-    bits = rng.integers(0, 4, (n_sw, E, B)).astype(np.uint8)
-    ua = rng.random((n_sw, E, B))
-    x = x.copy()
-    s, l = logamp(x)
-    for sw in range(n_sw):
-        for e, (i, j) in enumerate(edges):
-            ni, nj, flag = osamp.propose(x[:, i], x[:, j], uc[sw, e], bits[sw, e] & 1,
-                                         (bits[sw, e] >> 1) & 1)
-            if not flag.any():
-                continue
-            sel = np.nonzero(flag)[0]
-            prop = x[sel].copy()
-            prop[:, i], prop[:, j] = ni[sel], nj[sel]
-            ps, pl = logamp(prop)
-            r = np.exp(2.0 * (pl - l[sel]))
-            u = ua[sw, e, sel]
-            close = np.abs(u - r) < margin * np.maximum(r, 1e-300)
-            u[close] = 0.5 * np.minimum(r[close], 1.0)
-            ua[sw, e, sel] = u
-            acc = u < r
-            a = sel[acc]
-            x[a, i], x[a, j] = ni[a], nj[a]
-            s[a], l[a] = ps[acc], pl[acc]
-    return (uc, bits, ua), x, s, l
-
-
 @pytest.mark.parametrize("case", CASES)
 def test_nnbf_sweep_matches_oracle(device, case):
     Lx, Ly, nu, nd, hid = case
@@ -137,7 +106,7 @@ def test_nnbf_sweep_matches_oracle(device, case):
     edges = H.graph.sweep_edges()
     rng = np.random.default_rng(5)
     x0 = random_configs(rng, 48, Lx * Ly, nu, nd)
-    draws, xo, so, lo = _tie_free_draws(
+    draws, xo, so, lo = tie_free_draws(
         lambda c: onn.forward(params, c, 2, m.bf_scale)[:2], x0, edges, 2, rng)
     xd = torch.tensor(x0, device=device)
     dd = tuple(torch.tensor(a, device=device) for a in draws)

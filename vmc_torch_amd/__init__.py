@@ -6,13 +6,15 @@ libvmcx.so.  The library must be built (python -m vmc_torch_amd.build);
 nothing here falls back to the CPU.
 """
 from .hamiltonian import SquareLattice, SpinfulFermion, spinful_Fermi_Hubbard_square_lattice_torch
-from .models import WavefunctionModel_GPU, SlaterDeterminant_GPU, NNBF_GPU
+from .models import (WavefunctionModel_GPU, SlaterDeterminant_GPU, NNBF_GPU,
+                     fPEPS_Model_GPU, NNfTNS_Model_GPU)
 from .sampler import SamplerGPU, MetropolisExchangeSpinfulSamplerGPU
 from .vmc_utils import random_initial_config, initialize_walkers, evaluate_energy, compute_grads_gpu
 
 __all__ = [
     "SquareLattice", "SpinfulFermion", "spinful_Fermi_Hubbard_square_lattice_torch",
-    "WavefunctionModel_GPU", "SlaterDeterminant_GPU", "NNBF_GPU", "SamplerGPU",
+    "WavefunctionModel_GPU", "SlaterDeterminant_GPU", "NNBF_GPU", "fPEPS_Model_GPU",
+    "NNfTNS_Model_GPU", "SamplerGPU",
     "MetropolisExchangeSpinfulSamplerGPU", "random_initial_config", "initialize_walkers",
     "evaluate_energy", "compute_grads_gpu",
 ]

@@ -49,6 +49,14 @@ _SIGNATURES = {
                                     _P, _P]),
     "vmcx_nnbf_logderiv": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _I, _I64, _P, _P, _P]),
     "vmcx_to_bf16": (_I, [_I64, _P, _P, _P]),
+    "vmcx_peps_dense_size": (_I64, [_P]),
+    "vmcx_peps_workspace_bytes": (_I64, [_P, _I64, _I64, _I]),
+    "vmcx_peps_forward": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _P, _P]),
+    "vmcx_peps_sweep": (_I, [_P, _I64, _I, _P, _D, _I, _U64, _I64, _P, _P, _P, _P, _P, _P,
+                             _P, _P, _I64, _P]),
+    "vmcx_peps_local_energy": (_I, [_P, _I64, _I, _P, _D, _D, _P, _P, _P, _P, _I64, _I64,
+                                    _P, _P]),
+    "vmcx_peps_logderiv": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _I, _I64, _P, _P, _P]),
 }
 
 
@@ -60,6 +68,15 @@ class NnbfModelStruct(ctypes.Structure):
                 ("bf_scale", ctypes.c_double),
                 ("M", _P), ("W1", _P), ("b1", _P), ("W2", _P), ("b2", _P),
                 ("Wout", _P), ("bout", _P), ("W2_bf16", _P), ("Wout_bf16", _P)]
+
+
+class PepsModelStruct(ctypes.Structure):
+    """vmcx_peps_model (include/vmcx.h)."""
+    _fields_ = [("Lx", ctypes.c_int32), ("Ly", ctypes.c_int32), ("D", ctypes.c_int32),
+                ("n_params", ctypes.c_int32), ("dense_map", _P), ("theta", _P),
+                ("hidden", ctypes.c_int32), ("act", ctypes.c_int32), ("eta", ctypes.c_double),
+                ("W1", _P), ("b1", _P), ("W2", _P), ("b2", _P), ("Wo", _P), ("bo", _P)]
+
 
 _LIB = None
 

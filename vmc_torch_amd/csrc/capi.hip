@@ -37,7 +37,7 @@ static inline int status_of(int rc) {
 
 extern "C" {
 
-int vmcx_abi_version(void) { return 3; }
+int vmcx_abi_version(void) { return 4; }
 
 const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
 
@@ -193,6 +193,46 @@ int vmcx_to_bf16(int64_t n, const float *x, void *y, void *stream) {
 int vmcx_sgd_update(int64_t n, double *params, const double *dp, double lr,
                     void *stream) {
   return status_of(vmcx::sgd_update(n, params, dp, lr, (hipStream_t)stream));
+}
+
+int64_t vmcx_peps_dense_size(const vmcx_peps_model *m) { return vmcx::peps_dense_size(m); }
+
+int64_t vmcx_peps_workspace_bytes(const vmcx_peps_model *m, int64_t B, int64_t chunk,
+                                  int n_edges) {
+  return vmcx::peps_workspace_bytes(m, B, chunk, n_edges);
+}
+
+int vmcx_peps_forward(const vmcx_peps_model *m, int64_t B, const int64_t *configs,
+                      void *workspace, int64_t ws_bytes, int64_t chunk, double *sign,
+                      double *logabs, void *stream) {
+  return status3(vmcx::peps_forward(m, B, configs, workspace, ws_bytes, chunk, sign, logabs,
+                                    (hipStream_t)stream));
+}
+
+int vmcx_peps_sweep(const vmcx_peps_model *m, int64_t B, int n_edges, const int32_t *edges_host,
+                    double hopping_rate, int n_sweeps, uint64_t seed, int64_t sweep_offset,
+                    const double *u_choice, const uint8_t *bits, const double *u_acc,
+                    int64_t *configs, double *sign, double *logabs, int64_t *n_accept,
+                    void *workspace, int64_t ws_bytes, void *stream) {
+  return status3(vmcx::peps_sweep(m, B, n_edges, edges_host, hopping_rate, n_sweeps, seed,
+                                  sweep_offset, u_choice, bits, u_acc, configs, sign, logabs,
+                                  n_accept, workspace, ws_bytes, (hipStream_t)stream));
+}
+
+int vmcx_peps_local_energy(const vmcx_peps_model *m, int64_t B, int n_edges,
+                           const int32_t *edges, double t, double U, const int64_t *configs,
+                           const double *sign, const double *logabs, void *workspace,
+                           int64_t ws_bytes, int64_t chunk, double *eloc, void *stream) {
+  return status3(vmcx::peps_local_energy(m, B, n_edges, edges, t, U, configs, sign, logabs,
+                                         workspace, ws_bytes, chunk, eloc,
+                                         (hipStream_t)stream));
+}
+
+int vmcx_peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *configs,
+                       void *workspace, int64_t ws_bytes, int64_t chunk, void *O, int o_dtype,
+                       int64_t ld, double *sign, double *logabs, void *stream) {
+  return status3(vmcx::peps_logderiv(m, B, configs, workspace, ws_bytes, chunk, O, o_dtype, ld,
+                                     sign, logabs, (hipStream_t)stream));
 }
 
 }  // extern "C"

@@ -1,0 +1,948 @@
+// Z2-symmetric fermionic PEPS (fPEPS) and NN-fTNS walkers on gfx950.
+//
+// Amplitude of a configuration x (one workgroup of 256 threads per
+// configuration):
+//   psi(x) = (-1)^{C(P,2)} contract_s S_s,
+//   S_s[l,u,d,r] = T_s[x_s,l,u,d,r] (-1)^{q(d) (q(r) + P_right(s))},
+// P = number of singly occupied sites, P_right(s) = those right of s in its
+// row, q(k) = parity of virtual index k (k >= D/2; dimension-1 boundary legs
+// even).  This is the Grassmann tensor network with modes in the Hamiltonian's
+// Jordan-Wigner order (sites row-major, down before up) and leg order
+// (l, u, d, r); the local factor is what the crossings of odd bonds reduce to.
+//
+// Contraction (exact for Lx <= 4 and bond dimension <= 4, the top and bottom
+// halves being at most two rows): column by column, the top half (rows
+// 0 .. Lx/2-1) and the bottom half (rows Lx/2 .. Lx-1) of column y are
+// combined into Top_y[a][v][a'] and Bot_y[b][v][b'] (a, b: left bonds of the
+// half, v: the vertical bond between the halves), and the environment
+//   env_{y+1}[a'][b'] = sum_{a,b,v} env_y[a][b] Top_y[a][v][a'] Bot_y[b][v][b']
+// is renormalised after every column (log scale carried).  The log-derivative
+// kernel keeps every env_y, sweeps back from the right with the right
+// environment and differentiates Top/Bot, then the rows inside each half.
+//
+// NN-fTNS: theta(x) = theta + eta * MLP(n(x)), n = [up | down] occupations;
+// first layer as a gather of W1 columns, the dense layers as rocBLAS DGEMMs,
+// every tensor entry then read per configuration from theta(x).
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cmath>
+
+#include "kernels.h"
+
+namespace vmcx {
+
+namespace {
+
+constexpr int PMAX_LX = 4, PMAX_LY = 8, PMAX_D = 4, PMAX_N = PMAX_LX * PMAX_LY;
+constexpr int PB = 16;              // max bond of a half (D^2)
+constexpr int PT = 256;             // threads per configuration
+constexpr int SLICE = PMAX_D * PMAX_D * PMAX_D * PMAX_D;
+
+struct PepsGeo {
+  int Lx, Ly, D, N, tr;  // tr = rows in the top half
+};
+
+__device__ __forceinline__ int qpar(int k, int d) { return d == 1 ? 0 : (k >= (d >> 1)); }
+
+__device__ __forceinline__ void site_dims(const PepsGeo &g, int x, int y, int &dl, int &du,
+                                          int &dd, int &dr) {
+  dl = (y == 0) ? 1 : g.D;
+  du = (x == 0) ? 1 : g.D;
+  dd = (x == g.Lx - 1) ? 1 : g.D;
+  dr = (y == g.Ly - 1) ? 1 : g.D;
+}
+
+// per-configuration data shared by the workgroup
+struct CfgSh {
+  int p[PMAX_N];      // local states
+  int pr[PMAX_N];     // P_right parity
+  int doff[PMAX_N];   // dense offset of each site tensor
+  double gsign;       // (-1)^{C(P,2)}
+};
+
+__device__ void load_cfg(const PepsGeo &g, const int64_t *c, CfgSh &cs) {
+  const int tid = threadIdx.x;
+  if (tid < g.N) cs.p[tid] = (int)c[tid];
+  __syncthreads();
+  if (tid < g.N) {
+    const int x = tid / g.Ly, y = tid - x * g.Ly;
+    int pr = 0;
+    for (int yy = y + 1; yy < g.Ly; ++yy) {
+      const int v = cs.p[x * g.Ly + yy];
+      pr += (v == 1 || v == 2);
+    }
+    cs.pr[tid] = pr & 1;
+  }
+  if (tid == 0) {
+    int P = 0, off = 0;
+    for (int s = 0; s < g.N; ++s) {
+      const int v = cs.p[s];
+      P += (v == 1 || v == 2);
+      const int x = s / g.Ly, y = s - x * g.Ly;
+      int dl, du, dd, dr;
+      site_dims(g, x, y, dl, du, dd, dr);
+      cs.doff[s] = off;
+      off += 4 * dl * du * dd * dr;
+    }
+    cs.gsign = ((P * (P - 1) / 2) & 1) ? -1.0 : 1.0;
+  }
+  __syncthreads();
+}
+
+// signed slice of site (x, y) into dst[l][u][d][r] (dense dims)
+__device__ void load_slice(const PepsGeo &g, const CfgSh &cs, int x, int y,
+                           const int32_t *__restrict__ dmap, const double *__restrict__ th,
+                           double *dst) {
+  int dl, du, dd, dr;
+  site_dims(g, x, y, dl, du, dd, dr);
+  const int s = x * g.Ly + y;
+  const int n = dl * du * dd * dr;
+  const int base = cs.doff[s] + cs.p[s] * n;
+  const int pr = cs.pr[s];
+  for (int idx = threadIdx.x; idx < n; idx += PT) {
+    const int r = idx % dr, d = (idx / dr) % dd;
+    const int k = dmap[base + idx];
+    double v = k >= 0 ? th[k] : 0.0;
+    if ((qpar(d, dd) * (qpar(r, dr) + pr)) & 1) v = -v;
+    dst[idx] = v;
+  }
+}
+
+struct ColSh {
+  double sl[PMAX_LX][SLICE];        // signed slices of the column
+  double top[PB * PMAX_D * PB];     // Top[a][v][a']
+  double bot[PB * PMAX_D * PB];     // Bot[b][v][b']
+  double X[PB * PMAX_D * PB];
+  int da, da2, db, db2, dv;
+};
+
+// slices, Top and Bot of column y
+__device__ void build_column(const PepsGeo &g, const CfgSh &cs, int y,
+                             const int32_t *dmap, const double *th, ColSh &c) {
+  for (int x = 0; x < g.Lx; ++x) load_slice(g, cs, x, y, dmap, th, c.sl[x]);
+  const int D = g.D;
+  const int dlc = (y == 0) ? 1 : D, drc = (y == g.Ly - 1) ? 1 : D;
+  const int br = g.Lx - g.tr;
+  const int da = g.tr == 2 ? dlc * dlc : dlc, da2 = g.tr == 2 ? drc * drc : drc;
+  const int db = br == 2 ? dlc * dlc : dlc, db2 = br == 2 ? drc * drc : drc;
+  const int dv = D;
+  if (threadIdx.x == 0) {
+    c.da = da; c.da2 = da2; c.db = db; c.db2 = db2; c.dv = dv;
+  }
+  __syncthreads();
+  // top
+  const int ntop = da * dv * da2;
+  for (int idx = threadIdx.x; idx < ntop; idx += PT) {
+    const int a2 = idx % da2, v = (idx / da2) % dv, a = idx / (da2 * dv);
+    double acc;
+    if (g.tr == 1) {
+      // row 0: [l][u=0][d=v][r]
+      acc = c.sl[0][(a * dv + v) * drc + a2];
+    } else {
+      const int l0 = a / dlc, l1 = a % dlc, r0 = a2 / drc, r1 = a2 % drc;
+      // row 0 [l0][0][m][r0], row 1 [l1][m][v][r1]
+      acc = 0.0;
+      for (int m = 0; m < D; ++m)
+        acc += c.sl[0][(l0 * D + m) * drc + r0] * c.sl[1][((l1 * D + m) * dv + v) * drc + r1];
+    }
+    c.top[idx] = acc;
+  }
+  // bottom
+  const int nbot = db * dv * db2;
+  const int xb = g.tr;  // first bottom row
+  for (int idx = threadIdx.x; idx < nbot; idx += PT) {
+    const int b2 = idx % db2, v = (idx / db2) % dv, b = idx / (db2 * dv);
+    double acc;
+    if (br == 1) {
+      // last row [l][u=v][d=0][r]
+      acc = c.sl[xb][(b * dv + v) * drc + b2];
+    } else {
+      const int l2 = b / dlc, l3 = b % dlc, r2 = b2 / drc, r3 = b2 % drc;
+      // row xb [l2][v][m][r2], row xb+1 [l3][m][0][r3]
+      acc = 0.0;
+      for (int m = 0; m < D; ++m)
+        acc += c.sl[xb][((l2 * dv + v) * D + m) * drc + r2] *
+               c.sl[xb + 1][(l3 * D + m) * drc + r3];
+    }
+    c.bot[idx] = acc;
+  }
+  __syncthreads();
+}
+
+// env_out[a'][b'] = sum env[a][b] Top[a][v][a'] Bot[b][v][b'] (strides PB)
+__device__ void absorb_left(ColSh &c, const double *env, double *env_out) {
+  const int da = c.da, da2 = c.da2, db = c.db, db2 = c.db2, dv = c.dv;
+  const int nx = db * dv * da2;
+  for (int idx = threadIdx.x; idx < nx; idx += PT) {
+    const int a2 = idx % da2, v = (idx / da2) % dv, b = idx / (da2 * dv);
+    double acc = 0.0;
+    for (int a = 0; a < da; ++a) acc += env[a * PB + b] * c.top[(a * dv + v) * da2 + a2];
+    c.X[idx] = acc;  // X[b][v][a']
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < da2 * db2; idx += PT) {
+    const int b2 = idx % db2, a2 = idx / db2;
+    double acc = 0.0;
+    for (int b = 0; b < db; ++b)
+      for (int v = 0; v < dv; ++v)
+        acc += c.X[(b * dv + v) * da2 + a2] * c.bot[(b * dv + v) * db2 + b2];
+    env_out[a2 * PB + b2] = acc;
+  }
+  __syncthreads();
+}
+
+// R_out[a][b] = sum Top[a][v][a'] Bot[b][v][b'] R[a'][b']
+__device__ void absorb_right(ColSh &c, const double *R, double *R_out) {
+  const int da = c.da, da2 = c.da2, db = c.db, db2 = c.db2, dv = c.dv;
+  // Z[a][v][b'] = sum_a' Top[a][v][a'] R[a'][b']
+  for (int idx = threadIdx.x; idx < da * dv * db2; idx += PT) {
+    const int b2 = idx % db2, v = (idx / db2) % dv, a = idx / (db2 * dv);
+    double acc = 0.0;
+    for (int a2 = 0; a2 < da2; ++a2) acc += c.top[(a * dv + v) * da2 + a2] * R[a2 * PB + b2];
+    c.X[idx] = acc;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < da * db; idx += PT) {
+    const int b = idx % db, a = idx / db;
+    double acc = 0.0;
+    for (int v = 0; v < dv; ++v)
+      for (int b2 = 0; b2 < db2; ++b2)
+        acc += c.X[(a * dv + v) * db2 + b2] * c.bot[(b * dv + v) * db2 + b2];
+    R_out[a * PB + b] = acc;
+  }
+  __syncthreads();
+}
+
+// divide env (da x db, stride PB) by its largest magnitude; returns log of it
+__device__ double renorm(double *env, int da, int db, double *red) {
+  double m = 0.0;
+  for (int idx = threadIdx.x; idx < da * db; idx += PT) {
+    const int a = idx / db, b = idx % db;
+    m = fmax(m, fabs(env[a * PB + b]));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  double mm = 0.0;
+  for (int k = 0; k < PT / 64; ++k) mm = fmax(mm, red[k]);
+  __syncthreads();
+  if (mm > 0.0) {
+    const double inv = 1.0 / mm;
+    for (int idx = threadIdx.x; idx < da * db; idx += PT) {
+      const int a = idx / db, b = idx % db;
+      env[a * PB + b] *= inv;
+    }
+  }
+  __syncthreads();
+  return mm > 0.0 ? log(mm) : 0.0;
+}
+
+struct PepsArgs {
+  PepsGeo g;
+  const int32_t *dmap;
+  const double *theta;
+  int64_t theta_stride;   // per-configuration theta rows (0: shared)
+  int64_t n_params;
+};
+
+}  // namespace
+
+// ------------------------------------------------------------ amplitudes
+// skip (may be NULL): configuration rows with !(skip[row] & 1) are left alone
+__global__ void __launch_bounds__(PT)
+    peps_amp_kernel(PepsArgs A, int64_t rows, const int64_t *__restrict__ cfg,
+                    const int32_t *__restrict__ flag, double *sign, double *logabs) {
+  const int64_t w = blockIdx.x;
+  if (w >= rows) return;
+  if (flag && !(flag[w] & 1)) return;
+  __shared__ CfgSh cs;
+  __shared__ ColSh col;
+  __shared__ double env[2][PB * PB];
+  __shared__ double red[PT / 64];
+  const PepsGeo &g = A.g;
+  load_cfg(g, cfg + w * g.N, cs);
+  const double *th = A.theta + w * A.theta_stride;
+  if (threadIdx.x == 0) env[0][0] = 1.0;
+  __syncthreads();
+  double lsc = 0.0;
+  int cur = 0;
+  for (int y = 0; y < g.Ly; ++y) {
+    build_column(g, cs, y, A.dmap, th, col);
+    absorb_left(col, env[cur], env[cur ^ 1]);
+    cur ^= 1;
+    lsc += renorm(env[cur], col.da2, col.db2, red);
+  }
+  if (threadIdx.x == 0) {
+    const double v = env[cur][0] * cs.gsign;
+    const double lo = log(1e-45);
+    if (v == 0.0 || !std::isfinite(v)) {
+      sign[w] = 0.0;
+      logabs[w] = lo;
+    } else {
+      sign[w] = v > 0.0 ? 1.0 : -1.0;
+      logabs[w] = fmax(log(fabs(v)) + lsc, lo);
+    }
+  }
+}
+
+// ------------------------------------------------------------ log-derivatives
+// G[row][k] = d ln|psi| / d theta_k for the tensor entries (zero for entries
+// of local states other than x_s); sign / logabs of each configuration.
+__global__ void __launch_bounds__(PT)
+    peps_grad_kernel(PepsArgs A, int64_t rows, const int64_t *__restrict__ cfg,
+                     double *G, int64_t ldg, double *sign, double *logabs) {
+  const int64_t w = blockIdx.x;
+  if (w >= rows) return;
+  __shared__ CfgSh cs;
+  __shared__ ColSh col;
+  __shared__ double Lenv[PMAX_LY + 1][PB * PB];
+  __shared__ double Lsc[PMAX_LY + 1];
+  __shared__ double R[2][PB * PB];
+  __shared__ double dTop[PB * PMAX_D * PB], dBot[PB * PMAX_D * PB];
+  __shared__ double dsl[PMAX_LX][SLICE];
+  __shared__ double red[PT / 64];
+  const PepsGeo &g = A.g;
+  load_cfg(g, cfg + w * g.N, cs);
+  const double *th = A.theta + w * A.theta_stride;
+  double *grow = G + w * ldg;
+  for (int64_t k = threadIdx.x; k < A.n_params; k += PT) grow[k] = 0.0;
+  if (threadIdx.x == 0) {
+    Lenv[0][0] = 1.0;
+    Lsc[0] = 0.0;
+  }
+  __syncthreads();
+  for (int y = 0; y < g.Ly; ++y) {
+    build_column(g, cs, y, A.dmap, th, col);
+    absorb_left(col, Lenv[y], Lenv[y + 1]);
+    const double l = renorm(Lenv[y + 1], col.da2, col.db2, red);
+    if (threadIdx.x == 0) Lsc[y + 1] = Lsc[y] + l;
+    __syncthreads();
+  }
+  const double fin = Lenv[g.Ly][0];
+  const double lpsi = Lsc[g.Ly];
+  if (threadIdx.x == 0) {
+    const double v = fin * cs.gsign;
+    const double lo = log(1e-45);
+    if (v == 0.0 || !std::isfinite(v)) {
+      sign[w] = 0.0;
+      logabs[w] = lo;
+    } else {
+      sign[w] = v > 0.0 ? 1.0 : -1.0;
+      logabs[w] = fmax(log(fabs(v)) + lpsi, lo);
+    }
+  }
+  if (fin == 0.0 || !std::isfinite(fin)) return;   // zero amplitude: O row stays 0
+  // right sweep
+  if (threadIdx.x == 0) R[0][0] = 1.0;
+  __syncthreads();
+  int cur = 0;
+  double rsc = 0.0;
+  const int D = g.D;
+  const int br = g.Lx - g.tr;
+  for (int y = g.Ly - 1; y >= 0; --y) {
+    build_column(g, cs, y, A.dmap, th, col);
+    const double *Ly_ = Lenv[y];
+    const double *Rr = R[cur];
+    const int da = col.da, da2 = col.da2, db = col.db, db2 = col.db2, dv = col.dv;
+    // d psi / d Top[a][v][a'] = sum_{b,b'} L[a][b] Bot[b][v][b'] R[a'][b']
+    // X[a][v][b'] = sum_b L[a][b] Bot[b][v][b']
+    for (int idx = threadIdx.x; idx < da * dv * db2; idx += PT) {
+      const int b2 = idx % db2, v = (idx / db2) % dv, a = idx / (db2 * dv);
+      double acc = 0.0;
+      for (int b = 0; b < db; ++b) acc += Ly_[a * PB + b] * col.bot[(b * dv + v) * db2 + b2];
+      col.X[idx] = acc;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < da * dv * da2; idx += PT) {
+      const int a2 = idx % da2, v = (idx / da2) % dv, a = idx / (da2 * dv);
+      double acc = 0.0;
+      for (int b2 = 0; b2 < db2; ++b2) acc += col.X[(a * dv + v) * db2 + b2] * Rr[a2 * PB + b2];
+      dTop[idx] = acc;
+    }
+    __syncthreads();
+    // d psi / d Bot[b][v][b'] = sum_{a,a'} L[a][b] Top[a][v][a'] R[a'][b']
+    for (int idx = threadIdx.x; idx < db * dv * da2; idx += PT) {
+      const int a2 = idx % da2, v = (idx / da2) % dv, b = idx / (da2 * dv);
+      double acc = 0.0;
+      for (int a = 0; a < da; ++a) acc += Ly_[a * PB + b] * col.top[(a * dv + v) * da2 + a2];
+      col.X[idx] = acc;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < db * dv * db2; idx += PT) {
+      const int b2 = idx % db2, v = (idx / db2) % dv, b = idx / (db2 * dv);
+      double acc = 0.0;
+      for (int a2 = 0; a2 < da2; ++a2) acc += col.X[(b * dv + v) * da2 + a2] * Rr[a2 * PB + b2];
+      dBot[idx] = acc;
+    }
+    __syncthreads();
+    // rows of each half
+    const int dlc = (y == 0) ? 1 : D, drc = (y == g.Ly - 1) ? 1 : D;
+    if (g.tr == 1) {
+      for (int idx = threadIdx.x; idx < dlc * dv * drc; idx += PT) dsl[0][idx] = dTop[idx];
+    } else {
+      // dA[l0][m][r0] (row 0: [l0][0][m][r0])
+      for (int idx = threadIdx.x; idx < dlc * D * drc; idx += PT) {
+        const int r0 = idx % drc, m = (idx / drc) % D, l0 = idx / (drc * D);
+        double acc = 0.0;
+        for (int l1 = 0; l1 < dlc; ++l1)
+          for (int v = 0; v < dv; ++v)
+            for (int r1 = 0; r1 < drc; ++r1)
+              acc += dTop[((l0 * dlc + l1) * dv + v) * da2 + r0 * drc + r1] *
+                     col.sl[1][((l1 * D + m) * dv + v) * drc + r1];
+        dsl[0][idx] = acc;
+      }
+      // dB[l1][m][v][r1]
+      for (int idx = threadIdx.x; idx < dlc * D * dv * drc; idx += PT) {
+        const int r1 = idx % drc, v = (idx / drc) % dv, m = (idx / (drc * dv)) % D,
+                  l1 = idx / (drc * dv * D);
+        double acc = 0.0;
+        for (int l0 = 0; l0 < dlc; ++l0)
+          for (int r0 = 0; r0 < drc; ++r0)
+            acc += dTop[((l0 * dlc + l1) * dv + v) * da2 + r0 * drc + r1] *
+                   col.sl[0][(l0 * D + m) * drc + r0];
+        dsl[1][idx] = acc;
+      }
+    }
+    const int xb = g.tr;
+    if (br == 1) {
+      for (int idx = threadIdx.x; idx < dlc * dv * drc; idx += PT) dsl[xb][idx] = dBot[idx];
+    } else {
+      // dC[l2][v][m][r2] (row xb), dE[l3][m][0][r3] (row xb+1)
+      for (int idx = threadIdx.x; idx < dlc * dv * D * drc; idx += PT) {
+        const int r2 = idx % drc, m = (idx / drc) % D, v = (idx / (drc * D)) % dv,
+                  l2 = idx / (drc * D * dv);
+        double acc = 0.0;
+        for (int l3 = 0; l3 < dlc; ++l3)
+          for (int r3 = 0; r3 < drc; ++r3)
+            acc += dBot[((l2 * dlc + l3) * dv + v) * db2 + r2 * drc + r3] *
+                   col.sl[xb + 1][(l3 * D + m) * drc + r3];
+        dsl[xb][idx] = acc;
+      }
+      for (int idx = threadIdx.x; idx < dlc * D * drc; idx += PT) {
+        const int r3 = idx % drc, m = (idx / drc) % D, l3 = idx / (drc * D);
+        double acc = 0.0;
+        for (int l2 = 0; l2 < dlc; ++l2)
+          for (int v = 0; v < dv; ++v)
+            for (int r2 = 0; r2 < drc; ++r2)
+              acc += dBot[((l2 * dlc + l3) * dv + v) * db2 + r2 * drc + r3] *
+                     col.sl[xb][((l2 * dv + v) * D + m) * drc + r2];
+        dsl[xb + 1][idx] = acc;
+      }
+    }
+    __syncthreads();
+    // scatter: d ln|psi| / d T = d psi / d S * sign / psi
+    const double f = exp(Lsc[y] + rsc - lpsi) / fin;
+    for (int x = 0; x < g.Lx; ++x) {
+      int dl, du, dd, dr;
+      site_dims(g, x, y, dl, du, dd, dr);
+      const int s = x * g.Ly + y;
+      const int n = dl * du * dd * dr;
+      const int base = cs.doff[s] + cs.p[s] * n;
+      const int pr = cs.pr[s];
+      for (int idx = threadIdx.x; idx < n; idx += PT) {
+        const int r = idx % dr, d = (idx / dr) % dd;
+        const int k = A.dmap[base + idx];
+        if (k < 0) continue;
+        double v = dsl[x][idx] * f;
+        if ((qpar(d, dd) * (qpar(r, dr) + pr)) & 1) v = -v;
+        grow[k] = v;
+      }
+    }
+    // right environment for the next column to the left
+    absorb_right(col, Rr, R[cur ^ 1]);
+    cur ^= 1;
+    rsc += renorm(R[cur], col.da, col.db, red);
+  }
+}
+
+// ------------------------------------------------------------ NN-fTNS MLP
+namespace {
+
+__device__ __forceinline__ double act_d(double z, int act) {
+  return act == 0 ? (z > 0.0 ? z : 0.0) : tanh(z);
+}
+
+__device__ __forceinline__ double dact_d(double z, int act) {
+  if (act == 0) return z > 0.0 ? 1.0 : 0.0;
+  const double t = tanh(z);
+  return 1.0 - t * t;
+}
+
+}  // namespace
+
+// z1[row][u] = b1[u] + sum_{occupied orbitals o} W1[u][o]; h1 = act(z1)
+__global__ void __launch_bounds__(256)
+    pnn_layer1_kernel(int64_t rows, int N, int H, int act, const int64_t *__restrict__ cfg,
+                      const double *__restrict__ W1, const double *__restrict__ b1,
+                      double *z1, double *h1) {
+  const int64_t w = blockIdx.x;
+  if (w >= rows) return;
+  __shared__ int occ[2 * PMAX_N];
+  __shared__ int nocc;
+  if (threadIdx.x == 0) {
+    int k = 0;
+    const int64_t *c = cfg + w * N;
+    for (int i = 0; i < N; ++i)
+      if (c[i] & 2) occ[k++] = i;
+    for (int i = 0; i < N; ++i)
+      if (c[i] & 1) occ[k++] = N + i;
+    nocc = k;
+  }
+  __syncthreads();
+  const int n_orb = 2 * N;
+  for (int u = threadIdx.x; u < H; u += blockDim.x) {
+    double z = b1[u];
+    for (int k = 0; k < nocc; ++k) z += W1[(int64_t)u * n_orb + occ[k]];
+    z1[w * H + u] = z;
+    h1[w * H + u] = act_d(z, act);
+  }
+}
+
+// z += b ; h = act(z)
+__global__ void pnn_bias_act_kernel(int64_t n, int H, int act, double *z,
+                                    const double *__restrict__ b, double *h) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = z[i] + b[i % H];
+    z[i] = v;
+    h[i] = act_d(v, act);
+  }
+}
+
+// out[row][k] = theta[k] + eta * (out[row][k] + bo[k])
+__global__ void pnn_theta_kernel(int64_t n, int64_t P, double eta, const double *theta,
+                                 const double *__restrict__ bo, double *out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = i % P;
+    out[i] = theta[k] + eta * (out[i] + bo[k]);
+  }
+}
+
+// dz[i] = dh[i] * act'(z[i])
+__global__ void pnn_dact_kernel(int64_t n, int act, const double *z, double *dh) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dh[i] *= dact_d(z[i], act);
+}
+
+// O row: [g (P) | W1 (H x 2N) | b1 | W2 (H x H) | b2 | Wo (P x H) | bo (P)]
+// with dOut = eta g; one block per sample
+template <typename TO>
+__global__ void __launch_bounds__(256)
+    pnn_write_kernel(int64_t rows, int N, int H, int64_t P, int act, double eta,
+                     const int64_t *__restrict__ cfg, const double *__restrict__ G,
+                     const double *__restrict__ z1, const double *__restrict__ z2,
+                     const double *__restrict__ dz1, const double *__restrict__ dz2,
+                     TO *O, int64_t ld) {
+  const int64_t w = blockIdx.x;
+  if (w >= rows) return;
+  extern __shared__ double dsm[];
+  double *h1 = dsm, *h2 = h1 + H, *d1 = h2 + H, *d2 = d1 + H;
+  __shared__ float nvec[2 * PMAX_N];
+  const int n_orb = 2 * N;
+  const int64_t *c = cfg + w * N;
+  for (int o = threadIdx.x; o < n_orb; o += blockDim.x)
+    nvec[o] = (o < N) ? ((c[o] & 2) ? 1.0f : 0.0f) : ((c[o - N] & 1) ? 1.0f : 0.0f);
+  for (int u = threadIdx.x; u < H; u += blockDim.x) {
+    h1[u] = act_d(z1[w * H + u], act);
+    h2[u] = act_d(z2[w * H + u], act);
+    d1[u] = dz1[w * H + u];
+    d2[u] = dz2[w * H + u];
+  }
+  __syncthreads();
+  const double *g = G + w * P;
+  TO *row = O + w * ld;
+  for (int64_t k = threadIdx.x; k < P; k += blockDim.x) row[k] = (TO)g[k];
+  int64_t off = P;
+  for (int64_t idx = threadIdx.x; idx < (int64_t)H * n_orb; idx += blockDim.x) {
+    const int u = (int)(idx / n_orb), o = (int)(idx - (int64_t)u * n_orb);
+    row[off + idx] = (TO)(d1[u] * nvec[o]);
+  }
+  off += (int64_t)H * n_orb;
+  for (int u = threadIdx.x; u < H; u += blockDim.x) row[off + u] = (TO)d1[u];
+  off += H;
+  for (int64_t idx = threadIdx.x; idx < (int64_t)H * H; idx += blockDim.x) {
+    const int u = (int)(idx / H), i = (int)(idx - (int64_t)u * H);
+    row[off + idx] = (TO)(d2[u] * h1[i]);
+  }
+  off += (int64_t)H * H;
+  for (int u = threadIdx.x; u < H; u += blockDim.x) row[off + u] = (TO)d2[u];
+  off += H;
+  for (int64_t idx = threadIdx.x; idx < P * H; idx += blockDim.x) {
+    const int64_t k = idx / H;
+    const int u = (int)(idx - k * H);
+    row[off + idx] = (TO)(eta * g[k] * h2[u]);
+  }
+  off += P * H;
+  for (int64_t k = threadIdx.x; k < P; k += blockDim.x) row[off + k] = (TO)(eta * g[k]);
+}
+
+// FP32 O rows from the FP64 G rows (plain fPEPS)
+__global__ void peps_copy_g_kernel(int64_t rows, int64_t P, const double *__restrict__ G,
+                                   float *O, int64_t ld) {
+  const int64_t w = blockIdx.x;
+  if (w >= rows) return;
+  for (int64_t k = threadIdx.x; k < P; k += blockDim.x) O[w * ld + k] = (float)G[w * P + k];
+}
+
+// proposed configurations: copy of cfg with (i, j) <- (ni, nj) where flagged
+__global__ void peps_prop_cfg_kernel(int64_t B, int N, int i, int j,
+                                     const int64_t *__restrict__ cfg,
+                                     const int32_t *__restrict__ prop, int64_t *pcfg) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B * N) return;
+  const int64_t w = idx / N;
+  const int s = (int)(idx - w * N);
+  int64_t v = cfg[idx];
+  const int p = prop[w];
+  if (p & 1) {
+    if (s == i) v = (p >> 8) & 0xF;
+    if (s == j) v = (p >> 12) & 0xF;
+  }
+  pcfg[idx] = v;
+}
+
+// Metropolis test with |psi'|^2 / |psi|^2 and commit
+__global__ void peps_accept_kernel(int64_t B, int N, int i, int j, int64_t *cfg,
+                                   const int32_t *__restrict__ prop,
+                                   const double *__restrict__ u_acc,
+                                   const double *__restrict__ psign,
+                                   const double *__restrict__ plog, double *sign,
+                                   double *logabs, int64_t *n_accept) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= B) return;
+  const int p = prop[w];
+  if (!(p & 1)) return;
+  const double ratio = exp(2.0 * (plog[w] - logabs[w]));
+  if (!(u_acc[w] < ratio)) return;
+  cfg[w * N + i] = (p >> 8) & 0xF;
+  cfg[w * N + j] = (p >> 12) & 0xF;
+  sign[w] = psign[w];
+  logabs[w] = plog[w];
+  if (n_accept) n_accept[w] += 1;
+}
+
+// ============================================================ host side
+namespace {
+
+struct PepsWs {
+  double *theta_eff, *z1, *z2, *h1, *h2, *G, *d1, *d2;
+  int32_t *prop;
+  uint32_t *moves;
+  double *uacc, *psign, *plog;
+  int64_t *pcfg;
+  int64_t *counts, *offsets, *total, *etas;
+  double *coeffs, *csign, *clog;
+  size_t bytes;
+};
+
+size_t alb(size_t x) { return (x + 255) & ~(size_t)255; }
+
+PepsWs peps_layout(const vmcx_peps_model &m, int64_t B, int64_t chunk, int n_edges,
+                   char *base) {
+  PepsWs L{};
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    const size_t o = off;
+    off += alb(b);
+    return base ? (void *)(base + o) : nullptr;
+  };
+  const int64_t P = m.n_params, H = m.hidden > 0 ? m.hidden : 0;
+  const int N = m.Lx * m.Ly;
+  const int64_t rows = chunk > B ? chunk : B;   // the sweep evaluates B at once
+  L.theta_eff = (double *)take(H ? (size_t)rows * P * 8 : 0);
+  L.z1 = (double *)take((size_t)rows * H * 8);
+  L.z2 = (double *)take((size_t)rows * H * 8);
+  L.h1 = (double *)take((size_t)rows * H * 8);
+  L.h2 = (double *)take((size_t)rows * H * 8);
+  L.d1 = (double *)take((size_t)rows * H * 8);
+  L.d2 = (double *)take((size_t)rows * H * 8);
+  L.G = (double *)take((size_t)chunk * P * 8);
+  L.prop = (int32_t *)take((size_t)B * 4);
+  L.moves = (uint32_t *)take((size_t)B * 4);
+  L.uacc = (double *)take((size_t)B * 8);
+  L.psign = (double *)take((size_t)B * 8);
+  L.plog = (double *)take((size_t)B * 8);
+  L.pcfg = (int64_t *)take((size_t)B * N * 8);
+  const int64_t T = B * 2 * (int64_t)n_edges;
+  L.counts = (int64_t *)take((size_t)B * 8);
+  L.offsets = (int64_t *)take((size_t)B * 8);
+  L.total = (int64_t *)take(8);
+  L.etas = (int64_t *)take((size_t)T * N * 8);
+  L.coeffs = (double *)take((size_t)T * 8);
+  L.csign = (double *)take((size_t)T * 8);
+  L.clog = (double *)take((size_t)T * 8);
+  L.bytes = off;
+  return L;
+}
+
+int check_peps(const vmcx_peps_model *m) {
+  VMCX_CHECK_ARG(m != nullptr, "null model");
+  VMCX_CHECK_ARG(m->Lx >= 2 && m->Lx <= PMAX_LX, "Lx must be 2..4 (exact boundary contraction)");
+  VMCX_CHECK_ARG(m->Ly >= 1 && m->Ly <= PMAX_LY, "Ly must be 1..8");
+  VMCX_CHECK_ARG(m->D >= 1 && m->D <= PMAX_D, "D must be 1..4");
+  VMCX_CHECK_ARG(m->n_params > 0 && m->dense_map && m->theta, "missing parameters");
+  VMCX_CHECK_ARG(m->hidden >= 0 && m->hidden <= 1024, "hidden must be 0..1024");
+  VMCX_CHECK_ARG(m->act == 0 || m->act == 1, "act must be 0 (ReLU) or 1 (tanh)");
+  if (m->hidden > 0)
+    VMCX_CHECK_ARG(m->W1 && m->b1 && m->W2 && m->b2 && m->Wo && m->bo, "missing MLP weights");
+  return 0;
+}
+
+PepsArgs args_of(const vmcx_peps_model &m, const double *theta, int64_t stride) {
+  PepsArgs A;
+  A.g.Lx = m.Lx;
+  A.g.Ly = m.Ly;
+  A.g.D = m.D;
+  A.g.N = m.Lx * m.Ly;
+  A.g.tr = m.Lx / 2;
+  A.dmap = m.dense_map;
+  A.theta = theta;
+  A.theta_stride = stride;
+  A.n_params = m.n_params;
+  return A;
+}
+
+#define PEPS_BLAS(call)                                                       \
+  do {                                                                       \
+    rocblas_status s_ = (call);                                              \
+    if (s_ != rocblas_status_success) {                                      \
+      ::vmcx::set_error(std::string(__func__) + ": " #call " -> " +          \
+                        rocblas_status_to_string(s_));                       \
+      return 3;                                                              \
+    }                                                                        \
+  } while (0)
+
+// theta(x) rows for `rows` configurations (NN-fTNS); keeps z1 / z2
+int mlp_theta(const vmcx_peps_model &m, const PepsWs &L, int64_t rows, const int64_t *cfg,
+              hipStream_t st) {
+  const int H = m.hidden, N = m.Lx * m.Ly;
+  const int64_t P = m.n_params;
+  rocblas_handle h = blas_handle_for(st);
+  VMCX_CHECK_ARG(h != nullptr, "rocblas handle");
+  hipLaunchKernelGGL(pnn_layer1_kernel, dim3((unsigned)rows), dim3(256), 0, st, rows, N, H,
+                     m.act, cfg, m.W1, m.b1, L.z1, L.h1);
+  VMCX_LAUNCHED();
+  const double one = 1.0, zero = 0.0;
+  // column-major: z2 (H x rows) = W2^T(col-major view of row-major W2) h1
+  PEPS_BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, H,
+                          (rocblas_int)rows, H, &one, m.W2, H, L.h1, H, &zero, L.z2, H));
+  const int64_t nh = rows * H;
+  const unsigned gh = (unsigned)((nh + 255) / 256 < 8192 ? (nh + 255) / 256 : 8192);
+  hipLaunchKernelGGL(pnn_bias_act_kernel, dim3(gh), dim3(256), 0, st, nh, H, m.act, L.z2, m.b2,
+                     L.h2);
+  VMCX_LAUNCHED();
+  PEPS_BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none,
+                          (rocblas_int)P, (rocblas_int)rows, H, &one, m.Wo, H, L.h2, H, &zero,
+                          L.theta_eff, (rocblas_int)P));
+  const int64_t nt = rows * P;
+  const unsigned gt = (unsigned)((nt + 255) / 256 < 16384 ? (nt + 255) / 256 : 16384);
+  hipLaunchKernelGGL(pnn_theta_kernel, dim3(gt), dim3(256), 0, st, nt, P, m.eta, m.theta, m.bo,
+                     L.theta_eff);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+// amplitudes of `rows` configurations (rows <= chunk for NN-fTNS)
+int amp_rows(const vmcx_peps_model &m, const PepsWs &L, int64_t rows, const int64_t *cfg,
+             const int32_t *flag, double *sign, double *logabs, hipStream_t st) {
+  if (rows == 0) return 0;
+  PepsArgs A = args_of(m, m.theta, 0);
+  if (m.hidden > 0) {
+    const int rc = mlp_theta(m, L, rows, cfg, st);
+    if (rc) return rc;
+    A = args_of(m, L.theta_eff, m.n_params);
+  }
+  hipLaunchKernelGGL(peps_amp_kernel, dim3((unsigned)rows), dim3(PT), 0, st, A, rows, cfg, flag,
+                     sign, logabs);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+}  // namespace
+
+int64_t peps_dense_size(const vmcx_peps_model *m) {
+  if (!m || m->Lx < 1 || m->Ly < 1 || m->D < 1) return -1;
+  int64_t n = 0;
+  for (int x = 0; x < m->Lx; ++x)
+    for (int y = 0; y < m->Ly; ++y) {
+      const int dl = (y == 0) ? 1 : m->D, du = (x == 0) ? 1 : m->D;
+      const int dd = (x == m->Lx - 1) ? 1 : m->D, dr = (y == m->Ly - 1) ? 1 : m->D;
+      n += 4 * dl * du * dd * dr;
+    }
+  return n;
+}
+
+int64_t peps_workspace_bytes(const vmcx_peps_model *m, int64_t B, int64_t chunk, int n_edges) {
+  if (check_peps(m) || B < 0 || chunk < 1 || n_edges < 0) return -1;
+  return (int64_t)peps_layout(*m, B, chunk, n_edges, nullptr).bytes;
+}
+
+int peps_forward(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void *ws,
+                 int64_t ws_bytes, int64_t chunk, double *sign, double *logabs,
+                 hipStream_t st) {
+  if (check_peps(m)) return 1;
+  VMCX_CHECK_ARG(B >= 0 && chunk >= 1, "bad sizes");
+  if (B == 0) return 0;
+  VMCX_CHECK_ARG(cfg && sign && logabs && ws, "null pointer");
+  const PepsWs L = peps_layout(*m, B, chunk, 0, (char *)ws);
+  VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
+  const int N = m->Lx * m->Ly;
+  for (int64_t r0 = 0; r0 < B; r0 += chunk) {
+    const int64_t rows = (B - r0 < chunk) ? B - r0 : chunk;
+    const int rc = amp_rows(*m, L, rows, cfg + r0 * N, nullptr, sign + r0, logabs + r0, st);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int peps_sweep(const vmcx_peps_model *m, int64_t B, int n_edges, const int32_t *edges_host,
+               double hopping_rate, int n_sweeps, uint64_t seed, int64_t sweep_offset,
+               const double *u_choice, const uint8_t *bits, const double *u_acc, int64_t *cfg,
+               double *sign, double *logabs, int64_t *n_accept, void *ws, int64_t ws_bytes,
+               hipStream_t st) {
+  if (check_peps(m)) return 1;
+  VMCX_CHECK_ARG(B >= 0 && n_edges >= 0 && n_sweeps >= 0, "negative size");
+  VMCX_CHECK_ARG((u_choice == nullptr) == (u_acc == nullptr) &&
+                     (u_choice == nullptr) == (bits == nullptr),
+                 "injected draws need u_choice, bits and u_acc together");
+  if (B == 0 || n_sweeps == 0 || n_edges == 0) return 0;
+  VMCX_CHECK_ARG(cfg && sign && logabs && ws && edges_host, "null pointer");
+  const PepsWs L = peps_layout(*m, B, B, 0, (char *)ws);
+  VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small (chunk must be >= B)");
+  const int N = m->Lx * m->Ly;
+  for (int e = 0; e < n_edges; ++e) {
+    const int i = edges_host[2 * e], j = edges_host[2 * e + 1];
+    VMCX_CHECK_ARG(i >= 0 && i < N && j >= 0 && j < N && i != j, "edge out of range");
+  }
+  const unsigned gw = (unsigned)((B + 255) / 256);
+  const unsigned gc = (unsigned)((B * N + 255) / 256);
+  for (int sw = 0; sw < n_sweeps; ++sw) {
+    for (int e = 0; e < n_edges; ++e) {
+      const int i = edges_host[2 * e], j = edges_host[2 * e + 1];
+      const size_t di = ((size_t)sw * n_edges + e) * B;
+      hipLaunchKernelGGL(nnbf_propose_kernel, dim3(gw), dim3(256), 0, st, B, N, i, j, e,
+                         sweep_offset + sw, seed, 1.0 - hopping_rate,
+                         u_choice ? u_choice + di : nullptr, bits ? bits + di : nullptr,
+                         u_acc ? u_acc + di : nullptr, cfg, L.prop, L.moves, L.uacc);
+      hipLaunchKernelGGL(peps_prop_cfg_kernel, dim3(gc), dim3(256), 0, st, B, N, i, j, cfg,
+                         L.prop, L.pcfg);
+      VMCX_LAUNCHED();
+      int rc = amp_rows(*m, L, B, L.pcfg, L.prop, L.psign, L.plog, st);
+      if (rc) return rc;
+      hipLaunchKernelGGL(peps_accept_kernel, dim3(gw), dim3(256), 0, st, B, N, i, j, cfg,
+                         L.prop, L.uacc, L.psign, L.plog, sign, logabs, n_accept);
+      VMCX_LAUNCHED();
+    }
+  }
+  return 0;
+}
+
+int peps_local_energy(const vmcx_peps_model *m, int64_t B, int n_edges, const int32_t *edges,
+                      double t, double U, const int64_t *cfg, const double *sign,
+                      const double *logabs, void *ws, int64_t ws_bytes, int64_t chunk,
+                      double *eloc, hipStream_t st) {
+  if (check_peps(m)) return 1;
+  VMCX_CHECK_ARG(B >= 0 && n_edges >= 0 && chunk >= 1, "bad sizes");
+  if (B == 0) return 0;
+  VMCX_CHECK_ARG(cfg && sign && logabs && ws && eloc && (n_edges == 0 || edges),
+                 "null pointer");
+  const PepsWs L = peps_layout(*m, B, chunk, n_edges, (char *)ws);
+  VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
+  const int N = m->Lx * m->Ly;
+  const unsigned gw = (unsigned)((B + 255) / 256);
+  hipLaunchKernelGGL(conn_count_walker_kernel, dim3(gw), dim3(256), 0, st, B, N, n_edges,
+                     edges, cfg, L.counts);
+  hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, st, B, L.counts, L.offsets,
+                     L.total);
+  hipLaunchKernelGGL(conn_fill_walker_kernel, dim3(gw), dim3(256), 0, st, B, N, n_edges, edges,
+                     t, cfg, L.offsets, L.etas, L.coeffs);
+  VMCX_LAUNCHED();
+  int64_t T = 0;
+  VMCX_HIP(hipMemcpyAsync(&T, L.total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  VMCX_HIP(hipStreamSynchronize(st));
+  for (int64_t r0 = 0; r0 < T; r0 += chunk) {
+    const int64_t rows = (T - r0 < chunk) ? T - r0 : chunk;
+    const int rc = amp_rows(*m, L, rows, L.etas + r0 * N, nullptr, L.csign + r0, L.clog + r0,
+                            st);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(eloc_segment_kernel, dim3(gw), dim3(256), 0, st, B, N, U, cfg, L.offsets,
+                     L.counts, L.coeffs, L.csign, L.clog, sign, logabs, eloc);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+int peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void *ws,
+                  int64_t ws_bytes, int64_t chunk, void *O, int o_dtype, int64_t ld,
+                  double *sign, double *logabs, hipStream_t st) {
+  if (check_peps(m)) return 1;
+  VMCX_CHECK_ARG(B >= 0 && chunk >= 1, "bad sizes");
+  VMCX_CHECK_ARG(o_dtype == 0 || o_dtype == 1, "o_dtype must be 0 (f64) or 1 (f32)");
+  if (B == 0) return 0;
+  VMCX_CHECK_ARG(cfg && O && sign && logabs && ws, "null pointer");
+  const int64_t P = m->n_params, H = m->hidden;
+  const int N = m->Lx * m->Ly;
+  const int64_t np_total =
+      H > 0 ? P + H * 2 * N + H + H * H + H + P * H + P : P;
+  VMCX_CHECK_ARG(ld >= np_total, "ld smaller than the parameter count");
+  const PepsWs L = peps_layout(*m, B, chunk, 0, (char *)ws);
+  VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
+  rocblas_handle h = blas_handle_for(st);
+  VMCX_CHECK_ARG(h != nullptr, "rocblas handle");
+  for (int64_t r0 = 0; r0 < B; r0 += chunk) {
+    const int64_t rows = (B - r0 < chunk) ? B - r0 : chunk;
+    const int64_t *c = cfg + r0 * N;
+    PepsArgs A = args_of(*m, m->theta, 0);
+    if (H > 0) {
+      const int rc = mlp_theta(*m, L, rows, c, st);
+      if (rc) return rc;
+      A = args_of(*m, L.theta_eff, P);
+    }
+    // FP64 G (the upstream of the MLP backward); for plain fPEPS written
+    // straight into an FP64 O
+    const bool direct = (H == 0 && o_dtype == 0);
+    double *G = direct ? (double *)O + r0 * ld : L.G;
+    const int64_t ldg = direct ? ld : P;
+    hipLaunchKernelGGL(peps_grad_kernel, dim3((unsigned)rows), dim3(PT), 0, st, A, rows, c, G,
+                       ldg, sign + r0, logabs + r0);
+    VMCX_LAUNCHED();
+    if (direct) continue;
+    if (H == 0) {
+      // FP32 O: plain copy of G
+      hipLaunchKernelGGL(peps_copy_g_kernel, dim3((unsigned)rows), dim3(256), 0, st, rows, P, G,
+                         (float *)O + r0 * ld, ld);
+      VMCX_LAUNCHED();
+      continue;
+    }
+    // dh2 = (eta G) Wo ; dz2 = dh2 act'(z2) ; dh1 = dz2 W2 ; dz1 = dh1 act'(z1)
+    const double etav = m->eta, one = 1.0, zero = 0.0;
+    PEPS_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)H,
+                            (rocblas_int)rows, (rocblas_int)P, &etav, m->Wo, (rocblas_int)H,
+                            G, (rocblas_int)P, &zero, L.d2, (rocblas_int)H));
+    const int64_t nh = rows * H;
+    const unsigned gh = (unsigned)((nh + 255) / 256 < 8192 ? (nh + 255) / 256 : 8192);
+    hipLaunchKernelGGL(pnn_dact_kernel, dim3(gh), dim3(256), 0, st, nh, m->act, L.z2, L.d2);
+    VMCX_LAUNCHED();
+    PEPS_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)H,
+                            (rocblas_int)rows, (rocblas_int)H, &one, m->W2, (rocblas_int)H,
+                            L.d2, (rocblas_int)H, &zero, L.d1, (rocblas_int)H));
+    hipLaunchKernelGGL(pnn_dact_kernel, dim3(gh), dim3(256), 0, st, nh, m->act, L.z1, L.d1);
+    VMCX_LAUNCHED();
+    const size_t sm = (size_t)4 * H * 8;
+    if (o_dtype == 0)
+      hipLaunchKernelGGL(pnn_write_kernel<double>, dim3((unsigned)rows), dim3(256), sm, st, rows,
+                         N, (int)H, P, m->act, m->eta, c, G, L.z1, L.z2, L.d1, L.d2,
+                         (double *)O + r0 * ld, ld);
+    else
+      hipLaunchKernelGGL(pnn_write_kernel<float>, dim3((unsigned)rows), dim3(256), sm, st, rows,
+                         N, (int)H, P, m->act, m->eta, c, G, L.z1, L.z2, L.d1, L.d2,
+                         (float *)O + r0 * ld, ld);
+    VMCX_LAUNCHED();
+  }
+  return 0;
+}
+
+}  // namespace vmcx

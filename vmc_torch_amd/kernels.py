@@ -297,3 +297,65 @@ def nnbf_workspace(struct, B, chunk, n_edges, device, holder):
 def to_bf16(x, out):
     check(lib().vmcx_to_bf16(x.numel(), ptr(x), ptr(out), stream_ptr(x.device)))
     return out
+
+
+# ------------------------------------------------------------------ fPEPS
+def peps_layout(Lx, Ly, D):
+    """(dense_map int32 numpy [n_dense], per-site allowed sizes): the flat
+    theta layout lists the Z2-even entries of each site tensor
+    T_s[p, l, u, d, r] (site-major, row-major) -- virtual index k of a bond
+    of dimension D has parity k >= D // 2, boundary legs have dimension 1."""
+    import numpy as np
+    pp = np.array([0, 1, 1, 0])
+
+    def bpar(d):
+        return np.zeros(1, dtype=np.int64) if d == 1 else (np.arange(d) >= d // 2).astype(np.int64)
+    maps, sizes, k = [], [], 0
+    for i in range(Lx * Ly):
+        x, y = divmod(i, Ly)
+        dl, du = (1 if y == 0 else D), (1 if x == 0 else D)
+        dd, dr = (1 if x == Lx - 1 else D), (1 if y == Ly - 1 else D)
+        par = (pp[:, None, None, None, None] + bpar(dl)[None, :, None, None, None]
+               + bpar(du)[None, None, :, None, None] + bpar(dd)[None, None, None, :, None]
+               + bpar(dr)[None, None, None, None, :]) % 2
+        ok = (par == 0).ravel()
+        m = np.full(ok.size, -1, dtype=np.int32)
+        m[ok] = np.arange(k, k + int(ok.sum()), dtype=np.int32)
+        k += int(ok.sum())
+        maps.append(m)
+        sizes.append(int(ok.sum()))
+    return np.concatenate(maps), sizes
+
+
+def peps_struct(Lx, Ly, D, dense_map, theta, nn=None, act=0, eta=0.0):
+    """vmcx_peps_model from device tensors: dense_map int32, theta float64,
+    nn = dict W1, b1, W2, b2, Wo, bo (float64) or None."""
+    s = _lib.PepsModelStruct()
+    require(dense_map, I32, "dense_map")
+    require(theta, F64, "theta")
+    s.Lx, s.Ly, s.D, s.n_params = int(Lx), int(Ly), int(D), int(theta.numel())
+    s.dense_map, s.theta = dense_map.data_ptr(), theta.data_ptr()
+    s.hidden = 0 if nn is None else int(nn["b1"].numel())
+    s.act, s.eta = int(act), float(eta)
+    for k in ("W1", "b1", "W2", "b2", "Wo", "bo"):
+        if nn is None:
+            setattr(s, k, None)
+        else:
+            require(nn[k], F64, k)
+            setattr(s, k, nn[k].data_ptr())
+    return s
+
+
+def peps_workspace(struct, B, chunk, n_edges, device, holder):
+    import ctypes
+    need = int(lib().vmcx_peps_workspace_bytes(ctypes.byref(struct), int(B), int(chunk),
+                                                int(n_edges)))
+    if need < 0:
+        raise ValueError("bad fPEPS workspace sizes / model")
+    buf = holder.get("ws")
+    if buf is None or buf.numel() < need or buf.device != device:
+        buf = None
+        holder.pop("ws", None)
+        buf = torch.empty(need, dtype=torch.uint8, device=device)
+        holder["ws"] = buf
+    return buf

@@ -278,3 +278,190 @@ class NNBF_GPU(WavefunctionModel_GPU):
             chunk, kernels.ptr(O), code, O.stride(0), kernels.ptr(sign), kernels.ptr(logabs),
             kernels.stream_ptr(fxs.device)))
         return O, (sign, logabs)
+
+
+class fPEPS_Model_GPU(WavefunctionModel_GPU):
+    """Z2-symmetric fermionic PEPS with exact boundary contraction
+    (GPU/models/pureTNS.py:24-156, chi <= 0 / chi >= D^2 on Lx <= 4).
+
+    Parameters: one flat FP64 vector of the Z2-even entries of the site
+    tensors T_s[p, l, u, d, r] (site-major, row-major), the layout of
+    kernels.peps_layout.  Fermionic convention: modes in the Hamiltonian's
+    Jordan-Wigner order, psi(x) = (-1)^{C(P,2)} contract_s S_s with
+    S_s = T_s[x_s] (-1)^{q(d)(q(r) + P_right(s))} (see include/vmcx.h).
+    theta: initial vector (numpy or tensor); default: entries N(0, 1) from
+    `generator`, each site tensor scaled to unit norm.
+    """
+
+    def __init__(self, Lx, Ly, D, theta=None, generator=None, dtype=torch.float64,
+                 max_bond=-1, chunk=65536, _nn=None, act="relu", eta=0.0):
+        if dtype != torch.float64:
+            raise ValueError("the fPEPS kernels are FP64")
+        if not (2 <= Lx <= 4 and 1 <= Ly <= 8 and 1 <= D <= 4):
+            raise NotImplementedError("exact contraction needs 2 <= Lx <= 4, Ly <= 8, D <= 4")
+        dmap, sizes = kernels.peps_layout(Lx, Ly, D)
+        n = int(sum(sizes))
+        if theta is None:
+            th = torch.randn(n, dtype=torch.float64, generator=generator)
+            off = 0
+            for k in sizes:
+                th[off:off + k] /= th[off:off + k].norm()
+                off += k
+        else:
+            th = torch.as_tensor(theta, dtype=torch.float64).clone().reshape(-1)
+            if th.numel() != n:
+                raise ValueError(f"theta must have {n} entries")
+        super().__init__([th] + ([] if _nn is None else list(_nn)))
+        self.register_buffer("dense_map", torch.as_tensor(dmap, dtype=torch.int32))
+        self.Lx, self.Ly, self.D, self.n_sites = Lx, Ly, D, Lx * Ly
+        self.n_tn_params = n
+        self.site_sizes = sizes
+        self.chi = max_bond
+        self.chunk = chunk
+        self.act_name = act
+        self.eta = float(eta)
+        self._struct_obj = None
+        self._ws = {}
+
+    # ------------------------------------------------------ device struct
+    def _nn_dict(self):
+        return None
+
+    def _sync(self):
+        changed = self.params_changed()
+        if changed or self._struct_obj is None:
+            for p in self.params:
+                if not p.data.is_contiguous():
+                    p.data = p.data.contiguous()
+            self._struct_obj = kernels.peps_struct(
+                self.Lx, self.Ly, self.D, self.dense_map, self.params[0].data,
+                self._nn_dict(), 0 if self.act_name == "relu" else 1, self.eta)
+            self._state_key = None
+            return True
+        return False
+
+    def _ws_for(self, B, chunk, n_edges, device):
+        return kernels.peps_workspace(self._struct_obj, B, chunk, n_edges, device, self._ws)
+
+    def forward_log(self, x):
+        x = x.contiguous()
+        kernels.require(x, torch.int64, "configs")
+        self._sync()
+        B = x.shape[0]
+        sign = torch.empty(B, dtype=torch.float64, device=x.device)
+        logabs = torch.empty(B, dtype=torch.float64, device=x.device)
+        chunk = min(self.chunk, max(B, 1))
+        ws = self._ws_for(B, chunk, 0, x.device)
+        kernels.check(kernels.lib().vmcx_peps_forward(
+            ctypes.byref(self._struct_obj), B, kernels.ptr(x), kernels.ptr(ws), ws.numel(), chunk,
+            kernels.ptr(sign), kernels.ptr(logabs), kernels.stream_ptr(x.device)))
+        return sign, logabs
+
+    # --------------------------------------------------------- walker state
+    def walker_state(self, fxs, refresh=False):
+        changed = self._sync()
+        if not refresh and not changed and self._state is not None and \
+                self._state_key == self._walker_key(fxs):
+            return self._state
+        B = fxs.shape[0]
+        sign, logabs = self.forward_log(fxs)
+        st = {"sign": sign, "logabs": logabs,
+              "n_accept": (self._state["n_accept"] if self._state is not None and
+                           self._state["n_accept"].shape[0] == B
+                           else torch.zeros(B, dtype=torch.int64, device=fxs.device))}
+        self._state, self._state_key = st, self._walker_key(fxs)
+        return st
+
+    def mcmc_sweep(self, fxs, edges, n_sweeps, seed, sweep_offset, hopping_rate=0.25,
+                   refresh_every=64, draws=None):
+        st = self.walker_state(fxs)
+        B = fxs.shape[0]
+        edges_host = edges.detach().to("cpu", torch.int32).contiguous()
+        uc = bits = ua = None
+        if draws is not None:
+            uc, bits, ua = draws
+        ws = self._ws_for(B, B, 0, fxs.device)
+        kernels.check(kernels.lib().vmcx_peps_sweep(
+            ctypes.byref(self._struct_obj), B, edges_host.shape[0],
+            ctypes.c_void_p(edges_host.data_ptr()), float(hopping_rate), int(n_sweeps),
+            int(seed) & (2 ** 64 - 1), int(sweep_offset), kernels.ptr(uc), kernels.ptr(bits),
+            kernels.ptr(ua), kernels.ptr(fxs), kernels.ptr(st["sign"]),
+            kernels.ptr(st["logabs"]), kernels.ptr(st["n_accept"]), kernels.ptr(ws), ws.numel(),
+            kernels.stream_ptr(fxs.device)))
+        return st["sign"], st["logabs"], st["n_accept"]
+
+    def local_energy(self, fxs, H):
+        st = self.walker_state(fxs)
+        B = fxs.shape[0]
+        edges = H.graph.edge_tensor(fxs.device, "graph")
+        chunk = self.chunk
+        ws = self._ws_for(B, chunk, edges.shape[0], fxs.device)
+        eloc = torch.empty(B, dtype=torch.float64, device=fxs.device)
+        kernels.check(kernels.lib().vmcx_peps_local_energy(
+            ctypes.byref(self._struct_obj), B, edges.shape[0], kernels.ptr(edges), float(H.t),
+            float(H.U), kernels.ptr(fxs), kernels.ptr(st["sign"]), kernels.ptr(st["logabs"]),
+            kernels.ptr(ws), ws.numel(), chunk, kernels.ptr(eloc),
+            kernels.stream_ptr(fxs.device)))
+        return eloc
+
+    def log_derivative(self, fxs, dtype=torch.float64, out=None, chunk=4096):
+        self._sync()
+        B = fxs.shape[0]
+        Np = self.n_params
+        O = torch.empty(B, Np, dtype=dtype, device=fxs.device) if out is None else out
+        sign = torch.empty(B, dtype=torch.float64, device=fxs.device)
+        logabs = torch.empty(B, dtype=torch.float64, device=fxs.device)
+        chunk = min(chunk, max(B, 1))
+        ws = self._ws_for(B, chunk, 0, fxs.device)
+        code = {torch.float64: 0, torch.float32: 1}[dtype]
+        kernels.check(kernels.lib().vmcx_peps_logderiv(
+            ctypes.byref(self._struct_obj), B, kernels.ptr(fxs), kernels.ptr(ws), ws.numel(),
+            chunk, kernels.ptr(O), code, O.stride(0), kernels.ptr(sign), kernels.ptr(logabs),
+            kernels.stream_ptr(fxs.device)))
+        return O, (sign, logabs)
+
+
+class NNfTNS_Model_GPU(fPEPS_Model_GPU):
+    """NN-fTNS (GPU/models/NNfTNS.py:280-955): the fPEPS above with the
+    additive backflow theta(x) = theta + nn_eta * MLP(n(x)) on every tensor
+    entry (_apply_backflow, NNfTNS.py:390-437).  MLP: n = [up | down]
+    occupations (2N) -> hidden_dim -> hidden_dim -> n_tn_params, ReLU or
+    tanh, FP64.  Parameter order [theta, W1, b1, W2, b2, Wo, bo].
+    init: Xavier-uniform hidden layers, output layer N(0, init_scale^2),
+    zero biases (the model starts near the plain fPEPS)."""
+
+    def __init__(self, Lx, Ly, D, hidden_dim=128, nn_eta=1.0, activation="relu",
+                 init_scale=1e-2, theta=None, generator=None, dtype=torch.float64, max_bond=-1,
+                 chunk=65536):
+        if activation not in ("relu", "tanh"):
+            raise ValueError("activation must be relu or tanh")
+        _, sizes = kernels.peps_layout(Lx, Ly, D)
+        n = int(sum(sizes))
+        N = Lx * Ly
+        g = generator
+
+        def xav(fo, fi):
+            a = (6.0 / (fi + fo)) ** 0.5
+            return torch.empty(fo, fi, dtype=torch.float64).uniform_(-a, a, generator=g)
+        th = None
+        if theta is None:
+            th = torch.randn(n, dtype=torch.float64, generator=g)
+            off = 0
+            for k in sizes:
+                th[off:off + k] /= th[off:off + k].norm()
+                off += k
+        else:
+            th = theta
+        W1, W2 = xav(hidden_dim, 2 * N), xav(hidden_dim, hidden_dim)
+        Wo = torch.randn(n, hidden_dim, dtype=torch.float64, generator=g) * init_scale
+        nnp = [W1, torch.zeros(hidden_dim, dtype=torch.float64), W2,
+               torch.zeros(hidden_dim, dtype=torch.float64), Wo,
+               torch.zeros(n, dtype=torch.float64)]
+        super().__init__(Lx, Ly, D, theta=th, generator=g, dtype=dtype, max_bond=max_bond,
+                         chunk=chunk, _nn=nnp, act=activation, eta=nn_eta)
+        self.hidden = hidden_dim
+        self.nn_eta = float(nn_eta)
+
+    def _nn_dict(self):
+        names = ["W1", "b1", "W2", "b2", "Wo", "bo"]
+        return {k: p.data for k, p in zip(names, self.params[1:])}
