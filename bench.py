@@ -52,6 +52,16 @@ WORKLOADS = {
         gram="bf16", hidden=256, bf_scale=0.01, mlp="bf16",
         desc=("6x6 OBC Hubbard t=1 U=8 16u+16d, NNBF 72-256-256-2304 ReLU, bf_scale 0.01 "
               "(678,912 params), BF16 GEMMs, 4096 walkers")),
+    "hubbard4x4_nnftns": dict(
+        kind="peps", Lx=4, Ly=4, nu=8, nd=8, B=2048, burn=100, shift=5e-4, lr=0.1,
+        gram="fp64", D=4, hidden=128, eta=1.0,
+        desc=("4x4 OBC Hubbard t=1 U=8 8u+8d, Z2 fPEPS D=4 (exact contraction, chi=16) with "
+              "NN-fTNS backflow 32-128-128-3200 ReLU (436,736 params), FP64, 2048 walkers")),
+    "hubbard4x4_fpeps": dict(
+        kind="peps", Lx=4, Ly=4, nu=8, nd=8, B=2048, burn=100, shift=5e-4, lr=0.1,
+        gram="fp64", D=4, hidden=0, eta=0.0,
+        desc=("4x4 OBC Hubbard t=1 U=8 8u+8d, Z2 fPEPS D=4 (exact contraction, chi=16), "
+              "3200 params, FP64, 2048 walkers")),
 }
 
 
@@ -80,6 +90,8 @@ def _cpu_worker(args):
     """Reference algorithm on one core: batched full-amplitude sweep, E_loc by
     get_conn + full amplitudes, O_k, for a chunk of walkers."""
     os.environ["OMP_NUM_THREADS"] = "1"
+    import numpy as np
+
     from oracle import hubbard as ohub
     from oracle import nnbf as onn
     from oracle import sampler as osamp
@@ -96,13 +108,34 @@ def _cpu_worker(args):
         def work(xs, s, l):
             osl.local_energy(H, M, xs, logamp=(s, l))
             osl.log_derivative(M, xs)
-    else:
+    elif kind == "nnbf":
         def logamp(c):
             return onn.forward(params, c, 2, bf)[:2]
 
         def work(xs, s, l):
             onn.local_energy(H, params, xs, 2, bf)
             onn.log_derivative(params, xs, 2, bf)
+    else:
+        from oracle import fpeps as ofp
+        sh = ofp.PEPSShape(Lx, Ly, bf["D"])
+        th, nn, eta = params[0], (params[1:] or None), bf["eta"]
+
+        def amp(c):
+            return (ofp.amplitude(sh, th, c) if nn is None
+                    else ofp.nnftn_amplitude(sh, th, nn, eta, c))
+
+        def logamp(cs):
+            a = np.array([amp(c) for c in np.atleast_2d(cs)])
+            with np.errstate(divide="ignore"):
+                return np.sign(a), np.maximum(np.log(np.abs(a)), np.log(1e-45))
+
+        def work(xs, s, l):
+            ofp.local_energy(H, sh, th, xs, nn, eta)
+            for c in xs:
+                if nn is None:
+                    ofp.log_derivative(sh, th, c)
+                else:
+                    ofp.nnftn_log_derivative(sh, th, nn, eta, c)
     done, t0 = 0, time.perf_counter()
     chunk = 4
     while time.perf_counter() - t0 < budget:
@@ -329,6 +362,60 @@ class NnbfRun:
                 for p in self.model.params], self.w["bf_scale"]
 
 
+class PepsRun(NnbfRun):
+    """fPEPS / NN-fTNS walkers through the reference-shaped model classes."""
+
+    def __init__(self, w, a, dev, H, fxs):
+        import torch
+
+        from vmc_torch_amd import NNfTNS_Model_GPU, fPEPS_Model_GPU, kernels
+        self.k, self.a, self.w, self.H, self.fxs = kernels, a, w, H, fxs
+        N, nf, B = w["Lx"] * w["Ly"], w["nu"] + w["nd"], w["B"]
+        self.N, self.nf, self.B = N, nf, B
+        g = torch.Generator().manual_seed(0)
+        if w["hidden"] > 0:
+            self.model = NNfTNS_Model_GPU(w["Lx"], w["Ly"], w["D"], hidden_dim=w["hidden"],
+                                          nn_eta=w["eta"], generator=g).to(dev)
+        else:
+            self.model = fPEPS_Model_GPU(w["Lx"], w["Ly"], w["D"], generator=g).to(dev)
+        self.Np = self.model.n_params
+        self.e_sweep = H.graph.edge_tensor(dev, "sweep")
+        self.e_graph = H.graph.edge_tensor(dev, "graph")
+        odt = torch.float64 if w["gram"] == "fp64" else torch.float32
+        self.O = torch.empty(B, self.Np, dtype=odt, device=dev)
+        self.dp = torch.empty(self.Np, dtype=torch.float64, device=dev)
+        self.ws = self.k.SRWorkspace()
+
+    def roofline_items(self):
+        """Sweep and local energy: FP64 flops of every amplitude evaluated
+        (column-pair contraction; the MLP when present).  logderiv: the O write."""
+        D, Ly, Hd = self.w["D"], self.w["Ly"], self.w["hidden"]
+        contr = 0
+        for y in range(Ly):
+            dl = 1 if y == 0 else D
+            dr = 1 if y == Ly - 1 else D
+            da, da2 = dl * dl, dr * dr
+            contr += 2 * (da * D * da2 * D) + da2 * D * da2 * da + da2 * da2 * da2 * D
+        flops = 2.0 * contr
+        if Hd:
+            flops += 2.0 * (2 * self.N * Hd + Hd * Hd + Hd * self.model.n_tn_params)
+        n_prop = self.e_sweep.shape[0] * self.B
+        x, e = self.fxs, self.e_graph.long()
+        n_conn = 0
+        for bit in (1, 2):
+            occ = (x & bit) != 0
+            n_conn += int((occ[:, e[:, 0]] != occ[:, e[:, 1]]).sum().item())
+        return {
+            "sweep": ("fp64", n_prop * flops),
+            "local_energy": ("fp64", n_conn * flops),
+            "logderiv": ("hbm", self.B * self.Np * float(self.O.element_size())),
+        }
+
+    def oracle_params(self):
+        ps = [p.detach().cpu().numpy() for p in self.model.params]
+        return ps, {"D": self.w["D"], "eta": self.w["eta"]}
+
+
 # --------------------------------------------------------------------- main
 def main():
     a = parse()
@@ -357,7 +444,7 @@ def main():
     base[: N - nf] = 0
     perms = torch.argsort(torch.rand(B, N, generator=torch.Generator().manual_seed(42)), dim=1)
     fxs = base[perms].contiguous().to(dev)
-    run = (SlaterRun if w["kind"] == "slater" else NnbfRun)(w, a, dev, H, fxs)
+    run = {"slater": SlaterRun, "nnbf": NnbfRun, "peps": PepsRun}[w["kind"]](w, a, dev, H, fxs)
     seed, sweep_no = 0x5EED, 0
     log(f"{a.workload}: {B} walkers, {run.Np} parameters; burn-in {w['burn']} sweeps")
     run.burn(w["burn"], seed, 0)
@@ -428,6 +515,11 @@ def main():
         if bound == "hbm":
             kern[k] = {"ms": kt[k], "bound": "hbm", "achieved": work / secs / 1e9,
                        "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": work / secs / PEAK_HBM}
+        elif bound == "fp64":
+            kern[k] = {"ms": kt[k], "bound": "mfma", "achieved": work / secs / 1e12,
+                       "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                       "frac": work / secs / PEAK_FP64,
+                       "note": "FP64 flops of every amplitude evaluated (contraction + MLP)"}
         else:
             kern[k] = {"ms": kt[k], "bound": "mfma", "achieved": work / secs / 1e12,
                        "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
@@ -466,6 +558,8 @@ def main():
 
     if w["kind"] == "slater":
         dtype = "f64" if gram == "fp64" else f"f64+{gram}-gram"
+    elif w["kind"] == "peps":
+        dtype = "f64" if gram == "fp64" else f"f64/f32-O/{gram}-gram/f64-cholesky"
     else:
         dtype = f"{w['mlp']}-gemm/f64-det/f32-O/{gram}-gram/f64-cholesky"
     out = {
@@ -496,6 +590,7 @@ def main():
 
     if not a.no_cpu_baseline:
         # FP64 oracle on a subset of the final walkers: |dE/site|, then the timed baseline
+        from oracle import fpeps as ofp
         from oracle import hubbard as ohub
         from oracle import nnbf as onn
         from oracle import slater as osl
@@ -506,8 +601,11 @@ def main():
         Ho = ohub.Hubbard(Lx, Ly, 1.0, 8.0, nu, nd)
         if w["kind"] == "slater":
             Eo = osl.local_energy(Ho, params[0], xs)
-        else:
+        elif w["kind"] == "nnbf":
             Eo = onn.local_energy(Ho, params, xs, 2, bf)
+        else:
+            Eo = ofp.local_energy(Ho, ofp.PEPSShape(Lx, Ly, bf["D"]), params[0], xs,
+                                  params[1:] or None, bf["eta"])
         out["dE_per_site_vs_fp64_oracle"] = {
             "mean": float(abs(Eg.mean() - Eo.mean()) / N),
             "max_per_sample": float(np.abs(Eg - Eo).max() / N), "samples": int(len(xs))}

@@ -26,7 +26,8 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a - b)
 
 
-@pytest.mark.parametrize("Ns,Np", [(1, 5), (100, 60), (129, 300), (257, 200), (1000, 700)])
+@pytest.mark.parametrize("Ns,Np", [(1, 5), (100, 60), (129, 300), (257, 200), (1000, 700),
+                                   (300, 5000), (64, 40000)])
 def test_minsr_fp64_matches_oracle(device, Ns, Np):
     O, E = _problem(Ns, Np, Ns)
     want = osr.minsr(O, E, E.mean(), 1e-3)
