@@ -488,7 +488,7 @@ def main():
         sweep_no += 1
         sr_phases.append(ph)
     torch.cuda.synchronize()
-    phase_names = ["center", "gram", "factor", "solve", "dp_stats"]
+    phase_names = ["center", "gram", "factor", "solve", "dp_stats", "chol_diag_launch"]
     phase_ms = {k: sum(p[i] for p in sr_phases) / len(sr_phases)
                 for i, k in enumerate(phase_names)}
     acc_rate = float(torch.stack(accs).double().mean()) / run.e_sweep.shape[0]
@@ -540,8 +540,9 @@ def main():
         return {"ms": ms, "bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9,
                 "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / PEAK_HBM}
     kern["sr.gram"] = flop_roof(gram_flops, phase_ms["gram"], peak_sr,
-                                "Ns^2 Np (FP64: SYRK lower half; reduced precision: full "
-                                "GEMM), one rocBLAS call")
+                                "algorithmic flops Ns^2 Np (FP64, symmetric product) or "
+                                "2 Ns^2 Np (reduced-precision GEMM); rocBLAS SYRK, or "
+                                "split-K batched DGEMM when Ot is short and wide")
     kern["sr.factor"] = flop_roof(chol_flops, phase_ms["factor"], PEAK_FP64,
                                   "Ns^3/3: blocked FP64 Cholesky (LDS diagonal kernel + "
                                   "TRSM/SYRK)")
@@ -551,7 +552,24 @@ def main():
     else:
         cbytes = 2.0 * Ns * Np * osz + Ns * Np * (osz + 2)    # + in-place O and BF16/FP32 copy
     kern["sr.center"] = hbm_roof(cbytes, phase_ms["center"])
-    dom = max((k for k in kern if "frac" in kern[k]), key=lambda k: kern[k]["ms"])
+    # one launch of the diagonal-block Cholesky kernel: Nb^3/3 (factor) +
+    # Nb^3/3 (block inverse) flops, Nb = 128; Ns/128 launches per SR call
+    nb = 128
+    n_diag = (Ns + nb - 1) // nb
+    dl_ms = phase_ms["chol_diag_launch"]
+    kern["sr.chol_diag"] = dict(
+        flop_roof(2.0 * nb ** 3 / 3, dl_ms, PEAK_FP64,
+                  "one launch: 128x128 FP64 Cholesky + triangular inverse in LDS "
+                  "(latency bound)"), launches_per_step=n_diag, per_step_ms=dl_ms * n_diag)
+    # the roofline line is the kernel (or single library call) with the
+    # largest time per SR iteration
+    single = {"sr.gram": 1, "sr.chol_diag": n_diag}
+    if w["kind"] == "slater":
+        single.update({"sweep": 1, "local_energy": 1, "logderiv": 1, "refresh": 1})
+    for k, n in single.items():
+        kern[k].setdefault("launches_per_step", n)
+        kern[k].setdefault("per_step_ms", kern[k]["ms"])
+    dom = max(single, key=lambda k: kern[k]["per_step_ms"])
     roof = {kk: kern[dom][kk] for kk in ("bound", "achieved", "peak", "unit", "frac")}
     roof["kernel"] = dom
     roof["traffic"] = None

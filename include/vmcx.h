@@ -144,10 +144,11 @@ int vmcx_hubbard_conn_fill(int64_t B, int n_sites, int n_edges,
  *   ||eps||, ||g|| (g = Ot^T eps, the energy gradient), PCG iterations.
  *   info (may be NULL): Cholesky info (0 = success, k > 0 = leading minor k
  *   not positive).
- *   phase_ms (HOST pointer, may be NULL): [5] float = milliseconds of the
+ *   phase_ms (HOST pointer, may be NULL): [6] float = milliseconds of the
  *   centring, Gram, factorisation, solve + PCG and dp / statistics phases,
- *   timed with HIP events on `stream`; when given, the call waits for the
- *   stream before returning.
+ *   and [5] the mean duration of one diagonal-block launch of the
+ *   factorisation, all timed with HIP events on `stream`; when given, the
+ *   call waits for the stream before returning.
  * Replaces: minSR_solver_gpu (GPU/optimizer.py:471-614; MinSRGPU.solve
  *   :873-895), whose LU solve becomes a Cholesky solve of the same SPD system.
  */

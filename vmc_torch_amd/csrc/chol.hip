@@ -433,7 +433,7 @@ static LookAhead *lookahead_for_device(int nev) {
 // between two copies: step k+1 writes the one the side stream of step k-1
 // read, which the main stream has waited for at step k.
 int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
-                int32_t *info, hipStream_t st) {
+                int32_t *info, hipStream_t st, float *diag_ms) {
   rocblas_handle h = (rocblas_handle)handle;
   const int64_t nblk = (n + CHOL_NB - 1) / CHOL_NB;
   double *Dinv = dinv, *DinvT = dinv + nblk * CHOL_NB * CHOL_NB;
@@ -449,13 +449,23 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
   VMCX_HIP(hipEventRecord(la->ev[0], st));
   VMCX_HIP(hipStreamWaitEvent(la->side, la->ev[0], 0));
   hipEvent_t side_done = nullptr;  // side-stream work of the previous step
+  // optional timing of every diagonal-block launch (events on the main stream)
+  std::vector<hipEvent_t> tev;
+  if (diag_ms) {
+    tev.resize(2 * nblk);
+    for (auto &e : tev) VMCX_HIP(hipEventCreate(&e));
+  }
+  int64_t ndiag = 0;
   for (int64_t kb = 0; kb < nblk; ++kb) {
     const int64_t k0 = kb * CHOL_NB;
     const int nbk = (int)((n - k0 < CHOL_NB) ? n - k0 : CHOL_NB);
     double *Dk = Dinv + kb * CHOL_NB * CHOL_NB;
+    if (diag_ms) VMCX_HIP(hipEventRecord(tev[2 * kb], st));
     hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(DIAG_THREADS), 0, st, A, lda, k0, nbk,
                        info, Dk, DinvT + kb * CHOL_NB * CHOL_NB);
     VMCX_LAUNCHED();
+    if (diag_ms) VMCX_HIP(hipEventRecord(tev[2 * kb + 1], st));
+    ++ndiag;
     const int64_t rest = n - k0 - nbk;
     if (rest <= 0) break;
     double *P = A + k0 * lda + k0 + nbk;
@@ -489,6 +499,17 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
     VMCX_HIP(hipEventRecord(side_done, la->side));
   }
   if (side_done) VMCX_HIP(hipStreamWaitEvent(st, side_done, 0));
+  if (diag_ms) {
+    VMCX_HIP(hipEventSynchronize(tev[2 * ndiag - 1]));
+    double tot = 0.0;
+    for (int64_t k = 0; k < ndiag; ++k) {
+      float ms = 0.0f;
+      VMCX_HIP(hipEventElapsedTime(&ms, tev[2 * k], tev[2 * k + 1]));
+      tot += ms;
+    }
+    for (auto &e : tev) (void)hipEventDestroy(e);
+    *diag_ms = (float)(tot / (double)ndiag);
+  }
   return 0;
 }
 

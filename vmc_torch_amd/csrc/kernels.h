@@ -59,8 +59,10 @@ int sr_minres(int64_t Ns, int64_t Np, const void *O, int o_dtype, int64_t ldo, c
               double energy_mean, double diag_shift, double rtol, int maxiter, void *ws,
               int64_t ws_bytes, double *dp, double *stats, int32_t *info, hipStream_t st);
 int64_t chol_dinv_bytes(int64_t n);
+// diag_ms != NULL: mean duration (ms) of one diagonal-block launch, timed with
+// events on `st` (the call then waits for the stream)
 int chol_factor(void *rocblas_handle_, int64_t n, double *A, int64_t lda,
-                double *dinv, int32_t *info, hipStream_t st);
+                double *dinv, int32_t *info, hipStream_t st, float *diag_ms = nullptr);
 // skip != NULL and *skip != 0 (a device flag): every launch returns at once
 int chol_solve(int64_t n, const double *L, int64_t lda, const double *dinv,
                const double *b, double *x, double *tmp, double *tmp2,

@@ -551,7 +551,7 @@ static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, cons
   }
   timer.mark();
   // FP64 Cholesky of the (shifted) Gram
-  rc = chol_factor((void *)h, Ns, T64, Ns, dinv, iinfo, st);
+  rc = chol_factor((void *)h, Ns, T64, Ns, dinv, iinfo, st, phase_ms ? phase_ms + 5 : nullptr);
   if (rc) return rc;
   timer.mark();
   rc = chol_solve(Ns, T64, Ns, dinv, eps, alpha, t1, t2, st);

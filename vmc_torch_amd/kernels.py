@@ -188,7 +188,8 @@ def sr_minsr(O, E, energy_mean, diag_shift, gram="fp64", max_iter=0, tol=1e-10,
     row-major (overwritten by the centred matrix when inplace=True).
     Returns (dp [Np] float64, stats [5] float64 device tensor or None, info
     [1] int32 device tensor).  phase_ms: optional list, filled with the 5
-    phase durations (ms; the call then synchronises the stream)."""
+    phase durations and the mean diagonal-block launch of the factorisation
+    (ms; the call then synchronises the stream)."""
     import ctypes
     if O.dtype not in (torch.float64, torch.float32):
         raise TypeError("O must be float64 or float32")
@@ -204,7 +205,7 @@ def sr_minsr(O, E, energy_mean, diag_shift, gram="fp64", max_iter=0, tol=1e-10,
     stats = torch.empty(5, dtype=F64, device=O.device) if with_stats else None
     info = torch.empty(1, dtype=I32, device=O.device)
     em = float("nan") if energy_mean is None else float(energy_mean)
-    ph = (ctypes.c_float * 5)() if phase_ms is not None else None
+    ph = (ctypes.c_float * 6)() if phase_ms is not None else None
     check(lib().vmcx_sr_minsr(Ns, Np, ptr(O), od, O.stride(0), flags, ptr(E), em,
                               float(diag_shift), GRAM_CODES[gram], int(max_iter), float(tol),
                               ptr(ws), ws.numel(), ptr(dp), ptr(stats), ptr(info),
