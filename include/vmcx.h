@@ -324,6 +324,57 @@ int vmcx_peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *confi
                        void *workspace, int64_t ws_bytes, int64_t chunk, void *O, int o_dtype,
                        int64_t ld, double *sign, double *logabs, void *stream);
 
+/*
+ * Learned Metropolis proposal (no counterpart in the reference; the closest is
+ * the external-proposal sampler MetropolisMPSSamplerSpinful, sampler.py:
+ * 2120-2238, acceptance min(1, P'/P * q/q') at :2214-2218).
+ *   Net: occupation grid (n_up, n_dn) of an Lx x Ly lattice -> three 3x3
+ *   convolutions (zero padding, C channels, ReLU) -> per-edge head
+ *   logit[e][t] = bA[t] + A[t][:C] . h[:, i] + A[t][C:] . h[:, j].
+ *   params: FP32 flat [W1 (C,2,3,3) | b1 | W2 (C,C,3,3) | b2 | W3 | b3 |
+ *   A (3, 2C) | bA (3)], vmcx_lp_n_params(C) entries, C <= 32, N <= 64.
+ *   Moves m = 3 e + t, edges e = (i, j), i < j, sorted ((s, s+1) then
+ *   (s, s+Ly)); t = 0 hop of an up electron, 1 hop of a down electron,
+ *   2 exchange of a down and an up electron; q(m|x) = softmax over the valid
+ *   moves; every move is its own reverse.
+ *   Draws: Philox4x32-10, counter (step lo, step hi, walker, 0x4C50), key =
+ *   seed; u_choice = top 53 bits of (o0:o1), u_acc of (o2:o3); injected
+ *   arrays (may be NULL) replace them.
+ */
+int vmcx_lp_n_params(int C);
+/* 3 x number of edges of the Lx x Ly open lattice; -1 on bad sizes. */
+int vmcx_lp_n_moves(int Lx, int Ly);
+/* logits [B][n_moves] float64, -inf for invalid moves. */
+int vmcx_lp_logits(const float *params, int C, int Lx, int Ly, int64_t B,
+                   const int64_t *configs, double *logits, void *stream);
+/* Draw m ~ q(.|x) for every walker, write the proposed configuration and
+ * ln q(m|x), then evaluate the net at x' for ln q(m|x') (move = -1 when x
+ * has no valid move).  logits_ws: [B][n_moves] float64 scratch. */
+int vmcx_lp_propose(const float *params, int C, int Lx, int Ly, int64_t B,
+                    const int64_t *configs, uint64_t seed, int64_t step, const double *u_choice,
+                    double *logits_ws, int32_t *move, double *logq_fwd, int64_t *proposed,
+                    double *logq_rev, void *stream);
+/* Metropolis-Hastings: accept when u_acc < |psi'|^2/|psi|^2 q(m|x')/q(m|x),
+ * from (psign, plogabs) of the proposed configurations; commits configs,
+ * sign, logabs, n_accept (may be NULL) and accepted[B] (may be NULL). */
+int vmcx_lp_accept(int64_t B, int n_sites, uint64_t seed, int64_t step, const double *u_acc,
+                   const int32_t *move, const int64_t *proposed, const double *psign,
+                   const double *plogabs, const double *logq_fwd, const double *logq_rev,
+                   int64_t *configs, double *sign, double *logabs, int64_t *n_accept,
+                   uint8_t *accepted, void *stream);
+/* grad[n_params] = sum_b d(sum_m dlogits[b][m] logits[b][m]) / d params
+ * (FP32; per-configuration rows in `workspace`, summed in a fixed order). */
+int64_t vmcx_lp_backward_workspace_bytes(int C, int64_t B);
+int vmcx_lp_backward(const float *params, int C, int Lx, int Ly, int64_t B,
+                     const int64_t *configs, const double *dlogits, void *workspace,
+                     int64_t ws_bytes, float *grad, void *stream);
+/* out[B][n_moves][N]: every move applied to each configuration (a copy for
+ * invalid moves): the candidates whose amplitudes give the heat-bath target. */
+int vmcx_lp_candidates(int64_t B, int Lx, int Ly, const int64_t *configs, int64_t *out,
+                       void *stream);
+/* y += a x (FP32): the parameter update of the proposal net. */
+int vmcx_lp_axpy(int64_t n, float a, const float *x, float *y, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,16 @@ _SIGNATURES = {
                                     _P, _P]),
     "vmcx_nnbf_logderiv": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _I, _I64, _P, _P, _P]),
     "vmcx_to_bf16": (_I, [_I64, _P, _P, _P]),
+    "vmcx_lp_n_params": (_I, [_I]),
+    "vmcx_lp_n_moves": (_I, [_I, _I]),
+    "vmcx_lp_logits": (_I, [_P, _I, _I, _I, _I64, _P, _P, _P]),
+    "vmcx_lp_propose": (_I, [_P, _I, _I, _I, _I64, _P, _U64, _I64, _P, _P, _P, _P, _P, _P, _P]),
+    "vmcx_lp_accept": (_I, [_I64, _I, _U64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                            _P, _P]),
+    "vmcx_lp_backward_workspace_bytes": (_I64, [_I, _I64]),
+    "vmcx_lp_backward": (_I, [_P, _I, _I, _I, _I64, _P, _P, _P, _I64, _P, _P]),
+    "vmcx_lp_candidates": (_I, [_I64, _I, _I, _P, _P, _P]),
+    "vmcx_lp_axpy": (_I, [_I64, _c.c_float, _P, _P, _P]),
     "vmcx_peps_dense_size": (_I64, [_P]),
     "vmcx_peps_workspace_bytes": (_I64, [_P, _I64, _I64, _I]),
     "vmcx_peps_forward": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _P, _P]),

@@ -37,7 +37,7 @@ static inline int status_of(int rc) {
 
 extern "C" {
 
-int vmcx_abi_version(void) { return 5; }
+int vmcx_abi_version(void) { return 6; }
 
 const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
 
@@ -193,6 +193,54 @@ int vmcx_to_bf16(int64_t n, const float *x, void *y, void *stream) {
 int vmcx_sgd_update(int64_t n, double *params, const double *dp, double lr,
                     void *stream) {
   return status_of(vmcx::sgd_update(n, params, dp, lr, (hipStream_t)stream));
+}
+
+int vmcx_lp_n_params(int C) { return vmcx::lp_n_params(C); }
+
+int vmcx_lp_n_moves(int Lx, int Ly) { return vmcx::lp_n_moves(Lx, Ly); }
+
+int vmcx_lp_logits(const float *params, int C, int Lx, int Ly, int64_t B,
+                   const int64_t *configs, double *logits, void *stream) {
+  return status_of(vmcx::lp_logits(params, C, Lx, Ly, B, configs, logits, (hipStream_t)stream));
+}
+
+int vmcx_lp_propose(const float *params, int C, int Lx, int Ly, int64_t B,
+                    const int64_t *configs, uint64_t seed, int64_t step, const double *u_choice,
+                    double *logits_ws, int32_t *move, double *logq_fwd, int64_t *proposed,
+                    double *logq_rev, void *stream) {
+  return status_of(vmcx::lp_propose(params, C, Lx, Ly, B, configs, seed, step, u_choice,
+                                    logits_ws, move, logq_fwd, proposed, logq_rev,
+                                    (hipStream_t)stream));
+}
+
+int vmcx_lp_accept(int64_t B, int n_sites, uint64_t seed, int64_t step, const double *u_acc,
+                   const int32_t *move, const int64_t *proposed, const double *psign,
+                   const double *plogabs, const double *logq_fwd, const double *logq_rev,
+                   int64_t *configs, double *sign, double *logabs, int64_t *n_accept,
+                   uint8_t *accepted, void *stream) {
+  return status_of(vmcx::lp_accept(B, n_sites, seed, step, u_acc, move, proposed, psign,
+                                   plogabs, logq_fwd, logq_rev, configs, sign, logabs, n_accept,
+                                   accepted, (hipStream_t)stream));
+}
+
+int64_t vmcx_lp_backward_workspace_bytes(int C, int64_t B) {
+  return vmcx::lp_backward_workspace_bytes(C, B);
+}
+
+int vmcx_lp_backward(const float *params, int C, int Lx, int Ly, int64_t B,
+                     const int64_t *configs, const double *dlogits, void *workspace,
+                     int64_t ws_bytes, float *grad, void *stream) {
+  return status_of(vmcx::lp_backward(params, C, Lx, Ly, B, configs, dlogits, workspace, ws_bytes,
+                                     grad, (hipStream_t)stream));
+}
+
+int vmcx_lp_candidates(int64_t B, int Lx, int Ly, const int64_t *configs, int64_t *out,
+                       void *stream) {
+  return status_of(vmcx::lp_candidates(B, Lx, Ly, configs, out, (hipStream_t)stream));
+}
+
+int vmcx_lp_axpy(int64_t n, float a, const float *x, float *y, void *stream) {
+  return status_of(vmcx::lp_axpy(n, a, x, y, (hipStream_t)stream));
 }
 
 int64_t vmcx_peps_dense_size(const vmcx_peps_model *m) { return vmcx::peps_dense_size(m); }

@@ -109,6 +109,24 @@ int peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void 
                   int64_t ws_bytes, int64_t chunk, void *O, int o_dtype, int64_t ld,
                   double *sign, double *logabs, hipStream_t st);
 
+int lp_n_params(int C);
+int lp_n_moves(int Lx, int Ly);
+int lp_logits(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,
+              double *logits, hipStream_t st);
+int lp_propose(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,
+               uint64_t seed, int64_t step, const double *u_choice, double *logits,
+               int32_t *move, double *logq_f, int64_t *pcfg, double *logq_r, hipStream_t st);
+int lp_accept(int64_t B, int N, uint64_t seed, int64_t step, const double *u_acc,
+              const int32_t *move, const int64_t *pcfg, const double *psign,
+              const double *plog, const double *logq_f, const double *logq_r, int64_t *cfg,
+              double *sign, double *logabs, int64_t *n_accept, uint8_t *accepted,
+              hipStream_t st);
+int64_t lp_backward_workspace_bytes(int C, int64_t B);
+int lp_backward(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,
+                const double *dlogits, void *ws, int64_t ws_bytes, float *grad, hipStream_t st);
+int lp_candidates(int64_t B, int Lx, int Ly, const int64_t *cfg, int64_t *out, hipStream_t st);
+int lp_axpy(int64_t n, float a, const float *x, float *y, hipStream_t st);
+
 int64_t nnbf_workspace_bytes(const vmcx_nnbf_model *m, int64_t B, int64_t chunk, int n_edges);
 int nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
                  int64_t ws_bytes, int64_t chunk, double *sign, double *logabs, float *z1_out,

@@ -360,3 +360,85 @@ def peps_workspace(struct, B, chunk, n_edges, device, holder):
         buf = torch.empty(need, dtype=torch.uint8, device=device)
         holder["ws"] = buf
     return buf
+
+
+# ------------------------------------------------------- learned proposal
+def lp_n_params(C):
+    return int(lib().vmcx_lp_n_params(int(C)))
+
+
+def lp_n_moves(Lx, Ly):
+    return int(lib().vmcx_lp_n_moves(int(Lx), int(Ly)))
+
+
+def _lp_check(params, C, configs, Lx, Ly):
+    require(params, torch.float32, "proposal params", (lp_n_params(C),))
+    B, N = _check_configs(configs, params.device)
+    if N != Lx * Ly:
+        raise ValueError("configs must have Lx * Ly sites")
+    return B, N
+
+
+def lp_logits(params, C, Lx, Ly, configs):
+    B, _ = _lp_check(params, C, configs, Lx, Ly)
+    out = torch.empty(B, lp_n_moves(Lx, Ly), dtype=F64, device=configs.device)
+    check(lib().vmcx_lp_logits(ptr(params), int(C), int(Lx), int(Ly), B, ptr(configs), ptr(out),
+                               stream_ptr(configs.device)))
+    return out
+
+
+def lp_propose(params, C, Lx, Ly, configs, seed, step, u_choice=None):
+    """(move int32 [B], ln q(m|x) [B], proposed configs [B, N], ln q(m|x') [B])."""
+    B, N = _lp_check(params, C, configs, Lx, Ly)
+    dev = configs.device
+    logits = torch.empty(B, lp_n_moves(Lx, Ly), dtype=F64, device=dev)
+    move = torch.empty(B, dtype=I32, device=dev)
+    lqf = torch.empty(B, dtype=F64, device=dev)
+    lqr = torch.empty(B, dtype=F64, device=dev)
+    prop = torch.empty(B, N, dtype=I64, device=dev)
+    if u_choice is not None:
+        require(u_choice, F64, "u_choice", (B,), dev)
+    check(lib().vmcx_lp_propose(ptr(params), int(C), int(Lx), int(Ly), B, ptr(configs),
+                                int(seed) & (2 ** 64 - 1), int(step), ptr(u_choice), ptr(logits),
+                                ptr(move), ptr(lqf), ptr(prop), ptr(lqr), stream_ptr(dev)))
+    return move, lqf, prop, lqr
+
+
+def lp_accept(configs, sign, logabs, move, proposed, psign, plogabs, logq_f, logq_r, seed, step,
+              u_acc=None, n_accept=None):
+    B, N = _check_configs(configs, configs.device)
+    dev = configs.device
+    acc = torch.empty(B, dtype=U8, device=dev)
+    if u_acc is not None:
+        require(u_acc, F64, "u_acc", (B,), dev)
+    check(lib().vmcx_lp_accept(B, N, int(seed) & (2 ** 64 - 1), int(step), ptr(u_acc), ptr(move),
+                               ptr(proposed), ptr(psign), ptr(plogabs), ptr(logq_f), ptr(logq_r),
+                               ptr(configs), ptr(sign), ptr(logabs), ptr(n_accept), ptr(acc),
+                               stream_ptr(dev)))
+    return acc
+
+
+def lp_backward(params, C, Lx, Ly, configs, dlogits):
+    B, _ = _lp_check(params, C, configs, Lx, Ly)
+    require(dlogits, F64, "dlogits", (B, lp_n_moves(Lx, Ly)), configs.device)
+    need = int(lib().vmcx_lp_backward_workspace_bytes(int(C), B))
+    ws = torch.empty(max(need, 1), dtype=U8, device=configs.device)
+    grad = torch.empty_like(params)
+    check(lib().vmcx_lp_backward(ptr(params), int(C), int(Lx), int(Ly), B, ptr(configs),
+                                 ptr(dlogits), ptr(ws), need, ptr(grad),
+                                 stream_ptr(configs.device)))
+    return grad
+
+
+def lp_candidates(Lx, Ly, configs):
+    B, N = _check_configs(configs, configs.device)
+    out = torch.empty(B, lp_n_moves(Lx, Ly), N, dtype=I64, device=configs.device)
+    check(lib().vmcx_lp_candidates(B, int(Lx), int(Ly), ptr(configs), ptr(out),
+                                   stream_ptr(configs.device)))
+    return out
+
+
+def lp_axpy(a, x, y):
+    require(x, torch.float32, "x")
+    require(y, torch.float32, "y", (x.numel(),), x.device)
+    check(lib().vmcx_lp_axpy(x.numel(), float(a), ptr(x), ptr(y), stream_ptr(x.device)))

@@ -71,3 +71,104 @@ class MetropolisExchangeSpinfulSamplerGPU(SamplerGPU):
     def burn_in(self, fxs, model, graph, n_steps, **kwargs):
         self.sweeps(fxs, model, graph, int(n_steps))
         return fxs
+
+
+class LearnedProposalSamplerGPU(SamplerGPU):
+    """Metropolis-Hastings with a learned local proposal (no counterpart in the
+    reference; its external-proposal sampler MetropolisMPSSamplerSpinful,
+    sampler.py:2120-2238, uses the same acceptance min(1, P'/P q/q'),
+    :2214-2218).
+
+    A 3-layer 3x3 convolutional net on the (n_up, n_dn) grid scores every
+    local move (hop of an up or a down electron across an edge, exchange of a
+    down and an up electron); q(m|x) is the softmax over the valid moves and
+    the acceptance uses q(m|x') from the net at the proposed configuration,
+    so the chain samples |psi|^2 exactly for any net.  `train_step` fits the
+    net to the heat-bath distribution over the move set,
+    q*(m|x) ~ |psi(m(x))|^2, by cross entropy.  The net (FP32, `params`) is
+    size-independent: one instance can serve lattices of different shapes
+    (pass the same tensor).
+    """
+
+    def __init__(self, Lx, Ly, channels=32, seed=0, params=None, generator=None,
+                 device=None):
+        from . import kernels
+        self.k = kernels
+        self.Lx, self.Ly, self.C = int(Lx), int(Ly), int(channels)
+        self.seed = int(seed)
+        self.step_counter = 0
+        self.last_acceptance = None
+        if params is None:
+            params = self.init_params(self.C, generator)
+        self.params = params if device is None else params.to(device)
+
+    @staticmethod
+    def init_params(C, generator=None):
+        """Kaiming-uniform convolutions, head N(0, 0.1^2), zero biases."""
+        shapes = [(C, 2, 3, 3), (C,), (C, C, 3, 3), (C,), (C, C, 3, 3), (C,), (3, 2 * C), (3,)]
+        out = []
+        for sh in shapes:
+            if len(sh) == 4:
+                a = (6.0 / (sh[1] * 9)) ** 0.5
+                out.append(torch.empty(sh).uniform_(-a, a, generator=generator).reshape(-1))
+            elif len(sh) == 2:
+                out.append((torch.randn(sh, generator=generator) * 0.1).reshape(-1))
+            else:
+                out.append(torch.zeros(sh).reshape(-1))
+        return torch.cat(out).to(torch.float32)
+
+    @torch.inference_mode()
+    def sweeps(self, fxs, model, n_steps=1, draws=None):
+        """n_steps proposals per walker; returns (sign, log|psi|).  draws: list
+        of (u_choice, u_acc) device arrays per step (tests)."""
+        if self.params.device != fxs.device:
+            self.params = self.params.to(fxs.device)
+        sign, logabs = model.forward_log(fxs)
+        sign, logabs = sign.clone(), logabs.clone()
+        n_acc = torch.zeros(fxs.shape[0], dtype=torch.int64, device=fxs.device)
+        for s in range(n_steps):
+            uc = ua = None
+            if draws is not None:
+                uc, ua = draws[s]
+            move, lqf, prop, lqr = self.k.lp_propose(self.params, self.C, self.Lx, self.Ly, fxs,
+                                                     self.seed, self.step_counter, uc)
+            ps, pl = model.forward_log(prop)
+            self.k.lp_accept(fxs, sign, logabs, move, prop, ps, pl, lqf, lqr, self.seed,
+                             self.step_counter, ua, n_acc)
+            self.step_counter += 1
+        if hasattr(model, "invalidate"):
+            model.invalidate()      # the model's cached walker state predates the moves
+        self.last_acceptance = (n_acc, n_steps)
+        return sign, logabs
+
+    @torch.inference_mode()
+    def step(self, fxs, model, graph=None, compile=False, verbose=False, use_log_amp=False,
+             **kwargs):
+        sign, logabs = self.sweeps(fxs, model, 1)
+        if use_log_amp:
+            return fxs, (sign, logabs)
+        return fxs, sign * torch.exp(logabs)
+
+    def burn_in(self, fxs, model, graph=None, n_steps=1, **kwargs):
+        self.sweeps(fxs, model, int(n_steps))
+        return fxs
+
+    @torch.inference_mode()
+    def train_step(self, fxs, model, lr=1e-2):
+        """One gradient step of the heat-bath cross entropy on the walkers
+        `fxs`; returns the loss before the step."""
+        if self.params.device != fxs.device:
+            self.params = self.params.to(fxs.device)
+        B = fxs.shape[0]
+        logits = self.k.lp_logits(self.params, self.C, self.Lx, self.Ly, fxs)
+        valid = torch.isfinite(logits)
+        cand = self.k.lp_candidates(self.Lx, self.Ly, fxs)
+        _, lc = model.forward_log(cand.reshape(-1, fxs.shape[1]))
+        target = torch.where(valid, 2.0 * lc.reshape(B, -1), torch.full_like(logits, -torch.inf))
+        qstar = torch.softmax(target, dim=1)
+        logq = torch.log_softmax(logits, dim=1)
+        loss = -(qstar * torch.where(valid, logq, torch.zeros_like(logq))).sum() / B
+        dl = torch.where(valid, (torch.exp(logq) - qstar) / B, torch.zeros_like(logits))
+        grad = self.k.lp_backward(self.params, self.C, self.Lx, self.Ly, fxs, dl.contiguous())
+        self.k.lp_axpy(-lr, grad, self.params)
+        return float(loss)
