@@ -160,6 +160,22 @@ int vmcx_sr_minsr(int64_t Ns, int64_t Np, void *O, int o_dtype, int64_t ldo, int
                   double *stats, int32_t *info, float *phase_ms, void *stream);
 
 /*
+ * FP64 symmetric rank-K update on the matrix cores (a building block of the
+ * minSR Gram and of the blocked Cholesky, exported for direct use):
+ *   C = beta C + alpha M M^T on the lower triangle of the n x n column-major
+ *   C (ldc >= n); the strictly upper triangle is not touched.
+ *   M is n x K: rowmajor = 1: M(i, k) = X[i*ldx + k] (ldx >= K);
+ *   rowmajor = 0: M(i, k) = X[k*ldx + i] (ldx >= n).
+ *   splits > 1: the reduction is cut into `splits` equal chunks; chunk s
+ *   writes beta C_s + alpha M_s M_s^T into C + s*cstride (cstride >= ldc*n).
+ * Replaces: torch.matmul(O, O.T) of the minSR Gram (GPU/optimizer.py:530-536)
+ *   and the trailing update of the LU/Cholesky that follows it.
+ */
+int vmcx_syrk_f64(int64_t n, int64_t K, const double *X, int64_t ldx, int rowmajor,
+                  double alpha, double beta, double *C, int64_t ldc, int splits,
+                  int64_t cstride, void *stream);
+
+/*
  * Stochastic reconfiguration in parameter space by MINRES:
  *   (S + diag_shift I) dp = g,  S x = O^T (O x) / Ns - (Obar . x) Obar,
  *   g = (1/Ns) sum_s (E_s - energy_mean) O_s,  Obar = mean_s O_s,

@@ -132,6 +132,13 @@ int vmcx_sr_minsr(int64_t Ns, int64_t Np, void *O, int o_dtype, int64_t ldo, int
   return rc == 3 ? VMCX_ELIB : status_of(rc);
 }
 
+int vmcx_syrk_f64(int64_t n, int64_t K, const double *X, int64_t ldx, int rowmajor,
+                  double alpha, double beta, double *C, int64_t ldc, int splits,
+                  int64_t cstride, void *stream) {
+  return status_of(vmcx::syrk_f64(n, K, X, ldx, rowmajor, alpha, beta, C, ldc, splits, cstride,
+                                  (hipStream_t)stream));
+}
+
 int64_t vmcx_sr_minres_workspace_bytes(int64_t Ns, int64_t Np) {
   return vmcx::sr_minres_workspace_bytes(Ns, Np);
 }

@@ -4,7 +4,9 @@
 // Factorisation, right-looking with block NB = 128:
 //   for each diagonal block k:
 //     chol_diag_kernel  : L_kk = chol(A_kk) in LDS (one workgroup), and
-//                         Dinv_k = L_kk^-1 (kept for the solves)
+//                         Dinv_k = L_kk^-1 (kept for the solves; L_kk itself
+//                         is never written back: every later use of the
+//                         diagonal block goes through Dinv_k)
 //     rocblas_dtrsm     : A_ik <- A_ik L_kk^-T          (panel below)
 //     rocblas_dsyrk     : A_22 <- A_22 - P P^T          (trailing update)
 // The rocSOLVER panel kernel is latency bound at this size; the diagonal
@@ -30,12 +32,13 @@ constexpr int CHOL_NB = 128;
 // DinvT_k (column-major):       DinvT[c*NB + m] = (L_kk^-1)[c][m].
 //
 // The 128 x 128 diagonal block is factorised in LDS in 16-column steps t:
-//   leaf     : one wave factorises the 16 x 16 diagonal tile in registers
-//              (lane i holds row i; columns broadcast with DPP row_newbcast)
-//              and inverts it there; L_tt stays in the lower triangle of the
-//              tile, the strictly lower part of X_tt = L_tt^-1 goes
-//              transposed into its (unused) upper triangle, and the
-//              reciprocal diagonal 1 / L_ii (= X_ii) into rdiag
+//   leaf     : one wave factorises and inverts the 16 x 16 diagonal tile
+//              as 2 x 2 blocks of 8 x 8 (each 8 x 8 Cholesky + inverse in
+//              one lane's registers, the coupling blocks lane-parallel);
+//              L_tt stays in the lower triangle of the tile, the strictly
+//              lower part of X_tt = L_tt^-1 goes transposed into its
+//              (unused) upper triangle, and the reciprocal diagonal
+//              1 / L_ii (= X_ii) into rdiag
 //   panel    : L_I,t = A_I,t X_tt^T for every tile I below (matrix cores)
 //   trailing : C_IJ -= L_I,t L_J,t^T for t < J <= I (matrix cores); the wave
 //              that updates tile (t+1, t+1) factorises it at once, while the
@@ -51,15 +54,7 @@ constexpr int LEAF = 16;
 constexpr int DIAG_THREADS = 512;
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-// 1/d and 1/sqrt(d) for d > 0: hardware estimate and two Newton steps
-__device__ __forceinline__ double rcp_f64(double d) {
-  double y = __builtin_amdgcn_rcp(d);
-  double e = fma(-d, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-d, y, 1.0);
-  return fma(y, e, y);
-}
-
+// 1/sqrt(d) for d > 0: hardware estimate and two Newton steps
 __device__ __forceinline__ double rsqrt_f64(double d) {
   double y = __builtin_amdgcn_rsq(d);
 #pragma unroll
@@ -71,93 +66,140 @@ __device__ __forceinline__ double rsqrt_f64(double d) {
   return y;
 }
 
-// value of v in lane S of each row of 16 lanes (DPP row_newbcast)
-template <int S>
-__device__ __forceinline__ double bcast16_c(double v) {
-  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, 0x150 + S, 0xF, 0xF, false);
-  const int hi =
-      __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x150 + S, 0xF, 0xF, false);
-  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
-
-// s must fold to a constant (fully unrolled loops)
-__device__ __forceinline__ double bcast16(double v, int s) {
-  switch (s) {
-    case 0: return bcast16_c<0>(v);
-    case 1: return bcast16_c<1>(v);
-    case 2: return bcast16_c<2>(v);
-    case 3: return bcast16_c<3>(v);
-    case 4: return bcast16_c<4>(v);
-    case 5: return bcast16_c<5>(v);
-    case 6: return bcast16_c<6>(v);
-    case 7: return bcast16_c<7>(v);
-    case 8: return bcast16_c<8>(v);
-    case 9: return bcast16_c<9>(v);
-    case 10: return bcast16_c<10>(v);
-    case 11: return bcast16_c<11>(v);
-    case 12: return bcast16_c<12>(v);
-    case 13: return bcast16_c<13>(v);
-    case 14: return bcast16_c<14>(v);
-    default: return bcast16_c<15>(v);
-  }
-}
-
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Leaf of step kb (wave 0): factor and invert the tile at (kb, kb).
-__device__ __forceinline__ void chol_leaf(double (*L)[CHOL_NB + 1], double *rdiag_all, int kb,
-                                          int lane, int *bad) {
-  const int i = lane & 15;
-  const bool act = lane < LEAF;
-  double r[LEAF];
-#pragma unroll
-  for (int c = 0; c < LEAF; ++c) r[c] = (c <= i) ? L[kb + i][kb + c] : 0.0;
-  double myrs = 1.0;
+// 8x8 Cholesky and inverse of the lower triangle stored packed in a[]
+// (a[i(i+1)/2 + j], j <= i), all in one lane's registers: a becomes L,
+// x = L^-1 (same packing), rs[i] = 1 / L_ii; returns the first non-positive
+// pivot (1-based) or 0.
+__device__ __forceinline__ int chol8_inv8(double *a, double *x, double *rs) {
   int badj = 0;
 #pragma unroll
-  for (int j = 0; j < LEAF; ++j) {
-    // unscaled column j first (L_ij L_lj = a_ij a_lj / a_jj), so that the
-    // broadcasts do not wait for the square root
-    const double djj = bcast16(r[j], j);
-    double col[LEAF];
-#pragma unroll
-    for (int l = j + 1; l < LEAF; ++l) col[l] = bcast16(r[j], l);
-    const bool ok = djj > 0.0;
+  for (int j = 0; j < 8; ++j) {
+    const int jj = j * (j + 1) / 2 + j;
+    const double d = a[jj];
+    const bool ok = d > 0.0;
     if (!ok && badj == 0) badj = j + 1;
-    const double inv = ok ? rcp_f64(djj) : 1.0;
-    const double rs = ok ? rsqrt_f64(djj) : 1.0;
-    const double t = r[j] * inv;
+    const double r = ok ? rsqrt_f64(d) : 1.0;
+    rs[j] = r;
+    a[jj] = ok ? d * r : 1.0;
 #pragma unroll
-    for (int l = j + 1; l < LEAF; ++l) r[l] = (i >= l) ? fma(-t, col[l], r[l]) : r[l];
-    r[j] = (i == j) ? (ok ? djj * rs : 1.0) : ((i > j) ? r[j] * rs : r[j]);
-    if (i == j) myrs = rs;
+    for (int i = j + 1; i < 8; ++i) a[i * (i + 1) / 2 + j] *= r;
+#pragma unroll
+    for (int i = j + 1; i < 8; ++i)
+#pragma unroll
+      for (int l = j + 1; l <= i; ++l)
+        a[i * (i + 1) / 2 + l] = fma(-a[i * (i + 1) / 2 + j], a[l * (l + 1) / 2 + j],
+                                     a[i * (i + 1) / 2 + l]);
   }
-  // X = L_tt^-1 by rows: X_ii = 1 / L_ii, X_ic = -X_ii sum_{c <= q < i} L_iq X_qc
-  double acc[LEAF];
 #pragma unroll
-  for (int c = 0; c < LEAF; ++c) acc[c] = 0.0;
+  for (int i = 0; i < 8; ++i) {
+    x[i * (i + 1) / 2 + i] = rs[i];
 #pragma unroll
-  for (int q = 0; q < LEAF - 1; ++q) {
+    for (int c = 0; c < i; ++c) {
+      double acc = 0.0;
 #pragma unroll
-    for (int c = 0; c <= q; ++c) {
-      const double v = (c == q) ? myrs : -myrs * acc[c];  // X[q][c] in lane q
-      const double xb = bcast16(v, q);
-      acc[c] = (i > q) ? fma(r[q], xb, acc[c]) : acc[c];
+      for (int q = c; q < i; ++q) acc = fma(a[i * (i + 1) / 2 + q], x[q * (q + 1) / 2 + c], acc);
+      x[i * (i + 1) / 2 + c] = -rs[i] * acc;
     }
   }
-  if (act) {
+  return badj;
+}
+
+// Leaf of step kb (wave 0): factor and invert the 16 x 16 tile at (kb, kb)
+// as 2 x 2 blocks of 8 x 8:  L11 = chol(A11), X11 = L11^-1 (one lane);
+// L21 = A21 X11^T (8 lanes);  A22 -= L21 L21^T (36 lanes);  L22, X22 (one
+// lane);  X21 = -X22 L21 X11 (64 lanes, via Y = L21 X11 in `scr`).  L stays
+// in the lower triangle of the tile; the strictly lower part of
+// X = L^-1 goes transposed into its upper triangle; rdiag_all = 1 / L_ii.
+__device__ __forceinline__ void chol_leaf(double (*L)[CHOL_NB + 1], double *rdiag_all, int kb,
+                                          int lane, int *bad, double *scr) {
+  double (*T)[CHOL_NB + 1] = reinterpret_cast<double (*)[CHOL_NB + 1]>(&L[kb][kb]);
+  // ---- 1: L11, X11
+  if (lane == 0) {
+    double a[36], x[36], rs[8];
 #pragma unroll
-    for (int c = 0; c < LEAF; ++c) {
-      if (c <= i) L[kb + i][kb + c] = r[c];
-      if (c < i) L[kb + c][kb + i] = -myrs * acc[c];
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j <= i; ++j) a[i * (i + 1) / 2 + j] = T[i][j];
+    const int bj = chol8_inv8(a, x, rs);
+    if (bj && *bad == 0) *bad = kb + bj;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      rdiag_all[kb + i] = rs[i];
+#pragma unroll
+      for (int j = 0; j <= i; ++j) T[i][j] = a[i * (i + 1) / 2 + j];
+#pragma unroll
+      for (int c = 0; c < i; ++c) T[c][i] = x[i * (i + 1) / 2 + c];
     }
-    rdiag_all[kb + i] = myrs;
-    if (lane == 0 && badj && *bad == 0) *bad = kb + badj;
+  }
+  wave_lds_sync();
+  // ---- 2: L21[r][c] = sum_{q <= c} A21[r][q] X11[c][q]  (lane r - 8)
+  if (lane < 8) {
+    const int r = 8 + lane;
+    double arow[8], out[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) arow[q] = T[r][q];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      double acc = arow[c] * rdiag_all[kb + c];
+#pragma unroll
+      for (int q = 0; q < c; ++q) acc = fma(arow[q], T[q][c], acc);   // X11[c][q] at T[q][c]
+      out[c] = acc;
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) T[r][c] = out[c];
+  }
+  wave_lds_sync();
+  // ---- 3: A22[i][l] -= L21[i] . L21[l]  (i >= l, packed over 36 lanes)
+  if (lane < 36) {
+    int i = 0;
+    while ((i + 1) * (i + 2) / 2 <= lane) ++i;
+    const int l = lane - i * (i + 1) / 2;
+    double acc = T[8 + i][8 + l];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc = fma(-T[8 + i][c], T[8 + l][c], acc);
+    T[8 + i][8 + l] = acc;
+  }
+  wave_lds_sync();
+  // ---- 4: L22, X22
+  if (lane == 0) {
+    double a[36], x[36], rs[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j <= i; ++j) a[i * (i + 1) / 2 + j] = T[8 + i][8 + j];
+    const int bj = chol8_inv8(a, x, rs);
+    if (bj && *bad == 0) *bad = kb + 8 + bj;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      rdiag_all[kb + 8 + i] = rs[i];
+#pragma unroll
+      for (int j = 0; j <= i; ++j) T[8 + i][8 + j] = a[i * (i + 1) / 2 + j];
+#pragma unroll
+      for (int c = 0; c < i; ++c) T[8 + c][8 + i] = x[i * (i + 1) / 2 + c];
+    }
+  }
+  wave_lds_sync();
+  // ---- 5: Y = L21 X11 (Y[i][c] = sum_{q >= c} L21[i][q] X11[q][c]), then
+  //         X21 = -X22 Y (X21[i][c] = -sum_{q <= i} X22[i][q] Y[q][c])
+  {
+    const int i = lane >> 3, c = lane & 7;
+    double acc = T[8 + i][c] * rdiag_all[kb + c];
+#pragma unroll
+    for (int q = c + 1; q < 8; ++q) acc = fma(T[8 + i][q], T[c][q], acc);  // X11[q][c] at T[c][q]
+    scr[lane] = acc;
+  }
+  wave_lds_sync();
+  {
+    const int i = lane >> 3, c = lane & 7;
+    double acc = rdiag_all[kb + 8 + i] * scr[i * 8 + c];
+    for (int q = 0; q < i; ++q) acc = fma(T[8 + q][8 + i], scr[q * 8 + c], acc);  // X22[i][q]
+    T[c][8 + i] = -acc;   // X[8+i][c], stored transposed
   }
 }
 
@@ -204,7 +246,7 @@ __global__ void __launch_bounds__(DIAG_THREADS)
   }
   if (tid == 0) bad = 0;
   __syncthreads();
-  if (wave == 0) chol_leaf(L, rdiag_all, 0, lane, &bad);
+  if (wave == 0) chol_leaf(L, rdiag_all, 0, lane, &bad, &W[0][0]);
   __syncthreads();
 
   for (int t = 0; t < NT; ++t) {
@@ -252,7 +294,7 @@ __global__ void __launch_bounds__(DIAG_THREADS)
       }
       if (wave == 0) {
         wave_lds_sync();
-        chol_leaf(L, rdiag_all, kb + LEAF, lane, &bad);
+        chol_leaf(L, rdiag_all, kb + LEAF, lane, &bad, &W[0][0]);
       }
     }
     __syncthreads();
@@ -261,11 +303,7 @@ __global__ void __launch_bounds__(DIAG_THREADS)
     if (tid == 0 && info && info[0] == 0 && bad <= nbk) info[0] = (int32_t)(k0 + bad);
     return;
   }
-  for (int idx = tid; idx < nbk * nbk; idx += DIAG_THREADS) {
-    const int j = idx / nbk, i = idx - j * nbk;
-    if (i >= j) Akk[(int64_t)j * lda + i] = L[i][j];
-  }
-  __syncthreads();
+  // (L_kk itself is not stored: the solves and the panel use its inverse)
   // ---- X = L^-1 in place (off-diagonal tiles; the diagonal tiles already
   //      hold their inverses), tile column jt from the right
   for (int jt = NT - 2; jt >= 0; --jt) {
@@ -302,24 +340,35 @@ __global__ void __launch_bounds__(DIAG_THREADS)
     }
     __syncthreads();
   }
-  // Dinv[a*NB + b] = X[b][a];  DinvT[a*NB + b] = X[a][b]
+  // Dinv[a*NB + b] = X[b][a]  (DinvT, its transpose, is written after the
+  // factorisation by chol_dinv_transpose_kernel, off the critical path)
   for (int idx = tid; idx < CHOL_NB * CHOL_NB; idx += DIAG_THREADS) {
     const int a = idx / CHOL_NB, b = idx - a * CHOL_NB;
-    double xba = 0.0, xab = 0.0;
+    double xba = 0.0;
     if (a < nbk && b < nbk) {
       if ((a >> 4) == (b >> 4)) {
         const int tb = a & ~15;
         xba = leaf_inv(L, rdiag_all, tb, b - tb, a - tb);
-        xab = leaf_inv(L, rdiag_all, tb, a - tb, b - tb);
       } else if (b > a) {
         xba = L[b][a];
-      } else {
-        xab = L[a][b];
       }
     }
     Dinv[idx] = xba;
-    DinvT[idx] = xab;
   }
+  (void)DinvT;
+}
+
+// DinvT_k = Dinv_k^T for every block (grid: blocks x 64)
+__global__ void __launch_bounds__(256)
+    chol_dinv_transpose_kernel(int64_t nblk, const double *Dinv, double *DinvT) {
+  const int64_t kb = blockIdx.x / 64;
+  const int part = blockIdx.x % 64;
+  if (kb >= nblk) return;
+  const double *S = Dinv + kb * CHOL_NB * CHOL_NB;
+  double *D = DinvT + kb * CHOL_NB * CHOL_NB;
+  const int idx = part * 256 + threadIdx.x;          // 64 x 256 = NB^2
+  const int a = idx / CHOL_NB, b = idx - a * CHOL_NB;
+  D[idx] = S[b * CHOL_NB + a];
 }
 
 // forward block step: y_k = Dinv_k b_k ; b[r] -= sum_c L[r][k0+c] y_k[c] for r >= k0+nbk
@@ -488,9 +537,8 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
     const int64_t rest2 = rest - nb1;
     if (rest2 > 0) {
       double *A22 = A + (k1 + nb1) * lda + k1 + nb1;
-      VMCX_BLAS2(rocblas_dsyrk(la->h2, rocblas_fill_lower, rocblas_operation_none,
-                               (rocblas_int)rest2, nbk, &mone, Pw + nb1, ldp, &one, A22,
-                               (rocblas_int)lda));
+      const int rc = syrk_f64(rest2, nbk, Pw + nb1, ldp, 0, -1.0, 1.0, A22, lda, 1, 0, la->side);
+      if (rc) return rc;
     }
     VMCX_HIP(hipMemcpy2DAsync(P, (size_t)lda * sizeof(double), Pw, (size_t)ldp * sizeof(double),
                               (size_t)rest * sizeof(double), (size_t)nbk,
@@ -499,6 +547,9 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
     VMCX_HIP(hipEventRecord(side_done, la->side));
   }
   if (side_done) VMCX_HIP(hipStreamWaitEvent(st, side_done, 0));
+  hipLaunchKernelGGL(chol_dinv_transpose_kernel, dim3((unsigned)(nblk * 64)), dim3(256), 0, st,
+                     nblk, Dinv, DinvT);
+  VMCX_LAUNCHED();
   if (diag_ms) {
     VMCX_HIP(hipEventSynchronize(tev[2 * ndiag - 1]));
     double tot = 0.0;

@@ -58,6 +58,11 @@ int64_t sr_minres_workspace_bytes(int64_t Ns, int64_t Np);
 int sr_minres(int64_t Ns, int64_t Np, const void *O, int o_dtype, int64_t ldo, const double *E,
               double energy_mean, double diag_shift, double rtol, int maxiter, void *ws,
               int64_t ws_bytes, double *dp, double *stats, int32_t *info, hipStream_t st);
+// splits >= 2: split 0 writes C, split s >= 1 writes Crest + (s - 1) cstride
+// (Crest == NULL: C + cstride, i.e. contiguous splits)
+int syrk_f64(int64_t n, int64_t K, const double *X, int64_t ldx, int rowmajor, double alpha,
+             double beta, double *C, int64_t ldc, int splits, int64_t cstride, hipStream_t st,
+             double *Crest = nullptr);
 int64_t chol_dinv_bytes(int64_t n);
 // diag_ms != NULL: mean duration (ms) of one diagonal-block launch, timed with
 // events on `st` (the call then waits for the stream)

@@ -179,16 +179,17 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// T = sum over parts (lower triangle, column-major) + shift on the diagonal
+// T += sum over the partial triangles (lower, column-major) + shift on the
+// diagonal
 __global__ void __launch_bounds__(256)
-    sum_parts_kernel(int64_t Ns, int parts, const double *__restrict__ Tp, double *T,
+    add_parts_kernel(int64_t Ns, int parts, const double *__restrict__ Tp, double *T,
                      double shift) {
   const int64_t n = Ns * Ns;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t j = idx / Ns, i = idx - j * Ns;
     if (i < j) continue;
-    double v = 0.0;
+    double v = T[idx];
     for (int p = 0; p < parts; ++p) v += Tp[(int64_t)p * n + idx];
     if (i == j) v += shift;
     T[idx] = v;
@@ -270,16 +271,18 @@ struct SrLayout {
       t2, total;
 };
 
-// FP64 Gram of a short, wide Ot (few output tiles, long reduction): split the
-// reduction into chunks, one GEMM per chunk in a strided batch (rocBLAS's
-// batched SYRK is far slower here), then sum the lower triangles.
-// Chunks until the lower triangle has ~2048 tiles of 128 x 128, at most 32.
+// FP64 Gram on the matrix-core SYRK (syrk.hip, one workgroup per 128 x 128
+// lower tile, two per CU): when the triangle has too few tiles to fill the
+// chip a few times over, the reduction is split; split 0 writes T itself and
+// splits 1.. write partial triangles that one pass adds into T.
+// Splits until ~3000 tiles (about six waves of 512 workgroups), at most 32,
+// chunks of at least 1024 parameters.
 static int gram_splits(int64_t Ns, int64_t Np) {
   const int64_t t = (Ns + 127) / 128;
   const int64_t tiles = t * (t + 1) / 2;
-  int64_t s = (2048 + tiles - 1) / tiles;
+  int64_t s = (3000 + tiles - 1) / tiles;
   if (s > 32) s = 32;
-  if (s > Np / 1024) s = Np / 1024;   // chunks of at least 1024 parameters
+  if (s > Np / 1024) s = Np / 1024;
   return s < 2 ? 1 : (int)s;
 }
 
@@ -299,7 +302,7 @@ static SrLayout sr_layout(int64_t Ns, int64_t Np, int o_dtype, int gram, int inp
   L.T = take((size_t)Ns * Ns * 8);
   L.Tlo = take(gram ? (size_t)Ns * Ns * 4 : 0);
   const int ns = gram == 0 ? gram_splits(Ns, Np) : 1;
-  L.Tpart = take(ns > 1 ? (size_t)ns * Ns * Ns * 8 : 0);
+  L.Tpart = take(ns > 1 ? (size_t)(ns - 1) * Ns * Ns * 8 : 0);
   L.partial = take((size_t)2 * R * Np * 8);
   L.mean = take((size_t)Np * 8);
   L.eps = take((size_t)Ns * 8);
@@ -514,26 +517,15 @@ static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, cons
   const double one = 1.0, zero = 0.0;
   if (gram == 0) {
     const int ns = gram_splits(Ns, Np);
-    if (ns == 1) {
-      VMCX_BLAS(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, nsi, npi,
-                              &one, (const double *)X, (int)ldx, &zero, T64, nsi));
+    double *Tp = (double *)(w + L.Tpart);
+    rc = syrk_f64(Ns, Np, (const double *)X, ldx, 1, 1.0, 0.0, T64, Ns, ns, Ns * Ns, st, Tp);
+    if (rc) return rc;
+    if (ns == 1)
       hipLaunchKernelGGL(diag_shift_kernel, dim3((unsigned)((Ns + 255) / 256)), dim3(256), 0,
                          st, Ns, T64, shift);
-    } else {
-      // chunk c covers parameters [c kc, (c+1) kc); the last one takes the rest
-      const int64_t kc = Np / ns;
-      double *Tp = (double *)(w + L.Tpart);
-      VMCX_BLAS(rocblas_dgemm_strided_batched(
-          h, rocblas_operation_transpose, rocblas_operation_none, nsi, nsi, (int)kc, &one,
-          (const double *)X, (int)ldx, (rocblas_stride)kc, (const double *)X, (int)ldx,
-          (rocblas_stride)kc, &zero, Tp, nsi, (rocblas_stride)(Ns * Ns), ns - 1));
-      const int64_t k0 = kc * (ns - 1);
-      VMCX_BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nsi, nsi,
-                              (int)(Np - k0), &one, (const double *)X + k0, (int)ldx,
-                              (const double *)X + k0, (int)ldx, &zero,
-                              Tp + (int64_t)(ns - 1) * Ns * Ns, nsi));
-      hipLaunchKernelGGL(sum_parts_kernel, dim3(4096), dim3(256), 0, st, Ns, ns, Tp, T64, shift);
-    }
+    else
+      hipLaunchKernelGGL(add_parts_kernel, dim3(4096), dim3(256), 0, st, Ns, ns - 1, Tp, T64,
+                         shift);
     VMCX_LAUNCHED();
   } else {
     const float onef = 1.0f, zerof = 0.0f;

@@ -265,6 +265,29 @@ def sgd_update(param, dp, lr):
                                 stream_ptr(param.device)))
 
 
+def syrk_f64(M, C, alpha=1.0, beta=0.0, splits=1, rowmajor=True):
+    """C = beta C + alpha M M^T on the lower triangle of C (matrix cores).
+    rowmajor=True: M is a row-major [n, K] tensor (row stride >= K);
+    rowmajor=False: M is the column-major n x K matrix stored as a row-major
+    [K, n] tensor (column k of M is row k).  C: column-major n x n, i.e. a
+    row-major [n, n] tensor holding C^T; with splits > 1, C is [splits, n, n]
+    and split s reduces the s-th chunk of k into C[s]."""
+    if not isinstance(M, torch.Tensor) or M.device.type != "cuda" or M.dtype != F64:
+        raise TypeError("M must be a float64 HIP tensor")
+    if M.dim() != 2 or M.stride(1) != 1 or M.stride(0) < M.shape[1]:
+        raise ValueError("M must be a 2-D tensor with unit column stride (a row slice "
+                         "of a wider tensor is allowed)")
+    n, K = (M.shape[0], M.shape[1]) if rowmajor else (M.shape[1], M.shape[0])
+    shape = (n, n) if splits == 1 else (splits, n, n)
+    require(C, F64, "C", shape, M.device)
+    if not C.is_contiguous():
+        raise ValueError("C must be contiguous")
+    check(lib().vmcx_syrk_f64(n, K, ptr(M), M.stride(0), 1 if rowmajor else 0, float(alpha),
+                              float(beta), ptr(C), n, int(splits), n * n,
+                              stream_ptr(M.device)))
+    return C
+
+
 # ------------------------------------------------------------------ NNBF
 def nnbf_struct(n_sites, n_f, hidden, act, gemm, bf_scale, w32, wbf):
     """Fill the vmcx_nnbf_model struct from FP32 (and BF16) device copies.
