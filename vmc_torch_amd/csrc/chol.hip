@@ -1,5 +1,5 @@
 // Blocked FP64 Cholesky factorisation and triangular solves for the SR
-// system (column-major n x n, lower triangle).
+// system (column-major n x n, lower triangle of A as input).
 //
 // Factorisation, right-looking with block NB = 128:
 //   for each diagonal block k:
@@ -7,10 +7,11 @@
 //                         Dinv_k = L_kk^-1 (kept for the solves; L_kk itself
 //                         is never written back: every later use of the
 //                         diagonal block goes through Dinv_k)
-//     rocblas_dtrsm     : A_ik <- A_ik L_kk^-T          (panel below)
-//     rocblas_dsyrk     : A_22 <- A_22 - P P^T          (trailing update)
-// The rocSOLVER panel kernel is latency bound at this size; the diagonal
-// block here stays in LDS for all NB steps.
+//     GEMM with Dinv_k  : W_k = A_k,rest L_kk^-T        (panel below, into W)
+//     GEMM / SYRK       : A_rest <- A_rest - W W^T      (trailing update)
+// The factor is Dinv (diagonal blocks, inverted) plus W (off-diagonal
+// blocks); A is left as scratch.  The rocSOLVER panel kernel is latency
+// bound at this size; the diagonal block here stays in LDS for all NB steps.
 //
 // Solves use the inverted diagonal blocks so that each block step is one
 // launch: forward  y_k = Dinv_k (b_k) then b_rest -= L_rest,k y_k;
@@ -426,10 +427,16 @@ __global__ void __launch_bounds__(256)
 
 __global__ void zero_i32(int32_t *p) { p[0] = 0; }
 
-// [Dinv blocks | DinvT blocks | two n x NB panel buffers]
+// [Dinv blocks | DinvT blocks | W: n x n column-major, the strictly lower
+//  off-diagonal blocks of L (W[c*n + r] = L[r][c] for r in a later block)]
 int64_t chol_dinv_bytes(int64_t n) {
   const int64_t nblk = (n + CHOL_NB - 1) / CHOL_NB;
-  return (2 * nblk * CHOL_NB * CHOL_NB + 2 * n * CHOL_NB) * (int64_t)sizeof(double);
+  return (2 * nblk * CHOL_NB * CHOL_NB + n * n) * (int64_t)sizeof(double);
+}
+
+const double *chol_panels(int64_t n, const double *dinv) {
+  const int64_t nblk = (n + CHOL_NB - 1) / CHOL_NB;
+  return dinv + 2 * nblk * CHOL_NB * CHOL_NB;
 }
 
 #define VMCX_BLAS2(call)                                                      \
@@ -445,7 +452,6 @@ int64_t chol_dinv_bytes(int64_t n) {
 // Side stream, rocBLAS handle and event pool for the look-ahead (per device).
 struct LookAhead {
   hipStream_t side = nullptr;
-  rocblas_handle h2 = nullptr;
   std::vector<hipEvent_t> ev;
 };
 
@@ -459,9 +465,6 @@ static LookAhead *lookahead_for_device(int nev) {
   if (la == nullptr) {
     la = new LookAhead();
     if (hipStreamCreateWithFlags(&la->side, hipStreamNonBlocking) != hipSuccess) return nullptr;
-    if (rocblas_create_handle(&la->h2) != rocblas_status_success) return nullptr;
-    rocblas_set_stream(la->h2, la->side);
-    rocblas_set_pointer_mode(la->h2, rocblas_pointer_mode_host);
     per_dev[dev] = la;
   }
   while ((int)la->ev.size() < nev) {
@@ -472,23 +475,32 @@ static LookAhead *lookahead_for_device(int nev) {
   return la;
 }
 
-// Right-looking blocked Cholesky with one-block look-ahead:
-//   main stream : diag_k (factor + inverse) -> Pw = A_k,rest Dinv_k^T (one GEMM)
-//                 -> [wait side stream of step k-1] -> next-column GEMM_k
-//   side stream : [wait Pw] -> SYRK of the trailing matrix minus the next block
-//                 column -> copy Pw into the factor
-// so the large trailing updates overlap the latency-bound diagonal
-// factorisation and panel of the next block.  The panel buffer alternates
-// between two copies: step k+1 writes the one the side stream of step k-1
-// read, which the main stream has waited for at step k.
+// Right-looking blocked Cholesky with look-ahead and grouped trailing updates.
+// The factor's off-diagonal blocks go to W (a separate n x n buffer), so the
+// panel product needs no copy back and G = 2 consecutive panels sit side by
+// side as one n x 2NB operand:
+//   main stream : diag_k (factor + inverse) -> W_k = A_k,rest Dinv_k^T (GEMM)
+//                 -> next block column k+1 -= W_(E,k] W_(E,k]^T (GEMM over the
+//                 panels no trailing update has applied to it yet)
+//   side stream : after the last panel e of each group, one matrix-core SYRK
+//                 of the trailing matrix beyond block column e + G with the
+//                 group's G panels (K = G NB)
+// A trailing update therefore has G main-stream steps to finish before the
+// main stream needs the columns it writes, and A is swept once per G panels.
+// Block column c gets panel j from the SYRK of j's group when that group
+// ends at e <= c - 1 - G, otherwise from the main stream's GEMM at step c-1;
+// before that GEMM the main stream waits for the SYRK of group end
+// E(c-1) = the last e <= c - 1 - G (earlier ones precede it on the side
+// stream).  Regions written concurrently are disjoint: the SYRK of e writes
+// block columns >= e + 1 + G, the main stream up to then only columns <= e + G.
 int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
                 int32_t *info, hipStream_t st, float *diag_ms) {
+  constexpr int64_t G = 2;
   rocblas_handle h = (rocblas_handle)handle;
   const int64_t nblk = (n + CHOL_NB - 1) / CHOL_NB;
   double *Dinv = dinv, *DinvT = dinv + nblk * CHOL_NB * CHOL_NB;
-  double *Pbuf[2] = {dinv + 2 * nblk * CHOL_NB * CHOL_NB,
-                     dinv + 2 * nblk * CHOL_NB * CHOL_NB + n * CHOL_NB};
-  const int ldp = (int)n;
+  double *W = dinv + 2 * nblk * CHOL_NB * CHOL_NB;
+  const int64_t ldw = n;
   LookAhead *la = lookahead_for_device((int)(2 * nblk + 2));
   VMCX_CHECK_ARG(la != nullptr, "look-ahead stream / handle creation failed");
   hipLaunchKernelGGL(zero_i32, dim3(1), dim3(1), 0, st, info);
@@ -497,7 +509,8 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
   // side stream starts after everything already queued on the main stream
   VMCX_HIP(hipEventRecord(la->ev[0], st));
   VMCX_HIP(hipStreamWaitEvent(la->side, la->ev[0], 0));
-  hipEvent_t side_done = nullptr;  // side-stream work of the previous step
+  int64_t last_syrk = -1;    // last group end with a SYRK queued
+  int64_t waited = -1;       // last group end the main stream waited for
   // optional timing of every diagonal-block launch (events on the main stream)
   std::vector<hipEvent_t> tev;
   if (diag_ms) {
@@ -515,38 +528,41 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
     VMCX_LAUNCHED();
     if (diag_ms) VMCX_HIP(hipEventRecord(tev[2 * kb + 1], st));
     ++ndiag;
-    const int64_t rest = n - k0 - nbk;
-    if (rest <= 0) break;
-    double *P = A + k0 * lda + k0 + nbk;
-    double *Pw = Pbuf[kb & 1];
-    // Pw = P L_kk^-T  (Dk = L_kk^-1, column-major NB x NB, zero above the diagonal)
-    VMCX_BLAS2(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose,
-                             (rocblas_int)rest, nbk, nbk, &one, P, (rocblas_int)lda, Dk,
-                             CHOL_NB, &zero, Pw, ldp));
-    hipEvent_t panel_done = la->ev[1 + 2 * kb];
-    VMCX_HIP(hipEventRecord(panel_done, st));
-    // next block column: A[k1:, k1:k1+nb1] -= Pw Pw[0:nb1]^T (main stream)
     const int64_t k1 = k0 + nbk;
-    const int nb1 = (int)((n - k1 < CHOL_NB) ? n - k1 : CHOL_NB);
-    if (side_done) VMCX_HIP(hipStreamWaitEvent(st, side_done, 0));
+    const int64_t rest = n - k1;
+    if (rest <= 0) break;
+    // W_k = P L_kk^-T  (Dk = L_kk^-1, column-major NB x NB, zero above the diagonal)
     VMCX_BLAS2(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose,
-                             (rocblas_int)rest, nb1, nbk, &mone, Pw, ldp, Pw, ldp, &one,
-                             A + k1 * lda + k1, (rocblas_int)lda));
-    // side stream: the rest of the trailing matrix, then the panel into the factor
-    VMCX_HIP(hipStreamWaitEvent(la->side, panel_done, 0));
-    const int64_t rest2 = rest - nb1;
-    if (rest2 > 0) {
-      double *A22 = A + (k1 + nb1) * lda + k1 + nb1;
-      const int rc = syrk_f64(rest2, nbk, Pw + nb1, ldp, 0, -1.0, 1.0, A22, lda, 1, 0, la->side);
-      if (rc) return rc;
+                             (rocblas_int)rest, nbk, nbk, &one, A + k0 * lda + k1,
+                             (rocblas_int)lda, Dk, CHOL_NB, &zero, W + k0 * ldw + k1,
+                             (rocblas_int)ldw));
+    // next block column k+1: the panels after the last group end E <= kb - G
+    const int nb1 = (int)((rest < CHOL_NB) ? rest : CHOL_NB);
+    const int64_t E = (kb - 2 * G + 1 >= 0) ? ((kb - 2 * G + 1) / G) * G + G - 1 : -1;
+    if (E >= 0 && E > waited) {
+      VMCX_HIP(hipStreamWaitEvent(st, la->ev[2 + 2 * E], 0));
+      waited = E;
     }
-    VMCX_HIP(hipMemcpy2DAsync(P, (size_t)lda * sizeof(double), Pw, (size_t)ldp * sizeof(double),
-                              (size_t)rest * sizeof(double), (size_t)nbk,
-                              hipMemcpyDeviceToDevice, la->side));
-    side_done = la->ev[2 + 2 * kb];
-    VMCX_HIP(hipEventRecord(side_done, la->side));
+    const int64_t p0 = (E + 1) * CHOL_NB;
+    VMCX_BLAS2(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose,
+                             (rocblas_int)rest, nb1, (rocblas_int)(k1 - p0), &mone,
+                             W + p0 * ldw + k1, (rocblas_int)ldw, W + p0 * ldw + k1,
+                             (rocblas_int)ldw, &one, A + k1 * lda + k1, (rocblas_int)lda));
+    // group end: the group's trailing update beyond block column kb + G
+    if ((kb + 1) % G == 0) {
+      const int64_t s0 = (kb + 1 + G) * CHOL_NB;
+      if (s0 < n) {
+        VMCX_HIP(hipEventRecord(la->ev[1 + 2 * kb], st));
+        VMCX_HIP(hipStreamWaitEvent(la->side, la->ev[1 + 2 * kb], 0));
+        const int rc = syrk_f64(n - s0, G * CHOL_NB, W + (kb + 1 - G) * CHOL_NB * ldw + s0, ldw,
+                                0, -1.0, 1.0, A + s0 * lda + s0, lda, 1, 0, la->side);
+        if (rc) return rc;
+        VMCX_HIP(hipEventRecord(la->ev[2 + 2 * kb], la->side));
+        last_syrk = kb;
+      }
+    }
   }
-  if (side_done) VMCX_HIP(hipStreamWaitEvent(st, side_done, 0));
+  if (last_syrk > waited) VMCX_HIP(hipStreamWaitEvent(st, la->ev[2 + 2 * last_syrk], 0));
   hipLaunchKernelGGL(chol_dinv_transpose_kernel, dim3((unsigned)(nblk * 64)), dim3(256), 0, st,
                      nblk, Dinv, DinvT);
   VMCX_LAUNCHED();

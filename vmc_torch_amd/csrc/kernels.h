@@ -64,6 +64,8 @@ int syrk_f64(int64_t n, int64_t K, const double *X, int64_t ldx, int rowmajor, d
              double beta, double *C, int64_t ldc, int splits, int64_t cstride, hipStream_t st,
              double *Crest = nullptr);
 int64_t chol_dinv_bytes(int64_t n);
+// the off-diagonal blocks of the factor (n x n, ld n) inside the dinv workspace
+const double *chol_panels(int64_t n, const double *dinv);
 // diag_ms != NULL: mean duration (ms) of one diagonal-block launch, timed with
 // events on `st` (the call then waits for the stream)
 int chol_factor(void *rocblas_handle_, int64_t n, double *A, int64_t lda,

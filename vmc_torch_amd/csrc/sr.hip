@@ -546,7 +546,7 @@ static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, cons
   rc = chol_factor((void *)h, Ns, T64, Ns, dinv, iinfo, st, phase_ms ? phase_ms + 5 : nullptr);
   if (rc) return rc;
   timer.mark();
-  rc = chol_solve(Ns, T64, Ns, dinv, eps, alpha, t1, t2, st);
+  rc = chol_solve(Ns, chol_panels(Ns, dinv), Ns, dinv, eps, alpha, t1, t2, st);
   if (rc) return rc;
   // dp = Ot^T alpha and u = Ot^T eps (= the energy gradient, for ||g||) from
   // one read of Ot
@@ -566,7 +566,7 @@ static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, cons
     if (rc) return rc;
     hipLaunchKernelGGL(residual2_kernel, dim3(gv), dim3(256), 0, st, Ns, eps, q, alpha, shift, r);
     VMCX_LAUNCHED();
-    rc = chol_solve(Ns, T64, Ns, dinv, r, z, t1, t2, st);
+    rc = chol_solve(Ns, chol_panels(Ns, dinv), Ns, dinv, r, z, t1, t2, st);
     if (rc) return rc;
     VMCX_HIP(hipMemcpyAsync(pv, z, Ns * 8, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, r, z, sc + 0);
@@ -587,7 +587,7 @@ static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, cons
       hipLaunchKernelGGL(pcg_check, dim3(1), dim3(1), 0, st, tol, sc);
       VMCX_LAUNCHED();
       if (it + 1 == max_iter) break;
-      rc = chol_solve(Ns, T64, Ns, dinv, r, z, t1, t2, st, done);
+      rc = chol_solve(Ns, chol_panels(Ns, dinv), Ns, dinv, r, z, t1, t2, st, done);
       if (rc) return rc;
       hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, r, z, sc + 2);
       hipLaunchKernelGGL(pcg_update_p, dim3(gv), dim3(256), 0, st, Ns, pv, z, sc);
