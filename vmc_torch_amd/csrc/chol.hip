@@ -372,57 +372,143 @@ __global__ void __launch_bounds__(256)
   D[idx] = S[b * CHOL_NB + a];
 }
 
-// forward block step: y_k = Dinv_k b_k ; b[r] -= sum_c L[r][k0+c] y_k[c] for r >= k0+nbk
-__global__ void __launch_bounds__(256)
-    trsv_fwd_kernel(const double *L, int64_t lda, int64_t n, int64_t k0, int nbk,
-                    const double *Dinv, double *b, double *y, const double *skip) {
+// ---------------------------------------------------------------- solves
+// Each solve is a chain of one launch per diagonal block, so the launches are
+// built for latency.  Workgroups of 1024 threads hold a 128 x 128 block as
+// 128 rows x 8 groups of 16, so every thread issues its 16 (+16) loads at once
+// (one memory round trip), and workgroup 0 of each launch also forms the
+// next block's solution, so no launch repeats a diagonal-block product.
+constexpr int TS_THREADS = 1024;
+constexpr int TS_G = TS_THREADS / CHOL_NB;  // 8 groups of 16
+constexpr int TS_W = CHOL_NB / TS_G;        // 16
+
+// Share of thread t in z = D v (D column-major NB x NB, D[m*NB + i] is
+// element (i, m)): row i = t % NB, m in [16 (t / NB), +16).
+struct TsDiag {
+  double d[TS_W];
+  __device__ __forceinline__ void load(const double *D, int nb, int tid) {
+    const int i = tid % CHOL_NB, m0 = (tid / CHOL_NB) * TS_W;
+#pragma unroll
+    for (int m = 0; m < TS_W; ++m)
+      d[m] = (i < nb && m0 + m < nb) ? D[(m0 + m) * CHOL_NB + i] : 0.0;
+  }
+  // z[i] = sum_m D[i][m] v[m]  (v, z, part in LDS; part: TS_G x NB)
+  __device__ __forceinline__ void apply(const double *v, double *z, double (*part)[CHOL_NB],
+                                        int tid) const {
+    const int i = tid % CHOL_NB, g = tid / CHOL_NB, m0 = g * TS_W;
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < TS_W; ++m) s = fma(d[m], v[m0 + m], s);
+    part[g][i] = s;
+    __syncthreads();
+    if (tid < CHOL_NB) {
+      double t = 0.0;
+#pragma unroll
+      for (int q = 0; q < TS_G; ++q) t += part[q][tid];
+      z[tid] = t;
+    }
+    __syncthreads();
+  }
+};
+
+// out = D v for one block (the first block of the forward solve, the last of
+// the backward one)
+__global__ void __launch_bounds__(TS_THREADS)
+    trsv_head_kernel(const double *D, const double *v, int nb, double *out, const double *skip) {
   if (skip && skip[0] != 0.0) return;
-  __shared__ double z[CHOL_NB];
+  __shared__ double vin[CHOL_NB], z[CHOL_NB], part[TS_G][CHOL_NB];
   const int tid = threadIdx.x;
-  if (tid < CHOL_NB) {
-    double s = 0.0;
-    if (tid < nbk)
-#pragma unroll 16
-      for (int m = 0; m < nbk; ++m) s += Dinv[m * CHOL_NB + tid] * b[k0 + m];
-    z[tid] = s;
-  }
+  TsDiag ds;
+  ds.load(D, nb, tid);
+  if (tid < CHOL_NB) vin[tid] = tid < nb ? v[tid] : 0.0;
   __syncthreads();
-  if (blockIdx.x == 0 && tid < nbk) y[k0 + tid] = z[tid];
-  const int64_t r = k0 + nbk + (int64_t)blockIdx.x * blockDim.x + tid;
-  if (r < n) {
-    double s = 0.0;
-#pragma unroll 16
-    for (int c = 0; c < nbk; ++c) s += L[(k0 + c) * lda + r] * z[c];
-    b[r] -= s;
-  }
+  ds.apply(vin, z, part, tid);
+  if (tid < nb) out[tid] = z[tid];
 }
 
-// backward block step: x_k = Dinv_k^T y_k ; y[c'] -= sum_r L[k0+r][c'] x_k[r] for c' < k0
-// one wave per output column c'
-__global__ void __launch_bounds__(256)
-    trsv_bwd_kernel(const double *L, int64_t lda, int64_t k0, int nbk,
-                    const double *DinvT, double *yv, double *x, const double *skip) {
+// Forward step k (y_k final in y):  b[r] -= sum_c L[r][k0+c] y_k[c] for rows r
+// >= k1 = k0 + nbk.  Workgroup 0 takes block k+1 (rows [k1, k1+nb1)) and
+// forms y_{k+1} = Dinv_{k+1} b_{k+1} from it; workgroups g >= 1 take rows
+// k1 + nb1 + 128 (g - 1) + [0, 128).  Thread t: row t % 128, columns
+// [16 (t / 128), +16) (lanes on consecutive rows: coalesced column reads).
+__global__ void __launch_bounds__(TS_THREADS)
+    trsv_fwd_kernel(const double *L, int64_t lda, int64_t n, int64_t k0, int nbk, int nb1,
+                    const double *Dnext, double *b, double *y, const double *skip) {
   if (skip && skip[0] != 0.0) return;
-  __shared__ double z[CHOL_NB];
-  const int tid = threadIdx.x;
-  if (tid < CHOL_NB) {
-    double s = 0.0;
-    if (tid < nbk)
-#pragma unroll 16
-      for (int m = 0; m < nbk; ++m) s += DinvT[m * CHOL_NB + tid] * yv[k0 + m];
-    z[tid] = s;  // z[r] = sum_m (L^-1)[m][r] y[m]
-  }
+  __shared__ double z[CHOL_NB], bn[CHOL_NB], part[TS_G][CHOL_NB];
+  const int tid = threadIdx.x, i = tid % CHOL_NB, g = tid / CHOL_NB, c0 = g * TS_W;
+  const bool lead = blockIdx.x == 0;
+  const int64_t k1 = k0 + nbk;
+  const int64_t r = lead ? k1 + i : k1 + nb1 + (int64_t)(blockIdx.x - 1) * CHOL_NB + i;
+  const bool row_ok = lead ? (i < nb1) : (r < n);
+  double lv[TS_W];
+#pragma unroll
+  for (int c = 0; c < TS_W; ++c)
+    lv[c] = (row_ok && c0 + c < nbk) ? L[(k0 + c0 + c) * lda + r] : 0.0;
+  TsDiag ds;
+  if (lead) ds.load(Dnext, nb1, tid);
+  double bi = 0.0;
+  if (g == 0 && row_ok) bi = b[r];
+  if (tid < CHOL_NB) z[tid] = tid < nbk ? y[k0 + tid] : 0.0;
   __syncthreads();
-  if (blockIdx.x == 0 && tid < nbk) x[k0 + tid] = z[tid];
-  const int lane = tid & 63, wv = tid >> 6;
-  const int64_t cp = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
-  if (cp < k0) {
-    const double *col = L + cp * lda + k0;
-    double s = 0.0;
-    for (int r = lane; r < nbk; r += 64) s += col[r] * z[r];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-    if (lane == 0) yv[cp] -= s;
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < TS_W; ++c) s = fma(lv[c], z[c0 + c], s);
+  part[g][i] = s;
+  __syncthreads();
+  double t = 0.0;
+  if (g == 0) {
+#pragma unroll
+    for (int q = 0; q < TS_G; ++q) t += part[q][i];
   }
+  if (!lead) {
+    if (g == 0 && row_ok) b[r] = bi - t;
+    return;
+  }
+  if (g == 0) bn[i] = row_ok ? bi - t : 0.0;
+  __syncthreads();
+  ds.apply(bn, z, part, tid);
+  if (tid < nb1) y[k1 + tid] = z[tid];
+}
+
+// Backward step k (x_k final in x):  y[c] -= sum_r L[k0+r][c] x_k[r] for
+// columns c < k0.  Workgroup 0 takes block k-1 (columns [k0-128, k0)) and
+// forms x_{k-1} = Dinv_{k-1}^T y_{k-1} from it; workgroups g >= 1 take
+// columns 128 (g - 1) + [0, 128).  Thread t: column t / 8, rows
+// [16 (t % 8), +16) (eight lanes read one column's 128 contiguous rows).
+__global__ void __launch_bounds__(TS_THREADS)
+    trsv_bwd_kernel(const double *L, int64_t lda, int64_t k0, int nbk, const double *DTprev,
+                    double *yv, double *x, const double *skip) {
+  if (skip && skip[0] != 0.0) return;
+  __shared__ double z[CHOL_NB], yn[CHOL_NB], part[TS_G][CHOL_NB];
+  const int tid = threadIdx.x, cl = tid / TS_G, rg = tid % TS_G, r0 = rg * TS_W;
+  const bool lead = blockIdx.x == 0;
+  const int64_t c = lead ? k0 - CHOL_NB + cl : (int64_t)(blockIdx.x - 1) * CHOL_NB + cl;
+  const bool col_ok = lead ? true : (c < k0 - CHOL_NB);
+  double lv[TS_W];
+  const double *col = L + c * lda + k0 + r0;
+#pragma unroll
+  for (int j = 0; j < TS_W; ++j) lv[j] = (col_ok && r0 + j < nbk) ? col[j] : 0.0;
+  TsDiag ds;
+  if (lead) ds.load(DTprev, CHOL_NB, tid);
+  double yc = 0.0;
+  if (rg == 0 && col_ok) yc = yv[c];
+  if (tid < CHOL_NB) z[tid] = tid < nbk ? x[k0 + tid] : 0.0;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < TS_W; ++j) s = fma(lv[j], z[r0 + j], s);
+  // the eight row groups of a column are eight consecutive lanes
+#pragma unroll
+  for (int o = TS_G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (!lead) {
+    if (rg == 0 && col_ok) yv[c] = yc - s;
+    return;
+  }
+  if (rg == 0) yn[cl] = yc - s;
+  __syncthreads();
+  ds.apply(yn, z, part, tid);
+  if (tid < CHOL_NB) x[k0 - CHOL_NB + tid] = z[tid];
 }
 
 __global__ void zero_i32(int32_t *p) { p[0] = 0; }
@@ -581,32 +667,57 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
 }
 
 // x = (L L^T)^-1 b ; tmp: n doubles; b is preserved
-int chol_solve(int64_t n, const double *L, int64_t lda, const double *dinv,
-               const double *b, double *x, double *tmp, double *tmp2, hipStream_t st,
-               const double *skip) {
+// Forward half: y = L^-1 b (b is overwritten).
+int chol_solve_fwd(int64_t n, const double *L, int64_t lda, const double *dinv, double *b,
+                   double *y, hipStream_t st, const double *skip) {
   const int64_t nblk = (n + CHOL_NB - 1) / CHOL_NB;
-  const double *Dinv = dinv, *DinvT = dinv + nblk * CHOL_NB * CHOL_NB;
-  VMCX_HIP(hipMemcpyAsync(tmp, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-  // forward: tmp2 = L^-1 b
-  for (int64_t kb = 0; kb < nblk; ++kb) {
+  const double *Dinv = dinv;
+  const int64_t blk = (int64_t)CHOL_NB * CHOL_NB;
+  auto nb_of = [&](int64_t kb) { return (int)((n - kb * CHOL_NB < CHOL_NB) ? n - kb * CHOL_NB : CHOL_NB); };
+  hipLaunchKernelGGL(trsv_head_kernel, dim3(1), dim3(TS_THREADS), 0, st, Dinv, b, nb_of(0), y,
+                     skip);
+  VMCX_LAUNCHED();
+  for (int64_t kb = 0; kb + 1 < nblk; ++kb) {
     const int64_t k0 = kb * CHOL_NB;
-    const int nbk = (int)((n - k0 < CHOL_NB) ? n - k0 : CHOL_NB);
-    const int64_t rest = n - k0 - nbk;
-    const unsigned grid = (unsigned)((rest + 255) / 256 > 0 ? (rest + 255) / 256 : 1);
-    hipLaunchKernelGGL(trsv_fwd_kernel, dim3(grid), dim3(256), 0, st, L, lda, n, k0, nbk,
-                       Dinv + kb * CHOL_NB * CHOL_NB, tmp, tmp2, skip);
-    VMCX_LAUNCHED();
-  }
-  // backward: x = L^-T tmp2
-  for (int64_t kb = nblk - 1; kb >= 0; --kb) {
-    const int64_t k0 = kb * CHOL_NB;
-    const int nbk = (int)((n - k0 < CHOL_NB) ? n - k0 : CHOL_NB);
-    const unsigned grid = (unsigned)((k0 + 3) / 4 > 0 ? (k0 + 3) / 4 : 1);
-    hipLaunchKernelGGL(trsv_bwd_kernel, dim3(grid), dim3(256), 0, st, L, lda, k0, nbk,
-                       DinvT + kb * CHOL_NB * CHOL_NB, tmp2, x, skip);
+    const int nbk = nb_of(kb), nb1 = nb_of(kb + 1);
+    const int64_t rest = n - k0 - nbk - nb1;
+    const unsigned grid = (unsigned)(1 + (rest + CHOL_NB - 1) / CHOL_NB);
+    hipLaunchKernelGGL(trsv_fwd_kernel, dim3(grid), dim3(TS_THREADS), 0, st, L, lda, n, k0, nbk,
+                       nb1, Dinv + (kb + 1) * blk, b, y, skip);
     VMCX_LAUNCHED();
   }
   return 0;
+}
+
+// Backward half: x = L^-T y (y is overwritten).
+int chol_solve_bwd(int64_t n, const double *L, int64_t lda, const double *dinv, double *y,
+                   double *x, hipStream_t st, const double *skip) {
+  const int64_t nblk = (n + CHOL_NB - 1) / CHOL_NB;
+  const double *DinvT = dinv + nblk * CHOL_NB * CHOL_NB;
+  const int64_t blk = (int64_t)CHOL_NB * CHOL_NB;
+  const int64_t kl = nblk - 1;
+  const int nbl = (int)(n - kl * CHOL_NB);
+  hipLaunchKernelGGL(trsv_head_kernel, dim3(1), dim3(TS_THREADS), 0, st, DinvT + kl * blk,
+                     y + kl * CHOL_NB, nbl, x + kl * CHOL_NB, skip);
+  VMCX_LAUNCHED();
+  for (int64_t kb = kl; kb >= 1; --kb) {
+    const int64_t k0 = kb * CHOL_NB;
+    const int nbk = (int)((n - k0 < CHOL_NB) ? n - k0 : CHOL_NB);
+    const unsigned grid = (unsigned)(1 + (k0 - CHOL_NB + CHOL_NB - 1) / CHOL_NB);
+    hipLaunchKernelGGL(trsv_bwd_kernel, dim3(grid), dim3(TS_THREADS), 0, st, L, lda, k0, nbk,
+                       DinvT + (kb - 1) * blk, y, x, skip);
+    VMCX_LAUNCHED();
+  }
+  return 0;
+}
+
+int chol_solve(int64_t n, const double *L, int64_t lda, const double *dinv,
+               const double *b, double *x, double *tmp, double *tmp2, hipStream_t st,
+               const double *skip) {
+  VMCX_HIP(hipMemcpyAsync(tmp, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  int rc = chol_solve_fwd(n, L, lda, dinv, tmp, tmp2, st, skip);
+  if (rc) return rc;
+  return chol_solve_bwd(n, L, lda, dinv, tmp2, x, st, skip);
 }
 
 }  // namespace vmcx

@@ -70,6 +70,11 @@ const double *chol_panels(int64_t n, const double *dinv);
 // events on `st` (the call then waits for the stream)
 int chol_factor(void *rocblas_handle_, int64_t n, double *A, int64_t lda,
                 double *dinv, int32_t *info, hipStream_t st, float *diag_ms = nullptr);
+// halves of chol_solve: y = L^-1 b (b overwritten), x = L^-T y (y overwritten)
+int chol_solve_fwd(int64_t n, const double *L, int64_t lda, const double *dinv, double *b,
+                   double *y, hipStream_t st, const double *skip = nullptr);
+int chol_solve_bwd(int64_t n, const double *L, int64_t lda, const double *dinv, double *y,
+                   double *x, hipStream_t st, const double *skip = nullptr);
 // skip != NULL and *skip != 0 (a device flag): every launch returns at once
 int chol_solve(int64_t n, const double *L, int64_t lda, const double *dinv,
                const double *b, double *x, double *tmp, double *tmp2,
