@@ -474,21 +474,21 @@ __global__ void __launch_bounds__(TS_THREADS)
 // Backward step k (x_k final in x):  y[c] -= sum_r L[k0+r][c] x_k[r] for
 // columns c < k0.  Workgroup 0 takes block k-1 (columns [k0-128, k0)) and
 // forms x_{k-1} = Dinv_{k-1}^T y_{k-1} from it; workgroups g >= 1 take
-// columns 128 (g - 1) + [0, 128).  Thread t: column t / 8, rows
-// [16 (t % 8), +16) (eight lanes read one column's 128 contiguous rows).
+// columns 128 (g - 1) + [0, 128).  Thread t: column t / 8, rows t % 8 + 8 j
+// (per load, eight consecutive lanes read 64 contiguous bytes of a column).
 __global__ void __launch_bounds__(TS_THREADS)
     trsv_bwd_kernel(const double *L, int64_t lda, int64_t k0, int nbk, const double *DTprev,
                     double *yv, double *x, const double *skip) {
   if (skip && skip[0] != 0.0) return;
   __shared__ double z[CHOL_NB], yn[CHOL_NB], part[TS_G][CHOL_NB];
-  const int tid = threadIdx.x, cl = tid / TS_G, rg = tid % TS_G, r0 = rg * TS_W;
+  const int tid = threadIdx.x, cl = tid / TS_G, rg = tid % TS_G;
   const bool lead = blockIdx.x == 0;
   const int64_t c = lead ? k0 - CHOL_NB + cl : (int64_t)(blockIdx.x - 1) * CHOL_NB + cl;
   const bool col_ok = lead ? true : (c < k0 - CHOL_NB);
   double lv[TS_W];
-  const double *col = L + c * lda + k0 + r0;
+  const double *col = L + c * lda + k0 + rg;
 #pragma unroll
-  for (int j = 0; j < TS_W; ++j) lv[j] = (col_ok && r0 + j < nbk) ? col[j] : 0.0;
+  for (int j = 0; j < TS_W; ++j) lv[j] = (col_ok && rg + TS_G * j < nbk) ? col[TS_G * j] : 0.0;
   TsDiag ds;
   if (lead) ds.load(DTprev, CHOL_NB, tid);
   double yc = 0.0;
@@ -497,7 +497,7 @@ __global__ void __launch_bounds__(TS_THREADS)
   __syncthreads();
   double s = 0.0;
 #pragma unroll
-  for (int j = 0; j < TS_W; ++j) s = fma(lv[j], z[r0 + j], s);
+  for (int j = 0; j < TS_W; ++j) s = fma(lv[j], z[rg + TS_G * j], s);
   // the eight row groups of a column are eight consecutive lanes
 #pragma unroll
   for (int o = TS_G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
