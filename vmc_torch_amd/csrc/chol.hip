@@ -537,7 +537,8 @@ const double *chol_panels(int64_t n, const double *dinv) {
 
 // Side stream, rocBLAS handle and event pool for the look-ahead (per device).
 struct LookAhead {
-  hipStream_t side = nullptr;
+  hipStream_t side = nullptr;  // trailing SYRKs
+  hipStream_t aux = nullptr;   // forward solve along the factorisation
   std::vector<hipEvent_t> ev;
 };
 
@@ -551,6 +552,7 @@ static LookAhead *lookahead_for_device(int nev) {
   if (la == nullptr) {
     la = new LookAhead();
     if (hipStreamCreateWithFlags(&la->side, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    if (hipStreamCreateWithFlags(&la->aux, hipStreamNonBlocking) != hipSuccess) return nullptr;
     per_dev[dev] = la;
   }
   while ((int)la->ev.size() < nev) {
@@ -579,15 +581,20 @@ static LookAhead *lookahead_for_device(int nev) {
 // E(c-1) = the last e <= c - 1 - G (earlier ones precede it on the side
 // stream).  Regions written concurrently are disjoint: the SYRK of e writes
 // block columns >= e + 1 + G, the main stream up to then only columns <= e + G.
+//
+// fwd_b != NULL: the forward solve y = L^-1 b (b overwritten, y = fwd_y) runs
+// on a third stream as the blocks of L appear: its step k needs W_k and
+// Dinv_{k+1}, so it follows diag_{k+1} and stays off the critical path.
 int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
-                int32_t *info, hipStream_t st, float *diag_ms) {
+                int32_t *info, hipStream_t st, float *diag_ms, double *fwd_b, double *fwd_y) {
   constexpr int64_t G = 2;
   rocblas_handle h = (rocblas_handle)handle;
   const int64_t nblk = (n + CHOL_NB - 1) / CHOL_NB;
   double *Dinv = dinv, *DinvT = dinv + nblk * CHOL_NB * CHOL_NB;
   double *W = dinv + 2 * nblk * CHOL_NB * CHOL_NB;
   const int64_t ldw = n;
-  LookAhead *la = lookahead_for_device((int)(2 * nblk + 2));
+  LookAhead *la = lookahead_for_device((int)(3 * nblk + 3));
+  hipEvent_t *evf = la ? la->ev.data() + 2 * nblk + 2 : nullptr;  // after diag_k
   VMCX_CHECK_ARG(la != nullptr, "look-ahead stream / handle creation failed");
   hipLaunchKernelGGL(zero_i32, dim3(1), dim3(1), 0, st, info);
   VMCX_LAUNCHED();
@@ -595,6 +602,7 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
   // side stream starts after everything already queued on the main stream
   VMCX_HIP(hipEventRecord(la->ev[0], st));
   VMCX_HIP(hipStreamWaitEvent(la->side, la->ev[0], 0));
+  if (fwd_b) VMCX_HIP(hipStreamWaitEvent(la->aux, la->ev[0], 0));
   int64_t last_syrk = -1;    // last group end with a SYRK queued
   int64_t waited = -1;       // last group end the main stream waited for
   // optional timing of every diagonal-block launch (events on the main stream)
@@ -614,6 +622,21 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
     VMCX_LAUNCHED();
     if (diag_ms) VMCX_HIP(hipEventRecord(tev[2 * kb + 1], st));
     ++ndiag;
+    if (fwd_b) {  // forward-solve step kb - 1 (or the head) on the third stream
+      VMCX_HIP(hipEventRecord(evf[kb], st));
+      VMCX_HIP(hipStreamWaitEvent(la->aux, evf[kb], 0));
+      if (kb == 0) {
+        hipLaunchKernelGGL(trsv_head_kernel, dim3(1), dim3(TS_THREADS), 0, la->aux, Dinv, fwd_b,
+                           nbk, fwd_y, nullptr);
+      } else {
+        const int64_t kp0 = k0 - CHOL_NB;
+        const int64_t rest_p = n - k0 - nbk;
+        hipLaunchKernelGGL(trsv_fwd_kernel, dim3((unsigned)(1 + (rest_p + CHOL_NB - 1) / CHOL_NB)),
+                           dim3(TS_THREADS), 0, la->aux, W, ldw, n, kp0, CHOL_NB, nbk, Dk, fwd_b,
+                           fwd_y, nullptr);
+      }
+      VMCX_LAUNCHED();
+    }
     const int64_t k1 = k0 + nbk;
     const int64_t rest = n - k1;
     if (rest <= 0) break;
@@ -649,6 +672,10 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
     }
   }
   if (last_syrk > waited) VMCX_HIP(hipStreamWaitEvent(st, la->ev[2 + 2 * last_syrk], 0));
+  if (fwd_b) {
+    VMCX_HIP(hipEventRecord(evf[nblk], la->aux));
+    VMCX_HIP(hipStreamWaitEvent(st, evf[nblk], 0));
+  }
   hipLaunchKernelGGL(chol_dinv_transpose_kernel, dim3((unsigned)(nblk * 64)), dim3(256), 0, st,
                      nblk, Dinv, DinvT);
   VMCX_LAUNCHED();

@@ -68,8 +68,11 @@ int64_t chol_dinv_bytes(int64_t n);
 const double *chol_panels(int64_t n, const double *dinv);
 // diag_ms != NULL: mean duration (ms) of one diagonal-block launch, timed with
 // events on `st` (the call then waits for the stream)
+// fwd_b != NULL: also y = L^-1 fwd_b into fwd_y (fwd_b overwritten), computed
+// on a third stream while the factorisation proceeds
 int chol_factor(void *rocblas_handle_, int64_t n, double *A, int64_t lda,
-                double *dinv, int32_t *info, hipStream_t st, float *diag_ms = nullptr);
+                double *dinv, int32_t *info, hipStream_t st, float *diag_ms = nullptr,
+                double *fwd_b = nullptr, double *fwd_y = nullptr);
 // halves of chol_solve: y = L^-1 b (b overwritten), x = L^-T y (y overwritten)
 int chol_solve_fwd(int64_t n, const double *L, int64_t lda, const double *dinv, double *b,
                    double *y, hipStream_t st, const double *skip = nullptr);

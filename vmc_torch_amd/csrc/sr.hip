@@ -543,10 +543,13 @@ static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, cons
   }
   timer.mark();
   // FP64 Cholesky of the (shifted) Gram
-  rc = chol_factor((void *)h, Ns, T64, Ns, dinv, iinfo, st, phase_ms ? phase_ms + 5 : nullptr);
+  // (the forward half of the first solve, L^-1 eps, runs along the factorisation)
+  VMCX_HIP(hipMemcpyAsync(t1, eps, Ns * sizeof(double), hipMemcpyDeviceToDevice, st));
+  rc = chol_factor((void *)h, Ns, T64, Ns, dinv, iinfo, st, phase_ms ? phase_ms + 5 : nullptr,
+                   t1, t2);
   if (rc) return rc;
   timer.mark();
-  rc = chol_solve(Ns, chol_panels(Ns, dinv), Ns, dinv, eps, alpha, t1, t2, st);
+  rc = chol_solve_bwd(Ns, chol_panels(Ns, dinv), Ns, dinv, t2, alpha, st);
   if (rc) return rc;
   // dp = Ot^T alpha and u = Ot^T eps (= the energy gradient, for ||g||) from
   // one read of Ot
