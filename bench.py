@@ -541,11 +541,13 @@ def main():
                 "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / PEAK_HBM}
     kern["sr.gram"] = flop_roof(gram_flops, phase_ms["gram"], peak_sr,
                                 "algorithmic flops Ns^2 Np (FP64, symmetric product) or "
-                                "2 Ns^2 Np (reduced-precision GEMM); rocBLAS SYRK, or "
-                                "split-K batched DGEMM when Ot is short and wide")
+                                "2 Ns^2 Np (reduced-precision GEMM); FP64: the matrix-core "
+                                "SYRK kernel (one workgroup per 128x128 lower tile, the "
+                                "reduction split when the triangle has few tiles) plus the "
+                                "pass that adds the split partials")
     kern["sr.factor"] = flop_roof(chol_flops, phase_ms["factor"], PEAK_FP64,
-                                  "Ns^3/3: blocked FP64 Cholesky (LDS diagonal kernel + "
-                                  "TRSM/SYRK)")
+                                  "Ns^3/3: blocked FP64 Cholesky (LDS diagonal kernel, "
+                                  "panel GEMM, matrix-core SYRK per two panels)")
     osz = run.O.element_size()
     if gram == "fp64":
         cbytes = 2.0 * Ns * Np * osz + Ns * Np * osz          # read O twice, write X
