@@ -65,6 +65,11 @@ WORKLOADS = {
 }
 
 
+# measured HBM traffic (bytes per call) of a roofline kernel:
+# profiles/pmc_traffic_sr_gram_hubbard8x8_slater.md (Gram: SYRK + partial-sum pass)
+TRAFFIC = {("hubbard8x8_slater", "sr.gram", "fp64"): 20.33e9}
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -77,6 +82,9 @@ def parse():
     p.add_argument("--tau-sweeps", type=int, default=64)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--copy-probe-mib", type=int, default=0,
+                   help="first copy a buffer of this many MiB on the device (a plain copy "
+                        "kernel of known size, to scale hardware byte counters)")
     return p.parse_args()
 
 
@@ -436,6 +444,13 @@ def main():
     N, nf = Lx * Ly, nu + nd
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if a.copy_probe_mib > 0:
+        src = torch.ones(a.copy_probe_mib << 17, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {a.copy_probe_mib} MiB device-to-device")
+        del src, dst
     H = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, nf,
                                                    n_fermions_per_spin=(nu, nd))
     H.precompute_hops_gpu(dev)
@@ -574,7 +589,10 @@ def main():
     dom = max(single, key=lambda k: kern[k]["per_step_ms"])
     roof = {kk: kern[dom][kk] for kk in ("bound", "achieved", "peak", "unit", "frac")}
     roof["kernel"] = dom
-    roof["traffic"] = None
+    # HBM bytes per call of the roofline kernel from rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs scaled by a 1 GiB copy (profiles/pmc_traffic_*.md);
+    # null where no such measurement exists for the workload and kernel
+    roof["traffic"] = TRAFFIC.get((a.workload, dom, gram))
 
     if w["kind"] == "slater":
         dtype = "f64" if gram == "fp64" else f"f64+{gram}-gram"
