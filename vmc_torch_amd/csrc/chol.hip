@@ -305,44 +305,87 @@ __global__ void __launch_bounds__(DIAG_THREADS)
     return;
   }
   // (L_kk itself is not stored: the solves and the panel use its inverse)
-  // ---- X = L^-1 in place (off-diagonal tiles; the diagonal tiles already
-  //      hold their inverses), tile column jt from the right
-  for (int jt = NT - 2; jt >= 0; --jt) {
-    const int jb = jt * LEAF, nt = NT - 1 - jt;
-    if (wave < nt) {
-      const int T = jt + 1 + wave;
-      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-      for (int K = jt + 1; K <= T; ++K) {
+  // ---- X = L^-1 in place by recursive doubling (the diagonal tiles already
+  //      hold their inverses): for block sizes h = 16, 32, 64 and each pair
+  //      (A, B) of consecutive h-blocks,  X_BA = -X_BB (L_BA X_AA).  A level
+  //      forms all tiles of P = L_BA X_AA in registers, stores them over L_BA
+  //      after a barrier, then does the same for X_BA = -X_BB P.  Levels
+  //      h = 16, 32 have 4 and 8 tiles (one per wave); at h = 64 (16 tiles)
+  //      a wave takes P(i, j) and P(i, 3 - j), then X(i, j) and X(3 - i, j):
+  //      P(i, j) sums 4 - j tile products and X(i, j) sums i + 1, so every
+  //      wave does five.
 #pragma unroll
-        for (int s4 = 0; s4 < LEAF; s4 += 4) {
-          const int kk = s4 + lk;
-          const double a = (K == T) ? leaf_inv(L, rdiag_all, LEAF * T, li, kk)
-                                    : L[LEAF * T + li][LEAF * K + kk];
-          const double b = L[LEAF * K + kk][jb + li];
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  for (int lvl = 0; lvl < 3; ++lvl) {
+    const int ht = 1 << lvl;  // tiles per half block
+#pragma unroll
+    for (int phase = 0; phase < 2; ++phase) {
+      int ti[2], tj[2], tA0[2];
+      bool on[2];
+      if (lvl < 2) {
+        const int pair = wave / (ht * ht), r = wave % (ht * ht);
+        on[0] = wave < 4 * ht;
+        on[1] = false;
+        ti[0] = ti[1] = r / ht;
+        tj[0] = tj[1] = r % ht;
+        tA0[0] = tA0[1] = 2 * ht * pair;
+      } else {
+        on[0] = on[1] = true;
+        tA0[0] = tA0[1] = 0;
+        if (phase == 0) {
+          ti[0] = ti[1] = wave / 2;
+          tj[0] = wave % 2;
+          tj[1] = 3 - wave % 2;
+        } else {
+          tj[0] = tj[1] = wave / 2;
+          ti[0] = wave % 2;
+          ti[1] = 3 - wave % 2;
         }
       }
+      dbl4 acc[2];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) W[LEAF * wave + 4 * r + lk][li] = acc[r];
-    }
-    __syncthreads();
-    if (wave < nt) {
-      const int T = jt + 1 + wave;
-      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+      for (int q = 0; q < 2; ++q) {
+        acc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
+        if (!on[q]) continue;
+        const int i = ti[q], j = tj[q], A0 = tA0[q], B0 = A0 + ht;
+        if (phase == 0) {  // P(i, j) = sum_{k >= j} L_BA(i, k) X_AA(k, j)
+          for (int k = j; k < ht; ++k) {
 #pragma unroll
-      for (int s4 = 0; s4 < LEAF; s4 += 4) {
-        const int kk = s4 + lk;
-        const double a = -W[LEAF * wave + li][kk];
-        const double b = leaf_inv(L, rdiag_all, jb, kk, li);  // X_jj[kk][li]
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            for (int s4 = 0; s4 < LEAF; s4 += 4) {
+              const int kk = s4 + lk;
+              const double a = L[LEAF * (B0 + i) + li][LEAF * (A0 + k) + kk];
+              const double b = (k == j) ? leaf_inv(L, rdiag_all, LEAF * (A0 + j), kk, li)
+                                        : L[LEAF * (A0 + k) + kk][LEAF * (A0 + j) + li];
+              acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
+            }
+          }
+        } else {  // X_BA(i, j) = -sum_{k <= i} X_BB(i, k) P(k, j)
+          for (int k = 0; k <= i; ++k) {
+#pragma unroll
+            for (int s4 = 0; s4 < LEAF; s4 += 4) {
+              const int kk = s4 + lk;
+              const double a = -((k == i) ? leaf_inv(L, rdiag_all, LEAF * (B0 + i), li, kk)
+                                          : L[LEAF * (B0 + i) + li][LEAF * (B0 + k) + kk]);
+              const double b = L[LEAF * (B0 + k) + kk][LEAF * (A0 + j) + li];
+              acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
+            }
+          }
+        }
       }
+      __syncthreads();
 #pragma unroll
-      for (int r = 0; r < 4; ++r) L[LEAF * T + 4 * r + lk][jb + li] = acc[r];
+      for (int q = 0; q < 2; ++q) {
+        if (!on[q]) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          L[LEAF * (tA0[q] + ht + ti[q]) + 4 * r + lk][LEAF * (tA0[q] + tj[q]) + li] = acc[q][r];
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
   // Dinv[a*NB + b] = X[b][a]  (DinvT, its transpose, is written after the
-  // factorisation by chol_dinv_transpose_kernel, off the critical path)
+  // factorisation by chol_dinv_transpose_kernel, off the critical path);
+  // unrolled so that the LDS reads of eight stores are in flight together
+#pragma unroll 8
   for (int idx = tid; idx < CHOL_NB * CHOL_NB; idx += DIAG_THREADS) {
     const int a = idx / CHOL_NB, b = idx - a * CHOL_NB;
     double xba = 0.0;
