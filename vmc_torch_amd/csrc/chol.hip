@@ -74,10 +74,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // 8x8 Cholesky and inverse of the lower triangle stored packed in a[]
-// (a[i(i+1)/2 + j], j <= i), all in one lane's registers: a becomes L,
-// x = L^-1 (same packing), rs[i] = 1 / L_ii; returns the first non-positive
-// pivot (1-based) or 0.
-__device__ __forceinline__ int chol8_inv8(double *a, double *x, double *rs) {
+// (a[i(i+1)/2 + j], j <= i), in one lane's registers: a becomes L, rs[i] =
+// 1 / L_ii, and X = L^-1 is formed one column at a time (eight live values)
+// and stored transposed through T: T[o + c][o + i] = X[i][c] for i > c.
+// Returns the first non-positive pivot (1-based) or 0.
+__device__ __forceinline__ int chol8_inv8(double *a, double *rs, double (*T)[LEAF + 1], int o) {
   int badj = 0;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -98,17 +99,36 @@ __device__ __forceinline__ int chol8_inv8(double *a, double *x, double *rs) {
                                      a[i * (i + 1) / 2 + l]);
   }
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    x[i * (i + 1) / 2 + i] = rs[i];
+  for (int c = 0; c < 8; ++c) {
+    double x[8];
+    x[c] = rs[c];
 #pragma unroll
-    for (int c = 0; c < i; ++c) {
+    for (int i = c + 1; i < 8; ++i) {
       double acc = 0.0;
 #pragma unroll
-      for (int q = c; q < i; ++q) acc = fma(a[i * (i + 1) / 2 + q], x[q * (q + 1) / 2 + c], acc);
-      x[i * (i + 1) / 2 + c] = -rs[i] * acc;
+      for (int q = c; q < i; ++q) acc = fma(a[i * (i + 1) / 2 + q], x[q], acc);
+      x[i] = -rs[i] * acc;
+      T[o + c][o + i] = x[i];
     }
   }
   return badj;
+}
+
+// The 128 x 128 block lives in LDS as its 36 lower 16 x 16 tiles (rows padded
+// to 17): 78 KB, so the kernel fits on a CU beside one workgroup of the
+// trailing-update SYRK (74 KB) instead of waiting for a whole CU.  Tile
+// (I, J), I >= J, is Lt[I (I + 1) / 2 + J]; a diagonal tile's upper triangle
+// holds the strictly lower part of its leaf inverse, transposed.
+constexpr int LTP = LEAF + 1;
+typedef double LTile[LEAF][LTP];
+constexpr int NLT = (CHOL_NB / LEAF) * (CHOL_NB / LEAF + 1) / 2;
+__device__ __forceinline__ double &lt_at(LTile *Lt, int i, int j) {
+  const int I = i >> 4, J = j >> 4;
+  return Lt[I * (I + 1) / 2 + J][i & (LEAF - 1)][j & (LEAF - 1)];
+}
+__device__ __forceinline__ const double &lt_at(const LTile *Lt, int i, int j) {
+  const int I = i >> 4, J = j >> 4;
+  return Lt[I * (I + 1) / 2 + J][i & (LEAF - 1)][j & (LEAF - 1)];
 }
 
 // Leaf of step kb (wave 0): factor and invert the 16 x 16 tile at (kb, kb)
@@ -117,25 +137,23 @@ __device__ __forceinline__ int chol8_inv8(double *a, double *x, double *rs) {
 // lane);  X21 = -X22 L21 X11 (64 lanes, via Y = L21 X11 in `scr`).  L stays
 // in the lower triangle of the tile; the strictly lower part of
 // X = L^-1 goes transposed into its upper triangle; rdiag_all = 1 / L_ii.
-__device__ __forceinline__ void chol_leaf(double (*L)[CHOL_NB + 1], double *rdiag_all, int kb,
+__device__ __forceinline__ void chol_leaf(LTile *L, double *rdiag_all, int kb,
                                           int lane, int *bad, double *scr) {
-  double (*T)[CHOL_NB + 1] = reinterpret_cast<double (*)[CHOL_NB + 1]>(&L[kb][kb]);
+  double (*T)[LTP] = L[(kb >> 4) * ((kb >> 4) + 1) / 2 + (kb >> 4)];
   // ---- 1: L11, X11
   if (lane == 0) {
-    double a[36], x[36], rs[8];
+    double a[36], rs[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
       for (int j = 0; j <= i; ++j) a[i * (i + 1) / 2 + j] = T[i][j];
-    const int bj = chol8_inv8(a, x, rs);
+    const int bj = chol8_inv8(a, rs, T, 0);
     if (bj && *bad == 0) *bad = kb + bj;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       rdiag_all[kb + i] = rs[i];
 #pragma unroll
       for (int j = 0; j <= i; ++j) T[i][j] = a[i * (i + 1) / 2 + j];
-#pragma unroll
-      for (int c = 0; c < i; ++c) T[c][i] = x[i * (i + 1) / 2 + c];
     }
   }
   wave_lds_sync();
@@ -169,20 +187,18 @@ __device__ __forceinline__ void chol_leaf(double (*L)[CHOL_NB + 1], double *rdia
   wave_lds_sync();
   // ---- 4: L22, X22
   if (lane == 0) {
-    double a[36], x[36], rs[8];
+    double a[36], rs[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
       for (int j = 0; j <= i; ++j) a[i * (i + 1) / 2 + j] = T[8 + i][8 + j];
-    const int bj = chol8_inv8(a, x, rs);
+    const int bj = chol8_inv8(a, rs, T, 8);
     if (bj && *bad == 0) *bad = kb + 8 + bj;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       rdiag_all[kb + 8 + i] = rs[i];
 #pragma unroll
       for (int j = 0; j <= i; ++j) T[8 + i][8 + j] = a[i * (i + 1) / 2 + j];
-#pragma unroll
-      for (int c = 0; c < i; ++c) T[8 + c][8 + i] = x[i * (i + 1) / 2 + c];
     }
   }
   wave_lds_sync();
@@ -206,19 +222,22 @@ __device__ __forceinline__ void chol_leaf(double (*L)[CHOL_NB + 1], double *rdia
 
 // X_tt (lower, the leaf inverse of tile t at kb) as a matrix-core B operand:
 // element [k][j] of X_tt^T (transpose = 0) or of X_tt (transpose = 1)
-__device__ __forceinline__ double leaf_inv(const double (*L)[CHOL_NB + 1],
+__device__ __forceinline__ double leaf_inv(const LTile *L,
                                           const double *rdiag_all, int kb, int row, int col) {
   // X_tt[row][col]
   if (row == col) return rdiag_all[kb + row];
-  return (row > col) ? L[kb + col][kb + row] : 0.0;
+  return (row > col) ? lt_at(L, kb + col, kb + row) : 0.0;
 }
 
-__global__ void __launch_bounds__(DIAG_THREADS)
+__global__ void __launch_bounds__(DIAG_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4)))
     chol_diag_kernel(double *A, int64_t lda, int64_t k0, int nbk, int32_t *info,
                      double *Dinv, double *DinvT) {
-  __shared__ double L[CHOL_NB][CHOL_NB + 1];  // L[i][j], row i, column j
-  __shared__ double W[CHOL_NB][LEAF + 1];     // inverse phase: W_T tiles
-  __shared__ double rdiag_all[CHOL_NB];       // 1 / L[i][i]
+  // the factorisation's critical path: its waves go first when the CU is
+  // shared with trailing-update workgroups
+  __builtin_amdgcn_s_setprio(3);
+  __shared__ LTile L[NLT];                    // lower tiles of the block
+  __shared__ double W[64];                    // leaf scratch
+  __shared__ double rdiag_all[CHOL_NB];       // 1 / lt_at(L, i, i)
   __shared__ int bad;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -242,12 +261,12 @@ __global__ void __launch_bounds__(DIAG_THREADS)
     for (int q = 0; q < CHOL_NB * CHOL_NB / DIAG_THREADS; ++q) {
       const int idx = tid + q * DIAG_THREADS;
       const int j = idx / CHOL_NB, i = idx - j * CHOL_NB;
-      L[i][j] = v[q];
+      if ((i >> 4) >= (j >> 4)) lt_at(L, i, j) = v[q];  // lower tiles only
     }
   }
   if (tid == 0) bad = 0;
   __syncthreads();
-  if (wave == 0) chol_leaf(L, rdiag_all, 0, lane, &bad, &W[0][0]);
+  if (wave == 0) chol_leaf(L, rdiag_all, 0, lane, &bad, W);
   __syncthreads();
 
   for (int t = 0; t < NT; ++t) {
@@ -261,13 +280,13 @@ __global__ void __launch_bounds__(DIAG_THREADS)
 #pragma unroll
       for (int s4 = 0; s4 < LEAF; s4 += 4) {
         const int kk = s4 + lk;
-        const double a = L[r0 + li][kb + kk];
+        const double a = lt_at(L, r0 + li, kb + kk);
         const double b = leaf_inv(L, rdiag_all, kb, li, kk);  // X_tt^T[kk][li]
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
       }
       wave_lds_sync();
 #pragma unroll
-      for (int r = 0; r < 4; ++r) L[r0 + 4 * r + lk][kb + li] = acc[r];
+      for (int r = 0; r < 4; ++r) lt_at(L, r0 + 4 * r + lk, kb + li) = acc[r];
     }
     __syncthreads();
     // ---- trailing update C_IJ -= P_I P_J^T (1 <= J <= I <= nt, relative
@@ -283,19 +302,19 @@ __global__ void __launch_bounds__(DIAG_THREADS)
         const int r0 = kb + LEAF * I, c0 = kb + LEAF * J;
         dbl4 acc;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = L[r0 + 4 * r + lk][c0 + li];
+        for (int r = 0; r < 4; ++r) acc[r] = lt_at(L, r0 + 4 * r + lk, c0 + li);
 #pragma unroll
         for (int s4 = 0; s4 < LEAF; s4 += 4) {
-          const double a = -L[r0 + li][kb + s4 + lk];
-          const double b = L[c0 + li][kb + s4 + lk];
+          const double a = -lt_at(L, r0 + li, kb + s4 + lk);
+          const double b = lt_at(L, c0 + li, kb + s4 + lk);
           acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) L[r0 + 4 * r + lk][c0 + li] = acc[r];
+        for (int r = 0; r < 4; ++r) lt_at(L, r0 + 4 * r + lk, c0 + li) = acc[r];
       }
       if (wave == 0) {
         wave_lds_sync();
-        chol_leaf(L, rdiag_all, kb + LEAF, lane, &bad, &W[0][0]);
+        chol_leaf(L, rdiag_all, kb + LEAF, lane, &bad, W);
       }
     }
     __syncthreads();
@@ -352,9 +371,9 @@ __global__ void __launch_bounds__(DIAG_THREADS)
 #pragma unroll
             for (int s4 = 0; s4 < LEAF; s4 += 4) {
               const int kk = s4 + lk;
-              const double a = L[LEAF * (B0 + i) + li][LEAF * (A0 + k) + kk];
+              const double a = lt_at(L, LEAF * (B0 + i) + li, LEAF * (A0 + k) + kk);
               const double b = (k == j) ? leaf_inv(L, rdiag_all, LEAF * (A0 + j), kk, li)
-                                        : L[LEAF * (A0 + k) + kk][LEAF * (A0 + j) + li];
+                                        : lt_at(L, LEAF * (A0 + k) + kk, LEAF * (A0 + j) + li);
               acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
             }
           }
@@ -364,8 +383,8 @@ __global__ void __launch_bounds__(DIAG_THREADS)
             for (int s4 = 0; s4 < LEAF; s4 += 4) {
               const int kk = s4 + lk;
               const double a = -((k == i) ? leaf_inv(L, rdiag_all, LEAF * (B0 + i), li, kk)
-                                          : L[LEAF * (B0 + i) + li][LEAF * (B0 + k) + kk]);
-              const double b = L[LEAF * (B0 + k) + kk][LEAF * (A0 + j) + li];
+                                          : lt_at(L, LEAF * (B0 + i) + li, LEAF * (B0 + k) + kk));
+              const double b = lt_at(L, LEAF * (B0 + k) + kk, LEAF * (A0 + j) + li);
               acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
             }
           }
@@ -377,7 +396,8 @@ __global__ void __launch_bounds__(DIAG_THREADS)
         if (!on[q]) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          L[LEAF * (tA0[q] + ht + ti[q]) + 4 * r + lk][LEAF * (tA0[q] + tj[q]) + li] = acc[q][r];
+          lt_at(L, LEAF * (tA0[q] + ht + ti[q]) + 4 * r + lk, LEAF * (tA0[q] + tj[q]) + li) =
+              acc[q][r];
       }
       __syncthreads();
     }
@@ -394,7 +414,7 @@ __global__ void __launch_bounds__(DIAG_THREADS)
         const int tb = a & ~15;
         xba = leaf_inv(L, rdiag_all, tb, b - tb, a - tb);
       } else if (b > a) {
-        xba = L[b][a];
+        xba = lt_at(L, b, a);
       }
     }
     Dinv[idx] = xba;
