@@ -27,7 +27,8 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("Ns,Np", [(1, 5), (100, 60), (129, 300), (257, 200), (1000, 700),
-                                   (300, 5000), (64, 40000), (640, 300), (1537, 400)])
+                                   (300, 5000), (64, 40000), (640, 300), (1537, 400),
+                                   (2100, 300)])
 def test_minsr_fp64_matches_oracle(device, Ns, Np):
     O, E = _problem(Ns, Np, Ns)
     want = osr.minsr(O, E, E.mean(), 1e-3)
