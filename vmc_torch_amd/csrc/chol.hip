@@ -231,7 +231,7 @@ __device__ __forceinline__ double leaf_inv(const LTile *L,
 
 __global__ void __launch_bounds__(DIAG_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4)))
     chol_diag_kernel(double *A, int64_t lda, int64_t k0, int nbk, int32_t *info,
-                     double *Dinv, double *DinvT) {
+                     double *Dinv) {
   // the factorisation's critical path: its waves go first when the CU is
   // shared with trailing-update workgroups
   __builtin_amdgcn_s_setprio(3);
@@ -419,7 +419,6 @@ __global__ void __launch_bounds__(DIAG_THREADS) __attribute__((amdgpu_waves_per_
     }
     Dinv[idx] = xba;
   }
-  (void)DinvT;
 }
 
 // DinvT_k = Dinv_k^T for every block (grid: blocks x 64)
@@ -681,7 +680,7 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
     double *Dk = Dinv + kb * CHOL_NB * CHOL_NB;
     if (diag_ms) VMCX_HIP(hipEventRecord(tev[2 * kb], st));
     hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(DIAG_THREADS), 0, st, A, lda, k0, nbk,
-                       info, Dk, DinvT + kb * CHOL_NB * CHOL_NB);
+                       info, Dk);
     VMCX_LAUNCHED();
     if (diag_ms) VMCX_HIP(hipEventRecord(tev[2 * kb + 1], st));
     ++ndiag;
