@@ -14,7 +14,6 @@
 // rocSOLVER); the fused centring / conversion / shift kernels are ours.
 #include <hip/hip_bf16.h>
 #include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include <cmath>
 #include <cstdlib>
