@@ -10,8 +10,8 @@
 //     GEMM with Dinv_k  : W_k = A_k,rest L_kk^-T        (panel below, into W)
 //     GEMM / SYRK       : A_rest <- A_rest - W W^T      (trailing update)
 // The factor is Dinv (diagonal blocks, inverted) plus W (off-diagonal
-// blocks); A is left as scratch.  The rocSOLVER panel kernel is latency
-// bound at this size; the diagonal block here stays in LDS for all NB steps.
+// blocks); A is left as scratch.  The diagonal block stays in LDS for all
+// NB steps of its factorisation and inversion (one launch per block).
 //
 // Solves use the inverted diagonal blocks so that each block step is one
 // launch: forward  y_k = Dinv_k (b_k) then b_rest -= L_rest,k y_k;

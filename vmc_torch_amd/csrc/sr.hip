@@ -10,8 +10,10 @@
 //   dp  = Ot^T alpha
 //
 // Column means and the energy moments use Neumaier-compensated sums.  The
-// Gram, GEMV and Cholesky are plain library BLAS / LAPACK calls (rocBLAS,
-// rocSOLVER); the fused centring / conversion / shift kernels are ours.
+// FP64 Gram is the matrix-core SYRK kernel (syrk.hip), the Cholesky and its
+// solves are chol.hip, the GEMV passes over Ot and the fused centring /
+// conversion / shift kernels are below; rocBLAS serves only the
+// reduced-precision Gram (gemm_ex).
 #include <hip/hip_bf16.h>
 #include <rocblas/rocblas.h>
 
