@@ -43,6 +43,24 @@ def test_minsr_fp64_matches_oracle(device, Ns, Np):
     assert s[3] == pytest.approx(np.linalg.norm(g), rel=1e-9)
 
 
+@pytest.mark.parametrize("Ns,Np", [(40, 6), (64, 12), (300, 20), (400, 140), (700, 300)])
+def test_minsr_reports_first_nonpositive_minor(device, Ns, Np):
+    """Ot Ot^T has rank Np, so with shift -1e-9 the leading minors of
+    T = Ot Ot^T + shift I are positive definite up to order Np (smallest
+    eigenvalue >= 5e-8 here) and not at Np + 1: info must be Np + 1, in the
+    first or second 8 x 8 half of a leaf, a later leaf or a later diagonal
+    block."""
+    shift = -1e-9
+    O, E = _problem(Ns, Np, Ns + 7)
+    Ot = (O - O.mean(0)) / np.sqrt(Ns)
+    T = Ot @ Ot.T + shift * np.eye(Ns)
+    assert np.linalg.eigvalsh(T[:Np, :Np])[0] > 4e-8
+    assert np.linalg.eigvalsh(T[:Np + 1, :Np + 1])[0] < 0
+    _, _, info = kernels.sr_minsr(torch.tensor(O, device=device), torch.tensor(E, device=device),
+                                  None, shift, "fp64", max_iter=0)
+    assert int(info.item()) == Np + 1
+
+
 @pytest.mark.parametrize("odt", [torch.float64, torch.float32])
 @pytest.mark.parametrize("gram", ["fp32", "bf16"])
 @pytest.mark.parametrize("shift", [1e-2, 1e-4])
