@@ -18,6 +18,17 @@ One timed step = one SR iteration: one sweep of every walker (one sample per
 walker), the local energies, the per-sample log-derivatives O, the minSR solve
 and the SGD update, all on the device.
 
+hubbard_mixed_lp: the batched learned-proposal form.  An uneven batch of 4x4
+(n = 1, 8192 walkers), 6x6 (n = 0.89, 4096) and 8x8 (n = 1, 4096) Hubbard
+lattices, each with an NNBF ansatz of the 6x6 family (2N -> 256 -> 256 ->
+2N x nf, BF16 GEMMs); one proposal net (three 3x3 convolutions, 32 channels)
+serves all three and is trained on the chains' own walkers.  A step there is
+one sample per walker of every lattice: |E|/4 learned proposals (the plain
+form: one sweep of |E| edge proposals) and the local energies, with a
+training step of the net every 10 steps; the wavefunction parameters stay
+fixed (sampling form).  Reported for both samplers: tau_int, acceptance,
+E/site and effective samples/s.
+
 Printed: one JSON line with effective samples/s as `value`, SR iterations/s,
 walker-sweeps/s, the roofline of the dominant component (HIP-event timing on
 the launch stream), per-component timings, the CPU baseline (the FP64 oracle
@@ -62,6 +73,14 @@ WORKLOADS = {
         gram="fp64", D=4, hidden=0, eta=0.0,
         desc=("4x4 OBC Hubbard t=1 U=8 8u+8d, Z2 fPEPS D=4 (exact contraction, chi=16), "
               "3200 params, FP64, 2048 walkers")),
+    "hubbard_mixed_lp": dict(
+        kind="mixed_lp", instances=[(4, 4, 8, 8, 8192), (6, 6, 16, 16, 4096), (8, 8, 32, 32, 4096)],
+        hidden=256, bf_scale=0.01, mlp="bf16", burn=50, channels=32, train_iters=100,
+        train_every=10, train_walkers=1024, train_lr=0.05,
+        desc=("uneven batch of OBC Hubbard t=1 U=8 lattices: 4x4 8u+8d (8192 walkers), 6x6 "
+              "16u+16d (4096), 8x8 32u+32d (4096); NNBF 2N-256-256-(2N x nf) ReLU, bf_scale "
+              "0.01, BF16 GEMMs, per lattice; one learned proposal net (3 x 3x3 conv, 32 "
+              "channels, FP32) shared by the three")),
 }
 
 
@@ -104,7 +123,8 @@ def _cpu_worker(args):
     from oracle import nnbf as onn
     from oracle import sampler as osamp
     from oracle import slater as osl
-    kind, Lx, Ly, nu, nd, params, bf, x, seed, budget = args
+    kind, Lx, Ly, nu, nd, params, bf, x, seed, budget = args[:10]
+    grad = args[10] if len(args) > 10 else True
     H = ohub.Hubbard(Lx, Ly, 1.0, 8.0, nu, nd)
     edges = H.graph.sweep_edges()
     if kind == "slater":
@@ -122,7 +142,8 @@ def _cpu_worker(args):
 
         def work(xs, s, l):
             onn.local_energy(H, params, xs, 2, bf)
-            onn.log_derivative(params, xs, 2, bf)
+            if grad:
+                onn.log_derivative(params, xs, 2, bf)
     else:
         from oracle import fpeps as ofp
         sh = ofp.PEPSShape(Lx, Ly, bf["D"])
@@ -432,6 +453,8 @@ def main():
                                    "sharding is designed but not part of this build",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if WORKLOADS[a.workload]["kind"] == "mixed_lp":
+        return main_mixed_lp(a, WORKLOADS[a.workload])
     import numpy as np
     import torch
 
@@ -651,6 +674,243 @@ def main():
         out["cpu_baseline"] = cpu_baseline(w["kind"], Lx, Ly, nu, nd, params, bf,
                                            fxs[:4096].cpu().numpy(), Ns, tau, w["shift"],
                                            a.cpu_seconds)
+    print(json.dumps(out))
+
+
+
+# ------------------------------------------------- batched learned proposal
+def lp_logits_flops(N, E, C):
+    """FP32 flops of one net evaluation: three 3x3 convolutions (2 -> C, C -> C,
+    C -> C; every tap counted, including the zero-padding ones) and the head
+    (3 move types x 2C per edge)."""
+    return 2.0 * N * 9 * (2 * C + 2 * C * C) + 2.0 * 3 * E * 2 * C
+
+
+def main_mixed_lp(a, w):
+    """The learned proposal on an uneven batch of lattices against the plain
+    sweep sampler, on the same wavefunctions and walkers."""
+    import numpy as np
+    import torch
+
+    from vmc_torch_amd import NNBF_GPU, kernels
+    from vmc_torch_amd.hamiltonian import spinful_Fermi_Hubbard_square_lattice_torch
+    from vmc_torch_amd.sampler import (LearnedProposalSamplerGPU,
+                                       MetropolisExchangeSpinfulSamplerGPU)
+    from vmc_torch_amd.stats import tau_int
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    C = w["channels"]
+    net = LearnedProposalSamplerGPU.init_params(C, torch.Generator().manual_seed(0)).to(dev)
+    inst = []
+    for n_i, (Lx, Ly, nu, nd, B) in enumerate(w["instances"]):
+        N, nf = Lx * Ly, nu + nd
+        H = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, nf,
+                                                       n_fermions_per_spin=(nu, nd))
+        H.precompute_hops_gpu(dev)
+        model = NNBF_GPU(N, nf, hidden_dim=w["hidden"], activation="relu",
+                         bf_scale=w["bf_scale"], gemm=w["mlp"], init="xavier", m_std=0.1,
+                         generator=torch.Generator().manual_seed(n_i)).to(dev)
+        base = torch.tensor([1, 2] * (N // 2), dtype=torch.int64)
+        base[: N - nf] = 0
+        perms = torch.argsort(torch.rand(B, N, generator=torch.Generator().manual_seed(42 + n_i)),
+                              dim=1)
+        E = H.graph.edge_tensor(dev, "sweep").shape[0]
+        inst.append(dict(
+            name=f"{Lx}x{Ly}", Lx=Lx, Ly=Ly, nu=nu, nd=nd, B=B, N=N, nf=nf, H=H, model=model,
+            x=base[perms].contiguous().to(dev), E=E, k=max(1, E // 4),
+            plain=MetropolisExchangeSpinfulSamplerGPU(seed=0x5EED + n_i),
+            lp=LearnedProposalSamplerGPU(Lx, Ly, channels=C, seed=0x1EA + n_i, params=net)))
+    Btot = sum(I["B"] for I in inst)
+    log(f"{a.workload}: {Btot} walkers on {len(inst)} lattices; burn-in {w['burn']} sweeps")
+    for I in inst:
+        I["plain"].sweeps(I["x"], I["model"], I["H"].graph, w["burn"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = []
+    for it in range(w["train_iters"]):
+        for I in inst:
+            I["lp"].sweeps(I["x"], I["model"], 2)
+            loss = I["lp"].train_step(I["x"][:w["train_walkers"]], I["model"], lr=w["train_lr"])
+            if it in (0, w["train_iters"] - 1):
+                losses.append(loss)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    log(f"burn-in and {w['train_iters']} training iterations done ({t_train:.1f} s)")
+
+    def plain_step(I):
+        I["plain"].sweeps(I["x"], I["model"], I["H"].graph, 1)
+        return I["model"].local_energy(I["x"], I["H"])
+
+    def lp_step(I):
+        I["lp"].sweeps(I["x"], I["model"], I["k"])
+        return I["model"].local_energy(I["x"], I["H"])
+
+    def form(name):
+        step = plain_step if name == "plain" else lp_step
+        for _ in range(a.warmup):
+            for I in inst:
+                step(I)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s_ in range(a.steps):
+            for I in inst:
+                step(I)
+            if name == "learned" and (s_ + 1) % w["train_every"] == 0:
+                for I in inst:
+                    I["lp"].train_step(I["x"][:w["train_walkers"]], I["model"],
+                                       lr=w["train_lr"])
+        torch.cuda.synchronize()
+        t_step = (time.perf_counter() - t0) / a.steps
+        # tau_int of E_loc at fixed parameters and a fixed net, one sample per step
+        per = []
+        for I in inst:
+            smp = I["plain"] if name == "plain" else I["lp"]
+            series, acc = [], 0.0
+            for _ in range(a.tau_sweeps):
+                series.append(step(I) / I["N"])
+                n_acc, n_prop = smp.last_acceptance
+                acc += float(n_acc.double().mean()) / n_prop
+            ser = torch.stack(series, dim=1)
+            per.append({"lattice": I["name"], "walkers": I["B"], "tau_int": tau_int(ser),
+                        "acceptance_per_proposal": acc / a.tau_sweeps,
+                        "proposals_per_sample": I["E"] if name == "plain" else I["k"],
+                        "energy_per_site": float(ser.mean()),
+                        "energy_per_site_stderr": float(ser.mean(dim=1).std()) / I["B"] ** 0.5})
+        torch.cuda.synchronize()
+        ess = sum(I["B"] / r["tau_int"] for I, r in zip(inst, per)) / t_step
+        log(f"{name}: {1e3 * t_step:.1f} ms per step, {ess:.0f} effective samples/s, tau_int "
+            + ", ".join(f"{r['lattice']} {r['tau_int']:.2f}" for r in per))
+        return {"effective_samples_per_s": ess, "ms_per_step": 1e3 * t_step,
+                "lattices": per}
+
+    plain = form("plain")
+    learned = form("learned")
+
+    # per-step time of the components of the learned form, HIP events on the
+    # launch stream (torch's current stream, which the library uses)
+    def ev_ms(fn, reps=5):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    comp = {"lp_logits": 0.0, "amplitude": 0.0, "local_energy": 0.0}
+    logits_flops, logits_launch_ms, n_launch = 0.0, 0.0, 0
+    amp_flops, el_flops = 0.0, 0.0
+    for I in inst:
+        t_l = ev_ms(lambda: kernels.lp_logits(net, C, I["Lx"], I["Ly"], I["x"]))
+        t_a = ev_ms(lambda: I["model"].forward_log(I["x"]))
+        t_e = ev_ms(lambda: I["model"].local_energy(I["x"], I["H"]), 2)
+        comp["lp_logits"] += 2 * I["k"] * t_l      # at x and at x' per proposal
+        comp["amplitude"] += I["k"] * t_a
+        comp["local_energy"] += t_e
+        logits_flops += 2 * I["k"] * I["B"] * lp_logits_flops(I["N"], I["E"], C)
+        logits_launch_ms += 2 * I["k"] * t_l
+        n_launch += 2 * I["k"]
+        mlp = 2.0 * (w["hidden"] ** 2 + w["hidden"] * 2 * I["N"] * I["nf"])
+        amp_flops += I["k"] * I["B"] * mlp
+        e = I["H"].graph.edge_tensor(dev, "graph").long()
+        n_conn = 0
+        for bit in (1, 2):
+            occ = (I["x"] & bit) != 0
+            n_conn += int((occ[:, e[:, 0]] != occ[:, e[:, 1]]).sum().item())
+        el_flops += n_conn * mlp
+    kern = {
+        "lp_logits": {"ms": comp["lp_logits"], "launches_per_step": n_launch, "bound": "mfma",
+                      "achieved": logits_flops / (logits_launch_ms * 1e-3) / 1e12,
+                      "peak": PEAK_FP32_MFMA / 1e12, "unit": "TFLOP/s",
+                      "frac": logits_flops / (logits_launch_ms * 1e-3) / PEAK_FP32_MFMA,
+                      "note": ("conv net forward, FP32 on the vector ALUs (one thread per "
+                               "configuration and site), flops of every tap; peak = MI355X "
+                               "FP32 figure")},
+        "amplitude": {"ms": comp["amplitude"], "bound": "mfma",
+                      "achieved": amp_flops / (comp["amplitude"] * 1e-3) / 1e12,
+                      "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                      "frac": amp_flops / (comp["amplitude"] * 1e-3) / PEAK_BF16,
+                      "note": "NNBF amplitude of the proposed walkers: dense-layer flops, BF16"},
+        "local_energy": {"ms": comp["local_energy"], "bound": "mfma",
+                         "achieved": el_flops / (comp["local_energy"] * 1e-3) / 1e12,
+                         "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                         "frac": el_flops / (comp["local_energy"] * 1e-3) / PEAK_BF16,
+                         "note": "NNBF amplitude per connected configuration, BF16"},
+    }
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    roof = {kk: kern[dom][kk] for kk in ("bound", "achieved", "peak", "unit", "frac")}
+    roof["kernel"] = dom
+    roof["traffic"] = None
+    out = {
+        "metric": "effective samples/s + SR iters/s at 8x8 Hubbard; |dE/site| vs FP64 oracle",
+        "value": learned["effective_samples_per_s"],
+        "unit": "effective samples/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": learned["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-proposal-net/bf16-gemm/f64-det",
+        "data": "synthetic (seeded parameters, seeded initial walkers)",
+        "config": {"workload": a.workload, "description": w["desc"], "walkers": Btot,
+                   "lattices": [I["name"] for I in inst],
+                   "walkers_per_lattice": [I["B"] for I in inst],
+                   "learned_proposals_per_sample": [I["k"] for I in inst],
+                   "plain_proposals_per_sample": [I["E"] for I in inst],
+                   "net_channels": C, "train_iters_before": w["train_iters"],
+                   "train_every_steps": w["train_every"],
+                   "train_walkers_per_lattice": w["train_walkers"],
+                   "parallelism": "walkers on 1 GPU"},
+        "step": ("one sample per walker of every lattice (learned: |E|/4 proposals; plain: "
+                 "one sweep of |E| edge proposals) plus its local energy; a training step "
+                 "of the shared net every 10 steps; wavefunction parameters fixed"),
+        "learned": learned,
+        "plain": plain,
+        "ess_ratio_learned_over_plain": (learned["effective_samples_per_s"]
+                                         / plain["effective_samples_per_s"]),
+        "training": {"seconds": t_train, "heat_bath_loss_first": losses[:len(inst)],
+                     "heat_bath_loss_last": losses[len(inst):]},
+        "roofline": roof,
+        "kernels": kern,
+    }
+    if not a.no_cpu_baseline:
+        import multiprocessing as mp
+
+        from oracle import hubbard as ohub
+        from oracle import nnbf as onn
+        checks, rates = [], []
+        cores = min(os.cpu_count() or 1, 16)
+        ctx = mp.get_context("spawn")
+        for I in inst:
+            params = [p.detach().to(torch.float32).to(torch.float64).cpu().numpy()
+                      for p in I["model"].params]
+            xs = I["x"][:8].cpu().numpy()
+            Eg = I["model"].local_energy(I["x"], I["H"])[:8].cpu().numpy()
+            Eo = onn.local_energy(ohub.Hubbard(I["Lx"], I["Ly"], 1.0, 8.0, I["nu"], I["nd"]),
+                                  params, xs, 2, w["bf_scale"])
+            checks.append({"lattice": I["name"], "mean": float(abs(Eg.mean() - Eo.mean()) / I["N"]),
+                           "max_per_sample": float(np.abs(Eg - Eo).max() / I["N"]),
+                           "samples": int(len(xs))})
+            x_np = I["x"][:1024].cpu().numpy()
+            per = max(1, len(x_np) // cores)
+            jobs = [("nnbf", I["Lx"], I["Ly"], I["nu"], I["nd"], params, w["bf_scale"],
+                     x_np[c * per:(c + 1) * per], 2000 + c, a.cpu_seconds / len(inst), False)
+                    for c in range(cores)]
+            with ctx.Pool(cores) as pool:
+                res = pool.map(_cpu_worker, jobs)
+            rates.append(sum(r[0] for r in res) / max(r[1] for r in res))
+            log(f"CPU baseline {I['name']}: {rates[-1]:.1f} walker-samples/s")
+        out["dE_per_site_vs_fp64_oracle"] = checks
+        t_cpu = sum(I["B"] / r for I, r in zip(inst, rates))
+        out["cpu_baseline"] = {
+            "value": sum(I["B"] / p["tau_int"] for I, p in zip(inst, plain["lattices"])) / t_cpu,
+            "unit": "effective samples/s", "cores": cores, "kind": "port",
+            "sample": ("FP64 oracle restating the reference's plain sampler (full amplitude per "
+                       "proposal, one sweep per sample) and E_loc on each lattice, one process "
+                       f"per core, {a.cpu_seconds / len(inst):.0f} s per lattice; rates "
+                       "combined over the batch with the GPU plain chain's tau_int"),
+            "walker_samples_per_s": dict(zip([I["name"] for I in inst], rates)),
+            "cpu_model": _cpu_model(),
+        }
     print(json.dumps(out))
 
 

@@ -17,8 +17,10 @@ import torch
 from helpers import random_configs
 from oracle import proposal as op
 from oracle import slater as osl
-from vmc_torch_amd import SlaterDeterminant_GPU, kernels
+from vmc_torch_amd import (MetropolisExchangeSpinfulSamplerGPU, SlaterDeterminant_GPU, kernels,
+                           spinful_Fermi_Hubbard_square_lattice_torch)
 from vmc_torch_amd.sampler import LearnedProposalSamplerGPU
+from vmc_torch_amd.stats import tau_int
 
 pytestmark = pytest.mark.gpu
 
@@ -160,3 +162,91 @@ def test_shared_net_across_lattices(device):
         assert np.all(((xc >> 1) & 1).sum(1) == nu) and np.all((xc & 1).sum(1) == nd)
         n_acc, n = s.last_acceptance
         assert 0 < float(n_acc.double().mean()) / n <= 1.0
+
+
+def _afm_slater(Lx, Ly, nu, nd, delta, device):
+    """Mean-field antiferromagnet: per spin sigma the lowest orbitals of
+    -t (nearest-neighbour hops) + sigma * delta * (-1)^(x+y); rows are the
+    modes [u_0..u_{N-1} | d_0..d_{N-1}], columns nu up then nd down orbitals.
+    A structured |psi|^2 on which most local moves are rejected."""
+    N = Lx * Ly
+    h = np.zeros((N, N))
+    for s in range(N):
+        x, y = divmod(s, Ly)
+        if y + 1 < Ly:
+            h[s, s + 1] = h[s + 1, s] = -1.0
+        if x + 1 < Lx:
+            h[s, s + Ly] = h[s + Ly, s] = -1.0
+    stag = np.diag([(-1.0) ** (s // Ly + s % Ly) for s in range(N)])
+    M = np.zeros((2 * N, nu + nd))
+    M[:N, :nu] = np.linalg.eigh(h + delta * stag)[1][:, :nu]
+    M[N:, nu:] = np.linalg.eigh(h - delta * stag)[1][:, :nd]
+    m = SlaterDeterminant_GPU(2 * N, nu + nd).to(device)
+    with torch.no_grad():
+        m.params[0].copy_(torch.tensor(M))
+    return m
+
+
+def test_trained_net_gains_over_untrained(device, capsys):
+    """What the trained network gains.  4x4 U=8 Hubbard, 8 up + 8 down, on a
+    mean-field antiferromagnet (staggered field 3): the same learned-proposal
+    chain with the untrained net and with the net after 300 heat-bath
+    training steps on the chain's own walkers, 24 proposals per sample (as
+    many as one plain sweep has edges), 2048 chains x 150 samples each.
+    Measured on MI355X: acceptance per proposal 3.3 % untrained -> 74 %
+    trained, tau_int of E_loc 3.6 -> 1.5 samples; the plain sweep sampler has
+    tau_int 6.0 per 24-proposal sweep.  All three chains sample |psi|^2: the
+    mean energies agree within 5 standard errors."""
+    Lx, Ly, nu, nd, B, T, k = 4, 4, 8, 8, 2048, 150, 24
+    N = Lx * Ly
+    H = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, nu + nd,
+                                                   n_fermions_per_spin=(nu, nd))
+    H.precompute_hops_gpu(device)
+    model = _afm_slater(Lx, Ly, nu, nd, 3.0, device)
+    x0 = torch.tensor(random_configs(np.random.default_rng(2), B, N, nu, nd), device=device)
+    plain = MetropolisExchangeSpinfulSamplerGPU(seed=11)
+    plain.sweeps(x0, model, H.graph, 100)
+
+    def run(advance):
+        series, acc = [], 0.0
+        for _ in range(T):
+            acc += advance()
+            series.append(model.local_energy(x, H) / N)
+        s = torch.stack(series, 1)
+        return (acc / T, tau_int(s), float(s.mean()),
+                float(s.mean(1).std()) / np.sqrt(B))
+
+    x = x0.clone()
+
+    def plain_step():
+        plain.sweeps(x, model, H.graph, 1)
+        n, m = plain.last_acceptance
+        return float(n.double().mean()) / m
+    res = {"plain": run(plain_step)}
+    P0 = LearnedProposalSamplerGPU.init_params(32, torch.Generator().manual_seed(5)).to(device)
+    trainer = LearnedProposalSamplerGPU(Lx, Ly, channels=32, params=P0.clone(), seed=9)
+    xt = x0.clone()
+    for _ in range(300):
+        trainer.sweeps(xt, model, 2)
+        trainer.train_step(xt[:1024], model, lr=0.05)
+    for tag, P, seed in [("untrained", P0, 7), ("trained", trainer.params, 13)]:
+        lp = LearnedProposalSamplerGPU(Lx, Ly, channels=32, params=P.clone(), seed=seed)
+        x = x0.clone()
+        lp.sweeps(x, model, 2 * k)
+
+        def lp_step():
+            lp.sweeps(x, model, k)
+            n, m = lp.last_acceptance
+            return float(n.double().mean()) / m
+        res[tag] = run(lp_step)
+    with capsys.disabled():
+        for tag, (acc, tau, e, se) in res.items():
+            print(f"\n  {tag:9s}: acceptance {acc:.4f}, tau_int {tau:.2f} per {k} proposals, "
+                  f"E/site {e:.5f} +- {se:.5f}")
+    acc_u, tau_u, e_u, se_u = res["untrained"]
+    acc_t, tau_t, e_t, se_t = res["trained"]
+    _, tau_p, e_p, se_p = res["plain"]
+    assert acc_t > 5.0 * acc_u
+    assert tau_t < 0.7 * tau_u and tau_t < tau_p
+    for e, se in [(e_u, se_u), (e_t, se_t)]:
+        assert abs(e - e_p) < 5.0 * np.hypot(se, se_p)

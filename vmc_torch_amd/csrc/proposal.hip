@@ -119,55 +119,157 @@ __device__ __forceinline__ bool lp_valid(int t, int xi, int xj) {
 
 }  // namespace
 
-// logits[b][m] (FP64; -inf for invalid moves)
+// Forward pass batched over configurations: G configurations per workgroup,
+// one thread per (configuration, site).  Each thread accumulates CB output
+// channels of its site at a time; the weights are workgroup-uniform (scalar
+// loads, one FMA per weight and thread), the activations ping-pong between
+// two LDS buffers laid out [channel][G * (N + 1)] with a zero slot per
+// configuration that the padding taps read.  The sums run in the order of
+// the per-configuration form (bias, then input channel, then tap), so the
+// logits are the same.
+constexpr int LPB_SLOTS = 240;   // G * (N + 1) <= LPB_SLOTS (LDS <= 64 KiB), G * N <= threads
+
+// configurations per workgroup for a block of `threads` threads
+__host__ __device__ inline int lpb_group(int N, int threads) {
+  int g = threads / N;
+  if (g * (N + 1) > LPB_SLOTS) g = LPB_SLOTS / (N + 1);
+  return g < 1 ? 1 : g;
+}
+
+__host__ __device__ inline size_t lpb_lds_bytes(int C, int N, int G) {
+  const int R = G * (N + 1);
+  return (size_t)(2 * C * R + R + 2 * LP_EMAX) * sizeof(float);
+}
+
+template <int CB>
 __global__ void __launch_bounds__(LP_T)
-    lp_logits_kernel(const float *__restrict__ P, int C, int Lx, int Ly, int64_t B,
-                     const int64_t *__restrict__ cfg, double *logits) {
-  const int64_t b = blockIdx.x;
-  if (b >= B) return;
-  __shared__ LpSh S;
-  const int N = Lx * Ly, E = lp_n_edges(Lx, Ly);
-  lp_net(P, C, Lx, Ly, cfg + b * N, S);
+    lp_logits_batched_kernel(const float *__restrict__ P, int C, int Lx, int Ly, int G,
+                             int64_t B, const int64_t *__restrict__ cfg,
+                             double *__restrict__ logits) {
+  extern __shared__ float lpb_sh[];
+  const int N = Lx * Ly, E = lp_n_edges(Lx, Ly), R = G * (N + 1);
+  float *hb0 = lpb_sh, *hb1 = lpb_sh + C * R;
+  int *xs = reinterpret_cast<int *>(lpb_sh + 2 * C * R);
+  int *ei = xs + R, *ej = ei + LP_EMAX;
+  const int64_t b0 = (int64_t)blockIdx.x * G;
+  const int t = threadIdx.x;
+  const bool site = t < G * N;             // this thread owns a (configuration, site)
+  const int g = site ? t / N : 0, s = site ? t - (t / N) * N : 0;
+  // input channels (n_up, n_dn); configurations past B read as empty
+  for (int idx = t; idx < R; idx += blockDim.x) {
+    const int gg = idx / (N + 1), ss = idx - gg * (N + 1);
+    int v = 0;
+    if (ss < N && b0 + gg < B) v = (int)cfg[(b0 + gg) * N + ss];
+    xs[idx] = v;
+    hb0[idx] = (float)((v >> 1) & 1);
+    hb0[R + idx] = (float)(v & 1);
+  }
+  // zero slots of every channel in both buffers
+  for (int idx = t; idx < C * G; idx += blockDim.x) {
+    const int c = idx / G, gg = idx - c * G;
+    hb0[c * R + gg * (N + 1) + N] = 0.0f;
+    hb1[c * R + gg * (N + 1) + N] = 0.0f;
+  }
+  lp_edges(Lx, Ly, ei, ej);
+  int off[9];
+  {
+    const int x = s / Ly, y = s - (s / Ly) * Ly, base = g * (N + 1);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      const int xx = x + q / 3 - 1, yy = y + q % 3 - 1;
+      off[q] = base + ((xx >= 0 && xx < Lx && yy >= 0 && yy < Ly) ? xx * Ly + yy : N);
+    }
+  }
+  __syncthreads();
   const LpOff o = lp_offsets(C);
+  const float *hin = hb0;
+  float *hout = hb1;
+  for (int layer = 0; layer < 3; ++layer) {
+    const int Cin = layer == 0 ? 2 : C;
+    const float *W = P + (layer == 0 ? o.W1 : (layer == 1 ? o.W2 : o.W3));
+    const float *bias = P + (layer == 0 ? o.b1 : (layer == 1 ? o.b2 : o.b3));
+    if (site) {
+      for (int co0 = 0; co0 < C; co0 += CB) {
+        float acc[CB];
+#pragma unroll
+        for (int k = 0; k < CB; ++k) acc[k] = co0 + k < C ? bias[co0 + k] : 0.0f;
+        for (int ci = 0; ci < Cin; ++ci) {
+          float v[9];
+#pragma unroll
+          for (int q = 0; q < 9; ++q) v[q] = hin[ci * R + off[q]];
+#pragma unroll
+          for (int k = 0; k < CB; ++k) {
+            if (co0 + k < C) {
+              const float *w = W + ((co0 + k) * Cin + ci) * 9;
+#pragma unroll
+              for (int q = 0; q < 9; ++q) acc[k] += w[q] * v[q];
+            }
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < CB; ++k)
+          if (co0 + k < C) hout[(co0 + k) * R + off[4]] = acc[k] > 0.0f ? acc[k] : 0.0f;
+      }
+    }
+    __syncthreads();
+    const float *tmp = hout;
+    hout = const_cast<float *>(hin);
+    hin = tmp;
+  }
+  // head: one thread per (configuration, move), FP64 accumulation
   const float *A = P + o.A, *bA = P + o.bA;
-  for (int m = threadIdx.x; m < 3 * E; m += LP_T) {
-    const int e = m / 3, t = m - 3 * e;
-    const int i = S.ei[e], j = S.ej[e];
+  for (int idx = t; idx < G * 3 * E; idx += blockDim.x) {
+    const int gg = idx / (3 * E), m = idx - gg * 3 * E;
+    if (b0 + gg >= B) break;
+    const int e = m / 3, tt = m - 3 * e, base = gg * (N + 1);
+    const int i = base + ei[e], j = base + ej[e];
     double v = -INFINITY;
-    if (lp_valid(t, S.x[i], S.x[j])) {
-      double acc = bA[t];
+    if (lp_valid(tt, xs[i], xs[j])) {
+      double acc = bA[tt];
       for (int c = 0; c < C; ++c)
-        acc += (double)A[t * 2 * C + c] * S.h[3][c * N + i] +
-               (double)A[t * 2 * C + C + c] * S.h[3][c * N + j];
+        acc += (double)A[tt * 2 * C + c] * hin[c * R + i] +
+               (double)A[tt * 2 * C + C + c] * hin[c * R + j];
       v = acc;
     }
-    logits[b * 3 * E + m] = v;
+    logits[(b0 + gg) * 3 * E + m] = v;
   }
 }
 
-// one thread per configuration: ln q of the chosen move (sample = 1: draw
-// it by inverse CDF from u_choice / Philox and build the proposed config)
-__global__ void lp_choose_kernel(int64_t B, int N, int Lx, int Ly, int E, int sample,
-                                 const double *__restrict__ logits, uint64_t seed,
-                                 int64_t step, const double *__restrict__ u_choice,
-                                 const int64_t *__restrict__ cfg, int32_t *move,
-                                 double *logq, int64_t *pcfg) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  const double *l = logits + b * 3 * E;
+// one wave per configuration: ln q of the chosen move (sample = 1: draw it
+// by inverse CDF from u_choice / Philox and build the proposed
+// configuration).  Lane l owns the moves [l * per, (l + 1) * per); the
+// maximum and the normaliser are butterfly reductions (every lane ends with
+// the same bits), the CDF is an inclusive scan of the per-lane masses and
+// the lane whose running total first passes u scans its own moves in order.
+constexpr int LPC_WAVES = 4;
+
+__global__ void __launch_bounds__(64 * LPC_WAVES)
+    lp_choose_kernel(int64_t B, int N, int Lx, int Ly, int E, int sample,
+                     const double *__restrict__ logits, uint64_t seed, int64_t step,
+                     const double *__restrict__ u_choice, const int64_t *__restrict__ cfg,
+                     int32_t *move, double *logq, int64_t *pcfg) {
+  const int lane = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * LPC_WAVES + (threadIdx.x >> 6);
+  if (b >= B) return;   // whole wave
+  const int M = 3 * E, per = (M + 63) / 64;
+  const int m0 = min(M, lane * per), m1 = min(M, m0 + per);
+  const double *l = logits + b * M;
   double mx = -INFINITY;
-  for (int m = 0; m < 3 * E; ++m) mx = fmax(mx, l[m]);
+  for (int m = m0; m < m1; ++m) mx = fmax(mx, l[m]);
+  for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+  const int64_t *c = cfg + b * N;
   if (mx == -INFINITY) {   // no valid move
     if (sample) {
-      move[b] = -1;
-      for (int s = 0; s < N; ++s) pcfg[b * N + s] = cfg[b * N + s];
+      if (lane == 0) move[b] = -1;
+      if (lane < N) pcfg[b * N + lane] = c[lane];
     }
-    logq[b] = 0.0;
+    if (lane == 0) logq[b] = 0.0;
     return;
   }
   double z = 0.0;
-  for (int m = 0; m < 3 * E; ++m)
+  for (int m = m0; m < m1; ++m)
     if (l[m] != -INFINITY) z += exp(l[m] - mx);
+  for (int o = 1; o < 64; o <<= 1) z += __shfl_xor(z, o);
   const double lse = mx + log(z);
   int m_sel;
   if (sample) {
@@ -180,43 +282,59 @@ __global__ void lp_choose_kernel(int64_t B, int N, int Lx, int Ly, int E, int sa
                                     (uint32_t)(seed >> 32));
       u = u53(r.x, r.y);
     }
-    double c = 0.0;
-    m_sel = -1;
-    for (int m = 0; m < 3 * E; ++m) {
-      if (l[m] == -INFINITY) continue;
-      c += exp(l[m] - lse);
-      m_sel = m;
-      if (u < c) break;
+    double pl = 0.0;
+    for (int m = m0; m < m1; ++m)
+      if (l[m] != -INFINITY) pl += exp(l[m] - lse);
+    double inc = pl;
+    for (int o = 1; o < 64; o <<= 1) {
+      const double v = __shfl_up(inc, o);
+      if (lane >= o) inc += v;
     }
-    move[b] = m_sel;
-    // proposed configuration
-    int i = -1, j = -1;
-    {
-      const int e = m_sel / 3;
-      int k = 0;
-      for (int s = 0; s < N && i < 0; ++s) {
-        const int xx = s / Ly, yy = s - xx * Ly;
-        if (yy + 1 < Ly) { if (k == e) { i = s; j = s + 1; } ++k; }
-        if (i < 0 && xx + 1 < Lx) { if (k == e) { i = s; j = s + Ly; } ++k; }
+    double exc = __shfl_up(inc, 1);
+    if (lane == 0) exc = 0.0;
+    const unsigned long long pass = __ballot(u < inc);
+    int sel = -1, mine = -1;
+    if (pass) {
+      sel = __ffsll((long long)pass) - 1;
+      if (lane == sel) {
+        double cc = exc;
+        for (int m = m0; m < m1; ++m) {
+          if (l[m] == -INFINITY) continue;
+          cc += exp(l[m] - lse);
+          mine = m;
+          if (u < cc) break;
+        }
       }
+    } else {   // u beyond the rounded total: the last valid move
+      const unsigned long long nz = __ballot(pl > 0.0);
+      sel = 63 - __clzll((long long)nz);
+      if (lane == sel)
+        for (int m = m0; m < m1; ++m)
+          if (l[m] != -INFINITY) mine = m;
     }
-    const int t = m_sel - 3 * (m_sel / 3);
-    for (int s = 0; s < N; ++s) pcfg[b * N + s] = cfg[b * N + s];
-    const int64_t xi = cfg[b * N + i], xj = cfg[b * N + j];
-    if (t == 0) {
-      pcfg[b * N + i] = xi ^ 2;
-      pcfg[b * N + j] = xj ^ 2;
-    } else if (t == 1) {
-      pcfg[b * N + i] = xi ^ 1;
-      pcfg[b * N + j] = xj ^ 1;
-    } else {
-      pcfg[b * N + i] = xj;
-      pcfg[b * N + j] = xi;
+    m_sel = __shfl(mine, sel);
+    if (lane == 0) move[b] = m_sel;
+    // edge e = (i, j): i is the last site whose first edge index
+    // k_s = x (2 Ly - 1) + y + y [x + 1 < Lx] is <= e
+    const int e = m_sel / 3, t = m_sel - 3 * e;
+    int ks = 0, v = 0;
+    if (lane < N) {
+      const int x = lane / Ly, y = lane - x * Ly;
+      ks = x * (2 * Ly - 1) + y + (x + 1 < Lx ? y : 0);
+      v = (int)c[lane];
     }
+    const int i = 63 - __clzll((long long)__ballot(lane < N && ks <= e));
+    const int ki = __shfl(ks, i);
+    const int j = (e == ki && (i % Ly) + 1 < Ly) ? i + 1 : i + Ly;
+    const int xi = __shfl(v, i), xj = __shfl(v, j);
+    if (lane == i) v = t == 0 ? xi ^ 2 : (t == 1 ? xi ^ 1 : xj);
+    if (lane == j) v = t == 0 ? xj ^ 2 : (t == 1 ? xj ^ 1 : xi);
+    if (lane < N) pcfg[b * N + lane] = v;
   } else {
     m_sel = move[b];
   }
-  logq[b] = (m_sel >= 0 && l[m_sel] != -INFINITY) ? l[m_sel] - lse : -INFINITY;
+  if (lane == 0)
+    logq[b] = (m_sel >= 0 && l[m_sel] != -INFINITY) ? l[m_sel] - lse : -INFINITY;
 }
 
 // Metropolis-Hastings test and commit
@@ -409,8 +527,21 @@ int lp_logits(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *c
   VMCX_CHECK_ARG(B >= 0, "B < 0");
   if (B == 0) return 0;
   VMCX_CHECK_ARG(P && cfg && logits, "null pointer");
-  hipLaunchKernelGGL(lp_logits_kernel, dim3((unsigned)B), dim3(LP_T), 0, st, P, C, Lx, Ly, B,
-                     cfg, logits);
+  const int N = Lx * Ly;
+  // smallest block (64, 128, 256 threads) that leaves at most 1/8 of its
+  // lanes without a site
+  int threads = LP_T;
+  for (int T = 64; T < LP_T; T *= 2)
+    if (8 * lpb_group(N, T) * N >= 7 * T) { threads = T; break; }
+  const int G = lpb_group(N, threads);
+  const unsigned grid = (unsigned)((B + G - 1) / G);
+  const size_t lds = lpb_lds_bytes(C, N, G);
+  if (C > 8)
+    hipLaunchKernelGGL(lp_logits_batched_kernel<16>, dim3(grid), dim3(threads), lds, st, P, C,
+                       Lx, Ly, G, B, cfg, logits);
+  else
+    hipLaunchKernelGGL(lp_logits_batched_kernel<8>, dim3(grid), dim3(threads), lds, st, P, C,
+                       Lx, Ly, G, B, cfg, logits);
   VMCX_LAUNCHED();
   return 0;
 }
@@ -423,15 +554,15 @@ int lp_propose(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *
   if (rc || B == 0) return rc;
   VMCX_CHECK_ARG(move && logq_f && pcfg && logq_r, "null pointer");
   const int N = Lx * Ly, E = lp_n_edges(Lx, Ly);
-  const unsigned g = (unsigned)((B + 255) / 256);
-  hipLaunchKernelGGL(lp_choose_kernel, dim3(g), dim3(256), 0, st, B, N, Lx, Ly, E, 1, logits,
-                     seed, step, u_choice, cfg, move, logq_f, pcfg);
+  const unsigned g = (unsigned)((B + LPC_WAVES - 1) / LPC_WAVES);
+  hipLaunchKernelGGL(lp_choose_kernel, dim3(g), dim3(64 * LPC_WAVES), 0, st, B, N, Lx, Ly, E, 1,
+                     logits, seed, step, u_choice, cfg, move, logq_f, pcfg);
   VMCX_LAUNCHED();
   // reverse probability: the same move evaluated at the proposed configuration
   rc = lp_logits(P, C, Lx, Ly, B, pcfg, logits, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(lp_choose_kernel, dim3(g), dim3(256), 0, st, B, N, Lx, Ly, E, 0, logits,
-                     seed, step, (const double *)nullptr, pcfg, move, logq_r,
+  hipLaunchKernelGGL(lp_choose_kernel, dim3(g), dim3(64 * LPC_WAVES), 0, st, B, N, Lx, Ly, E, 0,
+                     logits, seed, step, (const double *)nullptr, pcfg, move, logq_r,
                      (int64_t *)nullptr);
   VMCX_LAUNCHED();
   return 0;
