@@ -80,7 +80,7 @@ WORKLOADS = {
         desc=("uneven batch of OBC Hubbard t=1 U=8 lattices: 4x4 8u+8d (8192 walkers), 6x6 "
               "16u+16d (4096), 8x8 32u+32d (4096); NNBF 2N-256-256-(2N x nf) ReLU, bf_scale "
               "0.01, BF16 GEMMs, per lattice; one learned proposal net (3 x 3x3 conv, 32 "
-              "channels, FP32) shared by the three")),
+              "channels, BF16 matrix cores) shared by the three")),
 }
 
 
@@ -802,7 +802,8 @@ def main_mixed_lp(a, w):
     logits_flops, logits_launch_ms, n_launch = 0.0, 0.0, 0
     amp_flops, el_flops = 0.0, 0.0
     for I in inst:
-        t_l = ev_ms(lambda: kernels.lp_logits(net, C, I["Lx"], I["Ly"], I["x"]))
+        t_l = ev_ms(lambda: kernels.lp_logits(net, C, I["Lx"], I["Ly"], I["x"],
+                                              precision=I["lp"].precision))
         t_a = ev_ms(lambda: I["model"].forward_log(I["x"]))
         t_e = ev_ms(lambda: I["model"].local_energy(I["x"], I["H"]), 2)
         comp["lp_logits"] += 2 * I["k"] * t_l      # at x and at x' per proposal
@@ -819,14 +820,17 @@ def main_mixed_lp(a, w):
             occ = (I["x"] & bit) != 0
             n_conn += int((occ[:, e[:, 0]] != occ[:, e[:, 1]]).sum().item())
         el_flops += n_conn * mlp
+    bf16_net = inst[0]["lp"].precision == "bf16"
+    lp_peak = PEAK_BF16 if bf16_net else PEAK_FP32_MFMA
     kern = {
         "lp_logits": {"ms": comp["lp_logits"], "launches_per_step": n_launch, "bound": "mfma",
                       "achieved": logits_flops / (logits_launch_ms * 1e-3) / 1e12,
-                      "peak": PEAK_FP32_MFMA / 1e12, "unit": "TFLOP/s",
-                      "frac": logits_flops / (logits_launch_ms * 1e-3) / PEAK_FP32_MFMA,
-                      "note": ("conv net forward, FP32 on the vector ALUs (one thread per "
-                               "configuration and site), flops of every tap; peak = MI355X "
-                               "FP32 figure")},
+                      "peak": lp_peak / 1e12, "unit": "TFLOP/s",
+                      "frac": logits_flops / (logits_launch_ms * 1e-3) / lp_peak,
+                      "note": (("conv net forward: layers 2-3 on the BF16 matrix cores, layer 1 "
+                                "and the head on the vector ALUs" if bf16_net else
+                                "conv net forward, FP32 on the vector ALUs") +
+                               "; flops of every tap, padding included")},
         "amplitude": {"ms": comp["amplitude"], "bound": "mfma",
                       "achieved": amp_flops / (comp["amplitude"] * 1e-3) / 1e12,
                       "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
@@ -849,7 +853,7 @@ def main_mixed_lp(a, w):
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": learned["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32-proposal-net/bf16-gemm/f64-det",
+        "dtype": f"{inst[0]['lp'].precision}-proposal-net/bf16-gemm/f64-det",
         "data": "synthetic (seeded parameters, seeded initial walkers)",
         "config": {"workload": a.workload, "description": w["desc"], "walkers": Btot,
                    "lattices": [I["name"] for I in inst],

@@ -360,16 +360,20 @@ int vmcx_peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *confi
 int vmcx_lp_n_params(int C);
 /* 3 x number of edges of the Lx x Ly open lattice; -1 on bad sizes. */
 int vmcx_lp_n_moves(int Lx, int Ly);
-/* logits [B][n_moves] float64, -inf for invalid moves. */
+/* logits [B][n_moves] float64, -inf for invalid moves.  precision 0: FP32
+ * convolutions on the vector ALUs (any C); 1: layer 1 FP32, layers 2-3 on
+ * the BF16 matrix cores with FP32 accumulation and BF16 activations (C = 32);
+ * the head accumulates in FP64 either way. */
 int vmcx_lp_logits(const float *params, int C, int Lx, int Ly, int64_t B,
-                   const int64_t *configs, double *logits, void *stream);
+                   const int64_t *configs, double *logits, int precision, void *stream);
 /* Draw m ~ q(.|x) for every walker, write the proposed configuration and
  * ln q(m|x), then evaluate the net at x' for ln q(m|x') (move = -1 when x
- * has no valid move).  logits_ws: [B][n_moves] float64 scratch. */
+ * has no valid move).  logits_ws: [B][n_moves] float64 scratch; precision as
+ * for vmcx_lp_logits (the same net at x and x', so the chain stays exact). */
 int vmcx_lp_propose(const float *params, int C, int Lx, int Ly, int64_t B,
                     const int64_t *configs, uint64_t seed, int64_t step, const double *u_choice,
                     double *logits_ws, int32_t *move, double *logq_fwd, int64_t *proposed,
-                    double *logq_rev, void *stream);
+                    double *logq_rev, int precision, void *stream);
 /* Metropolis-Hastings: accept when u_acc < |psi'|^2/|psi|^2 q(m|x')/q(m|x),
  * from (psign, plogabs) of the proposed configurations; commits configs,
  * sign, logabs, n_accept (may be NULL) and accepted[B] (may be NULL). */

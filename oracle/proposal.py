@@ -24,6 +24,12 @@ Training target (heat bath over the move set): q*(m | x) proportional to
 gradient w.r.t. the logits is q - q*.
 Parameter order of the flat vector: W1 (C, 2, 3, 3), b1, W2 (C, C, 3, 3),
 b2, W3 (C, C, 3, 3), b3, A (3, 2C), bA (3).
+
+BF16 form (forward(..., bf16=True)): the rounding of the matrix-core net
+restated -- every activation h1, h2, h3 rounded to bfloat16 after the ReLU,
+W2 and W3 rounded to bfloat16, sums in FP64 here (FP32 on the device), the
+head unchanged.  bf16_round is pinned against torch's float32 -> bfloat16
+conversion (round to nearest even).
 """
 import numpy as np
 
@@ -75,6 +81,14 @@ def init_params(rng, C, scale=1.0):
     return np.concatenate([p.ravel() for p in ps])
 
 
+def bf16_round(a):
+    """Round to the nearest bfloat16 (ties to even), via float32; returns float64."""
+    f = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    u = f.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return (r & 0xFFFFFFFF).astype(np.uint32).view(np.float32).astype(np.float64)
+
+
 def _conv(h, W, b):
     """h (Cin, Lx, Ly), W (Cout, Cin, 3, 3) -> (Cout, Lx, Ly), zero padding."""
     Cin, Lx, Ly = h.shape
@@ -87,17 +101,21 @@ def _conv(h, W, b):
     return out
 
 
-def forward(theta, C, Lx, Ly, x):
-    """logits (3E,), valid mask (3E,), cache."""
+def forward(theta, C, Lx, Ly, x, bf16=False):
+    """logits (3E,), valid mask (3E,), cache.  bf16: the matrix-core net's
+    rounding (activations and W2, W3 in bfloat16)."""
     W1, b1, W2, b2, W3, b3, A, bA = unpack(theta, C)
+    rnd = bf16_round if bf16 else (lambda v: v)
+    if bf16:
+        W2, W3 = bf16_round(W2), bf16_round(W3)
     x = np.asarray(x).reshape(Lx, Ly)
     h0 = np.stack([(x >> 1) & 1, x & 1]).astype(np.float64)
     z1 = _conv(h0, W1, b1)
-    h1 = np.maximum(z1, 0)
+    h1 = rnd(np.maximum(z1, 0))
     z2 = _conv(h1, W2, b2)
-    h2 = np.maximum(z2, 0)
+    h2 = rnd(np.maximum(z2, 0))
     z3 = _conv(h2, W3, b3)
-    h3 = np.maximum(z3, 0).reshape(C, -1)
+    h3 = rnd(np.maximum(z3, 0)).reshape(C, -1)
     E = edges(Lx, Ly)
     xf = x.reshape(-1)
     logits = np.zeros(3 * len(E))

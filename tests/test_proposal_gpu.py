@@ -106,15 +106,46 @@ def test_backward_matches_oracle(device):
     assert np.abs(g - want).max() < 1e-4 * np.abs(want).max()
 
 
-def test_chain_samples_psi_squared(device):
+@pytest.mark.parametrize("Lx,Ly", [(4, 4), (8, 8), (6, 6), (3, 5), (2, 2)])
+def test_bf16_logits_match_rounding_oracle(device, Lx, Ly):
+    """The BF16 matrix-core net against the oracle's bfloat16 restatement:
+    within one bfloat16 step (2^-8) of the logit scale -- the device sums in
+    FP32 and in another order, so an activation can round to the
+    neighbouring bfloat16 value and carry that step to the logits -- and, in
+    total, at most half as far from it as from the FP32 net.  40
+    configurations: several workgroups, the last one partly filled."""
+    C, N = 32, Lx * Ly
+    th, P = _net(C, N + 100)
+    x = random_configs(np.random.default_rng(5), 40, N, max(1, N // 3), max(1, N // 4))
+    L = kernels.lp_logits(P.to(device), C, Lx, Ly, torch.tensor(x, device=device),
+                          precision="bf16").cpu().numpy()
+    th32 = P.double().numpy()
+    e16 = e32 = 0.0
+    for b in range(len(x)):
+        want, valid, _ = op.forward(th32, C, Lx, Ly, x[b], bf16=True)
+        w32, _, _ = op.forward(th32, C, Lx, Ly, x[b])
+        assert np.array_equal(np.isfinite(L[b]), valid)
+        if not valid.any():
+            continue
+        d = np.abs(L[b][valid] - want[valid])
+        assert d.max() < 2.0 ** -8 * max(1.0, np.abs(want[valid]).max())
+        e16 += d.sum()
+        e32 += np.abs(L[b][valid] - w32[valid]).sum()
+    print(f"  summed |error| vs bf16 oracle {e16:.3e}, vs fp32 oracle {e32:.3e}")
+    assert e16 < 0.5 * e32
+
+
+@pytest.mark.parametrize("C,precision", [(8, "fp32"), (32, "bf16")])
+def test_chain_samples_psi_squared(device, C, precision):
     """2x2, one up and one down electron: the learned-proposal chain's
     histogram equals |psi|^2 (an untrained net with small weights, so the
     chain mixes within the run; stationarity for any net is pinned exactly on
-    the CPU)."""
+    the CPU), with the FP32 and with the BF16 net."""
     N = 4
     model, M = _slater(N, 2, 11, device)
-    th, P = _net(8, 12, scale=0.3, head_bias=False)
-    s = LearnedProposalSamplerGPU(2, 2, channels=8, params=P.to(device), seed=5)
+    th, P = _net(C, 12, scale=0.3, head_bias=False)
+    s = LearnedProposalSamplerGPU(2, 2, channels=C, params=P.to(device), seed=5,
+                                  precision=precision)
     B = 4096
     x0 = np.tile(np.array([[2, 1, 0, 0]], np.int64), (B, 1))
     xd = torch.tensor(x0, device=device)

@@ -107,3 +107,34 @@ def test_draws_are_uniform_and_keyed():
     assert 0.0 <= uc.min() and uc.max() < 1.0 and abs(uc.mean() - 0.5) < 0.01
     uc2, _ = op.lp_draws(7, 4, np.arange(5))
     assert not np.array_equal(uc[:5], uc2)
+
+
+def test_bf16_round_matches_torch():
+    """The oracle's bfloat16 rounding equals torch's float32 -> bfloat16
+    conversion (round to nearest, ties to even), including the tie cases."""
+    import torch
+    rng = np.random.default_rng(3)
+    v = np.concatenate([rng.normal(0, 10, 20000), rng.normal(0, 1e-3, 2000),
+                        [0.0, -0.0, 1.0, 1.00390625, 1.01171875, -3.5, 65504.0]]).astype(np.float32)
+    # exact ties: float32 values whose low 16 bits are 0x8000
+    ties = (rng.integers(0x3F800000 >> 16, 0x47000000 >> 16, 500, dtype=np.uint32) << 16) | 0x8000
+    v = np.concatenate([v, ties.astype(np.uint32).view(np.float32)])
+    want = torch.tensor(v).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(op.bf16_round(v), want)
+
+
+def test_bf16_forward_is_the_rounded_net():
+    """bf16=True changes the logits only by the bfloat16 rounding of the
+    activations and of W2, W3 (relative size ~2^-8), and every activation it
+    keeps is a bfloat16 value."""
+    rng = np.random.default_rng(4)
+    C, Lx, Ly = 32, 4, 4
+    th = op.init_params(rng, C)
+    x = np.array([1, 2, 0, 3, 0, 2, 1, 0, 2, 1, 2, 1, 0, 3, 1, 2])
+    l32, v32, _ = op.forward(th, C, Lx, Ly, x)
+    l16, v16, c16 = op.forward(th, C, Lx, Ly, x, bf16=True)
+    assert np.array_equal(v32, v16)
+    d = np.abs(l16[v16] - l32[v16]).max()
+    assert 0.0 < d < 2e-2 * max(1.0, np.abs(l32[v32]).max())
+    for h in c16[2], c16[4], c16[6]:            # h1, h2, h3 are bfloat16 values
+        assert np.array_equal(op.bf16_round(h), h)

@@ -52,8 +52,9 @@ _SIGNATURES = {
     "vmcx_to_bf16": (_I, [_I64, _P, _P, _P]),
     "vmcx_lp_n_params": (_I, [_I]),
     "vmcx_lp_n_moves": (_I, [_I, _I]),
-    "vmcx_lp_logits": (_I, [_P, _I, _I, _I, _I64, _P, _P, _P]),
-    "vmcx_lp_propose": (_I, [_P, _I, _I, _I, _I64, _P, _U64, _I64, _P, _P, _P, _P, _P, _P, _P]),
+    "vmcx_lp_logits": (_I, [_P, _I, _I, _I, _I64, _P, _P, _I, _P]),
+    "vmcx_lp_propose": (_I, [_P, _I, _I, _I, _I64, _P, _U64, _I64, _P, _P, _P, _P, _P, _P, _I,
+                             _P]),
     "vmcx_lp_accept": (_I, [_I64, _I, _U64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                             _P, _P]),
     "vmcx_lp_backward_workspace_bytes": (_I64, [_I, _I64]),

@@ -37,7 +37,7 @@ static inline int status_of(int rc) {
 
 extern "C" {
 
-int vmcx_abi_version(void) { return 7; }
+int vmcx_abi_version(void) { return 8; }
 
 const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
 
@@ -207,16 +207,17 @@ int vmcx_lp_n_params(int C) { return vmcx::lp_n_params(C); }
 int vmcx_lp_n_moves(int Lx, int Ly) { return vmcx::lp_n_moves(Lx, Ly); }
 
 int vmcx_lp_logits(const float *params, int C, int Lx, int Ly, int64_t B,
-                   const int64_t *configs, double *logits, void *stream) {
-  return status_of(vmcx::lp_logits(params, C, Lx, Ly, B, configs, logits, (hipStream_t)stream));
+                   const int64_t *configs, double *logits, int precision, void *stream) {
+  return status_of(vmcx::lp_logits(params, C, Lx, Ly, B, configs, logits, precision,
+                                   (hipStream_t)stream));
 }
 
 int vmcx_lp_propose(const float *params, int C, int Lx, int Ly, int64_t B,
                     const int64_t *configs, uint64_t seed, int64_t step, const double *u_choice,
                     double *logits_ws, int32_t *move, double *logq_fwd, int64_t *proposed,
-                    double *logq_rev, void *stream) {
+                    double *logq_rev, int precision, void *stream) {
   return status_of(vmcx::lp_propose(params, C, Lx, Ly, B, configs, seed, step, u_choice,
-                                    logits_ws, move, logq_fwd, proposed, logq_rev,
+                                    logits_ws, move, logq_fwd, proposed, logq_rev, precision,
                                     (hipStream_t)stream));
 }
 

@@ -127,10 +127,11 @@ int peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void 
 int lp_n_params(int C);
 int lp_n_moves(int Lx, int Ly);
 int lp_logits(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,
-              double *logits, hipStream_t st);
+              double *logits, int precision, hipStream_t st);
 int lp_propose(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,
                uint64_t seed, int64_t step, const double *u_choice, double *logits,
-               int32_t *move, double *logq_f, int64_t *pcfg, double *logq_r, hipStream_t st);
+               int32_t *move, double *logq_f, int64_t *pcfg, double *logq_r, int precision,
+               hipStream_t st);
 int lp_accept(int64_t B, int N, uint64_t seed, int64_t step, const double *u_acc,
               const int32_t *move, const int64_t *pcfg, const double *psign,
               const double *plog, const double *logq_f, const double *logq_r, int64_t *cfg,

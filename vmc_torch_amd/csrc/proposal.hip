@@ -235,6 +235,149 @@ __global__ void __launch_bounds__(LP_T)
   }
 }
 
+// BF16 matrix-core forward (C = 32).  Layer 1 (2 -> 32 channels) runs in
+// FP32 on the vector ALUs; layers 2 and 3 are implicit GEMMs
+//   z[co][col] = sum_{tap, ci} W[co][ci][tap] h[ci][nbr(col, tap)]
+// over the columns col = (configuration, site) of the workgroup, 32 x 32
+// output tiles of v_mfma_f32_32x32x16_bf16 (FP32 accumulation), one K chunk
+// = one tap and 16 input channels.  Activations are stored BF16 and
+// channel-minor ([slot][32], one zero slot per configuration) so that the 8
+// channels a lane contributes to a chunk are one 16-byte LDS read; the
+// layer's weights are staged BF16 in LDS in fragment order (chunk, lane, 8).
+// Bias and ReLU in FP32, the result rounded to BF16; the head accumulates in
+// FP64 from the BF16 activations.
+typedef __bf16 lp_bf8 __attribute__((ext_vector_type(8)));
+typedef float lp_f16v __attribute__((ext_vector_type(16)));
+
+constexpr int LPM_T = 256;         // 4 waves
+constexpr int LPM_COLS = 256;      // at most 8 column tiles per workgroup
+constexpr int LPM_RMAX = 288;      // slots G * (N + 1)
+constexpr int LPM_CH = 32;
+constexpr int LPM_KCH = 18;        // 9 taps x 2 halves of the 32 input channels
+
+__host__ __device__ inline int lpm_group(int N) {
+  int g = LPM_COLS / N;
+  if (g * (N + 1) > LPM_RMAX) g = LPM_RMAX / (N + 1);
+  return g < 1 ? 1 : g;
+}
+
+__global__ void __launch_bounds__(LPM_T)
+    lp_logits_mfma_kernel(const float *__restrict__ P, int Lx, int Ly, int G, int64_t B,
+                          const int64_t *__restrict__ cfg, double *__restrict__ logits) {
+  __shared__ __attribute__((aligned(16))) __bf16 act[2][LPM_RMAX * LPM_CH];
+  __shared__ __attribute__((aligned(16))) __bf16 wf[LPM_KCH * 64 * 8];
+  __shared__ int xs[LPM_RMAX];
+  __shared__ int ei[LP_EMAX], ej[LP_EMAX];
+  const int N = Lx * Ly, E = lp_n_edges(Lx, Ly), R = G * (N + 1), ncol = G * N;
+  const int64_t b0 = (int64_t)blockIdx.x * G;
+  const int t = threadIdx.x;
+  for (int idx = t; idx < R; idx += LPM_T) {
+    const int gg = idx / (N + 1), ss = idx - gg * (N + 1);
+    xs[idx] = (ss < N && b0 + gg < B) ? (int)cfg[(b0 + gg) * N + ss] : 0;
+  }
+  for (int idx = t; idx < G * LPM_CH; idx += LPM_T) {
+    const int gg = idx / LPM_CH, c = idx - gg * LPM_CH;
+    act[0][(gg * (N + 1) + N) * LPM_CH + c] = (__bf16)0.0f;
+    act[1][(gg * (N + 1) + N) * LPM_CH + c] = (__bf16)0.0f;
+  }
+  lp_edges(Lx, Ly, ei, ej);
+  __syncthreads();
+  const LpOff o = lp_offsets(LPM_CH);
+  // layer 1, FP32: thread = (column, half of the output channels)
+  {
+    const float *W1 = P + o.W1, *b1 = P + o.b1;
+    for (int idx = t; idx < 2 * ncol; idx += LPM_T) {
+      const int col = idx >> 1, half = idx & 1;
+      const int g = col / N, s = col - g * N, x = s / Ly, y = s - (s / Ly) * Ly;
+      const int base = g * (N + 1);
+      float hu[9], hd[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        const int xx = x + q / 3 - 1, yy = y + q % 3 - 1;
+        const int v = xs[base + ((xx >= 0 && xx < Lx && yy >= 0 && yy < Ly) ? xx * Ly + yy : N)];
+        hu[q] = (float)((v >> 1) & 1);
+        hd[q] = (float)(v & 1);
+      }
+      __bf16 *out = act[0] + (base + s) * LPM_CH;
+      for (int k = 0; k < 16; ++k) {
+        const int co = half * 16 + k;
+        float acc = b1[co];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc += W1[(co * 2) * 9 + q] * hu[q];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc += W1[(co * 2 + 1) * 9 + q] * hd[q];
+        out[co] = (__bf16)(acc > 0.0f ? acc : 0.0f);
+      }
+    }
+  }
+  const int wave = t >> 6, lane = t & 63, kh = lane >> 5;
+  for (int layer = 1; layer < 3; ++layer) {
+    const float *W = P + (layer == 1 ? o.W2 : o.W3), *bias = P + (layer == 1 ? o.b2 : o.b3);
+    const __bf16 *in = act[layer == 1 ? 0 : 1];
+    __bf16 *outb = act[layer == 1 ? 1 : 0];
+    __syncthreads();   // the previous layer's reads of wf and writes of `in` are done
+    for (int idx = t; idx < LPM_CH * LPM_CH * 9; idx += LPM_T) {
+      const int co = idx / (LPM_CH * 9), rem = idx - co * LPM_CH * 9;
+      const int ci = rem / 9, tap = rem - ci * 9;
+      const int chunk = tap * 2 + (ci >> 4), ln = co + 32 * ((ci & 15) >> 3);
+      wf[(chunk * 64 + ln) * 8 + (ci & 7)] = (__bf16)W[idx];
+    }
+    __syncthreads();
+    for (int tile = wave; tile * 32 < ncol; tile += LPM_T / 64) {
+      const int col = tile * 32 + (lane & 31);
+      const bool ok = col < ncol;
+      const int g = ok ? col / N : 0, s = ok ? col - g * N : 0;
+      const int x = s / Ly, y = s - (s / Ly) * Ly, base = g * (N + 1);
+      int nb[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        const int xx = x + q / 3 - 1, yy = y + q % 3 - 1;
+        nb[q] = base + ((ok && xx >= 0 && xx < Lx && yy >= 0 && yy < Ly) ? xx * Ly + yy : N);
+      }
+      lp_f16v acc;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 9; ++q)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const lp_bf8 a = *reinterpret_cast<const lp_bf8 *>(&wf[((q * 2 + h) * 64 + lane) * 8]);
+          const lp_bf8 bv =
+              *reinterpret_cast<const lp_bf8 *>(&in[nb[q] * LPM_CH + h * 16 + kh * 8]);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc, 0, 0, 0);
+        }
+      if (ok) {
+        // D lane l, register r: row (output channel) 8 (r / 4) + 4 (l / 32) + r % 4
+        __bf16 *out = outb + (base + s) * LPM_CH;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int co = 8 * (r >> 2) + 4 * kh + (r & 3);
+          const float v = acc[r] + bias[co];
+          out[co] = (__bf16)(v > 0.0f ? v : 0.0f);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const float *A = P + o.A, *bA = P + o.bA;
+  const __bf16 *h3 = act[0];
+  for (int idx = t; idx < G * 3 * E; idx += LPM_T) {
+    const int gg = idx / (3 * E), m = idx - gg * 3 * E;
+    if (b0 + gg >= B) break;
+    const int e = m / 3, tt = m - 3 * e, base = gg * (N + 1);
+    const int i = base + ei[e], j = base + ej[e];
+    double v = -INFINITY;
+    if (lp_valid(tt, xs[i], xs[j])) {
+      double acc = bA[tt];
+      for (int c = 0; c < LPM_CH; ++c)
+        acc += (double)A[tt * 2 * LPM_CH + c] * (double)(float)h3[i * LPM_CH + c] +
+               (double)A[tt * 2 * LPM_CH + LPM_CH + c] * (double)(float)h3[j * LPM_CH + c];
+      v = acc;
+    }
+    logits[(b0 + gg) * 3 * E + m] = v;
+  }
+}
+
 // one wave per configuration: ln q of the chosen move (sample = 1: draw it
 // by inverse CDF from u_choice / Philox and build the proposed
 // configuration).  Lane l owns the moves [l * per, (l + 1) * per); the
@@ -522,12 +665,21 @@ int lp_n_moves(int Lx, int Ly) {
 }
 
 int lp_logits(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,
-              double *logits, hipStream_t st) {
+              double *logits, int precision, hipStream_t st) {
   if (lp_check(C, Lx, Ly)) return 1;
   VMCX_CHECK_ARG(B >= 0, "B < 0");
+  VMCX_CHECK_ARG(precision == 0 || precision == 1, "precision must be 0 (fp32) or 1 (bf16)");
+  VMCX_CHECK_ARG(precision == 0 || C == LPM_CH, "the bf16 net needs C = 32");
   if (B == 0) return 0;
   VMCX_CHECK_ARG(P && cfg && logits, "null pointer");
   const int N = Lx * Ly;
+  if (precision == 1) {
+    const int G = lpm_group(N);
+    hipLaunchKernelGGL(lp_logits_mfma_kernel, dim3((unsigned)((B + G - 1) / G)), dim3(LPM_T), 0,
+                       st, P, Lx, Ly, G, B, cfg, logits);
+    VMCX_LAUNCHED();
+    return 0;
+  }
   // smallest block (64, 128, 256 threads) that leaves at most 1/8 of its
   // lanes without a site
   int threads = LP_T;
@@ -548,9 +700,9 @@ int lp_logits(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *c
 
 int lp_propose(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,
                uint64_t seed, int64_t step, const double *u_choice, double *logits,
-               int32_t *move, double *logq_f, int64_t *pcfg, double *logq_r,
+               int32_t *move, double *logq_f, int64_t *pcfg, double *logq_r, int precision,
                hipStream_t st) {
-  int rc = lp_logits(P, C, Lx, Ly, B, cfg, logits, st);
+  int rc = lp_logits(P, C, Lx, Ly, B, cfg, logits, precision, st);
   if (rc || B == 0) return rc;
   VMCX_CHECK_ARG(move && logq_f && pcfg && logq_r, "null pointer");
   const int N = Lx * Ly, E = lp_n_edges(Lx, Ly);
@@ -559,7 +711,7 @@ int lp_propose(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *
                      logits, seed, step, u_choice, cfg, move, logq_f, pcfg);
   VMCX_LAUNCHED();
   // reverse probability: the same move evaluated at the proposed configuration
-  rc = lp_logits(P, C, Lx, Ly, B, pcfg, logits, st);
+  rc = lp_logits(P, C, Lx, Ly, B, pcfg, logits, precision, st);
   if (rc) return rc;
   hipLaunchKernelGGL(lp_choose_kernel, dim3(g), dim3(64 * LPC_WAVES), 0, st, B, N, Lx, Ly, E, 0,
                      logits, seed, step, (const double *)nullptr, pcfg, move, logq_r,

@@ -402,15 +402,21 @@ def _lp_check(params, C, configs, Lx, Ly):
     return B, N
 
 
-def lp_logits(params, C, Lx, Ly, configs):
+LP_PRECISION = {"fp32": 0, "bf16": 1}
+
+
+def lp_logits(params, C, Lx, Ly, configs, precision="fp32"):
+    """Logits of every move [B, n_moves] (FP64, -inf for invalid moves).
+    precision "fp32": FP32 convolutions on the vector ALUs (any C <= 32);
+    "bf16": layers 2 and 3 on the BF16 matrix cores (C = 32)."""
     B, _ = _lp_check(params, C, configs, Lx, Ly)
     out = torch.empty(B, lp_n_moves(Lx, Ly), dtype=F64, device=configs.device)
     check(lib().vmcx_lp_logits(ptr(params), int(C), int(Lx), int(Ly), B, ptr(configs), ptr(out),
-                               stream_ptr(configs.device)))
+                               LP_PRECISION[precision], stream_ptr(configs.device)))
     return out
 
 
-def lp_propose(params, C, Lx, Ly, configs, seed, step, u_choice=None):
+def lp_propose(params, C, Lx, Ly, configs, seed, step, u_choice=None, precision="fp32"):
     """(move int32 [B], ln q(m|x) [B], proposed configs [B, N], ln q(m|x') [B])."""
     B, N = _lp_check(params, C, configs, Lx, Ly)
     dev = configs.device
@@ -423,7 +429,8 @@ def lp_propose(params, C, Lx, Ly, configs, seed, step, u_choice=None):
         require(u_choice, F64, "u_choice", (B,), dev)
     check(lib().vmcx_lp_propose(ptr(params), int(C), int(Lx), int(Ly), B, ptr(configs),
                                 int(seed) & (2 ** 64 - 1), int(step), ptr(u_choice), ptr(logits),
-                                ptr(move), ptr(lqf), ptr(prop), ptr(lqr), stream_ptr(dev)))
+                                ptr(move), ptr(lqf), ptr(prop), ptr(lqr), LP_PRECISION[precision],
+                                stream_ptr(dev)))
     return move, lqf, prop, lqr
 
 

@@ -87,14 +87,21 @@ class LearnedProposalSamplerGPU(SamplerGPU):
     net to the heat-bath distribution over the move set,
     q*(m|x) ~ |psi(m(x))|^2, by cross entropy.  The net (FP32, `params`) is
     size-independent: one instance can serve lattices of different shapes
-    (pass the same tensor).
+    (pass the same tensor).  precision: "bf16" (layers 2 and 3 on the BF16
+    matrix cores, C = 32) or "fp32"; default bf16 when C = 32.  The chain is
+    exact in either precision: x and x' are scored by the same net.
     """
 
     def __init__(self, Lx, Ly, channels=32, seed=0, params=None, generator=None,
-                 device=None):
+                 device=None, precision=None):
         from . import kernels
         self.k = kernels
         self.Lx, self.Ly, self.C = int(Lx), int(Ly), int(channels)
+        if precision is None:
+            precision = "bf16" if self.C == 32 else "fp32"
+        if precision not in ("bf16", "fp32") or (precision == "bf16" and self.C != 32):
+            raise ValueError("precision must be 'fp32' or 'bf16' (bf16 needs channels = 32)")
+        self.precision = precision
         self.seed = int(seed)
         self.step_counter = 0
         self.last_acceptance = None
@@ -131,7 +138,8 @@ class LearnedProposalSamplerGPU(SamplerGPU):
             if draws is not None:
                 uc, ua = draws[s]
             move, lqf, prop, lqr = self.k.lp_propose(self.params, self.C, self.Lx, self.Ly, fxs,
-                                                     self.seed, self.step_counter, uc)
+                                                     self.seed, self.step_counter, uc,
+                                                     precision=self.precision)
             ps, pl = model.forward_log(prop)
             self.k.lp_accept(fxs, sign, logabs, move, prop, ps, pl, lqf, lqr, self.seed,
                              self.step_counter, ua, n_acc)
@@ -160,7 +168,8 @@ class LearnedProposalSamplerGPU(SamplerGPU):
         if self.params.device != fxs.device:
             self.params = self.params.to(fxs.device)
         B = fxs.shape[0]
-        logits = self.k.lp_logits(self.params, self.C, self.Lx, self.Ly, fxs)
+        logits = self.k.lp_logits(self.params, self.C, self.Lx, self.Ly, fxs,
+                                  precision=self.precision)
         valid = torch.isfinite(logits)
         cand = self.k.lp_candidates(self.Lx, self.Ly, fxs)
         _, lc = model.forward_log(cand.reshape(-1, fxs.shape[1]))
