@@ -22,7 +22,10 @@ from vmc_torch_amd.hamiltonian import spinful_Fermi_Hubbard_square_lattice_torch
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(4, 2, 4, 4, 32), (6, 6, 16, 16, 64), (3, 3, 4, 3, 16)]   # Lx, Ly, nu, nd, hidden
+# Lx, Ly, nu, nd, hidden; the determinant layouts NFP = 8, 32, 8, 64, 16, 64
+# (nf = 35: 64 with identity padding)
+CASES = [(4, 2, 4, 4, 32), (6, 6, 16, 16, 64), (3, 3, 4, 3, 16), (8, 8, 32, 32, 32),
+         (4, 4, 5, 4, 32), (6, 6, 18, 17, 32)]
 
 
 def _model(Lx, Ly, nu, nd, hidden, gemm, device, scale=0.3, act="relu", seed=0):
@@ -84,7 +87,7 @@ def test_nnbf_logderiv_fp32(device, case):
     assert np.allclose(O32.cpu().numpy(), O, rtol=1e-5, atol=1e-6 * np.abs(Oo).max())
 
 
-@pytest.mark.parametrize("case", CASES[:2])
+@pytest.mark.parametrize("case", CASES[:2] + CASES[3:])
 def test_nnbf_local_energy_fp32(device, case):
     Lx, Ly, nu, nd, hid = case
     m, params = _model(Lx, Ly, nu, nd, hid, "fp32", device)

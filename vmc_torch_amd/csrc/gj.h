@@ -84,4 +84,138 @@ __device__ __forceinline__ GJResult gj_invert(double (&a)[NFP], int col, int gba
   return {logdet, parity, singular, my_step};
 }
 
+// Determinant only: LU with the same pivot rule on the same layout (lane k =
+// row k of B = A^T, register j = column j of B), eliminating below the pivot
+// only.  Each step is instantiated with its index as a template argument, so
+// step p touches registers p+1.. only (half the FMAs of Gauss-Jordan) and
+// the register array is never indexed at run time; |det| is accumulated as
+// a mantissa product and an exponent sum (frexp), one log per determinant.
+struct LuState {
+  double mprod;
+  int esum, parity;
+  uint64_t used;
+  bool singular;
+};
+
+template <int NFP, int P>
+struct LuStep {
+  static __device__ __forceinline__ void run(double (&a)[NFP], int col, int gbase,
+                                             double *prow, LuState &S) {
+    // pivot key: |a| rounded to float with its 6 low mantissa bits replaced
+    // by 63 - lane (largest magnitude to 2^-17, then the lowest lane); used
+    // lanes key 0.  NaN / Inf rank as 0 as in gj_invert.
+    const bool done = (S.used >> col) & 1ull;
+    const double mag = fabs(a[P]);
+    const float fm = (mag <= 1e300) ? (float)mag : 0.0f;
+    uint32_t key = done ? 0u : ((__float_as_uint(fm) & ~63u) | (uint32_t)(63 - col));
+#pragma unroll
+    for (int o = NFP / 2; o > 0; o >>= 1) key = max(key, (uint32_t)__shfl_xor((int)key, o, 64));
+    const int q = 63 - (int)(key & 63u);
+    const double piv = __shfl(a[P], gbase + q, 64);
+    if (!(piv != 0.0) || !(fabs(piv) <= 1e300)) S.singular = true;
+    if (P + 1 < NFP) {
+      if (col == q) {
+#pragma unroll
+        for (int j = P + 1; j < NFP; ++j) prow[j] = a[j];
+      }
+      wave_lds_fence();
+      if (!done && col != q) {
+        const double f = (piv != 0.0) ? a[P] / piv : 0.0;
+        // blocks of 16 pivot-row entries, kept apart by scheduling barriers
+        // so that at most 16 of them are live (no spills at NFP = 64)
+#pragma unroll
+        for (int j0 = P + 1; j0 < NFP; j0 += 16) {
+#pragma unroll
+          for (int j = j0; j < j0 + 16 && j < NFP; ++j) a[j] = fma(-f, prow[j], a[j]);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      wave_lds_fence();
+    }
+    if (piv != 0.0) {
+      int e;
+      S.mprod *= frexp(fabs(piv), &e);
+      S.esum += e;
+      S.mprod = frexp(S.mprod, &e);
+      S.esum += e;
+    }
+    S.parity += (piv < 0.0) ? 1 : 0;
+    S.parity += __popcll(S.used & ~((2ull << q) - 1ull));
+    S.used |= 1ull << q;
+    LuStep<NFP, P + 1>::run(a, col, gbase, prow, S);
+  }
+};
+
+template <int NFP>
+struct LuStep<NFP, NFP> {
+  static __device__ __forceinline__ void run(double (&)[NFP], int, int, double *, LuState &) {}
+};
+
+// One 64 x 64 matrix per wave: the pivot is the same for every lane, so the
+// arg-max is a DPP max-scan (row_shr 1, 2, 4, 8, row_bcast 15, 31; the wave
+// maximum ends in lane 63) and the pivot row is read with readlane into
+// scalar registers, no LDS.  Every lane applies the update (f = 1 on the
+// pivot lane, anything on finished lanes, whose registers are never read
+// again), so the FMAs need no lane mask.
+__device__ __forceinline__ double lu_readlane(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ uint32_t lu_wave_max(uint32_t k) {
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x111, 0xf, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x112, 0xf, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x114, 0xf, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x118, 0xf, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x142, 0xa, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x143, 0xc, 0xf, false));
+  return (uint32_t)__builtin_amdgcn_readlane((int)k, 63);
+}
+
+template <int P>
+struct LuStep64 {
+  static __device__ __forceinline__ void run(double (&a)[64], int col, LuState &S) {
+    const bool done = (S.used >> col) & 1ull;
+    const double mag = fabs(a[P]);
+    const float fm = (mag <= 1e300) ? (float)mag : 0.0f;
+    const uint32_t key = done ? 0u : ((__float_as_uint(fm) & ~63u) | (uint32_t)(63 - col));
+    const int q = 63 - (int)(lu_wave_max(key) & 63u);
+    const double piv = lu_readlane(a[P], q);
+    if (!(piv != 0.0) || !(fabs(piv) <= 1e300)) S.singular = true;
+    if (P + 1 < 64) {
+      const double f = (piv != 0.0) ? a[P] / piv : 0.0;
+#pragma unroll
+      for (int j = P + 1; j < 64; ++j) a[j] = fma(-f, lu_readlane(a[j], q), a[j]);
+    }
+    if (piv != 0.0) {
+      int e;
+      S.mprod *= frexp(fabs(piv), &e);
+      S.esum += e;
+      S.mprod = frexp(S.mprod, &e);
+      S.esum += e;
+    }
+    S.parity += (piv < 0.0) ? 1 : 0;
+    S.parity += __popcll(S.used & ~((2ull << q) - 1ull));
+    S.used |= 1ull << q;
+    LuStep64<P + 1>::run(a, col, S);
+  }
+};
+
+template <>
+struct LuStep64<64> {
+  static __device__ __forceinline__ void run(double (&)[64], int, LuState &) {}
+};
+
+template <int NFP>
+__device__ __forceinline__ GJResult lu_logdet(double (&a)[NFP], int col, int gbase,
+                                             double *prow) {
+  LuState S{1.0, 0, 0, 0ull, false};
+  if constexpr (NFP == 64)
+    LuStep64<0>::run(a, col, S);
+  else
+    LuStep<NFP, 0>::run(a, col, gbase, prow, S);
+  return {log(S.mprod) + (double)S.esum * 0.69314718055994530942, S.parity, S.singular, -1};
+}
+
 }  // namespace vmcx

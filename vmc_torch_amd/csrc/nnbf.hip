@@ -36,30 +36,55 @@ __device__ __forceinline__ __hip_bfloat16 to_t<__hip_bfloat16>(float v) {
 }
 
 // ------------------------------------------------------------ layer one
-// one block per configuration; z1[u] = b1[u] + sum over occupied orbitals
+// z1[w][u] = b1[u] + sum of W1[u][orbital] over the occupied spin-orbitals
+// (site ascending, up before down), h1 = act(z1).  A workgroup handles
+// L1_RB configurations x L1_UT hidden units: the W1 tile sits in LDS with
+// row stride 2N + 1 (odd, so the 64 lanes of a wave -- 64 consecutive units
+// -- read 64 different banks), each configuration's occupation is two 64-bit
+// masks from a wave ballot.
+constexpr int L1_RB = 64, L1_UT = 64;
+
 template <typename TI>
 __global__ void __launch_bounds__(256)
     nnbf_layer1_kernel(int64_t Bc, int N, int H, int act,
                        const int64_t *__restrict__ cfg, const float *__restrict__ W1,
                        const float *__restrict__ b1, float *z1, TI *h1) {
-  __shared__ int occ[128];
-  __shared__ int nocc;
-  const int64_t w = blockIdx.x;
-  if (threadIdx.x == 0) {
-    int k = 0;
-    const int64_t *c = cfg + w * N;
-    for (int s = 0; s < N; ++s) {
-      const int64_t x = c[s];
-      if (x & 2) occ[k++] = s;
-      if (x & 1) occ[k++] = N + s;
+  __shared__ float wt[L1_UT * (128 + 1)];
+  __shared__ uint64_t mu[L1_RB], md[L1_RB];
+  const int n_orb = 2 * N, ld = n_orb + 1;
+  const int64_t r0 = (int64_t)blockIdx.x * L1_RB;
+  const int u0 = blockIdx.y * L1_UT;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  for (int idx = t; idx < L1_UT * n_orb; idx += blockDim.x) {
+    const int uu = idx / n_orb, o = idx - uu * n_orb;
+    wt[uu * ld + o] = (u0 + uu < H) ? W1[(int64_t)(u0 + uu) * n_orb + o] : 0.0f;
+  }
+  for (int r = wave; r < L1_RB; r += blockDim.x / 64) {
+    const int64_t w = r0 + r;
+    int64_t x = 0;
+    if (w < Bc && lane < N) x = cfg[w * N + lane];
+    const uint64_t bu = __ballot((x & 2) != 0), bd = __ballot((x & 1) != 0);
+    if (lane == 0) {
+      mu[r] = bu;
+      md[r] = bd;
     }
-    nocc = k;
   }
   __syncthreads();
-  const int n_orb = 2 * N;
-  for (int u = threadIdx.x; u < H; u += blockDim.x) {
-    float z = b1[u];
-    for (int k = 0; k < nocc; ++k) z += W1[(int64_t)u * n_orb + occ[k]];
+  const int u = u0 + lane;
+  if (u >= H) return;
+  const float bias = b1[u];
+  for (int r = wave; r < L1_RB; r += blockDim.x / 64) {
+    const int64_t w = r0 + r;
+    if (w >= Bc) break;
+    const uint64_t bu = mu[r], bd = md[r];
+    uint64_t all = bu | bd;
+    float z = bias;
+    while (all) {
+      const int s = __ffsll((long long)all) - 1;
+      all &= all - 1;
+      if ((bu >> s) & 1) z += wt[lane * ld + s];
+      if ((bd >> s) & 1) z += wt[lane * ld + N + s];
+    }
     z1[w * H + u] = z;
     h1[w * H + u] = to_t<TI>(act_f(z, act));
   }
@@ -132,7 +157,8 @@ __device__ __forceinline__ void group_masks(const int64_t *cfg, int N, int col, 
   dn = d;
 }
 
-template <int NFP>
+// INV: also write A^-1 (Gauss-Jordan); otherwise the determinant only (LU)
+template <int NFP, bool INV>
 __global__ void __launch_bounds__(256)
     nnbf_det_kernel(int64_t Bc, int N, int nf, const int64_t *__restrict__ cfg,
                     const float *__restrict__ D, const float *__restrict__ M,
@@ -153,14 +179,15 @@ __global__ void __launch_bounds__(256)
   const bool count_ok = (__popcll(up) + __popcll(dn)) == nf;
   double a[NFP];
   nnbf_rows<NFP>(a, up, dn, N, nf, col, M, D + w * (int64_t)(2 * N * nf), bout, s);
-  const GJResult r = gj_invert<NFP>(a, col, grp * NFP, prow, qbuf);
+  const GJResult r = INV ? gj_invert<NFP>(a, col, grp * NFP, prow, qbuf)
+                         : lu_logdet<NFP>(a, col, grp * NFP, prow);
   const bool singular = r.singular || !count_ok;
   if (col == 0) {
     sign[w] = singular ? 0.0 : ((r.parity & 1) ? -1.0 : 1.0);
     logabs[w] = singular ? (count_ok ? log(1e-45) : __builtin_nan(""))
                          : fmax(r.logdet, log(1e-45));
   }
-  if (ainv != nullptr && r.my_step >= 0) {
+  if (INV && r.my_step >= 0) {
     double *dst = ainv + ((size_t)w * NFP + r.my_step) * NFP;
 #pragma unroll
     for (int c = 0; c < NFP; ++c) dst[qbuf[c]] = singular ? 0.0 : a[c];
@@ -274,7 +301,6 @@ __global__ void __launch_bounds__(256)
   const int p = prop[w];
   if (!(p & 1)) return;
   double *prow = reinterpret_cast<double *>(smem) + wl * NFP;
-  uint8_t *qbuf = reinterpret_cast<uint8_t *>(smem) + (size_t)per_block * NFP * 8 + wl * NFP;
   const int ni = (p >> 8) & 0xF, nj = (p >> 12) & 0xF;
   uint64_t up, dn;
   group_masks<NFP>(cfg + w * N, N, col, up, dn);
@@ -285,7 +311,7 @@ __global__ void __launch_bounds__(256)
        ((uint64_t)(nj & 1) << j);
   double a[NFP];
   nnbf_rows<NFP>(a, up, dn, N, nf, col, M, D + w * (int64_t)(2 * N * nf), bout, s);
-  const GJResult r = gj_invert<NFP>(a, col, grp * NFP, prow, qbuf);
+  const GJResult r = lu_logdet<NFP>(a, col, grp * NFP, prow);
   const double lp = r.singular ? log(1e-45) : fmax(r.logdet, log(1e-45));
   const double ratio = exp(2.0 * (lp - logabs[w]));
   if (!(u_acc[w] < ratio)) return;
@@ -616,9 +642,14 @@ static void launch_det(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
   constexpr int WPW = 64 / NFP;
   const int64_t per_block = 4 * WPW;
   const int64_t nblk = (rows + per_block - 1) / per_block;
-  hipLaunchKernelGGL(nnbf_det_kernel<NFP>, dim3((unsigned)nblk), dim3(256),
-                     (size_t)per_block * NFP * 9, st, rows, m.n_sites, m.n_f, cfg, L.D, m.M,
-                     m.bout, m.bf_scale, sign, logabs, ainv);
+  if (ainv)
+    hipLaunchKernelGGL((nnbf_det_kernel<NFP, true>), dim3((unsigned)nblk), dim3(256),
+                       (size_t)per_block * NFP * 9, st, rows, m.n_sites, m.n_f, cfg, L.D, m.M,
+                       m.bout, m.bf_scale, sign, logabs, ainv);
+  else
+    hipLaunchKernelGGL((nnbf_det_kernel<NFP, false>), dim3((unsigned)nblk), dim3(256),
+                       (size_t)per_block * NFP * 9, st, rows, m.n_sites, m.n_f, cfg, L.D, m.M,
+                       m.bout, m.bf_scale, sign, logabs, ainv);
 }
 
 // forward of `rows` configurations (rows <= chunk); ainv optional
@@ -626,13 +657,15 @@ static int forward_rows(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
                         const int64_t *cfg, double *sign, double *logabs, double *ainv,
                         hipStream_t st) {
   if (rows == 0) return 0;
+  const dim3 g1((unsigned)((rows + L1_RB - 1) / L1_RB),
+                (unsigned)((m.hidden + L1_UT - 1) / L1_UT));
   if (m.gemm == 1)
-    hipLaunchKernelGGL(nnbf_layer1_kernel<__hip_bfloat16>, dim3((unsigned)rows), dim3(256), 0,
-                       st, rows, m.n_sites, m.hidden, m.act, cfg, m.W1, m.b1, L.z1,
+    hipLaunchKernelGGL(nnbf_layer1_kernel<__hip_bfloat16>, g1, dim3(256), 0, st, rows,
+                       m.n_sites, m.hidden, m.act, cfg, m.W1, m.b1, L.z1,
                        (__hip_bfloat16 *)L.h1);
   else
-    hipLaunchKernelGGL(nnbf_layer1_kernel<float>, dim3((unsigned)rows), dim3(256), 0, st, rows,
-                       m.n_sites, m.hidden, m.act, cfg, m.W1, m.b1, L.z1, (float *)L.h1);
+    hipLaunchKernelGGL(nnbf_layer1_kernel<float>, g1, dim3(256), 0, st, rows, m.n_sites,
+                       m.hidden, m.act, cfg, m.W1, m.b1, L.z1, (float *)L.h1);
   VMCX_LAUNCHED();
   int rc = dense_layers(m, L, rows, st);
   if (rc) return rc;

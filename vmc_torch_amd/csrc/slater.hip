@@ -55,7 +55,8 @@ __device__ __forceinline__ void load_masks(const int64_t *cfg, int n_sites,
 // rotated by one after each step (NFP rotations restore the order).  The
 // in-place scheme leaves R[q_p][c] = B^-1[p][q_c] = A^-1[q_c][p] in lane
 // q_p, register c, which is scattered to ainv[p][q_c] on the way out.
-template <int NFP>
+// INV = false (no ainv requested): the determinant alone by LU (lu_logdet).
+template <int NFP, bool INV>
 __global__ void __launch_bounds__(256)
     refresh_kernel(int64_t B, int n_sites, int n_f, const double *__restrict__ M,
                    const int64_t *__restrict__ configs, double *ainv,
@@ -104,7 +105,8 @@ __global__ void __launch_bounds__(256)
   }
 
   bool singular = !count_ok;
-  const GJResult r = gj_invert<NFP>(a, col, grp * NFP, prow, qbuf);
+  const GJResult r = INV ? gj_invert<NFP>(a, col, grp * NFP, prow, qbuf)
+                         : lu_logdet<NFP>(a, col, grp * NFP, prow);
   singular = singular || r.singular;
   const double logdet = r.logdet;
   const int parity = r.parity;
@@ -119,7 +121,7 @@ __global__ void __launch_bounds__(256)
     }
     if (since_refresh) since_refresh[w] = 0;
   }
-  if (ainv != nullptr && my_step >= 0) {
+  if (INV && my_step >= 0) {
     // lane q_p holds column p of A^-1 with register c = row q_c
     double *dst = ainv + ((size_t)w * NFP + my_step) * NFP;
 #pragma unroll
@@ -475,9 +477,14 @@ static int launch_refresh(int64_t B, int n_sites, int n_f, const double *M,
   const int64_t per_block = waves * WPW;
   const int64_t nblk = (B + per_block - 1) / per_block;
   const size_t lds = (size_t)per_block * NFP * (sizeof(double) + 1);
-  hipLaunchKernelGGL(refresh_kernel<NFP>, dim3((unsigned)nblk), dim3(64 * waves),
-                     lds, st, B, n_sites, n_f, M, configs, ainv, slot, sign,
-                     logabs, since, only_if_since);
+  if (ainv)
+    hipLaunchKernelGGL((refresh_kernel<NFP, true>), dim3((unsigned)nblk), dim3(64 * waves),
+                       lds, st, B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since,
+                       only_if_since);
+  else
+    hipLaunchKernelGGL((refresh_kernel<NFP, false>), dim3((unsigned)nblk), dim3(64 * waves),
+                       lds, st, B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since,
+                       only_if_since);
   VMCX_LAUNCHED();
   return 0;
 }
