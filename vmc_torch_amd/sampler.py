@@ -90,10 +90,12 @@ class LearnedProposalSamplerGPU(SamplerGPU):
     (pass the same tensor).  precision: "bf16" (layers 2 and 3 on the BF16
     matrix cores, C = 32) or "fp32"; default bf16 when C = 32.  The chain is
     exact in either precision: x and x' are scored by the same net.
+    `step` (one sample, the sampler interface of the reference) makes
+    `proposals_per_step` proposals; `burn_in(n)` makes n of them.
     """
 
     def __init__(self, Lx, Ly, channels=32, seed=0, params=None, generator=None,
-                 device=None, precision=None):
+                 device=None, precision=None, proposals_per_step=1):
         from . import kernels
         self.k = kernels
         self.Lx, self.Ly, self.C = int(Lx), int(Ly), int(channels)
@@ -102,6 +104,9 @@ class LearnedProposalSamplerGPU(SamplerGPU):
         if precision not in ("bf16", "fp32") or (precision == "bf16" and self.C != 32):
             raise ValueError("precision must be 'fp32' or 'bf16' (bf16 needs channels = 32)")
         self.precision = precision
+        self.proposals_per_step = int(proposals_per_step)
+        if self.proposals_per_step < 1:
+            raise ValueError("proposals_per_step must be >= 1")
         self.seed = int(seed)
         self.step_counter = 0
         self.last_acceptance = None
@@ -152,7 +157,7 @@ class LearnedProposalSamplerGPU(SamplerGPU):
     @torch.inference_mode()
     def step(self, fxs, model, graph=None, compile=False, verbose=False, use_log_amp=False,
              **kwargs):
-        sign, logabs = self.sweeps(fxs, model, 1)
+        sign, logabs = self.sweeps(fxs, model, self.proposals_per_step)
         if use_log_amp:
             return fxs, (sign, logabs)
         return fxs, sign * torch.exp(logabs)
