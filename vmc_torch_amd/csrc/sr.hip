@@ -17,6 +17,7 @@
 #include <hip/hip_bf16.h>
 #include <rocblas/rocblas.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <mutex>
@@ -100,18 +101,22 @@ __global__ void __launch_bounds__(256)
 }
 
 // X[s][p] = (O[s][p] - mean[p]) * scale (X may alias O); optional low copy
+// Xlo[s][p] (row stride Np).  Thread = one column p, its mean in a register;
+// the workgroups of one column block stride through the rows (blockIdx.y),
+// so every load and store of a wave is 64 consecutive elements.
 template <typename T, typename TL>
 __global__ void __launch_bounds__(256)
     center_kernel(int64_t Ns, int64_t Np, const T *O, int64_t ldo,
                   const double *__restrict__ mean, double scale, T *X, int64_t ldx,
                   TL *Xlo) {
-  const int64_t n = Ns * Np;
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t s = idx / Np, p = idx - s * Np;
-    const double v = ((double)O[s * ldo + p] - mean[p]) * scale;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= Np) return;
+  const double m = mean[p];
+#pragma unroll 4
+  for (int64_t s = blockIdx.y; s < Ns; s += gridDim.y) {
+    const double v = ((double)O[s * ldo + p] - m) * scale;
     X[s * ldx + p] = (T)v;
-    if (Xlo) Xlo[idx] = (TL)v;
+    if (Xlo) Xlo[s * Np + p] = (TL)v;
   }
 }
 
@@ -495,15 +500,16 @@ static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, cons
   // column means (two-level compensated sums) and centring
   rc = xt_gemv_cols<T>(Ns, Np, O, ldo, nullptr, 1.0 / (double)Ns, partial, mean, st);
   if (rc) return rc;
-  const unsigned cgrid = 8192;
+  const unsigned cgx = (unsigned)((Np + 255) / 256);
+  const dim3 cgrid(cgx, (unsigned)std::max<int64_t>(1, std::min<int64_t>(Ns, 16384 / cgx)));
   if (gram == 2)
-    hipLaunchKernelGGL((center_kernel<T, __hip_bfloat16>), dim3(cgrid), dim3(256), 0, st, Ns,
+    hipLaunchKernelGGL((center_kernel<T, __hip_bfloat16>), cgrid, dim3(256), 0, st, Ns,
                        Np, O, ldo, mean, scale, X, ldx, (__hip_bfloat16 *)(w + L.Xlo));
   else if (gram == 1 && sizeof(T) == 8)
-    hipLaunchKernelGGL((center_kernel<T, float>), dim3(cgrid), dim3(256), 0, st, Ns, Np, O, ldo,
+    hipLaunchKernelGGL((center_kernel<T, float>), cgrid, dim3(256), 0, st, Ns, Np, O, ldo,
                        mean, scale, X, ldx, (float *)(w + L.Xlo));
   else
-    hipLaunchKernelGGL((center_kernel<T, float>), dim3(cgrid), dim3(256), 0, st, Ns, Np, O, ldo,
+    hipLaunchKernelGGL((center_kernel<T, float>), cgrid, dim3(256), 0, st, Ns, Np, O, ldo,
                        mean, scale, X, ldx, (float *)nullptr);
   VMCX_LAUNCHED();
   const int use_given = std::isfinite(energy_mean) ? 1 : 0;
