@@ -63,6 +63,11 @@ WORKLOADS = {
         gram="bf16", hidden=256, bf_scale=0.01, mlp="bf16",
         desc=("6x6 OBC Hubbard t=1 U=8 16u+16d, NNBF 72-256-256-2304 ReLU, bf_scale 0.01 "
               "(678,912 params), BF16 GEMMs, 4096 walkers")),
+    "hubbard8x8_nnbf": dict(
+        kind="nnbf", Lx=8, Ly=8, nu=32, nd=32, B=4096, burn=200, shift=1e-3, lr=0.05,
+        gram="fp64", o_dtype="fp64", hidden=256, bf_scale=0.01, mlp="bf16",
+        desc=("8x8 OBC Hubbard t=1 U=8 32u+32d, NNBF 128-256-256-8192 ReLU, bf_scale 0.01 "
+              "(2,212,352 params), BF16 GEMMs, FP64 O and Gram, 4096 walkers")),
     "hubbard4x4_nnftns": dict(
         kind="peps", Lx=4, Ly=4, nu=8, nd=8, B=2048, burn=100, shift=5e-4, lr=0.1,
         gram="fp64", D=4, hidden=128, eta=1.0,
@@ -95,9 +100,16 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="hubbard8x8_slater", choices=sorted(WORKLOADS))
-    p.add_argument("--gram", default=None, choices=["fp64", "fp32", "bf16"],
+    p.add_argument("--gram", default=None, choices=["fp64", "fp32", "bf16", "bf16x3"],
                    help="Gram precision (default: the workload's)")
     p.add_argument("--refresh-every", type=int, default=64)
+    p.add_argument("--mlp", default=None, choices=["fp32", "bf16"],
+                   help="NNBF dense-layer precision (default: the workload's)")
+    p.add_argument("--o-dtype", default=None, choices=["fp32", "fp64"],
+                   help="NNBF log-derivative matrix O (default: the workload's, else fp32)")
+    p.add_argument("--sampler", default="plain", choices=["plain", "learned"],
+                   help="NNBF: plain sweeps, or the learned proposal (|E|/4 proposals per "
+                        "sample, the net trained before and every 10 SR steps)")
     p.add_argument("--tau-sweeps", type=int, default=64)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -319,26 +331,50 @@ class NnbfRun:
         self.k, self.a, self.w, self.H, self.fxs = kernels, a, w, H, fxs
         N, nf, B = w["Lx"] * w["Ly"], w["nu"] + w["nd"], w["B"]
         self.N, self.nf, self.B = N, nf, B
+        self.mlp = a.mlp or w["mlp"]
         self.model = NNBF_GPU(N, nf, hidden_dim=w["hidden"], activation="relu",
-                              bf_scale=w["bf_scale"], gemm=w["mlp"], init="xavier", m_std=0.1,
+                              bf_scale=w["bf_scale"], gemm=self.mlp, init="xavier", m_std=0.1,
                               generator=torch.Generator().manual_seed(0)).to(dev)
         self.Np = self.model.n_params
         self.e_sweep = H.graph.edge_tensor(dev, "sweep")
         self.e_graph = H.graph.edge_tensor(dev, "graph")
-        self.O = torch.empty(B, self.Np, dtype=torch.float32, device=dev)
+        self.proposals_per_step = self.e_sweep.shape[0]
+        self.lp = None
+        if a.sampler == "learned":
+            from vmc_torch_amd.sampler import LearnedProposalSamplerGPU
+            self.lp = LearnedProposalSamplerGPU(w["Lx"], w["Ly"], channels=32, seed=0x1EA,
+                                                generator=torch.Generator().manual_seed(0),
+                                                device=dev)
+            self.proposals_per_step = max(1, self.e_sweep.shape[0] // 4)
+            self.n_steps = 0
+        self.o_dtype = a.o_dtype or w.get("o_dtype", "fp32")
+        o_dtype = torch.float64 if self.o_dtype == "fp64" else torch.float32
+        self.O = torch.empty(B, self.Np, dtype=o_dtype, device=dev)
         self.dp = torch.empty(self.Np, dtype=torch.float64, device=dev)
         self.ws = self.k.SRWorkspace()
 
     def burn(self, n, seed, off):
         self.model.mcmc_sweep(self.fxs, self.e_sweep, n, seed, off)
+        if self.lp is not None:       # train the proposal net on the chain's walkers
+            for _ in range(100):
+                self.lp.sweeps(self.fxs, self.model, 2)
+                self.lp.train_step(self.fxs[:1024], self.model, lr=0.05)
 
     def local_energy(self):
         return self.model.local_energy(self.fxs, self.H)
 
+    def _sample(self, seed, off):
+        """One sample per walker; returns the accepted moves per walker."""
+        if self.lp is None:
+            acc0 = self.model.walker_state(self.fxs)["n_accept"].clone()
+            _, _, nacc = self.model.mcmc_sweep(self.fxs, self.e_sweep, 1, seed, off)
+            return nacc - acc0
+        self.lp.sweeps(self.fxs, self.model, self.proposals_per_step)
+        return self.lp.last_acceptance[0]
+
     def step(self, mark, seed, off, gram, phases):
         m = [mark()]
-        acc0 = self.model.walker_state(self.fxs)["n_accept"].clone()
-        _, _, nacc = self.model.mcmc_sweep(self.fxs, self.e_sweep, 1, seed, off)
+        nacc = self._sample(seed, off)
         m.append(mark())
         E = self.local_energy()
         m.append(mark())
@@ -354,14 +390,18 @@ class NnbfRun:
             self.k.sgd_update(p.data.view(-1), self.dp[off_p:off_p + n], self.w["lr"])
             off_p += n
         self.model.invalidate()
+        if self.lp is not None:
+            self.n_steps += 1
+            if self.n_steps % 10 == 0:
+                self.lp.train_step(self.fxs[:1024], self.model, lr=0.05)
         m.append(mark())
-        return m, E, nacc - acc0, stats, info
+        return m, E, nacc, stats, info
 
     def pre_tau(self):
         pass
 
     def tau_sample(self, seed, off):
-        self.model.mcmc_sweep(self.fxs, self.e_sweep, 1, seed, off)
+        self._sample(seed, off)
         return self.local_energy()
 
     def roofline_items(self):
@@ -370,7 +410,7 @@ class NnbfRun:
         proposal and one per connected configuration.  logderiv: the O write."""
         H, nD = self.w["hidden"], 2 * self.N * self.nf
         mlp = 2.0 * (H * H + H * nD)
-        n_prop = self.e_sweep.shape[0] * self.B
+        n_prop = self.proposals_per_step * self.B
         x = self.fxs
         e = self.e_graph.long()
         n_conn = 0
@@ -381,7 +421,7 @@ class NnbfRun:
         return {
             "sweep": ("mfma_bf16", n_prop * mlp),
             "local_energy": ("mfma_bf16", n_conn * mlp),
-            "logderiv": ("hbm", self.B * self.Np * 4.0),
+            "logderiv": ("hbm", self.B * self.Np * float(self.O.element_size())),
         }
 
     def oracle_params(self):
@@ -414,6 +454,7 @@ class PepsRun(NnbfRun):
         self.O = torch.empty(B, self.Np, dtype=odt, device=dev)
         self.dp = torch.empty(self.Np, dtype=torch.float64, device=dev)
         self.ws = self.k.SRWorkspace()
+        self.lp, self.proposals_per_step = None, self.e_sweep.shape[0]
 
     def roofline_items(self):
         """Sweep and local energy: FP64 flops of every amplitude evaluated
@@ -455,6 +496,9 @@ def main():
         sys.exit(2)
     if WORKLOADS[a.workload]["kind"] == "mixed_lp":
         return main_mixed_lp(a, WORKLOADS[a.workload])
+    if a.sampler == "learned" and WORKLOADS[a.workload]["kind"] != "nnbf":
+        print(json.dumps({"error": "--sampler learned is wired for the NNBF workloads"}))
+        sys.exit(2)
     import numpy as np
     import torch
 
@@ -529,7 +573,8 @@ def main():
     phase_names = ["center", "gram", "factor", "solve", "dp_stats", "chol_diag_launch"]
     phase_ms = {k: sum(p[i] for p in sr_phases) / len(sr_phases)
                 for i, k in enumerate(phase_names)}
-    acc_rate = float(torch.stack(accs).double().mean()) / run.e_sweep.shape[0]
+    acc_rate = (float(torch.stack(accs).double().mean())
+                / getattr(run, "proposals_per_step", run.e_sweep.shape[0]))
     e_site = [float(v) / N for v in torch.stack(E_hist).mean(dim=1)]
 
     # tau_int of E_loc at fixed parameters, one sample per sweep (as in the loop)
@@ -563,8 +608,9 @@ def main():
                        "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
                        "frac": work / secs / PEAK_BF16,
                        "note": "dense-layer flops of every amplitude evaluated, BF16 operands"}
-    peak_sr = {"fp64": PEAK_FP64, "fp32": PEAK_FP32_MFMA, "bf16": PEAK_BF16}[gram]
-    gram_flops = float(Ns) * Ns * Np * (1 if gram == "fp64" else 2)
+    peak_sr = {"fp64": PEAK_FP64, "fp32": PEAK_FP32_MFMA, "bf16": PEAK_BF16,
+               "bf16x3": PEAK_BF16}[gram]
+    gram_flops = float(Ns) * Ns * Np * {"fp64": 1, "fp32": 2, "bf16": 2, "bf16x3": 4}[gram]
     chol_flops = float(Ns) ** 3 / 3
     kern["sr"] = {"ms": kt["sr"], "phases_ms": phase_ms}
     kern["update"] = {"ms": kt["update"]}
@@ -578,8 +624,9 @@ def main():
         return {"ms": ms, "bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9,
                 "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / PEAK_HBM}
     kern["sr.gram"] = flop_roof(gram_flops, phase_ms["gram"], peak_sr,
-                                "algorithmic flops Ns^2 Np (FP64, symmetric product) or "
-                                "2 Ns^2 Np (reduced-precision GEMM); FP64: the matrix-core "
+                                "algorithmic flops Ns^2 Np (FP64, symmetric product), "
+                                "2 Ns^2 Np (reduced-precision GEMM) or 4 Ns^2 Np (bf16x3: "
+                                "two full products); FP64: the matrix-core "
                                 "SYRK kernel (one workgroup per 128x128 lower tile, the "
                                 "reduction split when the triangle has few tiles) plus the "
                                 "pass that adds the split partials")
@@ -590,7 +637,7 @@ def main():
     if gram == "fp64":
         cbytes = 2.0 * Ns * Np * osz + Ns * Np * osz          # read O twice, write X
     else:
-        cbytes = 2.0 * Ns * Np * osz + Ns * Np * (osz + 2)    # + in-place O and BF16/FP32 copy
+        cbytes = 2.0 * Ns * Np * osz + Ns * Np * (osz + (4 if gram == "bf16x3" else 2))
     kern["sr.center"] = hbm_roof(cbytes, phase_ms["center"])
     # one launch of the diagonal-block Cholesky kernel: Nb^3/3 (factor) +
     # Nb^3/3 (block inverse) flops, Nb = 128; Ns/128 launches per SR call
@@ -622,7 +669,10 @@ def main():
     elif w["kind"] == "peps":
         dtype = "f64" if gram == "fp64" else f"f64/f32-O/{gram}-gram/f64-cholesky"
     else:
-        dtype = f"{w['mlp']}-gemm/f64-det/f32-O/{gram}-gram/f64-cholesky"
+        o = "f64" if run.o_dtype == "fp64" else "f32"
+        dtype = f"{run.mlp}-gemm/f64-det/{o}-O/{gram}-gram/f64-cholesky"
+        if a.sampler == "learned":
+            dtype += "/bf16-proposal-net"
     out = {
         "metric": "effective samples/s + SR iters/s at 8x8 Hubbard; |dE/site| vs FP64 oracle",
         "value": ess_per_s,
@@ -635,6 +685,8 @@ def main():
         "config": {"workload": a.workload, "description": w["desc"], "walkers": B,
                    "samples_per_sr_iter": Ns, "n_params": Np, "burn_in_sweeps": w["burn"],
                    "gram": gram, "diag_shift": w["shift"], "lr": w["lr"],
+                   "sampler": a.sampler,
+                   "proposals_per_sample": getattr(run, "proposals_per_step", None),
                    "parallelism": "walkers on 1 GPU"},
         "sr_iters_per_s": sr_iters,
         "walker_sweeps_per_s": samples_per_s,

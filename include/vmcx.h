@@ -135,15 +135,19 @@ int vmcx_hubbard_conn_fill(int64_t B, int n_sites, int n_edges,
  *   overwritten by Ot, saving an Ns x Np copy).  E: [Ns] float64.
  *   energy_mean: the mean used to centre E; NaN = the sample mean.
  *   gram_dtype: 0 = FP64 Gram (SYRK; needs FP64 O); 1 = FP32 and 2 = BF16
- *   operands with FP32 accumulation.  The shifted Gram is factorised by a
- *   blocked FP64 Cholesky.  max_iter > 0: preconditioned conjugate gradients
- *   on the exact operator Ot Ot^T + shift (FP64 accumulation), with that
- *   factor as the preconditioner, until ||res|| <= tol ||eps||.
+ *   operands with FP32 accumulation; 3 = three BF16 products of the split
+ *   Ot = hi + lo (hi hi^T + hi lo^T + lo hi^T, ~2^-16 operand rounding).
+ *   The shifted Gram is factorised by a blocked FP64 Cholesky.  max_iter > 0:
+ *   preconditioned conjugate gradients on the exact operator Ot Ot^T + shift
+ *   (FP64 accumulation), with that factor as the preconditioner, until
+ *   ||res|| <= tol ||eps||; a reduced-precision Gram that is not positive
+ *   definite at the shift gets its preconditioner shift raised tenfold (up to
+ *   six times) while the operator keeps diag_shift.
  *   workspace: >= vmcx_sr_workspace_bytes(Ns, Np, o_dtype, gram_dtype, flags).
  *   stats (may be NULL): [5] float64 = mean E, var E, ||T alpha - eps|| /
  *   ||eps||, ||g|| (g = Ot^T eps, the energy gradient), PCG iterations.
- *   info (may be NULL): Cholesky info (0 = success, k > 0 = leading minor k
- *   not positive).
+ *   info (may be NULL): Cholesky info of the factor used (0 = success, k > 0
+ *   = leading minor k not positive).
  *   phase_ms (HOST pointer, may be NULL): [6] float = milliseconds of the
  *   centring, Gram, factorisation, solve + PCG and dp / statistics phases,
  *   and [5] the mean duration of one diagonal-block launch of the

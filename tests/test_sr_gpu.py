@@ -62,7 +62,7 @@ def test_minsr_reports_first_nonpositive_minor(device, Ns, Np):
 
 
 @pytest.mark.parametrize("odt", [torch.float64, torch.float32])
-@pytest.mark.parametrize("gram", ["fp32", "bf16"])
+@pytest.mark.parametrize("gram", ["fp32", "bf16", "bf16x3"])
 @pytest.mark.parametrize("shift", [1e-2, 1e-4])
 def test_minsr_mixed_precision_pcg(device, odt, gram, shift):
     """Reduced-precision Gram + FP64 Cholesky as preconditioner + PCG on the
@@ -78,6 +78,42 @@ def test_minsr_mixed_precision_pcg(device, odt, gram, shift):
     tol = 1e-6 if odt == torch.float64 else 2e-5
     assert _rel(dp.cpu().numpy(), want) < tol
     assert stats[4].item() <= 60
+
+
+def test_minsr_split_bf16_gram_needs_fewer_iterations(device):
+    """The three-product BF16 Gram (hi hi^T + hi lo^T + lo hi^T) is a closer
+    preconditioner than the one-product BF16 Gram: fewer PCG iterations to
+    the same tolerance, same solution."""
+    O, E = _problem(600, 400, 9)
+    want = osr.minsr(O, E, E.mean(), 1e-4)
+    Ed = torch.tensor(E, device=device)
+    it = {}
+    for gram in ("bf16", "bf16x3"):
+        dp, stats, info = kernels.sr_minsr(torch.tensor(O, device=device, dtype=torch.float32),
+                                           Ed, None, 1e-4, gram, max_iter=100, tol=1e-10)
+        assert int(info.item()) == 0
+        assert _rel(dp.cpu().numpy(), want) < 2e-5
+        it[gram] = stats[4].item()
+    print(f"  PCG iterations: bf16 {it['bf16']:.0f}, bf16x3 {it['bf16x3']:.0f}")
+    assert it["bf16x3"] < it["bf16"]
+
+
+@pytest.mark.parametrize("gram", ["bf16", "bf16x3"])
+def test_minsr_indefinite_low_precision_gram(device, gram):
+    """More samples than parameters and a tiny shift: the BF16 Gram's
+    rounding leaves it indefinite at the shift.  The preconditioner's shift
+    is raised until its Cholesky succeeds (info 0); PCG still solves the
+    exact system (operator shift unchanged) to the FP64 solution."""
+    O, E = _problem(400, 150, 10)
+    shift = 1e-7
+    want = osr.minsr(O, E, E.mean(), shift)
+    dp, stats, info = kernels.sr_minsr(torch.tensor(O, device=device, dtype=torch.float64),
+                                       torch.tensor(E, device=device), None, shift, gram,
+                                       max_iter=400, tol=1e-10)
+    print(f"  {gram}: info {int(info.item())}, PCG iterations {stats[4].item():.0f}, "
+          f"residual {stats[2].item():.2e}, dp error {_rel(dp.cpu().numpy(), want):.2e}")
+    assert int(info.item()) == 0
+    assert _rel(dp.cpu().numpy(), want) < 1e-5
 
 
 def test_minsr_inplace_fp32(device):

@@ -120,6 +120,43 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Split form of the centred matrix for the three-product BF16 Gram:
+// X[s][p] = v (v = (O - mean) * scale, may alias O), Xhi = bf16(v),
+// Xlo = bf16(v - Xhi), so Xhi + Xlo carries v to ~2^-16.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    center_split_kernel(int64_t Ns, int64_t Np, const T *O, int64_t ldo,
+                        const double *__restrict__ mean, double scale, T *X, int64_t ldx,
+                        __hip_bfloat16 *Xhi, __hip_bfloat16 *Xlo) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= Np) return;
+  const double m = mean[p];
+#pragma unroll 4
+  for (int64_t s = blockIdx.y; s < Ns; s += gridDim.y) {
+    const double v = ((double)O[s * ldo + p] - m) * scale;
+    X[s * ldx + p] = (T)v;
+    const __hip_bfloat16 hi = __float2bfloat16((float)v);
+    Xhi[s * Np + p] = hi;
+    Xlo[s * Np + p] = __float2bfloat16((float)(v - (double)__bfloat162float(hi)));
+  }
+}
+
+// T (column-major, lower used) <- G + H + H^T (+ shift on the diagonal):
+// the three-product Gram hi hi^T + hi lo^T + lo hi^T from G = hi hi^T and
+// H = hi lo^T (FP32, column-major Ns x Ns)
+__global__ void __launch_bounds__(256)
+    shift3_kernel(int64_t Ns, const float *__restrict__ G, const float *__restrict__ H,
+                  double *T, double shift) {
+  const int64_t n = Ns * Ns;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = idx / Ns, i = idx - j * Ns;
+    double v = (double)G[idx] + (double)H[idx] + (double)H[i * Ns + j];
+    if (i == j) v += shift;
+    T[idx] = v;
+  }
+}
+
 // One block: mean and variance of E (Neumaier), eps = (E - Em) * scale.
 // stats: [0] mean, [1] var (<E^2> - <E>^2)
 __global__ void __launch_bounds__(1024)
@@ -273,7 +310,7 @@ rocblas_handle blas_handle_for(hipStream_t st) {
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct SrLayout {
-  size_t X, Xlo, T, Tlo, Tpart, partial, mean, eps, alpha, r, z, pv, q, u, stw, info, dinv, t1,
+  size_t X, Xlo, Xlo2, T, Tlo, Tlo2, Tpart, partial, mean, eps, alpha, r, z, pv, q, u, stw, info, dinv, t1,
       t2, total;
 };
 
@@ -303,10 +340,12 @@ static SrLayout sr_layout(int64_t Ns, int64_t Np, int o_dtype, int gram, int inp
   const int R = kColsumChunks;
   const size_t osz = o_dtype == 0 ? 8 : 4;
   L.X = take(inplace ? 0 : (size_t)Ns * Np * osz);
-  const bool need_lo = (gram == 2) || (gram == 1 && o_dtype == 0);
-  L.Xlo = take(need_lo ? (size_t)Ns * Np * (gram == 2 ? 2 : 4) : 0);
+  const bool need_lo = (gram >= 2) || (gram == 1 && o_dtype == 0);
+  L.Xlo = take(need_lo ? (size_t)Ns * Np * (gram >= 2 ? 2 : 4) : 0);
+  L.Xlo2 = take(gram == 3 ? (size_t)Ns * Np * 2 : 0);
   L.T = take((size_t)Ns * Ns * 8);
   L.Tlo = take(gram ? (size_t)Ns * Ns * 4 : 0);
+  L.Tlo2 = take(gram == 3 ? (size_t)Ns * Ns * 4 : 0);
   const int ns = gram == 0 ? gram_splits(Ns, Np) : 1;
   L.Tpart = take(ns > 1 ? (size_t)(ns - 1) * Ns * Ns * 8 : 0);
   L.partial = take((size_t)2 * R * Np * 8);
@@ -328,7 +367,7 @@ static SrLayout sr_layout(int64_t Ns, int64_t Np, int o_dtype, int gram, int inp
 }
 
 int64_t sr_workspace_bytes(int64_t Ns, int64_t Np, int o_dtype, int gram, int flags) {
-  if (Ns <= 0 || Np <= 0 || gram < 0 || gram > 2 || o_dtype < 0 || o_dtype > 1) return -1;
+  if (Ns <= 0 || Np <= 0 || gram < 0 || gram > 3 || o_dtype < 0 || o_dtype > 1) return -1;
   return (int64_t)sr_layout(Ns, Np, o_dtype, gram, flags & 1).total;
 }
 
@@ -502,7 +541,11 @@ static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, cons
   if (rc) return rc;
   const unsigned cgx = (unsigned)((Np + 255) / 256);
   const dim3 cgrid(cgx, (unsigned)std::max<int64_t>(1, std::min<int64_t>(Ns, 16384 / cgx)));
-  if (gram == 2)
+  if (gram == 3)
+    hipLaunchKernelGGL(center_split_kernel<T>, cgrid, dim3(256), 0, st, Ns, Np, O, ldo, mean,
+                       scale, X, ldx, (__hip_bfloat16 *)(w + L.Xlo),
+                       (__hip_bfloat16 *)(w + L.Xlo2));
+  else if (gram == 2)
     hipLaunchKernelGGL((center_kernel<T, __hip_bfloat16>), cgrid, dim3(256), 0, st, Ns,
                        Np, O, ldo, mean, scale, X, ldx, (__hip_bfloat16 *)(w + L.Xlo));
   else if (gram == 1 && sizeof(T) == 8)
@@ -545,7 +588,20 @@ static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, cons
                               npi, &onef, Xg, ab, ldg, Xg, ab, ldg, &zerof, Tlo,
                               rocblas_datatype_f32_r, nsi, Tlo, rocblas_datatype_f32_r, nsi,
                               rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0));
-    hipLaunchKernelGGL(shift_kernel<float>, dim3(4096), dim3(256), 0, st, Ns, Tlo, T64, shift);
+    if (gram == 3) {
+      // hi lo^T; the Gram is hi hi^T + hi lo^T + (hi lo^T)^T
+      float *Tlo2 = (float *)(w + L.Tlo2);
+      VMCX_BLAS(rocblas_gemm_ex(h, rocblas_operation_transpose, rocblas_operation_none, nsi,
+                                nsi, npi, &onef, Xg, ab, ldg, (const void *)(w + L.Xlo2), ab,
+                                ldg, &zerof, Tlo2, rocblas_datatype_f32_r, nsi, Tlo2,
+                                rocblas_datatype_f32_r, nsi, rocblas_datatype_f32_r,
+                                rocblas_gemm_algo_standard, 0, 0));
+      hipLaunchKernelGGL(shift3_kernel, dim3(4096), dim3(256), 0, st, Ns, Tlo, Tlo2, T64,
+                         shift);
+    } else {
+      hipLaunchKernelGGL(shift_kernel<float>, dim3(4096), dim3(256), 0, st, Ns, Tlo, T64,
+                         shift);
+    }
     VMCX_LAUNCHED();
   }
   timer.mark();
@@ -555,6 +611,31 @@ static int sr_minsr_t(int64_t Ns, int64_t Np, T *O, int64_t ldo, int flags, cons
   rc = chol_factor((void *)h, Ns, T64, Ns, dinv, iinfo, st, phase_ms ? phase_ms + 5 : nullptr,
                    t1, t2);
   if (rc) return rc;
+  if (gram != 0 && max_iter > 0) {
+    // The reduced-precision Gram only preconditions CG on the exact operator.
+    // Its rounding (~2^-8 |T| in BF16) can leave it indefinite at the given
+    // shift; then the preconditioner's shift is raised tenfold until the
+    // factorisation succeeds (up to 10^6 x), the operator keeps `shift`.
+    double pc_shift = shift;
+    for (int attempt = 0; attempt < 6; ++attempt) {
+      int32_t hinfo = 0;
+      VMCX_HIP(hipMemcpyAsync(&hinfo, iinfo, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+      VMCX_HIP(hipStreamSynchronize(st));
+      if (hinfo == 0) break;
+      pc_shift = (pc_shift > 0.0 ? pc_shift : 1e-8) * 10.0;
+      if (gram == 3)
+        hipLaunchKernelGGL(shift3_kernel, dim3(4096), dim3(256), 0, st, Ns,
+                           (const float *)(w + L.Tlo), (const float *)(w + L.Tlo2), T64,
+                           pc_shift);
+      else
+        hipLaunchKernelGGL(shift_kernel<float>, dim3(4096), dim3(256), 0, st, Ns,
+                           (const float *)(w + L.Tlo), T64, pc_shift);
+      VMCX_LAUNCHED();
+      VMCX_HIP(hipMemcpyAsync(t1, eps, Ns * sizeof(double), hipMemcpyDeviceToDevice, st));
+      rc = chol_factor((void *)h, Ns, T64, Ns, dinv, iinfo, st, nullptr, t1, t2);
+      if (rc) return rc;
+    }
+  }
   timer.mark();
   rc = chol_solve_bwd(Ns, chol_panels(Ns, dinv), Ns, dinv, t2, alpha, st);
   if (rc) return rc;
@@ -632,7 +713,8 @@ int sr_minsr(int64_t Ns, int64_t Np, void *O, int o_dtype, int64_t ldo, int flag
   VMCX_CHECK_ARG(Ns <= 0x7fffffff && Np <= 0x7fffffff, "Ns / Np exceed 32-bit BLAS");
   VMCX_CHECK_ARG(ldo >= Np, "ldo < Np");
   VMCX_CHECK_ARG(o_dtype == 0 || o_dtype == 1, "o_dtype must be 0 (f64) or 1 (f32)");
-  VMCX_CHECK_ARG(gram >= 0 && gram <= 2, "gram must be 0 (f64), 1 (f32) or 2 (bf16)");
+  VMCX_CHECK_ARG(gram >= 0 && gram <= 3,
+                 "gram must be 0 (f64), 1 (f32), 2 (bf16) or 3 (three-product bf16)");
   VMCX_CHECK_ARG(!(gram == 0 && o_dtype == 1), "an FP64 Gram needs FP64 O");
   VMCX_CHECK_ARG(max_iter >= 0 && max_iter <= 1000, "max_iter must be in [0, 1000]");
   VMCX_CHECK_ARG(O && E && ws && dp, "null pointer");

@@ -162,7 +162,7 @@ __all__ = ["SlaterWalkerState", "slater_nfp", "slater_refresh", "slater_sweep",
            "slater_local_energy", "slater_logderiv", "hubbard_get_conn", "_lib"]
 
 
-GRAM_CODES = {"fp64": 0, "fp32": 1, "bf16": 2}
+GRAM_CODES = {"fp64": 0, "fp32": 1, "bf16": 2, "bf16x3": 3}
 
 
 class SRWorkspace:
