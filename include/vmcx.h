@@ -399,6 +399,20 @@ int vmcx_lp_candidates(int64_t B, int Lx, int Ly, const int64_t *configs, int64_
 /* y += a x (FP32): the parameter update of the proposal net. */
 int vmcx_lp_axpy(int64_t n, float a, const float *x, float *y, void *stream);
 
+/*
+ * Point-group images for the symmetry-projected wavefunction (replaces the
+ * x[:, perms] gather and compute_fermion_signs of GPU/models/symmetry.py:
+ * 660-770, 893-951).  perms[g][s] (int32, G x N): site whose occupation
+ * image g puts on site s, images[b][g][s] = configs[b][perms[g][s]];
+ * inv_perms: the inverse permutations.  signs[b][g] = (-1)^(inversions,
+ * among the occupied modes of configs[b] in the interleaved order
+ * [d0, u0, d1, u1, ...], of the mode permutation 2s + spin ->
+ * 2 inv_perms[g][s] + spin).  N <= 64, G <= 64.
+ */
+int vmcx_sym_images(int64_t B, int n_sites, int n_group, const int64_t *configs,
+                    const int32_t *perms, const int32_t *inv_perms, int64_t *images,
+                    double *signs, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

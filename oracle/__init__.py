@@ -11,6 +11,7 @@ reference's VMC hot path (sjdu10/vmc_torch):
   * minSR / dense SR / MINRES and the SGD update   -> oracle.sr
   * integrated autocorrelation time and ESS        -> oracle.stats
   * Philox4x32-10 counter-based RNG                -> oracle.philox
+  * point-group projection with fermionic signs    -> oracle.symmetry
 
 Each function cites the reference file:line it follows (paths relative to
 the reference checkout).
@@ -32,4 +33,8 @@ Parity status (what pins each part; see the tests named):
             MINRES on the same operator
   stats     pinned: AR(1) closed-form tau_int
   philox    pinned: Random123 known-answer vectors
+  symmetry  pinned: the signed permutation operators form a representation
+            of C4v / C2v and commute with the dense Hubbard H; ground states
+            fall in one irrep; planted sign mistakes caught
+            (tests/test_symmetry_oracle.py)
 """

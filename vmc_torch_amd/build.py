@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvmcx.so")
-SOURCES = ["capi.hip", "slater.hip", "hubbard.hip", "sr.hip", "chol.hip", "minres.hip", "nnbf.hip", "fpeps.hip", "proposal.hip", "syrk.hip"]
+SOURCES = ["capi.hip", "slater.hip", "hubbard.hip", "sr.hip", "chol.hip", "minres.hip", "nnbf.hip", "fpeps.hip", "proposal.hip", "syrk.hip", "symmetry.hip"]
 ARCH = os.environ.get("VMCX_OFFLOAD_ARCH", "gfx950")
 
 

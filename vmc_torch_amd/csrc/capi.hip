@@ -37,7 +37,7 @@ static inline int status_of(int rc) {
 
 extern "C" {
 
-int vmcx_abi_version(void) { return 8; }
+int vmcx_abi_version(void) { return 9; }
 
 const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
 
@@ -249,6 +249,13 @@ int vmcx_lp_candidates(int64_t B, int Lx, int Ly, const int64_t *configs, int64_
 
 int vmcx_lp_axpy(int64_t n, float a, const float *x, float *y, void *stream) {
   return status_of(vmcx::lp_axpy(n, a, x, y, (hipStream_t)stream));
+}
+
+int vmcx_sym_images(int64_t B, int n_sites, int n_group, const int64_t *configs,
+                    const int32_t *perms, const int32_t *inv_perms, int64_t *images,
+                    double *signs, void *stream) {
+  return status_of(vmcx::sym_images(B, n_sites, n_group, configs, perms, inv_perms, images,
+                                    signs, (hipStream_t)stream));
 }
 
 int64_t vmcx_peps_dense_size(const vmcx_peps_model *m) { return vmcx::peps_dense_size(m); }

@@ -143,6 +143,9 @@ int lp_backward(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t 
 int lp_candidates(int64_t B, int Lx, int Ly, const int64_t *cfg, int64_t *out, hipStream_t st);
 int lp_axpy(int64_t n, float a, const float *x, float *y, hipStream_t st);
 
+int sym_images(int64_t B, int N, int G, const int64_t *cfg, const int32_t *perm,
+               const int32_t *inv, int64_t *images, double *sign, hipStream_t st);
+
 int64_t nnbf_workspace_bytes(const vmcx_nnbf_model *m, int64_t B, int64_t chunk, int n_edges);
 int nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
                  int64_t ws_bytes, int64_t chunk, double *sign, double *logabs, float *z1_out,

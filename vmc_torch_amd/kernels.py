@@ -472,3 +472,17 @@ def lp_axpy(a, x, y):
     require(x, torch.float32, "x")
     require(y, torch.float32, "y", (x.numel(),), x.device)
     check(lib().vmcx_lp_axpy(x.numel(), float(a), ptr(x), ptr(y), stream_ptr(x.device)))
+
+
+def sym_images(configs, perms, inv_perms):
+    """(images [B, G, N] int64, fermionic signs [B, G] float64) of the walkers
+    under the G site permutations (int32 [G, N], with their inverses)."""
+    B, N = _check_configs(configs, configs.device)
+    G = perms.shape[0]
+    require(perms, torch.int32, "perms", (G, N), configs.device)
+    require(inv_perms, torch.int32, "inv_perms", (G, N), configs.device)
+    images = torch.empty(B, G, N, dtype=I64, device=configs.device)
+    signs = torch.empty(B, G, dtype=F64, device=configs.device)
+    check(lib().vmcx_sym_images(B, N, G, ptr(configs), ptr(perms), ptr(inv_perms), ptr(images),
+                                ptr(signs), stream_ptr(configs.device)))
+    return images, signs

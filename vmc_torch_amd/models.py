@@ -60,6 +60,11 @@ class WavefunctionModel_GPU(nn.Module):
         self._snapshot = None
 
     def _walker_key(self, fxs):
+        """Identity of the walker tensor (storage, shape, in-place version).
+        Inference-mode tensors carry no version counter: their key never
+        matches, so their state is rebuilt on every call."""
+        if fxs.is_inference():
+            return object()
         return (fxs.data_ptr(), tuple(fxs.shape), fxs._version)
 
 
