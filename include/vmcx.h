@@ -378,6 +378,20 @@ int vmcx_lp_propose(const float *params, int C, int Lx, int Ly, int64_t B,
                     const int64_t *configs, uint64_t seed, int64_t step, const double *u_choice,
                     double *logits_ws, int32_t *move, double *logq_fwd, int64_t *proposed,
                     double *logq_rev, int precision, void *stream);
+/* vmcx_lp_propose with the logits at the current configurations given
+ * (logits_x, [B][n_moves]): the net runs once, on the proposed
+ * configurations (logits_p out).  After the accept step,
+ * vmcx_lp_commit_logits copies the accepted walkers' rows of logits_p into
+ * logits_x, so a chain evaluates the net once per proposal; the draws and
+ * decisions are those of vmcx_lp_propose (the same net on the same
+ * configuration gives the same bits).  No counterpart in the reference. */
+int vmcx_lp_propose_given(const float *params, int C, int Lx, int Ly, int64_t B,
+                          const int64_t *configs, const double *logits_x, uint64_t seed,
+                          int64_t step, const double *u_choice, double *logits_p,
+                          int32_t *move, double *logq_fwd, int64_t *proposed, double *logq_rev,
+                          int precision, void *stream);
+int vmcx_lp_commit_logits(int64_t B, int n_moves, const uint8_t *accepted,
+                          const double *logits_p, double *logits_x, void *stream);
 /* Metropolis-Hastings: accept when u_acc < |psi'|^2/|psi|^2 q(m|x')/q(m|x),
  * from (psign, plogabs) of the proposed configurations; commits configs,
  * sign, logabs, n_accept (may be NULL) and accepted[B] (may be NULL). */

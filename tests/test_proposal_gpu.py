@@ -325,3 +325,26 @@ def test_vmc_loop_with_learned_proposal(device):
     assert np.array_equal(model.M.detach().cpu().numpy(), M0)
     for e, var in log:
         assert abs(e - e_exact) < 5.0 * np.sqrt(var / B), (log, e_exact)
+
+
+@pytest.mark.parametrize("C,precision", [(16, "fp32"), (32, "bf16")])
+def test_cached_logits_chain_equals_full_evaluation(device, C, precision):
+    """The sampler evaluates the net once per proposal (logits at x carried
+    over from the accepted proposals); the chain equals the one that scores
+    x and x' afresh every step (vmcx_lp_propose), configuration for
+    configuration, with the same Philox draws."""
+    N, nu, nd, B, steps = 16, 6, 5, 128, 25
+    model, _ = _slater(N, nu + nd, 21, device)
+    P = LearnedProposalSamplerGPU.init_params(C, torch.Generator().manual_seed(22)).to(device)
+    x0 = torch.tensor(random_configs(np.random.default_rng(23), B, N, nu, nd), device=device)
+    a = x0.clone()
+    s = LearnedProposalSamplerGPU(4, 4, channels=C, params=P, seed=24, precision=precision)
+    s.sweeps(a, model, steps)
+    b = x0.clone()
+    sign, logabs = model.forward_log(b)
+    sign, logabs = sign.clone(), logabs.clone()
+    for step in range(steps):
+        move, lqf, prop, lqr = kernels.lp_propose(P, C, 4, 4, b, 24, step, precision=precision)
+        ps, pl = model.forward_log(prop)
+        kernels.lp_accept(b, sign, logabs, move, prop, ps, pl, lqf, lqr, 24, step)
+    assert torch.equal(a, b)

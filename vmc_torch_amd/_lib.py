@@ -61,6 +61,9 @@ _SIGNATURES = {
     "vmcx_lp_backward": (_I, [_P, _I, _I, _I, _I64, _P, _P, _P, _I64, _P, _P]),
     "vmcx_lp_candidates": (_I, [_I64, _I, _I, _P, _P, _P]),
     "vmcx_lp_axpy": (_I, [_I64, _c.c_float, _P, _P, _P]),
+    "vmcx_lp_propose_given": (_I, [_P, _I, _I, _I, _I64, _P, _P, _U64, _I64, _P, _P, _P, _P,
+                                   _P, _P, _I, _P]),
+    "vmcx_lp_commit_logits": (_I, [_I64, _I, _P, _P, _P, _P]),
     "vmcx_sym_images": (_I, [_I64, _I, _I, _P, _P, _P, _P, _P, _P]),
     "vmcx_peps_dense_size": (_I64, [_P]),
     "vmcx_peps_workspace_bytes": (_I64, [_P, _I64, _I64, _I]),

@@ -37,7 +37,7 @@ static inline int status_of(int rc) {
 
 extern "C" {
 
-int vmcx_abi_version(void) { return 9; }
+int vmcx_abi_version(void) { return 10; }
 
 const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
 
@@ -249,6 +249,22 @@ int vmcx_lp_candidates(int64_t B, int Lx, int Ly, const int64_t *configs, int64_
 
 int vmcx_lp_axpy(int64_t n, float a, const float *x, float *y, void *stream) {
   return status_of(vmcx::lp_axpy(n, a, x, y, (hipStream_t)stream));
+}
+
+int vmcx_lp_propose_given(const float *params, int C, int Lx, int Ly, int64_t B,
+                          const int64_t *configs, const double *logits_x, uint64_t seed,
+                          int64_t step, const double *u_choice, double *logits_p,
+                          int32_t *move, double *logq_fwd, int64_t *proposed, double *logq_rev,
+                          int precision, void *stream) {
+  return status_of(vmcx::lp_propose_given(params, C, Lx, Ly, B, configs, logits_x, seed, step,
+                                          u_choice, logits_p, move, logq_fwd, proposed,
+                                          logq_rev, precision, (hipStream_t)stream));
+}
+
+int vmcx_lp_commit_logits(int64_t B, int n_moves, const uint8_t *accepted,
+                          const double *logits_p, double *logits_x, void *stream) {
+  return status_of(vmcx::lp_commit_logits(B, n_moves, accepted, logits_p, logits_x,
+                                          (hipStream_t)stream));
 }
 
 int vmcx_sym_images(int64_t B, int n_sites, int n_group, const int64_t *configs,

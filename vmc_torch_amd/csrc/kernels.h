@@ -143,6 +143,13 @@ int lp_backward(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t 
 int lp_candidates(int64_t B, int Lx, int Ly, const int64_t *cfg, int64_t *out, hipStream_t st);
 int lp_axpy(int64_t n, float a, const float *x, float *y, hipStream_t st);
 
+int lp_propose_given(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,
+                     const double *logits_x, uint64_t seed, int64_t step,
+                     const double *u_choice, double *logits_p, int32_t *move, double *logq_f,
+                     int64_t *pcfg, double *logq_r, int precision, hipStream_t st);
+int lp_commit_logits(int64_t B, int M, const uint8_t *accepted, const double *logits_p,
+                     double *logits_x, hipStream_t st);
+
 int sym_images(int64_t B, int N, int G, const int64_t *cfg, const int32_t *perm,
                const int32_t *inv, int64_t *images, double *sign, hipStream_t st);
 

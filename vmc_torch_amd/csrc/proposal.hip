@@ -720,6 +720,53 @@ int lp_propose(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *
   return 0;
 }
 
+// The same step with the logits at the current configurations given
+// (logits_x, [B][3E]): only the proposed configurations go through the net.
+// The chain is the one of lp_propose: logits at x are those the net gave
+// when x was proposed (or the first evaluation), the same bits.
+int lp_propose_given(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,
+                     const double *logits_x, uint64_t seed, int64_t step,
+                     const double *u_choice, double *logits_p, int32_t *move, double *logq_f,
+                     int64_t *pcfg, double *logq_r, int precision, hipStream_t st) {
+  if (lp_check(C, Lx, Ly)) return 1;
+  VMCX_CHECK_ARG(B >= 0, "B < 0");
+  if (B == 0) return 0;
+  VMCX_CHECK_ARG(P && cfg && logits_x && logits_p && move && logq_f && pcfg && logq_r,
+                 "null pointer");
+  const int N = Lx * Ly, E = lp_n_edges(Lx, Ly);
+  const unsigned g = (unsigned)((B + LPC_WAVES - 1) / LPC_WAVES);
+  hipLaunchKernelGGL(lp_choose_kernel, dim3(g), dim3(64 * LPC_WAVES), 0, st, B, N, Lx, Ly, E, 1,
+                     logits_x, seed, step, u_choice, cfg, move, logq_f, pcfg);
+  VMCX_LAUNCHED();
+  int rc = lp_logits(P, C, Lx, Ly, B, pcfg, logits_p, precision, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(lp_choose_kernel, dim3(g), dim3(64 * LPC_WAVES), 0, st, B, N, Lx, Ly, E, 0,
+                     logits_p, seed, step, (const double *)nullptr, pcfg, move, logq_r,
+                     (int64_t *)nullptr);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+// logits_x[b] = logits_p[b] for the walkers whose move was accepted
+__global__ void lp_commit_logits_kernel(int64_t B, int M, const uint8_t *__restrict__ accepted,
+                                        const double *__restrict__ logits_p, double *logits_x) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B * M) return;
+  if (accepted[idx / M]) logits_x[idx] = logits_p[idx];
+}
+
+int lp_commit_logits(int64_t B, int M, const uint8_t *accepted, const double *logits_p,
+                     double *logits_x, hipStream_t st) {
+  VMCX_CHECK_ARG(B >= 0 && M >= 1, "bad sizes");
+  if (B == 0) return 0;
+  VMCX_CHECK_ARG(accepted && logits_p && logits_x, "null pointer");
+  const int64_t n = B * M;
+  hipLaunchKernelGGL(lp_commit_logits_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     st, B, M, accepted, logits_p, logits_x);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
 int lp_accept(int64_t B, int N, uint64_t seed, int64_t step, const double *u_acc,
               const int32_t *move, const int64_t *pcfg, const double *psign,
               const double *plog, const double *logq_f, const double *logq_r, int64_t *cfg,

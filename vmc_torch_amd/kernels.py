@@ -434,6 +434,37 @@ def lp_propose(params, C, Lx, Ly, configs, seed, step, u_choice=None, precision=
     return move, lqf, prop, lqr
 
 
+def lp_propose_given(params, C, Lx, Ly, configs, logits_x, seed, step, u_choice=None,
+                     precision="fp32"):
+    """lp_propose with the logits at `configs` given: (move, ln q(m|x),
+    proposed configs, ln q(m|x'), logits at the proposed configs)."""
+    B, N = _lp_check(params, C, configs, Lx, Ly)
+    dev = configs.device
+    M = lp_n_moves(Lx, Ly)
+    require(logits_x, F64, "logits_x", (B, M), dev)
+    logits_p = torch.empty(B, M, dtype=F64, device=dev)
+    move = torch.empty(B, dtype=I32, device=dev)
+    lqf = torch.empty(B, dtype=F64, device=dev)
+    lqr = torch.empty(B, dtype=F64, device=dev)
+    prop = torch.empty(B, N, dtype=I64, device=dev)
+    if u_choice is not None:
+        require(u_choice, F64, "u_choice", (B,), dev)
+    check(lib().vmcx_lp_propose_given(ptr(params), int(C), int(Lx), int(Ly), B, ptr(configs),
+                                      ptr(logits_x), int(seed) & (2 ** 64 - 1), int(step),
+                                      ptr(u_choice), ptr(logits_p), ptr(move), ptr(lqf),
+                                      ptr(prop), ptr(lqr), LP_PRECISION[precision],
+                                      stream_ptr(dev)))
+    return move, lqf, prop, lqr, logits_p
+
+
+def lp_commit_logits(accepted, logits_p, logits_x):
+    B, M = logits_x.shape
+    require(logits_p, F64, "logits_p", (B, M), logits_x.device)
+    require(accepted, U8, "accepted", (B,), logits_x.device)
+    check(lib().vmcx_lp_commit_logits(B, M, ptr(accepted), ptr(logits_p), ptr(logits_x),
+                                      stream_ptr(logits_x.device)))
+
+
 def lp_accept(configs, sign, logabs, move, proposed, psign, plogabs, logq_f, logq_r, seed, step,
               u_acc=None, n_accept=None):
     B, N = _check_configs(configs, configs.device)

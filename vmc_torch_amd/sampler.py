@@ -138,16 +138,21 @@ class LearnedProposalSamplerGPU(SamplerGPU):
         sign, logabs = model.forward_log(fxs)
         sign, logabs = sign.clone(), logabs.clone()
         n_acc = torch.zeros(fxs.shape[0], dtype=torch.int64, device=fxs.device)
+        # logits at the current walkers: evaluated once, then carried over
+        # from the proposed configurations of the accepted moves
+        lx = self.k.lp_logits(self.params, self.C, self.Lx, self.Ly, fxs,
+                              precision=self.precision) if n_steps else None
         for s in range(n_steps):
             uc = ua = None
             if draws is not None:
                 uc, ua = draws[s]
-            move, lqf, prop, lqr = self.k.lp_propose(self.params, self.C, self.Lx, self.Ly, fxs,
-                                                     self.seed, self.step_counter, uc,
-                                                     precision=self.precision)
+            move, lqf, prop, lqr, lp = self.k.lp_propose_given(
+                self.params, self.C, self.Lx, self.Ly, fxs, lx, self.seed, self.step_counter, uc,
+                precision=self.precision)
             ps, pl = model.forward_log(prop)
-            self.k.lp_accept(fxs, sign, logabs, move, prop, ps, pl, lqf, lqr, self.seed,
-                             self.step_counter, ua, n_acc)
+            acc = self.k.lp_accept(fxs, sign, logabs, move, prop, ps, pl, lqf, lqr, self.seed,
+                                   self.step_counter, ua, n_acc)
+            self.k.lp_commit_logits(acc, lp, lx)
             self.step_counter += 1
         if hasattr(model, "invalidate"):
             model.invalidate()      # the model's cached walker state predates the moves
