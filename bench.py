@@ -859,14 +859,16 @@ def main_mixed_lp(a, w):
                                               precision=I["lp"].precision))
         t_a = ev_ms(lambda: I["model"].forward_log(I["x"]))
         t_e = ev_ms(lambda: I["model"].local_energy(I["x"], I["H"]), 2)
-        comp["lp_logits"] += 2 * I["k"] * t_l      # at x and at x' per proposal
+        # one launch per proposal (at x'; the logits at x carry over) plus
+        # the evaluation at the walkers when a sample's proposals start
+        comp["lp_logits"] += (I["k"] + 1) * t_l
         comp["amplitude"] += I["k"] * t_a
         comp["local_energy"] += t_e
         per_lattice[I["name"]] = {"lp_logits_launch_ms": t_l, "amplitude_ms": t_a,
                                   "local_energy_ms": t_e}
-        logits_flops += 2 * I["k"] * I["B"] * lp_logits_flops(I["N"], I["E"], C)
-        logits_launch_ms += 2 * I["k"] * t_l
-        n_launch += 2 * I["k"]
+        logits_flops += (I["k"] + 1) * I["B"] * lp_logits_flops(I["N"], I["E"], C)
+        logits_launch_ms += (I["k"] + 1) * t_l
+        n_launch += I["k"] + 1
         mlp = 2.0 * (w["hidden"] ** 2 + w["hidden"] * 2 * I["N"] * I["nf"])
         amp_flops += I["k"] * I["B"] * mlp
         det = 2.0 / 3.0 * I["nf"] ** 3                # FP64 LU of one amplitude
