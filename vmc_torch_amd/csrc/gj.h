@@ -11,6 +11,26 @@ __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// wave-wide helpers for one 64 x 64 matrix per wave: a double read from
+// lane l into scalar registers, and the maximum of a 32-bit key over the
+// wave by a DPP max-scan (row_shr 1, 2, 4, 8, row_bcast 15, 31: the maximum
+// ends in lane 63)
+__device__ __forceinline__ double lu_readlane(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ uint32_t lu_wave_max(uint32_t k) {
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x111, 0xf, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x112, 0xf, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x114, 0xf, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x118, 0xf, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x142, 0xa, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x143, 0xc, 0xf, false));
+  return (uint32_t)__builtin_amdgcn_readlane((int)k, 63);
+}
+
 struct GJResult {
   double logdet;
   int parity;
@@ -39,20 +59,34 @@ __device__ __forceinline__ GJResult gj_invert(double (&a)[NFP], int col, int gba
     // NaN / Inf entries rank as 0 so that the order stays total and every
     // lane is chosen exactly once over the NFP steps
     const double mag = fabs(a[0]);
-    double cand = ((used >> col) & 1ull) ? -1.0 : ((mag <= 1e300) ? mag : 0.0);
-    int cidx = col;
+    int q;
+    double piv;
+    if constexpr (NFP == 64) {
+      // one matrix per wave: DPP arg-max on a 32-bit key (|a| as float, the
+      // low 6 bits 63 - lane), pivot value by readlane
+      const float fm = (mag <= 1e300) ? (float)mag : 0.0f;
+      const uint32_t key = ((used >> col) & 1ull)
+                               ? 0u
+                               : ((__float_as_uint(fm) & ~63u) | (uint32_t)(63 - col));
+      q = 63 - (int)(lu_wave_max(key) & 63u);
+      piv = lu_readlane(a[0], q);
+      if (!(piv != 0.0) || !(fabs(piv) <= 1e300)) singular = true;
+    } else {
+      double cand = ((used >> col) & 1ull) ? -1.0 : ((mag <= 1e300) ? mag : 0.0);
+      int cidx = col;
 #pragma unroll
-    for (int o = NFP / 2; o > 0; o >>= 1) {
-      const double oc = __shfl_xor(cand, o, 64);
-      const int oi = __shfl_xor(cidx, o, 64);
-      if (oc > cand || (oc == cand && oi < cidx)) {
-        cand = oc;
-        cidx = oi;
+      for (int o = NFP / 2; o > 0; o >>= 1) {
+        const double oc = __shfl_xor(cand, o, 64);
+        const int oi = __shfl_xor(cidx, o, 64);
+        if (oc > cand || (oc == cand && oi < cidx)) {
+          cand = oc;
+          cidx = oi;
+        }
       }
+      q = cidx;
+      if (!(cand > 0.0)) singular = true;
+      piv = __shfl(a[0], gbase + q, 64);
     }
-    const int q = cidx;
-    if (!(cand > 0.0)) singular = true;
-    const double piv = __shfl(a[0], gbase + q, 64);
     const double f = a[0];
     if (col == q) {
       const double inv = (piv != 0.0) ? 1.0 / piv : 0.0;
@@ -157,22 +191,6 @@ struct LuStep<NFP, NFP> {
 // scalar registers, no LDS.  Every lane applies the update (f = 1 on the
 // pivot lane, anything on finished lanes, whose registers are never read
 // again), so the FMAs need no lane mask.
-__device__ __forceinline__ double lu_readlane(double v, int l) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-  return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ uint32_t lu_wave_max(uint32_t k) {
-  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x111, 0xf, 0xf, false));
-  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x112, 0xf, 0xf, false));
-  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x114, 0xf, 0xf, false));
-  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x118, 0xf, 0xf, false));
-  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x142, 0xa, 0xf, false));
-  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x143, 0xc, 0xf, false));
-  return (uint32_t)__builtin_amdgcn_readlane((int)k, 63);
-}
-
 template <int P>
 struct LuStep64 {
   static __device__ __forceinline__ void run(double (&a)[64], int col, LuState &S) {
