@@ -31,6 +31,23 @@ __device__ __forceinline__ uint32_t lu_wave_max(uint32_t k) {
   return (uint32_t)__builtin_amdgcn_readlane((int)k, 63);
 }
 
+// two 32-lane groups per wave: the maximum of each half (row_shr 1, 2, 4,
+// 8, row_bcast 15 into rows 1 and 3 leaves it in lanes 31 and 63), and
+// the pivot value of each half read from its group
+__device__ __forceinline__ void lu_half_pivot(uint32_t k, double v, int lane, int &q,
+                                              double &piv) {
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x111, 0xf, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x112, 0xf, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x114, 0xf, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x118, 0xf, 0xf, false));
+  k = max(k, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k, 0x142, 0xa, 0xf, false));
+  const int q0 = 63 - (int)((uint32_t)__builtin_amdgcn_readlane((int)k, 31) & 63u);
+  const int q1 = 63 - (int)((uint32_t)__builtin_amdgcn_readlane((int)k, 63) & 63u);
+  const double p0 = lu_readlane(v, q0), p1 = lu_readlane(v, 32 + q1);
+  q = lane < 32 ? q0 : q1;
+  piv = lane < 32 ? p0 : p1;
+}
+
 struct GJResult {
   double logdet;
   int parity;
@@ -61,15 +78,19 @@ __device__ __forceinline__ GJResult gj_invert(double (&a)[NFP], int col, int gba
     const double mag = fabs(a[0]);
     int q;
     double piv;
-    if constexpr (NFP == 64) {
-      // one matrix per wave: DPP arg-max on a 32-bit key (|a| as float, the
-      // low 6 bits 63 - lane), pivot value by readlane
+    if constexpr (NFP == 64 || NFP == 32) {
+      // one or two matrices per wave: DPP arg-max on a 32-bit key (|a| as
+      // float, the low 6 bits 63 - lane), pivot value by readlane
       const float fm = (mag <= 1e300) ? (float)mag : 0.0f;
       const uint32_t key = ((used >> col) & 1ull)
                                ? 0u
                                : ((__float_as_uint(fm) & ~63u) | (uint32_t)(63 - col));
-      q = 63 - (int)(lu_wave_max(key) & 63u);
-      piv = lu_readlane(a[0], q);
+      if constexpr (NFP == 64) {
+        q = 63 - (int)(lu_wave_max(key) & 63u);
+        piv = lu_readlane(a[0], q);
+      } else {
+        lu_half_pivot(key, a[0], gbase + col, q, piv);
+      }
       if (!(piv != 0.0) || !(fabs(piv) <= 1e300)) singular = true;
     } else {
       double cand = ((used >> col) & 1ull) ? -1.0 : ((mag <= 1e300) ? mag : 0.0);
@@ -142,10 +163,17 @@ struct LuStep {
     const double mag = fabs(a[P]);
     const float fm = (mag <= 1e300) ? (float)mag : 0.0f;
     uint32_t key = done ? 0u : ((__float_as_uint(fm) & ~63u) | (uint32_t)(63 - col));
+    int q;
+    double piv;
+    if constexpr (NFP == 32) {
+      lu_half_pivot(key, a[P], gbase + col, q, piv);
+    } else {
 #pragma unroll
-    for (int o = NFP / 2; o > 0; o >>= 1) key = max(key, (uint32_t)__shfl_xor((int)key, o, 64));
-    const int q = 63 - (int)(key & 63u);
-    const double piv = __shfl(a[P], gbase + q, 64);
+      for (int o = NFP / 2; o > 0; o >>= 1)
+        key = max(key, (uint32_t)__shfl_xor((int)key, o, 64));
+      q = 63 - (int)(key & 63u);
+      piv = __shfl(a[P], gbase + q, 64);
+    }
     if (!(piv != 0.0) || !(fabs(piv) <= 1e300)) S.singular = true;
     if (P + 1 < NFP) {
       if (col == q) {
