@@ -60,14 +60,15 @@ WORKLOADS = {
         desc="4x2 OBC Hubbard t=1 U=8 4u+4d, Slater 16x8 N(0,0.1), 1024 walkers"),
     "hubbard6x6_nnbf": dict(
         kind="nnbf", Lx=6, Ly=6, nu=16, nd=16, B=4096, burn=200, shift=1e-3, lr=0.05,
-        gram="bf16", hidden=256, bf_scale=0.01, mlp="bf16",
+        gram="factored", hidden=256, bf_scale=0.01, mlp="bf16",
         desc=("6x6 OBC Hubbard t=1 U=8 16u+16d, NNBF 72-256-256-2304 ReLU, bf_scale 0.01 "
-              "(678,912 params), BF16 GEMMs, 4096 walkers")),
+              "(678,912 params), BF16 GEMMs, FP64 Gram from the factors of O, 4096 walkers")),
     "hubbard8x8_nnbf": dict(
         kind="nnbf", Lx=8, Ly=8, nu=32, nd=32, B=4096, burn=200, shift=1e-3, lr=0.05,
-        gram="fp64", o_dtype="fp64", hidden=256, bf_scale=0.01, mlp="bf16",
+        gram="factored", o_dtype="fp64", hidden=256, bf_scale=0.01, mlp="bf16",
         desc=("8x8 OBC Hubbard t=1 U=8 32u+32d, NNBF 128-256-256-8192 ReLU, bf_scale 0.01 "
-              "(2,212,352 params), BF16 GEMMs, FP64 O and Gram, 4096 walkers")),
+              "(2,212,352 params), BF16 GEMMs, FP64 Gram from the factors of O, 4096 "
+              "walkers")),
     "hubbard4x4_nnftns": dict(
         kind="peps", Lx=4, Ly=4, nu=8, nd=8, B=2048, burn=100, shift=5e-4, lr=0.1,
         gram="fp64", D=4, hidden=128, eta=1.0,
@@ -100,8 +101,10 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="hubbard8x8_slater", choices=sorted(WORKLOADS))
-    p.add_argument("--gram", default=None, choices=["fp64", "fp32", "bf16", "bf16x3"],
-                   help="Gram precision (default: the workload's)")
+    p.add_argument("--gram", default=None,
+                   choices=["fp64", "fp32", "bf16", "bf16x3", "factored"],
+                   help="Gram precision (default: the workload's); factored (NNBF): the "
+                        "FP64 Gram from the Grams of the factors of O, no O matrix")
     p.add_argument("--refresh-every", type=int, default=64)
     p.add_argument("--mlp", default=None, choices=["fp32", "bf16"],
                    help="NNBF dense-layer precision (default: the workload's)")
@@ -323,6 +326,7 @@ class NnbfRun:
     """NNBF walkers through the reference-shaped model class NNBF_GPU."""
 
     names = ["sweep", "local_energy", "logderiv", "sr", "update"]
+    factored = False
 
     def __init__(self, w, a, dev, H, fxs):
         import torch
@@ -349,9 +353,17 @@ class NnbfRun:
             self.n_steps = 0
         self.o_dtype = a.o_dtype or w.get("o_dtype", "fp32")
         o_dtype = torch.float64 if self.o_dtype == "fp64" else torch.float32
-        self.O = torch.empty(B, self.Np, dtype=o_dtype, device=dev)
+        self.factored = (a.gram or w["gram"]) == "factored"
         self.dp = torch.empty(self.Np, dtype=torch.float64, device=dev)
-        self.ws = self.k.SRWorkspace()
+        if self.factored:
+            H = w["hidden"]
+            self.factor_widths = [2 * N * nf, 2 * N, H, H, H, H]
+            self.F = torch.empty(B, sum(self.factor_widths), dtype=torch.float64, device=dev)
+            self.O = self.F
+            self.ws = self.k.SRFactoredWorkspace()
+        else:
+            self.O = torch.empty(B, self.Np, dtype=o_dtype, device=dev)
+            self.ws = self.k.SRWorkspace()
 
     def burn(self, n, seed, off):
         self.model.mcmc_sweep(self.fxs, self.e_sweep, n, seed, off)
@@ -378,11 +390,17 @@ class NnbfRun:
         m.append(mark())
         E = self.local_energy()
         m.append(mark())
-        self.model.log_derivative(self.fxs, dtype=self.O.dtype, out=self.O)
-        m.append(mark())
-        _, stats, info = self.k.sr_minsr(self.O, E, None, self.w["shift"], gram,
-                                         0 if gram == "fp64" else 20, 1e-8, self.ws,
-                                         dp=self.dp, phase_ms=phases, inplace=True)
+        if self.factored:
+            FO, _ = self.model.log_derivative_factored(self.fxs, out=self.F)
+            m.append(mark())
+            _, stats, info = self.k.sr_minsr_factored(FO, E, None, self.w["shift"], self.ws,
+                                                      dp=self.dp, phase_ms=phases)
+        else:
+            self.model.log_derivative(self.fxs, dtype=self.O.dtype, out=self.O)
+            m.append(mark())
+            _, stats, info = self.k.sr_minsr(self.O, E, None, self.w["shift"], gram,
+                                             0 if gram == "fp64" else 20, 1e-8, self.ws,
+                                             dp=self.dp, phase_ms=phases, inplace=True)
         m.append(mark())
         off_p = 0
         for p in self.model.params:
@@ -421,7 +439,7 @@ class NnbfRun:
         return {
             "sweep": ("mfma_bf16", n_prop * mlp),
             "local_energy": ("mfma_bf16", n_conn * mlp),
-            "logderiv": ("hbm", self.B * self.Np * float(self.O.element_size())),
+            "logderiv": ("hbm", float(self.O.numel() * self.O.element_size())),
         }
 
     def oracle_params(self):
@@ -507,6 +525,9 @@ def main():
 
     w = WORKLOADS[a.workload]
     gram = a.gram or w["gram"]
+    if gram == "factored" and w["kind"] != "nnbf":
+        print(json.dumps({"error": "--gram factored is wired for the NNBF workloads"}))
+        sys.exit(2)
     Lx, Ly, nu, nd, B = w["Lx"], w["Ly"], w["nu"], w["nd"], w["B"]
     N, nf = Lx * Ly, nu + nd
     dev = torch.device("cuda:0")
@@ -571,6 +592,8 @@ def main():
         sr_phases.append(ph)
     torch.cuda.synchronize()
     phase_names = ["center", "gram", "factor", "solve", "dp_stats", "chol_diag_launch"]
+    if gram == "factored":     # vmcx_sr_minsr_factored reports the Grams first
+        phase_names = ["gram", "center", "factor", "solve", "dp_stats", "chol_diag_launch"]
     phase_ms = {k: sum(p[i] for p in sr_phases) / len(sr_phases)
                 for i, k in enumerate(phase_names)}
     acc_rate = (float(torch.stack(accs).double().mean())
@@ -609,8 +632,12 @@ def main():
                        "frac": work / secs / PEAK_BF16,
                        "note": "dense-layer flops of every amplitude evaluated, BF16 operands"}
     peak_sr = {"fp64": PEAK_FP64, "fp32": PEAK_FP32_MFMA, "bf16": PEAK_BF16,
-               "bf16x3": PEAK_BF16}[gram]
-    gram_flops = float(Ns) * Ns * Np * {"fp64": 1, "fp32": 2, "bf16": 2, "bf16x3": 4}[gram]
+               "bf16x3": PEAK_BF16, "factored": PEAK_FP64}[gram]
+    if gram == "factored":
+        # one symmetric product per factor: Ns^2 k_f, k_f = 2N nf, 2N, 4 x hidden
+        gram_flops = float(Ns) * Ns * sum(run.factor_widths)
+    else:
+        gram_flops = float(Ns) * Ns * Np * {"fp64": 1, "fp32": 2, "bf16": 2, "bf16x3": 4}[gram]
     chol_flops = float(Ns) ** 3 / 3
     kern["sr"] = {"ms": kt["sr"], "phases_ms": phase_ms}
     kern["update"] = {"ms": kt["update"]}
@@ -626,7 +653,8 @@ def main():
     kern["sr.gram"] = flop_roof(gram_flops, phase_ms["gram"], peak_sr,
                                 "algorithmic flops Ns^2 Np (FP64, symmetric product), "
                                 "2 Ns^2 Np (reduced-precision GEMM) or 4 Ns^2 Np (bf16x3: "
-                                "two full products); FP64: the matrix-core "
+                                "two full products), Ns^2 sum(k_f) (factored: one "
+                                "symmetric product per factor of O); FP64: the matrix-core "
                                 "SYRK kernel (one workgroup per 128x128 lower tile, the "
                                 "reduction split when the triangle has few tiles) plus the "
                                 "pass that adds the split partials")
@@ -634,7 +662,12 @@ def main():
                                   "Ns^3/3: blocked FP64 Cholesky (LDS diagonal kernel, "
                                   "panel GEMM, matrix-core SYRK per two panels)")
     osz = run.O.element_size()
-    if gram == "fp64":
+    if gram == "factored":
+        # combination: 7 factor-Gram triangles read, T written; row sums, the
+        # centring pass (read + write) and ||g|| read T; T copied once (full)
+        tri = Ns * (Ns + 1) / 2 * 8.0
+        cbytes = (7 + 1 + 1 + 2 + 1) * tri + 2.0 * Ns * Ns * 8
+    elif gram == "fp64":
         cbytes = 2.0 * Ns * Np * osz + Ns * Np * osz          # read O twice, write X
     else:
         cbytes = 2.0 * Ns * Np * osz + Ns * Np * (osz + (4 if gram == "bf16x3" else 2))
@@ -653,6 +686,10 @@ def main():
     single = {"sr.gram": 1, "sr.chol_diag": n_diag}
     if w["kind"] == "slater":
         single.update({"sweep": 1, "local_energy": 1, "logderiv": 1, "refresh": 1})
+    elif run.factored:
+        # with the Gram formed from the factors the SR step no longer sets
+        # the step: the sampling components (several kernels each) compete
+        single.update({"sweep": 1, "local_energy": 1, "logderiv": 1})
     for k, n in single.items():
         kern[k].setdefault("launches_per_step", n)
         kern[k].setdefault("per_step_ms", kern[k]["ms"])
@@ -671,6 +708,8 @@ def main():
     else:
         o = "f64" if run.o_dtype == "fp64" else "f32"
         dtype = f"{run.mlp}-gemm/f64-det/{o}-O/{gram}-gram/f64-cholesky"
+        if gram == "factored":
+            dtype = f"{run.mlp}-gemm/f64-det/f64-factors-of-O/f64-gram/f64-cholesky"
         if a.sampler == "learned":
             dtype += "/bf16-proposal-net"
     out = {

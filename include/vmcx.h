@@ -164,6 +164,45 @@ int vmcx_sr_minsr(int64_t Ns, int64_t Np, void *O, int o_dtype, int64_t ldo, int
                   double *stats, int32_t *info, float *phase_ms, void *stream);
 
 /*
+ * minSR on a factored O: the same step as vmcx_sr_minsr with gram_dtype 0
+ * (FP64 Gram, Cholesky solve), for models whose per-sample log-derivative is
+ * made of blocks that are a factor or an outer product of two factors (the
+ * dense layers of an MLP: dL/dW = delta x^T).  Factor f is the row-major
+ * FP64 matrix F[f] [Ns][ld[f]] of width k[f] (F: HOST array of device
+ * pointers; n_factors <= 8).  Block b (n_blocks <= 8):
+ *   b_factor < 0:  O[s][w_offset + i]         = scale F_a[s][i]
+ *   b_factor >= 0: O[s][w_offset + i k_b + j] = scale F_a[s][i] F_b[s][j]
+ *                  and, bias_offset >= 0, O[s][bias_offset + i] = scale F_a[s][i]
+ * (a = a_factor, b = b_factor).  The blocks must tile [0, Np) exactly.
+ * Then O O^T = sum over blocks of scale^2 G_a o (G_b [+ 1]) with G_f = F_f F_f^T
+ * (o: elementwise product; + 1 for a bias block): the factor Grams are formed
+ * on the FP64 matrix cores with inner dimension k[f] instead of Np.  The
+ * centred Gram is P (O O^T) P / Ns (P = I - 1 1^T / Ns, double centring),
+ * dp = O^T P alpha / sqrt(Ns), one FP64 GEMM per outer-product block.  No
+ * Ns x Np matrix is formed.
+ *   stats / info as vmcx_sr_minsr (iterations 0; the residual is that of the
+ *   Cholesky solve against the FP64 Gram).  phase_ms (HOST, may be NULL): [6]
+ *   float = factor Grams, combination + centring, factorisation, solve,
+ *   dp / statistics, [5] mean diagonal-block launch of the factorisation.
+ * Replaces: minSR_solver_gpu (GPU/optimizer.py:471-614) for models built
+ *   from dense layers (NNBF.py:36-162), with the Gram of GPU/optimizer.py:530-536
+ *   formed from the factors.
+ */
+typedef struct {
+  int32_t a_factor, b_factor;
+  double scale;
+  int64_t w_offset, bias_offset;
+} vmcx_sr_block;
+int64_t vmcx_sr_factored_workspace_bytes(int64_t Ns, int n_factors, const int32_t *k,
+                                         int n_blocks, const vmcx_sr_block *blocks);
+int vmcx_sr_minsr_factored(int64_t Ns, int n_factors, const double *const *F,
+                           const int64_t *ld, const int32_t *k, int n_blocks,
+                           const vmcx_sr_block *blocks, int64_t Np, const double *E,
+                           double energy_mean, double diag_shift, void *workspace,
+                           int64_t ws_bytes, double *dp, double *stats, int32_t *info,
+                           float *phase_ms, void *stream);
+
+/*
  * FP64 symmetric rank-K update on the matrix cores (a building block of the
  * minSR Gram and of the blocked Cholesky, exported for direct use):
  *   C = beta C + alpha M M^T on the lower triangle of the n x n column-major
@@ -267,6 +306,18 @@ int vmcx_nnbf_local_energy(const vmcx_nnbf_model *m, int64_t B, int n_edges,
 int vmcx_nnbf_logderiv(const vmcx_nnbf_model *m, int64_t B, const int64_t *configs,
                        void *workspace, int64_t ws_bytes, int64_t chunk, void *O, int o_dtype,
                        int64_t ld, double *sign, double *logabs, void *stream);
+
+/* The factors of the per-sample log-derivative instead of O: row s of F
+ * ([B][ld] float64, ld >= 2N nf + 2N + 4 hidden) is [gM | n | dz1 | h1 |
+ * dz2 | h2]: gM (2N x nf) = d ln|psi| / dM, n the occupations, dz1 / dz2 the
+ * back-propagated errors of the hidden layers, h1 / h2 their activations.
+ * O = [gM, dz1 n^T, dz1, dz2 h1^T, dz2, s gM h2^T, s gM] (s = bf_scale), the
+ * blocks vmcx_sr_minsr_factored consumes.  Also writes sign / logabs.
+ * Replaces: compute_grads_gpu for NNBF (GPU/vmc_utils.py:1389-1634) when the
+ * SR step takes the factored form. */
+int vmcx_nnbf_logderiv_factors(const vmcx_nnbf_model *m, int64_t B, const int64_t *configs,
+                               void *workspace, int64_t ws_bytes, int64_t chunk, double *F,
+                               int64_t ld, double *sign, double *logabs, void *stream);
 
 /* y (bf16) = x (fp32), n elements (weight copies for gemm = 1). */
 int vmcx_to_bf16(int64_t n, const float *x, void *y, void *stream);

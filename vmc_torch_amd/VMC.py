@@ -8,6 +8,7 @@ from typing import Any, Callable, Dict, Optional
 
 import torch
 
+from . import kernels
 from .vmc_utils import compute_grads_gpu, evaluate_energy
 
 
@@ -54,6 +55,10 @@ class VMC_GPU:
         """GPU/VMC.py:316-511: sweep, E_loc, O per chunk of B samples."""
         B = fxs.shape[0]
         t_samp = t_locE = t_grad = 0.0
+        factored = (getattr(self.preconditioner, "factored", False)
+                    and hasattr(model, "log_derivative_factored"))
+        if getattr(self.preconditioner, "factored", False) and not factored:
+            raise TypeError("the factored SR step needs a model with log_derivative_factored")
         if burn_in and burn_in_steps:
             fxs = self.sampler.burn_in(fxs, model, graph, burn_in_steps)
         E_list, O_list = [], []
@@ -65,10 +70,13 @@ class VMC_GPU:
             t1 = time.time()
             _, E = self.evaluate_energy_fn(fxs, model, hamiltonian, amps, use_log_amp=use_log_amp)
             t2 = time.time()
-            O, aux = self.compute_grads_fn(fxs, model, batch_size=grad_batch_size,
-                                           use_log_amp=use_log_amp)
-            if not use_log_amp:
-                O = O / aux.unsqueeze(1)
+            if factored:
+                O, aux = model.log_derivative_factored(fxs)
+            else:
+                O, aux = self.compute_grads_fn(fxs, model, batch_size=grad_batch_size,
+                                               use_log_amp=use_log_amp)
+                if not use_log_amp:
+                    O = O / aux.unsqueeze(1)
             t3 = time.time()
             t_samp += t1 - t0
             t_locE += t2 - t1
@@ -77,8 +85,13 @@ class VMC_GPU:
             O_list.append(O[:needed])
             count += needed
         E = torch.cat(E_list) if len(E_list) > 1 else E_list[0]
-        O = torch.cat(O_list) if len(O_list) > 1 else O_list[0]
-        if outlier_clip_factor > 0:
+        if factored:
+            O = kernels.FactoredO.cat(O_list) if len(O_list) > 1 else O_list[0]
+            if outlier_clip_factor > 0:
+                raise NotImplementedError("outlier clipping needs the dense O (gram != 'factored')")
+        else:
+            O = torch.cat(O_list) if len(O_list) > 1 else O_list[0]
+        if outlier_clip_factor > 0 and not factored:
             # GPU/VMC.py:476-498
             norms = O.norm(dim=1)
             mask = norms <= outlier_clip_factor * norms.median()

@@ -37,6 +37,9 @@ _SIGNATURES = {
     "vmcx_sr_workspace_bytes": (_I64, [_I64, _I64, _I, _I, _I]),
     "vmcx_sr_minsr": (_I, [_I64, _I64, _P, _I, _I64, _I, _P, _D, _D, _I, _I, _D, _P, _I64,
                            _P, _P, _P, _P, _P]),
+    "vmcx_sr_factored_workspace_bytes": (_I64, [_I64, _I, _P, _I, _P]),
+    "vmcx_sr_minsr_factored": (_I, [_I64, _I, _P, _P, _P, _I, _P, _I64, _P, _D, _D, _P, _I64,
+                                    _P, _P, _P, _P, _P]),
     "vmcx_sr_minres_workspace_bytes": (_I64, [_I64, _I64]),
     "vmcx_sr_minres": (_I, [_I64, _I64, _P, _I, _I64, _P, _D, _D, _D, _I, _P, _I64, _P, _P,
                             _P, _P]),
@@ -49,6 +52,7 @@ _SIGNATURES = {
     "vmcx_nnbf_local_energy": (_I, [_P, _I64, _I, _P, _D, _D, _P, _P, _P, _P, _I64, _I64,
                                     _P, _P]),
     "vmcx_nnbf_logderiv": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _I, _I64, _P, _P, _P]),
+    "vmcx_nnbf_logderiv_factors": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _I64, _P, _P, _P]),
     "vmcx_to_bf16": (_I, [_I64, _P, _P, _P]),
     "vmcx_lp_n_params": (_I, [_I]),
     "vmcx_lp_n_moves": (_I, [_I, _I]),
@@ -84,6 +88,13 @@ class NnbfModelStruct(ctypes.Structure):
                 ("bf_scale", ctypes.c_double),
                 ("M", _P), ("W1", _P), ("b1", _P), ("W2", _P), ("b2", _P),
                 ("Wout", _P), ("bout", _P), ("W2_bf16", _P), ("Wout_bf16", _P)]
+
+
+class SrBlockStruct(ctypes.Structure):
+    """vmcx_sr_block (include/vmcx.h)."""
+    _fields_ = [("a_factor", ctypes.c_int32), ("b_factor", ctypes.c_int32),
+                ("scale", ctypes.c_double), ("w_offset", ctypes.c_int64),
+                ("bias_offset", ctypes.c_int64)]
 
 
 class PepsModelStruct(ctypes.Structure):

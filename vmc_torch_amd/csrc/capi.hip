@@ -37,7 +37,7 @@ static inline int status_of(int rc) {
 
 extern "C" {
 
-int vmcx_abi_version(void) { return 10; }
+int vmcx_abi_version(void) { return 11; }
 
 const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
 
@@ -191,6 +191,30 @@ int vmcx_nnbf_logderiv(const vmcx_nnbf_model *m, int64_t B, const int64_t *confi
                        int64_t ld, double *sign, double *logabs, void *stream) {
   return status3(vmcx::nnbf_logderiv(m, B, configs, workspace, ws_bytes, chunk, O, o_dtype, ld,
                                      sign, logabs, (hipStream_t)stream));
+}
+
+int vmcx_nnbf_logderiv_factors(const vmcx_nnbf_model *m, int64_t B, const int64_t *configs,
+                               void *workspace, int64_t ws_bytes, int64_t chunk, double *F,
+                               int64_t ld, double *sign, double *logabs, void *stream) {
+  return status3(vmcx::nnbf_logderiv_factors(m, B, configs, workspace, ws_bytes, chunk, F, ld,
+                                             sign, logabs, (hipStream_t)stream));
+}
+
+int64_t vmcx_sr_factored_workspace_bytes(int64_t Ns, int n_factors, const int32_t *k,
+                                         int n_blocks, const vmcx_sr_block *blocks) {
+  return vmcx::sr_factored_workspace_bytes(Ns, n_factors, k, n_blocks, blocks);
+}
+
+int vmcx_sr_minsr_factored(int64_t Ns, int n_factors, const double *const *F,
+                           const int64_t *ld, const int32_t *k, int n_blocks,
+                           const vmcx_sr_block *blocks, int64_t Np, const double *E,
+                           double energy_mean, double diag_shift, void *workspace,
+                           int64_t ws_bytes, double *dp, double *stats, int32_t *info,
+                           float *phase_ms, void *stream) {
+  const int rc = vmcx::sr_minsr_factored(Ns, n_factors, F, ld, k, n_blocks, blocks, Np, E,
+                                         energy_mean, diag_shift, workspace, ws_bytes, dp,
+                                         stats, info, phase_ms, (hipStream_t)stream);
+  return rc == 3 ? VMCX_ELIB : status_of(rc);
 }
 
 int vmcx_to_bf16(int64_t n, const float *x, void *y, void *stream) {

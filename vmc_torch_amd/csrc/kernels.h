@@ -169,5 +169,15 @@ int nnbf_local_energy(const vmcx_nnbf_model *m, int64_t B, int n_edges, const in
 int nnbf_logderiv(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
                   int64_t ws_bytes, int64_t chunk, void *O, int o_dtype, int64_t ld,
                   double *sign, double *logabs, hipStream_t st);
+int nnbf_logderiv_factors(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
+                          int64_t ws_bytes, int64_t chunk, double *F, int64_t ld,
+                          double *sign, double *logabs, hipStream_t st);
 int to_bf16(int64_t n, const float *x, void *y, hipStream_t st);
+int64_t sr_factored_workspace_bytes(int64_t Ns, int n_factors, const int32_t *k, int n_blocks,
+                                    const vmcx_sr_block *blocks);
+int sr_minsr_factored(int64_t Ns, int n_factors, const double *const *F, const int64_t *ld,
+                      const int32_t *k, int n_blocks, const vmcx_sr_block *blocks, int64_t Np,
+                      const double *E, double energy_mean, double shift, void *ws,
+                      int64_t ws_bytes, double *dp, double *stats, int32_t *info,
+                      float *phase_ms, hipStream_t st);
 }  // namespace vmcx

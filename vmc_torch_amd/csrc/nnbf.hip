@@ -435,7 +435,11 @@ __global__ void __launch_bounds__(256)
 // ------------------------------------------------------------ backward
 // One block per sample: O = d ln|psi| / d theta in the order
 // [M, W1, b1, W2, b2, Wout, bout].
-template <typename TO>
+// FACT: the row holds the factors of O instead (nD + 2N + 4H numbers):
+// [gM | n | dz1 | h1 | dz2 | h2], from which every block of O is gM itself
+// (M), an outer product dz1 n^T, dz2 h1^T, s gM h2^T (W1, W2, Wout) or a
+// factor (b1 = dz1, b2 = dz2, bout = s gM).
+template <typename TO, bool FACT = false>
 __global__ void __launch_bounds__(256)
     nnbf_backward_kernel(int64_t Bc, int N, int nf, int H, int act, int nfp,
                          const int64_t *__restrict__ cfg, const float *__restrict__ z1,
@@ -504,6 +508,18 @@ __global__ void __launch_bounds__(256)
     dz1[i] = act == 0 ? (z > 0.0f ? acc : 0.0f) : acc * (1.0f - tanhf(z) * tanhf(z));
   }
   __syncthreads();
+  if (FACT) {
+    TO *f = row + nD;
+    for (int o = tid; o < n_orb; o += nt) f[o] = (TO)nvec[o];
+    f += n_orb;
+    for (int u = tid; u < H; u += nt) {
+      f[u] = (TO)dz1[u];
+      f[H + u] = (TO)h1[u];
+      f[2 * H + u] = (TO)dz2[u];
+      f[3 * H + u] = (TO)h2[u];
+    }
+    return;
+  }
   int64_t off = nD;
   for (int64_t idx = tid; idx < (int64_t)H * n_orb; idx += nt) {   // W1
     const int u = (int)(idx / n_orb), jj = (int)(idx - (int64_t)u * n_orb);
@@ -831,6 +847,31 @@ int nnbf_logderiv(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void 
       hipLaunchKernelGGL(nnbf_backward_kernel<float>, dim3((unsigned)rows), dim3(256), lds, st,
                          rows, N, nf, H, m->act, nfp_of(nf), cfg + r0 * N, L.z1, L.z2, L.ainv,
                          m->W2, m->Wout, m->bf_scale, (float *)O + r0 * ld, ld);
+    VMCX_LAUNCHED();
+  }
+  return 0;
+}
+
+int nnbf_logderiv_factors(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
+                          int64_t ws_bytes, int64_t chunk, double *F, int64_t ld,
+                          double *sign, double *logabs, hipStream_t st) {
+  if (check_model(m)) return 1;
+  VMCX_CHECK_ARG(B >= 0 && chunk >= 1, "bad sizes");
+  if (B == 0) return 0;
+  VMCX_CHECK_ARG(cfg && ws && F && sign && logabs, "null pointer");
+  const int N = m->n_sites, H = m->hidden, nf = m->n_f, nD = 2 * N * nf;
+  VMCX_CHECK_ARG(ld >= (int64_t)nD + 2 * N + 4 * H, "ld smaller than the factor width");
+  const NnbfWs L = nnbf_layout(*m, B, chunk, 0, (char *)ws);
+  VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
+  const size_t lds = (size_t)(nD + 4 * H + 2 * N) * sizeof(float);
+  VMCX_CHECK_ARG(lds <= 65536, "backward LDS exceeds 64 KB");
+  for (int64_t r0 = 0; r0 < B; r0 += chunk) {
+    const int64_t rows = (B - r0 < chunk) ? B - r0 : chunk;
+    int rc = forward_rows(*m, L, rows, cfg + r0 * N, sign + r0, logabs + r0, L.ainv, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL((nnbf_backward_kernel<double, true>), dim3((unsigned)rows), dim3(256),
+                       lds, st, rows, N, nf, H, m->act, nfp_of(nf), cfg + r0 * N, L.z1, L.z2,
+                       L.ainv, m->W2, m->Wout, m->bf_scale, F + r0 * ld, ld);
     VMCX_LAUNCHED();
   }
   return 0;

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <mutex>
 #include <unordered_map>
+#include <vector>
 
 #include "kernels.h"
 
@@ -725,6 +726,299 @@ int sr_minsr(int64_t Ns, int64_t Np, void *O, int o_dtype, int64_t ldo, int flag
                               max_iter, tol, (char *)ws, L, dp, stats, info, phase_ms, st);
   return sr_minsr_t<float>(Ns, Np, (float *)O, ldo, flags, E, energy_mean, shift, gram,
                            max_iter, tol, (char *)ws, L, dp, stats, info, phase_ms, st);
+}
+
+// ===================================================== minSR on a factored O
+// O = [blocks], each a factor F_a or an outer product scale F_a (x) F_b (+ a
+// bias block scale F_a).  Row s of an outer-product block is the flattened
+// F_a[s] F_b[s]^T, so
+//   (O O^T)[s][t] = sum_b scale_b^2 (F_a[s] . F_a[t]) (F_b[s] . F_b[t] [+ 1])
+// (Kronecker identity; the + 1 is the bias row).  The uncentred Gram K comes
+// from the factor Grams G_f = F_f F_f^T (SYRK, inner dimension k_f), the
+// centred one is P K P (P = I - 1 1^T / Ns): Ot Ot^T = P K P / Ns.  The
+// update dp = Ot^T alpha = O^T (P alpha) / sqrt(Ns) is, per outer-product
+// block, F_a^T diag(a~) F_b: one FP64 GEMM with inner dimension Ns.
+constexpr int kFactMax = 8;
+
+struct GramTerms {
+  const double *ga[kFactMax];
+  const double *gb[kFactMax];
+  double s2[kFactMax];
+  double bias[kFactMax];
+  int n;
+};
+
+// K (column-major, lower triangle) = sum_b s2_b G_a o (G_b + bias_b); G_b
+// NULL: s2_b G_a
+__global__ void __launch_bounds__(256)
+    gram_combine_kernel(int64_t Ns, GramTerms t, double *K) {
+  const int64_t n = Ns * Ns;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = idx / Ns, i = idx - j * Ns;
+    if (i < j) continue;
+    double v = 0.0;
+    for (int b = 0; b < t.n; ++b) {
+      const double g = t.gb[b] ? t.gb[b][idx] + t.bias[b] : 1.0;
+      v += t.s2[b] * t.ga[b][idx] * g;
+    }
+    K[idx] = v;
+  }
+}
+
+// T = (K - r 1^T / Ns - 1 r^T / Ns + sum(r) / Ns^2) / Ns + shift I on the
+// lower triangle (in place; r = K 1, *rsum = sum of r)
+__global__ void __launch_bounds__(256)
+    double_center_kernel(int64_t Ns, const double *__restrict__ r, const double *rsum,
+                         double shift, double *T) {
+  const double inv = 1.0 / (double)Ns;
+  const double mu = rsum[0] * inv * inv;
+  const int64_t n = Ns * Ns;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = idx / Ns, i = idx - j * Ns;
+    if (i < j) continue;
+    double v = (T[idx] - r[i] * inv - r[j] * inv + mu) * inv;
+    if (i == j) v += shift;
+    T[idx] = v;
+  }
+}
+
+__global__ void fill_kernel(int64_t n, double v, double *x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+// out = (a - sum(a) / n) * scale, sum(a) in *asum
+__global__ void center_vec_kernel(int64_t n, const double *a, const double *asum, double scale,
+                                  double *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (a[i] - asum[0] / (double)n) * scale;
+}
+
+// Y[s][j] = w[s] F[s][j]  (Y row-major [Ns][kb])
+__global__ void __launch_bounds__(256)
+    scale_rows_kernel(int64_t Ns, int kb, const double *__restrict__ F, int64_t ld,
+                      const double *__restrict__ w, double *Y) {
+  const int64_t n = Ns * kb;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = idx / kb, j = idx - s * kb;
+    Y[idx] = w[s] * F[s * ld + j];
+  }
+}
+
+// sc[12] = ||g|| = sqrt(eps^T (T - shift I) eps) from sc[14] = eps^T T eps and
+// sc[4] = eps^T eps
+__global__ void gnorm_kernel(double shift, double *sc) {
+  sc[12] = sqrt(fmax(0.0, sc[14] - shift * sc[4]));
+}
+
+struct FactLayout {
+  size_t G, Gp, T, Tc, dinv, vec, sc, info, Y, total;
+  int splits[kFactMax];
+};
+
+static int check_factored(int64_t Ns, int n_factors, const int32_t *k, int n_blocks,
+                          const vmcx_sr_block *blocks, int64_t Np) {
+  VMCX_CHECK_ARG(Ns >= 1 && Ns <= 0x7fffffff, "Ns must be in [1, 2^31)");
+  VMCX_CHECK_ARG(n_factors >= 1 && n_factors <= kFactMax, "n_factors must be in [1, 8]");
+  VMCX_CHECK_ARG(n_blocks >= 1 && n_blocks <= kFactMax, "n_blocks must be in [1, 8]");
+  VMCX_CHECK_ARG(k && blocks, "null pointer");
+  for (int f = 0; f < n_factors; ++f) VMCX_CHECK_ARG(k[f] >= 1, "factor width < 1");
+  if (Np < 0) return 0;   // workspace query: no coverage check
+  // the blocks must tile [0, Np)
+  std::vector<std::pair<int64_t, int64_t>> iv;
+  for (int b = 0; b < n_blocks; ++b) {
+    const vmcx_sr_block &B = blocks[b];
+    VMCX_CHECK_ARG(B.a_factor >= 0 && B.a_factor < n_factors, "a_factor out of range");
+    VMCX_CHECK_ARG(B.b_factor >= -1 && B.b_factor < n_factors, "b_factor out of range");
+    VMCX_CHECK_ARG(B.b_factor >= 0 || B.bias_offset < 0, "a plain block has no bias");
+    const int64_t ka = k[B.a_factor], kb = B.b_factor >= 0 ? k[B.b_factor] : 1;
+    VMCX_CHECK_ARG(B.w_offset >= 0, "negative w_offset");
+    iv.emplace_back(B.w_offset, B.w_offset + ka * kb);
+    if (B.bias_offset >= 0) iv.emplace_back(B.bias_offset, B.bias_offset + ka);
+  }
+  std::sort(iv.begin(), iv.end());
+  int64_t at = 0;
+  for (const auto &x : iv) {
+    VMCX_CHECK_ARG(x.first == at, "the blocks do not tile [0, Np) (gap or overlap)");
+    at = x.second;
+  }
+  VMCX_CHECK_ARG(at == Np, "the blocks do not tile [0, Np) (total differs from Np)");
+  return 0;
+}
+
+static FactLayout fact_layout(int64_t Ns, int n_factors, const int32_t *k, int n_blocks,
+                              const vmcx_sr_block *blocks) {
+  FactLayout L{};
+  const size_t nn = (size_t)Ns * (size_t)Ns * sizeof(double);
+  int max_s = 1, max_kb = 1;
+  for (int f = 0; f < n_factors; ++f) {
+    L.splits[f] = gram_splits(Ns, k[f]);
+    max_s = std::max(max_s, L.splits[f]);
+  }
+  for (int b = 0; b < n_blocks; ++b)
+    if (blocks[b].b_factor >= 0 && blocks[b].b_factor < n_factors)
+      max_kb = std::max(max_kb, (int)k[blocks[b].b_factor]);
+  size_t o = 0;
+  L.G = o;    o += align256(nn * n_factors);
+  L.Gp = o;   o += align256(nn * (max_s - 1));
+  L.T = o;    o += align256(nn);
+  L.Tc = o;   o += align256(nn);
+  L.dinv = o; o += align256((size_t)chol_dinv_bytes(Ns));
+  L.vec = o;  o += align256((size_t)Ns * 9 * sizeof(double));
+  L.sc = o;   o += 256;
+  L.info = o; o += 256;
+  L.Y = o;    o += align256((size_t)Ns * max_kb * sizeof(double));
+  L.total = o;
+  return L;
+}
+
+int64_t sr_factored_workspace_bytes(int64_t Ns, int n_factors, const int32_t *k, int n_blocks,
+                                    const vmcx_sr_block *blocks) {
+  if (check_factored(Ns, n_factors, k, n_blocks, blocks, -1)) return -1;
+  for (int b = 0; b < n_blocks; ++b)
+    if (blocks[b].b_factor >= n_factors) return -1;
+  return (int64_t)fact_layout(Ns, n_factors, k, n_blocks, blocks).total;
+}
+
+int sr_minsr_factored(int64_t Ns, int n_factors, const double *const *F, const int64_t *ld,
+                      const int32_t *k, int n_blocks, const vmcx_sr_block *blocks, int64_t Np,
+                      const double *E, double energy_mean, double shift, void *ws,
+                      int64_t ws_bytes, double *dp, double *stats, int32_t *info,
+                      float *phase_ms, hipStream_t st) {
+  VMCX_CHECK_ARG(Np >= 1, "Np must be positive");
+  int rc = check_factored(Ns, n_factors, k, n_blocks, blocks, Np);
+  if (rc) return rc;
+  VMCX_CHECK_ARG(F && ld && E && ws && dp, "null pointer");
+  for (int f = 0; f < n_factors; ++f) {
+    VMCX_CHECK_ARG(F[f] != nullptr, "null factor pointer");
+    VMCX_CHECK_ARG(ld[f] >= k[f] && ld[f] <= 0x7fffffff, "factor ld must be in [k, 2^31)");
+  }
+  const FactLayout L = fact_layout(Ns, n_factors, k, n_blocks, blocks);
+  VMCX_CHECK_ARG(ws_bytes >= (int64_t)L.total, "workspace too small");
+  char *w = (char *)ws;
+  const int64_t nn = Ns * Ns;
+  double *G = (double *)(w + L.G), *Gp = (double *)(w + L.Gp);
+  double *T = (double *)(w + L.T), *Tc = (double *)(w + L.Tc), *dinv = (double *)(w + L.dinv);
+  double *vec = (double *)(w + L.vec);
+  double *ones = vec, *r = vec + Ns, *eps = vec + 2 * Ns, *alpha = vec + 3 * Ns;
+  double *t1 = vec + 4 * Ns, *t2 = vec + 5 * Ns, *q = vec + 6 * Ns, *res = vec + 7 * Ns;
+  double *at = vec + 8 * Ns;
+  double *sc = (double *)(w + L.sc), *Y = (double *)(w + L.Y);
+  int32_t *iinfo = (int32_t *)(w + L.info);
+  const double scale = 1.0 / std::sqrt((double)Ns), one = 1.0, zero = 0.0;
+  const int nsi = (int)Ns;
+  const unsigned gv = (unsigned)((Ns + 255) / 256);
+  rocblas_handle h = blas_handle_for(st);
+  VMCX_CHECK_ARG(h != nullptr, "rocblas_create_handle failed");
+  PhaseTimer timer(phase_ms != nullptr, st);
+  timer.mark();
+
+  // factor Grams on the FP64 matrix cores
+  bool used[kFactMax] = {};
+  for (int b = 0; b < n_blocks; ++b) {
+    used[blocks[b].a_factor] = true;
+    if (blocks[b].b_factor >= 0) used[blocks[b].b_factor] = true;
+  }
+  for (int f = 0; f < n_factors; ++f) {
+    if (!used[f]) continue;
+    double *Gf = G + f * nn;
+    const int sp = L.splits[f];
+    rc = syrk_f64(Ns, k[f], F[f], ld[f], 1, 1.0, 0.0, Gf, Ns, sp, nn, st, Gp);
+    if (rc) return rc;
+    if (sp > 1) {
+      hipLaunchKernelGGL(add_parts_kernel, dim3(4096), dim3(256), 0, st, Ns, sp - 1, Gp, Gf,
+                         0.0);
+      VMCX_LAUNCHED();
+    }
+  }
+  timer.mark();
+
+  // K, its double centring, eps and ||g||
+  GramTerms terms{};
+  terms.n = n_blocks;
+  for (int b = 0; b < n_blocks; ++b) {
+    terms.ga[b] = G + blocks[b].a_factor * nn;
+    terms.gb[b] = blocks[b].b_factor >= 0 ? G + blocks[b].b_factor * nn : nullptr;
+    terms.s2[b] = blocks[b].scale * blocks[b].scale;
+    terms.bias[b] = blocks[b].bias_offset >= 0 ? 1.0 : 0.0;
+  }
+  hipLaunchKernelGGL(gram_combine_kernel, dim3(4096), dim3(256), 0, st, Ns, terms, T);
+  hipLaunchKernelGGL(set_scalars, dim3(1), dim3(1), 0, st, sc);
+  hipLaunchKernelGGL(fill_kernel, dim3(gv), dim3(256), 0, st, Ns, 1.0, ones);
+  VMCX_LAUNCHED();
+  VMCX_BLAS(rocblas_dsymv(h, rocblas_fill_lower, nsi, &one, T, nsi, ones, 1, &zero, r, 1));
+  hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, r, ones, sc + 13);
+  hipLaunchKernelGGL(double_center_kernel, dim3(4096), dim3(256), 0, st, Ns, r, sc + 13, shift,
+                     T);
+  VMCX_LAUNCHED();
+  VMCX_HIP(hipMemcpyAsync(Tc, T, nn * sizeof(double), hipMemcpyDeviceToDevice, st));
+  const int use_given = std::isfinite(energy_mean) ? 1 : 0;
+  hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(1024), 0, st, Ns, E, energy_mean, use_given,
+                     scale, eps, sc + 8);
+  VMCX_LAUNCHED();
+  VMCX_BLAS(rocblas_dsymv(h, rocblas_fill_lower, nsi, &one, Tc, nsi, eps, 1, &zero, q, 1));
+  hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, eps, q, sc + 14);
+  hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, eps, eps, sc + 4);
+  hipLaunchKernelGGL(gnorm_kernel, dim3(1), dim3(1), 0, st, shift, sc);
+  VMCX_LAUNCHED();
+  timer.mark();
+
+  // FP64 Cholesky (the forward half of the solve runs along) and solve
+  VMCX_HIP(hipMemcpyAsync(t1, eps, Ns * sizeof(double), hipMemcpyDeviceToDevice, st));
+  rc = chol_factor((void *)h, Ns, T, Ns, dinv, iinfo, st, phase_ms ? phase_ms + 5 : nullptr,
+                   t1, t2);
+  if (rc) return rc;
+  timer.mark();
+  rc = chol_solve_bwd(Ns, chol_panels(Ns, dinv), Ns, dinv, t2, alpha, st);
+  if (rc) return rc;
+  timer.mark();
+
+  if (stats) {
+    // residual of the solve against the FP64 Gram
+    VMCX_BLAS(rocblas_dsymv(h, rocblas_fill_lower, nsi, &one, Tc, nsi, alpha, 1, &zero, q, 1));
+    hipLaunchKernelGGL(residual2_kernel, dim3(gv), dim3(256), 0, st, Ns, eps, q, alpha, 0.0,
+                       res);
+    hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, st, Ns, res, sc + 10);
+    hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, st, Ns, eps, sc + 11);
+    VMCX_LAUNCHED();
+  }
+  // dp = O^T a~, a~ = (alpha - mean alpha) / sqrt(Ns)
+  hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, st, Ns, alpha, ones, sc + 15);
+  hipLaunchKernelGGL(center_vec_kernel, dim3(gv), dim3(256), 0, st, Ns, alpha, sc + 15, scale,
+                     at);
+  VMCX_LAUNCHED();
+  for (int b = 0; b < n_blocks; ++b) {
+    const vmcx_sr_block &B = blocks[b];
+    const int a = B.a_factor, fb = B.b_factor;
+    const double s = B.scale;
+    if (fb < 0) {
+      VMCX_BLAS(rocblas_dgemv(h, rocblas_operation_none, (int)k[a], nsi, &s, F[a], (int)ld[a],
+                              at, 1, &zero, dp + B.w_offset, 1));
+      continue;
+    }
+    // W[i][j] = s sum_n a~_n F_a[n][i] F_b[n][j]: column-major (k_b x k_a) =
+    // (diag(a~) F_b)^T-as-(k_b x Ns) times F_a (Ns x k_a)
+    hipLaunchKernelGGL(scale_rows_kernel, dim3(2048), dim3(256), 0, st, Ns, (int)k[fb], F[fb],
+                       ld[fb], at, Y);
+    VMCX_LAUNCHED();
+    VMCX_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, (int)k[fb],
+                            (int)k[a], nsi, &s, Y, (int)k[fb], F[a], (int)ld[a], &zero,
+                            dp + B.w_offset, (int)k[fb]));
+    if (B.bias_offset >= 0)
+      VMCX_BLAS(rocblas_dgemv(h, rocblas_operation_none, (int)k[a], nsi, &s, F[a], (int)ld[a],
+                              at, 1, &zero, dp + B.bias_offset, 1));
+  }
+  if (stats) {
+    hipLaunchKernelGGL(finish_stats, dim3(1), dim3(1), 0, st, sc, stats);
+    VMCX_LAUNCHED();
+  }
+  if (info) VMCX_HIP(hipMemcpyAsync(info, iinfo, sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+  timer.mark();
+  timer.read(phase_ms);
+  return 0;
 }
 
 // p <- p - lr * dp

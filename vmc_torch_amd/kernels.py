@@ -257,6 +257,121 @@ def sr_minres(O, E, energy_mean, diag_shift, rtol=5e-5, maxiter=100, workspace=N
     return dp, stats, info
 
 
+class FactoredO:
+    """Per-sample log-derivatives in factored form, the input of
+    vmcx_sr_minsr_factored.  F: (Ns, width) float64 row-major, the factor
+    matrices side by side; factors: [(column offset, width)]; blocks:
+    [(a, b, scale, w_offset, bias_offset)] with b = -1 for a plain block
+    (O[:, w_offset:+k_a] = scale F_a) and otherwise O[:, w_offset:+k_a k_b] =
+    scale F_a (x) F_b row by row, plus O[:, bias_offset:+k_a] = scale F_a when
+    bias_offset >= 0.  n_params: the width of O."""
+
+    def __init__(self, F, factors, blocks, n_params):
+        if not isinstance(F, torch.Tensor) or F.device.type != "cuda" or F.dtype != F64:
+            raise TypeError("F must be a float64 HIP tensor")
+        if F.dim() != 2 or F.stride(1) != 1 or F.stride(0) < F.shape[1]:
+            raise ValueError("F must be a 2-D tensor with unit column stride (rows of a "
+                             "wider tensor are allowed)")
+        self.F, self.factors, self.blocks = F, list(factors), list(blocks)
+        self.n_params = int(n_params)
+
+    @property
+    def shape(self):
+        return (self.F.shape[0], self.n_params)
+
+    @property
+    def dtype(self):
+        return F64
+
+    @property
+    def device(self):
+        return self.F.device
+
+    def factor(self, f):
+        c, k = self.factors[f]
+        return self.F[:, c:c + k]
+
+    def __getitem__(self, rows):
+        return FactoredO(self.F[rows], self.factors, self.blocks, self.n_params)
+
+    @staticmethod
+    def cat(parts):
+        p0 = parts[0]
+        return FactoredO(torch.cat([p.F for p in parts]), p0.factors, p0.blocks, p0.n_params)
+
+    def materialize(self):
+        """The dense (Ns, n_params) O (for checks on small cases)."""
+        Ns = self.F.shape[0]
+        O = torch.zeros(Ns, self.n_params, dtype=F64, device=self.F.device)
+        for a, b, sc, wo, bo in self.blocks:
+            Fa = self.factor(a)
+            if b < 0:
+                O[:, wo:wo + Fa.shape[1]] = sc * Fa
+                continue
+            Fb = self.factor(b)
+            ka, kb = Fa.shape[1], Fb.shape[1]
+            O[:, wo:wo + ka * kb] = sc * (Fa.unsqueeze(2) * Fb.unsqueeze(1)).reshape(Ns, ka * kb)
+            if bo >= 0:
+                O[:, bo:bo + ka] = sc * Fa
+        return O
+
+    def _c_args(self):
+        import ctypes
+        nf, nb = len(self.factors), len(self.blocks)
+        ptrs = (ctypes.c_void_p * nf)(*[self.F.data_ptr() + 8 * c for c, _ in self.factors])
+        lds = (ctypes.c_int64 * nf)(*[self.F.stride(0)] * nf)
+        ks = (ctypes.c_int32 * nf)(*[k for _, k in self.factors])
+        blk = (_lib.SrBlockStruct * nb)()
+        for i, (a, b, sc, wo, bo) in enumerate(self.blocks):
+            blk[i].a_factor, blk[i].b_factor, blk[i].scale = a, b, float(sc)
+            blk[i].w_offset, blk[i].bias_offset = wo, bo
+        return nf, ptrs, lds, ks, nb, blk
+
+
+class SRFactoredWorkspace:
+    """Device workspace of vmcx_sr_minsr_factored, grown on demand."""
+
+    def __init__(self):
+        self.buf = None
+
+    def get(self, FO):
+        import ctypes
+        nf, _, _, ks, nb, blk = FO._c_args()
+        need = int(lib().vmcx_sr_factored_workspace_bytes(FO.F.shape[0], nf, ks, nb, blk))
+        if need < 0:
+            raise ValueError("bad factored SR sizes")
+        if self.buf is None or self.buf.numel() < need or self.buf.device != FO.device:
+            self.buf = None
+            self.buf = torch.empty(need, dtype=torch.uint8, device=FO.device)
+        return self.buf
+
+
+def sr_minsr_factored(FO, E, energy_mean, diag_shift, workspace=None, dp=None,
+                      with_stats=True, phase_ms=None):
+    """minSR (FP64 Gram, Cholesky) on a FactoredO without forming O.
+    Returns (dp [Np] float64, stats [5] or None, info [1] int32); phase_ms:
+    optional list filled with [factor Grams, combination + centring,
+    factorisation, solve, dp + statistics, diagonal-block launch] in ms."""
+    import ctypes
+    Ns, Np = FO.shape
+    require(E, F64, "local_energies", (Ns,), FO.device)
+    ws = (workspace or SRFactoredWorkspace()).get(FO)
+    dp = torch.empty(Np, dtype=F64, device=FO.device) if dp is None else require(dp, F64, "dp", (Np,))
+    stats = torch.empty(5, dtype=F64, device=FO.device) if with_stats else None
+    info = torch.empty(1, dtype=I32, device=FO.device)
+    em = float("nan") if energy_mean is None else float(energy_mean)
+    ph = (ctypes.c_float * 6)() if phase_ms is not None else None
+    nf, ptrs, lds, ks, nb, blk = FO._c_args()
+    check(lib().vmcx_sr_minsr_factored(Ns, nf, ptrs, lds, ks, nb, blk, Np, ptr(E), em,
+                                       float(diag_shift), ptr(ws), ws.numel(), ptr(dp),
+                                       ptr(stats), ptr(info),
+                                       ctypes.cast(ph, ctypes.c_void_p) if ph is not None else None,
+                                       stream_ptr(FO.device)))
+    if ph is not None:
+        phase_ms[:] = list(ph)
+    return dp, stats, info
+
+
 def sgd_update(param, dp, lr):
     """param -= lr * dp (in place, flat view)."""
     require(param, F64, "param")

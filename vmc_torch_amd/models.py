@@ -285,6 +285,39 @@ class NNBF_GPU(WavefunctionModel_GPU):
         return O, (sign, logabs)
 
 
+    def log_derivative_factored(self, fxs, chunk=4096, out=None):
+        """The per-sample log-derivative as factors (kernels.FactoredO): row s
+        holds [gM | n | dz1 | h1 | dz2 | h2] (2N nf + 2N + 4 hidden numbers in
+        FP64) instead of the n_params-wide O row; O = [gM, dz1 n^T, dz1,
+        dz2 h1^T, dz2, s gM h2^T, s gM] (s = bf_scale), NNBF.py:44-48 order.
+        Returns (FactoredO, (sign, logabs))."""
+        self._sync_weights()
+        B = fxs.shape[0]
+        N, H, nf = self.n_sites, self.hidden, self.n_fermions
+        nD, n_orb = 2 * N * nf, 2 * N
+        width = nD + n_orb + 4 * H
+        F = torch.empty(B, width, dtype=torch.float64, device=fxs.device) if out is None else out
+        sign = torch.empty(B, dtype=torch.float64, device=fxs.device)
+        logabs = torch.empty(B, dtype=torch.float64, device=fxs.device)
+        chunk = min(chunk, max(B, 1))
+        ws = self._ws_for(B, chunk, 0, fxs.device)
+        kernels.check(kernels.lib().vmcx_nnbf_logderiv_factors(
+            ctypes.byref(self._struct_obj), B, kernels.ptr(fxs), kernels.ptr(ws), ws.numel(),
+            chunk, kernels.ptr(F), F.stride(0), kernels.ptr(sign), kernels.ptr(logabs),
+            kernels.stream_ptr(fxs.device)))
+        factors = [(0, nD), (nD, n_orb), (nD + n_orb, H), (nD + n_orb + H, H),
+                   (nD + n_orb + 2 * H, H), (nD + n_orb + 3 * H, H)]
+        o_w1 = nD
+        o_b1 = o_w1 + H * n_orb
+        o_w2 = o_b1 + H
+        o_b2 = o_w2 + H * H
+        o_wo = o_b2 + H
+        o_bo = o_wo + nD * H
+        blocks = [(0, -1, 1.0, 0, -1), (2, 1, 1.0, o_w1, o_b1), (4, 3, 1.0, o_w2, o_b2),
+                  (0, 5, float(self.bf_scale), o_wo, o_bo)]
+        return kernels.FactoredO(F, factors, blocks, self.n_params), (sign, logabs)
+
+
 class fPEPS_Model_GPU(WavefunctionModel_GPU):
     """Z2-symmetric fermionic PEPS with exact boundary contraction
     (GPU/models/pureTNS.py:24-156, chi <= 0 / chi >= D^2 on Lx <= 4).

@@ -128,8 +128,14 @@ class MinSRGPU(PreconditionerGPU):
     """
 
     def __init__(self, gram="fp64", max_iter=None, tol=1e-8, sync_info=True, inplace=False):
+        if gram not in kernels.GRAM_CODES and gram != "factored":
+            raise ValueError(f"unknown gram {gram!r}")
         self.gram = gram
-        self.max_iter = (0 if gram == "fp64" else 20) if max_iter is None else max_iter
+        # "factored": the driver hands over kernels.FactoredO (models with
+        # log_derivative_factored); the FP64 Gram is formed from the factors
+        self.factored = gram == "factored"
+        self.factored_workspace = kernels.SRFactoredWorkspace()
+        self.max_iter = (0 if gram in ("fp64", "factored") else 20) if max_iter is None else max_iter
         self.tol = tol
         self.sync_info = sync_info
         self.inplace = inplace
@@ -141,6 +147,18 @@ class MinSRGPU(PreconditionerGPU):
         t0 = time.time()
         if not run_sr:
             raise NotImplementedError("run_sr=False: use the energy gradient from MinSRGPU stats")
+        if isinstance(local_o, kernels.FactoredO):
+            E = local_energies.to(device=device, dtype=torch.float64).contiguous()
+            if local_o.shape != (total_samples, n_params):
+                raise ValueError("local_o must be (total_samples, n_params) on one GPU")
+            dp, stats, info = kernels.sr_minsr_factored(local_o, E, energy_mean, diag_shift,
+                                                        self.factored_workspace)
+            self.last_stats = stats
+            info_v = int(info.item()) if self.sync_info else info
+            return dp, time.time() - t0, info_v
+        if self.factored:
+            raise TypeError("gram='factored' takes a kernels.FactoredO "
+                            "(model.log_derivative_factored)")
         dt = local_o.dtype if local_o.dtype in (torch.float32, torch.float64) else torch.float64
         if self.gram == "fp64":
             dt = torch.float64
