@@ -71,9 +71,10 @@ WORKLOADS = {
               "walkers")),
     "hubbard4x4_nnftns": dict(
         kind="peps", Lx=4, Ly=4, nu=8, nd=8, B=2048, burn=100, shift=5e-4, lr=0.1,
-        gram="fp64", D=4, hidden=128, eta=1.0,
+        gram="factored", D=4, hidden=128, eta=1.0,
         desc=("4x4 OBC Hubbard t=1 U=8 8u+8d, Z2 fPEPS D=4 (exact contraction, chi=16) with "
-              "NN-fTNS backflow 32-128-128-3200 ReLU (436,736 params), FP64, 2048 walkers")),
+              "NN-fTNS backflow 32-128-128-3200 ReLU (436,736 params), FP64 (Gram from the "
+              "factors of O), 2048 walkers")),
     "hubbard4x4_fpeps": dict(
         kind="peps", Lx=4, Ly=4, nu=8, nd=8, B=2048, burn=100, shift=5e-4, lr=0.1,
         gram="fp64", D=4, hidden=0, eta=0.0,
@@ -468,10 +469,19 @@ class PepsRun(NnbfRun):
         self.Np = self.model.n_params
         self.e_sweep = H.graph.edge_tensor(dev, "sweep")
         self.e_graph = H.graph.edge_tensor(dev, "graph")
-        odt = torch.float64 if w["gram"] == "fp64" else torch.float32
-        self.O = torch.empty(B, self.Np, dtype=odt, device=dev)
+        gram = a.gram or w["gram"]
+        self.factored = gram == "factored"
         self.dp = torch.empty(self.Np, dtype=torch.float64, device=dev)
-        self.ws = self.k.SRWorkspace()
+        if self.factored:
+            P, Hd = self.model.n_tn_params, w["hidden"]
+            self.factor_widths = [P, 2 * N, Hd, Hd, Hd, Hd] if Hd > 0 else [P]
+            self.F = torch.empty(B, sum(self.factor_widths), dtype=torch.float64, device=dev)
+            self.O = self.F
+            self.ws = self.k.SRFactoredWorkspace()
+        else:
+            odt = torch.float64 if gram == "fp64" else torch.float32
+            self.O = torch.empty(B, self.Np, dtype=odt, device=dev)
+            self.ws = self.k.SRWorkspace()
         self.lp, self.proposals_per_step = None, self.e_sweep.shape[0]
 
     def roofline_items(self):
@@ -525,8 +535,8 @@ def main():
 
     w = WORKLOADS[a.workload]
     gram = a.gram or w["gram"]
-    if gram == "factored" and w["kind"] != "nnbf":
-        print(json.dumps({"error": "--gram factored is wired for the NNBF workloads"}))
+    if gram == "factored" and w["kind"] not in ("nnbf", "peps"):
+        print(json.dumps({"error": "--gram factored is wired for the NNBF and fPEPS workloads"}))
         sys.exit(2)
     Lx, Ly, nu, nd, B = w["Lx"], w["Ly"], w["nu"], w["nd"], w["B"]
     N, nf = Lx * Ly, nu + nd
@@ -666,7 +676,8 @@ def main():
         # combination: 7 factor-Gram triangles read, T written; row sums, the
         # centring pass (read + write) and ||g|| read T; T copied once (full)
         tri = Ns * (Ns + 1) / 2 * 8.0
-        cbytes = (7 + 1 + 1 + 2 + 1) * tri + 2.0 * Ns * Ns * 8
+        n_reads = 7 if len(run.factor_widths) > 1 else 1
+        cbytes = (n_reads + 1 + 1 + 2 + 1) * tri + 2.0 * Ns * Ns * 8
     elif gram == "fp64":
         cbytes = 2.0 * Ns * Np * osz + Ns * Np * osz          # read O twice, write X
     else:
@@ -705,6 +716,8 @@ def main():
         dtype = "f64" if gram == "fp64" else f"f64+{gram}-gram"
     elif w["kind"] == "peps":
         dtype = "f64" if gram == "fp64" else f"f64/f32-O/{gram}-gram/f64-cholesky"
+        if gram == "factored":
+            dtype = "f64/factors-of-O/f64-gram/f64-cholesky"
     else:
         o = "f64" if run.o_dtype == "fp64" else "f32"
         dtype = f"{run.mlp}-gemm/f64-det/{o}-O/{gram}-gram/f64-cholesky"

@@ -395,6 +395,18 @@ int vmcx_peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *confi
                        void *workspace, int64_t ws_bytes, int64_t chunk, void *O, int o_dtype,
                        int64_t ld, double *sign, double *logabs, void *stream);
 
+/* The factors of the per-sample log-derivative instead of O (NN-fTNS:
+ * row s of F, [B][ld] float64, ld >= n_params + 2N + 4 hidden, is
+ * [g | n | dz1 | h1 | dz2 | h2] with g = d ln|psi| / d theta_eff; plain fPEPS
+ * (hidden 0): g alone, ld >= n_params).  O = [g, dz1 n^T, dz1, dz2 h1^T, dz2,
+ * eta g h2^T, eta g], the blocks vmcx_sr_minsr_factored consumes.  Also
+ * writes sign / logabs.  Replaces: compute_grads_gpu for NN-fTNS
+ * (GPU/vmc_utils.py:1389-1634, GPU/models/NNfTNS.py:280-955) when the SR
+ * step takes the factored form. */
+int vmcx_peps_logderiv_factors(const vmcx_peps_model *m, int64_t B, const int64_t *configs,
+                               void *workspace, int64_t ws_bytes, int64_t chunk, double *F,
+                               int64_t ld, double *sign, double *logabs, void *stream);
+
 /*
  * Learned Metropolis proposal (no counterpart in the reference; the closest is
  * the external-proposal sampler MetropolisMPSSamplerSpinful, sampler.py:

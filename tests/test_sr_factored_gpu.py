@@ -168,3 +168,44 @@ def test_vmc_loop_factored_matches_dense(device):
         assert lf[k][1] == pytest.approx(ld[k][1], rel=1e-9)
     for a, b in zip(pf, pd):
         assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
+
+
+def _peps(Lx, Ly, D, H, seed, device):
+    from oracle import fpeps as fp
+    from vmc_torch_amd import NNfTNS_Model_GPU, fPEPS_Model_GPU
+    sh = fp.PEPSShape(Lx, Ly, D)
+    rng = np.random.default_rng(seed)
+    th = fp.random_theta(sh, rng)
+    if H == 0:
+        return fPEPS_Model_GPU(Lx, Ly, D, theta=th).to(device)
+    nn = fp.mlp_init(rng, 2 * sh.N, H, sh.n_params, 0.2)
+    m = NNfTNS_Model_GPU(Lx, Ly, D, hidden_dim=H, nn_eta=0.8, activation="relu", theta=th)
+    with torch.no_grad():
+        for p, v in zip(m.params[1:], nn):
+            p.copy_(torch.as_tensor(v))
+    return m.to(device)
+
+
+@pytest.mark.parametrize("H", [0, 16])
+def test_peps_factors_match_dense_O(device, H):
+    """fPEPS / NN-fTNS (FP64 throughout): the dense O and the factors
+    multiplied out agree to FP64 rounding; the factored SR step equals the
+    dense one on the same O within 1e-7 relative.  The factored form centres
+    the Gram after forming it, so its FP64 rounding is relative to the
+    uncentred |O O^T| (large column means here), and with Np = 96 < Ns = 300
+    (plain fPEPS) T is rank-deficient and lifted by the shift alone, whose
+    condition number carries that rounding to ~1e-8 in dp."""
+    print(f"peps factors H={H}", flush=True)
+    m = _peps(4, 2, 2, H, 12, device)
+    x = torch.tensor(random_configs(np.random.default_rng(13), 300, 8, 3, 3), device=device)
+    O, (s, l) = m.log_derivative(x, dtype=torch.float64)
+    FO, (sf, lf) = m.log_derivative_factored(x)
+    assert torch.equal(s, sf) and torch.equal(l, lf)
+    assert FO.shape == tuple(O.shape)
+    Of = FO.materialize()
+    assert float((Of - O).abs().max()) <= 1e-13 * float(O.abs().max())
+    E = torch.tensor(np.random.default_rng(14).normal(size=300) + 1.0, device=device)
+    dp, _, info = kernels.sr_minsr_factored(FO, E, None, 1e-3)
+    dpd, _, _ = kernels.sr_minsr(Of, E, None, 1e-3, "fp64")
+    assert int(info.item()) == 0
+    assert _rel(dp.cpu().numpy(), dpd.cpu().numpy()) < 1e-7

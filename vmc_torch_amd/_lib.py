@@ -77,6 +77,7 @@ _SIGNATURES = {
     "vmcx_peps_local_energy": (_I, [_P, _I64, _I, _P, _D, _D, _P, _P, _P, _P, _I64, _I64,
                                     _P, _P]),
     "vmcx_peps_logderiv": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _I, _I64, _P, _P, _P]),
+    "vmcx_peps_logderiv_factors": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _I64, _P, _P, _P]),
 }
 
 

@@ -338,4 +338,11 @@ int vmcx_peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *confi
                                      sign, logabs, (hipStream_t)stream));
 }
 
+int vmcx_peps_logderiv_factors(const vmcx_peps_model *m, int64_t B, const int64_t *configs,
+                               void *workspace, int64_t ws_bytes, int64_t chunk, double *F,
+                               int64_t ld, double *sign, double *logabs, void *stream) {
+  return status3(vmcx::peps_logderiv_factors(m, B, configs, workspace, ws_bytes, chunk, F, ld,
+                                             sign, logabs, (hipStream_t)stream));
+}
+
 }  // extern "C"

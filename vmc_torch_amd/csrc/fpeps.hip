@@ -528,8 +528,10 @@ __global__ void pnn_dact_kernel(int64_t n, int act, const double *z, double *dh)
 }
 
 // O row: [g (P) | W1 (H x 2N) | b1 | W2 (H x H) | b2 | Wo (P x H) | bo (P)]
-// with dOut = eta g; one block per sample
-template <typename TO>
+// with dOut = eta g; one block per sample.  FACT: the factors of that row
+// instead, [g | n (2N) | dz1 | h1 | dz2 | h2] (W1 = dz1 n^T, W2 = dz2 h1^T,
+// Wo = eta g h2^T, the biases dz1, dz2, eta g).
+template <typename TO, bool FACT = false>
 __global__ void __launch_bounds__(256)
     pnn_write_kernel(int64_t rows, int N, int H, int64_t P, int act, double eta,
                      const int64_t *__restrict__ cfg, const double *__restrict__ G,
@@ -555,6 +557,18 @@ __global__ void __launch_bounds__(256)
   const double *g = G + w * P;
   TO *row = O + w * ld;
   for (int64_t k = threadIdx.x; k < P; k += blockDim.x) row[k] = (TO)g[k];
+  if (FACT) {
+    TO *f = row + P;
+    for (int o = threadIdx.x; o < n_orb; o += blockDim.x) f[o] = (TO)nvec[o];
+    f += n_orb;
+    for (int u = threadIdx.x; u < H; u += blockDim.x) {
+      f[u] = (TO)d1[u];
+      f[H + u] = (TO)h1[u];
+      f[2 * H + u] = (TO)d2[u];
+      f[3 * H + u] = (TO)h2[u];
+    }
+    return;
+  }
   int64_t off = P;
   for (int64_t idx = threadIdx.x; idx < (int64_t)H * n_orb; idx += blockDim.x) {
     const int u = (int)(idx / n_orb), o = (int)(idx - (int64_t)u * n_orb);
@@ -875,9 +889,10 @@ int peps_local_energy(const vmcx_peps_model *m, int64_t B, int n_edges, const in
   return 0;
 }
 
-int peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void *ws,
-                  int64_t ws_bytes, int64_t chunk, void *O, int o_dtype, int64_t ld,
-                  double *sign, double *logabs, hipStream_t st) {
+static int peps_logderiv_impl(const vmcx_peps_model *m, int64_t B, const int64_t *cfg,
+                              void *ws, int64_t ws_bytes, int64_t chunk, void *O, int o_dtype,
+                              int64_t ld, bool fact, double *sign, double *logabs,
+                              hipStream_t st) {
   if (check_peps(m)) return 1;
   VMCX_CHECK_ARG(B >= 0 && chunk >= 1, "bad sizes");
   VMCX_CHECK_ARG(o_dtype == 0 || o_dtype == 1, "o_dtype must be 0 (f64) or 1 (f32)");
@@ -886,8 +901,10 @@ int peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void 
   const int64_t P = m->n_params, H = m->hidden;
   const int N = m->Lx * m->Ly;
   const int64_t np_total =
-      H > 0 ? P + H * 2 * N + H + H * H + H + P * H + P : P;
-  VMCX_CHECK_ARG(ld >= np_total, "ld smaller than the parameter count");
+      fact ? (H > 0 ? P + 2 * N + 4 * H : P)
+           : (H > 0 ? P + H * 2 * N + H + H * H + H + P * H + P : P);
+  VMCX_CHECK_ARG(ld >= np_total, fact ? "ld smaller than the factor width"
+                                      : "ld smaller than the parameter count");
   const PepsWs L = peps_layout(*m, B, chunk, 0, (char *)ws);
   VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
   rocblas_handle h = blas_handle_for(st);
@@ -932,7 +949,11 @@ int peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void 
     hipLaunchKernelGGL(pnn_dact_kernel, dim3(gh), dim3(256), 0, st, nh, m->act, L.z1, L.d1);
     VMCX_LAUNCHED();
     const size_t sm = (size_t)4 * H * 8;
-    if (o_dtype == 0)
+    if (fact)
+      hipLaunchKernelGGL((pnn_write_kernel<double, true>), dim3((unsigned)rows), dim3(256), sm,
+                         st, rows, N, (int)H, P, m->act, m->eta, c, G, L.z1, L.z2, L.d1, L.d2,
+                         (double *)O + r0 * ld, ld);
+    else if (o_dtype == 0)
       hipLaunchKernelGGL(pnn_write_kernel<double>, dim3((unsigned)rows), dim3(256), sm, st, rows,
                          N, (int)H, P, m->act, m->eta, c, G, L.z1, L.z2, L.d1, L.d2,
                          (double *)O + r0 * ld, ld);
@@ -943,6 +964,19 @@ int peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void 
     VMCX_LAUNCHED();
   }
   return 0;
+}
+
+int peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void *ws,
+                  int64_t ws_bytes, int64_t chunk, void *O, int o_dtype, int64_t ld,
+                  double *sign, double *logabs, hipStream_t st) {
+  return peps_logderiv_impl(m, B, cfg, ws, ws_bytes, chunk, O, o_dtype, ld, false, sign, logabs,
+                            st);
+}
+
+int peps_logderiv_factors(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void *ws,
+                          int64_t ws_bytes, int64_t chunk, double *F, int64_t ld,
+                          double *sign, double *logabs, hipStream_t st) {
+  return peps_logderiv_impl(m, B, cfg, ws, ws_bytes, chunk, F, 0, ld, true, sign, logabs, st);
 }
 
 }  // namespace vmcx

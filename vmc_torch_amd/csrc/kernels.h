@@ -123,6 +123,9 @@ int peps_local_energy(const vmcx_peps_model *m, int64_t B, int n_edges, const in
 int peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void *ws,
                   int64_t ws_bytes, int64_t chunk, void *O, int o_dtype, int64_t ld,
                   double *sign, double *logabs, hipStream_t st);
+int peps_logderiv_factors(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void *ws,
+                          int64_t ws_bytes, int64_t chunk, double *F, int64_t ld,
+                          double *sign, double *logabs, hipStream_t st);
 
 int lp_n_params(int C);
 int lp_n_moves(int Lx, int Ly);

@@ -459,6 +459,43 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
         return O, (sign, logabs)
 
 
+    def log_derivative_factored(self, fxs, chunk=4096, out=None):
+        """The per-sample log-derivative as factors (kernels.FactoredO): with
+        the backflow MLP, row s holds [g | n | dz1 | h1 | dz2 | h2] in FP64
+        (g = d ln|psi| / d theta_eff) and O = [g, dz1 n^T, dz1, dz2 h1^T, dz2,
+        eta g h2^T, eta g]; plain fPEPS: O = g.  Returns (FactoredO,
+        (sign, logabs))."""
+        self._sync()
+        B = fxs.shape[0]
+        N = self.Lx * self.Ly
+        P = self.params[0].numel()
+        H = int(getattr(self, "hidden", 0) or 0)
+        width = P + 2 * N + 4 * H if H > 0 else P
+        F = torch.empty(B, width, dtype=torch.float64, device=fxs.device) if out is None else out
+        sign = torch.empty(B, dtype=torch.float64, device=fxs.device)
+        logabs = torch.empty(B, dtype=torch.float64, device=fxs.device)
+        chunk = min(chunk, max(B, 1))
+        ws = self._ws_for(B, chunk, 0, fxs.device)
+        kernels.check(kernels.lib().vmcx_peps_logderiv_factors(
+            ctypes.byref(self._struct_obj), B, kernels.ptr(fxs), kernels.ptr(ws), ws.numel(),
+            chunk, kernels.ptr(F), F.stride(0), kernels.ptr(sign), kernels.ptr(logabs),
+            kernels.stream_ptr(fxs.device)))
+        if H == 0:
+            return kernels.FactoredO(F, [(0, P)], [(0, -1, 1.0, 0, -1)], P), (sign, logabs)
+        n_orb = 2 * N
+        factors = [(0, P), (P, n_orb), (P + n_orb, H), (P + n_orb + H, H),
+                   (P + n_orb + 2 * H, H), (P + n_orb + 3 * H, H)]
+        o_w1 = P
+        o_b1 = o_w1 + H * n_orb
+        o_w2 = o_b1 + H
+        o_b2 = o_w2 + H * H
+        o_wo = o_b2 + H
+        o_bo = o_wo + P * H
+        blocks = [(0, -1, 1.0, 0, -1), (2, 1, 1.0, o_w1, o_b1), (4, 3, 1.0, o_w2, o_b2),
+                  (0, 5, float(self.nn_eta), o_wo, o_bo)]
+        return kernels.FactoredO(F, factors, blocks, self.n_params), (sign, logabs)
+
+
 class NNfTNS_Model_GPU(fPEPS_Model_GPU):
     """NN-fTNS (GPU/models/NNfTNS.py:280-955): the fPEPS above with the
     additive backflow theta(x) = theta + nn_eta * MLP(n(x)) on every tensor
