@@ -871,9 +871,12 @@ int peps_local_energy(const vmcx_peps_model *m, int64_t B, int n_edges, const in
                      edges, cfg, L.counts);
   hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, st, B, L.counts, L.offsets,
                      L.total);
-  hipLaunchKernelGGL(conn_fill_walker_kernel, dim3(gw), dim3(256), 0, st, B, N, n_edges, edges,
-                     t, cfg, L.offsets, L.etas, L.coeffs);
   VMCX_LAUNCHED();
+  {
+    const int rc = launch_conn_fill(B, N, n_edges, edges, t, cfg, L.offsets, L.etas, L.coeffs,
+                                    st);
+    if (rc) return rc;
+  }
   int64_t T = 0;
   VMCX_HIP(hipMemcpyAsync(&T, L.total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   VMCX_HIP(hipStreamSynchronize(st));

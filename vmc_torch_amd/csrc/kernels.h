@@ -96,6 +96,9 @@ __global__ void nnbf_propose_kernel(int64_t B, int N, int i, int j, int e, int64
                                     double *u_acc);
 __global__ void conn_count_walker_kernel(int64_t B, int N, int E, const int32_t *edges,
                                          const int64_t *cfg, int64_t *counts);
+int launch_conn_fill(int64_t B, int N, int E, const int32_t *edges, double t,
+                     const int64_t *cfg, const int64_t *offsets, int64_t *etas, double *coeffs,
+                     hipStream_t st);
 __global__ void exclusive_scan_kernel(int64_t n, const int64_t *in, int64_t *out,
                                       int64_t *total);
 __global__ void conn_fill_walker_kernel(int64_t B, int N, int E, const int32_t *edges,

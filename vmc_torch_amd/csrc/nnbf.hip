@@ -379,38 +379,74 @@ __global__ void __launch_bounds__(1024)
   if (threadIdx.x == 0 && total) total[0] = carry;
 }
 
+// bits a..b of a 64-bit mask (0 when a > b; 0 <= a, b <= 63)
+__device__ __forceinline__ uint64_t bits_between(int a, int b) {
+  if (a > b) return 0;
+  const uint64_t hi = (b >= 63) ? ~0ull : ((1ull << (b + 1)) - 1);
+  return hi & ~((1ull << a) - 1);
+}
+
+// One wave per walker (N <= 64: lane s holds site s): the 64 lanes take 32
+// edges x 2 spins per round in (edge, spin) order, a ballot prefix gives each
+// valid hop its slot, and every connected configuration is written by the
+// 64 lanes at once (one site each).  The Jordan-Wigner parity of a hop i <-> j
+// counts the modes strictly between them in the order (site, up before
+// down): popcounts of the occupation masks over the site range.
 __global__ void __launch_bounds__(256)
     conn_fill_walker_kernel(int64_t B, int N, int E, const int32_t *__restrict__ edges,
                             double t, const int64_t *__restrict__ cfg,
                             const int64_t *__restrict__ offsets, int64_t *etas,
                             double *coeffs) {
-  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= B) return;
-  const int64_t *c = cfg + w * N;
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= B) return;   // uniform per wave
+  const int64_t cs = lane < N ? cfg[w * N + lane] : 0;
+  const uint64_t up = __ballot(((cs >> 1) & 1) != 0), dn = __ballot((cs & 1) != 0);
   int64_t pos = offsets[w];
-  for (int e = 0; e < E; ++e) {
-    const int i = edges[2 * e], j = edges[2 * e + 1];
-    if ((unsigned)i >= (unsigned)N || (unsigned)j >= (unsigned)N || i == j) continue;
-    for (int sp = 0; sp < 2; ++sp) {
-      const int bit = sp == 0 ? 2 : 1;
-      if (((c[i] & bit) != 0) == ((c[j] & bit) != 0)) continue;
-      const int lo = i < j ? i : j, hi = i < j ? j : i;
-      int par = 0;
-      if (bit == 2) {
-        for (int s = lo + 1; s < hi; ++s) par += (int)((c[s] >> 1) & 1);
-        for (int s = lo + 1; s <= hi; ++s) par += (int)(c[s] & 1);
-      } else {
-        for (int s = lo; s < hi; ++s) par += (int)((c[s] >> 1) & 1);
-        for (int s = lo + 1; s < hi; ++s) par += (int)(c[s] & 1);
+  for (int e0 = 0; e0 < E; e0 += 32) {
+    const int e = e0 + (lane >> 1), sp = lane & 1;
+    const int bit = sp == 0 ? 2 : 1;
+    int i = 0, j = 0, par = 0;
+    bool valid = false;
+    if (e < E) {
+      i = edges[2 * e];
+      j = edges[2 * e + 1];
+      if ((unsigned)i < (unsigned)N && (unsigned)j < (unsigned)N && i != j) {
+        const uint64_t m = bit == 2 ? up : dn;
+        valid = ((m >> i) & 1) != ((m >> j) & 1);
+        const int lo = i < j ? i : j, hi = i < j ? j : i;
+        if (bit == 2)
+          par = __popcll(up & bits_between(lo + 1, hi - 1)) +
+                __popcll(dn & bits_between(lo + 1, hi));
+        else
+          par = __popcll(up & bits_between(lo, hi - 1)) +
+                __popcll(dn & bits_between(lo + 1, hi - 1));
       }
-      int64_t *eta = etas + pos * N;
-      for (int s = 0; s < N; ++s) eta[s] = c[s];
-      eta[i] = c[i] ^ bit;
-      eta[j] = c[j] ^ bit;
-      coeffs[pos] = (par & 1) ? t : -t;
-      ++pos;
     }
+    const uint64_t vb = __ballot(valid);
+    if (valid) coeffs[pos + __popcll(vb & ((1ull << lane) - 1))] = (par & 1) ? t : -t;
+    uint64_t rem = vb;
+    int64_t k = pos;
+    while (rem) {
+      const int src = __ffsll((long long)rem) - 1;
+      rem &= rem - 1;
+      const int ii = __shfl(i, src, 64), jj = __shfl(j, src, 64), bb = __shfl(bit, src, 64);
+      if (lane < N) etas[k * N + lane] = (lane == ii || lane == jj) ? (cs ^ bb) : cs;
+      ++k;
+    }
+    pos = k;
   }
+}
+
+int launch_conn_fill(int64_t B, int N, int E, const int32_t *edges, double t,
+                     const int64_t *cfg, const int64_t *offsets, int64_t *etas, double *coeffs,
+                     hipStream_t st) {
+  VMCX_CHECK_ARG(N >= 1 && N <= 64, "connected-configuration fill needs 1 <= N <= 64");
+  if (B == 0) return 0;
+  hipLaunchKernelGGL(conn_fill_walker_kernel, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, st,
+                     B, N, E, edges, t, cfg, offsets, etas, coeffs);
+  VMCX_LAUNCHED();
+  return 0;
 }
 
 __global__ void __launch_bounds__(256)
@@ -801,9 +837,11 @@ int nnbf_local_energy(const vmcx_nnbf_model *m, int64_t B, int n_edges, const in
   hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, st, B, L.counts,
                      L.offsets, L.total);
   VMCX_LAUNCHED();
-  hipLaunchKernelGGL(conn_fill_walker_kernel, dim3(gw), dim3(256), 0, st, B, N, n_edges,
-                     edges, t, cfg, L.offsets, L.etas, L.coeffs);
-  VMCX_LAUNCHED();
+  {
+    const int rc = launch_conn_fill(B, N, n_edges, edges, t, cfg, L.offsets, L.etas, L.coeffs,
+                                    st);
+    if (rc) return rc;
+  }
   int64_t T = 0;
   VMCX_HIP(hipMemcpyAsync(&T, L.total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   VMCX_HIP(hipStreamSynchronize(st));
