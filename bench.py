@@ -107,6 +107,9 @@ def parse():
                    help="Gram precision (default: the workload's); factored (NNBF): the "
                         "FP64 Gram from the Grams of the factors of O, no O matrix")
     p.add_argument("--refresh-every", type=int, default=64)
+    p.add_argument("--sweeps-per-sample", type=int, default=1,
+                   help="Slater: Metropolis sweeps between samples (1 = the reference's "
+                        "one sample per sweep)")
     p.add_argument("--mlp", default=None, choices=["fp32", "bf16"],
                    help="NNBF dense-layer precision (default: the workload's)")
     p.add_argument("--o-dtype", default=None, choices=["fp32", "fp64"],
@@ -269,6 +272,8 @@ class SlaterRun:
         self.O = torch.empty(B, self.Np, dtype=torch.float64, device=dev)
         self.dp = torch.empty(self.Np, dtype=torch.float64, device=dev)
         self.ws = kernels.SRWorkspace()
+        self.spp = max(1, a.sweeps_per_sample)
+        self.proposals_per_step = self.e_sweep.shape[0] * self.spp
 
     def _sweep(self, n, seed, off):
         self.k.slater_sweep(self.M, self.e_sweep, self.fxs, self.st, n_sweeps=n, seed=seed,
@@ -286,7 +291,7 @@ class SlaterRun:
         self.k.slater_refresh(self.M, self.fxs, self.st)     # M changed in the last update
         m.append(mark())
         acc0 = self.st.n_accept.clone()
-        self._sweep(1, seed, off)
+        self._sweep(self.spp, seed, off)
         m.append(mark())
         E = self.local_energy()
         m.append(mark())
@@ -304,16 +309,16 @@ class SlaterRun:
         self.k.slater_refresh(self.M, self.fxs, self.st)
 
     def tau_sample(self, seed, off):
-        self._sweep(1, seed, off)
+        self._sweep(self.spp, seed, off)
         return self.local_energy()
 
     def roofline_items(self):
         """Algorithmic HBM bytes per step: A^-1 (B nfp^2 FP64) read and, for
-        the sweep, written once; configs and the per-walker rows."""
+        the sweep, written once per sweep; configs and the per-walker rows."""
         B, N, nf = self.B, self.N, self.nf
         nfp = self.k.slater_nfp(nf)
         return {
-            "sweep": ("hbm", B * (2 * nfp * nfp * 8 + 2 * 2 * N + 2 * 8 * N)),
+            "sweep": ("hbm", self.spp * B * (2 * nfp * nfp * 8 + 2 * 2 * N + 2 * 8 * N)),
             "local_energy": ("hbm", B * (nfp * nfp * 8 + 2 * N + 8 * N + 8)),
             "logderiv": ("hbm", B * (nf * nfp * 8 + 2 * N + self.Np * 8)),
             "refresh": ("hbm", B * (nfp * nfp * 8 + 2 * N + 8 * N)),
@@ -559,6 +564,10 @@ def main():
     fxs = base[perms].contiguous().to(dev)
     run = {"slater": SlaterRun, "nnbf": NnbfRun, "peps": PepsRun}[w["kind"]](w, a, dev, H, fxs)
     seed, sweep_no = 0x5EED, 0
+    spp = getattr(run, "spp", 1)
+    if a.sweeps_per_sample != 1 and w["kind"] != "slater":
+        print(json.dumps({"error": "--sweeps-per-sample is wired for the Slater workloads"}))
+        sys.exit(2)
     log(f"{a.workload}: {B} walkers, {run.Np} parameters; burn-in {w['burn']} sweeps")
     run.burn(w["burn"], seed, 0)
     sweep_no += w["burn"]
@@ -575,12 +584,12 @@ def main():
     E_hist, accs = [], []
     for _ in range(a.warmup):
         run.step(mark, seed, sweep_no, gram, None)
-        sweep_no += 1
+        sweep_no += spp
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
         m, E, acc, stats, info = run.step(mark, seed, sweep_no, gram, None)
-        sweep_no += 1
+        sweep_no += spp
         for k, x0, x1 in zip(names, m[:-1], m[1:]):
             ev[k].append((x0, x1))
         E_hist.append(E.clone())
@@ -598,7 +607,7 @@ def main():
     for _ in range(3):
         ph = []
         run.step(mark, seed, sweep_no, gram, ph)
-        sweep_no += 1
+        sweep_no += spp
         sr_phases.append(ph)
     torch.cuda.synchronize()
     phase_names = ["center", "gram", "factor", "solve", "dp_stats", "chol_diag_launch"]
@@ -615,7 +624,7 @@ def main():
     series = []
     for _ in range(a.tau_sweeps):
         series.append(run.tau_sample(seed, sweep_no).clone())
-        sweep_no += 1
+        sweep_no += spp
     torch.cuda.synchronize()
     tau = tau_int(torch.stack(series, dim=1))
     log(f"SR phases and tau_int ({a.tau_sweeps} sweeps) done: tau_int = {tau:.3f}")
@@ -737,12 +746,12 @@ def main():
         "config": {"workload": a.workload, "description": w["desc"], "walkers": B,
                    "samples_per_sr_iter": Ns, "n_params": Np, "burn_in_sweeps": w["burn"],
                    "gram": gram, "diag_shift": w["shift"], "lr": w["lr"],
-                   "sampler": a.sampler,
+                   "sampler": a.sampler, "sweeps_per_sample": spp,
                    "proposals_per_sample": getattr(run, "proposals_per_step", None),
                    "parallelism": "walkers on 1 GPU"},
         "sr_iters_per_s": sr_iters,
-        "walker_sweeps_per_s": samples_per_s,
-        "sweep_only_walker_sweeps_per_s": B / (kt["sweep"] * 1e-3),
+        "walker_sweeps_per_s": samples_per_s * spp,
+        "sweep_only_walker_sweeps_per_s": B * spp / (kt["sweep"] * 1e-3),
         "tau_int": tau, "acceptance": acc_rate,
         "energy_per_site_first_last": [e_site[0], e_site[-1]],
         "sr_solver": {"info": info_v, "rel_residual": st_v[2], "grad_norm": st_v[3],
@@ -753,7 +762,7 @@ def main():
     if w["kind"] == "slater":
         out["config"]["refresh_every"] = a.refresh_every
 
-    if not a.no_cpu_baseline:
+    if not a.no_cpu_baseline and spp == 1:   # the CPU sample is one sweep per sample
         # FP64 oracle on a subset of the final walkers: |dE/site|, then the timed baseline
         from oracle import fpeps as ofp
         from oracle import hubbard as ohub
