@@ -141,3 +141,14 @@ def test_workspace_bytes():
     assert need >= 4 * nn
     assert need < 12 * nn
     assert lib.vmcx_sr_factored_workspace_bytes(0, 2, k, 2, blocks) == -1
+
+
+def test_factor_entry_points_reject_bad_arguments():
+    """The factor outputs of the NNBF and fPEPS log-derivatives refuse a
+    missing model before any device call."""
+    lib = _lib.lib()
+    p = ctypes.c_void_p(16)
+    rc = lib.vmcx_nnbf_logderiv_factors(None, 4, p, p, 1 << 20, 4, p, 100, p, p, None)
+    assert rc == 1 and lib.vmcx_last_error().decode()
+    rc = lib.vmcx_peps_logderiv_factors(None, 4, p, p, 1 << 20, 4, p, 100, p, p, None)
+    assert rc == 1 and lib.vmcx_last_error().decode()
