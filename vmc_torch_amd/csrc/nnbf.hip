@@ -469,30 +469,25 @@ __global__ void __launch_bounds__(256)
 }
 
 // ------------------------------------------------------------ backward
-// One block per sample: O = d ln|psi| / d theta in the order
-// [M, W1, b1, W2, b2, Wout, bout].
-// FACT: the row holds the factors of O instead (nD + 2N + 4H numbers):
-// [gM | n | dz1 | h1 | dz2 | h2], from which every block of O is gM itself
-// (M), an outer product dz1 n^T, dz2 h1^T, s gM h2^T (W1, W2, Wout) or a
-// factor (b1 = dz1, b2 = dz2, bout = s gM).
-template <typename TO, bool FACT = false>
+// O = d ln|psi| / d theta in the order [M, W1, b1, W2, b2, Wout, bout]:
+//   gM = d ln|det A| / dA scattered to the occupied orbital rows (the M
+//   block), dOut = s gM, dh2 = dOut Wout, dz2 = dh2 act'(z2), dh1 = dz2 W2,
+//   dz1 = dh1 act'(z1); W1 = dz1 n^T, W2 = dz2 h1^T, Wout = dOut h2^T.
+// The two products over the batch are SGEMMs; the per-sample kernels around
+// them write the rows.  FACT: the row holds the factors instead (nD + 2N +
+// 4H numbers): [gM | n | dz1 | h1 | dz2 | h2].
+
+// head, one block per sample: gM (the M block of the row, TO) and s gM into
+// G (FP32, [rows][nD], the GEMM operand)
+template <typename TO>
 __global__ void __launch_bounds__(256)
-    nnbf_backward_kernel(int64_t Bc, int N, int nf, int H, int act, int nfp,
-                         const int64_t *__restrict__ cfg, const float *__restrict__ z1,
-                         const float *__restrict__ z2, const double *__restrict__ ainv,
-                         const float *__restrict__ W2, const float *__restrict__ Wout,
-                         double s, TO *O, int64_t ld) {
-  extern __shared__ __align__(16) float fsm[];
-  const int n_orb = 2 * N, nD = n_orb * nf;
-  float *gD = fsm;                 // [nD]
-  float *h1 = gD + nD;             // [H]
-  float *h2 = h1 + H;              // [H]
-  float *dz2 = h2 + H;             // [H]
-  float *dz1 = dz2 + H;            // [H]
-  float *nvec = dz1 + H;           // [n_orb]
+    nnbf_grad_head_kernel(int64_t Bc, int N, int nf, int nfp, const int64_t *__restrict__ cfg,
+                          const double *__restrict__ ainv, double s, float *G, TO *O,
+                          int64_t ld) {
   __shared__ int slot_of[128];
   const int64_t w = blockIdx.x;
   const int tid = threadIdx.x, nt = blockDim.x;
+  const int n_orb = 2 * N, nD = n_orb * nf;
   if (tid == 0) {
     int k = 0;
     const int64_t *c = cfg + w * N;
@@ -503,80 +498,89 @@ __global__ void __launch_bounds__(256)
       if (c[x] & 1) slot_of[N + x] = k++;
   }
   __syncthreads();
-  for (int o = tid; o < n_orb; o += nt) nvec[o] = slot_of[o] >= 0 ? 1.0f : 0.0f;
   const double *Ai = ainv + (size_t)w * nfp * nfp;
+  const float sf = (float)s;
+  TO *row = O + w * ld;
+  float *g = G + w * (int64_t)nD;
   for (int idx = tid; idx < nD; idx += nt) {
     const int o = idx / nf, c = idx - o * nf;
     const int r = slot_of[o];
     // d ln|det A| / d A[r][c] = (A^-1)[c][r] = ainv[r][c]
-    gD[idx] = (r >= 0) ? (float)Ai[(size_t)r * nfp + c] : 0.0f;
+    const float v = (r >= 0) ? (float)Ai[(size_t)r * nfp + c] : 0.0f;
+    row[idx] = (TO)v;
+    g[idx] = v * sf;
   }
+}
+
+// d *= act'(z) elementwise
+__global__ void __launch_bounds__(256)
+    nnbf_dact_kernel(int64_t n, int act, const float *__restrict__ z, float *d) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = z[i];
+    d[i] = act == 0 ? (v > 0.0f ? d[i] : 0.0f) : d[i] * (1.0f - tanhf(v) * tanhf(v));
+  }
+}
+
+// tail, one block per sample: the rest of the row from dz1, dz2 ([rows][H]),
+// z1, z2 and s gM (G)
+template <typename TO, bool FACT>
+__global__ void __launch_bounds__(256)
+    nnbf_grad_tail_kernel(int64_t Bc, int N, int nf, int H, int act,
+                          const int64_t *__restrict__ cfg, const float *__restrict__ z1,
+                          const float *__restrict__ z2, const float *__restrict__ dz1,
+                          const float *__restrict__ dz2, const float *__restrict__ G, TO *O,
+                          int64_t ld) {
+  extern __shared__ float tsm[];
+  float *h1 = tsm, *h2 = h1 + H, *d1 = h2 + H, *d2 = d1 + H, *nvec = d2 + H;   // 4H + 2N
+  const int64_t w = blockIdx.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int n_orb = 2 * N, nD = n_orb * nf;
+  const int64_t *c = cfg + w * N;
+  for (int o = tid; o < n_orb; o += nt)
+    nvec[o] = (o < N) ? ((c[o] & 2) ? 1.0f : 0.0f) : ((c[o - N] & 1) ? 1.0f : 0.0f);
   for (int u = tid; u < H; u += nt) {
     h1[u] = act_f(z1[w * H + u], act);
     h2[u] = act_f(z2[w * H + u], act);
+    d1[u] = dz1[w * H + u];
+    d2[u] = dz2[w * H + u];
   }
   __syncthreads();
   TO *row = O + w * ld;
-  // M block: gM (unscaled)
-  for (int idx = tid; idx < nD; idx += nt) row[idx] = (TO)gD[idx];
-  __syncthreads();
-  const float sf = (float)s;
-  for (int idx = tid; idx < nD; idx += nt) gD[idx] *= sf;
-  __syncthreads();
-  // dh2[k] = sum_i gD[i] Wout[i][k] over occupied rows only
-  for (int k = tid; k < H; k += nt) {
-    float acc = 0.0f;
-    for (int o = 0; o < n_orb; ++o) {
-      if (slot_of[o] < 0) continue;
-      const float *wr = Wout + (int64_t)o * nf * H + k;
-      const float *g = gD + o * nf;
-      for (int c = 0; c < nf; ++c) acc += g[c] * wr[(int64_t)c * H];
-    }
-    const float z = z2[w * H + k];
-    dz2[k] = act == 0 ? (z > 0.0f ? acc : 0.0f) : acc * (1.0f - tanhf(z) * tanhf(z));
-  }
-  __syncthreads();
-  // dh1[i] = sum_u dz2[u] W2[u][i]
-  for (int i = tid; i < H; i += nt) {
-    float acc = 0.0f;
-    for (int u = 0; u < H; ++u) acc += dz2[u] * W2[(int64_t)u * H + i];
-    const float z = z1[w * H + i];
-    dz1[i] = act == 0 ? (z > 0.0f ? acc : 0.0f) : acc * (1.0f - tanhf(z) * tanhf(z));
-  }
-  __syncthreads();
   if (FACT) {
     TO *f = row + nD;
     for (int o = tid; o < n_orb; o += nt) f[o] = (TO)nvec[o];
     f += n_orb;
     for (int u = tid; u < H; u += nt) {
-      f[u] = (TO)dz1[u];
+      f[u] = (TO)d1[u];
       f[H + u] = (TO)h1[u];
-      f[2 * H + u] = (TO)dz2[u];
+      f[2 * H + u] = (TO)d2[u];
       f[3 * H + u] = (TO)h2[u];
     }
     return;
   }
+  const float *g = G + w * (int64_t)nD;
   int64_t off = nD;
   for (int64_t idx = tid; idx < (int64_t)H * n_orb; idx += nt) {   // W1
     const int u = (int)(idx / n_orb), jj = (int)(idx - (int64_t)u * n_orb);
-    row[off + idx] = (TO)(dz1[u] * nvec[jj]);
+    row[off + idx] = (TO)(d1[u] * nvec[jj]);
   }
   off += (int64_t)H * n_orb;
-  for (int u = tid; u < H; u += nt) row[off + u] = (TO)dz1[u];       // b1
+  for (int u = tid; u < H; u += nt) row[off + u] = (TO)d1[u];       // b1
   off += H;
   for (int64_t idx = tid; idx < (int64_t)H * H; idx += nt) {         // W2
     const int u = (int)(idx / H), i = (int)(idx - (int64_t)u * H);
-    row[off + idx] = (TO)(dz2[u] * h1[i]);
+    row[off + idx] = (TO)(d2[u] * h1[i]);
   }
   off += (int64_t)H * H;
-  for (int u = tid; u < H; u += nt) row[off + u] = (TO)dz2[u];       // b2
+  for (int u = tid; u < H; u += nt) row[off + u] = (TO)d2[u];       // b2
   off += H;
   for (int64_t idx = tid; idx < (int64_t)nD * H; idx += nt) {        // Wout
     const int i = (int)(idx / H), k = (int)(idx - (int64_t)i * H);
-    row[off + idx] = (TO)(gD[i] * h2[k]);
+    row[off + idx] = (TO)(g[i] * h2[k]);
   }
   off += (int64_t)nD * H;
-  for (int idx = tid; idx < nD; idx += nt) row[off + idx] = (TO)gD[idx];   // bout
+  for (int idx = tid; idx < nD; idx += nt) row[off + idx] = (TO)g[idx];   // bout
 }
 
 // ============================================================ host side
@@ -584,6 +588,7 @@ struct NnbfWs {
   // per-chunk forward buffers
   float *z1, *z2, *D;
   void *h1, *h2;
+  float *g1, *g2;   // backward: dz1, dz2
   double *ainv;
   // sweep buffers (per walker)
   int32_t *prop;
@@ -613,6 +618,8 @@ static NnbfWs nnbf_layout(const vmcx_nnbf_model &m, int64_t B, int64_t chunk, in
   L.D = (float *)take((size_t)chunk * nD * 4);
   L.h1 = take((size_t)chunk * H * ti);
   L.h2 = take((size_t)chunk * H * ti);
+  L.g1 = (float *)take((size_t)chunk * H * 4);
+  L.g2 = (float *)take((size_t)chunk * H * 4);
   L.ainv = (double *)take((size_t)chunk * nfp * nfp * 8);
   L.prop = (int32_t *)take((size_t)B * 4);
   L.moves = (uint32_t *)take((size_t)B * 4);
@@ -857,6 +864,38 @@ int nnbf_local_energy(const vmcx_nnbf_model *m, int64_t B, int n_edges, const in
   return 0;
 }
 
+// rows of O (or of its factors) for `rows` configurations whose forward
+// pass left z1, z2 and A^-1 in the workspace
+template <typename TO, bool FACT>
+static int backward_rows(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
+                         const int64_t *cfg, TO *O, int64_t ld, hipStream_t st) {
+  const int N = m.n_sites, H = m.hidden, nf = m.n_f, nD = 2 * N * nf;
+  rocblas_handle h = blas_handle_for(st);
+  VMCX_CHECK_ARG(h != nullptr, "rocblas handle");
+  hipLaunchKernelGGL(nnbf_grad_head_kernel<TO>, dim3((unsigned)rows), dim3(256), 0, st, rows, N,
+                     nf, nfp_of(nf), cfg, L.ainv, m.bf_scale, L.D, O, ld);
+  VMCX_LAUNCHED();
+  const float one = 1.0f, zero = 0.0f;
+  const int ri = (int)rows;
+  // column-major views: dh2 (H x rows) = Wout^T (Wout row-major [nD][H]) * (s gM) (nD x rows)
+  NN_BLAS(rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, H, ri, nD, &one,
+                        m.Wout, H, L.D, nD, &zero, L.g2, H));
+  const int64_t nh = rows * H;
+  const unsigned gh = (unsigned)((nh + 255) / 256 < 8192 ? (nh + 255) / 256 : 8192);
+  hipLaunchKernelGGL(nnbf_dact_kernel, dim3(gh), dim3(256), 0, st, nh, m.act, L.z2, L.g2);
+  VMCX_LAUNCHED();
+  // dh1 (H x rows) = W2^T-as-stored (W2 row-major [u][i]: column-major (i, u)) * dz2
+  NN_BLAS(rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, H, ri, H, &one, m.W2,
+                        H, L.g2, H, &zero, L.g1, H));
+  hipLaunchKernelGGL(nnbf_dact_kernel, dim3(gh), dim3(256), 0, st, nh, m.act, L.z1, L.g1);
+  VMCX_LAUNCHED();
+  const size_t lds = (size_t)(4 * H + 2 * N) * sizeof(float);
+  hipLaunchKernelGGL((nnbf_grad_tail_kernel<TO, FACT>), dim3((unsigned)rows), dim3(256), lds, st,
+                     rows, N, nf, H, m.act, cfg, L.z1, L.z2, L.g1, L.g2, L.D, O, ld);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
 int nnbf_logderiv(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
                   int64_t ws_bytes, int64_t chunk, void *O, int o_dtype, int64_t ld,
                   double *sign, double *logabs, hipStream_t st) {
@@ -871,21 +910,17 @@ int nnbf_logderiv(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void 
   VMCX_CHECK_ARG(ld >= np, "ld smaller than the parameter count");
   const NnbfWs L = nnbf_layout(*m, B, chunk, 0, (char *)ws);
   VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
-  const size_t lds = (size_t)(nD + 4 * H + 2 * N) * sizeof(float);
-  VMCX_CHECK_ARG(lds <= 65536, "backward LDS exceeds 64 KB");
+  VMCX_CHECK_ARG((4 * H + 2 * N) * 4 <= 65536, "backward LDS exceeds 64 KB");
   for (int64_t r0 = 0; r0 < B; r0 += chunk) {
     const int64_t rows = (B - r0 < chunk) ? B - r0 : chunk;
     int rc = forward_rows(*m, L, rows, cfg + r0 * N, sign + r0, logabs + r0, L.ainv, st);
     if (rc) return rc;
-    if (o_dtype == 0)
-      hipLaunchKernelGGL(nnbf_backward_kernel<double>, dim3((unsigned)rows), dim3(256), lds,
-                         st, rows, N, nf, H, m->act, nfp_of(nf), cfg + r0 * N, L.z1, L.z2,
-                         L.ainv, m->W2, m->Wout, m->bf_scale, (double *)O + r0 * ld, ld);
-    else
-      hipLaunchKernelGGL(nnbf_backward_kernel<float>, dim3((unsigned)rows), dim3(256), lds, st,
-                         rows, N, nf, H, m->act, nfp_of(nf), cfg + r0 * N, L.z1, L.z2, L.ainv,
-                         m->W2, m->Wout, m->bf_scale, (float *)O + r0 * ld, ld);
-    VMCX_LAUNCHED();
+    rc = o_dtype == 0
+             ? backward_rows<double, false>(*m, L, rows, cfg + r0 * N, (double *)O + r0 * ld,
+                                            ld, st)
+             : backward_rows<float, false>(*m, L, rows, cfg + r0 * N, (float *)O + r0 * ld, ld,
+                                           st);
+    if (rc) return rc;
   }
   return 0;
 }
@@ -901,16 +936,13 @@ int nnbf_logderiv_factors(const vmcx_nnbf_model *m, int64_t B, const int64_t *cf
   VMCX_CHECK_ARG(ld >= (int64_t)nD + 2 * N + 4 * H, "ld smaller than the factor width");
   const NnbfWs L = nnbf_layout(*m, B, chunk, 0, (char *)ws);
   VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
-  const size_t lds = (size_t)(nD + 4 * H + 2 * N) * sizeof(float);
-  VMCX_CHECK_ARG(lds <= 65536, "backward LDS exceeds 64 KB");
+  VMCX_CHECK_ARG((4 * H + 2 * N) * 4 <= 65536, "backward LDS exceeds 64 KB");
   for (int64_t r0 = 0; r0 < B; r0 += chunk) {
     const int64_t rows = (B - r0 < chunk) ? B - r0 : chunk;
     int rc = forward_rows(*m, L, rows, cfg + r0 * N, sign + r0, logabs + r0, L.ainv, st);
     if (rc) return rc;
-    hipLaunchKernelGGL((nnbf_backward_kernel<double, true>), dim3((unsigned)rows), dim3(256),
-                       lds, st, rows, N, nf, H, m->act, nfp_of(nf), cfg + r0 * N, L.z1, L.z2,
-                       L.ainv, m->W2, m->Wout, m->bf_scale, F + r0 * ld, ld);
-    VMCX_LAUNCHED();
+    rc = backward_rows<double, true>(*m, L, rows, cfg + r0 * N, F + r0 * ld, ld, st);
+    if (rc) return rc;
   }
   return 0;
 }
