@@ -107,6 +107,8 @@ def parse():
                    help="Gram precision (default: the workload's); factored (NNBF): the "
                         "FP64 Gram from the Grams of the factors of O, no O matrix")
     p.add_argument("--refresh-every", type=int, default=64)
+    p.add_argument("--proposals-per-sample", type=int, default=0,
+                   help="NNBF --sampler learned: proposals per sample (0 = |E| / 4)")
     p.add_argument("--sweeps-per-sample", type=int, default=1,
                    help="Slater: Metropolis sweeps between samples (1 = the reference's "
                         "one sample per sweep)")
@@ -355,7 +357,8 @@ class NnbfRun:
             self.lp = LearnedProposalSamplerGPU(w["Lx"], w["Ly"], channels=32, seed=0x1EA,
                                                 generator=torch.Generator().manual_seed(0),
                                                 device=dev)
-            self.proposals_per_step = max(1, self.e_sweep.shape[0] // 4)
+            self.proposals_per_step = (a.proposals_per_sample if a.proposals_per_sample > 0
+                                       else max(1, self.e_sweep.shape[0] // 4))
             self.n_steps = 0
         self.o_dtype = a.o_dtype or w.get("o_dtype", "fp32")
         o_dtype = torch.float64 if self.o_dtype == "fp64" else torch.float32
