@@ -815,7 +815,7 @@ __global__ void gnorm_kernel(double shift, double *sc) {
 }
 
 struct FactLayout {
-  size_t G, Gp, T, Tc, dinv, vec, sc, info, Y, total;
+  size_t G, Gp, T, Tc, dinv, vec, sc, info, Y, part, total;
   int splits[kFactMax];
 };
 
@@ -853,14 +853,17 @@ static FactLayout fact_layout(int64_t Ns, int n_factors, const int32_t *k, int n
                               const vmcx_sr_block *blocks) {
   FactLayout L{};
   const size_t nn = (size_t)Ns * (size_t)Ns * sizeof(double);
-  int max_s = 1, max_kb = 1;
+  int max_s = 1, max_kb = 1, max_ka = 1;
   for (int f = 0; f < n_factors; ++f) {
     L.splits[f] = gram_splits(Ns, k[f]);
     max_s = std::max(max_s, L.splits[f]);
   }
-  for (int b = 0; b < n_blocks; ++b)
+  for (int b = 0; b < n_blocks; ++b) {
     if (blocks[b].b_factor >= 0 && blocks[b].b_factor < n_factors)
       max_kb = std::max(max_kb, (int)k[blocks[b].b_factor]);
+    if (blocks[b].a_factor >= 0 && blocks[b].a_factor < n_factors)
+      max_ka = std::max(max_ka, (int)k[blocks[b].a_factor]);
+  }
   size_t o = 0;
   L.G = o;    o += align256(nn * n_factors);
   L.Gp = o;   o += align256(nn * (max_s - 1));
@@ -871,6 +874,7 @@ static FactLayout fact_layout(int64_t Ns, int n_factors, const int32_t *k, int n
   L.sc = o;   o += 256;
   L.info = o; o += 256;
   L.Y = o;    o += align256((size_t)Ns * max_kb * sizeof(double));
+  L.part = o; o += align256((size_t)kColsumChunks * max_ka * sizeof(double));
   L.total = o;
   return L;
 }
@@ -907,6 +911,7 @@ int sr_minsr_factored(int64_t Ns, int n_factors, const double *const *F, const i
   double *t1 = vec + 4 * Ns, *t2 = vec + 5 * Ns, *q = vec + 6 * Ns, *res = vec + 7 * Ns;
   double *at = vec + 8 * Ns;
   double *sc = (double *)(w + L.sc), *Y = (double *)(w + L.Y);
+  double *part = (double *)(w + L.part);
   int32_t *iinfo = (int32_t *)(w + L.info);
   const double scale = 1.0 / std::sqrt((double)Ns), one = 1.0, zero = 0.0;
   const int nsi = (int)Ns;
@@ -995,8 +1000,9 @@ int sr_minsr_factored(int64_t Ns, int n_factors, const double *const *F, const i
     const int a = B.a_factor, fb = B.b_factor;
     const double s = B.scale;
     if (fb < 0) {
-      VMCX_BLAS(rocblas_dgemv(h, rocblas_operation_none, (int)k[a], nsi, &s, F[a], (int)ld[a],
-                              at, 1, &zero, dp + B.w_offset, 1));
+      // dp_block = s F_a^T a~: column sums of the row-major factor
+      rc = xt_gemv_cols<double>(Ns, k[a], F[a], ld[a], at, s, part, dp + B.w_offset, st);
+      if (rc) return rc;
       continue;
     }
     // W[i][j] = s sum_n a~_n F_a[n][i] F_b[n][j]: column-major (k_b x k_a) =
@@ -1007,9 +1013,10 @@ int sr_minsr_factored(int64_t Ns, int n_factors, const double *const *F, const i
     VMCX_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, (int)k[fb],
                             (int)k[a], nsi, &s, Y, (int)k[fb], F[a], (int)ld[a], &zero,
                             dp + B.w_offset, (int)k[fb]));
-    if (B.bias_offset >= 0)
-      VMCX_BLAS(rocblas_dgemv(h, rocblas_operation_none, (int)k[a], nsi, &s, F[a], (int)ld[a],
-                              at, 1, &zero, dp + B.bias_offset, 1));
+    if (B.bias_offset >= 0) {
+      rc = xt_gemv_cols<double>(Ns, k[a], F[a], ld[a], at, s, part, dp + B.bias_offset, st);
+      if (rc) return rc;
+    }
   }
   if (stats) {
     hipLaunchKernelGGL(finish_stats, dim3(1), dim3(1), 0, st, sc, stats);
