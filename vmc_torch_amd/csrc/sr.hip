@@ -12,8 +12,9 @@
 // Column means and the energy moments use Neumaier-compensated sums.  The
 // FP64 Gram is the matrix-core SYRK kernel (syrk.hip), the Cholesky and its
 // solves are chol.hip, the GEMV passes over Ot and the fused centring /
-// conversion / shift kernels are below; rocBLAS serves only the
-// reduced-precision Gram (gemm_ex).
+// conversion / shift kernels are below; rocBLAS serves the reduced-precision
+// Gram (gemm_ex) and, in the factored form at the end of this file, the
+// per-block update GEMMs (dgemm) and the symmetric products with T (dsymv).
 #include <hip/hip_bf16.h>
 #include <rocblas/rocblas.h>
 
