@@ -137,7 +137,8 @@ def test_nnbf_factored_sr_step(device):
     assert stats[3].item() == pytest.approx(statsd[3].item(), rel=1e-8)
 
 
-def test_vmc_loop_factored_matches_dense(device):
+@pytest.mark.parametrize("kind", ["nnbf", "nnftns"])
+def test_vmc_loop_factored_matches_dense(device, kind):
     Lx, Ly, nu, nd, hid, B = 4, 2, 3, 3, 16, 256
     N, nf = Lx * Ly, nu + nd
     H = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, nf,
@@ -150,7 +151,10 @@ def test_vmc_loop_factored_matches_dense(device):
         FO, aux = model.log_derivative_factored(fxs)
         return FO.materialize(), aux
     for gram in ("fp64", "factored"):
-        model = _model(Lx, Ly, nu, nd, hid, "fp32", device, seed=9)
+        if kind == "nnbf":
+            model = _model(Lx, Ly, nu, nd, hid, "fp32", device, seed=9)
+        else:
+            model = _peps(Lx, Ly, 2, hid, 9, device)
         sampler = MetropolisExchangeSpinfulSamplerGPU(seed=11, refresh_every=64)
         vmc = VMC_GPU(sampler, MinSRGPU(gram), SGDGPU(0.05),
                       compute_grads_fn=dense_of_factors)
