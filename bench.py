@@ -107,6 +107,8 @@ def parse():
                    help="Gram precision (default: the workload's); factored (NNBF): the "
                         "FP64 Gram from the Grams of the factors of O, no O matrix")
     p.add_argument("--refresh-every", type=int, default=64)
+    p.add_argument("--chunk", type=int, default=65536,
+                   help="NNBF: amplitudes per batched forward in the local energy")
     p.add_argument("--proposals-per-sample", type=int, default=0,
                    help="NNBF --sampler learned: proposals per sample (0 = |E| / 4)")
     p.add_argument("--sweeps-per-sample", type=int, default=1,
@@ -346,7 +348,8 @@ class NnbfRun:
         self.mlp = a.mlp or w["mlp"]
         self.model = NNBF_GPU(N, nf, hidden_dim=w["hidden"], activation="relu",
                               bf_scale=w["bf_scale"], gemm=self.mlp, init="xavier", m_std=0.1,
-                              generator=torch.Generator().manual_seed(0)).to(dev)
+                              generator=torch.Generator().manual_seed(0),
+                              chunk=a.chunk).to(dev)
         self.Np = self.model.n_params
         self.e_sweep = H.graph.edge_tensor(dev, "sweep")
         self.e_graph = H.graph.edge_tensor(dev, "graph")
