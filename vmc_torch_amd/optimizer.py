@@ -117,10 +117,13 @@ class DistributedSRMinresGPU(PreconditionerGPU):
 class MinSRGPU(PreconditionerGPU):
     """Sample-space SR (GPU/optimizer.py:471-614, MinSRGPU at :873-895).
 
-    gram: "fp64" (plain form), "fp32" or "bf16" (Gram operands in reduced
-    precision with FP32 accumulation; the FP64 Cholesky factor of that Gram
-    then preconditions conjugate gradients on the exact operator, at most
-    `max_iter` iterations to relative residual `tol`).
+    gram: "fp64" (plain form), "fp32", "bf16" or "bf16x3" (Gram operands in
+    reduced precision with FP32 accumulation; the FP64 Cholesky factor of
+    that Gram then preconditions conjugate gradients on the exact operator,
+    at most `max_iter` iterations to relative residual `tol`), or "factored"
+    (the same FP64 step as "fp64" for models made of dense layers: local_o
+    is a kernels.FactoredO from model.log_derivative_factored and the Gram
+    is formed from the Grams of its factors, no Ns x Np matrix).
     inplace: centre O in place (O is overwritten; saves an Ns x Np copy).
     Returns (dp device tensor, seconds, info) with info = 0 on success, or
     the Cholesky failure index (the reference returns 1 after an lstsq
