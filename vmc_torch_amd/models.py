@@ -284,7 +284,6 @@ class NNBF_GPU(WavefunctionModel_GPU):
             kernels.stream_ptr(fxs.device)))
         return O, (sign, logabs)
 
-
     def log_derivative_factored(self, fxs, chunk=4096, out=None):
         """The per-sample log-derivative as factors (kernels.FactoredO): row s
         holds [gM | n | dz1 | h1 | dz2 | h2] (2N nf + 2N + 4 hidden numbers in
@@ -457,7 +456,6 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
             chunk, kernels.ptr(O), code, O.stride(0), kernels.ptr(sign), kernels.ptr(logabs),
             kernels.stream_ptr(fxs.device)))
         return O, (sign, logabs)
-
 
     def log_derivative_factored(self, fxs, chunk=4096, out=None):
         """The per-sample log-derivative as factors (kernels.FactoredO): with
