@@ -203,6 +203,29 @@ int vmcx_sr_minsr_factored(int64_t Ns, int n_factors, const double *const *F,
                            float *phase_ms, void *stream);
 
 /*
+ * The energy gradient on a factored O (the blocks of vmcx_sr_minsr_factored):
+ *   g = (1/Ns) sum_s (E_s - energy_mean) O_s  (energy_mean NaN = sample mean),
+ * without the Gram or the factorisation.  workspace: the same size as
+ * vmcx_sr_minsr_factored's.  stats (may be NULL): [5] = mean E, var E, 0,
+ * ||g||, 0.
+ * Replaces: the run_SR=False branch of minSR_solver_gpu
+ *   (GPU/optimizer.py:581-614) for models built from dense layers.
+ */
+int vmcx_sr_factored_gradient(int64_t Ns, int n_factors, const double *const *F,
+                              const int64_t *ld, const int32_t *k, int n_blocks,
+                              const vmcx_sr_block *blocks, int64_t Np, const double *E,
+                              double energy_mean, void *workspace, int64_t ws_bytes, double *g,
+                              double *stats, void *stream);
+
+/*
+ * out[0] = mean of E[0..n), out[1] = its variance (1/n normalisation), both
+ * from compensated (Neumaier) sums, the variance as the mean of (E - mean)^2.
+ * Replaces: VMC_GPU.compute_global_energy_stats (GPU/VMC.py:513-534) on one
+ *   device, whose sums are naive.
+ */
+int vmcx_energy_stats(int64_t n, const double *E, double *out, void *stream);
+
+/*
  * FP64 symmetric rank-K update on the matrix cores (a building block of the
  * minSR Gram and of the blocked Cholesky, exported for direct use):
  *   C = beta C + alpha M M^T on the lower triangle of the n x n column-major

@@ -287,17 +287,32 @@ def mlp_forward(nn, xs, act="relu"):
     return h2 @ Wo.T + bo, (n, z1, h1, z2, h2)
 
 
-def nnftn_amplitude(shape, theta, nn, eta, x, act="relu"):
+def theta_eff(theta, out, eta, mode="add"):
+    """The configuration-dependent tensor entries.  mode "add": theta +
+    eta out, the reference's additive backflow (GPU/models/NNfTNS.py:390-437);
+    mode "mul": theta * (1 + eta out), the multiplicative site-tensor
+    correction of the NN-fTNS benchmark configuration (no reference form)."""
+    return theta + eta * out if mode == "add" else theta * (1.0 + eta * out)
+
+
+def nnftn_amplitude(shape, theta, nn, eta, x, act="relu", mode="add"):
     out, _ = mlp_forward(nn, x, act)
-    return amplitude(shape, theta + eta * out[0], x)
+    return amplitude(shape, theta_eff(theta, out[0], eta, mode), x)
 
 
-def nnftn_log_derivative(shape, theta, nn, eta, x, act="relu"):
-    """[d/d theta, d/d (W1, b1, W2, b2, Wo, bo)] of ln|psi| for one config."""
+def nnftn_log_derivative(shape, theta, nn, eta, x, act="relu", mode="add"):
+    """[d/d theta, d/d (W1, b1, W2, b2, Wo, bo)] of ln|psi| for one config.
+    With g = d ln|psi| / d theta_eff: mode "add": d/d theta = g and
+    d/d out = eta g; mode "mul": d/d theta = g (1 + eta out) and
+    d/d out = eta g theta."""
     out, (n, z1, h1, z2, h2) = mlp_forward(nn, x, act)
-    g = log_derivative(shape, theta + eta * out[0], x)      # d ln|psi| / d theta_eff
+    g = log_derivative(shape, theta_eff(theta, out[0], eta, mode), x)
     W1, b1, W2, b2, Wo, bo = nn
-    dout = eta * g
+    if mode == "add":
+        dout = eta * g
+    else:
+        dout = eta * g * theta
+        g = g * (1.0 + eta * out[0])
     dWo = np.outer(dout, h2[0])
     dh2 = Wo.T @ dout
     dz2 = dh2 * ((z2[0] > 0) if act == "relu" else (1 - np.tanh(z2[0]) ** 2))
@@ -308,14 +323,14 @@ def nnftn_log_derivative(shape, theta, nn, eta, x, act="relu"):
     return np.concatenate([g, dW1.ravel(), dz1, dW2.ravel(), dz2, dWo.ravel(), dout])
 
 
-def local_energy(H, shape, theta, xs, nn=None, eta=0.0, act="relu"):
+def local_energy(H, shape, theta, xs, nn=None, eta=0.0, act="relu", mode="add"):
     """E_loc via the term-major get_conn and full amplitudes."""
     xs = np.atleast_2d(xs)
 
     def amp(c):
         if nn is None:
             return amplitude(shape, theta, c)
-        return nnftn_amplitude(shape, theta, nn, eta, c, act)
+        return nnftn_amplitude(shape, theta, nn, eta, c, act, mode)
     psi = np.array([amp(c) for c in xs])
     etas, coeffs, bids = H.get_conn_batch(xs)
     E = np.zeros(xs.shape[0])

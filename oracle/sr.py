@@ -94,3 +94,19 @@ def sr_minres(O, E, energy_mean, diag_shift, rtol=1e-4, maxiter=100):
 def sgd(params, dp, lr):
     """p <- p - lr dp (GPU/optimizer.py:178-186; optimizer.py:339-340)."""
     return params - lr * dp
+
+
+def outlier_mask(E, O, clip_factor):
+    """GPU/VMC.py:476-498: drop the samples whose |O_s| exceeds clip_factor
+    times the median norm; torch's median of an even count is the lower of
+    the two middle values.  Dropped rows of O become 0, their energies the
+    mean of the kept ones.  Returns (E, O, keep mask)."""
+    E = np.array(E, dtype=np.float64)
+    O = np.array(O, dtype=np.float64)
+    norms = np.sqrt((O * O).sum(axis=1))
+    med = np.sort(norms)[(norms.size - 1) // 2]
+    keep = norms <= clip_factor * med
+    if not keep.all():
+        E[~keep] = E[keep].mean()
+        O[~keep] = 0.0
+    return E, O, keep

@@ -120,6 +120,47 @@ def test_cpu_checkpoint_refuses_other_objects(tmp_path):
     assert CALLED == []
 
 
+def _tensor_record_file(path, n_storage, offset, size, stride):
+    """A torch-format zip whose data.pkl (written opcode by opcode) holds one
+    _rebuild_tensor_v2 record over an FP64 storage of n_storage elements."""
+    import struct
+    import zipfile
+
+    def uni(t):
+        b = t.encode()
+        return b"X" + struct.pack("<I", len(b)) + b
+
+    def i32(v):
+        return b"J" + struct.pack("<i", v)
+
+    def tup(vals):
+        return b"(" + b"".join(i32(v) for v in vals) + b"t"
+    pkl = (b"\x80\x02ctorch._utils\n_rebuild_tensor_v2\n("
+           + b"(" + uni("storage") + b"ctorch\nDoubleStorage\n" + uni("0") + uni("cpu")
+           + i32(n_storage) + b"tQ"
+           + i32(offset) + tup(size) + tup(stride) + b"\x89" + b"}" + b"tR.")
+    with zipfile.ZipFile(path, "w") as z:
+        z.writestr("archive/data.pkl", pkl)
+        z.writestr("archive/data/0", np.arange(n_storage, dtype=np.float64).tobytes())
+    return str(path)
+
+
+def test_cpu_checkpoint_tensor_record_bounds(tmp_path):
+    """Tensor records are checked against their storage before any view is
+    built: a record past the end, a negative offset or stride, or a stride
+    count that does not match the sizes is refused."""
+    ok = _tensor_record_file(tmp_path / "ok.pth", 4, 2, (2,), (1,))
+    assert np.array_equal(ck.read_cpu_checkpoint(ok), [2.0, 3.0])
+    ok2 = _tensor_record_file(tmp_path / "ok2.pth", 6, 1, (2, 2), (3, 1))
+    assert np.array_equal(ck.read_cpu_checkpoint(ok2), [[1.0, 2.0], [4.0, 5.0]])
+    for i, (off, size, stride) in enumerate([(2, (8,), (1,)), (-1, (2,), (1,)),
+                                             (0, (2,), (-1,)), (0, (2, 2), (3,)),
+                                             (4, (), ()), (0, (3,), (2,))]):
+        bad = _tensor_record_file(tmp_path / f"bad{i}.pth", 4, off, size, stride)
+        with pytest.raises(ValueError, match="tensor record"):
+            ck.read_cpu_checkpoint(bad)
+
+
 @pytest.mark.skipif(not os.path.isdir(REF), reason="reference checkout not present")
 def test_reference_cpu_checkpoints():
     gold = json.load(open(GOLD))

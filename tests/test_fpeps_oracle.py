@@ -116,6 +116,57 @@ def test_nnftn_eta_zero_and_gradient():
             assert g[k] == pytest.approx(fd, rel=1e-5, abs=1e-8)
 
 
+@pytest.mark.parametrize("mode", ["add", "mul"])
+def test_nnftn_constant_output_reduces_to_fpeps(mode):
+    """With Wo = 0 the network output is the constant bo, so NN-fTNS is the
+    plain fPEPS at theta + eta bo ("add") or theta * (1 + eta bo) ("mul");
+    with W1 = 0 as well the output is Wo act(W2 act(b1) + b2) + bo, fixed by
+    the layer order.  Each amplitude is compared with the plain contraction
+    (pinned against the Grassmann definition) at every basis state of 2x2."""
+    rng = np.random.default_rng(12)
+    sh = fp.PEPSShape(2, 2, 2)
+    th = fp.random_theta(sh, rng)
+    W1, b1, W2, b2, Wo, bo = fp.mlp_init(rng, 2 * sh.N, 6, sh.n_params, 0.3)
+    b1, b2, bo = rng.normal(size=6), rng.normal(size=6), rng.normal(size=sh.n_params)
+    eta = 0.4
+    cases = [([W1, b1, W2, b2, np.zeros_like(Wo), bo], bo),
+             ([np.zeros_like(W1), b1, W2, b2, Wo, bo],
+              Wo @ np.maximum(W2 @ np.maximum(b1, 0.0) + b2, 0.0) + bo)]
+    xs = [np.array(c) for c in np.ndindex(4, 4, 4, 4)]
+    for nn, out in cases:
+        eff = th + eta * out if mode == "add" else th * (1.0 + eta * out)
+        for x in xs:
+            assert fp.nnftn_amplitude(sh, th, nn, eta, x, "relu", mode) == pytest.approx(
+                fp.amplitude(sh, eff, x), rel=1e-12, abs=1e-15)
+
+
+def test_nnftn_multiplicative_gradient():
+    """The multiplicative form's log-derivative against finite differences of
+    its amplitude, for both activations."""
+    rng = np.random.default_rng(8)
+    sh = fp.PEPSShape(2, 3, 2)
+    th = fp.random_theta(sh, rng)
+    nn = fp.mlp_init(rng, 2 * sh.N, 8, sh.n_params, 0.3)
+    nn[1], nn[3], nn[5] = rng.normal(size=8), rng.normal(size=8), rng.normal(size=sh.n_params)
+    x = np.array([1, 2, 0, 3, 2, 1])
+    for act in ("relu", "tanh"):
+        g = fp.nnftn_log_derivative(sh, th, nn, 0.7, x, act, "mul")
+        flat = np.concatenate([th] + [p.ravel() for p in nn])
+        sizes = [th.size] + [p.size for p in nn]
+
+        def amp(v):
+            parts = np.split(v, np.cumsum(sizes)[:-1])
+            nnp = [p.reshape(q.shape) for p, q in zip(parts[1:], nn)]
+            return fp.nnftn_amplitude(sh, parts[0], nnp, 0.7, x, act, "mul")
+        psi = amp(flat)
+        h = 1e-6
+        for k in rng.choice(flat.size, 40, replace=False):
+            e = np.zeros(flat.size)
+            e[k] = h
+            fd = (amp(flat + e) - amp(flat - e)) / (2 * h) / psi
+            assert g[k] == pytest.approx(fd, rel=1e-5, abs=1e-8)
+
+
 @pytest.mark.parametrize("Lx,Ly,D", [(4, 4, 4), (2, 5, 3), (3, 2, 1)])
 def test_device_layout_matches_oracle(Lx, Ly, D):
     dmap, sizes = kernels.peps_layout(Lx, Ly, D)

@@ -292,6 +292,34 @@ def test_nnbf_zero_scale_is_slater():
     assert np.array_equal(s, s0) and np.allclose(l, l0, atol=1e-13)
 
 
+def test_nnbf_constant_output_reduces_to_slater():
+    """With W_out = 0 the backflow is the constant b_out, so psi is the Slater
+    determinant of M + s reshape(b_out, (2N, n_f)) (row-major); with W_1 = 0
+    as well, h_1 = act(b_1) and the backflow is W_out act(W_2 act(b_1) + b_2)
+    + b_out.  Both are compared with slater.slogdet at every basis state:
+    a transposed reshape, a dropped bias or swapped layers fail them."""
+    from oracle import nnbf
+    nnbf_mod, H, params, scale = _nnbf_small()
+    M, W1, b1, W2, b2, Wo, bo = params
+    x = H.all_states()
+    n_orb, nf = M.shape
+    rng = np.random.default_rng(4)
+    for act in ("relu", "tanh"):
+        f = (lambda z: np.maximum(z, 0.0)) if act == "relu" else np.tanh
+        for p, D in (([M, W1, b1, W2, b2, np.zeros_like(Wo), bo], bo),
+                     ([M, np.zeros_like(W1), b1, W2, b2, Wo, bo],
+                      Wo @ f(W2 @ f(b1) + b2) + bo)):
+            s, l, _ = nnbf.forward(p, x, 2, scale, act)
+            s0, l0 = slater.slogdet(M + scale * D.reshape(n_orb, nf), x)
+            assert np.array_equal(s, s0) and np.allclose(l, l0, rtol=1e-13, atol=1e-13)
+        # a transposed reshape of the constant backflow is caught
+        C = rng.normal(size=n_orb * nf)
+        p = [M, W1, b1, W2, b2, np.zeros_like(Wo), C]
+        _, l, _ = nnbf.forward(p, x, 2, 1.0, act)
+        _, lt = slater.slogdet(M + C.reshape(nf, n_orb).T, x)
+        assert not np.allclose(l, lt, atol=1e-6)
+
+
 def test_nnbf_log_derivative_finite_difference():
     nnbf, H, params, scale = _nnbf_small()
     x = H.all_states()[[1, 9, 30]]

@@ -137,8 +137,9 @@ def test_nnbf_factored_sr_step(device):
     assert stats[3].item() == pytest.approx(statsd[3].item(), rel=1e-8)
 
 
-@pytest.mark.parametrize("kind", ["nnbf", "nnftns"])
-def test_vmc_loop_factored_matches_dense(device, kind):
+@pytest.mark.parametrize("kind,clip", [("nnbf", 0.0), ("nnftns", 0.0), ("nnbf", 1.3),
+                                       ("nnftns", 1.3)])
+def test_vmc_loop_factored_matches_dense(device, kind, clip):
     Lx, Ly, nu, nd, hid, B = 4, 2, 3, 3, 16, 256
     N, nf = Lx * Ly, nu + nd
     H = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, nf,
@@ -160,16 +161,22 @@ def test_vmc_loop_factored_matches_dense(device, kind):
                       compute_grads_fn=dense_of_factors)
         cfg = VMCLoopConfig(vmc_steps=3, ns_per_rank=B, n_params=model.n_params, nsites=N,
                             learning_rate=0.05, diag_shift=1e-3, burn_in_steps=4,
-                            outlier_clip_factor=0.0, use_log_amp=True)
+                            outlier_clip_factor=clip, use_log_amp=True)
         log = []
         vmc.run_vmc_loop(x0.clone(), model, H, H.graph, config=cfg,
                          on_step_end=lambda d: log.append((d["fxs"].cpu().numpy().copy(),
-                                                           d["energy_mean"])))
+                                                           d["energy_mean"],
+                                                           d["n_outliers"])))
         runs.append((log, [p.detach().cpu().numpy().copy() for p in model.params]))
     (ld, pd), (lf, pf) = runs
     for k in range(3):
         assert np.array_equal(ld[k][0], lf[k][0]), f"step {k}: configurations differ"
         assert lf[k][1] == pytest.approx(ld[k][1], rel=1e-9)
+        # the outlier mask drops the same samples from the dense and the
+        # factored O (GPU/VMC.py:476-498)
+        assert lf[k][2] == ld[k][2]
+        if clip > 0:
+            assert lf[k][2] > 0
     for a, b in zip(pf, pd):
         assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
 

@@ -3,6 +3,7 @@ GPU: sampler.step -> evaluate_energy_fn -> compute_grads_fn ->
 preconditioner.solve -> optimizer.step, with the walker batch on the device.
 """
 import time
+import warnings
 from dataclasses import dataclass, fields
 from typing import Any, Callable, Dict, Optional
 
@@ -36,6 +37,74 @@ class VMCLoopConfig:
         for k, v in kwargs.items():
             setattr(self, k, v)
 
+    @classmethod
+    def from_vmc_config(cls, cfg, *, n_params, nsites):
+        """GPU/VMC.py:120-148: every field of a run-script config dataclass
+        (plus its extra attributes), with resume_step -> step_offset and
+        sr_diag_shift -> diag_shift; n_params and nsites are given."""
+        import dataclasses
+        names = {f.name for f in dataclasses.fields(cfg)}
+        kwargs = {name: getattr(cfg, name) for name in names}
+        for k, v in vars(cfg).items():
+            if k not in names and k not in kwargs:
+                kwargs[k] = v
+        if hasattr(cfg, "resume_step"):
+            kwargs["step_offset"] = cfg.resume_step
+        if "sr_diag_shift" in kwargs:
+            kwargs["diag_shift"] = kwargs.pop("sr_diag_shift")
+        kwargs["n_params"] = n_params
+        kwargs["nsites"] = nsites
+        return cls(**kwargs)
+
+
+def factored_row_norms(F, factors, blocks):
+    """|O_s| of a factored O (kernels.FactoredO layout) from its factors: a
+    plain block contributes scale^2 |F_a,s|^2, an outer-product block
+    scale^2 |F_a,s|^2 |F_b,s|^2 and its bias block scale^2 |F_a,s|^2."""
+    sq = [(F[:, c:c + k] ** 2).sum(dim=1) for c, k in factors]
+    n2 = torch.zeros(F.shape[0], dtype=F.dtype, device=F.device)
+    for a, b, sc, _wo, bo in blocks:
+        term = sq[a] if b < 0 else sq[a] * (sq[b] + (1.0 if bo >= 0 else 0.0))
+        n2 += (sc * sc) * term
+    return n2.sqrt()
+
+
+def outlier_mask(E, O, clip_factor):
+    """GPU/VMC.py:476-498: keep the samples whose per-sample log-derivative
+    norm is at most clip_factor times the median norm (torch's median: the
+    lower middle value); the dropped rows of O become 0 and their energies
+    the mean of the kept ones.  O is a dense (Ns, Np) tensor or a
+    kernels.FactoredO, whose row norms come from its factors (an
+    outer-product block contributes |a|^2 |b|^2, its bias block |a|^2) and
+    whose dropped rows are zeroed in every factor.  Returns (E, O, n_dropped);
+    E and O are copies when rows are dropped, the inputs otherwise."""
+    if clip_factor <= 0:
+        return E, O, 0
+    if isinstance(O, kernels.FactoredO):
+        norms = factored_row_norms(O.F, O.factors, O.blocks)
+    else:
+        norms = O.norm(dim=1)
+    mask = norms <= clip_factor * norms.median()
+    n_drop = int((~mask).sum())
+    if n_drop == 0:
+        return E, O, 0
+    E = E.clone()
+    E[~mask] = E[mask].mean()
+    if isinstance(O, kernels.FactoredO):
+        F = O.F.clone()
+        F[~mask] = 0.0
+        O = kernels.FactoredO(F, O.factors, O.blocks, O.n_params)
+    else:
+        O = O.clone()
+        O[~mask] = 0.0
+    return E, O, n_drop
+
+
+def _all_finite(*tensors):
+    """One device-side reduction per tensor, one host read in total."""
+    flags = [torch.isfinite(t).all() for t in tensors if t is not None and t.numel()]
+    return bool(torch.stack(flags).all()) if flags else True
+
 
 class VMC_GPU:
     """GPU/VMC.py:151-172: dependency-injected driver."""
@@ -48,6 +117,7 @@ class VMC_GPU:
         self.optimizer = optimizer
         self.evaluate_energy_fn = evaluate_energy_fn
         self.compute_grads_fn = compute_grads_fn
+        self.last_n_dropped = 0
 
     def _run_sampling_phase(self, fxs, model, hamiltonian, graph, ns_per_rank,
                             grad_batch_size=0, burn_in=False, burn_in_steps=0,
@@ -87,32 +157,23 @@ class VMC_GPU:
         E = torch.cat(E_list) if len(E_list) > 1 else E_list[0]
         if factored:
             O = kernels.FactoredO.cat(O_list) if len(O_list) > 1 else O_list[0]
-            if outlier_clip_factor > 0:
-                raise NotImplementedError("outlier clipping needs the dense O (gram != 'factored')")
+            if not _all_finite(O.F):
+                # GPU/vmc_utils.py:1330-1386 (_check_grads_amps)
+                raise ValueError("NaN/Inf in gradients (factors of the log-derivative)")
         else:
             O = torch.cat(O_list) if len(O_list) > 1 else O_list[0]
-        if outlier_clip_factor > 0 and not factored:
-            # GPU/VMC.py:476-498
-            norms = O.norm(dim=1)
-            mask = norms <= outlier_clip_factor * norms.median()
-            if not bool(mask.all()):
-                O = O.clone()
-                E = E.clone()
-                O[~mask] = 0.0
-                E[~mask] = E[mask].mean()
+        E, O, self.last_n_dropped = outlier_mask(E, O, outlier_clip_factor)
         times = {"t_samp": t_samp, "t_locE": t_locE, "t_grad": t_grad}
         return (E, O), fxs, t_samp + t_locE + t_grad, times
 
     def compute_global_energy_stats(self, local_energies, world_size=1):
-        """GPU/VMC.py:513-534 (single device)."""
-        n = local_energies.numel() * world_size
-        s = local_energies.sum()
-        s2 = (local_energies * local_energies).sum()
-        if world_size > 1 and torch.distributed.is_initialized():
-            torch.distributed.all_reduce(s)
-            torch.distributed.all_reduce(s2)
-        mean = float(s) / n
-        return n, mean, float(s2) / n - mean * mean
+        """GPU/VMC.py:513-534 on one device: (n, mean, variance), the sums
+        compensated (vmcx_energy_stats) where the reference's are naive."""
+        if world_size != 1:
+            raise NotImplementedError("one GPU: the walker batch stands in for the ranks")
+        E = local_energies.to(torch.float64).contiguous()
+        st = kernels.energy_stats(E).cpu()
+        return E.numel(), float(st[0]), float(st[1])
 
     def solve_sr_step(self, local_o, local_energies, energy_mean, total_samples, n_params,
                       diag_shift, device, run_sr):
@@ -137,12 +198,25 @@ class VMC_GPU:
                 burn_in=(step == 0), burn_in_steps=config.burn_in_steps,
                 outlier_clip_factor=config.outlier_clip_factor, use_log_amp=config.use_log_amp)
             n, emean, evar = self.compute_global_energy_stats(E, world_size)
+            if emean != emean:
+                # GPU/VMC.py:705-723 reports NaN inputs of the SR step
+                warnings.warn(f"[WARNING] SR inputs: local_E has NaN at step "
+                              f"{step + config.step_offset}")
             dp, t_sr, info = self.solve_sr_step(O, E, emean, n, config.n_params,
                                                 config.diag_shift, device, config.run_sr)
             lr = config.learning_rate
             if config.lr_scheduler is not None:
                 lr = config.learning_rate = config.lr_scheduler(step + config.step_offset)
-            self.apply_parameter_update(model, dp, lr, device)
+            dp_t = torch.as_tensor(dp, device=device)
+            skipped = not _all_finite(dp_t)
+            if skipped:
+                # GPU/VMC.py:757-769 warns about a NaN/Inf dp (and the CPU driver,
+                # VMC.py:173-175, raises); here the update is skipped so the
+                # parameters stay finite
+                warnings.warn(f"[WARNING] SR dp has NaN/Inf at step {step + config.step_offset}:"
+                              " parameter update skipped")
+            else:
+                self.apply_parameter_update(model, dp, lr, device)
             e_site = emean / config.nsites
             err = (max(evar, 0.0) / n) ** 0.5 / config.nsites
             history.append(e_site)
@@ -152,5 +226,7 @@ class VMC_GPU:
                              "error_per_site": err, "total_samples": n,
                              "sample_time": sample_time, "phase_times": times,
                              "sr_time": t_sr, "total_time": time.time() - t0,
-                             "solver_info": info, "fxs": fxs, "local_energies": E})
+                             "solver_info": info, "update_skipped": skipped,
+                             "n_outliers": self.last_n_dropped, "fxs": fxs,
+                             "local_energies": E})
         return history, fxs

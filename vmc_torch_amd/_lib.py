@@ -40,6 +40,9 @@ _SIGNATURES = {
     "vmcx_sr_factored_workspace_bytes": (_I64, [_I64, _I, _P, _I, _P]),
     "vmcx_sr_minsr_factored": (_I, [_I64, _I, _P, _P, _P, _I, _P, _I64, _P, _D, _D, _P, _I64,
                                     _P, _P, _P, _P, _P]),
+    "vmcx_sr_factored_gradient": (_I, [_I64, _I, _P, _P, _P, _I, _P, _I64, _P, _D, _P, _I64,
+                                       _P, _P, _P]),
+    "vmcx_energy_stats": (_I, [_I64, _P, _P, _P]),
     "vmcx_sr_minres_workspace_bytes": (_I64, [_I64, _I64]),
     "vmcx_sr_minres": (_I, [_I64, _I64, _P, _I, _I64, _P, _D, _D, _D, _I, _P, _I64, _P, _P,
                             _P, _P]),

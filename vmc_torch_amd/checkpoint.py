@@ -146,6 +146,28 @@ def _finish(v):
     return v
 
 
+def _check_tensor_record(n, offset, size, stride):
+    """A tensor record must address only its storage's n elements: offset,
+    sizes and strides non-negative integers, one stride per dimension, and
+    (for a non-empty tensor) offset + sum((size_i - 1) stride_i) < n."""
+    ints = all(isinstance(v, int) and not isinstance(v, bool)
+               for v in (offset, *size, *stride))
+    if not ints or len(size) != len(stride):
+        raise ValueError("checkpoint tensor record has a malformed offset, size or stride")
+    if offset < 0 or any(d < 0 for d in size) or any(st < 0 for st in stride):
+        raise ValueError("checkpoint tensor record has a negative offset, size or stride")
+    if not size:
+        if offset >= n:
+            raise ValueError("checkpoint tensor record lies outside its storage")
+        return
+    if all(d > 0 for d in size):
+        last = offset + sum((d - 1) * st for d, st in zip(size, stride))
+        if last >= n:
+            raise ValueError("checkpoint tensor record lies outside its storage")
+    elif offset > n:
+        raise ValueError("checkpoint tensor record lies outside its storage")
+
+
 def _call(fn, args, storages):
     args = _finish(args)
     k = fn.key
@@ -156,6 +178,7 @@ def _call(fn, args, storages):
     if k == "torch._utils._rebuild_tensor_v2":
         storage, offset, size, stride = args[0], args[1], tuple(args[2]), tuple(args[3])
         flat = storage
+        _check_tensor_record(len(flat), offset, size, stride)
         if not size:
             return flat[offset:offset + 1].reshape(()).copy()
         isz = flat.dtype.itemsize

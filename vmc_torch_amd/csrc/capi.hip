@@ -37,7 +37,7 @@ static inline int status_of(int rc) {
 
 extern "C" {
 
-int vmcx_abi_version(void) { return 11; }
+int vmcx_abi_version(void) { return 12; }
 
 const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
 
@@ -215,6 +215,21 @@ int vmcx_sr_minsr_factored(int64_t Ns, int n_factors, const double *const *F,
                                          energy_mean, diag_shift, workspace, ws_bytes, dp,
                                          stats, info, phase_ms, (hipStream_t)stream);
   return rc == 3 ? VMCX_ELIB : status_of(rc);
+}
+
+int vmcx_sr_factored_gradient(int64_t Ns, int n_factors, const double *const *F,
+                              const int64_t *ld, const int32_t *k, int n_blocks,
+                              const vmcx_sr_block *blocks, int64_t Np, const double *E,
+                              double energy_mean, void *workspace, int64_t ws_bytes, double *g,
+                              double *stats, void *stream) {
+  const int rc = vmcx::sr_factored_gradient(Ns, n_factors, F, ld, k, n_blocks, blocks, Np, E,
+                                            energy_mean, workspace, ws_bytes, g, stats,
+                                            (hipStream_t)stream);
+  return rc == 3 ? VMCX_ELIB : status_of(rc);
+}
+
+int vmcx_energy_stats(int64_t n, const double *E, double *out, void *stream) {
+  return status_of(vmcx::energy_stats(n, E, out, (hipStream_t)stream));
 }
 
 int vmcx_to_bf16(int64_t n, const float *x, void *y, void *stream) {

@@ -19,6 +19,7 @@
 #include <rocblas/rocblas.h>
 
 #include <mutex>
+#include <map>
 #include <unordered_map>
 #include <vector>
 
@@ -597,25 +598,27 @@ const double *chol_panels(int64_t n, const double *dinv) {
     }                                                                        \
   } while (0)
 
-// Side stream, rocBLAS handle and event pool for the look-ahead (per device).
+// Side streams and event pool for the look-ahead, one set per (device,
+// caller stream): two callers on different streams never share a side
+// stream or an event, so their factorisations cannot interleave.
 struct LookAhead {
   hipStream_t side = nullptr;  // trailing SYRKs
   hipStream_t aux = nullptr;   // forward solve along the factorisation
   std::vector<hipEvent_t> ev;
 };
 
-static LookAhead *lookahead_for_device(int nev) {
+static LookAhead *lookahead_for(hipStream_t caller, int nev) {
   static std::mutex mu;
-  static std::unordered_map<int, LookAhead *> per_dev;
+  static std::map<std::pair<int, hipStream_t>, LookAhead *> per_stream;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   std::lock_guard<std::mutex> lk(mu);
-  LookAhead *la = per_dev[dev];
+  LookAhead *&la = per_stream[std::make_pair(dev, caller)];
   if (la == nullptr) {
-    la = new LookAhead();
-    if (hipStreamCreateWithFlags(&la->side, hipStreamNonBlocking) != hipSuccess) return nullptr;
-    if (hipStreamCreateWithFlags(&la->aux, hipStreamNonBlocking) != hipSuccess) return nullptr;
-    per_dev[dev] = la;
+    LookAhead *n = new LookAhead();
+    if (hipStreamCreateWithFlags(&n->side, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    if (hipStreamCreateWithFlags(&n->aux, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    la = n;
   }
   while ((int)la->ev.size() < nev) {
     hipEvent_t e;
@@ -655,7 +658,7 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
   double *Dinv = dinv, *DinvT = dinv + nblk * CHOL_NB * CHOL_NB;
   double *W = dinv + 2 * nblk * CHOL_NB * CHOL_NB;
   const int64_t ldw = n;
-  LookAhead *la = lookahead_for_device((int)(3 * nblk + 3));
+  LookAhead *la = lookahead_for(st, (int)(3 * nblk + 3));
   hipEvent_t *evf = la ? la->ev.data() + 2 * nblk + 2 : nullptr;  // after diag_k
   VMCX_CHECK_ARG(la != nullptr, "look-ahead stream / handle creation failed");
   hipLaunchKernelGGL(zero_i32, dim3(1), dim3(1), 0, st, info);

@@ -186,4 +186,9 @@ int sr_minsr_factored(int64_t Ns, int n_factors, const double *const *F, const i
                       const double *E, double energy_mean, double shift, void *ws,
                       int64_t ws_bytes, double *dp, double *stats, int32_t *info,
                       float *phase_ms, hipStream_t st);
+int sr_factored_gradient(int64_t Ns, int n_factors, const double *const *F, const int64_t *ld,
+                         const int32_t *k, int n_blocks, const vmcx_sr_block *blocks, int64_t Np,
+                         const double *E, double energy_mean, void *ws, int64_t ws_bytes,
+                         double *g, double *stats, hipStream_t st);
+int energy_stats(int64_t n, const double *E, double *out, hipStream_t st);
 }  // namespace vmcx

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <mutex>
+#include <map>
 #include <unordered_map>
 #include <vector>
 
@@ -280,22 +281,23 @@ __global__ void __launch_bounds__(1024)
 }
 
 // --------------------------------------------------------------- handles
+// One rocBLAS handle per (device, caller stream): a handle is bound to the
+// stream it was created for, so callers on different streams never share
+// one (and never see another caller's stream or pointer mode).
 rocblas_handle blas_handle_for(hipStream_t st) {
   static std::mutex mu;
-  static std::unordered_map<int, rocblas_handle> handles;
+  static std::map<std::pair<int, hipStream_t>, rocblas_handle> handles;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   std::lock_guard<std::mutex> lk(mu);
-  auto it = handles.find(dev);
+  const auto key = std::make_pair(dev, st);
+  auto it = handles.find(key);
+  if (it != handles.end()) return it->second;
   rocblas_handle h;
-  if (it == handles.end()) {
-    if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
-    handles[dev] = h;
-  } else {
-    h = it->second;
-  }
+  if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
   rocblas_set_stream(h, st);
   rocblas_set_pointer_mode(h, rocblas_pointer_mode_host);
+  handles[key] = h;
   return h;
 }
 
@@ -888,6 +890,42 @@ int64_t sr_factored_workspace_bytes(int64_t Ns, int n_factors, const int32_t *k,
   return (int64_t)fact_layout(Ns, n_factors, k, n_blocks, blocks).total;
 }
 
+// out = O^T w for a factored O (the blocks of vmcx_sr_minsr_factored), w a
+// length-Ns vector: plain and bias blocks as compensated column sums of the
+// factor, outer-product blocks as one FP64 GEMM each (Y: Ns x max k_b scratch,
+// part: the column-sum partials).
+static int factored_ot(int64_t Ns, const double *const *F, const int64_t *ld, const int32_t *k,
+                       int n_blocks, const vmcx_sr_block *blocks, const double *w, double *part,
+                       double *Y, double *out, rocblas_handle h, hipStream_t st) {
+  const double zero = 0.0;
+  const int nsi = (int)Ns;
+  int rc = 0;
+  for (int b = 0; b < n_blocks; ++b) {
+    const vmcx_sr_block &B = blocks[b];
+    const int a = B.a_factor, fb = B.b_factor;
+    const double s = B.scale;
+    if (fb < 0) {
+      // out_block = s F_a^T w: column sums of the row-major factor
+      rc = xt_gemv_cols<double>(Ns, k[a], F[a], ld[a], w, s, part, out + B.w_offset, st);
+      if (rc) return rc;
+      continue;
+    }
+    // W[i][j] = s sum_n w_n F_a[n][i] F_b[n][j]: column-major (k_b x k_a) =
+    // (diag(w) F_b)^T-as-(k_b x Ns) times F_a (Ns x k_a)
+    hipLaunchKernelGGL(scale_rows_kernel, dim3(2048), dim3(256), 0, st, Ns, (int)k[fb], F[fb],
+                       ld[fb], w, Y);
+    VMCX_LAUNCHED();
+    VMCX_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, (int)k[fb],
+                            (int)k[a], nsi, &s, Y, (int)k[fb], F[a], (int)ld[a], &zero,
+                            out + B.w_offset, (int)k[fb]));
+    if (B.bias_offset >= 0) {
+      rc = xt_gemv_cols<double>(Ns, k[a], F[a], ld[a], w, s, part, out + B.bias_offset, st);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
 int sr_minsr_factored(int64_t Ns, int n_factors, const double *const *F, const int64_t *ld,
                       const int32_t *k, int n_blocks, const vmcx_sr_block *blocks, int64_t Np,
                       const double *E, double energy_mean, double shift, void *ws,
@@ -996,29 +1034,8 @@ int sr_minsr_factored(int64_t Ns, int n_factors, const double *const *F, const i
   hipLaunchKernelGGL(center_vec_kernel, dim3(gv), dim3(256), 0, st, Ns, alpha, sc + 15, scale,
                      at);
   VMCX_LAUNCHED();
-  for (int b = 0; b < n_blocks; ++b) {
-    const vmcx_sr_block &B = blocks[b];
-    const int a = B.a_factor, fb = B.b_factor;
-    const double s = B.scale;
-    if (fb < 0) {
-      // dp_block = s F_a^T a~: column sums of the row-major factor
-      rc = xt_gemv_cols<double>(Ns, k[a], F[a], ld[a], at, s, part, dp + B.w_offset, st);
-      if (rc) return rc;
-      continue;
-    }
-    // W[i][j] = s sum_n a~_n F_a[n][i] F_b[n][j]: column-major (k_b x k_a) =
-    // (diag(a~) F_b)^T-as-(k_b x Ns) times F_a (Ns x k_a)
-    hipLaunchKernelGGL(scale_rows_kernel, dim3(2048), dim3(256), 0, st, Ns, (int)k[fb], F[fb],
-                       ld[fb], at, Y);
-    VMCX_LAUNCHED();
-    VMCX_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, (int)k[fb],
-                            (int)k[a], nsi, &s, Y, (int)k[fb], F[a], (int)ld[a], &zero,
-                            dp + B.w_offset, (int)k[fb]));
-    if (B.bias_offset >= 0) {
-      rc = xt_gemv_cols<double>(Ns, k[a], F[a], ld[a], at, s, part, dp + B.bias_offset, st);
-      if (rc) return rc;
-    }
-  }
+  rc = factored_ot(Ns, F, ld, k, n_blocks, blocks, at, part, Y, dp, h, st);
+  if (rc) return rc;
   if (stats) {
     hipLaunchKernelGGL(finish_stats, dim3(1), dim3(1), 0, st, sc, stats);
     VMCX_LAUNCHED();
@@ -1026,6 +1043,95 @@ int sr_minsr_factored(int64_t Ns, int n_factors, const double *const *F, const i
   if (info) VMCX_HIP(hipMemcpyAsync(info, iinfo, sizeof(int32_t), hipMemcpyDeviceToDevice, st));
   timer.mark();
   timer.read(phase_ms);
+  return 0;
+}
+
+// The energy gradient g = O^T (E - E_mean) / Ns on a factored O (the
+// reference's run_SR = False branch): no Gram, no factorisation.
+int sr_factored_gradient(int64_t Ns, int n_factors, const double *const *F, const int64_t *ld,
+                         const int32_t *k, int n_blocks, const vmcx_sr_block *blocks, int64_t Np,
+                         const double *E, double energy_mean, void *ws, int64_t ws_bytes,
+                         double *g, double *stats, hipStream_t st) {
+  VMCX_CHECK_ARG(Np >= 1, "Np must be positive");
+  int rc = check_factored(Ns, n_factors, k, n_blocks, blocks, Np);
+  if (rc) return rc;
+  VMCX_CHECK_ARG(F && ld && E && ws && g, "null pointer");
+  for (int f = 0; f < n_factors; ++f) {
+    VMCX_CHECK_ARG(F[f] != nullptr, "null factor pointer");
+    VMCX_CHECK_ARG(ld[f] >= k[f] && ld[f] <= 0x7fffffff, "factor ld must be in [k, 2^31)");
+  }
+  const FactLayout L = fact_layout(Ns, n_factors, k, n_blocks, blocks);
+  VMCX_CHECK_ARG(ws_bytes >= (int64_t)L.total, "workspace too small");
+  char *w = (char *)ws;
+  double *vec = (double *)(w + L.vec), *sc = (double *)(w + L.sc);
+  double *Y = (double *)(w + L.Y), *part = (double *)(w + L.part);
+  double *wE = vec;
+  rocblas_handle h = blas_handle_for(st);
+  VMCX_CHECK_ARG(h != nullptr, "rocblas_create_handle failed");
+  hipLaunchKernelGGL(set_scalars, dim3(1), dim3(1), 0, st, sc);
+  const int use_given = std::isfinite(energy_mean) ? 1 : 0;
+  // wE = (E - E_mean) / Ns (compensated mean and variance into sc[8], sc[9])
+  hipLaunchKernelGGL(energy_kernel, dim3(1), dim3(1024), 0, st, Ns, E, energy_mean, use_given,
+                     1.0 / (double)Ns, wE, sc + 8);
+  VMCX_LAUNCHED();
+  rc = factored_ot(Ns, F, ld, k, n_blocks, blocks, wE, part, Y, g, h, st);
+  if (rc) return rc;
+  if (stats) {
+    hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, st, Np, g, sc + 12);
+    hipLaunchKernelGGL(finish_stats, dim3(1), dim3(1), 0, st, sc, stats);
+    VMCX_LAUNCHED();
+  }
+  return 0;
+}
+
+// [mean, variance] of E: compensated sum for the mean, then a compensated
+// sum of (E - mean)^2 (one workgroup; the driver's energy statistics).
+__global__ void __launch_bounds__(1024)
+    energy_stats_kernel(int64_t n, const double *__restrict__ E, double *out) {
+  __shared__ double sh[2][16];
+  __shared__ double mean_sh;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int pass = 0; pass < 2; ++pass) {
+    const double m = pass ? mean_sh : 0.0;
+    Neu a;
+    for (int64_t s = threadIdx.x; s < n; s += blockDim.x) {
+      const double v = pass ? (E[s] - m) * (E[s] - m) : E[s];
+      a.add(v);
+    }
+    double v0 = a.s, v1 = a.c;
+    for (int o = 32; o > 0; o >>= 1) {
+      Neu x;
+      x.s = v0; x.c = v1; x.add(__shfl_down(v0, o, 64)); x.c += __shfl_down(v1, o, 64);
+      v0 = x.s; v1 = x.c;
+    }
+    if (lane == 0) {
+      sh[0][wv] = v0;
+      sh[1][wv] = v1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      Neu x;
+      for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+        x.add(sh[0][k]);
+        x.c += sh[1][k];
+      }
+      const double r = x.value() / (double)n;
+      if (pass == 0) {
+        mean_sh = r;
+        out[0] = r;
+      } else {
+        out[1] = r;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+int energy_stats(int64_t n, const double *E, double *out, hipStream_t st) {
+  VMCX_CHECK_ARG(n >= 1, "n must be positive");
+  VMCX_CHECK_ARG(E && out, "null pointer");
+  hipLaunchKernelGGL(energy_stats_kernel, dim3(1), dim3(1024), 0, st, n, E, out);
+  VMCX_LAUNCHED();
   return 0;
 }
 

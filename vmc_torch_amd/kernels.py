@@ -372,6 +372,33 @@ def sr_minsr_factored(FO, E, energy_mean, diag_shift, workspace=None, dp=None,
     return dp, stats, info
 
 
+def sr_factored_gradient(FO, E, energy_mean, workspace=None, g=None, with_stats=True):
+    """The energy gradient g = O^T (E - energy_mean) / Ns of a FactoredO
+    (no Gram).  Returns (g [Np] float64, stats [5] or None)."""
+    Ns, Np = FO.shape
+    require(E, F64, "local_energies", (Ns,), FO.device)
+    ws = (workspace or SRFactoredWorkspace()).get(FO)
+    g = torch.empty(Np, dtype=F64, device=FO.device) if g is None else require(g, F64, "g", (Np,))
+    stats = torch.empty(5, dtype=F64, device=FO.device) if with_stats else None
+    em = float("nan") if energy_mean is None else float(energy_mean)
+    nf, ptrs, lds, ks, nb, blk = FO._c_args()
+    check(lib().vmcx_sr_factored_gradient(Ns, nf, ptrs, lds, ks, nb, blk, Np, ptr(E), em,
+                                          ptr(ws), ws.numel(), ptr(g), ptr(stats),
+                                          stream_ptr(FO.device)))
+    return g, stats
+
+
+def energy_stats(E, out=None):
+    """[mean, variance] of E (compensated sums) as a [2] float64 device
+    tensor; nothing is copied to the host."""
+    require(E, F64, "local_energies")
+    if E.dim() != 1 or E.numel() < 1:
+        raise ValueError("local_energies must be a non-empty vector")
+    out = torch.empty(2, dtype=F64, device=E.device) if out is None else require(out, F64, "out", (2,))
+    check(lib().vmcx_energy_stats(E.numel(), ptr(E), ptr(out), stream_ptr(E.device)))
+    return out
+
+
 def sgd_update(param, dp, lr):
     """param -= lr * dp (in place, flat view)."""
     require(param, F64, "param")

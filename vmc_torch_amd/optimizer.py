@@ -57,13 +57,15 @@ class SGDGPU(OptimizerGPU):
         params = list(model.parameters())
         dev = params[0].device
         d = torch.as_tensor(direction, device=dev, dtype=torch.float64).contiguous()
+        total = sum(p.numel() for p in params)
+        if d.numel() != total:
+            # checked before any parameter is touched
+            raise ValueError(f"direction has {d.numel()} entries, parameters {total}")
         off = 0
         for p in params:
             n = p.numel()
             kernels.sgd_update(p.data.view(-1), d[off:off + n], lr)
             off += n
-        if off != d.numel():
-            raise ValueError(f"direction has {d.numel()} entries, parameters {off}")
         if hasattr(model, "invalidate"):
             model.invalidate()
 
@@ -125,12 +127,20 @@ class MinSRGPU(PreconditionerGPU):
     is a kernels.FactoredO from model.log_derivative_factored and the Gram
     is formed from the Grams of its factors, no Ns x Np matrix).
     inplace: centre O in place (O is overwritten; saves an Ns x Np copy).
-    Returns (dp device tensor, seconds, info) with info = 0 on success, or
-    the Cholesky failure index (the reference returns 1 after an lstsq
-    fallback).
+    Returns (dp device tensor, seconds, info) with info = 0 on success.
+    When the Cholesky factorisation fails (a non-positive leading minor, the
+    reference's cue to fall back to lstsq, GPU/optimizer.py:557-561), the
+    step is solved again with the shift raised tenfold (a shift <= 0 is
+    first replaced by max(10 |shift|, 1e-8)), at most
+    `max_retries` times, and info = 1 as for the reference's fallback;
+    last_fallback records the failing minor and the shift that succeeded
+    (None if every retry failed: then dp is NaN, and the driver skips the
+    update).  run_sr=False returns the energy gradient
+    g = <E O> - <E><O> (GPU/optimizer.py:581-614).
     """
 
-    def __init__(self, gram="fp64", max_iter=None, tol=1e-8, sync_info=True, inplace=False):
+    def __init__(self, gram="fp64", max_iter=None, tol=1e-8, sync_info=True, inplace=False,
+                 max_retries=3):
         if gram not in kernels.GRAM_CODES and gram != "factored":
             raise ValueError(f"unknown gram {gram!r}")
         self.gram = gram
@@ -143,35 +153,67 @@ class MinSRGPU(PreconditionerGPU):
         self.sync_info = sync_info
         self.inplace = inplace
         self.workspace = kernels.SRWorkspace()
+        self.minres_workspace = None
+        self.max_retries = max_retries
         self.last_stats = None
+        self.last_fallback = None
 
     def solve(self, *, local_o, local_energies, energy_mean, total_samples, n_params,
               diag_shift, device, run_sr=True):
         t0 = time.time()
-        if not run_sr:
-            raise NotImplementedError("run_sr=False: use the energy gradient from MinSRGPU stats")
+        E = local_energies.to(device=device, dtype=torch.float64).contiguous()
         if isinstance(local_o, kernels.FactoredO):
-            E = local_energies.to(device=device, dtype=torch.float64).contiguous()
             if local_o.shape != (total_samples, n_params):
                 raise ValueError("local_o must be (total_samples, n_params) on one GPU")
-            dp, stats, info = kernels.sr_minsr_factored(local_o, E, energy_mean, diag_shift,
+            if not run_sr:
+                g, stats = kernels.sr_factored_gradient(local_o, E, energy_mean,
                                                         self.factored_workspace)
-            self.last_stats = stats
-            info_v = int(info.item()) if self.sync_info else info
-            return dp, time.time() - t0, info_v
-        if self.factored:
-            raise TypeError("gram='factored' takes a kernels.FactoredO "
-                            "(model.log_derivative_factored)")
-        dt = local_o.dtype if local_o.dtype in (torch.float32, torch.float64) else torch.float64
-        if self.gram == "fp64":
-            dt = torch.float64
-        O = local_o.to(device=device, dtype=dt).contiguous()
-        E = local_energies.to(device=device, dtype=torch.float64).contiguous()
-        if O.shape[0] != total_samples or O.shape[1] != n_params:
-            raise ValueError("local_o must be (total_samples, n_params) on one GPU")
-        dp, stats, info = kernels.sr_minsr(O, E, energy_mean, diag_shift, self.gram,
-                                           self.max_iter, self.tol, self.workspace,
-                                           inplace=self.inplace)
+                self.last_stats = stats
+                return g, time.time() - t0, None
+
+            def run(shift):
+                return kernels.sr_minsr_factored(local_o, E, energy_mean, shift,
+                                                 self.factored_workspace)
+        else:
+            if self.factored:
+                raise TypeError("gram='factored' takes a kernels.FactoredO "
+                                "(model.log_derivative_factored)")
+            dt = local_o.dtype if local_o.dtype in (torch.float32, torch.float64) \
+                else torch.float64
+            if self.gram == "fp64" or not run_sr:
+                dt = torch.float64
+            O = local_o.to(device=device, dtype=dt).contiguous()
+            if O.shape[0] != total_samples or O.shape[1] != n_params:
+                raise ValueError("local_o must be (total_samples, n_params) on one GPU")
+            if not run_sr:
+                # the MINRES entry point with no iterations returns g
+                if self.minres_workspace is None:
+                    self.minres_workspace = kernels.MinresWorkspace()
+                g, stats, _ = kernels.sr_minres(O, E, energy_mean, 0.0, maxiter=0,
+                                                workspace=self.minres_workspace)
+                self.last_stats = stats
+                return g, time.time() - t0, None
+
+            def run(shift):
+                return kernels.sr_minsr(O, E, energy_mean, shift, self.gram, self.max_iter,
+                                        self.tol, self.workspace, inplace=self.inplace)
+        dp, stats, info = run(diag_shift)
         self.last_stats = stats
-        info_v = int(info.item()) if self.sync_info else info
-        return dp, time.time() - t0, info_v
+        self.last_fallback = None
+        if not self.sync_info:
+            return dp, time.time() - t0, info
+        info_v = int(info.item())
+        if info_v == 0:
+            return dp, time.time() - t0, 0
+        # non-positive leading minor: raise the shift (the reference falls back
+        # to lstsq here); info 1 as for the reference's fallback
+        failed, shift = info_v, diag_shift
+        for _ in range(self.max_retries):
+            shift = shift * 10.0 if shift > 0 else max(-10.0 * shift, 1e-8)
+            dp, stats, info = run(shift)
+            if int(info.item()) == 0:
+                self.last_stats = stats
+                self.last_fallback = {"cholesky_info": failed, "diag_shift": shift}
+                return dp, time.time() - t0, 1
+        dp = torch.full_like(dp, float("nan"))
+        return dp, time.time() - t0, 1

@@ -2,6 +2,8 @@
 the reference's interface (GPU/vmc_utils.py, GPU/vmc_setup.py)."""
 import torch
 
+from . import kernels
+
 
 
 def random_initial_config(N_f, N_sites, seed=None):
@@ -41,7 +43,29 @@ def evaluate_energy(fxs, model, H, current_amps=None, verbose=False, use_log_amp
     """
     _check_model(model, "local_energy")
     eloc = model.local_energy(fxs, H)
-    return eloc.mean(), eloc
+    return kernels.energy_stats(eloc)[0], eloc
+
+
+def check_grads_amps(O, sign, logabs, configs=None):
+    """GPU/vmc_utils.py:1330-1386 (_check_grads_amps): raise ValueError when
+    the amplitudes or the per-sample gradients hold NaN or Inf (one device
+    reduction per tensor and one host read).  An amplitude is its sign and
+    ln|psi|; ln|psi| = -inf is a zero amplitude, which the log form clamps
+    to ln(1e-45), so any non-finite value is an error."""
+    bad_amp = ~(torch.isfinite(sign) & torch.isfinite(logabs))
+    # a NaN or Inf anywhere in a row makes its sum non-finite; the row sums
+    # avoid a full-size boolean temporary of O
+    bad_o = ~torch.isfinite(O.sum(dim=1)) if O.numel() else torch.zeros_like(bad_amp)
+    counts = torch.stack([bad_amp.sum(), bad_o.sum()]).cpu()
+    if int(counts[0]):
+        idx = bad_amp.nonzero()[:20, 0].tolist()
+        where = "" if configs is None else "; first configs: " + "; ".join(
+            str(configs[i].tolist()) for i in idx)
+        raise ValueError(f"NaN/Inf in amplitudes: {int(counts[0])} out of "
+                         f"{sign.numel()} samples{where}")
+    if int(counts[1]):
+        raise ValueError(f"NaN/Inf in gradients: {int(counts[1])}/{O.shape[0]} samples "
+                         "affected")
 
 
 def compute_grads_gpu(fxs, model, batch_size=None, verbose=False, offload_to_cpu=False,
@@ -56,6 +80,7 @@ def compute_grads_gpu(fxs, model, batch_size=None, verbose=False, offload_to_cpu
             O, (sign, logabs) = model.log_derivative(fxs)
         else:
             O, (sign, logabs) = model.log_derivative(fxs, dtype=out_dtype)
+        check_grads_amps(O, sign, logabs, fxs)
         if use_log_amp:
             out = O, (sign, logabs)
         else:
