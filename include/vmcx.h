@@ -267,6 +267,25 @@ int vmcx_sr_minres(int64_t Ns, int64_t Np, const void *O, int o_dtype, int64_t l
                    int32_t *info, void *stream);
 
 /*
+ * vmcx_sr_minres without a workspace argument and without any allocation:
+ * no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The workspace is a fixed 20 MiB device array inside the
+ * library, one per GPU, held until the process ends:
+ * vmcx_sr_minres_workspace_bytes(Ns, Np) must not exceed
+ * vmcx_sr_minres_noalloc_max_bytes() (e.g. Np <= 16384 with Ns <= 65536),
+ * otherwise VMCX_EINVAL.  Arguments and results as vmcx_sr_minres.  Calls
+ * that overlap on one GPU (two streams, two threads) share that array and
+ * race; calls in sequence on one stream are safe (no entry is read before the
+ * same call writes it).
+ */
+int64_t vmcx_sr_minres_noalloc_max_bytes(void);
+int vmcx_sr_minres_noalloc(int64_t Ns, int64_t Np, const void *O, int o_dtype, int64_t ldo,
+                           const double *E, double energy_mean, double diag_shift, double rtol,
+                           int maxiter, double *dp, double *stats, int32_t *info,
+                           void *stream);
+
+/*
  * Neural-network backflow (NNBF) model: two hidden layers of width `hidden`.
  *   psi(x) = det((M + bf_scale * reshape(Wout act(W2 act(W1 n + b1) + b2)
  *                 + bout, (2N, n_f)))[occupied rows ascending])

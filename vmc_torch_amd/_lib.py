@@ -46,6 +46,9 @@ _SIGNATURES = {
     "vmcx_sr_minres_workspace_bytes": (_I64, [_I64, _I64]),
     "vmcx_sr_minres": (_I, [_I64, _I64, _P, _I, _I64, _P, _D, _D, _D, _I, _P, _I64, _P, _P,
                             _P, _P]),
+    "vmcx_sr_minres_noalloc_max_bytes": (_I64, []),
+    "vmcx_sr_minres_noalloc": (_I, [_I64, _I64, _P, _I, _I64, _P, _D, _D, _D, _I, _P, _P, _P,
+                                    _P]),
     "vmcx_sgd_update": (_I, [_I64, _P, _P, _D, _P]),
     "vmcx_syrk_f64": (_I, [_I64, _I64, _P, _I64, _I, _D, _D, _P, _I64, _I, _I64, _P]),
     "vmcx_nnbf_workspace_bytes": (_I64, [_P, _I64, _I64, _I]),
