@@ -114,8 +114,15 @@ def parse():
     p.add_argument("--sweeps-per-sample", type=int, default=1,
                    help="Slater: Metropolis sweeps between samples (1 = the reference's "
                         "one sample per sweep)")
-    p.add_argument("--mlp", default=None, choices=["fp32", "bf16"],
-                   help="NNBF dense-layer precision (default: the workload's)")
+    p.add_argument("--mlp", default=None, choices=["fp64", "fp32", "bf16"],
+                   help="NNBF network precision (default: the workload's)")
+    p.add_argument("--slater-precision", default="fp64", choices=["fp64", "fp32"],
+                   help="Slater walker state and sweep arithmetic (fp32: the FP32 form, "
+                        "FP64 refresh and SR)")
+    p.add_argument("--no-target", action="store_true",
+                   help="skip the 8x8 NNBF target comparison of the default run")
+    p.add_argument("--target-steps", type=int, default=8,
+                   help="timed SR iterations per form in the target comparison")
     p.add_argument("--o-dtype", default=None, choices=["fp32", "fp64"],
                    help="NNBF log-derivative matrix O (default: the workload's, else fp32)")
     p.add_argument("--sampler", default="plain", choices=["plain", "learned"],
@@ -204,7 +211,9 @@ def cpu_baseline(kind, Lx, Ly, nu, nd, params, bf, x, n_samples_step, tau, shift
     import numpy as np
 
     from oracle import sr as osr
-    cores = min(os.cpu_count() or 1, 16)
+    # every core of this process's CPU share: one GPU's share of the box is
+    # 16 cores (the processes beyond it would only time-slice)
+    cores = min(len(os.sched_getaffinity(0)), 16)
     ctx = mp.get_context("spawn")
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     per = max(1, len(x) // cores)
@@ -215,18 +224,28 @@ def cpu_baseline(kind, Lx, Ly, nu, nd, params, bf, x, n_samples_step, tau, shift
     samples = sum(r[0] for r in res)
     wall = max(r[1] for r in res)
     samples_per_s = samples / wall
-    # SR on the host at a reduced sample count, extrapolated to the step's Ns
-    # by the Gram cost Ns^2 Np (the dominant term at these shapes)
+    # SR on the host (numpy, multithreaded BLAS): the Gram O O^T timed at a
+    # reduced sample count and scaled by Ns^2 (its cost is Ns^2 Np), the
+    # dense solve of the Ns x Ns system (the reference's linalg.solve, LU,
+    # 2/3 Ns^3) timed at the full Ns
     Np = int(sum(p.size for p in params))
-    ns_cpu = 512 if Np < 100000 else 128
+    ns_cpu = min(2048, int(n_samples_step)) if Np < 100000 else 128
     rng = np.random.default_rng(0)
     O = rng.normal(size=(ns_cpu, Np))
-    E = rng.normal(size=ns_cpu)
     t0 = time.perf_counter()
-    osr.minsr(O, E, E.mean(), shift)
-    t_sr_small = time.perf_counter() - t0
-    del O
-    t_sr = t_sr_small * (n_samples_step / ns_cpu) ** 2
+    Oc = O - O.mean(axis=0)
+    _ = Oc @ Oc.T
+    t_gram_small = time.perf_counter() - t0
+    del O, Oc
+    Ns = int(n_samples_step)
+    X = rng.normal(size=(Ns, 64))
+    T = X @ X.T / 64 + shift * np.eye(Ns) + np.eye(Ns)
+    eps = rng.normal(size=Ns)
+    t0 = time.perf_counter()
+    np.linalg.solve(T, eps)
+    t_solve = time.perf_counter() - t0
+    del X, T
+    t_sr = t_gram_small * (n_samples_step / ns_cpu) ** 2 + t_solve
     t_iter = n_samples_step / samples_per_s + t_sr
     return {
         "value": n_samples_step / t_iter / tau,
@@ -235,9 +254,10 @@ def cpu_baseline(kind, Lx, Ly, nu, nd, params, bf, x, n_samples_step, tau, shift
         "kind": "port",
         "sample": (f"FP64 oracle restating the reference algorithm (full amplitude per "
                    f"proposal and per connected configuration), {samples} walker-samples "
-                   f"(sweep + E_loc + O_k) on {cores} processes in {wall:.1f} s; minSR "
-                   f"timed at Ns={ns_cpu}, Np={Np} ({t_sr_small:.2f} s) and extrapolated by "
-                   f"Ns^2 to Ns={n_samples_step}; same tau_int as the GPU chain"),
+                   f"(sweep + E_loc + O_k) on {cores} processes in {wall:.1f} s; minSR: "
+                   f"Gram timed at Ns={ns_cpu}, Np={Np} ({t_gram_small:.2f} s) and scaled "
+                   f"by Ns^2 to Ns={Ns}, plus the dense solve timed at Ns={Ns} "
+                   f"({t_solve:.2f} s); same tau_int as the GPU chain"),
         "walker_sweeps_per_s": samples_per_s,
         "sr_iters_per_s": 1.0 / t_iter,
         "cpu_model": _cpu_model(),
@@ -245,13 +265,21 @@ def cpu_baseline(kind, Lx, Ly, nu, nd, params, bf, x, n_samples_step, tau, shift
 
 
 def _cpu_model():
+    """The host CPU as /proc/cpuinfo names it (model name, family / model /
+    stepping numbers) and the core counts (this process's share of all)."""
+    f = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                f.setdefault(k.strip(), v.strip())
     except OSError:
         pass
-    return "unknown"
+    name = f.get("model name", "unknown")
+    ids = ", ".join(f"{k} {f[k]}" for k in ("vendor_id", "cpu family", "model", "stepping")
+                    if k in f)
+    return (f"{name} ({ids}); {len(os.sched_getaffinity(0))} of {os.cpu_count()} host "
+            "cores in this process's share")
 
 
 # ------------------------------------------------------------- workloads
@@ -270,9 +298,16 @@ class SlaterRun:
         self.N, self.nf, self.B, self.Np = N, nf, B, 2 * N * nf
         g = torch.Generator().manual_seed(0)
         self.M = (torch.randn(2 * N, nf, generator=g, dtype=torch.float64) * 0.1).to(dev)
+        # the FP32 form: M rounded to FP32 (the model's parameters), A^-1 in FP32;
+        # the exact refresh takes M widened back to FP64
+        self.fp32 = getattr(a, "slater_precision", "fp64") == "fp32"
+        if self.fp32:
+            self.M32 = self.M.to(torch.float32)
+            self.M = self.M32.to(torch.float64)
         self.e_sweep = H.graph.edge_tensor(dev, "sweep")
         self.e_graph = H.graph.edge_tensor(dev, "graph")
-        self.st = kernels.SlaterWalkerState.empty(B, N, nf, dev)
+        self.st = kernels.SlaterWalkerState.empty(
+            B, N, nf, dev, dtype=torch.float32 if self.fp32 else torch.float64)
         self.O = torch.empty(B, self.Np, dtype=torch.float64, device=dev)
         self.dp = torch.empty(self.Np, dtype=torch.float64, device=dev)
         self.ws = kernels.SRWorkspace()
@@ -281,14 +316,16 @@ class SlaterRun:
 
     def _sweep(self, n, seed, off):
         self.k.slater_sweep(self.M, self.e_sweep, self.fxs, self.st, n_sweeps=n, seed=seed,
-                            sweep_offset=off, refresh_every=self.a.refresh_every)
+                            sweep_offset=off, refresh_every=self.a.refresh_every,
+                            M32=self.M32 if self.fp32 else None)
 
     def burn(self, n, seed, off):
         self.k.slater_refresh(self.M, self.fxs, self.st)
         self._sweep(n, seed, off)
 
     def local_energy(self):
-        return self.k.slater_local_energy(self.M, self.e_graph, 1.0, 8.0, self.fxs, self.st)
+        return self.k.slater_local_energy(self.M, self.e_graph, 1.0, 8.0, self.fxs, self.st,
+                                          M32=self.M32 if self.fp32 else None)
 
     def step(self, mark, seed, off, gram, phases):
         m = [mark()]
@@ -305,7 +342,11 @@ class SlaterRun:
                                          0 if gram == "fp64" else 20, 1e-8, self.ws,
                                          dp=self.dp, phase_ms=phases)
         m.append(mark())
-        self.k.sgd_update(self.M.view(-1), self.dp, self.w["lr"])
+        if self.fp32:
+            self.k.sgd_update(self.M32.view(-1), self.dp, self.w["lr"])
+            self.M.copy_(self.M32)
+        else:
+            self.k.sgd_update(self.M.view(-1), self.dp, self.w["lr"])
         m.append(mark())
         return m, E, self.st.n_accept - acc0, stats, info
 
@@ -321,11 +362,12 @@ class SlaterRun:
         the sweep, written once per sweep; configs and the per-walker rows."""
         B, N, nf = self.B, self.N, self.nf
         nfp = self.k.slater_nfp(nf)
+        sz = 4 if self.fp32 else 8
         return {
-            "sweep": ("hbm", self.spp * B * (2 * nfp * nfp * 8 + 2 * 2 * N + 2 * 8 * N)),
-            "local_energy": ("hbm", B * (nfp * nfp * 8 + 2 * N + 8 * N + 8)),
-            "logderiv": ("hbm", B * (nf * nfp * 8 + 2 * N + self.Np * 8)),
-            "refresh": ("hbm", B * (nfp * nfp * 8 + 2 * N + 8 * N)),
+            "sweep": ("hbm", self.spp * B * (2 * nfp * nfp * sz + 2 * 2 * N + 2 * 8 * N)),
+            "local_energy": ("hbm", B * (nfp * nfp * sz + 2 * N + 8 * N + 8)),
+            "logderiv": ("hbm", B * (nf * nfp * sz + 2 * N + self.Np * 8)),
+            "refresh": ("hbm", B * (nfp * nfp * sz + 2 * N + 8 * N)),
         }
 
     def oracle_params(self):
@@ -455,10 +497,9 @@ class NnbfRun:
         }
 
     def oracle_params(self):
-        import torch
-        # the device computes with FP32 copies of the FP64 master parameters
-        return [p.detach().to(torch.float32).to(torch.float64).cpu().numpy()
-                for p in self.model.params], self.w["bf_scale"]
+        # the FP64 master parameters: the oracle is FP64 throughout, so the
+        # difference includes the rounding of the FP32 / BF16 modes
+        return [p.detach().cpu().numpy() for p in self.model.params], self.w["bf_scale"]
 
 
 class PepsRun(NnbfRun):
@@ -568,6 +609,7 @@ def main():
     base[: N - nf] = 0
     perms = torch.argsort(torch.rand(B, N, generator=torch.Generator().manual_seed(42)), dim=1)
     fxs = base[perms].contiguous().to(dev)
+    fxs0 = fxs.clone()
     run = {"slater": SlaterRun, "nnbf": NnbfRun, "peps": PepsRun}[w["kind"]](w, a, dev, H, fxs)
     seed, sweep_no = 0x5EED, 0
     spp = getattr(run, "spp", 1)
@@ -767,6 +809,15 @@ def main():
     }
     if w["kind"] == "slater":
         out["config"]["refresh_every"] = a.refresh_every
+        out["config"]["precision"] = a.slater_precision
+        if a.slater_precision == "fp32":
+            out["dtype"] = "f32-state/f64-refresh/f64-sr"
+    if a.workload == "hubbard8x8_slater" and a.slater_precision == "fp64":
+        # the FP32 form of the same configuration (the FP64 line is the headline)
+        out["fp32_form"] = slater_fp32_form(a, w, fxs0, dev, H, a.steps, a.warmup)
+        log(f"FP32 form: {out['fp32_form']['ms_per_step']:.2f} ms per SR iteration")
+        if not a.no_target:
+            out["target"] = nnbf_target(a, dev, a.target_steps)
 
     if not a.no_cpu_baseline and spp == 1:   # the CPU sample is one sweep per sample
         # FP64 oracle on a subset of the final walkers: |dE/site|, then the timed baseline
@@ -795,6 +846,181 @@ def main():
                                            a.cpu_seconds)
     print(json.dumps(out))
 
+
+
+# ------------------------------------------------ forms beside the headline
+def measure_form(run, w, a, gram, steps, warmup, seed, sweep_no, tau_sweeps, tau=None):
+    """Time `steps` SR iterations of a prepared run (after `warmup` untimed
+    ones), then tau_int of E_loc over `tau_sweeps` samples at fixed
+    parameters (skipped when tau is given: a form with the same chain)."""
+    import torch
+
+    from vmc_torch_amd.stats import tau_int
+
+    def mark():
+        return None
+
+    def nomark():
+        return None
+    for _ in range(warmup):
+        run.step(nomark, seed, sweep_no, gram, None)
+        sweep_no += 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    accs = []
+    for _ in range(steps):
+        _, E, acc, stats, info = run.step(mark, seed, sweep_no, gram, None)
+        sweep_no += 1
+        accs.append(acc)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if tau is None:
+        run.pre_tau()
+        series = []
+        for _ in range(tau_sweeps):
+            series.append(run.tau_sample(seed, sweep_no).clone())
+            sweep_no += 1
+        torch.cuda.synchronize()
+        tau = tau_int(torch.stack(series, dim=1))
+    B = run.B
+    sr = steps / wall
+    return {"ms_per_step": 1e3 * wall / steps, "sr_iters_per_s": sr, "tau_int": tau,
+            "effective_samples_per_s": B * sr / tau,
+            "acceptance": float(torch.stack(accs).double().mean()) / run.proposals_per_step,
+            "sr_info": int(info.item()), "energy_per_site": float(E.mean()) / run.N}, sweep_no
+
+
+def slater_fp32_form(a, w, fxs0, dev, H, steps, warmup):
+    """The FP32 form of the headline configuration on the same initial
+    walkers: FP32 parameters and A^-1, FP32 ratios and Sherman-Morrison
+    updates, FP64 refresh, local-energy sums and SR."""
+    import argparse as ap
+    import copy
+
+    import numpy as np
+
+    from oracle import hubbard as ohub
+    from oracle import slater as osl
+    a32 = ap.Namespace(**vars(copy.deepcopy(a)))
+    a32.slater_precision = "fp32"
+    run = SlaterRun(w, a32, dev, H, fxs0.clone())
+    seed = 0x5EED
+    run.burn(w["burn"], seed, 0)
+    res, sweep_no = measure_form(run, w, a32, "fp64", steps, warmup, seed, w["burn"],
+                                 a.tau_sweeps)
+    # the sweep alone, both forms, on the walkers of the FP32 run (HIP events)
+    import torch
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    run._sweep(4, seed, sweep_no)
+    ev[1].record()
+    torch.cuda.synchronize()
+    res["sweep_ms"] = ev[0].elapsed_time(ev[1]) / 4
+    xs = run.fxs[:128].cpu().numpy()
+    Eg = run.local_energy()[:128].cpu().numpy()
+    Eo = osl.local_energy(ohub.Hubbard(w["Lx"], w["Ly"], 1.0, 8.0, w["nu"], w["nd"]),
+                          run.M.cpu().numpy(), xs)
+    N = w["Lx"] * w["Ly"]
+    res["dE_per_site_vs_fp64_oracle"] = {"mean": float(abs(Eg.mean() - Eo.mean()) / N),
+                                         "max_per_sample": float(np.abs(Eg - Eo).max() / N),
+                                         "samples": int(len(xs))}
+    res["dtype"] = "f32-state/f64-refresh/f64-sr"
+    return res
+
+
+def nnbf_target(a, dev, steps):
+    """The north-star comparison on 8x8 U=8 with the NNBF ansatz (128 -> 256
+    -> 256 -> 8192, 2,212,352 parameters, 4096 walkers): the plain FP64 GPU
+    form (FP64 network and determinant, plain sweep sampler) with the exact
+    FP64 Gram formed from the factors of O, the same with O formed (FP64 Gram
+    of the 4096 x 2.2 M matrix, for reference), and the mixed-precision form
+    (BF16 network with FP32 accumulation, FP64 determinant, learned proposal,
+    the same factored FP64 SR).  One burn-in (plain sampler, BF16 network)
+    serves all forms; the parameters start equal (one seed)."""
+    import argparse as ap
+    import copy
+
+    import numpy as np
+    import torch
+
+    from oracle import hubbard as ohub
+    from oracle import nnbf as onn
+    from vmc_torch_amd.hamiltonian import spinful_Fermi_Hubbard_square_lattice_torch
+    w = WORKLOADS["hubbard8x8_nnbf"]
+    Lx, Ly, nu, nd, B = w["Lx"], w["Ly"], w["nu"], w["nd"], w["B"]
+    N, nf = Lx * Ly, nu + nd
+    H = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, nf,
+                                                   n_fermions_per_spin=(nu, nd))
+    H.precompute_hops_gpu(dev)
+    base = torch.tensor([1, 2] * (N // 2), dtype=torch.int64)
+    base[: N - nf] = 0
+    perms = torch.argsort(torch.rand(B, N, generator=torch.Generator().manual_seed(42)), dim=1)
+    fxs0 = base[perms].contiguous().to(dev)
+
+    def args(**kw):
+        x = ap.Namespace(**vars(copy.deepcopy(a)))
+        x.proposals_per_sample, x.chunk = 0, 65536
+        for k, v in kw.items():
+            setattr(x, k, v)
+        return x
+    seed = 0x5EED
+    burn_run = NnbfRun(w, args(mlp="bf16", gram="factored", o_dtype=None, sampler="plain"), dev,
+                       H, fxs0)
+    burn_run.burn(w["burn"], seed, 0)
+    torch.cuda.synchronize()
+    log(f"target: 8x8 NNBF burn-in ({w['burn']} sweeps) done")
+    forms = {}
+    specs = [("fp64_factored", dict(mlp="fp64", gram="factored", o_dtype=None,
+                                    sampler="plain"), steps, None),
+             ("fp64_O_formed", dict(mlp="fp64", gram="fp64", o_dtype="fp64", sampler="plain"),
+              max(2, steps // 3), "fp64_factored"),
+             ("bf16_learned", dict(mlp="bf16", gram="factored", o_dtype=None,
+                                   sampler="learned"), steps, None)]
+    bf_run = None
+    for name, kw, n_steps, same_chain in specs:
+        ai = args(**kw)
+        run = NnbfRun(w, ai, dev, H, fxs0.clone())
+        if run.lp is not None:
+            run.burn(0, seed, 0)              # trains the proposal net (100 iterations)
+        gram = ai.gram
+        tau = forms[same_chain]["tau_int"] if same_chain else None
+        res, _ = measure_form(run, w, ai, gram, n_steps, 1, seed, w["burn"], a.tau_sweeps, tau)
+        res["mlp"], res["gram"], res["sampler"] = ai.mlp, gram, ai.sampler
+        if same_chain:
+            res["tau_int_note"] = f"the chain of {same_chain} (same sampler and arithmetic)"
+        forms[name] = res
+        log(f"target form {name}: {res['ms_per_step']:.1f} ms per SR iteration, "
+            f"tau_int {res['tau_int']:.2f}")
+        if name == "bf16_learned":
+            bf_run = run
+        else:
+            del run
+        torch.cuda.empty_cache()
+    base_f, mixed = forms["fp64_factored"], forms["bf16_learned"]
+    out = {"workload": "hubbard8x8_nnbf", "description": w["desc"].replace(
+               "BF16 GEMMs", "network precision per form"),
+           "walkers": B, "forms": forms,
+           "ess_ratio_vs_fp64_factored": mixed["effective_samples_per_s"]
+           / base_f["effective_samples_per_s"],
+           "sr_ratio_vs_fp64_factored": mixed["sr_iters_per_s"] / base_f["sr_iters_per_s"],
+           "ess_ratio_vs_fp64_O_formed": mixed["effective_samples_per_s"]
+           / forms["fp64_O_formed"]["effective_samples_per_s"],
+           "sr_ratio_vs_fp64_O_formed": mixed["sr_iters_per_s"]
+           / forms["fp64_O_formed"]["sr_iters_per_s"],
+           "target": {"ess_ratio": 3.0, "sr_ratio": 2.0, "dE_per_site": 1e-4}}
+    # |dE/site| of the mixed form against the FP64 oracle, 1024 of its walkers
+    n_chk = 1024
+    xs = bf_run.fxs[:n_chk].cpu().numpy()
+    Eg = bf_run.local_energy()[:n_chk].cpu().numpy()
+    params, bf = bf_run.oracle_params()
+    Eo = onn.local_energy(ohub.Hubbard(Lx, Ly, 1.0, 8.0, nu, nd), params, xs, 2, bf, chunk=64)
+    d = np.abs(Eg - Eo) / N
+    out["dE_per_site_vs_fp64_oracle"] = {"estimate": float(abs(Eg.mean() - Eo.mean()) / N),
+                                         "per_sample_mean": float(d.mean()),
+                                         "per_sample_max": float(d.max()),
+                                         "samples": int(n_chk)}
+    log("target: FP64 oracle check of the mixed form done")
+    return out
 
 
 # ------------------------------------------------- batched learned proposal

@@ -41,6 +41,16 @@ const char *vmcx_last_error(void);
 int vmcx_slater_nfp(int n_f);
 
 /*
+ * Launch shapes picked by an autotuner (vmc_torch_amd/autotune.py): waves per
+ * workgroup (1, 2, 4 or 8; 0 = the built-in default 4) of a kernel family
+ * for a key.  kind 0 = the Slater sweep, 1 = the Slater local energy, 2 =
+ * the Slater refresh; key = 2 NFP + (1 for the FP32 form).  Process-wide,
+ * read at every launch.  No counterpart in the reference.
+ */
+int vmcx_set_launch_waves(int kind, int key, int waves);
+int vmcx_get_launch_waves(int kind, int key);
+
+/*
  * Exact (re)build of the Slater walker state from configs and M, by an
  * in-register Gauss-Jordan inversion with partial pivoting (FP64).
  *   ln|psi| and sign of psi(x) = det(M[occ ascending, :]).
@@ -67,6 +77,9 @@ int vmcx_slater_refresh(int64_t B, int n_sites, int n_f, const double *M,
  *   n_accept ([B] int64, may be NULL) is incremented per accepted move.
  *   refresh_every > 0: after each sweep, walkers with since_refresh >=
  *   refresh_every are rebuilt exactly (vmcx_slater_refresh semantics).
+ *   sweep_offset_dev (may be NULL): a device int64 added to sweep_offset when
+ *   the kernel runs, so a captured graph advances its draws between replays
+ *   (with vmcx_counter_add).
  * Replaces: MetropolisExchangeSpinfulSamplerGPU.step / burn_in
  *   (GPU/sampler.py:175-197, 216-388) with propose_exchange_or_hopping_vec
  *   (GPU/sampler.py:47-137).
@@ -78,7 +91,14 @@ int vmcx_slater_sweep(int64_t B, int n_sites, int n_f, const double *M,
                       const double *u_acc, int refresh_every,
                       int64_t *configs, double *ainv, uint8_t *slot,
                       double *sign, double *logabs, int32_t *since_refresh,
-                      int64_t *n_accept, void *stream);
+                      int64_t *n_accept, const int64_t *sweep_offset_dev, void *stream);
+
+/* counter[0] += inc on `stream` (a one-thread kernel: graph-capturable, no
+ * memset node).  Advances the device draw counters of vmcx_slater_sweep and
+ * vmcx_lp_propose_given / vmcx_lp_accept inside a captured graph.  No
+ * counterpart in the reference (its draws come from torch's host-seeded
+ * generator). */
+int vmcx_counter_add(int64_t *counter, int64_t inc, void *stream);
 
 /*
  * Hubbard local energy of every walker from its Slater state:
@@ -105,6 +125,43 @@ int vmcx_slater_local_energy(int64_t B, int n_sites, int n_f, const double *M,
 int vmcx_slater_logderiv(int64_t B, int n_sites, int n_f, const double *ainv,
                          const uint8_t *slot, void *O, int o_dtype, int64_t ld,
                          void *stream);
+
+/*
+ * The FP32 form of the four Slater entry points above (the precision form of
+ * the 8x8 Slater configuration).  The walker's A^-1 is stored in FP32
+ * ([B][NFP][NFP] float, same layout); the sweep forms ratios, determinant
+ * ratios and Sherman-Morrison / Woodbury updates in FP32 from M32 (M rounded
+ * to FP32, the same [2N][n_f] layout), compares the acceptance uniform with
+ * the squared ratio and accumulates sign and ln|psi| in FP64; the exact
+ * refresh (vmcx_slater_refresh_f32, and the refresh between sweeps) runs
+ * the FP64 Gauss-Jordan on the FP64 M and rounds A^-1 to FP32.  The local
+ * energy forms each ratio in FP32 and sums in FP64.  Draws, proposal order,
+ * acceptance rule and outputs otherwise as the FP64 entry points.
+ * Replaces: SlaterDeterminant_GPU with dtype=torch.float32
+ *   (GPU/models/slater.py:29-34) on the path of the FP64 entry points.
+ */
+int vmcx_slater_refresh_f32(int64_t B, int n_sites, int n_f, const double *M,
+                            const int64_t *configs, float *ainv, uint8_t *slot, double *sign,
+                            double *logabs, int32_t *since_refresh, int only_if_since,
+                            void *stream);
+int vmcx_slater_sweep_f32(int64_t B, int n_sites, int n_f, const double *M, const float *M32,
+                          int n_edges, const int32_t *edges, double hopping_rate, int n_sweeps,
+                          uint64_t seed, int64_t sweep_offset, const double *u_choice,
+                          const uint8_t *bits, const double *u_acc, int refresh_every,
+                          int64_t *configs, float *ainv, uint8_t *slot, double *sign,
+                          double *logabs, int32_t *since_refresh, int64_t *n_accept,
+                          const int64_t *sweep_offset_dev, void *stream);
+int vmcx_slater_local_energy_f32(int64_t B, int n_sites, int n_f, const float *M32,
+                                 int n_edges, const int32_t *edges, double t, double U,
+                                 const int64_t *configs, const float *ainv,
+                                 const uint8_t *slot, double *eloc, void *stream);
+int vmcx_slater_logderiv_f32(int64_t B, int n_sites, int n_f, const float *ainv,
+                             const uint8_t *slot, void *O, int o_dtype, int64_t ld,
+                             void *stream);
+/* params[i] -= lr * dp[i] for FP32 parameters (the product and difference in
+ * FP64, rounded once).  Replaces: SGDGPU.step (GPU/optimizer.py:144-186) on
+ * an FP32 model. */
+int vmcx_sgd_update_f32(int64_t n, float *params, const double *dp, double lr, void *stream);
 
 /*
  * Batched connected configurations of the Hubbard Hamiltonian in the
@@ -289,9 +346,18 @@ int vmcx_sr_minres_noalloc(int64_t Ns, int64_t Np, const void *O, int o_dtype, i
  * Neural-network backflow (NNBF) model: two hidden layers of width `hidden`.
  *   psi(x) = det((M + bf_scale * reshape(Wout act(W2 act(W1 n + b1) + b2)
  *                 + bout, (2N, n_f)))[occupied rows ascending])
- * All weights are FP32 device copies in the reference's layouts (row-major
- * [out][in]); with gemm = 1 the two dense layers use BF16 copies of W2 and
- * Wout (and BF16 activations) with FP32 accumulation.
+ * Weights in the reference's layouts (row-major [out][in]).  gemm = 0: FP32
+ * device copies, FP32 GEMMs; gemm = 1: the two dense layers on BF16 copies of
+ * W2 and Wout (and BF16 activations) with FP32 accumulation; gemm = 2: the
+ * FP64 weights (the *_64 pointers) and FP64 arithmetic throughout, the
+ * reference's float64 model.  The walker state z1 ([B][hidden]) is float for
+ * gemm 0 / 1 and double for gemm 2.  In BF16 mode (path 0) the forward of
+ * configurations (vmcx_nnbf_forward, the connected configurations of
+ * vmcx_nnbf_local_energy) runs both dense layers as the library's own
+ * matrix-core GEMMs and forms only the occupied orbital rows of the output
+ * layer (the determinant's matrix, FP32) instead of the whole backflow
+ * correction; path 1 keeps the rocBLAS GEMMs (sweeps and log-derivatives use
+ * them in either path).
  * Replaces: NNBF_GPU (GPU/models/NNBF.py:36-162).
  */
 typedef struct vmcx_nnbf_model {
@@ -299,11 +365,12 @@ typedef struct vmcx_nnbf_model {
   int32_t n_f;      /* electrons <= 64 */
   int32_t hidden;   /* width of both hidden layers */
   int32_t act;      /* 0 = ReLU, 1 = tanh */
-  int32_t gemm;     /* 0 = FP32 GEMMs, 1 = BF16 operands, FP32 accumulation */
-  int32_t reserved;
+  int32_t gemm;     /* 0 = FP32 GEMMs, 1 = BF16 operands, FP32 accumulation, 2 = FP64 */
+  int32_t path;     /* gemm 1: 0 = the library's own matrix-core layers, 1 = rocBLAS */
   double bf_scale;
-  const float *M, *W1, *b1, *W2, *b2, *Wout, *bout;
-  const void *W2_bf16, *Wout_bf16;
+  const float *M, *W1, *b1, *W2, *b2, *Wout, *bout;   /* gemm 0 / 1 */
+  const void *W2_bf16, *Wout_bf16;                     /* gemm 1 */
+  const double *M64, *W1_64, *b1_64, *W2_64, *b2_64, *Wout_64, *bout_64;  /* gemm 2 */
 } vmcx_nnbf_model;
 
 /* Workspace bytes for B walkers, forward chunks of `chunk` configurations
@@ -316,19 +383,20 @@ int64_t vmcx_nnbf_workspace_bytes(const vmcx_nnbf_model *m, int64_t B, int64_t c
  * Replaces: NNBF_GPU.forward_log via _base.vamp_log (GPU/models/_base.py:111-130). */
 int vmcx_nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *configs,
                       void *workspace, int64_t ws_bytes, int64_t chunk, double *sign,
-                      double *logabs, float *z1_out, void *stream);
+                      double *logabs, void *z1_out, void *stream);
 
 /* Metropolis sweeps for NNBF walkers: per edge, all walkers propose (same
  * proposal table and draws as vmcx_slater_sweep), the dense layers and the
  * determinant are evaluated for the proposed configurations, and accepted
- * moves update configs, z1 ([B][hidden] FP32), sign and logabs in place.
+ * moves update configs, z1 ([B][hidden], float or double as above), sign and
+ * logabs in place.
  * edges_host: HOST array [n_edges][2] int32.
  * Replaces: MetropolisExchangeSpinfulSamplerGPU.step with an NNBF model
  * (GPU/sampler.py:216-388). */
 int vmcx_nnbf_sweep(const vmcx_nnbf_model *m, int64_t B, int n_edges,
                     const int32_t *edges_host, double hopping_rate, int n_sweeps,
                     uint64_t seed, int64_t sweep_offset, const double *u_choice,
-                    const uint8_t *bits, const double *u_acc, int64_t *configs, float *z1,
+                    const uint8_t *bits, const double *u_acc, int64_t *configs, void *z1,
                     double *sign, double *logabs, int64_t *n_accept, void *workspace,
                     int64_t ws_bytes, void *stream);
 
@@ -385,8 +453,10 @@ int vmcx_sgd_update(int64_t n, double *params, const double *dp, double lr,
  *   Grassmann tensor network with leg order (l, u, d, r).  Contracted
  *   exactly (2 <= Lx <= 4, Ly <= 8, D <= 4: boundary bonds up to D^2).
  *   MLP (hidden > 0): n = [up | down] occupations (2N), two hidden layers
- *   act(W1 n + b1), act(W2 h + b2), output Wo h + bo (n_params), FP64,
- *   row-major [out][in] weights.
+ *   act(W1 n + b1), act(W2 h + b2), output y = Wo h + bo (n_params), FP64,
+ *   row-major [out][in] weights; theta(x) = theta + eta y (mode 0, the
+ *   reference's additive backflow) or theta * (1 + eta y) (mode 1, the
+ *   multiplicative site-tensor correction; no reference form).
  *   O layout (vmcx_peps_logderiv): [theta (n_params) | W1 | b1 | W2 | b2 | Wo
  *   | bo] (the MLP part only with hidden > 0).
  * Replaces: fPEPS_Model_GPU (GPU/models/pureTNS.py:24-156) and
@@ -402,6 +472,8 @@ typedef struct vmcx_peps_model {
   int32_t act;    /* 0 = ReLU, 1 = tanh */
   double eta;
   const double *W1, *b1, *W2, *b2, *Wo, *bo;
+  int32_t mode;   /* 0 = additive theta + eta y, 1 = multiplicative theta (1 + eta y) */
+  int32_t reserved;
 } vmcx_peps_model;
 
 /* Number of dense tensor positions (length of dense_map); -1 on bad sizes. */
@@ -441,7 +513,10 @@ int vmcx_peps_logderiv(const vmcx_peps_model *m, int64_t B, const int64_t *confi
  * row s of F, [B][ld] float64, ld >= n_params + 2N + 4 hidden, is
  * [g | n | dz1 | h1 | dz2 | h2] with g = d ln|psi| / d theta_eff; plain fPEPS
  * (hidden 0): g alone, ld >= n_params).  O = [g, dz1 n^T, dz1, dz2 h1^T, dz2,
- * eta g h2^T, eta g], the blocks vmcx_sr_minsr_factored consumes.  Also
+ * eta g h2^T, eta g], the blocks vmcx_sr_minsr_factored consumes.  Mode 1
+ * (multiplicative): row s is [g (1 + eta y) | n | dz1 | h1 | dz2 | h2 |
+ * g theta] (ld >= 2 n_params + 2N + 4 hidden) and O = [g (1 + eta y), dz1 n^T,
+ * dz1, dz2 h1^T, dz2, eta (g theta) h2^T, eta g theta].  Also
  * writes sign / logabs.  Replaces: compute_grads_gpu for NN-fTNS
  * (GPU/vmc_utils.py:1389-1634, GPU/models/NNfTNS.py:280-955) when the SR
  * step takes the factored form. */
@@ -494,17 +569,20 @@ int vmcx_lp_propose_given(const float *params, int C, int Lx, int Ly, int64_t B,
                           const int64_t *configs, const double *logits_x, uint64_t seed,
                           int64_t step, const double *u_choice, double *logits_p,
                           int32_t *move, double *logq_fwd, int64_t *proposed, double *logq_rev,
-                          int precision, void *stream);
+                          int precision, const int64_t *step_dev, void *stream);
 int vmcx_lp_commit_logits(int64_t B, int n_moves, const uint8_t *accepted,
                           const double *logits_p, double *logits_x, void *stream);
 /* Metropolis-Hastings: accept when u_acc < |psi'|^2/|psi|^2 q(m|x')/q(m|x),
  * from (psign, plogabs) of the proposed configurations; commits configs,
- * sign, logabs, n_accept (may be NULL) and accepted[B] (may be NULL). */
+ * sign, logabs, n_accept (may be NULL) and accepted[B] (may be NULL).
+ * step_dev (may be NULL; also in vmcx_lp_propose_given): a device int64
+ * added to step when the kernels run (draws advanced by vmcx_counter_add
+ * inside a captured graph). */
 int vmcx_lp_accept(int64_t B, int n_sites, uint64_t seed, int64_t step, const double *u_acc,
                    const int32_t *move, const int64_t *proposed, const double *psign,
                    const double *plogabs, const double *logq_fwd, const double *logq_rev,
                    int64_t *configs, double *sign, double *logabs, int64_t *n_accept,
-                   uint8_t *accepted, void *stream);
+                   uint8_t *accepted, const int64_t *step_dev, void *stream);
 /* grad[n_params] = sum_b d(sum_m dlogits[b][m] logits[b][m]) / d params
  * (FP32; per-configuration rows in `workspace`, summed in a fixed order). */
 int64_t vmcx_lp_backward_workspace_bytes(int C, int64_t B);

@@ -99,14 +99,19 @@ def init_params(rng, n_sites, n_f, hidden, n_layers, m_std=0.1, xavier_out=True)
     return params
 
 
-def local_energy(H, params, x, n_layers, bf_scale, act="relu"):
+def local_energy(H, params, x, n_layers, bf_scale, act="relu", bf16_gemm=False,
+                 chunk=None):
     """E_loc with the GPU term-major get_conn and NNBF amplitudes
-    (GPU/vmc_utils.py:322-476)."""
+    (GPU/vmc_utils.py:322-476); bf16_gemm as in forward.  chunk: walkers
+    per pass (bounds the memory of the connected amplitudes)."""
     x = np.atleast_2d(x)
-    s0, l0, _ = forward(params, x, n_layers, bf_scale, act)
+    if chunk is not None and x.shape[0] > chunk:
+        return np.concatenate([local_energy(H, params, x[i:i + chunk], n_layers, bf_scale, act,
+                                            bf16_gemm) for i in range(0, x.shape[0], chunk)])
+    s0, l0, _ = forward(params, x, n_layers, bf_scale, act, bf16_gemm)
     etas, coeffs, bids = H.get_conn_batch(x)
     E = np.zeros(x.shape[0])
     if etas.shape[0]:
-        s1, l1, _ = forward(params, etas, n_layers, bf_scale, act)
+        s1, l1, _ = forward(params, etas, n_layers, bf_scale, act, bf16_gemm)
         np.add.at(E, bids, coeffs * s1 * s0[bids] * np.exp(l1 - l0[bids]))
     return E

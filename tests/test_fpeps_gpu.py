@@ -25,12 +25,15 @@ def _plain(Lx, Ly, D, seed, device):
     return m, sh, th
 
 
-def _nn(Lx, Ly, D, H, seed, device, eta=0.8, act="relu"):
+def _nn(Lx, Ly, D, H, seed, device, eta=0.8, act="relu", correction="additive"):
     sh = fp.PEPSShape(Lx, Ly, D)
     rng = np.random.default_rng(seed)
     th = fp.random_theta(sh, rng)
     nn = fp.mlp_init(rng, 2 * sh.N, H, sh.n_params, 0.2)
-    m = NNfTNS_Model_GPU(Lx, Ly, D, hidden_dim=H, nn_eta=eta, activation=act, theta=th)
+    if correction == "multiplicative":
+        nn[5] = rng.normal(0.0, 0.3, size=sh.n_params)     # nonzero output bias
+    m = NNfTNS_Model_GPU(Lx, Ly, D, hidden_dim=H, nn_eta=eta, activation=act, theta=th,
+                         correction=correction)
     with torch.no_grad():
         for p, v in zip(m.params[1:], nn):
             p.copy_(torch.as_tensor(v))
@@ -157,3 +160,67 @@ def test_config4_shape(device):
     nn = [p.detach().cpu().numpy() for p in m.params[1:]]
     want = np.array([fp.nnftn_amplitude(sh, th, nn, 1.0, c) for c in x[:4]])
     assert np.abs(l.cpu().numpy()[:4] - np.log(np.abs(want))).max() < 1e-10
+
+
+MODE = {"additive": "add", "multiplicative": "mul"}
+
+
+@pytest.mark.parametrize("act", ["relu", "tanh"])
+def test_nnftn_multiplicative_matches_oracle(device, act):
+    """theta(x) = theta (1 + eta MLP(n(x))): amplitudes, FP64 O (dense and as
+    factors multiplied out), local energies and a tie-free sweep against the
+    oracle's multiplicative form (1e-10 / 1e-9, bit-exact configurations)."""
+    m, sh, th, nn = _nn(2, 3, 2, 8, 14, device, act=act, correction="multiplicative")
+    x = random_configs(np.random.default_rng(15), 12, sh.N, 3, 3)
+    xd = torch.tensor(x, device=device)
+    s, l = m.forward_log(xd)
+    want = np.array([fp.nnftn_amplitude(sh, th, nn, m.nn_eta, c, act, "mul") for c in x])
+    assert np.array_equal(s.cpu().numpy(), np.sign(want))
+    assert np.abs(l.cpu().numpy() - np.log(np.abs(want))).max() < 1e-10
+    O, _ = m.log_derivative(xd, dtype=torch.float64)
+    FO, _ = m.log_derivative_factored(xd)
+    Of = FO.materialize().cpu().numpy()
+    O = O.cpu().numpy()
+    for k in range(len(x)):
+        g = fp.nnftn_log_derivative(sh, th, nn, m.nn_eta, x[k], act, "mul")
+        sc = np.abs(g).max()
+        assert np.abs(O[k] - g).max() < 1e-10 * sc
+        assert np.abs(Of[k] - g).max() < 1e-10 * sc
+    Hm = spinful_Fermi_Hubbard_square_lattice_torch(2, 3, 1.0, 8.0, 6, n_fermions_per_spin=(3, 3))
+    E = m.local_energy(xd, Hm).cpu().numpy()
+    Eo = fp.local_energy(ohub.Hubbard(2, 3, 1.0, 8.0, 3, 3), sh, th, x, nn, m.nn_eta, act, "mul")
+    assert np.allclose(E, Eo, rtol=1e-9, atol=1e-9 * np.abs(Eo).max())
+    f = lambda c: fp.nnftn_amplitude(sh, th, nn, m.nn_eta, c, act, "mul")   # noqa: E731
+    edges = Hm.graph.sweep_edges()
+    rng = np.random.default_rng(16)
+    x0 = random_configs(rng, 24, sh.N, 3, 3)
+    draws, xo, so, lo = tie_free_draws(_oracle_logamp(f), x0, edges, 2, rng)
+    xd0 = torch.tensor(x0, device=device)
+    dd = tuple(torch.tensor(a, device=device) for a in draws)
+    sign, logabs, _ = m.mcmc_sweep(xd0, Hm.graph.edge_tensor(device, "sweep"), 2, 0, 0,
+                                   draws=dd)
+    assert np.array_equal(xd0.cpu().numpy(), xo)
+    assert np.array_equal(sign.cpu().numpy(), so)
+    assert np.abs(logabs.cpu().numpy() - lo).max() < 1e-10
+
+
+def test_nnftn_multiplicative_config4_factored_sr(device):
+    """The 4x4 D = 4 configuration with the multiplicative correction: the
+    factored SR step equals the dense FP64 step on the same O (1e-7, as the
+    additive form's test) and the parameter count is that of the additive
+    form (the correction changes the map, not the parameters)."""
+    from vmc_torch_amd import kernels
+    m = NNfTNS_Model_GPU(4, 4, 4, hidden_dim=32, nn_eta=0.5, correction="multiplicative",
+                         generator=torch.Generator().manual_seed(5)).to(device)
+    x = torch.tensor(random_configs(np.random.default_rng(17), 200, 16, 8, 8), device=device)
+    O, (s, l) = m.log_derivative(x, dtype=torch.float64)
+    FO, (sf, lf) = m.log_derivative_factored(x)
+    assert torch.equal(s, sf) and torch.equal(l, lf)
+    Of = FO.materialize()
+    assert float((Of - O).abs().max()) <= 1e-13 * float(O.abs().max())
+    E = torch.tensor(np.random.default_rng(18).normal(size=200) + 1.0, device=device)
+    dp, _, info = kernels.sr_minsr_factored(FO, E, None, 1e-3)
+    dpd, _, _ = kernels.sr_minsr(Of.contiguous(), E, None, 1e-3, "fp64")
+    assert int(info.item()) == 0
+    rel = float((dp - dpd).norm() / dpd.norm())
+    assert rel < 1e-7

@@ -18,7 +18,7 @@ def test_library_exports_header_symbols():
 
 def test_abi_version_and_nfp():
     lib = _lib.lib()
-    assert lib.vmcx_abi_version() == 12
+    assert lib.vmcx_abi_version() == 15
     assert [lib.vmcx_slater_nfp(n) for n in (1, 8, 9, 16, 17, 32, 33, 64, 65)] == \
         [8, 8, 16, 16, 32, 32, 64, 64, -1]
 
@@ -38,5 +38,5 @@ def test_sweep_rejects_partial_injected_draws():
     lib = _lib.lib()
     p = ctypes.c_void_p(16)  # never dereferenced: the check fails first
     rc = lib.vmcx_slater_sweep(1, 4, 2, p, 1, p, 0.25, 1, 0, 0, p, None, None, 0,
-                               p, p, p, p, p, p, None, None)
+                               p, p, p, p, p, p, None, None, None)
     assert rc == 1 and "injected" in lib.vmcx_last_error().decode()

@@ -1,8 +1,12 @@
 """GPU parity of the NNBF path (libvmcx through vmc_torch_amd.NNBF_GPU)
 against the FP64 oracle (oracle/nnbf.py).
 
-The MLP runs in FP32 (gemm="fp32") or BF16 with FP32 accumulation
-(gemm="bf16"); the determinant is FP64.  Tolerances, stated per test:
+The MLP runs in FP64 (gemm="fp64", the reference's float64 model), FP32
+(gemm="fp32") or BF16 with FP32 accumulation (gemm="bf16"); the determinant
+is FP64.  Tolerances, stated per test:
+  FP64 MLP : ln|psi| 1e-10 absolute, O and E_loc 1e-9 relative (the same
+             arithmetic in another order), sweeps bit-exact in
+             configurations and signs;
   FP32 MLP : ln|psi| within 2e-4 absolute, O within 2e-3 of max|O| per
              parameter block, E_loc within 1e-3 relative;
   BF16 MLP : against the oracle with the same BF16 operand rounding,
@@ -136,3 +140,136 @@ def test_nnbf_config3_shape(device):
     so, lo, _ = onn.forward(params, x, 2, m.bf_scale, bf16_gemm=True)
     assert np.abs(l.cpu().numpy() - lo).max() < 2e-4
     assert bool(torch.isfinite(O).all())
+
+
+def test_nnbf_8x8_bf16_target_shape(device):
+    """The target's shape: 8x8 U=8, 32 up + 32 down, NNBF 128 -> 256 -> 256
+    -> 8192 (ReLU, bf_scale 0.01, Xavier, M ~ N(0, 0.1^2)), BF16 GEMMs with
+    FP32 accumulation.  Against the oracle with the same BF16 operand
+    rounding (64 walkers): identical signs, ln|psi| within 2e-4, E_loc within
+    1e-3 relative.  Against the FP64 oracle (the FP64 weights, no rounding)
+    on 1024 walkers sampled from |psi|^2 (30 sweeps of the plain sampler
+    from random configurations): the energy estimate per site within 1e-4
+    (the north-star bound |dE/site|); the per-sample differences are printed.
+    (On unsampled random configurations near a node of psi a single E_loc can
+    move by O(1): the bound is on the estimate over sampled walkers.)"""
+    Lx, Ly, nu, nd = 8, 8, 32, 32
+    N = 64
+    m = NNBF_GPU(N, nu + nd, hidden_dim=256, activation="relu", bf_scale=0.01, gemm="bf16",
+                 init="xavier", m_std=0.1, generator=torch.Generator().manual_seed(3)).to(device)
+    p64 = [p.detach().cpu().numpy().copy() for p in m.params]
+    p32 = [p.detach().to(torch.float32).double().cpu().numpy() for p in m.params]
+    Hd = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, nu + nd,
+                                                    n_fermions_per_spin=(nu, nd))
+    Ho = ohub.Hubbard(Lx, Ly, 1.0, 8.0, nu, nd)
+    xd = torch.tensor(random_configs(np.random.default_rng(30), 1024, N, nu, nd), device=device)
+    m.mcmc_sweep(xd, Hd.graph.edge_tensor(device, "sweep"), 30, 7, 0)
+    x = xd.cpu().numpy()
+    s, l = m.forward_log(xd[:64])
+    so, lo, _ = onn.forward(p32, x[:64], 2, 0.01, bf16_gemm=True)
+    assert np.array_equal(s.cpu().numpy(), so)
+    assert np.abs(l.cpu().numpy() - lo).max() < 2e-4
+    E = m.local_energy(xd, Hd).cpu().numpy()
+    Eb = onn.local_energy(Ho, p32, x[:64], 2, 0.01, bf16_gemm=True, chunk=16)
+    assert np.allclose(E[:64], Eb, rtol=1e-3, atol=1e-3 * np.abs(Eb).max())
+    E64 = onn.local_energy(Ho, p64, x, 2, 0.01, chunk=64)
+    d = np.abs(E - E64) / N
+    dE = abs(E.mean() - E64.mean()) / N
+    print(f"  |dE/site| vs FP64 oracle over {len(x)} sampled walkers: {dE:.2e} "
+          f"(per sample: mean {d.mean():.2e}, median {np.median(d):.2e}, max {d.max():.2e})",
+          flush=True)
+    assert dE < 1e-4
+
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_nnbf_fp64_matches_oracle(device, case):
+    """gemm="fp64": every layer in FP64, so the device equals the FP64 oracle
+    to rounding: ln|psi| within 1e-10, O (dense and as factors multiplied
+    out) within 1e-9 of its largest entry per parameter block, E_loc within
+    1e-9 relative, and a tie-free sweep bit-exact."""
+    Lx, Ly, nu, nd, hid = case
+    m, _ = _model(Lx, Ly, nu, nd, hid, "fp64", device)
+    assert m.gemm == "fp64" and m.z_dtype == torch.float64
+    params = [p.detach().cpu().numpy() for p in m.params]
+    N = Lx * Ly
+    x = random_configs(np.random.default_rng(11), 32, N, nu, nd)
+    xd = torch.tensor(x, device=device)
+    s, l = m.forward_log(xd)
+    so, lo, _ = onn.forward(params, x, 2, m.bf_scale)
+    assert np.array_equal(s.cpu().numpy(), so)
+    assert np.abs(l.cpu().numpy() - lo).max() < 1e-10
+    O, _ = m.log_derivative(xd, dtype=torch.float64)
+    FO, _ = m.log_derivative_factored(xd)
+    Oo = onn.log_derivative(params, x, 2, m.bf_scale)
+    for got in (O.cpu().numpy(), FO.materialize().cpu().numpy()):
+        off = 0
+        for p in params:
+            blk = slice(off, off + p.size)
+            sc = max(np.abs(Oo[:, blk]).max(), 1e-300)
+            assert np.abs(got[:, blk] - Oo[:, blk]).max() <= 1e-9 * sc, p.shape
+            off += p.size
+    H = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, nu + nd,
+                                                   n_fermions_per_spin=(nu, nd))
+    E = m.local_energy(xd, H).cpu().numpy()
+    Eo = onn.local_energy(ohub.Hubbard(Lx, Ly, 1.0, 8.0, nu, nd), params, x, 2, m.bf_scale)
+    assert np.allclose(E, Eo, rtol=1e-9, atol=1e-9 * np.abs(Eo).max())
+    edges = H.graph.sweep_edges()
+    rng = np.random.default_rng(12)
+    x0 = random_configs(rng, 32, N, nu, nd)
+    draws, xo, so, lo = tie_free_draws(
+        lambda c: onn.forward(params, c, 2, m.bf_scale)[:2], x0, edges, 2, rng)
+    xd0 = torch.tensor(x0, device=device)
+    dd = tuple(torch.tensor(a, device=device) for a in draws)
+    sign, logabs, _ = m.mcmc_sweep(xd0, H.graph.edge_tensor(device, "sweep"), 2, 0, 0,
+                                   draws=dd)
+    assert np.array_equal(xd0.cpu().numpy(), xo)
+    assert np.array_equal(sign.cpu().numpy(), so)
+    assert np.abs(logabs.cpu().numpy() - lo).max() < 1e-10
+    z1 = m._state["z1"].clone()
+    m.invalidate()
+    assert torch.allclose(z1, m.walker_state(xd0)["z1"], rtol=0, atol=1e-12)
+
+
+def test_nnbf_default_is_fp64():
+    """The reference's NNBF_GPU defaults to float64 (NNBF.py:36-162): so does
+    the drop-in, with FP64 arithmetic unless a reduced-precision mode is
+    asked for."""
+    assert NNBF_GPU(4, 4, hidden_dim=8).gemm == "fp64"
+    assert NNBF_GPU(4, 4, hidden_dim=8, gemm="bf16").gemm == "bf16"
+    with pytest.raises(ValueError):
+        NNBF_GPU(4, 4, hidden_dim=8, dtype=torch.float32, gemm="fp64")
+
+
+
+@pytest.mark.parametrize("case", [(8, 8, 32, 32, 256), (6, 6, 16, 16, 256), (4, 4, 8, 8, 256),
+                                  (6, 6, 18, 17, 64), (4, 2, 4, 4, 32), (3, 3, 4, 3, 48)])
+def test_nnbf_bf16_matrix_core_layers(device, case):
+    """The library's own BF16 matrix-core layers (only the occupied orbital
+    rows of the output layer formed) against the rocBLAS GEMMs on the same
+    BF16 operands: identical signs and ln|psi| within 1e-4 (FP32 accumulation
+    in another order); against the BF16-rounding oracle, identical signs,
+    ln|psi| within 2e-5 on average and 2e-3 at most.  The oracle rounds
+    FP64 pre-activations to BF16 where the GPU rounds FP32 ones: an
+    activation that lies within FP32 rounding of a BF16 rounding boundary
+    rounds the other way (one BF16 ulp, 2^-8 relative), which moves one
+    configuration's ln|psi| by up to a few 1e-4 at bf_scale 0.3 on 8x8.
+    Shapes cover n_f a multiple of 4 and not (35), hidden widths 256 / 64 /
+    48 / 32 and a last partial block of 64 configurations."""
+    Lx, Ly, nu, nd, hid = case
+    m, params = _model(Lx, Ly, nu, nd, hid, "bf16", device, scale=0.3)
+    m_lib = NNBF_GPU(Lx * Ly, nu + nd, hidden_dim=hid, activation="relu", bf_scale=0.3,
+                     gemm="bf16", bf16_layers="rocblas").to(device)
+    with torch.no_grad():
+        for a, b in zip(m_lib.params, m.params):
+            a.copy_(b)
+    x = random_configs(np.random.default_rng(40), 100, Lx * Ly, nu, nd)
+    xd = torch.tensor(x, device=device)
+    s1, l1 = m.forward_log(xd)
+    s2, l2 = m_lib.forward_log(xd)
+    assert torch.equal(s1, s2)
+    assert (l1 - l2).abs().max().item() < 1e-4
+    so, lo, _ = onn.forward(params, x, 2, m.bf_scale, bf16_gemm=True)
+    assert np.array_equal(s1.cpu().numpy(), so)
+    d = np.abs(l1.cpu().numpy() - lo)
+    assert d.mean() < 2e-5 and d.max() < 2e-3, (float(d.mean()), float(d.max()))

@@ -25,12 +25,22 @@ _SIGNATURES = {
     "vmcx_abi_version": (_I, []),
     "vmcx_last_error": (_c.c_char_p, []),
     "vmcx_slater_nfp": (_I, [_I]),
+    "vmcx_set_launch_waves": (_I, [_I, _I, _I]),
+    "vmcx_get_launch_waves": (_I, [_I, _I]),
     "vmcx_slater_refresh": (_I, [_I64, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P]),
     "vmcx_slater_sweep": (_I, [_I64, _I, _I, _P, _I, _P, _D, _I, _U64, _I64,
-                               _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+                               _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "vmcx_counter_add": (_I, [_P, _I64, _P]),
     "vmcx_slater_local_energy": (_I, [_I64, _I, _I, _P, _I, _P, _D, _D, _P, _P,
                                       _P, _P, _P]),
     "vmcx_slater_logderiv": (_I, [_I64, _I, _I, _P, _P, _P, _I, _I64, _P]),
+    "vmcx_slater_refresh_f32": (_I, [_I64, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P]),
+    "vmcx_slater_sweep_f32": (_I, [_I64, _I, _I, _P, _P, _I, _P, _D, _I, _U64, _I64,
+                                   _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "vmcx_slater_local_energy_f32": (_I, [_I64, _I, _I, _P, _I, _P, _D, _D, _P, _P,
+                                          _P, _P, _P]),
+    "vmcx_slater_logderiv_f32": (_I, [_I64, _I, _I, _P, _P, _P, _I, _I64, _P]),
+    "vmcx_sgd_update_f32": (_I, [_I64, _P, _P, _D, _P]),
     "vmcx_hubbard_conn_count": (_I, [_I64, _I, _I, _P, _P, _P, _P]),
     "vmcx_hubbard_conn_fill": (_I, [_I64, _I, _I, _P, _D, _D, _P, _P, _P, _P,
                                     _P, _P]),
@@ -66,13 +76,13 @@ _SIGNATURES = {
     "vmcx_lp_propose": (_I, [_P, _I, _I, _I, _I64, _P, _U64, _I64, _P, _P, _P, _P, _P, _P, _I,
                              _P]),
     "vmcx_lp_accept": (_I, [_I64, _I, _U64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                            _P, _P]),
+                            _P, _P, _P]),
     "vmcx_lp_backward_workspace_bytes": (_I64, [_I, _I64]),
     "vmcx_lp_backward": (_I, [_P, _I, _I, _I, _I64, _P, _P, _P, _I64, _P, _P]),
     "vmcx_lp_candidates": (_I, [_I64, _I, _I, _P, _P, _P]),
     "vmcx_lp_axpy": (_I, [_I64, _c.c_float, _P, _P, _P]),
     "vmcx_lp_propose_given": (_I, [_P, _I, _I, _I, _I64, _P, _P, _U64, _I64, _P, _P, _P, _P,
-                                   _P, _P, _I, _P]),
+                                   _P, _P, _I, _P, _P]),
     "vmcx_lp_commit_logits": (_I, [_I64, _I, _P, _P, _P, _P]),
     "vmcx_sym_images": (_I, [_I64, _I, _I, _P, _P, _P, _P, _P, _P]),
     "vmcx_peps_dense_size": (_I64, [_P]),
@@ -91,10 +101,12 @@ class NnbfModelStruct(ctypes.Structure):
     """vmcx_nnbf_model (include/vmcx.h)."""
     _fields_ = [("n_sites", ctypes.c_int32), ("n_f", ctypes.c_int32),
                 ("hidden", ctypes.c_int32), ("act", ctypes.c_int32),
-                ("gemm", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("gemm", ctypes.c_int32), ("path", ctypes.c_int32),
                 ("bf_scale", ctypes.c_double),
                 ("M", _P), ("W1", _P), ("b1", _P), ("W2", _P), ("b2", _P),
-                ("Wout", _P), ("bout", _P), ("W2_bf16", _P), ("Wout_bf16", _P)]
+                ("Wout", _P), ("bout", _P), ("W2_bf16", _P), ("Wout_bf16", _P),
+                ("M64", _P), ("W1_64", _P), ("b1_64", _P), ("W2_64", _P), ("b2_64", _P),
+                ("Wout_64", _P), ("bout_64", _P)]
 
 
 class SrBlockStruct(ctypes.Structure):
@@ -109,7 +121,8 @@ class PepsModelStruct(ctypes.Structure):
     _fields_ = [("Lx", ctypes.c_int32), ("Ly", ctypes.c_int32), ("D", ctypes.c_int32),
                 ("n_params", ctypes.c_int32), ("dense_map", _P), ("theta", _P),
                 ("hidden", ctypes.c_int32), ("act", ctypes.c_int32), ("eta", ctypes.c_double),
-                ("W1", _P), ("b1", _P), ("W2", _P), ("b2", _P), ("Wo", _P), ("bo", _P)]
+                ("W1", _P), ("b1", _P), ("W2", _P), ("b2", _P), ("Wo", _P), ("bo", _P),
+                ("mode", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 _LIB = None

@@ -29,6 +29,29 @@ size_t max_lds_per_block() {
   return cache[dev];
 }
 
+// launch-shape table: (kind, key) -> waves per workgroup (1, 2, 4 or 8)
+static std::mutex g_tune_mu;
+static std::vector<std::vector<int>> g_tune(TUNE_KINDS);
+
+int tuned_waves(int kind, int key, int dflt) {
+  if (kind < 0 || kind >= TUNE_KINDS || key < 0) return dflt;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const std::vector<int> &t = g_tune[kind];
+  return (key < (int)t.size() && t[key] > 0) ? t[key] : dflt;
+}
+
+int set_tuned_waves(int kind, int key, int waves) {
+  VMCX_CHECK_ARG(kind >= 0 && kind < TUNE_KINDS, "unknown kernel kind");
+  VMCX_CHECK_ARG(key >= 0 && key < 1024, "key out of range");
+  VMCX_CHECK_ARG(waves == 0 || waves == 1 || waves == 2 || waves == 4 || waves == 8,
+                 "waves per workgroup must be 0 (default), 1, 2, 4 or 8");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  std::vector<int> &t = g_tune[kind];
+  if ((int)t.size() <= key) t.resize(key + 1, 0);
+  t[key] = waves;
+  return 0;
+}
+
 }  // namespace vmcx
 
 static inline int status_of(int rc) {
@@ -37,11 +60,17 @@ static inline int status_of(int rc) {
 
 extern "C" {
 
-int vmcx_abi_version(void) { return 12; }
+int vmcx_abi_version(void) { return 15; }
 
 const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
 
 int vmcx_slater_nfp(int n_f) { return (n_f >= 1) ? vmcx::nfp_of(n_f) : -1; }
+
+int vmcx_set_launch_waves(int kind, int key, int waves) {
+  return status_of(vmcx::set_tuned_waves(kind, key, waves));
+}
+
+int vmcx_get_launch_waves(int kind, int key) { return vmcx::tuned_waves(kind, key, 0); }
 
 int vmcx_slater_refresh(int64_t B, int n_sites, int n_f, const double *M,
                         const int64_t *configs, double *ainv, uint8_t *slot,
@@ -58,8 +87,10 @@ int vmcx_slater_sweep(int64_t B, int n_sites, int n_f, const double *M,
                       const double *u_choice, const uint8_t *bits,
                       const double *u_acc, int refresh_every, int64_t *configs,
                       double *ainv, uint8_t *slot, double *sign, double *logabs,
-                      int32_t *since_refresh, int64_t *n_accept, void *stream) {
+                      int32_t *since_refresh, int64_t *n_accept,
+                      const int64_t *sweep_offset_dev, void *stream) {
   vmcx::SweepArgs a;
+  a.offset_dev = sweep_offset_dev;
   a.B = B;
   a.n_sites = n_sites;
   a.n_f = n_f;
@@ -81,6 +112,69 @@ int vmcx_slater_sweep(int64_t B, int n_sites, int n_f, const double *M,
   a.since_refresh = since_refresh;
   a.n_accept = n_accept;
   return status_of(vmcx::slater_sweep(a, refresh_every, (hipStream_t)stream));
+}
+
+int vmcx_slater_refresh_f32(int64_t B, int n_sites, int n_f, const double *M,
+                            const int64_t *configs, float *ainv, uint8_t *slot, double *sign,
+                            double *logabs, int32_t *since_refresh, int only_if_since,
+                            void *stream) {
+  return status_of(vmcx::slater_refresh_f32(B, n_sites, n_f, M, configs, ainv, slot, sign,
+                                            logabs, since_refresh, only_if_since,
+                                            (hipStream_t)stream));
+}
+
+int vmcx_slater_sweep_f32(int64_t B, int n_sites, int n_f, const double *M, const float *M32,
+                          int n_edges, const int32_t *edges, double hopping_rate, int n_sweeps,
+                          uint64_t seed, int64_t sweep_offset, const double *u_choice,
+                          const uint8_t *bits, const double *u_acc, int refresh_every,
+                          int64_t *configs, float *ainv, uint8_t *slot, double *sign,
+                          double *logabs, int32_t *since_refresh, int64_t *n_accept,
+                          const int64_t *sweep_offset_dev, void *stream) {
+  vmcx::SweepArgs a;
+  a.offset_dev = sweep_offset_dev;
+  a.B = B;
+  a.n_sites = n_sites;
+  a.n_f = n_f;
+  a.n_edges = n_edges;
+  a.n_sweeps = n_sweeps;
+  a.M = M;
+  a.M32 = M32;
+  a.edges = edges;
+  a.keep_rate = 1.0 - hopping_rate;
+  a.seed = seed;
+  a.sweep_offset = sweep_offset;
+  a.u_choice = u_choice;
+  a.bits = bits;
+  a.u_acc = u_acc;
+  a.configs = configs;
+  a.ainv = nullptr;
+  a.ainv32 = ainv;
+  a.slot = slot;
+  a.sign = sign;
+  a.logabs = logabs;
+  a.since_refresh = since_refresh;
+  a.n_accept = n_accept;
+  return status_of(vmcx::slater_sweep_f32(a, refresh_every, (hipStream_t)stream));
+}
+
+int vmcx_slater_local_energy_f32(int64_t B, int n_sites, int n_f, const float *M32,
+                                 int n_edges, const int32_t *edges, double t, double U,
+                                 const int64_t *configs, const float *ainv,
+                                 const uint8_t *slot, double *eloc, void *stream) {
+  return status_of(vmcx::slater_local_energy_f32(B, n_sites, n_f, M32, n_edges, edges, t, U,
+                                                 configs, ainv, slot, eloc,
+                                                 (hipStream_t)stream));
+}
+
+int vmcx_slater_logderiv_f32(int64_t B, int n_sites, int n_f, const float *ainv,
+                             const uint8_t *slot, void *O, int o_dtype, int64_t ld,
+                             void *stream) {
+  return status_of(vmcx::slater_logderiv_f32(B, n_sites, n_f, ainv, slot, O, o_dtype, ld,
+                                             (hipStream_t)stream));
+}
+
+int vmcx_sgd_update_f32(int64_t n, float *params, const double *dp, double lr, void *stream) {
+  return status_of(vmcx::sgd_update_f32(n, params, dp, lr, (hipStream_t)stream));
 }
 
 int vmcx_slater_local_energy(int64_t B, int n_sites, int n_f, const double *M,
@@ -161,7 +255,7 @@ static inline int status3(int rc) { return rc == 3 ? VMCX_ELIB : status_of(rc); 
 
 int vmcx_nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *configs,
                       void *workspace, int64_t ws_bytes, int64_t chunk, double *sign,
-                      double *logabs, float *z1_out, void *stream) {
+                      double *logabs, void *z1_out, void *stream) {
   return status3(vmcx::nnbf_forward(m, B, configs, workspace, ws_bytes, chunk, sign, logabs,
                                     z1_out, (hipStream_t)stream));
 }
@@ -169,7 +263,7 @@ int vmcx_nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *config
 int vmcx_nnbf_sweep(const vmcx_nnbf_model *m, int64_t B, int n_edges,
                     const int32_t *edges_host, double hopping_rate, int n_sweeps,
                     uint64_t seed, int64_t sweep_offset, const double *u_choice,
-                    const uint8_t *bits, const double *u_acc, int64_t *configs, float *z1,
+                    const uint8_t *bits, const double *u_acc, int64_t *configs, void *z1,
                     double *sign, double *logabs, int64_t *n_accept, void *workspace,
                     int64_t ws_bytes, void *stream) {
   return status3(vmcx::nnbf_sweep(m, B, n_edges, edges_host, hopping_rate, n_sweeps, seed,
@@ -264,10 +358,14 @@ int vmcx_lp_accept(int64_t B, int n_sites, uint64_t seed, int64_t step, const do
                    const int32_t *move, const int64_t *proposed, const double *psign,
                    const double *plogabs, const double *logq_fwd, const double *logq_rev,
                    int64_t *configs, double *sign, double *logabs, int64_t *n_accept,
-                   uint8_t *accepted, void *stream) {
+                   uint8_t *accepted, const int64_t *step_dev, void *stream) {
   return status_of(vmcx::lp_accept(B, n_sites, seed, step, u_acc, move, proposed, psign,
                                    plogabs, logq_fwd, logq_rev, configs, sign, logabs, n_accept,
-                                   accepted, (hipStream_t)stream));
+                                   accepted, (hipStream_t)stream, step_dev));
+}
+
+int vmcx_counter_add(int64_t *counter, int64_t inc, void *stream) {
+  return status_of(vmcx::counter_add(counter, inc, (hipStream_t)stream));
 }
 
 int64_t vmcx_lp_backward_workspace_bytes(int C, int64_t B) {
@@ -294,10 +392,10 @@ int vmcx_lp_propose_given(const float *params, int C, int Lx, int Ly, int64_t B,
                           const int64_t *configs, const double *logits_x, uint64_t seed,
                           int64_t step, const double *u_choice, double *logits_p,
                           int32_t *move, double *logq_fwd, int64_t *proposed, double *logq_rev,
-                          int precision, void *stream) {
+                          int precision, const int64_t *step_dev, void *stream) {
   return status_of(vmcx::lp_propose_given(params, C, Lx, Ly, B, configs, logits_x, seed, step,
                                           u_choice, logits_p, move, logq_fwd, proposed,
-                                          logq_rev, precision, (hipStream_t)stream));
+                                          logq_rev, precision, (hipStream_t)stream, step_dev));
 }
 
 int vmcx_lp_commit_logits(int64_t B, int n_moves, const uint8_t *accepted,

@@ -50,6 +50,12 @@ inline int nfp_of(int n_f) {
 // Largest dynamic LDS a workgroup may request on the current device.
 size_t max_lds_per_block();
 
+// Launch-shape table filled by the autotuner (vmcx_set_launch_waves): waves
+// per workgroup of a kernel family for a key (its template width and
+// precision); `dflt` when nothing was set.
+enum { TUNE_SLATER_SWEEP = 0, TUNE_SLATER_ELOC = 1, TUNE_SLATER_REFRESH = 2, TUNE_KINDS = 3 };
+int tuned_waves(int kind, int key, int dflt);
+
 // ----------------------------------------------------------- device side
 // Philox4x32-10 (Salmon et al., SC'11): 10 rounds, multipliers
 // 0xD2511F53 / 0xCD9E8D57, Weyl key increments 0x9E3779B9 / 0xBB67AE85.

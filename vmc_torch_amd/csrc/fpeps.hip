@@ -20,7 +20,9 @@
 // kernel keeps every env_y, sweeps back from the right with the right
 // environment and differentiates Top/Bot, then the rows inside each half.
 //
-// NN-fTNS: theta(x) = theta + eta * MLP(n(x)), n = [up | down] occupations;
+// NN-fTNS: theta(x) = theta + eta * MLP(n(x)) (additive, the reference's
+// form) or theta(x) = theta * (1 + eta * MLP(n(x))) (multiplicative), n = [up |
+// down] occupations;
 // first layer as a gather of W1 columns, the dense layers as rocBLAS DGEMMs,
 // every tensor entry then read per configuration from theta(x).
 #include <hip/hip_runtime.h>
@@ -520,6 +522,33 @@ __global__ void pnn_theta_kernel(int64_t n, int64_t P, double eta, const double 
   }
 }
 
+// multiplicative correction: y[row][k] += bo[k] (the network output, kept for
+// the backward pass) and out[row][k] = theta[k] (1 + eta y[row][k])
+__global__ void pnn_theta_mul_kernel(int64_t n, int64_t P, double eta, const double *theta,
+                                     const double *__restrict__ bo, double *y, double *out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = i % P;
+    const double v = y[i] + bo[k];
+    y[i] = v;
+    out[i] = theta[k] * (1.0 + eta * v);
+  }
+}
+
+// multiplicative correction, backward: from G = d ln|psi| / d theta_eff,
+// y <- G (1 + eta y) (the theta block of O) and G <- G theta (the upstream of
+// the network, whose output enters as eta theta y)
+__global__ void pnn_mul_grad_kernel(int64_t n, int64_t P, double eta, const double *theta,
+                                    double *y, double *G) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = i % P;
+    const double g = G[i];
+    y[i] = g * (1.0 + eta * y[i]);
+    G[i] = g * theta[k];
+  }
+}
+
 // dz[i] = dh[i] * act'(z[i])
 __global__ void pnn_dact_kernel(int64_t n, int act, const double *z, double *dh) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -527,14 +556,16 @@ __global__ void pnn_dact_kernel(int64_t n, int act, const double *z, double *dh)
     dh[i] *= dact_d(z[i], act);
 }
 
-// O row: [g (P) | W1 (H x 2N) | b1 | W2 (H x H) | b2 | Wo (P x H) | bo (P)]
+// O row: [gth (P) | W1 (H x 2N) | b1 | W2 (H x H) | b2 | Wo (P x H) | bo (P)]
 // with dOut = eta g; one block per sample.  FACT: the factors of that row
-// instead, [g | n (2N) | dz1 | h1 | dz2 | h2] (W1 = dz1 n^T, W2 = dz2 h1^T,
-// Wo = eta g h2^T, the biases dz1, dz2, eta g).
+// instead, [gth | n (2N) | dz1 | h1 | dz2 | h2] (W1 = dz1 n^T, W2 = dz2 h1^T,
+// Wo = eta g h2^T, the biases dz1, dz2, eta g), followed by g itself when
+// gth != g (the multiplicative correction).  Additive correction: gth = g = G.
 template <typename TO, bool FACT = false>
 __global__ void __launch_bounds__(256)
     pnn_write_kernel(int64_t rows, int N, int H, int64_t P, int act, double eta,
                      const int64_t *__restrict__ cfg, const double *__restrict__ G,
+                     const double *__restrict__ Gth,
                      const double *__restrict__ z1, const double *__restrict__ z2,
                      const double *__restrict__ dz1, const double *__restrict__ dz2,
                      TO *O, int64_t ld) {
@@ -555,8 +586,9 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   const double *g = G + w * P;
+  const double *gth = Gth + w * P;
   TO *row = O + w * ld;
-  for (int64_t k = threadIdx.x; k < P; k += blockDim.x) row[k] = (TO)g[k];
+  for (int64_t k = threadIdx.x; k < P; k += blockDim.x) row[k] = (TO)gth[k];
   if (FACT) {
     TO *f = row + P;
     for (int o = threadIdx.x; o < n_orb; o += blockDim.x) f[o] = (TO)nvec[o];
@@ -566,6 +598,10 @@ __global__ void __launch_bounds__(256)
       f[H + u] = (TO)h1[u];
       f[2 * H + u] = (TO)d2[u];
       f[3 * H + u] = (TO)h2[u];
+    }
+    if (Gth != G) {
+      f += 4 * H;
+      for (int64_t k = threadIdx.x; k < P; k += blockDim.x) f[k] = (TO)g[k];
     }
     return;
   }
@@ -642,7 +678,7 @@ __global__ void peps_accept_kernel(int64_t B, int N, int i, int j, int64_t *cfg,
 namespace {
 
 struct PepsWs {
-  double *theta_eff, *z1, *z2, *h1, *h2, *G, *d1, *d2;
+  double *theta_eff, *y, *z1, *z2, *h1, *h2, *G, *d1, *d2;
   int32_t *prop;
   uint32_t *moves;
   double *uacc, *psign, *plog;
@@ -667,6 +703,8 @@ PepsWs peps_layout(const vmcx_peps_model &m, int64_t B, int64_t chunk, int n_edg
   const int N = m.Lx * m.Ly;
   const int64_t rows = chunk > B ? chunk : B;   // the sweep evaluates B at once
   L.theta_eff = (double *)take(H ? (size_t)rows * P * 8 : 0);
+  // the network output of the multiplicative correction (kept for its backward)
+  L.y = (double *)take(H && m.mode == 1 ? (size_t)rows * P * 8 : 0);
   L.z1 = (double *)take((size_t)rows * H * 8);
   L.z2 = (double *)take((size_t)rows * H * 8);
   L.h1 = (double *)take((size_t)rows * H * 8);
@@ -700,6 +738,8 @@ int check_peps(const vmcx_peps_model *m) {
   VMCX_CHECK_ARG(m->n_params > 0 && m->dense_map && m->theta, "missing parameters");
   VMCX_CHECK_ARG(m->hidden >= 0 && m->hidden <= 1024, "hidden must be 0..1024");
   VMCX_CHECK_ARG(m->act == 0 || m->act == 1, "act must be 0 (ReLU) or 1 (tanh)");
+  VMCX_CHECK_ARG(m->mode == 0 || m->mode == 1, "mode must be 0 (additive) or 1 (multiplicative)");
+  VMCX_CHECK_ARG(m->mode == 0 || m->hidden > 0, "the multiplicative correction needs hidden > 0");
   if (m->hidden > 0)
     VMCX_CHECK_ARG(m->W1 && m->b1 && m->W2 && m->b2 && m->Wo && m->bo, "missing MLP weights");
   return 0;
@@ -748,13 +788,18 @@ int mlp_theta(const vmcx_peps_model &m, const PepsWs &L, int64_t rows, const int
   hipLaunchKernelGGL(pnn_bias_act_kernel, dim3(gh), dim3(256), 0, st, nh, H, m.act, L.z2, m.b2,
                      L.h2);
   VMCX_LAUNCHED();
+  double *out = m.mode == 1 ? L.y : L.theta_eff;
   PEPS_BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none,
                           (rocblas_int)P, (rocblas_int)rows, H, &one, m.Wo, H, L.h2, H, &zero,
-                          L.theta_eff, (rocblas_int)P));
+                          out, (rocblas_int)P));
   const int64_t nt = rows * P;
   const unsigned gt = (unsigned)((nt + 255) / 256 < 16384 ? (nt + 255) / 256 : 16384);
-  hipLaunchKernelGGL(pnn_theta_kernel, dim3(gt), dim3(256), 0, st, nt, P, m.eta, m.theta, m.bo,
-                     L.theta_eff);
+  if (m.mode == 1)
+    hipLaunchKernelGGL(pnn_theta_mul_kernel, dim3(gt), dim3(256), 0, st, nt, P, m.eta, m.theta,
+                       m.bo, L.y, L.theta_eff);
+  else
+    hipLaunchKernelGGL(pnn_theta_kernel, dim3(gt), dim3(256), 0, st, nt, P, m.eta, m.theta,
+                       m.bo, L.theta_eff);
   VMCX_LAUNCHED();
   return 0;
 }
@@ -903,8 +948,9 @@ static int peps_logderiv_impl(const vmcx_peps_model *m, int64_t B, const int64_t
   VMCX_CHECK_ARG(cfg && O && sign && logabs && ws, "null pointer");
   const int64_t P = m->n_params, H = m->hidden;
   const int N = m->Lx * m->Ly;
+  const bool mult = H > 0 && m->mode == 1;
   const int64_t np_total =
-      fact ? (H > 0 ? P + 2 * N + 4 * H : P)
+      fact ? (H > 0 ? P + 2 * N + 4 * H + (mult ? P : 0) : P)
            : (H > 0 ? P + H * 2 * N + H + H * H + H + P * H + P : P);
   VMCX_CHECK_ARG(ld >= np_total, fact ? "ld smaller than the factor width"
                                       : "ld smaller than the parameter count");
@@ -937,6 +983,15 @@ static int peps_logderiv_impl(const vmcx_peps_model *m, int64_t B, const int64_t
       VMCX_LAUNCHED();
       continue;
     }
+    const int64_t nt = rows * P;
+    if (mult) {
+      // G -> G theta (upstream of the network), y -> G (1 + eta y) (theta block)
+      const unsigned gt = (unsigned)((nt + 255) / 256 < 16384 ? (nt + 255) / 256 : 16384);
+      hipLaunchKernelGGL(pnn_mul_grad_kernel, dim3(gt), dim3(256), 0, st, nt, P, m->eta,
+                         m->theta, L.y, G);
+      VMCX_LAUNCHED();
+    }
+    const double *Gth = mult ? L.y : G;
     // dh2 = (eta G) Wo ; dz2 = dh2 act'(z2) ; dh1 = dz2 W2 ; dz1 = dh1 act'(z1)
     const double etav = m->eta, one = 1.0, zero = 0.0;
     PEPS_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)H,
@@ -954,15 +1009,15 @@ static int peps_logderiv_impl(const vmcx_peps_model *m, int64_t B, const int64_t
     const size_t sm = (size_t)4 * H * 8;
     if (fact)
       hipLaunchKernelGGL((pnn_write_kernel<double, true>), dim3((unsigned)rows), dim3(256), sm,
-                         st, rows, N, (int)H, P, m->act, m->eta, c, G, L.z1, L.z2, L.d1, L.d2,
-                         (double *)O + r0 * ld, ld);
+                         st, rows, N, (int)H, P, m->act, m->eta, c, G, Gth, L.z1, L.z2, L.d1,
+                         L.d2, (double *)O + r0 * ld, ld);
     else if (o_dtype == 0)
       hipLaunchKernelGGL(pnn_write_kernel<double>, dim3((unsigned)rows), dim3(256), sm, st, rows,
-                         N, (int)H, P, m->act, m->eta, c, G, L.z1, L.z2, L.d1, L.d2,
+                         N, (int)H, P, m->act, m->eta, c, G, Gth, L.z1, L.z2, L.d1, L.d2,
                          (double *)O + r0 * ld, ld);
     else
       hipLaunchKernelGGL(pnn_write_kernel<float>, dim3((unsigned)rows), dim3(256), sm, st, rows,
-                         N, (int)H, P, m->act, m->eta, c, G, L.z1, L.z2, L.d1, L.d2,
+                         N, (int)H, P, m->act, m->eta, c, G, Gth, L.z1, L.z2, L.d1, L.d2,
                          (float *)O + r0 * ld, ld);
     VMCX_LAUNCHED();
   }

@@ -48,6 +48,19 @@ __device__ __forceinline__ void lu_half_pivot(uint32_t k, double v, int lane, in
   piv = lane < 32 ? p0 : p1;
 }
 
+// True when an entry of the group's matrix (lanes gbase .. gbase + NFP - 1)
+// is NaN, infinite or beyond 1e300: the determinant is then reported as NaN
+// (a non-finite parameter must not pass for a zero amplitude).
+template <int NFP>
+__device__ __forceinline__ bool group_nonfinite(const double (&a)[NFP], int gbase) {
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < NFP; ++j) bad |= !(fabs(a[j]) <= 1e300);
+  const uint64_t b = __ballot(bad);
+  const uint64_t m = (NFP == 64) ? ~0ull : (((1ull << NFP) - 1ull) << gbase);
+  return (b & m) != 0ull;
+}
+
 struct GJResult {
   double logdet;
   int parity;

@@ -24,6 +24,13 @@ struct SweepArgs {
   double *sign, *logabs;
   int32_t *since_refresh;
   int64_t *n_accept;
+  // the FP32 form: M rounded to FP32 and the FP32 A^-1 (M above stays the
+  // FP64 matrix of the exact refresh)
+  const float *M32 = nullptr;
+  float *ainv32 = nullptr;
+  // a device-resident addition to sweep_offset (NULL: none), read by the
+  // kernel: lets a captured graph advance the draws between replays
+  const int64_t *offset_dev = nullptr;
 };
 
 int slater_refresh(int64_t B, int n_sites, int n_f, const double *M,
@@ -38,6 +45,18 @@ int slater_local_energy(int64_t B, int n_sites, int n_f, const double *M,
 int slater_logderiv(int64_t B, int n_sites, int n_f, const double *ainv,
                     const uint8_t *slot, void *O, int o_dtype, int64_t ld,
                     hipStream_t st);
+int slater_refresh_f32(int64_t B, int n_sites, int n_f, const double *M,
+                       const int64_t *configs, float *ainv, uint8_t *slot, double *sign,
+                       double *logabs, int32_t *since, int only_if_since, hipStream_t st);
+int slater_sweep_f32(const SweepArgs &a, int refresh_every, hipStream_t st);
+int slater_local_energy_f32(int64_t B, int n_sites, int n_f, const float *M, int n_edges,
+                            const int32_t *edges, double t, double U, const int64_t *configs,
+                            const float *ainv, const uint8_t *slot, double *eloc,
+                            hipStream_t st);
+int slater_logderiv_f32(int64_t B, int n_sites, int n_f, const float *ainv,
+                        const uint8_t *slot, void *O, int o_dtype, int64_t ld,
+                        hipStream_t st);
+int sgd_update_f32(int64_t n, float *params, const double *dp, double lr, hipStream_t st);
 int hubbard_conn_count(int64_t B, int n_sites, int n_edges,
                        const int32_t *edges, const int64_t *configs,
                        int64_t *counts, hipStream_t st);
@@ -142,7 +161,8 @@ int lp_accept(int64_t B, int N, uint64_t seed, int64_t step, const double *u_acc
               const int32_t *move, const int64_t *pcfg, const double *psign,
               const double *plog, const double *logq_f, const double *logq_r, int64_t *cfg,
               double *sign, double *logabs, int64_t *n_accept, uint8_t *accepted,
-              hipStream_t st);
+              hipStream_t st, const int64_t *step_dev = nullptr);
+int counter_add(int64_t *ctr, int64_t inc, hipStream_t st);
 int64_t lp_backward_workspace_bytes(int C, int64_t B);
 int lp_backward(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,
                 const double *dlogits, void *ws, int64_t ws_bytes, float *grad, hipStream_t st);
@@ -152,7 +172,8 @@ int lp_axpy(int64_t n, float a, const float *x, float *y, hipStream_t st);
 int lp_propose_given(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,
                      const double *logits_x, uint64_t seed, int64_t step,
                      const double *u_choice, double *logits_p, int32_t *move, double *logq_f,
-                     int64_t *pcfg, double *logq_r, int precision, hipStream_t st);
+                     int64_t *pcfg, double *logq_r, int precision, hipStream_t st,
+                     const int64_t *step_dev = nullptr);
 int lp_commit_logits(int64_t B, int M, const uint8_t *accepted, const double *logits_p,
                      double *logits_x, hipStream_t st);
 
@@ -161,12 +182,12 @@ int sym_images(int64_t B, int N, int G, const int64_t *cfg, const int32_t *perm,
 
 int64_t nnbf_workspace_bytes(const vmcx_nnbf_model *m, int64_t B, int64_t chunk, int n_edges);
 int nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
-                 int64_t ws_bytes, int64_t chunk, double *sign, double *logabs, float *z1_out,
+                 int64_t ws_bytes, int64_t chunk, double *sign, double *logabs, void *z1_out,
                  hipStream_t st);
 int nnbf_sweep(const vmcx_nnbf_model *m, int64_t B, int n_edges, const int32_t *edges_host,
                double hopping_rate, int n_sweeps, uint64_t seed, int64_t sweep_offset,
                const double *u_choice, const uint8_t *bits, const double *u_acc,
-               int64_t *cfg, float *z1, double *sign, double *logabs, int64_t *n_accept,
+               int64_t *cfg, void *z1, double *sign, double *logabs, int64_t *n_accept,
                void *ws, int64_t ws_bytes, hipStream_t st);
 int nnbf_local_energy(const vmcx_nnbf_model *m, int64_t B, int n_edges, const int32_t *edges,
                       double t, double U, const int64_t *cfg, const double *sign,

@@ -8,8 +8,11 @@
 //   D   = Wout h2 + bout     GEMM  (2N*n_f outputs)
 //   psi = det((M + s D)[occupied rows])   FP64 Gauss-Jordan in registers
 //
-// GEMMs run on MFMA through rocBLAS gemm_ex: FP32 operands, or BF16 operands
-// with FP32 accumulation.  The sweep proposes one edge for all walkers at a
+// GEMMs run on the matrix cores through rocBLAS: FP32 operands (gemm_ex), BF16
+// operands with FP32 accumulation (gemm_ex), or FP64 throughout (dgemm; the
+// weights, activations and backflow in FP64, the plain FP64 form).  The
+// kernels below take the scalar type TW of weights and activations (float for
+// the FP32 and BF16 modes, double for FP64) and the GEMM input type TI.  The sweep proposes one edge for all walkers at a
 // time; z1 is carried per walker and updated by the W1 columns of the moved
 // electrons, so only the two dense layers and the determinant are recomputed
 // per proposal.
@@ -25,6 +28,9 @@ namespace vmcx {
 __device__ __forceinline__ float act_f(float z, int act) {
   return act == 0 ? fmaxf(z, 0.0f) : tanhf(z);
 }
+__device__ __forceinline__ double act_f(double z, int act) {
+  return act == 0 ? fmax(z, 0.0) : tanh(z);
+}
 
 template <typename T>
 __device__ __forceinline__ T to_t(float v);
@@ -34,6 +40,10 @@ template <>
 __device__ __forceinline__ __hip_bfloat16 to_t<__hip_bfloat16>(float v) {
   return __float2bfloat16(v);
 }
+template <typename T>
+__device__ __forceinline__ T to_t(double v);
+template <>
+__device__ __forceinline__ double to_t<double>(double v) { return v; }
 
 // ------------------------------------------------------------ layer one
 // z1[w][u] = b1[u] + sum of W1[u][orbital] over the occupied spin-orbitals
@@ -44,20 +54,24 @@ __device__ __forceinline__ __hip_bfloat16 to_t<__hip_bfloat16>(float v) {
 // masks from a wave ballot.
 constexpr int L1_RB = 64, L1_UT = 64;
 
-template <typename TI>
+template <typename TW, typename TI>
 __global__ void __launch_bounds__(256)
     nnbf_layer1_kernel(int64_t Bc, int N, int H, int act,
-                       const int64_t *__restrict__ cfg, const float *__restrict__ W1,
-                       const float *__restrict__ b1, float *z1, TI *h1) {
-  __shared__ float wt[L1_UT * (128 + 1)];
-  __shared__ uint64_t mu[L1_RB], md[L1_RB];
+                       const int64_t *__restrict__ cfg, const TW *__restrict__ W1,
+                       const TW *__restrict__ b1, TW *z1, TI *h1) {
+  // dynamic LDS: the W1 tile [L1_UT][2N + 1] then the masks [2][L1_RB]
+  extern __shared__ __align__(16) char l1sm[];
   const int n_orb = 2 * N, ld = n_orb + 1;
+  TW *wt = reinterpret_cast<TW *>(l1sm);
+  uint64_t *mu = reinterpret_cast<uint64_t *>(l1sm + (((size_t)L1_UT * ld * sizeof(TW) + 15) &
+                                                      ~(size_t)15));
+  uint64_t *md = mu + L1_RB;
   const int64_t r0 = (int64_t)blockIdx.x * L1_RB;
   const int u0 = blockIdx.y * L1_UT;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   for (int idx = t; idx < L1_UT * n_orb; idx += blockDim.x) {
     const int uu = idx / n_orb, o = idx - uu * n_orb;
-    wt[uu * ld + o] = (u0 + uu < H) ? W1[(int64_t)(u0 + uu) * n_orb + o] : 0.0f;
+    wt[uu * ld + o] = (u0 + uu < H) ? W1[(int64_t)(u0 + uu) * n_orb + o] : TW(0);
   }
   for (int r = wave; r < L1_RB; r += blockDim.x / 64) {
     const int64_t w = r0 + r;
@@ -72,13 +86,13 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   const int u = u0 + lane;
   if (u >= H) return;
-  const float bias = b1[u];
+  const TW bias = b1[u];
   for (int r = wave; r < L1_RB; r += blockDim.x / 64) {
     const int64_t w = r0 + r;
     if (w >= Bc) break;
     const uint64_t bu = mu[r], bd = md[r];
     uint64_t all = bu | bd;
-    float z = bias;
+    TW z = bias;
     while (all) {
       const int s = __ffsll((long long)all) - 1;
       all &= all - 1;
@@ -91,14 +105,13 @@ __global__ void __launch_bounds__(256)
 }
 
 // z += b (in place), h = act(z)
-template <typename TI>
+template <typename TW, typename TI>
 __global__ void __launch_bounds__(256)
-    bias_act_kernel(int64_t rows, int H, int act, float *z, const float *__restrict__ b,
-                    TI *h) {
+    bias_act_kernel(int64_t rows, int H, int act, TW *z, const TW *__restrict__ b, TI *h) {
   const int64_t n = rows * H;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const float v = z[i] + b[i % H];
+    const TW v = z[i] + b[i % H];
     z[i] = v;
     h[i] = to_t<TI>(act_f(v, act));
   }
@@ -108,11 +121,11 @@ __global__ void __launch_bounds__(256)
 // Builds A^T row-per-lane: a[j] = A[j][col] = M[orb_j][col] + s (D[orb_j*nf+col]
 // + bout[...]) for orb_j the j-th occupied orbital (ascending), with the
 // occupation masks given explicitly (up, dn).
-template <int NFP>
+template <int NFP, typename TW>
 __device__ __forceinline__ void nnbf_rows(double (&a)[NFP], uint64_t up, uint64_t dn, int N,
-                                          int nf, int col, const float *__restrict__ M,
-                                          const float *__restrict__ Drow,
-                                          const float *__restrict__ bout, double s) {
+                                          int nf, int col, const TW *__restrict__ M,
+                                          const TW *__restrict__ Drow,
+                                          const TW *__restrict__ bout, double s) {
   uint64_t mu = up, md = dn;
 #pragma unroll
   for (int j = 0; j < NFP; ++j) {
@@ -158,11 +171,11 @@ __device__ __forceinline__ void group_masks(const int64_t *cfg, int N, int col, 
 }
 
 // INV: also write A^-1 (Gauss-Jordan); otherwise the determinant only (LU)
-template <int NFP, bool INV>
+template <int NFP, bool INV, typename TW>
 __global__ void __launch_bounds__(256)
     nnbf_det_kernel(int64_t Bc, int N, int nf, const int64_t *__restrict__ cfg,
-                    const float *__restrict__ D, const float *__restrict__ M,
-                    const float *__restrict__ bout, double s, double *sign,
+                    const TW *__restrict__ D, const TW *__restrict__ M,
+                    const TW *__restrict__ bout, double s, double *sign,
                     double *logabs, double *ainv) {
   constexpr int WPW = 64 / NFP;
   extern __shared__ __align__(16) char smem[];
@@ -178,19 +191,229 @@ __global__ void __launch_bounds__(256)
   group_masks<NFP>(cfg + w * N, N, col, up, dn);
   const bool count_ok = (__popcll(up) + __popcll(dn)) == nf;
   double a[NFP];
-  nnbf_rows<NFP>(a, up, dn, N, nf, col, M, D + w * (int64_t)(2 * N * nf), bout, s);
+  nnbf_rows<NFP, TW>(a, up, dn, N, nf, col, M, D + w * (int64_t)(2 * N * nf), bout, s);
+  const bool nonfinite = group_nonfinite<NFP>(a, grp * NFP);
   const GJResult r = INV ? gj_invert<NFP>(a, col, grp * NFP, prow, qbuf)
                          : lu_logdet<NFP>(a, col, grp * NFP, prow);
   const bool singular = r.singular || !count_ok;
   if (col == 0) {
-    sign[w] = singular ? 0.0 : ((r.parity & 1) ? -1.0 : 1.0);
-    logabs[w] = singular ? (count_ok ? log(1e-45) : __builtin_nan(""))
-                         : fmax(r.logdet, log(1e-45));
+    sign[w] = nonfinite ? __builtin_nan("")
+                        : (singular ? 0.0 : ((r.parity & 1) ? -1.0 : 1.0));
+    logabs[w] = (nonfinite || !count_ok) ? __builtin_nan("")
+                                         : (singular ? log(1e-45) : fmax(r.logdet, log(1e-45)));
   }
   if (INV && r.my_step >= 0) {
     double *dst = ainv + ((size_t)w * NFP + r.my_step) * NFP;
 #pragma unroll
-    for (int c = 0; c < NFP; ++c) dst[qbuf[c]] = singular ? 0.0 : a[c];
+    for (int c = 0; c < NFP; ++c)
+      dst[qbuf[c]] = nonfinite ? __builtin_nan("") : (singular ? 0.0 : a[c]);
+  }
+}
+
+// ------------------------------------------------ BF16 matrix-core layers
+// The two dense layers of the BF16 mode as hand-written matrix-core GEMMs
+// (v_mfma_f32_32x32x16_bf16, FP32 accumulation).  A workgroup (4 waves)
+// owns NG_RB = 64 configurations: their BF16 input rows h ([64][K], padded
+// rows of K + 8 so that the 16-byte fragment reads of 16 lanes fall in 64
+// different banks) sit in LDS for the whole launch; each wave walks 32-row
+// tiles of the weight matrix W ([n_rows][K] BF16, the reference's [out][in]
+// layout), whose fragments it reads from global memory (L2), and keeps two
+// 32 x 32 accumulator tiles (the 64 configurations).  MFMA operands, lane l:
+// a = W[n0 + l % 32][k0 + 8 (l / 32) ... + 7], b = h[r0 + l % 32][k0 + 8 (l / 32)
+// ... + 7]; result register q of lane l is row n0 + 8 (q / 4) + 4 (l / 32) +
+// q % 4 of column r0 + l % 32.
+// Epilogues:
+//   NG_ACT:  out_h[r][n] = bf16(act(acc + b[n])) (the hidden layer, its
+//            pre-activation kept in z[r][n] when z != NULL);
+//   NG_SCAT: the output layer restricted to the occupied orbitals: for row
+//            n = o nf + j of an orbital o occupied in configuration r,
+//            Docc[r][slot_r(o)][j] = acc + b[n] (FP32, [nf][nf] per
+//            configuration; slot_r(o) = rank of o among the occupied
+//            orbitals).  The unoccupied half of the rows is computed and
+//            dropped: the GEMM stays dense while the full backflow D never
+//            reaches HBM.  The determinant kernel adds M in FP64.
+typedef __bf16 ng_bf8 __attribute__((ext_vector_type(8)));
+typedef float ng_f16v __attribute__((ext_vector_type(16)));
+constexpr int NG_RB = 64, NG_T = 256;
+constexpr int NG_ACT = 0, NG_SCAT = 1;
+
+// LDS bytes of nnbf_gemm_mfma_kernel for input width K
+__host__ __device__ inline size_t ng_lds_bytes(int K) {
+  return (size_t)NG_RB * (K + 8) * 2 + (size_t)NG_RB * 128;
+}
+
+// KFIX > 0: the input width known at compile time (the k loop fully unrolled,
+// all weight-fragment loads of a tile in flight at once); 0: K at run time.
+template <int EPI, int KFIX>
+__global__ void __launch_bounds__(NG_T)
+    nnbf_gemm_mfma_kernel(int64_t R, int n_rows, int Kr, const __hip_bfloat16 *__restrict__ W,
+                          const __hip_bfloat16 *__restrict__ hin, const float *__restrict__ bias,
+                          int act, float *z, __hip_bfloat16 *hout,
+                          const int64_t *__restrict__ cfg, int N, int nf, float *A) {
+  extern __shared__ __align__(16) char ngsm[];
+  const int K = KFIX > 0 ? KFIX : Kr;
+  const int ldh = K + 8;
+  __bf16 *hs = reinterpret_cast<__bf16 *>(ngsm);
+  uint8_t *slot = reinterpret_cast<uint8_t *>(ngsm + (size_t)NG_RB * ldh * 2);  // [64][128]
+  const int64_t r0 = (int64_t)blockIdx.x * NG_RB;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, kh = lane >> 5;
+  // stage the 64 input rows (16-byte chunks; rows past R are zero)
+  const int kc = K / 8;
+  for (int idx = t; idx < NG_RB * kc; idx += NG_T) {
+    const int r = idx / kc, c = idx - r * kc;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (r0 + r < R) v = reinterpret_cast<const uint4 *>(hin + (r0 + r) * K)[c];
+    *reinterpret_cast<uint4 *>(hs + r * ldh + c * 8) = v;
+  }
+  if (EPI == NG_SCAT) {
+    // slot of every orbital in each configuration (0xFF: empty), by a wave
+    // ballot over the sites and popcount prefixes
+    for (int r = wave; r < NG_RB; r += NG_T / 64) {
+      int64_t x = 0;
+      if (r0 + r < R && lane < N) x = cfg[(r0 + r) * N + lane];
+      const uint64_t up = __ballot((x & 2) != 0), dn = __ballot((x & 1) != 0);
+      const int nup = __popcll(up);
+      // entry e of the row: orbital e (up orbitals 0 .. N-1, down N .. 2N-1)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = h * 64 + lane;
+        uint8_t v = 0xFF;
+        if (e < 2 * N) {
+          const bool is_dn = e >= N;
+          const int site = is_dn ? e - N : e;
+          const uint64_t m = is_dn ? dn : up;
+          const uint64_t below = (site == 0) ? 0ull : ((1ull << site) - 1ull);
+          const int sl = (is_dn ? nup : 0) + __popcll(m & below);
+          // a slot past nf (more occupied orbitals than nf) is dropped: the
+          // determinant kernel reports such a configuration as NaN
+          if (((m >> site) & 1ull) && sl < nf) v = (uint8_t)sl;
+        }
+        slot[r * 128 + e] = v;
+      }
+    }
+  }
+  __syncthreads();
+  const int n_tiles = (n_rows + 31) / 32;
+  for (int tile = wave; tile < n_tiles; tile += NG_T / 64) {
+    const int n0 = tile * 32;
+    const int nrow = n0 + (lane & 31);
+    const __bf16 *wrow = reinterpret_cast<const __bf16 *>(W) +
+                         (int64_t)(nrow < n_rows ? nrow : n_rows - 1) * K + 8 * kh;
+    ng_f16v acc0, acc1;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      acc0[q] = 0.0f;
+      acc1[q] = 0.0f;
+    }
+    const __bf16 *b0p = hs + (lane & 31) * ldh + 8 * kh;
+    const __bf16 *b1p = hs + (32 + (lane & 31)) * ldh + 8 * kh;
+#pragma unroll (KFIX > 0 ? KFIX / 16 : 4)
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      const ng_bf8 a = *reinterpret_cast<const ng_bf8 *>(wrow + k0);
+      const ng_bf8 b0 = *reinterpret_cast<const ng_bf8 *>(b0p + k0);
+      const ng_bf8 b1 = *reinterpret_cast<const ng_bf8 *>(b1p + k0);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, acc1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const int rl = half * 32 + (lane & 31);
+      const int64_t r = r0 + rl;
+      if (r >= R) continue;
+      const ng_f16v &acc = half ? acc1 : acc0;
+      if (EPI == NG_ACT) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int n = n0 + 8 * (q >> 2) + 4 * kh + (q & 3);
+          if (n >= n_rows) continue;
+          const float v = acc[q] + bias[n];
+          if (z) z[r * n_rows + n] = v;
+          hout[r * n_rows + n] = __float2bfloat16(act == 0 ? fmaxf(v, 0.0f) : tanhf(v));
+        }
+      } else {
+        float *Ar = A + r * (int64_t)nf * nf;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int nb = n0 + 8 * g + 4 * kh;       // rows nb .. nb + 3
+          if (nb >= n_rows) continue;
+          const int o = nb / nf, j = nb - o * nf;
+          const int sl = slot[rl * 128 + o];
+          if (sl == 0xFF) {
+            // an orbital boundary inside the group (nf not a multiple of 4):
+            // the other rows may belong to an occupied orbital
+            if ((nf & 3) == 0 || j + 3 < nf) continue;
+          }
+          if ((nf & 3) == 0) {
+            float4 v;
+            v.x = acc[4 * g + 0] + bias[nb + 0];
+            v.y = acc[4 * g + 1] + bias[nb + 1];
+            v.z = acc[4 * g + 2] + bias[nb + 2];
+            v.w = acc[4 * g + 3] + bias[nb + 3];
+            *reinterpret_cast<float4 *>(Ar + (int64_t)sl * nf + j) = v;
+          } else {
+            for (int e = 0; e < 4; ++e) {
+              const int n = nb + e;
+              if (n >= n_rows) break;
+              const int oo = n / nf, jj = n - oo * nf;
+              const int ss = slot[rl * 128 + oo];
+              if (ss == 0xFF) continue;
+              Ar[(int64_t)ss * nf + jj] = acc[4 * g + e] + bias[n];
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// Determinant from the compact Docc ([nf][nf] FP32 per configuration,
+// written by the NG_SCAT epilogue): lane col of the group holds column col,
+// a[j] = M[orb_j][col] + s Docc[j][col] in FP64 for orb_j the j-th occupied
+// orbital (ascending); padding as nnbf_rows.
+template <int NFP>
+__global__ void __launch_bounds__(256)
+    nnbf_detA_kernel(int64_t Bc, int N, int nf, const int64_t *__restrict__ cfg,
+                     const float *__restrict__ A, const float *__restrict__ M, double s,
+                     double *sign, double *logabs) {
+  constexpr int WPW = 64 / NFP;
+  extern __shared__ __align__(16) char smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int grp = lane / NFP, col = lane % NFP;
+  const int wl = wave * WPW + grp;
+  const int per_block = (blockDim.x / 64) * WPW;
+  const int64_t w = (int64_t)blockIdx.x * per_block + wl;
+  if (w >= Bc) return;
+  double *prow = reinterpret_cast<double *>(smem) + wl * NFP;
+  uint64_t up, dn;
+  group_masks<NFP>(cfg + w * N, N, col, up, dn);
+  const bool count_ok = (__popcll(up) + __popcll(dn)) == nf;
+  const float *Aw = A + w * (int64_t)nf * nf;
+  double a[NFP];
+  uint64_t mu = up, md = dn;
+#pragma unroll
+  for (int j = 0; j < NFP; ++j) {
+    int orb = -1;
+    if (mu) {
+      orb = __ffsll((long long)mu) - 1;
+      mu &= mu - 1;
+    } else if (md) {
+      orb = N + __ffsll((long long)md) - 1;
+      md &= md - 1;
+    }
+    if (j < nf && col < nf)
+      a[j] = orb >= 0 ? (double)M[(int64_t)orb * nf + col] + s * (double)Aw[(int64_t)j * nf + col]
+                      : 0.0;
+    else
+      a[j] = (j == col) ? 1.0 : 0.0;
+  }
+  const bool nonfinite = group_nonfinite<NFP>(a, grp * NFP);
+  const GJResult r = lu_logdet<NFP>(a, col, grp * NFP, prow);
+  const bool singular = r.singular || !count_ok;
+  if (col == 0) {
+    sign[w] = nonfinite ? __builtin_nan("")
+                        : (singular ? 0.0 : ((r.parity & 1) ? -1.0 : 1.0));
+    logabs[w] = (nonfinite || !count_ok) ? __builtin_nan("")
+                                         : (singular ? log(1e-45) : fmax(r.logdet, log(1e-45)));
   }
 }
 
@@ -256,10 +479,10 @@ __global__ void __launch_bounds__(256)
 }
 
 // h1p = act(z1 + sum_moves (W1[:, to] - W1[:, from]))
-template <typename TI>
+template <typename TW, typename TI>
 __global__ void __launch_bounds__(256)
-    nnbf_h1_proposed_kernel(int64_t B, int N, int H, int act, const float *__restrict__ W1,
-                            const float *__restrict__ z1, const int32_t *__restrict__ prop,
+    nnbf_h1_proposed_kernel(int64_t B, int N, int H, int act, const TW *__restrict__ W1,
+                            const TW *__restrict__ z1, const int32_t *__restrict__ prop,
                             const uint32_t *__restrict__ moves, TI *h1p) {
   const int64_t n = B * H;
   const int n_orb = 2 * N;
@@ -267,7 +490,7 @@ __global__ void __launch_bounds__(256)
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t w = idx / H;
     const int u = (int)(idx - w * H);
-    float z = z1[idx];
+    TW z = z1[idx];
     const int p = prop[w];
     if (p & 1) {
       const int nm = (p >> 16) & 3;
@@ -282,13 +505,13 @@ __global__ void __launch_bounds__(256)
 }
 
 // determinant of the proposed configuration, Metropolis test, commit
-template <int NFP>
+template <int NFP, typename TW>
 __global__ void __launch_bounds__(256)
     nnbf_accept_kernel(int64_t B, int N, int nf, int H, int i, int j, int64_t *cfg,
                        const int32_t *__restrict__ prop, const uint32_t *__restrict__ moves,
-                       const double *__restrict__ u_acc, const float *__restrict__ D,
-                       const float *__restrict__ M, const float *__restrict__ bout, double s,
-                       const float *__restrict__ W1, float *z1, double *sign, double *logabs,
+                       const double *__restrict__ u_acc, const TW *__restrict__ D,
+                       const TW *__restrict__ M, const TW *__restrict__ bout, double s,
+                       const TW *__restrict__ W1, TW *z1, double *sign, double *logabs,
                        int64_t *n_accept) {
   constexpr int WPW = 64 / NFP;
   extern __shared__ __align__(16) char smem[];
@@ -310,7 +533,7 @@ __global__ void __launch_bounds__(256)
   dn = (dn & ~((1ull << i) | (1ull << j))) | ((uint64_t)(ni & 1) << i) |
        ((uint64_t)(nj & 1) << j);
   double a[NFP];
-  nnbf_rows<NFP>(a, up, dn, N, nf, col, M, D + w * (int64_t)(2 * N * nf), bout, s);
+  nnbf_rows<NFP, TW>(a, up, dn, N, nf, col, M, D + w * (int64_t)(2 * N * nf), bout, s);
   const GJResult r = lu_logdet<NFP>(a, col, grp * NFP, prow);
   const double lp = r.singular ? log(1e-45) : fmax(r.logdet, log(1e-45));
   const double ratio = exp(2.0 * (lp - logabs[w]));
@@ -327,7 +550,7 @@ __global__ void __launch_bounds__(256)
   const uint32_t mv = moves[w];
   const int n_orb = 2 * N;
   for (int u = col; u < H; u += NFP) {
-    float dz = 0.0f;
+    TW dz = 0;
     for (int m = 0; m < nm; ++m) {
       const int from = (mv >> (16 * m)) & 0xFF, to = (mv >> (16 * m + 8)) & 0xFF;
       dz += W1[(int64_t)u * n_orb + to] - W1[(int64_t)u * n_orb + from];
@@ -479,10 +702,10 @@ __global__ void __launch_bounds__(256)
 
 // head, one block per sample: gM (the M block of the row, TO) and s gM into
 // G (FP32, [rows][nD], the GEMM operand)
-template <typename TO>
+template <typename TO, typename TW>
 __global__ void __launch_bounds__(256)
     nnbf_grad_head_kernel(int64_t Bc, int N, int nf, int nfp, const int64_t *__restrict__ cfg,
-                          const double *__restrict__ ainv, double s, float *G, TO *O,
+                          const double *__restrict__ ainv, double s, TW *G, TO *O,
                           int64_t ld) {
   __shared__ int slot_of[128];
   const int64_t w = blockIdx.x;
@@ -499,46 +722,53 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   const double *Ai = ainv + (size_t)w * nfp * nfp;
-  const float sf = (float)s;
+  const TW sf = (TW)s;
   TO *row = O + w * ld;
-  float *g = G + w * (int64_t)nD;
+  TW *g = G + w * (int64_t)nD;
   for (int idx = tid; idx < nD; idx += nt) {
     const int o = idx / nf, c = idx - o * nf;
     const int r = slot_of[o];
     // d ln|det A| / d A[r][c] = (A^-1)[c][r] = ainv[r][c]
-    const float v = (r >= 0) ? (float)Ai[(size_t)r * nfp + c] : 0.0f;
+    const TW v = (r >= 0) ? (TW)Ai[(size_t)r * nfp + c] : TW(0);
     row[idx] = (TO)v;
     g[idx] = v * sf;
   }
 }
 
 // d *= act'(z) elementwise
+template <typename TW>
 __global__ void __launch_bounds__(256)
-    nnbf_dact_kernel(int64_t n, int act, const float *__restrict__ z, float *d) {
+    nnbf_dact_kernel(int64_t n, int act, const TW *__restrict__ z, TW *d) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const float v = z[i];
-    d[i] = act == 0 ? (v > 0.0f ? d[i] : 0.0f) : d[i] * (1.0f - tanhf(v) * tanhf(v));
+    const TW v = z[i];
+    if (act == 0) {
+      d[i] = v > TW(0) ? d[i] : TW(0);
+    } else {
+      const TW th = act_f(v, 1);
+      d[i] *= TW(1) - th * th;
+    }
   }
 }
 
 // tail, one block per sample: the rest of the row from dz1, dz2 ([rows][H]),
 // z1, z2 and s gM (G)
-template <typename TO, bool FACT>
+template <typename TO, bool FACT, typename TW>
 __global__ void __launch_bounds__(256)
     nnbf_grad_tail_kernel(int64_t Bc, int N, int nf, int H, int act,
-                          const int64_t *__restrict__ cfg, const float *__restrict__ z1,
-                          const float *__restrict__ z2, const float *__restrict__ dz1,
-                          const float *__restrict__ dz2, const float *__restrict__ G, TO *O,
+                          const int64_t *__restrict__ cfg, const TW *__restrict__ z1,
+                          const TW *__restrict__ z2, const TW *__restrict__ dz1,
+                          const TW *__restrict__ dz2, const TW *__restrict__ G, TO *O,
                           int64_t ld) {
-  extern __shared__ float tsm[];
-  float *h1 = tsm, *h2 = h1 + H, *d1 = h2 + H, *d2 = d1 + H, *nvec = d2 + H;   // 4H + 2N
+  extern __shared__ __align__(16) char tsm_raw[];
+  TW *tsm = reinterpret_cast<TW *>(tsm_raw);
+  TW *h1 = tsm, *h2 = h1 + H, *d1 = h2 + H, *d2 = d1 + H, *nvec = d2 + H;   // 4H + 2N
   const int64_t w = blockIdx.x;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int n_orb = 2 * N, nD = n_orb * nf;
   const int64_t *c = cfg + w * N;
   for (int o = tid; o < n_orb; o += nt)
-    nvec[o] = (o < N) ? ((c[o] & 2) ? 1.0f : 0.0f) : ((c[o - N] & 1) ? 1.0f : 0.0f);
+    nvec[o] = (o < N) ? ((c[o] & 2) ? TW(1) : TW(0)) : ((c[o - N] & 1) ? TW(1) : TW(0));
   for (int u = tid; u < H; u += nt) {
     h1[u] = act_f(z1[w * H + u], act);
     h2[u] = act_f(z2[w * H + u], act);
@@ -559,7 +789,7 @@ __global__ void __launch_bounds__(256)
     }
     return;
   }
-  const float *g = G + w * (int64_t)nD;
+  const TW *g = G + w * (int64_t)nD;
   int64_t off = nD;
   for (int64_t idx = tid; idx < (int64_t)H * n_orb; idx += nt) {   // W1
     const int u = (int)(idx / n_orb), jj = (int)(idx - (int64_t)u * n_orb);
@@ -584,11 +814,28 @@ __global__ void __launch_bounds__(256)
 }
 
 // ============================================================ host side
+// Weights of one mode: FP32 (gemm 0 / 1) or FP64 (gemm 2).
+template <typename TW>
+struct NnW {
+  const TW *M, *W1, *b1, *W2, *b2, *Wout, *bout;
+};
+template <typename TW>
+static NnW<TW> weights_of(const vmcx_nnbf_model &m);
+template <>
+NnW<float> weights_of<float>(const vmcx_nnbf_model &m) {
+  return {m.M, m.W1, m.b1, m.W2, m.b2, m.Wout, m.bout};
+}
+template <>
+NnW<double> weights_of<double>(const vmcx_nnbf_model &m) {
+  return {m.M64, m.W1_64, m.b1_64, m.W2_64, m.b2_64, m.Wout_64, m.bout_64};
+}
+
 struct NnbfWs {
-  // per-chunk forward buffers
-  float *z1, *z2, *D;
+  // per-chunk forward buffers (elements of the mode's weight type TW, h1 / h2
+  // of the GEMM input type)
+  void *z1, *z2, *D;
   void *h1, *h2;
-  float *g1, *g2;   // backward: dz1, dz2
+  void *g1, *g2;   // backward: dz1, dz2
   double *ainv;
   // sweep buffers (per walker)
   int32_t *prop;
@@ -612,14 +859,15 @@ static NnbfWs nnbf_layout(const vmcx_nnbf_model &m, int64_t B, int64_t chunk, in
     return base ? base + o : nullptr;
   };
   const int H = m.hidden, nD = 2 * m.n_sites * m.n_f, nfp = nfp_of(m.n_f);
-  const size_t ti = m.gemm == 1 ? 2 : 4;
-  L.z1 = (float *)take((size_t)chunk * H * 4);
-  L.z2 = (float *)take((size_t)chunk * H * 4);
-  L.D = (float *)take((size_t)chunk * nD * 4);
+  const size_t tw = m.gemm == 2 ? 8 : 4;
+  const size_t ti = m.gemm == 1 ? 2 : tw;
+  L.z1 = take((size_t)chunk * H * tw);
+  L.z2 = take((size_t)chunk * H * tw);
+  L.D = take((size_t)chunk * nD * tw);
   L.h1 = take((size_t)chunk * H * ti);
   L.h2 = take((size_t)chunk * H * ti);
-  L.g1 = (float *)take((size_t)chunk * H * 4);
-  L.g2 = (float *)take((size_t)chunk * H * 4);
+  L.g1 = take((size_t)chunk * H * tw);
+  L.g2 = take((size_t)chunk * H * tw);
   L.ainv = (double *)take((size_t)chunk * nfp * nfp * 8);
   L.prop = (int32_t *)take((size_t)B * 4);
   L.moves = (uint32_t *)take((size_t)B * 4);
@@ -642,10 +890,15 @@ static int check_model(const vmcx_nnbf_model *m) {
   VMCX_CHECK_ARG(m->n_f >= 1 && m->n_f <= 64 && m->n_f <= 2 * m->n_sites, "bad n_f");
   VMCX_CHECK_ARG(m->hidden >= 1 && m->hidden <= 4096, "hidden must be in [1, 4096]");
   VMCX_CHECK_ARG(m->act == 0 || m->act == 1, "act must be 0 (ReLU) or 1 (tanh)");
-  VMCX_CHECK_ARG(m->gemm == 0 || m->gemm == 1, "gemm must be 0 (FP32) or 1 (BF16)");
-  VMCX_CHECK_ARG(m->M && m->W1 && m->b1 && m->W2 && m->b2 && m->Wout && m->bout,
-                 "null weight pointer");
-  VMCX_CHECK_ARG(m->gemm == 0 || (m->W2_bf16 && m->Wout_bf16), "BF16 mode needs BF16 weights");
+  VMCX_CHECK_ARG(m->gemm >= 0 && m->gemm <= 2, "gemm must be 0 (FP32), 1 (BF16) or 2 (FP64)");
+  if (m->gemm == 2)
+    VMCX_CHECK_ARG(m->M64 && m->W1_64 && m->b1_64 && m->W2_64 && m->b2_64 && m->Wout_64 &&
+                       m->bout_64,
+                   "FP64 mode needs the FP64 weight pointers");
+  else
+    VMCX_CHECK_ARG(m->M && m->W1 && m->b1 && m->W2 && m->b2 && m->Wout && m->bout,
+                   "null weight pointer");
+  VMCX_CHECK_ARG(m->gemm != 1 || (m->W2_bf16 && m->Wout_bf16), "BF16 mode needs BF16 weights");
   return 0;
 }
 
@@ -664,117 +917,233 @@ int64_t nnbf_workspace_bytes(const vmcx_nnbf_model *m, int64_t B, int64_t chunk,
     }                                                                        \
   } while (0)
 
+// column-major C (m x n) = op(A) B for the dense layers: FP64 by dgemm, FP32 /
+// BF16 operands with FP32 results by gemm_ex
+static int nn_gemm(rocblas_handle h, rocblas_operation ta, int mm, int nn, int kk,
+                   const void *A, int lda, const void *Bm, int ldb, void *C, int ldc, int gemm) {
+  if (gemm == 2) {
+    const double one = 1.0, zero = 0.0;
+    NN_BLAS(rocblas_dgemm(h, ta, rocblas_operation_none, mm, nn, kk, &one, (const double *)A,
+                          lda, (const double *)Bm, ldb, &zero, (double *)C, ldc));
+    return 0;
+  }
+  const float one = 1.0f, zero = 0.0f;
+  const rocblas_datatype ti = gemm == 1 ? rocblas_datatype_bf16_r : rocblas_datatype_f32_r;
+  NN_BLAS(rocblas_gemm_ex(h, ta, rocblas_operation_none, mm, nn, kk, &one, A, ti, lda, Bm, ti,
+                          ldb, &zero, C, rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r,
+                          ldc, rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0));
+  return 0;
+}
+
 // z2 = W2 h1 ; D = Wout act(z2 + b2)  for `rows` rows already holding h1
+template <typename TW, typename TI>
 static int dense_layers(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
                         hipStream_t st) {
   rocblas_handle h = blas_handle_for(st);
   VMCX_CHECK_ARG(h != nullptr, "rocblas handle");
+  const NnW<TW> W = weights_of<TW>(m);
   const int H = m.hidden, nD = 2 * m.n_sites * m.n_f;
-  const float one = 1.0f, zero = 0.0f;
-  const rocblas_datatype ti = m.gemm == 1 ? rocblas_datatype_bf16_r : rocblas_datatype_f32_r;
-  const void *W2 = m.gemm == 1 ? m.W2_bf16 : (const void *)m.W2;
-  const void *Wo = m.gemm == 1 ? m.Wout_bf16 : (const void *)m.Wout;
+  const void *W2 = m.gemm == 1 ? m.W2_bf16 : (const void *)W.W2;
+  const void *Wo = m.gemm == 1 ? m.Wout_bf16 : (const void *)W.Wout;
   // column-major view: z2 (H x rows) = W2 (H x H, row-major = W2^T col-major) * h1 (H x rows)
-  NN_BLAS(rocblas_gemm_ex(h, rocblas_operation_transpose, rocblas_operation_none, H,
-                          (rocblas_int)rows, H, &one, W2, ti, H, L.h1, ti, H, &zero, L.z2,
-                          rocblas_datatype_f32_r, H, L.z2, rocblas_datatype_f32_r, H,
-                          rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0));
+  int rc = nn_gemm(h, rocblas_operation_transpose, H, (int)rows, H, W2, H, L.h1, H, L.z2, H,
+                   m.gemm);
+  if (rc) return rc;
   const unsigned g = (unsigned)(((rows * H) + 255) / 256 < 8192 ? ((rows * H) + 255) / 256 : 8192);
-  if (m.gemm == 1)
-    hipLaunchKernelGGL(bias_act_kernel<__hip_bfloat16>, dim3(g), dim3(256), 0, st, rows, H,
-                       m.act, L.z2, m.b2, (__hip_bfloat16 *)L.h2);
-  else
-    hipLaunchKernelGGL(bias_act_kernel<float>, dim3(g), dim3(256), 0, st, rows, H, m.act,
-                       L.z2, m.b2, (float *)L.h2);
+  hipLaunchKernelGGL((bias_act_kernel<TW, TI>), dim3(g), dim3(256), 0, st, rows, H, m.act,
+                     (TW *)L.z2, W.b2, (TI *)L.h2);
   VMCX_LAUNCHED();
-  NN_BLAS(rocblas_gemm_ex(h, rocblas_operation_transpose, rocblas_operation_none, nD,
-                          (rocblas_int)rows, H, &one, Wo, ti, H, L.h2, ti, H, &zero, L.D,
-                          rocblas_datatype_f32_r, nD, L.D, rocblas_datatype_f32_r, nD,
-                          rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0));
-  return 0;
+  return nn_gemm(h, rocblas_operation_transpose, nD, (int)rows, H, Wo, H, L.h2, H, L.D, nD,
+                 m.gemm);
 }
 
-template <int NFP>
+template <int NFP, typename TW>
 static void launch_det(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
                        const int64_t *cfg, double *sign, double *logabs, double *ainv,
                        hipStream_t st) {
   constexpr int WPW = 64 / NFP;
   const int64_t per_block = 4 * WPW;
   const int64_t nblk = (rows + per_block - 1) / per_block;
+  const NnW<TW> W = weights_of<TW>(m);
   if (ainv)
-    hipLaunchKernelGGL((nnbf_det_kernel<NFP, true>), dim3((unsigned)nblk), dim3(256),
-                       (size_t)per_block * NFP * 9, st, rows, m.n_sites, m.n_f, cfg, L.D, m.M,
-                       m.bout, m.bf_scale, sign, logabs, ainv);
+    hipLaunchKernelGGL((nnbf_det_kernel<NFP, true, TW>), dim3((unsigned)nblk), dim3(256),
+                       (size_t)per_block * NFP * 9, st, rows, m.n_sites, m.n_f, cfg,
+                       (const TW *)L.D, W.M, W.bout, m.bf_scale, sign, logabs, ainv);
   else
-    hipLaunchKernelGGL((nnbf_det_kernel<NFP, false>), dim3((unsigned)nblk), dim3(256),
-                       (size_t)per_block * NFP * 9, st, rows, m.n_sites, m.n_f, cfg, L.D, m.M,
-                       m.bout, m.bf_scale, sign, logabs, ainv);
+    hipLaunchKernelGGL((nnbf_det_kernel<NFP, false, TW>), dim3((unsigned)nblk), dim3(256),
+                       (size_t)per_block * NFP * 9, st, rows, m.n_sites, m.n_f, cfg,
+                       (const TW *)L.D, W.M, W.bout, m.bf_scale, sign, logabs, ainv);
 }
 
-// forward of `rows` configurations (rows <= chunk); ainv optional
-static int forward_rows(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
-                        const int64_t *cfg, double *sign, double *logabs, double *ainv,
-                        hipStream_t st) {
-  if (rows == 0) return 0;
+template <typename TW, typename TI>
+static int layer1(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows, const int64_t *cfg,
+                  hipStream_t st) {
   const dim3 g1((unsigned)((rows + L1_RB - 1) / L1_RB),
                 (unsigned)((m.hidden + L1_UT - 1) / L1_UT));
-  if (m.gemm == 1)
-    hipLaunchKernelGGL(nnbf_layer1_kernel<__hip_bfloat16>, g1, dim3(256), 0, st, rows,
-                       m.n_sites, m.hidden, m.act, cfg, m.W1, m.b1, L.z1,
-                       (__hip_bfloat16 *)L.h1);
-  else
-    hipLaunchKernelGGL(nnbf_layer1_kernel<float>, g1, dim3(256), 0, st, rows, m.n_sites,
-                       m.hidden, m.act, cfg, m.W1, m.b1, L.z1, (float *)L.h1);
+  const size_t lds = (((size_t)L1_UT * (2 * m.n_sites + 1) * sizeof(TW) + 15) & ~(size_t)15) +
+                     2 * L1_RB * sizeof(uint64_t);
+  VMCX_CHECK_ARG(lds <= max_lds_per_block(), "layer-1 LDS exceeds the device limit");
+  const NnW<TW> W = weights_of<TW>(m);
+  hipLaunchKernelGGL((nnbf_layer1_kernel<TW, TI>), g1, dim3(256), lds, st, rows, m.n_sites,
+                     m.hidden, m.act, cfg, W.W1, W.b1, (TW *)L.z1, (TI *)L.h1);
   VMCX_LAUNCHED();
-  int rc = dense_layers(m, L, rows, st);
+  return 0;
+}
+
+// the BF16 forward on the hand-written matrix-core layers: usable when the
+// hidden width is a multiple of 16 that fits the LDS staging (the input rows
+// of 64 configurations), for N <= 64
+static bool mfma_forward_ok(const vmcx_nnbf_model &m) {
+  return m.gemm == 1 && m.hidden % 16 == 0 && m.n_sites <= 64 &&
+         ng_lds_bytes(m.hidden) <= max_lds_per_block();
+}
+
+template <int EPI>
+static int launch_ng(int64_t R, int n_rows, int K, const void *W, const void *hin,
+                     const float *bias, int act, float *z, void *hout, const int64_t *cfg,
+                     int N, int nf, float *A, hipStream_t st) {
+  const dim3 g((unsigned)((R + NG_RB - 1) / NG_RB));
+  const size_t lds = ng_lds_bytes(K);
+  if (K == 256)
+    hipLaunchKernelGGL((nnbf_gemm_mfma_kernel<EPI, 256>), g, dim3(NG_T), lds, st, R, n_rows, K,
+                       (const __hip_bfloat16 *)W, (const __hip_bfloat16 *)hin, bias, act, z,
+                       (__hip_bfloat16 *)hout, cfg, N, nf, A);
+  else
+    hipLaunchKernelGGL((nnbf_gemm_mfma_kernel<EPI, 0>), g, dim3(NG_T), lds, st, R, n_rows, K,
+                       (const __hip_bfloat16 *)W, (const __hip_bfloat16 *)hin, bias, act, z,
+                       (__hip_bfloat16 *)hout, cfg, N, nf, A);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+template <int NFP>
+static void launch_detA(const vmcx_nnbf_model &m, int64_t rows, const int64_t *cfg,
+                        const float *A, double *sign, double *logabs, hipStream_t st) {
+  constexpr int WPW = 64 / NFP;
+  const int64_t per_block = 4 * WPW;
+  const int64_t nblk = (rows + per_block - 1) / per_block;
+  hipLaunchKernelGGL(nnbf_detA_kernel<NFP>, dim3((unsigned)nblk), dim3(256),
+                     (size_t)per_block * NFP * 8, st, rows, m.n_sites, m.n_f, cfg, A,
+                     (const float *)m.M, m.bf_scale, sign, logabs);
+}
+
+// layer 1, layer 2 and the occupied-row output layer on the matrix cores,
+// then the determinants: no rocBLAS call and no backflow matrix in HBM
+static int forward_rows_mfma(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
+                             const int64_t *cfg, double *sign, double *logabs, hipStream_t st) {
+  int rc = layer1<float, __hip_bfloat16>(m, L, rows, cfg, st);
+  if (rc) return rc;
+  const int H = m.hidden, nD = 2 * m.n_sites * m.n_f;
+  rc = launch_ng<NG_ACT>(rows, H, H, m.W2_bf16, L.h1, m.b2, m.act, (float *)L.z2, L.h2, nullptr,
+                         m.n_sites, m.n_f, nullptr, st);
+  if (rc) return rc;
+  float *A = (float *)L.D;   // [rows][nf][nf] fits the [rows][2N nf] buffer
+  rc = launch_ng<NG_SCAT>(rows, nD, H, m.Wout_bf16, L.h2, m.bout, m.act, nullptr, nullptr, cfg,
+                          m.n_sites, m.n_f, A, st);
   if (rc) return rc;
   switch (nfp_of(m.n_f)) {
-    case 8: launch_det<8>(m, L, rows, cfg, sign, logabs, ainv, st); break;
-    case 16: launch_det<16>(m, L, rows, cfg, sign, logabs, ainv, st); break;
-    case 32: launch_det<32>(m, L, rows, cfg, sign, logabs, ainv, st); break;
-    default: launch_det<64>(m, L, rows, cfg, sign, logabs, ainv, st); break;
+    case 8: launch_detA<8>(m, rows, cfg, A, sign, logabs, st); break;
+    case 16: launch_detA<16>(m, rows, cfg, A, sign, logabs, st); break;
+    case 32: launch_detA<32>(m, rows, cfg, A, sign, logabs, st); break;
+    default: launch_detA<64>(m, rows, cfg, A, sign, logabs, st); break;
   }
   VMCX_LAUNCHED();
   return 0;
 }
 
+// forward of `rows` configurations (rows <= chunk); ainv optional
+template <typename TW, typename TI>
+static int forward_rows_t(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
+                          const int64_t *cfg, double *sign, double *logabs, double *ainv,
+                          hipStream_t st) {
+  if (rows == 0) return 0;
+  int rc = layer1<TW, TI>(m, L, rows, cfg, st);
+  if (rc) return rc;
+  rc = dense_layers<TW, TI>(m, L, rows, st);
+  if (rc) return rc;
+  switch (nfp_of(m.n_f)) {
+    case 8: launch_det<8, TW>(m, L, rows, cfg, sign, logabs, ainv, st); break;
+    case 16: launch_det<16, TW>(m, L, rows, cfg, sign, logabs, ainv, st); break;
+    case 32: launch_det<32, TW>(m, L, rows, cfg, sign, logabs, ainv, st); break;
+    default: launch_det<64, TW>(m, L, rows, cfg, sign, logabs, ainv, st); break;
+  }
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+static int forward_rows(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
+                        const int64_t *cfg, double *sign, double *logabs, double *ainv,
+                        hipStream_t st) {
+  if (m.gemm == 2)
+    return forward_rows_t<double, double>(m, L, rows, cfg, sign, logabs, ainv, st);
+  if (m.gemm == 1 && ainv == nullptr && rows > 0 && mfma_forward_ok(m) && m.path == 0)
+    return forward_rows_mfma(m, L, rows, cfg, sign, logabs, st);
+  if (m.gemm == 1)
+    return forward_rows_t<float, __hip_bfloat16>(m, L, rows, cfg, sign, logabs, ainv, st);
+  return forward_rows_t<float, float>(m, L, rows, cfg, sign, logabs, ainv, st);
+}
+
 int nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
-                 int64_t ws_bytes, int64_t chunk, double *sign, double *logabs, float *z1_out,
+                 int64_t ws_bytes, int64_t chunk, double *sign, double *logabs, void *z1_out,
                  hipStream_t st) {
   if (check_model(m)) return 1;
   VMCX_CHECK_ARG(B >= 0 && chunk >= 1, "bad sizes");
   VMCX_CHECK_ARG(B == 0 || (cfg && sign && logabs && ws), "null pointer");
   const NnbfWs L = nnbf_layout(*m, B, chunk, 0, (char *)ws);
   VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
+  const size_t tw = m->gemm == 2 ? 8 : 4;
   for (int64_t r0 = 0; r0 < B; r0 += chunk) {
     const int64_t rows = (B - r0 < chunk) ? B - r0 : chunk;
     const int rc = forward_rows(*m, L, rows, cfg + r0 * m->n_sites, sign + r0, logabs + r0,
                                 nullptr, st);
     if (rc) return rc;
     if (z1_out)
-      VMCX_HIP(hipMemcpyAsync(z1_out + r0 * m->hidden, L.z1, (size_t)rows * m->hidden * 4,
-                              hipMemcpyDeviceToDevice, st));
+      VMCX_HIP(hipMemcpyAsync((char *)z1_out + (size_t)r0 * m->hidden * tw, L.z1,
+                              (size_t)rows * m->hidden * tw, hipMemcpyDeviceToDevice, st));
   }
   return 0;
 }
 
-template <int NFP>
+template <int NFP, typename TW>
 static void launch_accept(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t B, int i, int j,
-                          int64_t *cfg, float *z1, double *sign, double *logabs,
+                          int64_t *cfg, TW *z1, double *sign, double *logabs,
                           int64_t *n_accept, hipStream_t st) {
   constexpr int WPW = 64 / NFP;
   const int64_t per_block = 4 * WPW;
   const int64_t nblk = (B + per_block - 1) / per_block;
-  hipLaunchKernelGGL(nnbf_accept_kernel<NFP>, dim3((unsigned)nblk), dim3(256),
+  const NnW<TW> W = weights_of<TW>(m);
+  hipLaunchKernelGGL((nnbf_accept_kernel<NFP, TW>), dim3((unsigned)nblk), dim3(256),
                      (size_t)per_block * NFP * 9, st, B, m.n_sites, m.n_f, m.hidden, i, j, cfg,
-                     L.prop, L.moves, L.uacc, L.D, m.M, m.bout, m.bf_scale, m.W1, z1, sign,
-                     logabs, n_accept);
+                     L.prop, L.moves, L.uacc, (const TW *)L.D, W.M, W.bout, m.bf_scale, W.W1,
+                     z1, sign, logabs, n_accept);
+}
+
+template <typename TW, typename TI>
+static int sweep_edge(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t B, int i, int j,
+                      int64_t *cfg, TW *z1, double *sign, double *logabs, int64_t *n_accept,
+                      hipStream_t st) {
+  const int64_t nh = B * m.hidden;
+  const unsigned gh = (unsigned)((nh + 255) / 256 < 16384 ? (nh + 255) / 256 : 16384);
+  const NnW<TW> W = weights_of<TW>(m);
+  hipLaunchKernelGGL((nnbf_h1_proposed_kernel<TW, TI>), dim3(gh), dim3(256), 0, st, B,
+                     m.n_sites, m.hidden, m.act, W.W1, z1, L.prop, L.moves, (TI *)L.h1);
+  VMCX_LAUNCHED();
+  const int rc = dense_layers<TW, TI>(m, L, B, st);
+  if (rc) return rc;
+  switch (nfp_of(m.n_f)) {
+    case 8: launch_accept<8, TW>(m, L, B, i, j, cfg, z1, sign, logabs, n_accept, st); break;
+    case 16: launch_accept<16, TW>(m, L, B, i, j, cfg, z1, sign, logabs, n_accept, st); break;
+    case 32: launch_accept<32, TW>(m, L, B, i, j, cfg, z1, sign, logabs, n_accept, st); break;
+    default: launch_accept<64, TW>(m, L, B, i, j, cfg, z1, sign, logabs, n_accept, st); break;
+  }
+  VMCX_LAUNCHED();
+  return 0;
 }
 
 int nnbf_sweep(const vmcx_nnbf_model *m, int64_t B, int n_edges, const int32_t *edges_host,
                double hopping_rate, int n_sweeps, uint64_t seed, int64_t sweep_offset,
                const double *u_choice, const uint8_t *bits, const double *u_acc,
-               int64_t *cfg, float *z1, double *sign, double *logabs, int64_t *n_accept,
+               int64_t *cfg, void *z1, double *sign, double *logabs, int64_t *n_accept,
                void *ws, int64_t ws_bytes, hipStream_t st) {
   if (check_model(m)) return 1;
   VMCX_CHECK_ARG(B >= 0 && n_edges >= 0 && n_sweeps >= 0, "negative size");
@@ -792,8 +1161,6 @@ int nnbf_sweep(const vmcx_nnbf_model *m, int64_t B, int n_edges, const int32_t *
     VMCX_CHECK_ARG(i >= 0 && i < N && j >= 0 && j < N && i != j, "edge out of range");
   }
   const unsigned gw = (unsigned)((B + 255) / 256);
-  const int64_t nh = B * m->hidden;
-  const unsigned gh = (unsigned)((nh + 255) / 256 < 16384 ? (nh + 255) / 256 : 16384);
   for (int sw = 0; sw < n_sweeps; ++sw) {
     for (int e = 0; e < n_edges; ++e) {
       const int i = edges_host[2 * e], j = edges_host[2 * e + 1];
@@ -803,23 +1170,17 @@ int nnbf_sweep(const vmcx_nnbf_model *m, int64_t B, int n_edges, const int32_t *
                          u_choice ? u_choice + di : nullptr, bits ? bits + di : nullptr,
                          u_acc ? u_acc + di : nullptr, cfg, L.prop, L.moves, L.uacc);
       VMCX_LAUNCHED();
-      if (m->gemm == 1)
-        hipLaunchKernelGGL(nnbf_h1_proposed_kernel<__hip_bfloat16>, dim3(gh), dim3(256), 0, st,
-                           B, N, m->hidden, m->act, m->W1, z1, L.prop, L.moves,
-                           (__hip_bfloat16 *)L.h1);
+      int rc;
+      if (m->gemm == 2)
+        rc = sweep_edge<double, double>(*m, L, B, i, j, cfg, (double *)z1, sign, logabs,
+                                        n_accept, st);
+      else if (m->gemm == 1)
+        rc = sweep_edge<float, __hip_bfloat16>(*m, L, B, i, j, cfg, (float *)z1, sign, logabs,
+                                               n_accept, st);
       else
-        hipLaunchKernelGGL(nnbf_h1_proposed_kernel<float>, dim3(gh), dim3(256), 0, st, B, N,
-                           m->hidden, m->act, m->W1, z1, L.prop, L.moves, (float *)L.h1);
-      VMCX_LAUNCHED();
-      int rc = dense_layers(*m, L, B, st);
+        rc = sweep_edge<float, float>(*m, L, B, i, j, cfg, (float *)z1, sign, logabs, n_accept,
+                                      st);
       if (rc) return rc;
-      switch (nfp_of(m->n_f)) {
-        case 8: launch_accept<8>(*m, L, B, i, j, cfg, z1, sign, logabs, n_accept, st); break;
-        case 16: launch_accept<16>(*m, L, B, i, j, cfg, z1, sign, logabs, n_accept, st); break;
-        case 32: launch_accept<32>(*m, L, B, i, j, cfg, z1, sign, logabs, n_accept, st); break;
-        default: launch_accept<64>(*m, L, B, i, j, cfg, z1, sign, logabs, n_accept, st); break;
-      }
-      VMCX_LAUNCHED();
     }
   }
   return 0;
@@ -866,34 +1227,45 @@ int nnbf_local_energy(const vmcx_nnbf_model *m, int64_t B, int n_edges, const in
 
 // rows of O (or of its factors) for `rows` configurations whose forward
 // pass left z1, z2 and A^-1 in the workspace
-template <typename TO, bool FACT>
-static int backward_rows(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
-                         const int64_t *cfg, TO *O, int64_t ld, hipStream_t st) {
+template <typename TO, bool FACT, typename TW>
+static int backward_rows_t(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
+                           const int64_t *cfg, TO *O, int64_t ld, hipStream_t st) {
   const int N = m.n_sites, H = m.hidden, nf = m.n_f, nD = 2 * N * nf;
   rocblas_handle h = blas_handle_for(st);
   VMCX_CHECK_ARG(h != nullptr, "rocblas handle");
-  hipLaunchKernelGGL(nnbf_grad_head_kernel<TO>, dim3((unsigned)rows), dim3(256), 0, st, rows, N,
-                     nf, nfp_of(nf), cfg, L.ainv, m.bf_scale, L.D, O, ld);
+  const NnW<TW> W = weights_of<TW>(m);
+  hipLaunchKernelGGL((nnbf_grad_head_kernel<TO, TW>), dim3((unsigned)rows), dim3(256), 0, st,
+                     rows, N, nf, nfp_of(nf), cfg, L.ainv, m.bf_scale, (TW *)L.D, O, ld);
   VMCX_LAUNCHED();
-  const float one = 1.0f, zero = 0.0f;
   const int ri = (int)rows;
+  const int gm = m.gemm == 2 ? 2 : 0;   // FP64 or FP32 products (the backward is FP32 in BF16 mode)
   // column-major views: dh2 (H x rows) = Wout^T (Wout row-major [nD][H]) * (s gM) (nD x rows)
-  NN_BLAS(rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, H, ri, nD, &one,
-                        m.Wout, H, L.D, nD, &zero, L.g2, H));
+  int rc = nn_gemm(h, rocblas_operation_none, H, ri, nD, W.Wout, H, L.D, nD, L.g2, H, gm);
+  if (rc) return rc;
   const int64_t nh = rows * H;
   const unsigned gh = (unsigned)((nh + 255) / 256 < 8192 ? (nh + 255) / 256 : 8192);
-  hipLaunchKernelGGL(nnbf_dact_kernel, dim3(gh), dim3(256), 0, st, nh, m.act, L.z2, L.g2);
+  hipLaunchKernelGGL(nnbf_dact_kernel<TW>, dim3(gh), dim3(256), 0, st, nh, m.act,
+                     (const TW *)L.z2, (TW *)L.g2);
   VMCX_LAUNCHED();
   // dh1 (H x rows) = W2^T-as-stored (W2 row-major [u][i]: column-major (i, u)) * dz2
-  NN_BLAS(rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, H, ri, H, &one, m.W2,
-                        H, L.g2, H, &zero, L.g1, H));
-  hipLaunchKernelGGL(nnbf_dact_kernel, dim3(gh), dim3(256), 0, st, nh, m.act, L.z1, L.g1);
+  rc = nn_gemm(h, rocblas_operation_none, H, ri, H, W.W2, H, L.g2, H, L.g1, H, gm);
+  if (rc) return rc;
+  hipLaunchKernelGGL(nnbf_dact_kernel<TW>, dim3(gh), dim3(256), 0, st, nh, m.act,
+                     (const TW *)L.z1, (TW *)L.g1);
   VMCX_LAUNCHED();
-  const size_t lds = (size_t)(4 * H + 2 * N) * sizeof(float);
-  hipLaunchKernelGGL((nnbf_grad_tail_kernel<TO, FACT>), dim3((unsigned)rows), dim3(256), lds, st,
-                     rows, N, nf, H, m.act, cfg, L.z1, L.z2, L.g1, L.g2, L.D, O, ld);
+  const size_t lds = (size_t)(4 * H + 2 * N) * sizeof(TW);
+  hipLaunchKernelGGL((nnbf_grad_tail_kernel<TO, FACT, TW>), dim3((unsigned)rows), dim3(256), lds,
+                     st, rows, N, nf, H, m.act, cfg, (const TW *)L.z1, (const TW *)L.z2,
+                     (const TW *)L.g1, (const TW *)L.g2, (const TW *)L.D, O, ld);
   VMCX_LAUNCHED();
   return 0;
+}
+
+template <typename TO, bool FACT>
+static int backward_rows(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
+                         const int64_t *cfg, TO *O, int64_t ld, hipStream_t st) {
+  if (m.gemm == 2) return backward_rows_t<TO, FACT, double>(m, L, rows, cfg, O, ld, st);
+  return backward_rows_t<TO, FACT, float>(m, L, rows, cfg, O, ld, st);
 }
 
 int nnbf_logderiv(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
@@ -910,7 +1282,8 @@ int nnbf_logderiv(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void 
   VMCX_CHECK_ARG(ld >= np, "ld smaller than the parameter count");
   const NnbfWs L = nnbf_layout(*m, B, chunk, 0, (char *)ws);
   VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
-  VMCX_CHECK_ARG((4 * H + 2 * N) * 4 <= 65536, "backward LDS exceeds 64 KB");
+  const size_t tw = m->gemm == 2 ? 8 : 4;
+  VMCX_CHECK_ARG((4 * H + 2 * N) * tw <= 65536, "backward LDS exceeds 64 KB");
   for (int64_t r0 = 0; r0 < B; r0 += chunk) {
     const int64_t rows = (B - r0 < chunk) ? B - r0 : chunk;
     int rc = forward_rows(*m, L, rows, cfg + r0 * N, sign + r0, logabs + r0, L.ainv, st);
@@ -936,7 +1309,8 @@ int nnbf_logderiv_factors(const vmcx_nnbf_model *m, int64_t B, const int64_t *cf
   VMCX_CHECK_ARG(ld >= (int64_t)nD + 2 * N + 4 * H, "ld smaller than the factor width");
   const NnbfWs L = nnbf_layout(*m, B, chunk, 0, (char *)ws);
   VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
-  VMCX_CHECK_ARG((4 * H + 2 * N) * 4 <= 65536, "backward LDS exceeds 64 KB");
+  const size_t tw = m->gemm == 2 ? 8 : 4;
+  VMCX_CHECK_ARG((4 * H + 2 * N) * tw <= 65536, "backward LDS exceeds 64 KB");
   for (int64_t r0 = 0; r0 < B; r0 += chunk) {
     const int64_t rows = (B - r0 < chunk) ? B - r0 : chunk;
     int rc = forward_rows(*m, L, rows, cfg + r0 * N, sign + r0, logabs + r0, L.ainv, st);

@@ -388,9 +388,11 @@ constexpr int LPC_WAVES = 4;
 
 __global__ void __launch_bounds__(64 * LPC_WAVES)
     lp_choose_kernel(int64_t B, int N, int Lx, int Ly, int E, int sample,
-                     const double *__restrict__ logits, uint64_t seed, int64_t step,
+                     const double *__restrict__ logits, uint64_t seed, int64_t step0,
                      const double *__restrict__ u_choice, const int64_t *__restrict__ cfg,
-                     int32_t *move, double *logq, int64_t *pcfg) {
+                     int32_t *move, double *logq, int64_t *pcfg,
+                     const int64_t *__restrict__ step_dev) {
+  const int64_t step = step0 + (step_dev ? step_dev[0] : 0);
   const int lane = threadIdx.x & 63;
   const int64_t b = (int64_t)blockIdx.x * LPC_WAVES + (threadIdx.x >> 6);
   if (b >= B) return;   // whole wave
@@ -481,7 +483,7 @@ __global__ void __launch_bounds__(64 * LPC_WAVES)
 }
 
 // Metropolis-Hastings test and commit
-__global__ void lp_accept_kernel(int64_t B, int N, uint64_t seed, int64_t step,
+__global__ void lp_accept_kernel(int64_t B, int N, uint64_t seed, int64_t step0,
                                  const double *__restrict__ u_acc,
                                  const int32_t *__restrict__ move,
                                  const int64_t *__restrict__ pcfg,
@@ -490,9 +492,10 @@ __global__ void lp_accept_kernel(int64_t B, int N, uint64_t seed, int64_t step,
                                  const double *__restrict__ logq_f,
                                  const double *__restrict__ logq_r, int64_t *cfg,
                                  double *sign, double *logabs, int64_t *n_accept,
-                                 uint8_t *accepted) {
+                                 uint8_t *accepted, const int64_t *__restrict__ step_dev) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
+  const int64_t step = step0 + (step_dev ? step_dev[0] : 0);
   if (accepted) accepted[b] = 0;
   if (move[b] < 0) return;
   double u;
@@ -708,14 +711,15 @@ int lp_propose(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *
   const int N = Lx * Ly, E = lp_n_edges(Lx, Ly);
   const unsigned g = (unsigned)((B + LPC_WAVES - 1) / LPC_WAVES);
   hipLaunchKernelGGL(lp_choose_kernel, dim3(g), dim3(64 * LPC_WAVES), 0, st, B, N, Lx, Ly, E, 1,
-                     logits, seed, step, u_choice, cfg, move, logq_f, pcfg);
+                     logits, seed, step, u_choice, cfg, move, logq_f, pcfg,
+                     (const int64_t *)nullptr);
   VMCX_LAUNCHED();
   // reverse probability: the same move evaluated at the proposed configuration
   rc = lp_logits(P, C, Lx, Ly, B, pcfg, logits, precision, st);
   if (rc) return rc;
   hipLaunchKernelGGL(lp_choose_kernel, dim3(g), dim3(64 * LPC_WAVES), 0, st, B, N, Lx, Ly, E, 0,
                      logits, seed, step, (const double *)nullptr, pcfg, move, logq_r,
-                     (int64_t *)nullptr);
+                     (int64_t *)nullptr, (const int64_t *)nullptr);
   VMCX_LAUNCHED();
   return 0;
 }
@@ -727,7 +731,8 @@ int lp_propose(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *
 int lp_propose_given(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,
                      const double *logits_x, uint64_t seed, int64_t step,
                      const double *u_choice, double *logits_p, int32_t *move, double *logq_f,
-                     int64_t *pcfg, double *logq_r, int precision, hipStream_t st) {
+                     int64_t *pcfg, double *logq_r, int precision, hipStream_t st,
+                     const int64_t *step_dev) {
   if (lp_check(C, Lx, Ly)) return 1;
   VMCX_CHECK_ARG(B >= 0, "B < 0");
   if (B == 0) return 0;
@@ -736,13 +741,13 @@ int lp_propose_given(const float *P, int C, int Lx, int Ly, int64_t B, const int
   const int N = Lx * Ly, E = lp_n_edges(Lx, Ly);
   const unsigned g = (unsigned)((B + LPC_WAVES - 1) / LPC_WAVES);
   hipLaunchKernelGGL(lp_choose_kernel, dim3(g), dim3(64 * LPC_WAVES), 0, st, B, N, Lx, Ly, E, 1,
-                     logits_x, seed, step, u_choice, cfg, move, logq_f, pcfg);
+                     logits_x, seed, step, u_choice, cfg, move, logq_f, pcfg, step_dev);
   VMCX_LAUNCHED();
   int rc = lp_logits(P, C, Lx, Ly, B, pcfg, logits_p, precision, st);
   if (rc) return rc;
   hipLaunchKernelGGL(lp_choose_kernel, dim3(g), dim3(64 * LPC_WAVES), 0, st, B, N, Lx, Ly, E, 0,
                      logits_p, seed, step, (const double *)nullptr, pcfg, move, logq_r,
-                     (int64_t *)nullptr);
+                     (int64_t *)nullptr, step_dev);
   VMCX_LAUNCHED();
   return 0;
 }
@@ -771,14 +776,14 @@ int lp_accept(int64_t B, int N, uint64_t seed, int64_t step, const double *u_acc
               const int32_t *move, const int64_t *pcfg, const double *psign,
               const double *plog, const double *logq_f, const double *logq_r, int64_t *cfg,
               double *sign, double *logabs, int64_t *n_accept, uint8_t *accepted,
-              hipStream_t st) {
+              hipStream_t st, const int64_t *step_dev) {
   VMCX_CHECK_ARG(B >= 0 && N >= 1 && N <= LP_NMAX, "bad sizes");
   if (B == 0) return 0;
   VMCX_CHECK_ARG(move && pcfg && psign && plog && logq_f && logq_r && cfg && sign && logabs,
                  "null pointer");
   hipLaunchKernelGGL(lp_accept_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, st, B,
                      N, seed, step, u_acc, move, pcfg, psign, plog, logq_f, logq_r, cfg, sign,
-                     logabs, n_accept, accepted);
+                     logabs, n_accept, accepted, step_dev);
   VMCX_LAUNCHED();
   return 0;
 }

@@ -1,4 +1,6 @@
-// Slater-determinant walker kernels for gfx950 (CDNA4), FP64.
+// Slater-determinant walker kernels for gfx950 (CDNA4).  The walker state
+// A^-1 is stored in FP64 (the plain form) or FP32 (the precision form, the
+// `T = float` instances); the exact refresh always runs in FP64.
 //
 // Layout: a walker owns a group of NFP lanes (NFP = 8/16/32/64, so 64/NFP
 // walkers share one wave64).  Lane k of the group holds column k of A^-1 in
@@ -56,10 +58,10 @@ __device__ __forceinline__ void load_masks(const int64_t *cfg, int n_sites,
 // in-place scheme leaves R[q_p][c] = B^-1[p][q_c] = A^-1[q_c][p] in lane
 // q_p, register c, which is scattered to ainv[p][q_c] on the way out.
 // INV = false (no ainv requested): the determinant alone by LU (lu_logdet).
-template <int NFP, bool INV>
-__global__ void __launch_bounds__(256)
+template <int NFP, bool INV, typename T>
+__global__ void __launch_bounds__(512)
     refresh_kernel(int64_t B, int n_sites, int n_f, const double *__restrict__ M,
-                   const int64_t *__restrict__ configs, double *ainv,
+                   const int64_t *__restrict__ configs, T *ainv,
                    uint8_t *slot, double *sign, double *logabs,
                    int32_t *since_refresh, int only_if_since) {
   constexpr int WPW = 64 / NFP;
@@ -105,6 +107,7 @@ __global__ void __launch_bounds__(256)
   }
 
   bool singular = !count_ok;
+  const bool nonfinite = group_nonfinite<NFP>(a, grp * NFP);
   const GJResult r = INV ? gj_invert<NFP>(a, col, grp * NFP, prow, qbuf)
                          : lu_logdet<NFP>(a, col, grp * NFP, prow);
   singular = singular || r.singular;
@@ -112,7 +115,10 @@ __global__ void __launch_bounds__(256)
   const int parity = r.parity;
   const int my_step = r.my_step;
   if (col == 0) {
-    if (singular) {
+    if (nonfinite) {
+      sign[w] = __builtin_nan("");
+      logabs[w] = __builtin_nan("");
+    } else if (singular) {
       sign[w] = 0.0;
       logabs[w] = count_ok ? log(1e-45) : __builtin_nan("");
     } else {
@@ -123,9 +129,10 @@ __global__ void __launch_bounds__(256)
   }
   if (INV && my_step >= 0) {
     // lane q_p holds column p of A^-1 with register c = row q_c
-    double *dst = ainv + ((size_t)w * NFP + my_step) * NFP;
+    T *dst = ainv + ((size_t)w * NFP + my_step) * NFP;
 #pragma unroll
-    for (int c = 0; c < NFP; ++c) dst[qbuf[c]] = singular ? 0.0 : a[c];
+    for (int c = 0; c < NFP; ++c)
+      dst[qbuf[c]] = nonfinite ? T(__builtin_nan("")) : (singular ? T(0) : (T)a[c]);
   }
   if (slot != nullptr) {
     const int n_orb = 2 * n_sites;
@@ -145,15 +152,26 @@ __global__ void __launch_bounds__(256)
 
 // ================================================================ sweep
 // One wave group per walker; M (2N x NFP, zero padded) staged in LDS.
-template <int NFP>
-__device__ __forceinline__ double dot_row(const double *__restrict__ mrow,
-                                          const double (&a)[NFP]) {
-  const double2 *m2 = reinterpret_cast<const double2 *>(mrow);
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+
+template <int NFP, typename T>
+__device__ __forceinline__ T dot_row(const T *__restrict__ mrow, const T (&a)[NFP]) {
+  using T2 = typename Vec2<T>::type;
+  const T2 *m2 = reinterpret_cast<const T2 *>(mrow);
+  T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
   for (int j = 0; j < NFP / 2; j += 2) {
-    const double2 x = m2[j];
-    const double2 y = m2[j + 1];
+    const T2 x = m2[j];
+    const T2 y = m2[j + 1];
     s0 = fma(x.x, a[2 * j], s0);
     s1 = fma(x.y, a[2 * j + 1], s1);
     s2 = fma(y.x, a[2 * j + 2], s2);
@@ -163,20 +181,50 @@ __device__ __forceinline__ double dot_row(const double *__restrict__ mrow,
 }
 
 
-template <int NFP>
+// per-walker LDS: two broadcast columns of A^-1 (2 NFP values), the site
+// states (64 bytes) and the slots (128 bytes)
+template <int NFP, typename T>
 constexpr int walker_lds_bytes() {
-  return 16 * NFP + 64 + 128;
+  return 2 * NFP * (int)sizeof(T) + 64 + 128;
 }
 
-template <int NFP>
-__global__ void __launch_bounds__(256) sweep_kernel(SweepArgs g) {
+// the sweep's M (LDS copy) and A^-1 of the storage type T
+template <typename T>
+__device__ __forceinline__ const T *sweep_m(const SweepArgs &g);
+template <>
+__device__ __forceinline__ const double *sweep_m<double>(const SweepArgs &g) {
+  return g.M;
+}
+template <>
+__device__ __forceinline__ const float *sweep_m<float>(const SweepArgs &g) {
+  return g.M32;
+}
+template <typename T>
+__device__ __forceinline__ T *sweep_ainv(const SweepArgs &g);
+template <>
+__device__ __forceinline__ double *sweep_ainv<double>(const SweepArgs &g) {
+  return g.ainv;
+}
+template <>
+__device__ __forceinline__ float *sweep_ainv<float>(const SweepArgs &g) {
+  return g.ainv32;
+}
+
+// T = double: the plain FP64 form.  T = float: ratios, determinant ratios
+// and Sherman-Morrison updates in FP32 (half the registers and LDS per
+// walker); the acceptance test, the sign and ln|psi| in FP64.
+template <int NFP, typename T>
+__global__ void __launch_bounds__(512) sweep_kernel(SweepArgs g) {
   constexpr int WPW = 64 / NFP;
+  using T2 = typename Vec2<T>::type;
   extern __shared__ __align__(16) char smem[];
   const int N = g.n_sites, n_orb = 2 * N;
-  double *Ms = reinterpret_cast<double *>(smem);
+  T *Ms = reinterpret_cast<T *>(smem);
+  const T *Mg = sweep_m<T>(g);
+  T *ainv = sweep_ainv<T>(g);
   for (int idx = threadIdx.x; idx < n_orb * NFP; idx += blockDim.x) {
     const int r = idx / NFP, c = idx % NFP;
-    Ms[idx] = (c < g.n_f) ? g.M[(int64_t)r * g.n_f + c] : 0.0;
+    Ms[idx] = (c < g.n_f) ? Mg[(int64_t)r * g.n_f + c] : T(0);
   }
   __syncthreads();
 
@@ -187,19 +235,18 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs g) {
   if (w >= g.B) return;
   const int gbase = grp * NFP;
 
-  char *wbase = smem + (size_t)n_orb * NFP * sizeof(double) +
-                (size_t)wl * walker_lds_bytes<NFP>();
-  double *colbuf = reinterpret_cast<double *>(wbase);  // [2][NFP]
-  uint8_t *st = reinterpret_cast<uint8_t *>(wbase + 16 * NFP);
+  char *wbase = smem + (size_t)n_orb * NFP * sizeof(T) +
+                (size_t)wl * walker_lds_bytes<NFP, T>();
+  T *colbuf = reinterpret_cast<T *>(wbase);  // [2][NFP]
+  uint8_t *st = reinterpret_cast<uint8_t *>(wbase + 2 * NFP * sizeof(T));
   uint8_t *sl = st + 64;
 
-  double a[NFP];
+  T a[NFP];
   {
-    const double2 *src = reinterpret_cast<const double2 *>(
-        g.ainv + ((size_t)w * NFP + col) * NFP);
+    const T2 *src = reinterpret_cast<const T2 *>(ainv + ((size_t)w * NFP + col) * NFP);
 #pragma unroll
     for (int j = 0; j < NFP / 2; ++j) {
-      const double2 v = src[j];
+      const T2 v = src[j];
       a[2 * j] = v.x;
       a[2 * j + 1] = v.y;
     }
@@ -215,6 +262,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs g) {
   int since = g.since_refresh[w];
   int64_t nacc = 0;
   const uint32_t k0 = (uint32_t)g.seed, k1 = (uint32_t)(g.seed >> 32);
+  const int64_t off0 = g.sweep_offset + (g.offset_dev ? g.offset_dev[0] : 0);
 
 #pragma unroll 1
   for (int sw = 0; sw < g.n_sweeps; ++sw) {
@@ -227,9 +275,8 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs g) {
       double uc, ua;
       int b0, b2;
       if (g.u_choice == nullptr) {
-        const u32x4 o = philox4x32_10((uint32_t)e,
-                                      (uint32_t)(g.sweep_offset + sw),
-                                      (uint32_t)w, 0u, k0, k1);
+        const u32x4 o = philox4x32_10((uint32_t)e, (uint32_t)(off0 + sw), (uint32_t)w, 0u,
+                                      k0, k1);
         uc = u53(o.x, o.y);
         ua = u53(o.z, o.w);
         b0 = o.y & 1u;
@@ -275,13 +322,13 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs g) {
         ++nm;
       }
       const int kk1 = sl[af[0]];
-      const double v1 = dot_row<NFP>(Ms + bt[0] * NFP, a);
-      const double r11 = __shfl(v1, gbase + kk1, 64);
-      double det, v2 = 0.0, r12 = 0.0, r21 = 0.0, r22 = 0.0;
+      const T v1 = dot_row<NFP, T>(Ms + bt[0] * NFP, a);
+      const T r11 = __shfl(v1, gbase + kk1, 64);
+      T det, v2 = 0, r12 = 0, r21 = 0, r22 = 0;
       int kk2 = 0;
       if (nm == 2) {
         kk2 = sl[af[1]];
-        v2 = dot_row<NFP>(Ms + bt[1] * NFP, a);
+        v2 = dot_row<NFP, T>(Ms + bt[1] * NFP, a);
         r12 = __shfl(v1, gbase + kk2, 64);
         r21 = __shfl(v2, gbase + kk1, 64);
         r22 = __shfl(v2, gbase + kk2, 64);
@@ -289,7 +336,8 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs g) {
       } else {
         det = r11;
       }
-      if (!(ua < det * det)) continue;
+      const double detd = (double)det;
+      if (!(ua < detd * detd)) continue;
 
       // ---- accepted: Sherman-Morrison / Woodbury update of A^-1
       if (col == kk1) {
@@ -301,16 +349,16 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs g) {
         for (int jj = 0; jj < NFP; ++jj) colbuf[NFP + jj] = a[jj];
       }
       wave_lds_fence();
-      const double rdet = 1.0 / det;
+      const T rdet = T(1) / det;
       if (nm == 1) {
-        const double c1 = (v1 - (col == kk1 ? 1.0 : 0.0)) * rdet;
+        const T c1 = (v1 - (col == kk1 ? T(1) : T(0))) * rdet;
 #pragma unroll
         for (int jj = 0; jj < NFP; ++jj) a[jj] = fma(-colbuf[jj], c1, a[jj]);
       } else {
-        const double w1 = v1 - (col == kk1 ? 1.0 : 0.0);
-        const double w2 = v2 - (col == kk2 ? 1.0 : 0.0);
-        const double c1 = (r22 * w1 - r12 * w2) * rdet;
-        const double c2 = (r11 * w2 - r21 * w1) * rdet;
+        const T w1 = v1 - (col == kk1 ? T(1) : T(0));
+        const T w2 = v2 - (col == kk2 ? T(1) : T(0));
+        const T c1 = (r22 * w1 - r12 * w2) * rdet;
+        const T c2 = (r11 * w2 - r21 * w1) * rdet;
 #pragma unroll
         for (int jj = 0; jj < NFP; ++jj)
           a[jj] = fma(-colbuf[jj], c1, fma(-colbuf[NFP + jj], c2, a[jj]));
@@ -321,8 +369,8 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs g) {
         const int sa = af[m] - spin[m] * N, sb = bt[m] - spin[m] * N;
         cnt += __popcll((spin[m] ? dn : up) & between_mask(sa, sb));
       }
-      sgn *= ((det < 0.0) != (cnt & 1)) ? -1.0 : 1.0;
-      la += log(fabs(det));
+      sgn *= ((detd < 0.0) != (cnt & 1)) ? -1.0 : 1.0;
+      la += log(fabs(detd));
       // occupation and slot bookkeeping
       for (int m = 0; m < nm; ++m) {
         const int sa = af[m] - spin[m] * N, sb = bt[m] - spin[m] * N;
@@ -347,11 +395,14 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs g) {
   }
 
   {
-    double2 *dst = reinterpret_cast<double2 *>(g.ainv +
-                                               ((size_t)w * NFP + col) * NFP);
+    T2 *dst = reinterpret_cast<T2 *>(ainv + ((size_t)w * NFP + col) * NFP);
 #pragma unroll
-    for (int jj = 0; jj < NFP / 2; ++jj)
-      dst[jj] = make_double2(a[2 * jj], a[2 * jj + 1]);
+    for (int jj = 0; jj < NFP / 2; ++jj) {
+      T2 v;
+      v.x = a[2 * jj];
+      v.y = a[2 * jj + 1];
+      dst[jj] = v;
+    }
   }
   for (int s = col; s < N; s += NFP) cfg[s] = st[s];
   for (int o = col; o < n_orb; o += NFP) g.slot[(size_t)w * n_orb + o] = sl[o];
@@ -364,22 +415,25 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs g) {
 }
 
 // ========================================================= local energy
-template <int NFP>
-__global__ void __launch_bounds__(256)
+// T: the storage type of M and A^-1 (the ratios are formed in T, the sum
+// over connected configurations in FP64)
+template <int NFP, typename T>
+__global__ void __launch_bounds__(512)
     local_energy_kernel(int64_t B, int n_sites, int n_f,
-                        const double *__restrict__ M, int n_edges,
+                        const T *__restrict__ M, int n_edges,
                         const int32_t *__restrict__ edges, double t, double U,
                         const int64_t *__restrict__ configs,
-                        const double *__restrict__ ainv,
+                        const T *__restrict__ ainv,
                         const uint8_t *__restrict__ slot, double *eloc) {
   constexpr int WPW = 64 / NFP;
+  using T2 = typename Vec2<T>::type;
   extern __shared__ __align__(16) char smem[];
   const int N = n_sites, n_orb = 2 * N;
-  double *Ms = reinterpret_cast<double *>(smem);
-  uint64_t *nbr = reinterpret_cast<uint64_t *>(Ms + n_orb * NFP);  // [64]
+  T *Ms = reinterpret_cast<T *>(smem);
+  uint64_t *nbr = reinterpret_cast<uint64_t *>(smem + (size_t)n_orb * NFP * sizeof(T));
   for (int idx = threadIdx.x; idx < n_orb * NFP; idx += blockDim.x) {
     const int r = idx / NFP, c = idx % NFP;
-    Ms[idx] = (c < n_f) ? M[(int64_t)r * n_f + c] : 0.0;
+    Ms[idx] = (c < n_f) ? M[(int64_t)r * n_f + c] : T(0);
   }
   for (int s = threadIdx.x; s < N; s += blockDim.x) {
     uint64_t m = 0;
@@ -400,13 +454,12 @@ __global__ void __launch_bounds__(256)
   if (w >= B) return;
   int *orbs = reinterpret_cast<int *>(nbr + 64) + wl * NFP;
 
-  double a[NFP];
+  T a[NFP];
   {
-    const double2 *src = reinterpret_cast<const double2 *>(
-        ainv + ((size_t)w * NFP + col) * NFP);
+    const T2 *src = reinterpret_cast<const T2 *>(ainv + ((size_t)w * NFP + col) * NFP);
 #pragma unroll
     for (int j = 0; j < NFP / 2; ++j) {
-      const double2 v = src[j];
+      const T2 v = src[j];
       a[2 * j] = v.x;
       a[2 * j + 1] = v.y;
     }
@@ -430,7 +483,7 @@ __global__ void __launch_bounds__(256)
     const uint64_t occ = bspin ? dn : up;
     if ((occ >> bsite) & 1ull) continue;          // target must be empty
     if ((nbr[bsite] & occ) == 0ull) continue;     // needs an occupied neighbour
-    const double v = dot_row<NFP>(Ms + b * NFP, a);
+    const double v = (double)dot_row<NFP, T>(Ms + b * NFP, a);
     if (my_orb >= 0 && my_spin == bspin && ((nbr[bsite] >> my_site) & 1ull)) {
       const int lo = my_site < bsite ? my_site : bsite;
       const int hi = my_site < bsite ? bsite : my_site;
@@ -445,10 +498,10 @@ __global__ void __launch_bounds__(256)
 }
 
 // ======================================================= log-derivative
-template <typename T>
+template <typename T, typename TA>
 __global__ void __launch_bounds__(256)
     logderiv_kernel(int64_t B, int n_sites, int n_f, int nfp,
-                    const double *__restrict__ ainv,
+                    const TA *__restrict__ ainv,
                     const uint8_t *__restrict__ slot, T *O, int64_t ld) {
   __shared__ uint8_t sl[128];
   const int64_t w = blockIdx.x;
@@ -458,7 +511,7 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   const int total = n_orb * n_f;
   T *row = O + w * ld;
-  const double *base = ainv + (size_t)w * nfp * nfp;
+  const TA *base = ainv + (size_t)w * nfp * nfp;
   for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
     const int i = idx / n_f, j = idx - i * n_f;
     const uint8_t s = sl[i];
@@ -467,32 +520,38 @@ __global__ void __launch_bounds__(256)
 }
 
 // ============================================================ launchers
-template <int NFP>
+// key of the launch-shape table: the padded width and the storage precision
+template <int NFP, typename T>
+constexpr int tune_key() {
+  return 2 * NFP + (sizeof(T) == 4 ? 1 : 0);
+}
+
+template <int NFP, typename T>
 static int launch_refresh(int64_t B, int n_sites, int n_f, const double *M,
-                          const int64_t *configs, double *ainv, uint8_t *slot,
+                          const int64_t *configs, T *ainv, uint8_t *slot,
                           double *sign, double *logabs, int32_t *since,
                           int only_if_since, hipStream_t st) {
   constexpr int WPW = 64 / NFP;
-  const int waves = 4;
+  const int waves = tuned_waves(TUNE_SLATER_REFRESH, tune_key<NFP, T>(), 4);
   const int64_t per_block = waves * WPW;
   const int64_t nblk = (B + per_block - 1) / per_block;
   const size_t lds = (size_t)per_block * NFP * (sizeof(double) + 1);
   if (ainv)
-    hipLaunchKernelGGL((refresh_kernel<NFP, true>), dim3((unsigned)nblk), dim3(64 * waves),
+    hipLaunchKernelGGL((refresh_kernel<NFP, true, T>), dim3((unsigned)nblk), dim3(64 * waves),
                        lds, st, B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since,
                        only_if_since);
   else
-    hipLaunchKernelGGL((refresh_kernel<NFP, false>), dim3((unsigned)nblk), dim3(64 * waves),
+    hipLaunchKernelGGL((refresh_kernel<NFP, false, T>), dim3((unsigned)nblk), dim3(64 * waves),
                        lds, st, B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since,
                        only_if_since);
   VMCX_LAUNCHED();
   return 0;
 }
 
-int slater_refresh(int64_t B, int n_sites, int n_f, const double *M,
-                   const int64_t *configs, double *ainv, uint8_t *slot,
-                   double *sign, double *logabs, int32_t *since,
-                   int only_if_since, hipStream_t st) {
+template <typename T>
+static int refresh_t(int64_t B, int n_sites, int n_f, const double *M, const int64_t *configs,
+                     T *ainv, uint8_t *slot, double *sign, double *logabs, int32_t *since,
+                     int only_if_since, hipStream_t st) {
   VMCX_CHECK_ARG(B >= 0, "B < 0");
   VMCX_CHECK_ARG(n_sites >= 1 && n_sites <= 64, "n_sites must be in [1, 64]");
   VMCX_CHECK_ARG(n_f >= 1 && n_f <= 64, "n_f must be in [1, 64]");
@@ -502,34 +561,54 @@ int slater_refresh(int64_t B, int n_sites, int n_f, const double *M,
                  "ainv and slot must both be given or both be NULL");
   if (B == 0) return 0;
   switch (nfp_of(n_f)) {
-    case 8: return launch_refresh<8>(B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since, only_if_since, st);
-    case 16: return launch_refresh<16>(B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since, only_if_since, st);
-    case 32: return launch_refresh<32>(B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since, only_if_since, st);
-    default: return launch_refresh<64>(B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since, only_if_since, st);
+    case 8: return launch_refresh<8, T>(B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since, only_if_since, st);
+    case 16: return launch_refresh<16, T>(B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since, only_if_since, st);
+    case 32: return launch_refresh<32, T>(B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since, only_if_since, st);
+    default: return launch_refresh<64, T>(B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since, only_if_since, st);
   }
 }
 
-template <int NFP>
+int slater_refresh(int64_t B, int n_sites, int n_f, const double *M,
+                   const int64_t *configs, double *ainv, uint8_t *slot,
+                   double *sign, double *logabs, int32_t *since,
+                   int only_if_since, hipStream_t st) {
+  return refresh_t<double>(B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since,
+                           only_if_since, st);
+}
+
+int slater_refresh_f32(int64_t B, int n_sites, int n_f, const double *M,
+                       const int64_t *configs, float *ainv, uint8_t *slot, double *sign,
+                       double *logabs, int32_t *since, int only_if_since, hipStream_t st) {
+  return refresh_t<float>(B, n_sites, n_f, M, configs, ainv, slot, sign, logabs, since,
+                          only_if_since, st);
+}
+
+template <int NFP, typename T>
 static int launch_sweep(const SweepArgs &a, hipStream_t st) {
   constexpr int WPW = 64 / NFP;
-  const int waves = 4;
+  const int waves = tuned_waves(TUNE_SLATER_SWEEP, tune_key<NFP, T>(), 4);
   const int64_t per_block = waves * WPW;
   const int64_t nblk = (a.B + per_block - 1) / per_block;
-  const size_t lds = (size_t)2 * a.n_sites * NFP * sizeof(double) +
-                     (size_t)per_block * walker_lds_bytes<NFP>();
+  const size_t lds = (size_t)2 * a.n_sites * NFP * sizeof(T) +
+                     (size_t)per_block * walker_lds_bytes<NFP, T>();
   VMCX_CHECK_ARG(lds <= max_lds_per_block(), "LDS request exceeds the device limit");
-  hipLaunchKernelGGL(sweep_kernel<NFP>, dim3((unsigned)nblk), dim3(64 * waves),
-                     lds, st, a);
+  hipLaunchKernelGGL((sweep_kernel<NFP, T>), dim3((unsigned)nblk), dim3(64 * waves), lds, st,
+                     a);
   VMCX_LAUNCHED();
   return 0;
 }
 
-int slater_sweep(const SweepArgs &a, int refresh_every, hipStream_t st) {
+// T = double: a.M / a.ainv; T = float: a.M32 / a.ainv32 for the sweep and
+// a.M (FP64) for the exact refresh between sweeps
+template <typename T>
+static int sweep_t(const SweepArgs &a, int refresh_every, hipStream_t st) {
   VMCX_CHECK_ARG(a.B >= 0 && a.n_sweeps >= 0 && a.n_edges >= 0, "negative size");
   VMCX_CHECK_ARG(a.n_sites >= 1 && a.n_sites <= 64, "n_sites must be in [1, 64]");
   VMCX_CHECK_ARG(a.n_f >= 1 && a.n_f <= 64 && a.n_f <= 2 * a.n_sites, "bad n_f");
-  VMCX_CHECK_ARG(a.M && a.configs && a.ainv && a.slot && a.sign && a.logabs &&
-                     a.since_refresh, "null state pointer");
+  const bool f32 = sizeof(T) == 4;
+  VMCX_CHECK_ARG(a.M && a.configs && (f32 ? (a.ainv32 && a.M32) : a.ainv != nullptr) &&
+                     a.slot && a.sign && a.logabs && a.since_refresh,
+                 "null state pointer");
   VMCX_CHECK_ARG(a.n_edges == 0 || a.edges, "null edges");
   VMCX_CHECK_ARG((a.u_choice == nullptr) == (a.u_acc == nullptr) &&
                      (a.u_choice == nullptr) == (a.bits == nullptr),
@@ -550,45 +629,52 @@ int slater_sweep(const SweepArgs &a, int refresh_every, hipStream_t st) {
     }
     int rc;
     switch (nfp) {
-      case 8: rc = launch_sweep<8>(one, st); break;
-      case 16: rc = launch_sweep<16>(one, st); break;
-      case 32: rc = launch_sweep<32>(one, st); break;
-      default: rc = launch_sweep<64>(one, st); break;
+      case 8: rc = launch_sweep<8, T>(one, st); break;
+      case 16: rc = launch_sweep<16, T>(one, st); break;
+      case 32: rc = launch_sweep<32, T>(one, st); break;
+      default: rc = launch_sweep<64, T>(one, st); break;
     }
     if (rc) return rc;
     if (refresh_every > 0) {
-      rc = slater_refresh(a.B, a.n_sites, a.n_f, a.M, a.configs, a.ainv,
-                          a.slot, a.sign, a.logabs, a.since_refresh,
-                          refresh_every, st);
+      T *ainv = f32 ? (T *)a.ainv32 : (T *)a.ainv;
+      rc = refresh_t<T>(a.B, a.n_sites, a.n_f, a.M, a.configs, ainv, a.slot, a.sign,
+                        a.logabs, a.since_refresh, refresh_every, st);
       if (rc) return rc;
     }
   }
   return 0;
 }
 
-template <int NFP>
-static int launch_eloc(int64_t B, int n_sites, int n_f, const double *M,
-                       int n_edges, const int32_t *edges, double t, double U,
-                       const int64_t *configs, const double *ainv,
-                       const uint8_t *slot, double *eloc, hipStream_t st) {
+int slater_sweep(const SweepArgs &a, int refresh_every, hipStream_t st) {
+  return sweep_t<double>(a, refresh_every, st);
+}
+
+int slater_sweep_f32(const SweepArgs &a, int refresh_every, hipStream_t st) {
+  return sweep_t<float>(a, refresh_every, st);
+}
+
+template <int NFP, typename T>
+static int launch_eloc(int64_t B, int n_sites, int n_f, const T *M, int n_edges,
+                       const int32_t *edges, double t, double U, const int64_t *configs,
+                       const T *ainv, const uint8_t *slot, double *eloc, hipStream_t st) {
   constexpr int WPW = 64 / NFP;
-  const int waves = 4;
+  const int waves = tuned_waves(TUNE_SLATER_ELOC, tune_key<NFP, T>(), 4);
   const int64_t per_block = waves * WPW;
   const int64_t nblk = (B + per_block - 1) / per_block;
-  const size_t lds = (size_t)2 * n_sites * NFP * sizeof(double) +
+  const size_t lds = (size_t)2 * n_sites * NFP * sizeof(T) +
                      64 * sizeof(uint64_t) + (size_t)per_block * NFP * sizeof(int);
   VMCX_CHECK_ARG(lds <= max_lds_per_block(), "LDS request exceeds the device limit");
-  hipLaunchKernelGGL(local_energy_kernel<NFP>, dim3((unsigned)nblk),
-                     dim3(64 * waves), lds, st, B, n_sites, n_f, M, n_edges,
-                     edges, t, U, configs, ainv, slot, eloc);
+  hipLaunchKernelGGL((local_energy_kernel<NFP, T>), dim3((unsigned)nblk), dim3(64 * waves),
+                     lds, st, B, n_sites, n_f, M, n_edges, edges, t, U, configs, ainv, slot,
+                     eloc);
   VMCX_LAUNCHED();
   return 0;
 }
 
-int slater_local_energy(int64_t B, int n_sites, int n_f, const double *M,
-                        int n_edges, const int32_t *edges, double t, double U,
-                        const int64_t *configs, const double *ainv,
-                        const uint8_t *slot, double *eloc, hipStream_t st) {
+template <typename T>
+static int eloc_t(int64_t B, int n_sites, int n_f, const T *M, int n_edges,
+                  const int32_t *edges, double t, double U, const int64_t *configs,
+                  const T *ainv, const uint8_t *slot, double *eloc, hipStream_t st) {
   VMCX_CHECK_ARG(B >= 0 && n_edges >= 0, "negative size");
   VMCX_CHECK_ARG(n_sites >= 1 && n_sites <= 64, "n_sites must be in [1, 64]");
   VMCX_CHECK_ARG(n_f >= 1 && n_f <= 64 && n_f <= 2 * n_sites, "bad n_f");
@@ -596,16 +682,32 @@ int slater_local_energy(int64_t B, int n_sites, int n_f, const double *M,
   VMCX_CHECK_ARG(n_edges == 0 || edges, "null edges");
   if (B == 0) return 0;
   switch (nfp_of(n_f)) {
-    case 8: return launch_eloc<8>(B, n_sites, n_f, M, n_edges, edges, t, U, configs, ainv, slot, eloc, st);
-    case 16: return launch_eloc<16>(B, n_sites, n_f, M, n_edges, edges, t, U, configs, ainv, slot, eloc, st);
-    case 32: return launch_eloc<32>(B, n_sites, n_f, M, n_edges, edges, t, U, configs, ainv, slot, eloc, st);
-    default: return launch_eloc<64>(B, n_sites, n_f, M, n_edges, edges, t, U, configs, ainv, slot, eloc, st);
+    case 8: return launch_eloc<8, T>(B, n_sites, n_f, M, n_edges, edges, t, U, configs, ainv, slot, eloc, st);
+    case 16: return launch_eloc<16, T>(B, n_sites, n_f, M, n_edges, edges, t, U, configs, ainv, slot, eloc, st);
+    case 32: return launch_eloc<32, T>(B, n_sites, n_f, M, n_edges, edges, t, U, configs, ainv, slot, eloc, st);
+    default: return launch_eloc<64, T>(B, n_sites, n_f, M, n_edges, edges, t, U, configs, ainv, slot, eloc, st);
   }
 }
 
-int slater_logderiv(int64_t B, int n_sites, int n_f, const double *ainv,
-                    const uint8_t *slot, void *O, int o_dtype, int64_t ld,
-                    hipStream_t st) {
+int slater_local_energy(int64_t B, int n_sites, int n_f, const double *M,
+                        int n_edges, const int32_t *edges, double t, double U,
+                        const int64_t *configs, const double *ainv,
+                        const uint8_t *slot, double *eloc, hipStream_t st) {
+  return eloc_t<double>(B, n_sites, n_f, M, n_edges, edges, t, U, configs, ainv, slot, eloc,
+                        st);
+}
+
+int slater_local_energy_f32(int64_t B, int n_sites, int n_f, const float *M, int n_edges,
+                            const int32_t *edges, double t, double U, const int64_t *configs,
+                            const float *ainv, const uint8_t *slot, double *eloc,
+                            hipStream_t st) {
+  return eloc_t<float>(B, n_sites, n_f, M, n_edges, edges, t, U, configs, ainv, slot, eloc,
+                       st);
+}
+
+template <typename TA>
+static int logderiv_t(int64_t B, int n_sites, int n_f, const TA *ainv, const uint8_t *slot,
+                      void *O, int o_dtype, int64_t ld, hipStream_t st) {
   VMCX_CHECK_ARG(B >= 0, "B < 0");
   VMCX_CHECK_ARG(n_sites >= 1 && n_sites <= 64, "n_sites must be in [1, 64]");
   VMCX_CHECK_ARG(n_f >= 1 && n_f <= 64 && n_f <= 2 * n_sites, "bad n_f");
@@ -616,11 +718,49 @@ int slater_logderiv(int64_t B, int n_sites, int n_f, const double *ainv,
   if (B == 0) return 0;
   const int nfp = nfp_of(n_f);
   if (o_dtype == 0)
-    hipLaunchKernelGGL(logderiv_kernel<double>, dim3((unsigned)B), dim3(256), 0,
-                       st, B, n_sites, n_f, nfp, ainv, slot, (double *)O, ld);
+    hipLaunchKernelGGL((logderiv_kernel<double, TA>), dim3((unsigned)B), dim3(256), 0, st, B,
+                       n_sites, n_f, nfp, ainv, slot, (double *)O, ld);
   else
-    hipLaunchKernelGGL(logderiv_kernel<float>, dim3((unsigned)B), dim3(256), 0,
-                       st, B, n_sites, n_f, nfp, ainv, slot, (float *)O, ld);
+    hipLaunchKernelGGL((logderiv_kernel<float, TA>), dim3((unsigned)B), dim3(256), 0, st, B,
+                       n_sites, n_f, nfp, ainv, slot, (float *)O, ld);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+int slater_logderiv(int64_t B, int n_sites, int n_f, const double *ainv,
+                    const uint8_t *slot, void *O, int o_dtype, int64_t ld,
+                    hipStream_t st) {
+  return logderiv_t<double>(B, n_sites, n_f, ainv, slot, O, o_dtype, ld, st);
+}
+
+int slater_logderiv_f32(int64_t B, int n_sites, int n_f, const float *ainv,
+                        const uint8_t *slot, void *O, int o_dtype, int64_t ld,
+                        hipStream_t st) {
+  return logderiv_t<float>(B, n_sites, n_f, ainv, slot, O, o_dtype, ld, st);
+}
+
+// ctr[0] += inc (a draw counter advanced inside a captured graph)
+__global__ void counter_add_kernel(int64_t *ctr, int64_t inc) { ctr[0] += inc; }
+
+int counter_add(int64_t *ctr, int64_t inc, hipStream_t st) {
+  VMCX_CHECK_ARG(ctr != nullptr, "null counter");
+  hipLaunchKernelGGL(counter_add_kernel, dim3(1), dim3(1), 0, st, ctr, inc);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+// p <- p - lr * dp for FP32 parameters (the update computed in FP64)
+__global__ void axpy_f32_kernel(int64_t n, double a, const double *x, float *y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)((double)y[i] + a * x[i]);
+}
+
+int sgd_update_f32(int64_t n, float *params, const double *dp, double lr, hipStream_t st) {
+  VMCX_CHECK_ARG(n >= 0, "n < 0");
+  if (n == 0) return 0;
+  VMCX_CHECK_ARG(params && dp, "null pointer");
+  hipLaunchKernelGGL(axpy_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
+                     -lr, dp, params);
   VMCX_LAUNCHED();
   return 0;
 }

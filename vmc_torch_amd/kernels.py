@@ -20,8 +20,10 @@ def slater_nfp(n_f):
 
 @dataclass
 class SlaterWalkerState:
-    """Device-resident Slater state of a batch of walkers (see vmcx.h)."""
-    ainv: torch.Tensor        # [B, NFP, NFP] float64, column k of A^-1 contiguous
+    """Device-resident Slater state of a batch of walkers (see vmcx.h).  The
+    dtype of ainv selects the form: float64 (plain) or float32 (the FP32
+    form: FP32 ratios and updates, FP64 refresh)."""
+    ainv: torch.Tensor        # [B, NFP, NFP] float64 / float32, column k of A^-1 contiguous
     slot: torch.Tensor        # [B, 2N] uint8
     sign: torch.Tensor        # [B] float64
     logabs: torch.Tensor      # [B] float64
@@ -29,10 +31,12 @@ class SlaterWalkerState:
     n_accept: torch.Tensor    # [B] int64
 
     @classmethod
-    def empty(cls, B, n_sites, n_f, device):
+    def empty(cls, B, n_sites, n_f, device, dtype=F64):
+        if dtype not in (torch.float64, torch.float32):
+            raise TypeError("the Slater walker state is float64 or float32")
         nfp = slater_nfp(n_f)
         return cls(
-            ainv=torch.empty(B, nfp, nfp, dtype=F64, device=device),
+            ainv=torch.empty(B, nfp, nfp, dtype=dtype, device=device),
             slot=torch.empty(B, 2 * n_sites, dtype=U8, device=device),
             sign=torch.empty(B, dtype=F64, device=device),
             logabs=torch.empty(B, dtype=F64, device=device),
@@ -68,26 +72,35 @@ def slater_refresh(M, configs, state=None, only_if_since=-1, sign=None, logabs=N
                                         stream_ptr(M.device)))
         return sign, logabs
     nfp = slater_nfp(n_f)
-    require(state.ainv, F64, "ainv", (B, nfp, nfp), M.device)
+    f32 = state.ainv.dtype == torch.float32
+    require(state.ainv, state.ainv.dtype, "ainv", (B, nfp, nfp), M.device)
     require(state.slot, U8, "slot", (B, 2 * N), M.device)
     require(state.since_refresh, I32, "since_refresh", (B,), M.device)
-    check(lib().vmcx_slater_refresh(B, N, n_f, ptr(M), ptr(configs), ptr(state.ainv),
-                                    ptr(state.slot), ptr(state.sign), ptr(state.logabs),
-                                    ptr(state.since_refresh), int(only_if_since),
-                                    stream_ptr(M.device)))
+    fn = lib().vmcx_slater_refresh_f32 if f32 else lib().vmcx_slater_refresh
+    check(fn(B, N, n_f, ptr(M), ptr(configs), ptr(state.ainv), ptr(state.slot),
+             ptr(state.sign), ptr(state.logabs), ptr(state.since_refresh), int(only_if_since),
+             stream_ptr(M.device)))
     return state.sign, state.logabs
 
 
 def slater_sweep(M, edges, configs, state, n_sweeps=1, seed=0, sweep_offset=0,
-                 hopping_rate=0.25, refresh_every=64, draws=None):
+                 hopping_rate=0.25, refresh_every=64, draws=None, M32=None, offset_dev=None):
     """n_sweeps Metropolis sweeps in place.  draws = (u_choice, bits, u_acc),
-    each (n_sweeps, n_edges, B), replaces the Philox stream."""
+    each (n_sweeps, n_edges, B), replaces the Philox stream.  With a float32
+    state the sweep runs in FP32 from M32 (M rounded to FP32; formed here
+    when not given) and the exact refresh in FP64 from M.  offset_dev: a
+    one-element int64 device tensor added to sweep_offset when the kernel
+    runs (graph replays advance it with counter_add)."""
     B, N = _check_configs(configs, M.device)
     n_f = _check_m(M, N)
     nfp = slater_nfp(n_f)
     require(edges, I32, "edges", device=M.device)
     E = edges.shape[0]
-    for name, t, dt, shp in (("ainv", state.ainv, F64, (B, nfp, nfp)),
+    f32 = state.ainv.dtype == torch.float32
+    if f32:
+        M32 = M.to(torch.float32).contiguous() if M32 is None else M32
+        require(M32, torch.float32, "M32", tuple(M.shape), M.device)
+    for name, t, dt, shp in (("ainv", state.ainv, state.ainv.dtype, (B, nfp, nfp)),
                              ("slot", state.slot, U8, (B, 2 * N)),
                              ("sign", state.sign, F64, (B,)),
                              ("logabs", state.logabs, F64, (B,)),
@@ -100,20 +113,36 @@ def slater_sweep(M, edges, configs, state, n_sweeps=1, seed=0, sweep_offset=0,
         require(uc, F64, "u_choice", (n_sweeps, E, B), M.device)
         require(bits, U8, "bits", (n_sweeps, E, B), M.device)
         require(ua, F64, "u_acc", (n_sweeps, E, B), M.device)
-    check(lib().vmcx_slater_sweep(B, N, n_f, ptr(M), E, ptr(edges), float(hopping_rate),
-                                  int(n_sweeps), int(seed) & (2 ** 64 - 1), int(sweep_offset),
-                                  ptr(uc), ptr(bits), ptr(ua), int(refresh_every),
-                                  ptr(configs), ptr(state.ainv), ptr(state.slot),
-                                  ptr(state.sign), ptr(state.logabs),
-                                  ptr(state.since_refresh), ptr(state.n_accept),
-                                  stream_ptr(M.device)))
+    head = (B, N, n_f, ptr(M)) + ((ptr(M32),) if f32 else ())
+    fn = lib().vmcx_slater_sweep_f32 if f32 else lib().vmcx_slater_sweep
+    check(fn(*head, E, ptr(edges), float(hopping_rate), int(n_sweeps),
+             int(seed) & (2 ** 64 - 1), int(sweep_offset), ptr(uc), ptr(bits), ptr(ua),
+             int(refresh_every), ptr(configs), ptr(state.ainv), ptr(state.slot),
+             ptr(state.sign), ptr(state.logabs), ptr(state.since_refresh),
+             ptr(state.n_accept), ptr(offset_dev), stream_ptr(M.device)))
 
 
-def slater_local_energy(M, edges, t, U, configs, state, out=None):
+def counter_add(counter, inc):
+    """counter[0] += inc on the current stream (graph-capturable)."""
+    require(counter, I64, "counter", (1,))
+    check(lib().vmcx_counter_add(ptr(counter), int(inc), stream_ptr(counter.device)))
+
+
+def slater_local_energy(M, edges, t, U, configs, state, out=None, M32=None):
+    """E_loc of every walker from its state (a float32 state: ratios from
+    M32, M rounded to FP32, formed here when not given)."""
     B, N = _check_configs(configs, M.device)
     n_f = _check_m(M, N)
     require(edges, I32, "edges", device=M.device)
     out = torch.empty(B, dtype=F64, device=M.device) if out is None else require(out, F64, "eloc", (B,))
+    if state.ainv.dtype == torch.float32:
+        M32 = M.to(torch.float32).contiguous() if M32 is None else M32
+        require(M32, torch.float32, "M32", tuple(M.shape), M.device)
+        check(lib().vmcx_slater_local_energy_f32(B, N, n_f, ptr(M32), edges.shape[0],
+                                                 ptr(edges), float(t), float(U), ptr(configs),
+                                                 ptr(state.ainv), ptr(state.slot), ptr(out),
+                                                 stream_ptr(M.device)))
+        return out
     check(lib().vmcx_slater_local_energy(B, N, n_f, ptr(M), edges.shape[0], ptr(edges),
                                          float(t), float(U), ptr(configs), ptr(state.ainv),
                                          ptr(state.slot), ptr(out), stream_ptr(M.device)))
@@ -129,9 +158,10 @@ def slater_logderiv(state, n_sites, n_f, dtype=F64, out=None):
     if out.dim() != 2 or out.shape[0] != B or out.shape[1] < np_:
         raise ValueError("O must be (B, >= 2*n_sites*n_f)")
     code = {torch.float64: 0, torch.float32: 1}[dtype]
-    check(lib().vmcx_slater_logderiv(B, n_sites, n_f, ptr(state.ainv), ptr(state.slot),
-                                     ptr(out), code, out.stride(0),
-                                     stream_ptr(state.ainv.device)))
+    fn = lib().vmcx_slater_logderiv_f32 if state.ainv.dtype == torch.float32 \
+        else lib().vmcx_slater_logderiv
+    check(fn(B, n_sites, n_f, ptr(state.ainv), ptr(state.slot), ptr(out), code, out.stride(0),
+             stream_ptr(state.ainv.device)))
     return out
 
 
@@ -400,11 +430,14 @@ def energy_stats(E, out=None):
 
 
 def sgd_update(param, dp, lr):
-    """param -= lr * dp (in place, flat view)."""
-    require(param, F64, "param")
+    """param -= lr * dp (in place, flat view; float64 or float32 param, the
+    update computed in FP64)."""
+    if param.dtype not in (torch.float64, torch.float32):
+        raise TypeError("param must be float64 or float32")
+    require(param, param.dtype, "param")
     require(dp, F64, "dp", (param.numel(),), param.device)
-    check(lib().vmcx_sgd_update(param.numel(), ptr(param), ptr(dp), float(lr),
-                                stream_ptr(param.device)))
+    fn = lib().vmcx_sgd_update if param.dtype == torch.float64 else lib().vmcx_sgd_update_f32
+    check(fn(param.numel(), ptr(param), ptr(dp), float(lr), stream_ptr(param.device)))
 
 
 def syrk_f64(M, C, alpha=1.0, beta=0.0, splits=1, rowmajor=True):
@@ -431,17 +464,28 @@ def syrk_f64(M, C, alpha=1.0, beta=0.0, splits=1, rowmajor=True):
 
 
 # ------------------------------------------------------------------ NNBF
-def nnbf_struct(n_sites, n_f, hidden, act, gemm, bf_scale, w32, wbf):
-    """Fill the vmcx_nnbf_model struct from FP32 (and BF16) device copies.
-    w32: dict M, W1, b1, W2, b2, Wout, bout; wbf: dict W2, Wout or None."""
+def nnbf_struct(n_sites, n_f, hidden, act, gemm, bf_scale, w32, wbf, w64=None, path=0):
+    """Fill the vmcx_nnbf_model struct.  gemm 0 / 1: w32 = dict M, W1, b1,
+    W2, b2, Wout, bout of FP32 device copies (wbf: dict W2, Wout of BF16
+    copies for gemm 1, else None); gemm 2: w64 = the same dict of FP64
+    device tensors (w32 unused).  path (BF16 mode): 0 = the library's own
+    matrix-core layers for the forward, 1 = rocBLAS."""
     s = _lib.NnbfModelStruct()
     s.n_sites, s.n_f, s.hidden = n_sites, n_f, hidden
-    s.act, s.gemm, s.reserved = act, gemm, 0
+    s.act, s.gemm, s.path = act, gemm, int(path)
     s.bf_scale = float(bf_scale)
-    for k in ("M", "W1", "b1", "W2", "b2", "Wout", "bout"):
-        t = w32[k]
-        require(t, torch.float32, k)
-        setattr(s, k, t.data_ptr())
+    names = ("M", "W1", "b1", "W2", "b2", "Wout", "bout")
+    for k in names:
+        if gemm == 2:
+            setattr(s, k, None)
+            t = w64[k]
+            require(t, torch.float64, k)
+            setattr(s, k + "64" if k == "M" else k + "_64", t.data_ptr())
+        else:
+            t = w32[k]
+            require(t, torch.float32, k)
+            setattr(s, k, t.data_ptr())
+            setattr(s, k + "64" if k == "M" else k + "_64", None)
     s.W2_bf16 = wbf["W2"].data_ptr() if wbf else None
     s.Wout_bf16 = wbf["Wout"].data_ptr() if wbf else None
     return s
@@ -493,10 +537,12 @@ def peps_layout(Lx, Ly, D):
     return np.concatenate(maps), sizes
 
 
-def peps_struct(Lx, Ly, D, dense_map, theta, nn=None, act=0, eta=0.0):
+def peps_struct(Lx, Ly, D, dense_map, theta, nn=None, act=0, eta=0.0, mode=0):
     """vmcx_peps_model from device tensors: dense_map int32, theta float64,
-    nn = dict W1, b1, W2, b2, Wo, bo (float64) or None."""
+    nn = dict W1, b1, W2, b2, Wo, bo (float64) or None; mode 0 = additive,
+    1 = multiplicative network correction."""
     s = _lib.PepsModelStruct()
+    s.mode, s.reserved = int(mode), 0
     require(dense_map, I32, "dense_map")
     require(theta, F64, "theta")
     s.Lx, s.Ly, s.D, s.n_params = int(Lx), int(Ly), int(D), int(theta.numel())
@@ -577,25 +623,28 @@ def lp_propose(params, C, Lx, Ly, configs, seed, step, u_choice=None, precision=
 
 
 def lp_propose_given(params, C, Lx, Ly, configs, logits_x, seed, step, u_choice=None,
-                     precision="fp32"):
+                     precision="fp32", step_dev=None, out=None):
     """lp_propose with the logits at `configs` given: (move, ln q(m|x),
     proposed configs, ln q(m|x'), logits at the proposed configs)."""
     B, N = _lp_check(params, C, configs, Lx, Ly)
     dev = configs.device
     M = lp_n_moves(Lx, Ly)
     require(logits_x, F64, "logits_x", (B, M), dev)
-    logits_p = torch.empty(B, M, dtype=F64, device=dev)
-    move = torch.empty(B, dtype=I32, device=dev)
-    lqf = torch.empty(B, dtype=F64, device=dev)
-    lqr = torch.empty(B, dtype=F64, device=dev)
-    prop = torch.empty(B, N, dtype=I64, device=dev)
+    if out is None:
+        logits_p = torch.empty(B, M, dtype=F64, device=dev)
+        move = torch.empty(B, dtype=I32, device=dev)
+        lqf = torch.empty(B, dtype=F64, device=dev)
+        lqr = torch.empty(B, dtype=F64, device=dev)
+        prop = torch.empty(B, N, dtype=I64, device=dev)
+    else:
+        move, lqf, prop, lqr, logits_p = out
     if u_choice is not None:
         require(u_choice, F64, "u_choice", (B,), dev)
     check(lib().vmcx_lp_propose_given(ptr(params), int(C), int(Lx), int(Ly), B, ptr(configs),
                                       ptr(logits_x), int(seed) & (2 ** 64 - 1), int(step),
                                       ptr(u_choice), ptr(logits_p), ptr(move), ptr(lqf),
                                       ptr(prop), ptr(lqr), LP_PRECISION[precision],
-                                      stream_ptr(dev)))
+                                      ptr(step_dev), stream_ptr(dev)))
     return move, lqf, prop, lqr, logits_p
 
 
@@ -608,16 +657,16 @@ def lp_commit_logits(accepted, logits_p, logits_x):
 
 
 def lp_accept(configs, sign, logabs, move, proposed, psign, plogabs, logq_f, logq_r, seed, step,
-              u_acc=None, n_accept=None):
+              u_acc=None, n_accept=None, step_dev=None, accepted=None):
     B, N = _check_configs(configs, configs.device)
     dev = configs.device
-    acc = torch.empty(B, dtype=U8, device=dev)
+    acc = torch.empty(B, dtype=U8, device=dev) if accepted is None else accepted
     if u_acc is not None:
         require(u_acc, F64, "u_acc", (B,), dev)
     check(lib().vmcx_lp_accept(B, N, int(seed) & (2 ** 64 - 1), int(step), ptr(u_acc), ptr(move),
                                ptr(proposed), ptr(psign), ptr(plogabs), ptr(logq_f), ptr(logq_r),
                                ptr(configs), ptr(sign), ptr(logabs), ptr(n_accept), ptr(acc),
-                               stream_ptr(dev)))
+                               ptr(step_dev), stream_ptr(dev)))
     return acc
 
 

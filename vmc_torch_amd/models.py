@@ -10,6 +10,7 @@ operations on a walker batch, all in libvmcx:
     log_derivative(fxs, dtype) -> (O [B, Np], (sign, log|psi|))
 and caches the device-side walker state they share.
 """
+import contextlib
 import ctypes
 
 import torch
@@ -47,7 +48,11 @@ class WavefunctionModel_GPU(nn.Module):
 
     def params_changed(self):
         """True when the parameters differ (by value) from the last call that
-        built derived device data; optimizers may write through .data."""
+        built derived device data; optimizers may write through .data.  While
+        the model is frozen (graph capture) the parameters are taken as
+        unchanged without looking (the comparison would wait for the device)."""
+        if getattr(self, "_frozen", False):
+            return False
         flat = self._flat_params()
         if self._snapshot is None or self._snapshot.shape != flat.shape or \
                 self._snapshot.device != flat.device or not torch.equal(self._snapshot, flat):
@@ -58,6 +63,27 @@ class WavefunctionModel_GPU(nn.Module):
     def invalidate(self):
         self._state_key = None
         self._snapshot = None
+
+    @contextlib.contextmanager
+    def frozen(self):
+        """Parameters promised constant inside the block: no value checks
+        (each would synchronise the host), so the model's kernels can be
+        captured into a HIP graph.  Derived device data is brought up to date
+        on entry."""
+        self._frozen = False
+        self._prepare_device_data()
+        self._frozen = True
+        try:
+            yield self
+        finally:
+            self._frozen = False
+
+    def _prepare_device_data(self):
+        """Build the device copies the kernels read (weights, structs)."""
+
+    # True when forward_log launches libvmcx kernels only (no library call that
+    # may allocate or record memset nodes), so it can run under graph capture
+    graph_safe = False
 
     def _walker_key(self, fxs):
         """Identity of the walker tensor (storage, shape, in-place version).
@@ -72,26 +98,53 @@ class SlaterDeterminant_GPU(WavefunctionModel_GPU):
     """psi(x) = det(M[occupied, :]), occupied = netket-ordered occupied
     orbitals ascending (GPU/models/slater.py:15-61; CPU model.py:7437-7457).
 
-    M: (n_orbitals = 2*N_sites, n_fermions) float64.  `init_std` scales the
-    N(0, 1) draw (the reference uses std 1; the benchmark shapes use 0.1).
+    M: (n_orbitals = 2*N_sites, n_fermions), of `dtype` (slater.py:29-34).
+    float64: the plain form, FP64 throughout.  float32: the FP32 form — M and
+    the walkers' A^-1 in FP32, ratios and Sherman-Morrison updates in FP32,
+    the exact refresh, sign and ln|psi| in FP64 (from M widened to FP64,
+    which is exact).  `init_std` scales the N(0, 1) draw (the reference uses
+    std 1; the benchmark shapes use 0.1).
     """
 
     def __init__(self, n_orbitals, n_fermions, dtype=torch.float64, init_std=1.0,
                  generator=None):
-        if dtype != torch.float64:
-            raise ValueError("the Slater kernels are FP64")
-        M = torch.randn(n_orbitals, n_fermions, dtype=dtype, generator=generator) * init_std
+        if dtype not in (torch.float64, torch.float32):
+            raise ValueError("the Slater kernels take float64 or float32")
+        M = (torch.randn(n_orbitals, n_fermions, dtype=torch.float64, generator=generator)
+             * init_std).to(dtype)
         super().__init__([M])
         self.n_orbitals = n_orbitals
         self.n_fermions = n_fermions
         self.n_sites = n_orbitals // 2
+        self._m64 = None
 
     @property
     def M(self):
         return self.params[0]
 
+    @property
+    def fp32(self):
+        return self.M.dtype == torch.float32
+
+    graph_safe = True
+
+    def _prepare_device_data(self):
+        if self.params_changed():
+            self._state_key = None      # the walker state predates the change
+        self._m_f64()
+
+    def _m_f64(self):
+        """M in FP64 for the exact refresh (M itself in the plain form)."""
+        M = self.M.detach()
+        if not self.fp32:
+            return M
+        if self._m64 is None or self._m64[0] is not self._snapshot:
+            self._m64 = (self._snapshot, M.to(torch.float64).contiguous())
+        return self._m64[1]
+
     def forward_log(self, x):
-        return kernels.slater_refresh(self.M.detach(), x.contiguous())
+        self.params_changed()
+        return kernels.slater_refresh(self._m_f64(), x.contiguous())
 
     def walker_state(self, fxs, refresh=False):
         """State for the walker batch `fxs`, rebuilt exactly when fxs or M
@@ -102,24 +155,28 @@ class SlaterDeterminant_GPU(WavefunctionModel_GPU):
             return self._state
         B = fxs.shape[0]
         st = self._state
-        if st is None or st.ainv.shape[0] != B or st.ainv.device != fxs.device:
-            st = kernels.SlaterWalkerState.empty(B, self.n_sites, self.n_fermions, fxs.device)
-        kernels.slater_refresh(self.M.detach(), fxs, st)
+        if st is None or st.ainv.shape[0] != B or st.ainv.device != fxs.device or \
+                st.ainv.dtype != self.M.dtype:
+            st = kernels.SlaterWalkerState.empty(B, self.n_sites, self.n_fermions, fxs.device,
+                                                 dtype=self.M.dtype)
+        kernels.slater_refresh(self._m_f64(), fxs, st)
         self._state, self._state_key = st, self._walker_key(fxs)
         return st
 
     def mcmc_sweep(self, fxs, edges, n_sweeps, seed, sweep_offset, hopping_rate=0.25,
                    refresh_every=64, draws=None):
         st = self.walker_state(fxs)
-        kernels.slater_sweep(self.M.detach(), edges, fxs, st, n_sweeps=n_sweeps, seed=seed,
+        M32 = self.M.detach() if self.fp32 else None
+        kernels.slater_sweep(self._m_f64(), edges, fxs, st, n_sweeps=n_sweeps, seed=seed,
                              sweep_offset=sweep_offset, hopping_rate=hopping_rate,
-                             refresh_every=refresh_every, draws=draws)
+                             refresh_every=refresh_every, draws=draws, M32=M32)
         return st.sign, st.logabs, st.n_accept
 
     def local_energy(self, fxs, H):
         st = self.walker_state(fxs)
         edges = H.graph.edge_tensor(fxs.device, "graph")
-        return kernels.slater_local_energy(self.M.detach(), edges, H.t, H.U, fxs, st)
+        M32 = self.M.detach() if self.fp32 else None
+        return kernels.slater_local_energy(self._m_f64(), edges, H.t, H.U, fxs, st, M32=M32)
 
     def log_derivative(self, fxs, dtype=torch.float64):
         st = self.walker_state(fxs)
@@ -136,18 +193,32 @@ class NNBF_GPU(WavefunctionModel_GPU):
     init="reference": M ~ N(0,1), kaiming-uniform W, zero b, W_out = b_out = 0
     (NNBF.py:86-111); init="xavier": M ~ N(0, m_std^2), Xavier-uniform for
     every W including W_out, zero biases (the benchmark configuration).
-    gemm: "fp32" or "bf16" for the two dense layers (FP32 accumulation).
+    gemm: the arithmetic of the network: "fp64" (the weights, layers and
+    backflow in FP64, as the reference's float64 model; the default for
+    dtype=float64), "fp32" (FP32 copies of the weights, FP32 GEMMs) or
+    "bf16" (the two dense layers on BF16 operands with FP32 accumulation).
+    The determinant is FP64 in every mode.  bf16_layers (BF16 mode):
+    "matrix_cores" (the library's own GEMMs for the forward, only the
+    occupied orbital rows of the output layer formed) or "rocblas".
     """
 
     ACTS = {"relu": 0, "tanh": 1}
 
+    GEMMS = {"fp32": 0, "bf16": 1, "fp64": 2}
+
     def __init__(self, n_sites, n_fermions, hidden_dim=64, n_layers=2, activation="tanh",
-                 bf_scale=0.01, dtype=torch.float64, gemm="fp32", init="reference",
-                 m_std=1.0, generator=None, chunk=65536):
+                 bf_scale=0.01, dtype=torch.float64, gemm=None, init="reference",
+                 m_std=1.0, generator=None, chunk=65536, bf16_layers="matrix_cores"):
         if n_layers != 2:
             raise NotImplementedError("the NNBF kernels take exactly two hidden layers")
         if activation not in self.ACTS:
             raise ValueError(f"activation must be one of {list(self.ACTS)}")
+        if gemm is None:
+            gemm = "fp64" if dtype == torch.float64 else "fp32"
+        if gemm not in self.GEMMS:
+            raise ValueError(f"gemm must be one of {list(self.GEMMS)}")
+        if gemm == "fp64" and dtype != torch.float64:
+            raise ValueError("gemm='fp64' needs dtype=torch.float64")
         n_orb = 2 * n_sites
         g = generator
         if init == "reference":
@@ -173,6 +244,9 @@ class NNBF_GPU(WavefunctionModel_GPU):
         self.n_sites, self.n_fermions, self.hidden = n_sites, n_fermions, hidden_dim
         self.n_layers, self.activation, self.bf_scale = n_layers, activation, bf_scale
         self.gemm = gemm
+        if bf16_layers not in ("matrix_cores", "rocblas"):
+            raise ValueError("bf16_layers must be 'matrix_cores' or 'rocblas'")
+        self.bf16_layers = bf16_layers
         self.chunk = chunk
         self._w32 = None
         self._wbf = None
@@ -180,11 +254,30 @@ class NNBF_GPU(WavefunctionModel_GPU):
         self._struct_obj = None
 
     # ------------------------------------------------------ device weights
+    @property
+    def graph_safe(self):
+        """The BF16 forward on the library's own matrix-core layers launches
+        libvmcx kernels only."""
+        return self.gemm == "bf16" and self.bf16_layers == "matrix_cores"
+
+    def _prepare_device_data(self):
+        self._sync_weights()
+
+    @property
+    def z_dtype(self):
+        """dtype of the first-layer walker state z1."""
+        return torch.float64 if self.gemm == "fp64" else torch.float32
+
     def _sync_weights(self):
-        if self.params_changed() or self._w32 is None:
+        if self.params_changed() or self._struct_obj is None:
             names = ["M", "W1", "b1", "W2", "b2", "Wout", "bout"]
-            self._w32 = {k: p.detach().to(torch.float32).contiguous()
-                         for k, p in zip(names, self.params)}
+            w64 = None
+            if self.gemm == "fp64":
+                w64 = {k: p.detach().contiguous() for k, p in zip(names, self.params)}
+                self._w32 = w64          # keeps the tensors the struct points to alive
+            else:
+                self._w32 = {k: p.detach().to(torch.float32).contiguous()
+                             for k, p in zip(names, self.params)}
             if self.gemm == "bf16":
                 self._wbf = {k: kernels.to_bf16(self._w32[k], torch.empty_like(
                     self._w32[k], dtype=torch.bfloat16)) for k in ("W2", "Wout")}
@@ -192,7 +285,8 @@ class NNBF_GPU(WavefunctionModel_GPU):
                 self._wbf = None
             self._struct_obj = kernels.nnbf_struct(
                 self.n_sites, self.n_fermions, self.hidden, self.ACTS[self.activation],
-                1 if self.gemm == "bf16" else 0, self.bf_scale, self._w32, self._wbf)
+                self.GEMMS[self.gemm], self.bf_scale, self._w32, self._wbf, w64,
+                path=0 if self.bf16_layers == "matrix_cores" else 1)
             self._state_key = None
             return True
         return False
@@ -221,7 +315,7 @@ class NNBF_GPU(WavefunctionModel_GPU):
                 self._state_key == self._walker_key(fxs):
             return self._state
         B = fxs.shape[0]
-        st = {"z1": torch.empty(B, self.hidden, dtype=torch.float32, device=fxs.device),
+        st = {"z1": torch.empty(B, self.hidden, dtype=self.z_dtype, device=fxs.device),
               "sign": torch.empty(B, dtype=torch.float64, device=fxs.device),
               "logabs": torch.empty(B, dtype=torch.float64, device=fxs.device),
               "n_accept": (self._state["n_accept"] if self._state is not None and
@@ -331,7 +425,7 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
     """
 
     def __init__(self, Lx, Ly, D, theta=None, generator=None, dtype=torch.float64,
-                 max_bond=-1, chunk=65536, _nn=None, act="relu", eta=0.0):
+                 max_bond=-1, chunk=65536, _nn=None, act="relu", eta=0.0, _mode=0):
         if dtype != torch.float64:
             raise ValueError("the fPEPS kernels are FP64")
         if not (2 <= Lx <= 4 and 1 <= Ly <= 8 and 1 <= D <= 4):
@@ -357,6 +451,7 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
         self.chunk = chunk
         self.act_name = act
         self.eta = float(eta)
+        self.mode = int(_mode)
         self._struct_obj = None
         self._ws = {}
 
@@ -372,7 +467,7 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
                     p.data = p.data.contiguous()
             self._struct_obj = kernels.peps_struct(
                 self.Lx, self.Ly, self.D, self.dense_map, self.params[0].data,
-                self._nn_dict(), 0 if self.act_name == "relu" else 1, self.eta)
+                self._nn_dict(), 0 if self.act_name == "relu" else 1, self.eta, self.mode)
             self._state_key = None
             return True
         return False
@@ -468,7 +563,8 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
         N = self.Lx * self.Ly
         P = self.params[0].numel()
         H = int(getattr(self, "hidden", 0) or 0)
-        width = P + 2 * N + 4 * H if H > 0 else P
+        mult = H > 0 and self.mode == 1
+        width = (P + 2 * N + 4 * H + (P if mult else 0)) if H > 0 else P
         F = torch.empty(B, width, dtype=torch.float64, device=fxs.device) if out is None else out
         sign = torch.empty(B, dtype=torch.float64, device=fxs.device)
         logabs = torch.empty(B, dtype=torch.float64, device=fxs.device)
@@ -483,6 +579,12 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
         n_orb = 2 * N
         factors = [(0, P), (P, n_orb), (P + n_orb, H), (P + n_orb + H, H),
                    (P + n_orb + 2 * H, H), (P + n_orb + 3 * H, H)]
+        # the network's upstream: g itself (additive) or g theta, a seventh
+        # factor after h2 (multiplicative)
+        up = 0
+        if mult:
+            factors.append((P + n_orb + 4 * H, P))
+            up = 6
         o_w1 = P
         o_b1 = o_w1 + H * n_orb
         o_w2 = o_b1 + H
@@ -490,24 +592,31 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
         o_wo = o_b2 + H
         o_bo = o_wo + P * H
         blocks = [(0, -1, 1.0, 0, -1), (2, 1, 1.0, o_w1, o_b1), (4, 3, 1.0, o_w2, o_b2),
-                  (0, 5, float(self.nn_eta), o_wo, o_bo)]
+                  (up, 5, float(self.nn_eta), o_wo, o_bo)]
         return kernels.FactoredO(F, factors, blocks, self.n_params), (sign, logabs)
 
 
 class NNfTNS_Model_GPU(fPEPS_Model_GPU):
-    """NN-fTNS (GPU/models/NNfTNS.py:280-955): the fPEPS above with the
-    additive backflow theta(x) = theta + nn_eta * MLP(n(x)) on every tensor
-    entry (_apply_backflow, NNfTNS.py:390-437).  MLP: n = [up | down]
-    occupations (2N) -> hidden_dim -> hidden_dim -> n_tn_params, ReLU or
-    tanh, FP64.  Parameter order [theta, W1, b1, W2, b2, Wo, bo].
-    init: Xavier-uniform hidden layers, output layer N(0, init_scale^2),
-    zero biases (the model starts near the plain fPEPS)."""
+    """NN-fTNS (GPU/models/NNfTNS.py:280-955): the fPEPS above with a
+    configuration-dependent correction of every tensor entry by an MLP.
+    correction="additive": theta(x) = theta + nn_eta * MLP(n(x)), the
+    reference's _apply_backflow (NNfTNS.py:390-437); "multiplicative":
+    theta(x) = theta * (1 + nn_eta * MLP(n(x))), the multiplicative site-tensor
+    correction of the NN-fTNS benchmark configuration (no reference form).
+    MLP: n = [up | down] occupations (2N) -> hidden_dim -> hidden_dim ->
+    n_tn_params, ReLU or tanh, FP64.  Parameter order [theta, W1, b1, W2, b2,
+    Wo, bo].  init: Xavier-uniform hidden layers, output layer
+    N(0, init_scale^2), zero biases (the model starts near the plain fPEPS)."""
+
+    CORRECTIONS = {"additive": 0, "multiplicative": 1}
 
     def __init__(self, Lx, Ly, D, hidden_dim=128, nn_eta=1.0, activation="relu",
                  init_scale=1e-2, theta=None, generator=None, dtype=torch.float64, max_bond=-1,
-                 chunk=65536):
+                 chunk=65536, correction="additive"):
         if activation not in ("relu", "tanh"):
             raise ValueError("activation must be relu or tanh")
+        if correction not in self.CORRECTIONS:
+            raise ValueError(f"correction must be one of {list(self.CORRECTIONS)}")
         _, sizes = kernels.peps_layout(Lx, Ly, D)
         n = int(sum(sizes))
         N = Lx * Ly
@@ -531,9 +640,11 @@ class NNfTNS_Model_GPU(fPEPS_Model_GPU):
                torch.zeros(hidden_dim, dtype=torch.float64), Wo,
                torch.zeros(n, dtype=torch.float64)]
         super().__init__(Lx, Ly, D, theta=th, generator=g, dtype=dtype, max_bond=max_bond,
-                         chunk=chunk, _nn=nnp, act=activation, eta=nn_eta)
+                         chunk=chunk, _nn=nnp, act=activation, eta=nn_eta,
+                         _mode=self.CORRECTIONS[correction])
         self.hidden = hidden_dim
         self.nn_eta = float(nn_eta)
+        self.correction = correction
 
     def _nn_dict(self):
         names = ["W1", "b1", "W2", "b2", "Wo", "bo"]
