@@ -142,6 +142,37 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
+def library_factor_times(O, shift, reps=2):
+    """rocSOLVER (through torch.linalg) on the same SR matrix T = Oc Oc^T / Ns
+    + shift I that the library factorises (Oc: O with its column means
+    removed), timed after the measured loop for comparison with the
+    library's blocked Cholesky: potrf (torch.linalg.cholesky) and the LU
+    solve the reference calls (torch.linalg.solve, GPU/optimizer.py:546-563)."""
+    import torch
+    Ns = O.shape[0]
+    Oc = O - O.mean(0)
+    T = Oc @ Oc.T
+    del Oc
+    T /= Ns
+    T.diagonal().add_(shift)
+    e = torch.ones(Ns, dtype=torch.float64, device=O.device)
+    out = {}
+    for name, fn in (("torch_linalg_cholesky_ms", lambda: torch.linalg.cholesky(T)),
+                     ("torch_linalg_solve_ms", lambda: torch.linalg.solve(T, e))):
+        fn()
+        torch.cuda.synchronize()
+        s, t = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            fn()
+        t.record()
+        torch.cuda.synchronize()
+        out[name] = s.elapsed_time(t) / reps
+    log("rocSOLVER factorisation of the same T timed: " +
+        ", ".join(f"{k} {v:.2f}" for k, v in out.items()))
+    return out
+
+
 # ------------------------------------------------------------- CPU baseline
 def _cpu_worker(args):
     """Reference algorithm on one core: batched full-amplitude sweep, E_loc by
@@ -749,6 +780,8 @@ def main():
         flop_roof(2.0 * nb ** 3 / 3, dl_ms, PEAK_FP64,
                   "one launch: 128x128 FP64 Cholesky + triangular inverse in LDS "
                   "(latency bound)"), launches_per_step=n_diag, per_step_ms=dl_ms * n_diag)
+    if w["kind"] == "slater" and gram == "fp64":
+        kern["sr.factor"]["library_same_T"] = library_factor_times(run.O, w["shift"])
     # the roofline line is the kernel (or single library call) with the
     # largest time per SR iteration
     single = {"sr.gram": 1, "sr.chol_diag": n_diag}
