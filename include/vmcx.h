@@ -502,6 +502,42 @@ int vmcx_peps_local_energy(const vmcx_peps_model *m, int64_t B, int n_edges,
                            const int32_t *edges, double t, double U, const int64_t *configs,
                            const double *sign, const double *logabs, void *workspace,
                            int64_t ws_bytes, int64_t chunk, double *eloc, void *stream);
+/*
+ * Environment reuse (plain fPEPS, hidden = 0): each walker keeps its column
+ * environments L_y (columns < y contracted) and R_y (columns >= y), and a
+ * configuration that differs from the walker only in columns c0 .. c1 is
+ * contracted from L_c0 and R_{c1+1} through those columns alone.  The local
+ * signs of the sites left of c0 change with the hop (S_s carries
+ * (-1)^{q(d) P_right(s)}); on L_c0 that change is a sign times a parity on
+ * the bond indices of the affected rows, applied in the kernel.  Results
+ * equal vmcx_peps_sweep / vmcx_peps_local_energy up to rounding (same
+ * proposals, draws and tests).  NN-fTNS models are refused (theta(x) changes
+ * with every configuration).
+ */
+/* Workspace bytes of the reuse entry points (sweeps: chunk >= B, n_edges 0). */
+int64_t vmcx_peps_reuse_workspace_bytes(const vmcx_peps_model *m, int64_t B, int64_t chunk,
+                                        int n_edges);
+/* Metropolis sweeps as vmcx_peps_sweep, proposals from the cached
+ * environments; stale environments (columns changed by accepted moves) are
+ * rebuilt incrementally when an edge needs them.
+ * Replaces: MetropolisExchangeSpinfulSamplerReuse_GPU.step (GPU/sampler.py:
+ * 391-700) with fPEPS_Model_reuse_GPU.forward_reuse_log (GPU/models/
+ * pureTNS.py:1366-1455) and the environment caches / updates (:1456-2228). */
+int vmcx_peps_sweep_reuse(const vmcx_peps_model *m, int64_t B, int n_edges,
+                          const int32_t *edges_host, double hopping_rate, int n_sweeps,
+                          uint64_t seed, int64_t sweep_offset, const double *u_choice,
+                          const uint8_t *bits, const double *u_acc, int64_t *configs,
+                          double *sign, double *logabs, int64_t *n_accept, void *workspace,
+                          int64_t ws_bytes, void *stream);
+/* E_loc as vmcx_peps_local_energy: the walkers' environments once, each
+ * connected configuration through the one or two columns its hop touches.
+ * Replaces: evaluate_energy_reuse (GPU/vmc_utils.py:584-1320). */
+int vmcx_peps_local_energy_reuse(const vmcx_peps_model *m, int64_t B, int n_edges,
+                                 const int32_t *edges, double t, double U,
+                                 const int64_t *configs, const double *sign,
+                                 const double *logabs, void *workspace, int64_t ws_bytes,
+                                 int64_t chunk, double *eloc, void *stream);
+
 /* O[B][ld] = d ln|psi| / d theta in the layout above (o_dtype 0 f64, 1 f32),
  * with the environments of the exact contraction (and the MLP backward).
  * Replaces: compute_grads_gpu for fPEPS / NN-fTNS (GPU/vmc_utils.py:1389-1634). */

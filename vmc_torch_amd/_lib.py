@@ -92,6 +92,11 @@ _SIGNATURES = {
                              _P, _P, _I64, _P]),
     "vmcx_peps_local_energy": (_I, [_P, _I64, _I, _P, _D, _D, _P, _P, _P, _P, _I64, _I64,
                                     _P, _P]),
+    "vmcx_peps_reuse_workspace_bytes": (_I64, [_P, _I64, _I64, _I]),
+    "vmcx_peps_sweep_reuse": (_I, [_P, _I64, _I, _P, _D, _I, _U64, _I64, _P, _P, _P, _P, _P,
+                                   _P, _P, _P, _I64, _P]),
+    "vmcx_peps_local_energy_reuse": (_I, [_P, _I64, _I, _P, _D, _D, _P, _P, _P, _P, _I64,
+                                          _I64, _P, _P]),
     "vmcx_peps_logderiv": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _I, _I64, _P, _P, _P]),
     "vmcx_peps_logderiv_factors": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _I64, _P, _P, _P]),
 }

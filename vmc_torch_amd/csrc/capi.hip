@@ -60,7 +60,7 @@ static inline int status_of(int rc) {
 
 extern "C" {
 
-int vmcx_abi_version(void) { return 15; }
+int vmcx_abi_version(void) { return 16; }
 
 const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
 
@@ -416,6 +416,33 @@ int64_t vmcx_peps_dense_size(const vmcx_peps_model *m) { return vmcx::peps_dense
 int64_t vmcx_peps_workspace_bytes(const vmcx_peps_model *m, int64_t B, int64_t chunk,
                                   int n_edges) {
   return vmcx::peps_workspace_bytes(m, B, chunk, n_edges);
+}
+
+int64_t vmcx_peps_reuse_workspace_bytes(const vmcx_peps_model *m, int64_t B, int64_t chunk,
+                                        int n_edges) {
+  return vmcx::peps_reuse_workspace_bytes(m, B, chunk, n_edges);
+}
+
+int vmcx_peps_sweep_reuse(const vmcx_peps_model *m, int64_t B, int n_edges,
+                          const int32_t *edges_host, double hopping_rate, int n_sweeps,
+                          uint64_t seed, int64_t sweep_offset, const double *u_choice,
+                          const uint8_t *bits, const double *u_acc, int64_t *configs,
+                          double *sign, double *logabs, int64_t *n_accept, void *workspace,
+                          int64_t ws_bytes, void *stream) {
+  return status3(vmcx::peps_sweep_reuse(m, B, n_edges, edges_host, hopping_rate, n_sweeps, seed,
+                                        sweep_offset, u_choice, bits, u_acc, configs, sign,
+                                        logabs, n_accept, workspace, ws_bytes,
+                                        (hipStream_t)stream));
+}
+
+int vmcx_peps_local_energy_reuse(const vmcx_peps_model *m, int64_t B, int n_edges,
+                                 const int32_t *edges, double t, double U,
+                                 const int64_t *configs, const double *sign,
+                                 const double *logabs, void *workspace, int64_t ws_bytes,
+                                 int64_t chunk, double *eloc, void *stream) {
+  return status3(vmcx::peps_local_energy_reuse(m, B, n_edges, edges, t, U, configs, sign, logabs,
+                                               workspace, ws_bytes, chunk, eloc,
+                                               (hipStream_t)stream));
 }
 
 int vmcx_peps_forward(const vmcx_peps_model *m, int64_t B, const int64_t *configs,

@@ -288,6 +288,242 @@ __global__ void __launch_bounds__(PT)
   }
 }
 
+// ------------------------------------------------------------ environment reuse
+// Per-walker cache of the column environments of the exact contraction:
+// L_y = columns 0 .. y-1 absorbed from the left, R_y = columns y .. Ly-1
+// absorbed from the right (y = 0 .. Ly; both indexed [a][b] at the cut left
+// of column y, stride PB), each renormalised with its log scale.  A
+// configuration that differs from its walker only in columns c0 .. c1 has
+//   psi' = gsign' * twist(L_c0) . C'_c0 ... C'_c1 . R_{c1+1},
+// where the twist accounts for the local signs of the sites left of c0:
+// S_s carries (-1)^{q(d) P_right(s)}, and P_right(s) of a site in row x,
+// column < c0 changes parity exactly when the number of singly occupied
+// sites of row x in columns >= c0 does (flip_x).  Flipping P_right on all of
+// row x left of the cut multiplies those slices by (-1)^{q(d)}; each slice
+// is Z2-even, so (-1)^{q(d)} = (-1)^{q(p) + q(l) + q(u) + q(r)} on it, and
+// along the row the horizontal bonds cancel in pairs: the twist of row x
+// equals sigma_x (-1)^{q(h_x)} times the twist of row x-1 (its d legs are
+// row x's u legs), with sigma_x = (-1)^{singly occupied sites of row x left
+// of the cut} and h_x the horizontal bond of row x crossing the cut.  Hence
+//   twist = prod_{x: F_x odd} sigma_x (-1)^{q(h_x)},  F_x = sum_{x' >= x} flip_x',
+// a sign and a parity on the bond indices of L_c0.  R_{c1+1} needs none
+// (P_right looks right only).  L_y depends on the configuration right of
+// its cut through those parities, so each L_y keeps the row parities it was
+// built with, and flip_x is taken against them: an L_y stays usable after
+// accepted moves right of its cut (the sweep relies on this), and only a
+// change of the columns left of the cut makes it stale.
+__host__ __device__ inline int64_t env_stride(int Ly) {
+  const int64_t n = 2 * (int64_t)(Ly + 1) * PB * PB + 3 * (Ly + 1);
+  return (n + 31) & ~(int64_t)31;
+}
+__device__ inline double *env_L(double *e, int y) { return e + (int64_t)y * PB * PB; }
+__device__ inline double *env_R(double *e, int Ly, int y) {
+  return e + (int64_t)(Ly + 1 + y) * PB * PB;
+}
+// log scales: L at [y], R at [Ly + 1 + y]; then, at [2 (Ly + 1) + y], the
+// row mask of L_y: bit x = parity of the singly occupied sites of row x in
+// columns >= y when L_y was built (the P_right parities its slices carry)
+__device__ inline double *env_sc(double *e, int Ly) { return e + 2 * (int64_t)(Ly + 1) * PB * PB; }
+
+// bond dimension of a row's horizontal bond at the cut left of column y
+__device__ __forceinline__ int cut_dim(const PepsGeo &g, int y) {
+  return (y == 0 || y == g.Ly) ? 1 : g.D;
+}
+
+// singly occupied sites of row x in columns [ya, yb)
+__device__ __forceinline__ int singles(const PepsGeo &g, const int64_t *c, int x, int ya,
+                                      int yb) {
+  int n = 0;
+  for (int y = ya; y < yb; ++y) {
+    const int64_t v = c[x * g.Ly + y];
+    n += (v == 1 || v == 2);
+  }
+  return n;
+}
+
+// buf <- L_c0 of the environment block e, twisted to the configuration c
+// (whose columns left of c0 are those L_c0 was built from): the rows whose
+// P_right parities differ from the ones L_c0 carries (F_x odd) contribute
+// sigma_x (-1)^{q(h_x)}; the sign is folded into the entries.  Block-wide;
+// tw: a shared int.
+__device__ void load_left(const PepsGeo &g, const double *e, int c0, const int64_t *c,
+                          double *buf, int *tw) {
+  const double *sc = e + 2 * (int64_t)(g.Ly + 1) * PB * PB;
+  if (threadIdx.x == 0) {
+    const int built = (int)sc[2 * (g.Ly + 1) + c0];
+    int F = 0, rowsm = 0, sg = 0;
+    if (c0 > 0) {
+      for (int x = g.Lx - 1; x >= 0; --x) {
+        F ^= (singles(g, c, x, c0, g.Ly) ^ (built >> x)) & 1;
+        if (F) {
+          rowsm |= 1 << x;
+          sg ^= singles(g, c, x, 0, c0) & 1;
+        }
+      }
+    }
+    *tw = rowsm | (sg << 8);
+  }
+  __syncthreads();
+  const int twr = *tw & 0xFF;
+  const bool neg = (*tw >> 8) & 1;
+  const int dc = cut_dim(g, c0);
+  const int br = g.Lx - g.tr;
+  for (int idx = threadIdx.x; idx < PB * PB; idx += PT) {
+    const int a = idx / PB, b = idx - a * PB;
+    double v = e[(int64_t)c0 * PB * PB + idx];
+    int q = neg ? 1 : 0;
+    if (twr) {
+      // the row components of the cut's bond indices: a = h_0 (dc) + h_1
+      // (two top rows) or h_0; b likewise for the bottom rows
+      const int ht0 = g.tr == 2 ? a / dc : a, ht1 = g.tr == 2 ? a % dc : 0;
+      const int hb0 = br == 2 ? b / dc : b, hb1 = br == 2 ? b % dc : 0;
+      if (twr & 1) q += qpar(ht0, dc);
+      if (g.tr == 2 && (twr & 2)) q += qpar(ht1, dc);
+      if (twr & (1 << g.tr)) q += qpar(hb0, dc);
+      if (br == 2 && (twr & (1 << (g.tr + 1)))) q += qpar(hb1, dc);
+    }
+    buf[idx] = (q & 1) ? -v : v;
+  }
+  __syncthreads();
+}
+
+// L_0 = R_Ly = [[1]], scales 0
+__global__ void peps_env_init_kernel(int64_t B, int Ly, double *env) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= B) return;
+  double *e = env + w * env_stride(Ly);
+  env_L(e, 0)[0] = 1.0;
+  env_R(e, Ly, Ly)[0] = 1.0;
+  env_sc(e, Ly)[0] = 0.0;
+  env_sc(e, Ly)[Ly + 1 + Ly] = 0.0;
+  env_sc(e, Ly)[2 * (Ly + 1)] = 0.0;   // L_0 holds no site: no twist applies
+}
+
+// dir 0: L_{y+1} from L_y for y = y0 .. y1-1; dir 1: R_y from R_{y+1} for
+// y = y1-1 down to y0 (one workgroup per walker, the walker's columns)
+__global__ void __launch_bounds__(PT)
+    peps_env_kernel(PepsArgs A, int64_t B, const int64_t *__restrict__ cfg, int y0, int y1,
+                    int dir, double *env) {
+  const int64_t w = blockIdx.x;
+  if (w >= B) return;
+  __shared__ CfgSh cs;
+  __shared__ ColSh col;
+  __shared__ double buf[2][PB * PB];
+  __shared__ double red[PT / 64];
+  const PepsGeo &g = A.g;
+  load_cfg(g, cfg + w * g.N, cs);
+  double *e = env + w * env_stride(g.Ly);
+  double *sc = env_sc(e, g.Ly);
+  __shared__ int tw;
+  int cur = 0;
+  if (dir == 0) {
+    // L_y0 may predate accepted moves right of its cut: bring its signs to
+    // the walker's present configuration before absorbing further columns
+    load_left(g, e, y0, cfg + w * g.N, buf[0], &tw);
+    double lsc = sc[y0];
+    for (int y = y0; y < y1; ++y) {
+      build_column(g, cs, y, A.dmap, A.theta, col);
+      absorb_left(col, buf[cur], buf[cur ^ 1]);
+      cur ^= 1;
+      lsc += renorm(buf[cur], col.da2, col.db2, red);
+      for (int idx = threadIdx.x; idx < PB * PB; idx += PT) env_L(e, y + 1)[idx] = buf[cur][idx];
+      if (threadIdx.x == 0) {
+        sc[y + 1] = lsc;
+        int mask = 0;
+        for (int x = 0; x < g.Lx; ++x) mask |= (singles(g, cfg + w * g.N, x, y + 1, g.Ly) & 1) << x;
+        sc[2 * (g.Ly + 1) + y + 1] = (double)mask;
+      }
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < PB * PB; idx += PT) buf[0][idx] = env_R(e, g.Ly, y1)[idx];
+    double lsc = sc[g.Ly + 1 + y1];
+    __syncthreads();
+    for (int y = y1 - 1; y >= y0; --y) {
+      build_column(g, cs, y, A.dmap, A.theta, col);
+      absorb_right(col, buf[cur], buf[cur ^ 1]);
+      cur ^= 1;
+      lsc += renorm(buf[cur], col.da, col.db, red);
+      for (int idx = threadIdx.x; idx < PB * PB; idx += PT)
+        env_R(e, g.Ly, y)[idx] = buf[cur][idx];
+      if (threadIdx.x == 0) sc[g.Ly + 1 + y] = lsc;
+    }
+  }
+}
+
+// Amplitudes of `rows` configurations, row k equal to walker parent[k]
+// (parent NULL: walker k) outside columns c0 .. c1: from the walker's cached
+// L_c0 (twisted to row k's configuration) and R_{c1+1}, absorbing only
+// columns c0 .. c1.  cmove
+// (may be NULL): per row the hop from_orbital | to_orbital << 8 (orbital =
+// site, or N + site for down), c0 / c1 its columns; otherwise c0 / c1 as
+// given for every row.  flag (may be NULL): rows with !(flag & 1) skipped.
+__global__ void __launch_bounds__(PT)
+    peps_amp_env_kernel(PepsArgs A, int64_t rows, const int64_t *__restrict__ cfg_rows,
+                        const int32_t *__restrict__ parent, const uint32_t *__restrict__ cmove,
+                        int c0u, int c1u,
+                        const int32_t *__restrict__ flag, const double *__restrict__ env,
+                        double *sign, double *logabs) {
+  const int64_t k = blockIdx.x;
+  if (k >= rows) return;
+  if (flag && !(flag[k] & 1)) return;
+  __shared__ CfgSh cs;
+  __shared__ ColSh col;
+  __shared__ double buf[2][PB * PB];
+  __shared__ double red[PT / 64];
+  __shared__ int tw;
+  const PepsGeo &g = A.g;
+  const int64_t w = parent ? (int64_t)parent[k] : k;
+  const int64_t *cn = cfg_rows + k * g.N;
+  load_cfg(g, cn, cs);
+  int c0 = c0u, c1 = c1u;
+  if (cmove) {
+    const uint32_t mv = cmove[k];
+    int sa = mv & 0xFF, sb = (mv >> 8) & 0xFF;
+    if (sa >= g.N) sa -= g.N;
+    if (sb >= g.N) sb -= g.N;
+    const int ya = sa % g.Ly, yb = sb % g.Ly;
+    c0 = ya < yb ? ya : yb;
+    c1 = ya < yb ? yb : ya;
+  }
+  const double *e = env + w * env_stride(g.Ly);
+  const double *sc = e + 2 * (int64_t)(g.Ly + 1) * PB * PB;
+  load_left(g, e, c0, cn, buf[0], &tw);
+  double lsc = sc[c0];
+  int cur = 0;
+  for (int y = c0; y <= c1; ++y) {
+    build_column(g, cs, y, A.dmap, A.theta, col);
+    absorb_left(col, buf[cur], buf[cur ^ 1]);
+    cur ^= 1;
+    lsc += renorm(buf[cur], col.da2, col.db2, red);
+  }
+  // psi' / scales = sum_{a,b} L'[a][b] R_{c1+1}[a][b]
+  const double *Rr = e + (int64_t)(g.Ly + 1 + c1 + 1) * PB * PB;
+  const int dr_ = cut_dim(g, c1 + 1);
+  const int br = g.Lx - g.tr;
+  const int ea = g.tr == 2 ? dr_ * dr_ : dr_, eb = br == 2 ? dr_ * dr_ : dr_;
+  double part = 0.0;
+  for (int idx = threadIdx.x; idx < PB * PB; idx += PT) {
+    const int a = idx / PB, b = idx - a * PB;
+    if (a < ea && b < eb) part += buf[cur][idx] * Rr[idx];
+  }
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_down(part, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double dot = 0.0;
+    for (int q = 0; q < PT / 64; ++q) dot += red[q];
+    const double v = dot * cs.gsign;
+    const double lo = log(1e-45);
+    if (v == 0.0 || !std::isfinite(v)) {
+      sign[k] = 0.0;
+      logabs[k] = lo;
+    } else {
+      sign[k] = v > 0.0 ? 1.0 : -1.0;
+      logabs[k] = fmax(log(fabs(v)) + lsc + sc[g.Ly + 1 + c1 + 1], lo);
+    }
+  }
+}
+
 // ------------------------------------------------------------ log-derivatives
 // G[row][k] = d ln|psi| / d theta_k for the tensor entries (zero for entries
 // of local states other than x_s); sign / logabs of each configuration.
@@ -685,13 +921,18 @@ struct PepsWs {
   int64_t *pcfg;
   int64_t *counts, *offsets, *total, *etas;
   double *coeffs, *csign, *clog;
+  // environment reuse: the walkers' column environments, and the walker and
+  // hop of each connected configuration
+  double *env;
+  int32_t *cparent;
+  uint32_t *cmove;
   size_t bytes;
 };
 
 size_t alb(size_t x) { return (x + 255) & ~(size_t)255; }
 
 PepsWs peps_layout(const vmcx_peps_model &m, int64_t B, int64_t chunk, int n_edges,
-                   char *base) {
+                   char *base, bool reuse = false) {
   PepsWs L{};
   size_t off = 0;
   auto take = [&](size_t b) {
@@ -726,6 +967,11 @@ PepsWs peps_layout(const vmcx_peps_model &m, int64_t B, int64_t chunk, int n_edg
   L.coeffs = (double *)take((size_t)T * 8);
   L.csign = (double *)take((size_t)T * 8);
   L.clog = (double *)take((size_t)T * 8);
+  if (reuse) {
+    L.env = (double *)take((size_t)B * env_stride(m.Ly) * 8);
+    L.cparent = (int32_t *)take((size_t)T * 4);
+    L.cmove = (uint32_t *)take((size_t)T * 4);
+  }
   L.bytes = off;
   return L;
 }
@@ -839,6 +1085,36 @@ int64_t peps_workspace_bytes(const vmcx_peps_model *m, int64_t B, int64_t chunk,
   return (int64_t)peps_layout(*m, B, chunk, n_edges, nullptr).bytes;
 }
 
+int64_t peps_reuse_workspace_bytes(const vmcx_peps_model *m, int64_t B, int64_t chunk,
+                                   int n_edges) {
+  if (check_peps(m) || B < 0 || chunk < 1 || n_edges < 0) return -1;
+  return (int64_t)peps_layout(*m, B, chunk, n_edges, nullptr, true).bytes;
+}
+
+namespace {
+
+int check_reuse(const vmcx_peps_model *m) {
+  if (check_peps(m)) return 1;
+  VMCX_CHECK_ARG(m->hidden == 0,
+                 "environment reuse needs site tensors that do not depend on the "
+                 "configuration (plain fPEPS, hidden = 0)");
+  return 0;
+}
+
+// column environments of all walkers: L_{y0+1} .. L_{y1} (dir 0) or R_{y1-1}
+// .. R_{y0} (dir 1)
+int env_sweep(const vmcx_peps_model &m, const PepsWs &L, int64_t B, const int64_t *cfg, int y0,
+              int y1, int dir, hipStream_t st) {
+  if (y1 <= y0 || B == 0) return 0;
+  const PepsArgs A = args_of(m, m.theta, 0);
+  hipLaunchKernelGGL(peps_env_kernel, dim3((unsigned)B), dim3(PT), 0, st, A, B, cfg, y0, y1,
+                     dir, L.env);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+}  // namespace
+
 int peps_forward(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void *ws,
                  int64_t ws_bytes, int64_t chunk, double *sign, double *logabs,
                  hipStream_t st) {
@@ -930,6 +1206,126 @@ int peps_local_energy(const vmcx_peps_model *m, int64_t B, int n_edges, const in
     const int rc = amp_rows(*m, L, rows, L.etas + r0 * N, nullptr, L.csign + r0, L.clog + r0,
                             st);
     if (rc) return rc;
+  }
+  hipLaunchKernelGGL(eloc_segment_kernel, dim3(gw), dim3(256), 0, st, B, N, U, cfg, L.offsets,
+                     L.counts, L.coeffs, L.csign, L.clog, sign, logabs, eloc);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+// Metropolis sweeps with environment reuse: the same proposals, draws and
+// Metropolis tests as peps_sweep, each proposal evaluated from the walker's
+// cached environments (L_c0, R_{c1+1} for an edge spanning columns c0 .. c1)
+// instead of a full contraction.  The cache is kept lazily: after an edge,
+// L_y is stale for y > c0 and R_y for y <= c1 (those columns may have
+// changed), and a stale environment is rebuilt incrementally from the
+// nearest valid one when an edge needs it.  With the reference's edge order
+// (row edges row by row, then column edges column by column) that costs
+// about one column per edge on top of the proposal's own columns.
+int peps_sweep_reuse(const vmcx_peps_model *m, int64_t B, int n_edges, const int32_t *edges_host,
+                     double hopping_rate, int n_sweeps, uint64_t seed, int64_t sweep_offset,
+                     const double *u_choice, const uint8_t *bits, const double *u_acc,
+                     int64_t *cfg, double *sign, double *logabs, int64_t *n_accept, void *ws,
+                     int64_t ws_bytes, hipStream_t st) {
+  if (check_reuse(m)) return 1;
+  VMCX_CHECK_ARG(B >= 0 && n_edges >= 0 && n_sweeps >= 0, "negative size");
+  VMCX_CHECK_ARG((u_choice == nullptr) == (u_acc == nullptr) &&
+                     (u_choice == nullptr) == (bits == nullptr),
+                 "injected draws need u_choice, bits and u_acc together");
+  if (B == 0 || n_sweeps == 0 || n_edges == 0) return 0;
+  VMCX_CHECK_ARG(cfg && sign && logabs && ws && edges_host, "null pointer");
+  const PepsWs L = peps_layout(*m, B, B, 0, (char *)ws, true);
+  VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes,
+                 "workspace too small (vmcx_peps_reuse_workspace_bytes with chunk >= B)");
+  const int N = m->Lx * m->Ly, Ly = m->Ly;
+  for (int e = 0; e < n_edges; ++e) {
+    const int i = edges_host[2 * e], j = edges_host[2 * e + 1];
+    VMCX_CHECK_ARG(i >= 0 && i < N && j >= 0 && j < N && i != j, "edge out of range");
+  }
+  const unsigned gw = (unsigned)((B + 255) / 256);
+  const unsigned gc = (unsigned)((B * N + 255) / 256);
+  hipLaunchKernelGGL(peps_env_init_kernel, dim3(gw), dim3(256), 0, st, B, Ly, L.env);
+  VMCX_LAUNCHED();
+  const PepsArgs A = args_of(*m, m->theta, 0);
+  int Lv = 0, Rv = Ly;   // L_y valid for y <= Lv, R_y for y >= Rv
+  for (int sw = 0; sw < n_sweeps; ++sw) {
+    for (int e = 0; e < n_edges; ++e) {
+      const int i = edges_host[2 * e], j = edges_host[2 * e + 1];
+      const int ci = i % Ly, cj = j % Ly;
+      const int c0 = ci < cj ? ci : cj, c1 = ci < cj ? cj : ci;
+      if (Lv < c0) {
+        const int rc = env_sweep(*m, L, B, cfg, Lv, c0, 0, st);
+        if (rc) return rc;
+        Lv = c0;
+      }
+      if (Rv > c1 + 1) {
+        const int rc = env_sweep(*m, L, B, cfg, c1 + 1, Rv, 1, st);
+        if (rc) return rc;
+        Rv = c1 + 1;
+      }
+      const size_t di = ((size_t)sw * n_edges + e) * B;
+      hipLaunchKernelGGL(nnbf_propose_kernel, dim3(gw), dim3(256), 0, st, B, N, i, j, e,
+                         sweep_offset + sw, seed, 1.0 - hopping_rate,
+                         u_choice ? u_choice + di : nullptr, bits ? bits + di : nullptr,
+                         u_acc ? u_acc + di : nullptr, cfg, L.prop, L.moves, L.uacc);
+      hipLaunchKernelGGL(peps_prop_cfg_kernel, dim3(gc), dim3(256), 0, st, B, N, i, j, cfg,
+                         L.prop, L.pcfg);
+      hipLaunchKernelGGL(peps_amp_env_kernel, dim3((unsigned)B), dim3(PT), 0, st, A, B, L.pcfg,
+                         nullptr, nullptr, c0, c1, L.prop, L.env, L.psign, L.plog);
+      hipLaunchKernelGGL(peps_accept_kernel, dim3(gw), dim3(256), 0, st, B, N, i, j, cfg,
+                         L.prop, L.uacc, L.psign, L.plog, sign, logabs, n_accept);
+      VMCX_LAUNCHED();
+      if (Lv > c0) Lv = c0;
+      if (Rv < c1 + 1) Rv = c1 + 1;
+    }
+  }
+  return 0;
+}
+
+// E_loc with environment reuse: the walkers' environments once (2 Ly
+// columns each), then every connected configuration (one hop) from its
+// walker's environments, absorbing only the one or two columns the hop
+// touches.
+int peps_local_energy_reuse(const vmcx_peps_model *m, int64_t B, int n_edges,
+                            const int32_t *edges, double t, double U, const int64_t *cfg,
+                            const double *sign, const double *logabs, void *ws,
+                            int64_t ws_bytes, int64_t chunk, double *eloc, hipStream_t st) {
+  if (check_reuse(m)) return 1;
+  VMCX_CHECK_ARG(B >= 0 && n_edges >= 0 && chunk >= 1, "bad sizes");
+  if (B == 0) return 0;
+  VMCX_CHECK_ARG(cfg && sign && logabs && ws && eloc && (n_edges == 0 || edges),
+                 "null pointer");
+  const PepsWs L = peps_layout(*m, B, chunk, n_edges, (char *)ws, true);
+  VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes,
+                 "workspace too small (vmcx_peps_reuse_workspace_bytes)");
+  const int N = m->Lx * m->Ly, Ly = m->Ly;
+  const unsigned gw = (unsigned)((B + 255) / 256);
+  hipLaunchKernelGGL(conn_count_walker_kernel, dim3(gw), dim3(256), 0, st, B, N, n_edges,
+                     edges, cfg, L.counts);
+  hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, st, B, L.counts, L.offsets,
+                     L.total);
+  VMCX_LAUNCHED();
+  {
+    const int rc = launch_conn_fill(B, N, n_edges, edges, t, cfg, L.offsets, L.etas, L.coeffs,
+                                    st, L.cparent, L.cmove);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(peps_env_init_kernel, dim3(gw), dim3(256), 0, st, B, Ly, L.env);
+  VMCX_LAUNCHED();
+  int rc = env_sweep(*m, L, B, cfg, 0, Ly, 0, st);
+  if (rc) return rc;
+  rc = env_sweep(*m, L, B, cfg, 0, Ly, 1, st);
+  if (rc) return rc;
+  int64_t T = 0;
+  VMCX_HIP(hipMemcpyAsync(&T, L.total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  VMCX_HIP(hipStreamSynchronize(st));
+  const PepsArgs A = args_of(*m, m->theta, 0);
+  for (int64_t r0 = 0; r0 < T; r0 += chunk) {
+    const int64_t rows = (T - r0 < chunk) ? T - r0 : chunk;
+    hipLaunchKernelGGL(peps_amp_env_kernel, dim3((unsigned)rows), dim3(PT), 0, st, A, rows,
+                       L.etas + r0 * N, L.cparent + r0, L.cmove + r0, 0, 0, nullptr, L.env,
+                       L.csign + r0, L.clog + r0);
+    VMCX_LAUNCHED();
   }
   hipLaunchKernelGGL(eloc_segment_kernel, dim3(gw), dim3(256), 0, st, B, N, U, cfg, L.offsets,
                      L.counts, L.coeffs, L.csign, L.clog, sign, logabs, eloc);

@@ -115,14 +115,18 @@ __global__ void nnbf_propose_kernel(int64_t B, int N, int i, int j, int e, int64
                                     double *u_acc);
 __global__ void conn_count_walker_kernel(int64_t B, int N, int E, const int32_t *edges,
                                          const int64_t *cfg, int64_t *counts);
+// cparent / cmove (may be NULL): for connected configuration k, its walker
+// and the hop as from_orbital | to_orbital << 8 (orbital = site for up, N +
+// site for down)
 int launch_conn_fill(int64_t B, int N, int E, const int32_t *edges, double t,
                      const int64_t *cfg, const int64_t *offsets, int64_t *etas, double *coeffs,
-                     hipStream_t st);
+                     hipStream_t st, int32_t *cparent = nullptr, uint32_t *cmove = nullptr);
 __global__ void exclusive_scan_kernel(int64_t n, const int64_t *in, int64_t *out,
                                       int64_t *total);
 __global__ void conn_fill_walker_kernel(int64_t B, int N, int E, const int32_t *edges,
                                         double t, const int64_t *cfg, const int64_t *offsets,
-                                        int64_t *etas, double *coeffs);
+                                        int64_t *etas, double *coeffs, int32_t *cparent,
+                                        uint32_t *cmove);
 __global__ void eloc_segment_kernel(int64_t B, int N, double U, const int64_t *cfg,
                                     const int64_t *offsets, const int64_t *counts,
                                     const double *coeffs, const double *csign,
@@ -131,6 +135,17 @@ __global__ void eloc_segment_kernel(int64_t B, int N, double U, const int64_t *c
 
 int64_t peps_dense_size(const vmcx_peps_model *m);
 int64_t peps_workspace_bytes(const vmcx_peps_model *m, int64_t B, int64_t chunk, int n_edges);
+int64_t peps_reuse_workspace_bytes(const vmcx_peps_model *m, int64_t B, int64_t chunk,
+                                   int n_edges);
+int peps_sweep_reuse(const vmcx_peps_model *m, int64_t B, int n_edges, const int32_t *edges_host,
+                     double hopping_rate, int n_sweeps, uint64_t seed, int64_t sweep_offset,
+                     const double *u_choice, const uint8_t *bits, const double *u_acc,
+                     int64_t *cfg, double *sign, double *logabs, int64_t *n_accept, void *ws,
+                     int64_t ws_bytes, hipStream_t st);
+int peps_local_energy_reuse(const vmcx_peps_model *m, int64_t B, int n_edges,
+                            const int32_t *edges, double t, double U, const int64_t *cfg,
+                            const double *sign, const double *logabs, void *ws,
+                            int64_t ws_bytes, int64_t chunk, double *eloc, hipStream_t st);
 int peps_forward(const vmcx_peps_model *m, int64_t B, const int64_t *cfg, void *ws,
                  int64_t ws_bytes, int64_t chunk, double *sign, double *logabs, hipStream_t st);
 int peps_sweep(const vmcx_peps_model *m, int64_t B, int n_edges, const int32_t *edges_host,

@@ -504,6 +504,40 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// W1T[o][u] = W1[u][o]: the first-layer columns contiguous, so that the
+// incremental kernels below read a column with coalesced loads
+template <typename TW>
+__global__ void __launch_bounds__(256)
+    nnbf_w1t_kernel(int H, int n_orb, const TW *__restrict__ W1, TW *W1T) {
+  const int64_t n = (int64_t)H * n_orb;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int o = (int)(idx / H), u = (int)(idx - (int64_t)o * H);
+    W1T[idx] = W1[(int64_t)u * n_orb + o];
+  }
+}
+
+// First layer of connected configurations by one hop each (local energy):
+// h1[k] = act(z1w[parent_k] + W1[:, to_k] - W1[:, from_k]), the parent
+// walker's pre-activations updated by the moved electron's column pair.
+template <typename TW, typename TI>
+__global__ void __launch_bounds__(256)
+    nnbf_h1_conn_kernel(int64_t rows, int H, const TW *__restrict__ W1T,
+                        const TW *__restrict__ z1w, const int32_t *__restrict__ cparent,
+                        const uint32_t *__restrict__ cmove, int act, TI *h1) {
+  const int64_t n = rows * H;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = idx / H;
+    const int u = (int)(idx - k * H);
+    const uint32_t mv = cmove[k];
+    const int from = mv & 0xFF, to = (mv >> 8) & 0xFF;
+    const TW z = z1w[(int64_t)cparent[k] * H + u] + W1T[(int64_t)to * H + u] -
+                 W1T[(int64_t)from * H + u];
+    h1[idx] = to_t<TI>(act_f(z, act));
+  }
+}
+
 // determinant of the proposed configuration, Metropolis test, commit
 template <int NFP, typename TW>
 __global__ void __launch_bounds__(256)
@@ -619,7 +653,7 @@ __global__ void __launch_bounds__(256)
     conn_fill_walker_kernel(int64_t B, int N, int E, const int32_t *__restrict__ edges,
                             double t, const int64_t *__restrict__ cfg,
                             const int64_t *__restrict__ offsets, int64_t *etas,
-                            double *coeffs) {
+                            double *coeffs, int32_t *cparent, uint32_t *cmove) {
   const int lane = threadIdx.x & 63;
   const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (w >= B) return;   // uniform per wave
@@ -655,6 +689,13 @@ __global__ void __launch_bounds__(256)
       rem &= rem - 1;
       const int ii = __shfl(i, src, 64), jj = __shfl(j, src, 64), bb = __shfl(bit, src, 64);
       if (lane < N) etas[k * N + lane] = (lane == ii || lane == jj) ? (cs ^ bb) : cs;
+      if (cparent != nullptr && lane == 0) {
+        const uint64_t mm = bb == 2 ? up : dn;
+        const int from = ((mm >> ii) & 1ull) ? ii : jj, to = ii + jj - from;
+        const int ob = bb == 2 ? 0 : N;
+        cparent[k] = (int32_t)w;
+        cmove[k] = (uint32_t)(ob + from) | ((uint32_t)(ob + to) << 8);
+      }
       ++k;
     }
     pos = k;
@@ -663,11 +704,12 @@ __global__ void __launch_bounds__(256)
 
 int launch_conn_fill(int64_t B, int N, int E, const int32_t *edges, double t,
                      const int64_t *cfg, const int64_t *offsets, int64_t *etas, double *coeffs,
-                     hipStream_t st) {
+                     hipStream_t st, int32_t *cparent, uint32_t *cmove) {
   VMCX_CHECK_ARG(N >= 1 && N <= 64, "connected-configuration fill needs 1 <= N <= 64");
+  VMCX_CHECK_ARG((cparent == nullptr) == (cmove == nullptr), "cparent and cmove go together");
   if (B == 0) return 0;
   hipLaunchKernelGGL(conn_fill_walker_kernel, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, st,
-                     B, N, E, edges, t, cfg, offsets, etas, coeffs);
+                     B, N, E, edges, t, cfg, offsets, etas, coeffs, cparent, cmove);
   VMCX_LAUNCHED();
   return 0;
 }
@@ -844,6 +886,10 @@ struct NnbfWs {
   // local-energy buffers
   int64_t *counts, *offsets, *total, *etas;
   double *coeffs, *csign, *clog;
+  int32_t *cparent;   // walker of each connected configuration
+  uint32_t *cmove;    // its hop: from_orbital | to_orbital << 8
+  void *z1w;          // first-layer pre-activations of the walkers [B][H]
+  void *w1t;          // W1 transposed [2N][H]
   size_t bytes;
 };
 
@@ -880,6 +926,10 @@ static NnbfWs nnbf_layout(const vmcx_nnbf_model &m, int64_t B, int64_t chunk, in
   L.coeffs = (double *)take((size_t)Tmax * 8);
   L.csign = (double *)take((size_t)Tmax * 8);
   L.clog = (double *)take((size_t)Tmax * 8);
+  L.cparent = (int32_t *)take((size_t)Tmax * 4);
+  L.cmove = (uint32_t *)take((size_t)Tmax * 4);
+  L.z1w = take((size_t)B * H * tw);
+  L.w1t = take((size_t)2 * m.n_sites * H * tw);
   L.bytes = off;
   return L;
 }
@@ -976,8 +1026,8 @@ static void launch_det(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
 }
 
 template <typename TW, typename TI>
-static int layer1(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows, const int64_t *cfg,
-                  hipStream_t st) {
+static int layer1_to(const vmcx_nnbf_model &m, int64_t rows, const int64_t *cfg, TW *z1, TI *h1,
+                     hipStream_t st) {
   const dim3 g1((unsigned)((rows + L1_RB - 1) / L1_RB),
                 (unsigned)((m.hidden + L1_UT - 1) / L1_UT));
   const size_t lds = (((size_t)L1_UT * (2 * m.n_sites + 1) * sizeof(TW) + 15) & ~(size_t)15) +
@@ -985,9 +1035,15 @@ static int layer1(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows, const
   VMCX_CHECK_ARG(lds <= max_lds_per_block(), "layer-1 LDS exceeds the device limit");
   const NnW<TW> W = weights_of<TW>(m);
   hipLaunchKernelGGL((nnbf_layer1_kernel<TW, TI>), g1, dim3(256), lds, st, rows, m.n_sites,
-                     m.hidden, m.act, cfg, W.W1, W.b1, (TW *)L.z1, (TI *)L.h1);
+                     m.hidden, m.act, cfg, W.W1, W.b1, z1, h1);
   VMCX_LAUNCHED();
   return 0;
+}
+
+template <typename TW, typename TI>
+static int layer1(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows, const int64_t *cfg,
+                  hipStream_t st) {
+  return layer1_to<TW, TI>(m, rows, cfg, (TW *)L.z1, (TI *)L.h1, st);
 }
 
 // the BF16 forward on the hand-written matrix-core layers: usable when the
@@ -1030,8 +1086,9 @@ static void launch_detA(const vmcx_nnbf_model &m, int64_t rows, const int64_t *c
 // layer 1, layer 2 and the occupied-row output layer on the matrix cores,
 // then the determinants: no rocBLAS call and no backflow matrix in HBM
 static int forward_rows_mfma(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
-                             const int64_t *cfg, double *sign, double *logabs, hipStream_t st) {
-  int rc = layer1<float, __hip_bfloat16>(m, L, rows, cfg, st);
+                             const int64_t *cfg, double *sign, double *logabs, bool h1_ready,
+                             hipStream_t st) {
+  int rc = h1_ready ? 0 : layer1<float, __hip_bfloat16>(m, L, rows, cfg, st);
   if (rc) return rc;
   const int H = m.hidden, nD = 2 * m.n_sites * m.n_f;
   rc = launch_ng<NG_ACT>(rows, H, H, m.W2_bf16, L.h1, m.b2, m.act, (float *)L.z2, L.h2, nullptr,
@@ -1055,9 +1112,9 @@ static int forward_rows_mfma(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t 
 template <typename TW, typename TI>
 static int forward_rows_t(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
                           const int64_t *cfg, double *sign, double *logabs, double *ainv,
-                          hipStream_t st) {
+                          bool h1_ready, hipStream_t st) {
   if (rows == 0) return 0;
-  int rc = layer1<TW, TI>(m, L, rows, cfg, st);
+  int rc = h1_ready ? 0 : layer1<TW, TI>(m, L, rows, cfg, st);
   if (rc) return rc;
   rc = dense_layers<TW, TI>(m, L, rows, st);
   if (rc) return rc;
@@ -1071,16 +1128,51 @@ static int forward_rows_t(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t row
   return 0;
 }
 
+// h1_ready: the first layer's output is already in L.h1 (local energy:
+// formed incrementally from the walkers'), so the pass starts at layer 2
 static int forward_rows(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
                         const int64_t *cfg, double *sign, double *logabs, double *ainv,
-                        hipStream_t st) {
+                        hipStream_t st, bool h1_ready = false) {
   if (m.gemm == 2)
-    return forward_rows_t<double, double>(m, L, rows, cfg, sign, logabs, ainv, st);
+    return forward_rows_t<double, double>(m, L, rows, cfg, sign, logabs, ainv, h1_ready, st);
   if (m.gemm == 1 && ainv == nullptr && rows > 0 && mfma_forward_ok(m) && m.path == 0)
-    return forward_rows_mfma(m, L, rows, cfg, sign, logabs, st);
+    return forward_rows_mfma(m, L, rows, cfg, sign, logabs, h1_ready, st);
   if (m.gemm == 1)
-    return forward_rows_t<float, __hip_bfloat16>(m, L, rows, cfg, sign, logabs, ainv, st);
-  return forward_rows_t<float, float>(m, L, rows, cfg, sign, logabs, ainv, st);
+    return forward_rows_t<float, __hip_bfloat16>(m, L, rows, cfg, sign, logabs, ainv, h1_ready,
+                                                 st);
+  return forward_rows_t<float, float>(m, L, rows, cfg, sign, logabs, ainv, h1_ready, st);
+}
+
+// walkers' first-layer pre-activations into L.z1w (in chunks through L.h1)
+// and W1 transposed into L.w1t: the inputs of the incremental first layer
+template <typename TW, typename TI>
+static int conn_layer1_setup(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t B, int64_t chunk,
+                             const int64_t *cfg, hipStream_t st) {
+  const int H = m.hidden, n_orb = 2 * m.n_sites;
+  for (int64_t r0 = 0; r0 < B; r0 += chunk) {
+    const int64_t rows = (B - r0 < chunk) ? B - r0 : chunk;
+    const int rc = layer1_to<TW, TI>(m, rows, cfg + r0 * m.n_sites, (TW *)L.z1w + r0 * H,
+                                     (TI *)L.h1, st);
+    if (rc) return rc;
+  }
+  const NnW<TW> W = weights_of<TW>(m);
+  const int64_t n = (int64_t)H * n_orb;
+  hipLaunchKernelGGL((nnbf_w1t_kernel<TW>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     H, n_orb, W.W1, (TW *)L.w1t);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+template <typename TW, typename TI>
+static int conn_layer1(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t r0, int64_t rows,
+                       hipStream_t st) {
+  const int64_t n = rows * m.hidden;
+  const unsigned g = (unsigned)((n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384);
+  hipLaunchKernelGGL((nnbf_h1_conn_kernel<TW, TI>), dim3(g), dim3(256), 0, st, rows, m.hidden,
+                     (const TW *)L.w1t, (const TW *)L.z1w, L.cparent + r0, L.cmove + r0, m.act,
+                     (TI *)L.h1);
+  VMCX_LAUNCHED();
+  return 0;
 }
 
 int nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
@@ -1207,7 +1299,16 @@ int nnbf_local_energy(const vmcx_nnbf_model *m, int64_t B, int n_edges, const in
   VMCX_LAUNCHED();
   {
     const int rc = launch_conn_fill(B, N, n_edges, edges, t, cfg, L.offsets, L.etas, L.coeffs,
-                                    st);
+                                    st, L.cparent, L.cmove);
+    if (rc) return rc;
+  }
+  // the first layer of every connected configuration from its walker's
+  // (one hop = one column pair of W1), the rest of the pass per chunk
+  {
+    int rc;
+    if (m->gemm == 2) rc = conn_layer1_setup<double, double>(*m, L, B, chunk, cfg, st);
+    else if (m->gemm == 1) rc = conn_layer1_setup<float, __hip_bfloat16>(*m, L, B, chunk, cfg, st);
+    else rc = conn_layer1_setup<float, float>(*m, L, B, chunk, cfg, st);
     if (rc) return rc;
   }
   int64_t T = 0;
@@ -1215,8 +1316,13 @@ int nnbf_local_energy(const vmcx_nnbf_model *m, int64_t B, int n_edges, const in
   VMCX_HIP(hipStreamSynchronize(st));
   for (int64_t r0 = 0; r0 < T; r0 += chunk) {
     const int64_t rows = (T - r0 < chunk) ? T - r0 : chunk;
-    const int rc = forward_rows(*m, L, rows, L.etas + r0 * N, L.csign + r0, L.clog + r0,
-                                nullptr, st);
+    int rc;
+    if (m->gemm == 2) rc = conn_layer1<double, double>(*m, L, r0, rows, st);
+    else if (m->gemm == 1) rc = conn_layer1<float, __hip_bfloat16>(*m, L, r0, rows, st);
+    else rc = conn_layer1<float, float>(*m, L, r0, rows, st);
+    if (rc) return rc;
+    rc = forward_rows(*m, L, rows, L.etas + r0 * N, L.csign + r0, L.clog + r0, nullptr, st,
+                      true);
     if (rc) return rc;
   }
   hipLaunchKernelGGL(eloc_segment_kernel, dim3(gw), dim3(256), 0, st, B, N, U, cfg, L.offsets,

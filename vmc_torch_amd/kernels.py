@@ -558,10 +558,12 @@ def peps_struct(Lx, Ly, D, dense_map, theta, nn=None, act=0, eta=0.0, mode=0):
     return s
 
 
-def peps_workspace(struct, B, chunk, n_edges, device, holder):
+def peps_workspace(struct, B, chunk, n_edges, device, holder, reuse=False):
+    """Device workspace of the fPEPS entry points (reuse: the environment-
+    reuse ones), grown on demand and kept in `holder`."""
     import ctypes
-    need = int(lib().vmcx_peps_workspace_bytes(ctypes.byref(struct), int(B), int(chunk),
-                                                int(n_edges)))
+    fn = lib().vmcx_peps_reuse_workspace_bytes if reuse else lib().vmcx_peps_workspace_bytes
+    need = int(fn(ctypes.byref(struct), int(B), int(chunk), int(n_edges)))
     if need < 0:
         raise ValueError("bad fPEPS workspace sizes / model")
     buf = holder.get("ws")

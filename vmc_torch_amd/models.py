@@ -70,6 +70,9 @@ class WavefunctionModel_GPU(nn.Module):
         (each would synchronise the host), so the model's kernels can be
         captured into a HIP graph.  Derived device data is brought up to date
         on entry."""
+        if getattr(self, "_frozen", False):     # nested: the outer block owns it
+            yield self
+            return
         self._frozen = False
         self._prepare_device_data()
         self._frozen = True
@@ -595,6 +598,75 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
                   (up, 5, float(self.nn_eta), o_wo, o_bo)]
         return kernels.FactoredO(F, factors, blocks, self.n_params), (sign, logabs)
 
+
+
+class fPEPS_Model_reuse_GPU(fPEPS_Model_GPU):
+    """fPEPS with environment reuse (GPU/models/pureTNS.py:163-2375,
+    fPEPS_Model_reuse_GPU): the same amplitude as fPEPS_Model_GPU, with
+    sweeps and local energies that evaluate each proposal or connected
+    configuration from its walker's cached column environments, contracting
+    only the one or two columns the move touches (vmcx_peps_sweep_reuse,
+    vmcx_peps_local_energy_reuse).  The reference caches boundary MPSs of a
+    chi-truncated contraction per row and column direction; here the
+    environments are the exact ones of the column-by-column contraction
+    (Lx <= 4, D <= 4), rebuilt incrementally as accepted moves change
+    columns.  `mcmc_sweep` and `local_energy` use the reuse path, so the
+    plain sampler and evaluate_energy get it too; the `_full` methods keep
+    the full contraction for comparison."""
+
+    def __init__(self, Lx, Ly, D, theta=None, generator=None, dtype=torch.float64,
+                 max_bond=-1, chunk=65536, **kwargs):
+        super().__init__(Lx, Ly, D, theta=theta, generator=generator, dtype=dtype,
+                         max_bond=max_bond, chunk=chunk)
+        self.radius = 0
+
+    def _ws_reuse(self, B, chunk, n_edges, device):
+        return kernels.peps_workspace(self._struct_obj, B, chunk, n_edges, device, self._ws,
+                                      reuse=True)
+
+    def mcmc_sweep(self, fxs, edges, n_sweeps, seed, sweep_offset, hopping_rate=0.25,
+                   refresh_every=64, draws=None):
+        st = self.walker_state(fxs)
+        B = fxs.shape[0]
+        edges_host = edges.detach().to("cpu", torch.int32).contiguous()
+        uc = bits = ua = None
+        if draws is not None:
+            uc, bits, ua = draws
+        ws = self._ws_reuse(B, B, 0, fxs.device)
+        kernels.check(kernels.lib().vmcx_peps_sweep_reuse(
+            ctypes.byref(self._struct_obj), B, edges_host.shape[0],
+            ctypes.c_void_p(edges_host.data_ptr()), float(hopping_rate), int(n_sweeps),
+            int(seed) & (2 ** 64 - 1), int(sweep_offset), kernels.ptr(uc), kernels.ptr(bits),
+            kernels.ptr(ua), kernels.ptr(fxs), kernels.ptr(st["sign"]),
+            kernels.ptr(st["logabs"]), kernels.ptr(st["n_accept"]), kernels.ptr(ws), ws.numel(),
+            kernels.stream_ptr(fxs.device)))
+        return st["sign"], st["logabs"], st["n_accept"]
+
+    def local_energy(self, fxs, H):
+        st = self.walker_state(fxs)
+        B = fxs.shape[0]
+        edges = H.graph.edge_tensor(fxs.device, "graph")
+        chunk = self.chunk
+        ws = self._ws_reuse(B, chunk, edges.shape[0], fxs.device)
+        eloc = torch.empty(B, dtype=torch.float64, device=fxs.device)
+        kernels.check(kernels.lib().vmcx_peps_local_energy_reuse(
+            ctypes.byref(self._struct_obj), B, edges.shape[0], kernels.ptr(edges), float(H.t),
+            float(H.U), kernels.ptr(fxs), kernels.ptr(st["sign"]), kernels.ptr(st["logabs"]),
+            kernels.ptr(ws), ws.numel(), chunk, kernels.ptr(eloc),
+            kernels.stream_ptr(fxs.device)))
+        return eloc
+
+    def mcmc_sweep_full(self, *args, **kwargs):
+        return fPEPS_Model_GPU.mcmc_sweep(self, *args, **kwargs)
+
+    def local_energy_full(self, fxs, H):
+        return fPEPS_Model_GPU.local_energy(self, fxs, H)
+
+    # the reference's names for the reuse path
+    def forward_reuse_log(self, x, **kwargs):
+        """(sign, ln|psi|) of x; the cached environments are internal to the
+        sweep and the local energy, so this is the full contraction."""
+        return self.forward_log(x)
 
 class NNfTNS_Model_GPU(fPEPS_Model_GPU):
     """NN-fTNS (GPU/models/NNfTNS.py:280-955): the fPEPS above with a

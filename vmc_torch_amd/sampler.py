@@ -7,6 +7,7 @@ proposal per edge per walker, acceptance u < |psi'|^2/|psi|^2.  The whole
 sweep runs in one HIP launch per sweep with Sherman-Morrison updates; the
 draws come from Philox4x32-10 keyed by (seed, sweep, edge, walker).
 """
+import contextlib
 from typing import Tuple
 
 import torch
@@ -73,6 +74,26 @@ class MetropolisExchangeSpinfulSamplerGPU(SamplerGPU):
         return fxs
 
 
+class MetropolisExchangeSpinfulSamplerReuse_GPU(MetropolisExchangeSpinfulSamplerGPU):
+    """Exchange / hopping sampler with environment reuse
+    (GPU/sampler.py:391-700): the same proposals, Philox draws and
+    Metropolis tests as MetropolisExchangeSpinfulSamplerGPU, in the same edge
+    order (row edges row by row, then column edges column by column); every
+    proposal is evaluated from the walker's cached column environments
+    (vmcx_peps_sweep_reuse), so the chain equals the plain sampler's up to
+    rounding.  The model must be a models.fPEPS_Model_reuse_GPU."""
+
+    def _check(self, model):
+        from .models import fPEPS_Model_reuse_GPU
+        if not isinstance(model, fPEPS_Model_reuse_GPU):
+            raise TypeError("MetropolisExchangeSpinfulSamplerReuse_GPU needs an "
+                            f"fPEPS_Model_reuse_GPU, got {type(model).__name__}")
+
+    def sweeps(self, fxs, model, graph, n_sweeps=1, draws=None):
+        self._check(model)
+        return super().sweeps(fxs, model, graph, n_sweeps, draws=draws)
+
+
 class LearnedProposalSamplerGPU(SamplerGPU):
     """Metropolis-Hastings with a learned local proposal (no counterpart in the
     reference; its external-proposal sampler MetropolisMPSSamplerSpinful,
@@ -135,25 +156,30 @@ class LearnedProposalSamplerGPU(SamplerGPU):
         of (u_choice, u_acc) device arrays per step (tests)."""
         if self.params.device != fxs.device:
             self.params = self.params.to(fxs.device)
-        sign, logabs = model.forward_log(fxs)
-        sign, logabs = sign.clone(), logabs.clone()
-        n_acc = torch.zeros(fxs.shape[0], dtype=torch.int64, device=fxs.device)
-        # logits at the current walkers: evaluated once, then carried over
-        # from the proposed configurations of the accepted moves
-        lx = self.k.lp_logits(self.params, self.C, self.Lx, self.Ly, fxs,
-                              precision=self.precision) if n_steps else None
-        for s in range(n_steps):
-            uc = ua = None
-            if draws is not None:
-                uc, ua = draws[s]
-            move, lqf, prop, lqr, lp = self.k.lp_propose_given(
-                self.params, self.C, self.Lx, self.Ly, fxs, lx, self.seed, self.step_counter, uc,
-                precision=self.precision)
-            ps, pl = model.forward_log(prop)
-            acc = self.k.lp_accept(fxs, sign, logabs, move, prop, ps, pl, lqf, lqr, self.seed,
-                                   self.step_counter, ua, n_acc)
-            self.k.lp_commit_logits(acc, lp, lx)
-            self.step_counter += 1
+        # the model's parameters are constant over the proposals: one check
+        # on entry instead of a value comparison (a host synchronisation) per
+        # proposal
+        frozen = model.frozen() if hasattr(model, "frozen") else contextlib.nullcontext()
+        with frozen:
+            sign, logabs = model.forward_log(fxs)
+            sign, logabs = sign.clone(), logabs.clone()
+            n_acc = torch.zeros(fxs.shape[0], dtype=torch.int64, device=fxs.device)
+            # logits at the current walkers: evaluated once, then carried over
+            # from the proposed configurations of the accepted moves
+            lx = self.k.lp_logits(self.params, self.C, self.Lx, self.Ly, fxs,
+                                  precision=self.precision) if n_steps else None
+            for s in range(n_steps):
+                uc = ua = None
+                if draws is not None:
+                    uc, ua = draws[s]
+                move, lqf, prop, lqr, lp = self.k.lp_propose_given(
+                    self.params, self.C, self.Lx, self.Ly, fxs, lx, self.seed, self.step_counter,
+                    uc, precision=self.precision)
+                ps, pl = model.forward_log(prop)
+                acc = self.k.lp_accept(fxs, sign, logabs, move, prop, ps, pl, lqf, lqr,
+                                       self.seed, self.step_counter, ua, n_acc)
+                self.k.lp_commit_logits(acc, lp, lx)
+                self.step_counter += 1
         if hasattr(model, "invalidate"):
             model.invalidate()      # the model's cached walker state predates the moves
         self.last_acceptance = (n_acc, n_steps)

@@ -46,6 +46,23 @@ def evaluate_energy(fxs, model, H, current_amps=None, verbose=False, use_log_amp
     return kernels.energy_stats(eloc)[0], eloc
 
 
+def evaluate_energy_reuse(fxs, model, H, current_amps=None, use_log_amp=False, verbose=False,
+                          return_bMPS=False, **kwargs):
+    """GPU/vmc_utils.py:584-1320 (evaluate_energy_reuse): (E_mean, E_loc[B])
+    with every connected configuration evaluated from its walker's cached
+    column environments (vmcx_peps_local_energy_reuse).  model: a
+    models.fPEPS_Model_reuse_GPU.  The environments live inside the call, so
+    return_bMPS adds None as the third element (the reference's cache
+    object)."""
+    from .models import fPEPS_Model_reuse_GPU
+    if not isinstance(model, fPEPS_Model_reuse_GPU):
+        raise TypeError(f"evaluate_energy_reuse needs an fPEPS_Model_reuse_GPU, got "
+                        f"{type(model).__name__}")
+    eloc = model.local_energy(fxs, H)
+    mean = kernels.energy_stats(eloc)[0]
+    return (mean, eloc, None) if return_bMPS else (mean, eloc)
+
+
 def check_grads_amps(O, sign, logabs, configs=None):
     """GPU/vmc_utils.py:1330-1386 (_check_grads_amps): raise ValueError when
     the amplitudes or the per-sample gradients hold NaN or Inf (one device
