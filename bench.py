@@ -116,6 +116,9 @@ def parse():
                         "one sample per sweep)")
     p.add_argument("--mlp", default=None, choices=["fp64", "fp32", "bf16"],
                    help="NNBF network precision (default: the workload's)")
+    p.add_argument("--det", default=None, choices=["fp64", "fp32"],
+                   help="NNBF determinant arithmetic of the amplitude-only passes (fp32: "
+                        "BF16 MLP on the matrix-core layers only; default fp64)")
     p.add_argument("--slater-precision", default="fp64", choices=["fp64", "fp32"],
                    help="Slater walker state and sweep arithmetic (fp32: the FP32 form, "
                         "FP64 refresh and SR)")
@@ -128,6 +131,9 @@ def parse():
     p.add_argument("--sampler", default="plain", choices=["plain", "learned"],
                    help="NNBF: plain sweeps, or the learned proposal (|E|/4 proposals per "
                         "sample, the net trained before and every 10 SR steps)")
+    p.add_argument("--peps-reuse", action="store_true",
+                   help="fPEPS: sweeps and local energies from the walkers' cached column "
+                        "environments (fPEPS_Model_reuse_GPU)")
     p.add_argument("--tau-sweeps", type=int, default=64)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -419,10 +425,11 @@ class NnbfRun:
         N, nf, B = w["Lx"] * w["Ly"], w["nu"] + w["nd"], w["B"]
         self.N, self.nf, self.B = N, nf, B
         self.mlp = a.mlp or w["mlp"]
+        self.det = getattr(a, "det", None) or "fp64"
         self.model = NNBF_GPU(N, nf, hidden_dim=w["hidden"], activation="relu",
                               bf_scale=w["bf_scale"], gemm=self.mlp, init="xavier", m_std=0.1,
                               generator=torch.Generator().manual_seed(0),
-                              chunk=a.chunk).to(dev)
+                              chunk=a.chunk, det=self.det).to(dev)
         self.Np = self.model.n_params
         self.e_sweep = H.graph.edge_tensor(dev, "sweep")
         self.e_graph = H.graph.edge_tensor(dev, "graph")
@@ -539,7 +546,7 @@ class PepsRun(NnbfRun):
     def __init__(self, w, a, dev, H, fxs):
         import torch
 
-        from vmc_torch_amd import NNfTNS_Model_GPU, fPEPS_Model_GPU, kernels
+        from vmc_torch_amd import NNfTNS_Model_GPU, fPEPS_Model_GPU, fPEPS_Model_reuse_GPU, kernels
         self.k, self.a, self.w, self.H, self.fxs = kernels, a, w, H, fxs
         N, nf, B = w["Lx"] * w["Ly"], w["nu"] + w["nd"], w["B"]
         self.N, self.nf, self.B = N, nf, B
@@ -548,7 +555,8 @@ class PepsRun(NnbfRun):
             self.model = NNfTNS_Model_GPU(w["Lx"], w["Ly"], w["D"], hidden_dim=w["hidden"],
                                           nn_eta=w["eta"], generator=g).to(dev)
         else:
-            self.model = fPEPS_Model_GPU(w["Lx"], w["Ly"], w["D"], generator=g).to(dev)
+            cls = fPEPS_Model_reuse_GPU if getattr(a, "peps_reuse", False) else fPEPS_Model_GPU
+            self.model = cls(w["Lx"], w["Ly"], w["D"], generator=g).to(dev)
         self.Np = self.model.n_params
         self.e_sweep = H.graph.edge_tensor(dev, "sweep")
         self.e_graph = H.graph.edge_tensor(dev, "graph")
@@ -829,7 +837,9 @@ def main():
                    "gram": gram, "diag_shift": w["shift"], "lr": w["lr"],
                    "sampler": a.sampler, "sweeps_per_sample": spp,
                    "proposals_per_sample": getattr(run, "proposals_per_step", None),
-                   "parallelism": "walkers on 1 GPU"},
+                   "parallelism": "walkers on 1 GPU",
+                   **({"environment_reuse": bool(a.peps_reuse)}
+                      if w["kind"] == "peps" and w.get("hidden", 0) == 0 else {})},
         "sr_iters_per_s": sr_iters,
         "walker_sweeps_per_s": samples_per_s * spp,
         "sweep_only_walker_sweeps_per_s": B * spp / (kt["sweep"] * 1e-3),
@@ -967,9 +977,11 @@ def nnbf_target(a, dev, steps):
     form (FP64 network and determinant, plain sweep sampler) with the exact
     FP64 Gram formed from the factors of O, the same with O formed (FP64 Gram
     of the 4096 x 2.2 M matrix, for reference), and the mixed-precision form
-    (BF16 network with FP32 accumulation, FP64 determinant, learned proposal,
-    the same factored FP64 SR).  One burn-in (plain sampler, BF16 network)
-    serves all forms; the parameters start equal (one seed)."""
+    (BF16 network with FP32 accumulation on the matrix cores, FP32
+    determinants of the amplitudes from entries formed in FP64, learned
+    proposal, the same factored FP64 SR; O_k from FP64 inverses).  One
+    burn-in (plain sampler, BF16 network) serves all forms; the parameters
+    start equal (one seed)."""
     import argparse as ap
     import copy
 
@@ -997,18 +1009,20 @@ def nnbf_target(a, dev, steps):
             setattr(x, k, v)
         return x
     seed = 0x5EED
-    burn_run = NnbfRun(w, args(mlp="bf16", gram="factored", o_dtype=None, sampler="plain"), dev,
+    burn_run = NnbfRun(w, args(mlp="bf16", gram="factored", o_dtype=None, sampler="plain",
+                               det="fp64"), dev,
                        H, fxs0)
     burn_run.burn(w["burn"], seed, 0)
     torch.cuda.synchronize()
     log(f"target: 8x8 NNBF burn-in ({w['burn']} sweeps) done")
     forms = {}
     specs = [("fp64_factored", dict(mlp="fp64", gram="factored", o_dtype=None,
-                                    sampler="plain"), steps, None),
-             ("fp64_O_formed", dict(mlp="fp64", gram="fp64", o_dtype="fp64", sampler="plain"),
+                                    sampler="plain", det="fp64"), steps, None),
+             ("fp64_O_formed", dict(mlp="fp64", gram="fp64", o_dtype="fp64", sampler="plain",
+                                    det="fp64"),
               max(2, steps // 3), "fp64_factored"),
              ("bf16_learned", dict(mlp="bf16", gram="factored", o_dtype=None,
-                                   sampler="learned"), steps, None)]
+                                   sampler="learned", det="fp32"), steps, None)]
     bf_run = None
     for name, kw, n_steps, same_chain in specs:
         ai = args(**kw)
@@ -1019,6 +1033,7 @@ def nnbf_target(a, dev, steps):
         tau = forms[same_chain]["tau_int"] if same_chain else None
         res, _ = measure_form(run, w, ai, gram, n_steps, 1, seed, w["burn"], a.tau_sweeps, tau)
         res["mlp"], res["gram"], res["sampler"] = ai.mlp, gram, ai.sampler
+        res["det"] = getattr(ai, "det", None) or "fp64"
         if same_chain:
             res["tau_int_note"] = f"the chain of {same_chain} (same sampler and arithmetic)"
         forms[name] = res

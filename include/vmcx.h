@@ -371,6 +371,10 @@ typedef struct vmcx_nnbf_model {
   const float *M, *W1, *b1, *W2, *b2, *Wout, *bout;   /* gemm 0 / 1 */
   const void *W2_bf16, *Wout_bf16;                     /* gemm 1 */
   const double *M64, *W1_64, *b1_64, *W2_64, *b2_64, *Wout_64, *bout_64;  /* gemm 2 */
+  int32_t det_fp32;   /* gemm 1, path 0: determinants of the amplitude-only passes
+                         (forward, local energy, proposals) eliminated in FP32 (the
+                         mixed-precision form); A^-1 (sweeps, O_k) stays FP64 */
+  int32_t reserved2;
 } vmcx_nnbf_model;
 
 /* Workspace bytes for B walkers, forward chunks of `chunk` configurations

@@ -273,3 +273,59 @@ def test_nnbf_bf16_matrix_core_layers(device, case):
     assert np.array_equal(s1.cpu().numpy(), so)
     d = np.abs(l1.cpu().numpy() - lo)
     assert d.mean() < 2e-5 and d.max() < 2e-3, (float(d.mean()), float(d.max()))
+
+
+@pytest.mark.parametrize("case", [(8, 8, 32, 32, 256), (6, 6, 16, 16, 256), (4, 4, 8, 8, 64),
+                                  (6, 6, 18, 17, 64)])
+def test_nnbf_bf16_fp32_determinant(device, case):
+    """det="fp32" (the mixed-precision amplitude: FP32 elimination of entries
+    formed in FP64) against the FP64 elimination of the same BF16 network on
+    100 random configurations: identical signs, ln|psi| within 1e-4 on
+    average and 1e-2 at most (FP32 LU: ~cond(A) x 6e-8 relative; random
+    configurations at bf_scale 0.3 include badly conditioned ones); against
+    the BF16-rounding oracle, where the BF16 rounding flips add theirs,
+    within 5e-4 on average and 5e-2 at most.  The bound that matters for the
+    mixed form, the energy over sampled walkers, is the next test's."""
+    Lx, Ly, nu, nd, hid = case
+    m64, params = _model(Lx, Ly, nu, nd, hid, "bf16", device, scale=0.3)
+    m32 = NNBF_GPU(Lx * Ly, nu + nd, hidden_dim=hid, activation="relu", bf_scale=0.3,
+                   gemm="bf16", det="fp32").to(device)
+    with torch.no_grad():
+        for a, b in zip(m32.params, m64.params):
+            a.copy_(b)
+    xd = torch.tensor(random_configs(np.random.default_rng(41), 100, Lx * Ly, nu, nd),
+                      device=device)
+    s1, l1 = m32.forward_log(xd)
+    s2, l2 = m64.forward_log(xd)
+    d = (l1 - l2).abs()
+    assert torch.equal(s1, s2)
+    assert d.mean().item() < 1e-4 and d.max().item() < 1e-2, (d.mean().item(), d.max().item())
+    so, lo, _ = onn.forward(params, xd.cpu().numpy(), 2, m64.bf_scale, bf16_gemm=True)
+    assert np.array_equal(s1.cpu().numpy(), so)
+    do = np.abs(l1.cpu().numpy() - lo)
+    assert do.mean() < 5e-4 and do.max() < 5e-2, (do.mean(), do.max())
+
+
+def test_nnbf_8x8_mixed_form_energy(device):
+    """The target's mixed form (BF16 network on the matrix-core layers, FP32
+    determinants) on 1024 walkers sampled from |psi|^2: the energy estimate per
+    site within 1e-4 of the FP64 oracle's (FP64 weights, FP64 everything)."""
+    Lx, Ly, nu, nd = 8, 8, 32, 32
+    N = 64
+    m = NNBF_GPU(N, nu + nd, hidden_dim=256, activation="relu", bf_scale=0.01, gemm="bf16",
+                 det="fp32", init="xavier", m_std=0.1,
+                 generator=torch.Generator().manual_seed(3)).to(device)
+    p64 = [p.detach().cpu().numpy().copy() for p in m.params]
+    Hd = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, nu + nd,
+                                                    n_fermions_per_spin=(nu, nd))
+    Ho = ohub.Hubbard(Lx, Ly, 1.0, 8.0, nu, nd)
+    xd = torch.tensor(random_configs(np.random.default_rng(30), 1024, N, nu, nd), device=device)
+    m.mcmc_sweep(xd, Hd.graph.edge_tensor(device, "sweep"), 30, 7, 0)
+    m.invalidate()
+    E = m.local_energy(xd, Hd).cpu().numpy()
+    E64 = onn.local_energy(Ho, p64, xd.cpu().numpy(), 2, 0.01, chunk=64)
+    d = np.abs(E - E64) / N
+    dE = abs(E.mean() - E64.mean()) / N
+    print(f"  mixed form |dE/site| vs FP64 oracle over 1024 sampled walkers: {dE:.2e} "
+          f"(per sample: mean {d.mean():.2e}, max {d.max():.2e})", flush=True)
+    assert dE < 1e-4

@@ -111,7 +111,8 @@ class NnbfModelStruct(ctypes.Structure):
                 ("M", _P), ("W1", _P), ("b1", _P), ("W2", _P), ("b2", _P),
                 ("Wout", _P), ("bout", _P), ("W2_bf16", _P), ("Wout_bf16", _P),
                 ("M64", _P), ("W1_64", _P), ("b1_64", _P), ("W2_64", _P), ("b2_64", _P),
-                ("Wout_64", _P), ("bout_64", _P)]
+                ("Wout_64", _P), ("bout_64", _P), ("det_fp32", ctypes.c_int32),
+                ("reserved2", ctypes.c_int32)]
 
 
 class SrBlockStruct(ctypes.Structure):

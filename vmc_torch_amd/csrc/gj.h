@@ -51,11 +51,25 @@ __device__ __forceinline__ void lu_half_pivot(uint32_t k, double v, int lane, in
 // True when an entry of the group's matrix (lanes gbase .. gbase + NFP - 1)
 // is NaN, infinite or beyond 1e300: the determinant is then reported as NaN
 // (a non-finite parameter must not pass for a zero amplitude).
-template <int NFP>
-__device__ __forceinline__ bool group_nonfinite(const double (&a)[NFP], int gbase) {
+// largest magnitude an LU entry may have (larger ones rank as non-finite)
+template <typename T>
+__device__ __forceinline__ T lu_big();
+template <>
+__device__ __forceinline__ double lu_big<double>() { return 1e300; }
+template <>
+__device__ __forceinline__ float lu_big<float>() { return 3.0e38f; }
+
+// lane l's value on every lane (readlane into scalar registers)
+__device__ __forceinline__ double lu_bcast(double v, int l) { return lu_readlane(v, l); }
+__device__ __forceinline__ float lu_bcast(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+template <int NFP, typename T>
+__device__ __forceinline__ bool group_nonfinite(const T (&a)[NFP], int gbase) {
   bool bad = false;
 #pragma unroll
-  for (int j = 0; j < NFP; ++j) bad |= !(fabs(a[j]) <= 1e300);
+  for (int j = 0; j < NFP; ++j) bad |= !(fabs(a[j]) <= lu_big<T>());
   const uint64_t b = __ballot(bad);
   const uint64_t m = (NFP == 64) ? ~0ull : (((1ull << NFP) - 1ull) << gbase);
   return (b & m) != 0ull;
@@ -165,21 +179,27 @@ struct LuState {
   bool singular;
 };
 
-template <int NFP, int P>
+template <int NFP, int P, typename T>
 struct LuStep {
-  static __device__ __forceinline__ void run(double (&a)[NFP], int col, int gbase,
-                                             double *prow, LuState &S) {
+  static __device__ __forceinline__ void run(T (&a)[NFP], int col, int gbase, T *prow,
+                                             LuState &S) {
     // pivot key: |a| rounded to float with its 6 low mantissa bits replaced
     // by 63 - lane (largest magnitude to 2^-17, then the lowest lane); used
     // lanes key 0.  NaN / Inf rank as 0 as in gj_invert.
     const bool done = (S.used >> col) & 1ull;
-    const double mag = fabs(a[P]);
-    const float fm = (mag <= 1e300) ? (float)mag : 0.0f;
+    const T mag = fabs(a[P]);
+    const float fm = (mag <= lu_big<T>()) ? (float)mag : 0.0f;
     uint32_t key = done ? 0u : ((__float_as_uint(fm) & ~63u) | (uint32_t)(63 - col));
     int q;
-    double piv;
+    T piv;
     if constexpr (NFP == 32) {
-      lu_half_pivot(key, a[P], gbase + col, q, piv);
+      if constexpr (sizeof(T) == 8) {
+        lu_half_pivot(key, a[P], gbase + col, q, piv);
+      } else {
+        double pd;
+        lu_half_pivot(key, (double)a[P], gbase + col, q, pd);
+        piv = (T)pd;
+      }
     } else {
 #pragma unroll
       for (int o = NFP / 2; o > 0; o >>= 1)
@@ -187,7 +207,7 @@ struct LuStep {
       q = 63 - (int)(key & 63u);
       piv = __shfl(a[P], gbase + q, 64);
     }
-    if (!(piv != 0.0) || !(fabs(piv) <= 1e300)) S.singular = true;
+    if (!(piv != T(0)) || !(fabs(piv) <= lu_big<T>())) S.singular = true;
     if (P + 1 < NFP) {
       if (col == q) {
 #pragma unroll
@@ -195,7 +215,7 @@ struct LuStep {
       }
       wave_lds_fence();
       if (!done && col != q) {
-        const double f = (piv != 0.0) ? a[P] / piv : 0.0;
+        const T f = (piv != T(0)) ? a[P] / piv : T(0);
         // blocks of 16 pivot-row entries, kept apart by scheduling barriers
         // so that at most 16 of them are live (no spills at NFP = 64)
 #pragma unroll
@@ -207,23 +227,23 @@ struct LuStep {
       }
       wave_lds_fence();
     }
-    if (piv != 0.0) {
+    if (piv != T(0)) {
       int e;
-      S.mprod *= frexp(fabs(piv), &e);
+      S.mprod *= frexp(fabs((double)piv), &e);
       S.esum += e;
       S.mprod = frexp(S.mprod, &e);
       S.esum += e;
     }
-    S.parity += (piv < 0.0) ? 1 : 0;
+    S.parity += (piv < T(0)) ? 1 : 0;
     S.parity += __popcll(S.used & ~((2ull << q) - 1ull));
     S.used |= 1ull << q;
-    LuStep<NFP, P + 1>::run(a, col, gbase, prow, S);
+    LuStep<NFP, P + 1, T>::run(a, col, gbase, prow, S);
   }
 };
 
-template <int NFP>
-struct LuStep<NFP, NFP> {
-  static __device__ __forceinline__ void run(double (&)[NFP], int, int, double *, LuState &) {}
+template <int NFP, typename T>
+struct LuStep<NFP, NFP, T> {
+  static __device__ __forceinline__ void run(T (&)[NFP], int, int, T *, LuState &) {}
 };
 
 // One 64 x 64 matrix per wave: the pivot is the same for every lane, so the
@@ -232,48 +252,49 @@ struct LuStep<NFP, NFP> {
 // scalar registers, no LDS.  Every lane applies the update (f = 1 on the
 // pivot lane, anything on finished lanes, whose registers are never read
 // again), so the FMAs need no lane mask.
-template <int P>
+template <int P, typename T>
 struct LuStep64 {
-  static __device__ __forceinline__ void run(double (&a)[64], int col, LuState &S) {
+  static __device__ __forceinline__ void run(T (&a)[64], int col, LuState &S) {
     const bool done = (S.used >> col) & 1ull;
-    const double mag = fabs(a[P]);
-    const float fm = (mag <= 1e300) ? (float)mag : 0.0f;
+    const T mag = fabs(a[P]);
+    const float fm = (mag <= lu_big<T>()) ? (float)mag : 0.0f;
     const uint32_t key = done ? 0u : ((__float_as_uint(fm) & ~63u) | (uint32_t)(63 - col));
     const int q = 63 - (int)(lu_wave_max(key) & 63u);
-    const double piv = lu_readlane(a[P], q);
-    if (!(piv != 0.0) || !(fabs(piv) <= 1e300)) S.singular = true;
+    const T piv = lu_bcast(a[P], q);
+    if (!(piv != T(0)) || !(fabs(piv) <= lu_big<T>())) S.singular = true;
     if (P + 1 < 64) {
-      const double f = (piv != 0.0) ? a[P] / piv : 0.0;
+      const T f = (piv != T(0)) ? a[P] / piv : T(0);
 #pragma unroll
-      for (int j = P + 1; j < 64; ++j) a[j] = fma(-f, lu_readlane(a[j], q), a[j]);
+      for (int j = P + 1; j < 64; ++j) a[j] = fma(-f, lu_bcast(a[j], q), a[j]);
     }
-    if (piv != 0.0) {
+    if (piv != T(0)) {
       int e;
-      S.mprod *= frexp(fabs(piv), &e);
+      S.mprod *= frexp(fabs((double)piv), &e);
       S.esum += e;
       S.mprod = frexp(S.mprod, &e);
       S.esum += e;
     }
-    S.parity += (piv < 0.0) ? 1 : 0;
+    S.parity += (piv < T(0)) ? 1 : 0;
     S.parity += __popcll(S.used & ~((2ull << q) - 1ull));
     S.used |= 1ull << q;
-    LuStep64<P + 1>::run(a, col, S);
+    LuStep64<P + 1, T>::run(a, col, S);
   }
 };
 
-template <>
-struct LuStep64<64> {
-  static __device__ __forceinline__ void run(double (&)[64], int, LuState &) {}
+template <typename T>
+struct LuStep64<64, T> {
+  static __device__ __forceinline__ void run(T (&)[64], int, LuState &) {}
 };
 
-template <int NFP>
-__device__ __forceinline__ GJResult lu_logdet(double (&a)[NFP], int col, int gbase,
-                                             double *prow) {
+// T = double, or float (the mixed-precision amplitude: FP32 elimination,
+// |det| still accumulated as an FP64 mantissa product and exponent sum)
+template <int NFP, typename T>
+__device__ __forceinline__ GJResult lu_logdet(T (&a)[NFP], int col, int gbase, T *prow) {
   LuState S{1.0, 0, 0, 0ull, false};
   if constexpr (NFP == 64)
-    LuStep64<0>::run(a, col, S);
+    LuStep64<0, T>::run(a, col, S);
   else
-    LuStep<NFP, 0>::run(a, col, gbase, prow, S);
+    LuStep<NFP, 0, T>::run(a, col, gbase, prow, S);
   return {log(S.mprod) + (double)S.esum * 0.69314718055994530942, S.parity, S.singular, -1};
 }
 

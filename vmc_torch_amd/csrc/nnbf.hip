@@ -293,8 +293,12 @@ __global__ void __launch_bounds__(NG_T)
     }
   }
   __syncthreads();
+  // blockIdx.y: a contiguous slice of the 32-row tiles of W (small batches
+  // spread over more workgroups)
   const int n_tiles = (n_rows + 31) / 32;
-  for (int tile = wave; tile < n_tiles; tile += NG_T / 64) {
+  const int tps = (n_tiles + (int)gridDim.y - 1) / (int)gridDim.y;
+  const int t_end = min(n_tiles, ((int)blockIdx.y + 1) * tps);
+  for (int tile = (int)blockIdx.y * tps + wave; tile < t_end; tile += NG_T / 64) {
     const int n0 = tile * 32;
     const int nrow = n0 + (lane & 31);
     const __bf16 *wrow = reinterpret_cast<const __bf16 *>(W) +
@@ -370,7 +374,9 @@ __global__ void __launch_bounds__(NG_T)
 // written by the NG_SCAT epilogue): lane col of the group holds column col,
 // a[j] = M[orb_j][col] + s Docc[j][col] in FP64 for orb_j the j-th occupied
 // orbital (ascending); padding as nnbf_rows.
-template <int NFP>
+// TL: the elimination's type (double; float for the FP32 determinant of the
+// mixed-precision amplitude: the entries are formed in FP64 and rounded)
+template <int NFP, typename TL>
 __global__ void __launch_bounds__(256)
     nnbf_detA_kernel(int64_t Bc, int N, int nf, const int64_t *__restrict__ cfg,
                      const float *__restrict__ A, const float *__restrict__ M, double s,
@@ -383,12 +389,12 @@ __global__ void __launch_bounds__(256)
   const int per_block = (blockDim.x / 64) * WPW;
   const int64_t w = (int64_t)blockIdx.x * per_block + wl;
   if (w >= Bc) return;
-  double *prow = reinterpret_cast<double *>(smem) + wl * NFP;
+  TL *prow = reinterpret_cast<TL *>(smem) + wl * NFP;
   uint64_t up, dn;
   group_masks<NFP>(cfg + w * N, N, col, up, dn);
   const bool count_ok = (__popcll(up) + __popcll(dn)) == nf;
   const float *Aw = A + w * (int64_t)nf * nf;
-  double a[NFP];
+  TL a[NFP];
   uint64_t mu = up, md = dn;
 #pragma unroll
   for (int j = 0; j < NFP; ++j) {
@@ -401,10 +407,11 @@ __global__ void __launch_bounds__(256)
       md &= md - 1;
     }
     if (j < nf && col < nf)
-      a[j] = orb >= 0 ? (double)M[(int64_t)orb * nf + col] + s * (double)Aw[(int64_t)j * nf + col]
-                      : 0.0;
+      a[j] = orb >= 0 ? (TL)((double)M[(int64_t)orb * nf + col] +
+                             s * (double)Aw[(int64_t)j * nf + col])
+                      : TL(0);
     else
-      a[j] = (j == col) ? 1.0 : 0.0;
+      a[j] = (j == col) ? TL(1) : TL(0);
   }
   const bool nonfinite = group_nonfinite<NFP>(a, grp * NFP);
   const GJResult r = lu_logdet<NFP>(a, col, grp * NFP, prow);
@@ -949,6 +956,8 @@ static int check_model(const vmcx_nnbf_model *m) {
     VMCX_CHECK_ARG(m->M && m->W1 && m->b1 && m->W2 && m->b2 && m->Wout && m->bout,
                    "null weight pointer");
   VMCX_CHECK_ARG(m->gemm != 1 || (m->W2_bf16 && m->Wout_bf16), "BF16 mode needs BF16 weights");
+  VMCX_CHECK_ARG(m->det_fp32 == 0 || (m->gemm == 1 && m->path == 0),
+                 "det_fp32 needs the BF16 mode on the library's matrix-core layers");
   return 0;
 }
 
@@ -1058,7 +1067,14 @@ template <int EPI>
 static int launch_ng(int64_t R, int n_rows, int K, const void *W, const void *hin,
                      const float *bias, int act, float *z, void *hout, const int64_t *cfg,
                      int N, int nf, float *A, hipStream_t st) {
-  const dim3 g((unsigned)((R + NG_RB - 1) / NG_RB));
+  // a batch of few 64-configuration blocks (the proposals of one step)
+  // splits W's tiles over blockIdx.y, keeping >= 4 tiles (one per wave) per
+  // workgroup, until about 1024 workgroups are in flight
+  const int64_t rb = (R + NG_RB - 1) / NG_RB;
+  const int n_tiles = (n_rows + 31) / 32;
+  int S = 1;
+  while (rb * S < 1024 && S * 2 * 4 <= n_tiles) S *= 2;
+  const dim3 g((unsigned)rb, (unsigned)S);
   const size_t lds = ng_lds_bytes(K);
   if (K == 256)
     hipLaunchKernelGGL((nnbf_gemm_mfma_kernel<EPI, 256>), g, dim3(NG_T), lds, st, R, n_rows, K,
@@ -1078,9 +1094,14 @@ static void launch_detA(const vmcx_nnbf_model &m, int64_t rows, const int64_t *c
   constexpr int WPW = 64 / NFP;
   const int64_t per_block = 4 * WPW;
   const int64_t nblk = (rows + per_block - 1) / per_block;
-  hipLaunchKernelGGL(nnbf_detA_kernel<NFP>, dim3((unsigned)nblk), dim3(256),
-                     (size_t)per_block * NFP * 8, st, rows, m.n_sites, m.n_f, cfg, A,
-                     (const float *)m.M, m.bf_scale, sign, logabs);
+  if (m.det_fp32)
+    hipLaunchKernelGGL((nnbf_detA_kernel<NFP, float>), dim3((unsigned)nblk), dim3(256),
+                       (size_t)per_block * NFP * 8, st, rows, m.n_sites, m.n_f, cfg, A,
+                       (const float *)m.M, m.bf_scale, sign, logabs);
+  else
+    hipLaunchKernelGGL((nnbf_detA_kernel<NFP, double>), dim3((unsigned)nblk), dim3(256),
+                       (size_t)per_block * NFP * 8, st, rows, m.n_sites, m.n_f, cfg, A,
+                       (const float *)m.M, m.bf_scale, sign, logabs);
 }
 
 // layer 1, layer 2 and the occupied-row output layer on the matrix cores,

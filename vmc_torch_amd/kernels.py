@@ -464,15 +464,18 @@ def syrk_f64(M, C, alpha=1.0, beta=0.0, splits=1, rowmajor=True):
 
 
 # ------------------------------------------------------------------ NNBF
-def nnbf_struct(n_sites, n_f, hidden, act, gemm, bf_scale, w32, wbf, w64=None, path=0):
+def nnbf_struct(n_sites, n_f, hidden, act, gemm, bf_scale, w32, wbf, w64=None, path=0,
+                det_fp32=False):
     """Fill the vmcx_nnbf_model struct.  gemm 0 / 1: w32 = dict M, W1, b1,
     W2, b2, Wout, bout of FP32 device copies (wbf: dict W2, Wout of BF16
     copies for gemm 1, else None); gemm 2: w64 = the same dict of FP64
     device tensors (w32 unused).  path (BF16 mode): 0 = the library's own
-    matrix-core layers for the forward, 1 = rocBLAS."""
+    matrix-core layers for the forward, 1 = rocBLAS.  det_fp32 (BF16 mode,
+    path 0): the amplitude-only determinants eliminated in FP32."""
     s = _lib.NnbfModelStruct()
     s.n_sites, s.n_f, s.hidden = n_sites, n_f, hidden
     s.act, s.gemm, s.path = act, gemm, int(path)
+    s.det_fp32 = 1 if det_fp32 else 0
     s.bf_scale = float(bf_scale)
     names = ("M", "W1", "b1", "W2", "b2", "Wout", "bout")
     for k in names:

@@ -200,9 +200,14 @@ class NNBF_GPU(WavefunctionModel_GPU):
     backflow in FP64, as the reference's float64 model; the default for
     dtype=float64), "fp32" (FP32 copies of the weights, FP32 GEMMs) or
     "bf16" (the two dense layers on BF16 operands with FP32 accumulation).
-    The determinant is FP64 in every mode.  bf16_layers (BF16 mode):
-    "matrix_cores" (the library's own GEMMs for the forward, only the
-    occupied orbital rows of the output layer formed) or "rocblas".
+    bf16_layers (BF16 mode): "matrix_cores" (the library's own GEMMs for the
+    forward, only the occupied orbital rows of the output layer formed) or
+    "rocblas".  det: the determinant's arithmetic, "fp64" (default), or
+    "fp32" in the BF16 mode on the matrix-core layers (the mixed-precision
+    form: the amplitude-only passes -- forward, local energy, learned
+    proposals -- eliminate in FP32 from entries formed in FP64; A^-1 for the
+    plain sweep and O_k stays FP64; pair it with the learned sampler, whose
+    acceptance then compares amplitudes of one precision).
     """
 
     ACTS = {"relu": 0, "tanh": 1}
@@ -211,7 +216,8 @@ class NNBF_GPU(WavefunctionModel_GPU):
 
     def __init__(self, n_sites, n_fermions, hidden_dim=64, n_layers=2, activation="tanh",
                  bf_scale=0.01, dtype=torch.float64, gemm=None, init="reference",
-                 m_std=1.0, generator=None, chunk=65536, bf16_layers="matrix_cores"):
+                 m_std=1.0, generator=None, chunk=65536, bf16_layers="matrix_cores",
+                 det="fp64"):
         if n_layers != 2:
             raise NotImplementedError("the NNBF kernels take exactly two hidden layers")
         if activation not in self.ACTS:
@@ -250,6 +256,11 @@ class NNBF_GPU(WavefunctionModel_GPU):
         if bf16_layers not in ("matrix_cores", "rocblas"):
             raise ValueError("bf16_layers must be 'matrix_cores' or 'rocblas'")
         self.bf16_layers = bf16_layers
+        if det not in ("fp64", "fp32"):
+            raise ValueError("det must be 'fp64' or 'fp32'")
+        if det == "fp32" and not (gemm == "bf16" and bf16_layers == "matrix_cores"):
+            raise ValueError("det='fp32' needs gemm='bf16' on the matrix-core layers")
+        self.det = det
         self.chunk = chunk
         self._w32 = None
         self._wbf = None
@@ -289,7 +300,8 @@ class NNBF_GPU(WavefunctionModel_GPU):
             self._struct_obj = kernels.nnbf_struct(
                 self.n_sites, self.n_fermions, self.hidden, self.ACTS[self.activation],
                 self.GEMMS[self.gemm], self.bf_scale, self._w32, self._wbf, w64,
-                path=0 if self.bf16_layers == "matrix_cores" else 1)
+                path=0 if self.bf16_layers == "matrix_cores" else 1,
+                det_fp32=self.det == "fp32")
             self._state_key = None
             return True
         return False
