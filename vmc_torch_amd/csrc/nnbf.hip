@@ -54,26 +54,28 @@ __device__ __forceinline__ double to_t<double>(double v) { return v; }
 // masks from a wave ballot.
 constexpr int L1_RB = 64, L1_UT = 64;
 
-template <typename TW, typename TI>
+// RB: configurations per workgroup (64; 16 for small batches, which then
+// spread over four times the workgroups)
+template <typename TW, typename TI, int RB = L1_RB>
 __global__ void __launch_bounds__(256)
     nnbf_layer1_kernel(int64_t Bc, int N, int H, int act,
                        const int64_t *__restrict__ cfg, const TW *__restrict__ W1,
                        const TW *__restrict__ b1, TW *z1, TI *h1) {
-  // dynamic LDS: the W1 tile [L1_UT][2N + 1] then the masks [2][L1_RB]
+  // dynamic LDS: the W1 tile [L1_UT][2N + 1] then the masks [2][RB]
   extern __shared__ __align__(16) char l1sm[];
   const int n_orb = 2 * N, ld = n_orb + 1;
   TW *wt = reinterpret_cast<TW *>(l1sm);
   uint64_t *mu = reinterpret_cast<uint64_t *>(l1sm + (((size_t)L1_UT * ld * sizeof(TW) + 15) &
                                                       ~(size_t)15));
-  uint64_t *md = mu + L1_RB;
-  const int64_t r0 = (int64_t)blockIdx.x * L1_RB;
+  uint64_t *md = mu + RB;
+  const int64_t r0 = (int64_t)blockIdx.x * RB;
   const int u0 = blockIdx.y * L1_UT;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   for (int idx = t; idx < L1_UT * n_orb; idx += blockDim.x) {
     const int uu = idx / n_orb, o = idx - uu * n_orb;
     wt[uu * ld + o] = (u0 + uu < H) ? W1[(int64_t)(u0 + uu) * n_orb + o] : TW(0);
   }
-  for (int r = wave; r < L1_RB; r += blockDim.x / 64) {
+  for (int r = wave; r < RB; r += blockDim.x / 64) {
     const int64_t w = r0 + r;
     int64_t x = 0;
     if (w < Bc && lane < N) x = cfg[w * N + lane];
@@ -87,7 +89,7 @@ __global__ void __launch_bounds__(256)
   const int u = u0 + lane;
   if (u >= H) return;
   const TW bias = b1[u];
-  for (int r = wave; r < L1_RB; r += blockDim.x / 64) {
+  for (int r = wave; r < RB; r += blockDim.x / 64) {
     const int64_t w = r0 + r;
     if (w >= Bc) break;
     const uint64_t bu = mu[r], bd = md[r];
@@ -1037,14 +1039,22 @@ static void launch_det(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
 template <typename TW, typename TI>
 static int layer1_to(const vmcx_nnbf_model &m, int64_t rows, const int64_t *cfg, TW *z1, TI *h1,
                      hipStream_t st) {
-  const dim3 g1((unsigned)((rows + L1_RB - 1) / L1_RB),
-                (unsigned)((m.hidden + L1_UT - 1) / L1_UT));
+  const unsigned ub = (unsigned)((m.hidden + L1_UT - 1) / L1_UT);
+  // small batches (fewer than ~1024 workgroups of 64 configurations) take
+  // 16 configurations per workgroup
+  const bool small = ((rows + L1_RB - 1) / L1_RB) * ub < 1024;
+  const int rb = small ? 16 : L1_RB;
+  const dim3 g1((unsigned)((rows + rb - 1) / rb), ub);
   const size_t lds = (((size_t)L1_UT * (2 * m.n_sites + 1) * sizeof(TW) + 15) & ~(size_t)15) +
-                     2 * L1_RB * sizeof(uint64_t);
+                     2 * rb * sizeof(uint64_t);
   VMCX_CHECK_ARG(lds <= max_lds_per_block(), "layer-1 LDS exceeds the device limit");
   const NnW<TW> W = weights_of<TW>(m);
-  hipLaunchKernelGGL((nnbf_layer1_kernel<TW, TI>), g1, dim3(256), lds, st, rows, m.n_sites,
-                     m.hidden, m.act, cfg, W.W1, W.b1, z1, h1);
+  if (small)
+    hipLaunchKernelGGL((nnbf_layer1_kernel<TW, TI, 16>), g1, dim3(256), lds, st, rows,
+                       m.n_sites, m.hidden, m.act, cfg, W.W1, W.b1, z1, h1);
+  else
+    hipLaunchKernelGGL((nnbf_layer1_kernel<TW, TI, L1_RB>), g1, dim3(256), lds, st, rows,
+                       m.n_sites, m.hidden, m.act, cfg, W.W1, W.b1, z1, h1);
   VMCX_LAUNCHED();
   return 0;
 }
