@@ -249,53 +249,97 @@ struct LuStep<NFP, NFP, T> {
 // One 64 x 64 matrix per wave: the pivot is the same for every lane, so the
 // arg-max is a DPP max-scan (row_shr 1, 2, 4, 8, row_bcast 15, 31; the wave
 // maximum ends in lane 63) and the pivot row is read with readlane into
-// scalar registers, no LDS.  Every lane applies the update (f = 1 on the
-// pivot lane, anything on finished lanes, whose registers are never read
-// again), so the FMAs need no lane mask.
+// scalar registers, no LDS.  Every lane applies the update, the pivot lane
+// with multiplier exactly 1: its registers past P become exact zeros, so a
+// lane once chosen keys at most 63 (the lane bits alone) at every later step
+// and loses to any unchosen lane with a nonzero entry; no per-lane "done"
+// test is needed.  (If every unchosen entry is zero the matrix is singular
+// and whichever lane wins carries a zero pivot.)  The pivot of each step is
+// written into the chosen lane of `pivs`; sign and log|det| are formed from
+// them once, after the last step (lu64_finish).
+// multiplier a / piv: FP32 by the hardware reciprocal (1 ulp), FP64 by the
+// correctly rounded division
+__device__ __forceinline__ float lu_mult(float a, float piv) {
+  return a * __builtin_amdgcn_rcpf(piv);
+}
+__device__ __forceinline__ double lu_mult(double a, double piv) { return a / piv; }
+
 template <int P, typename T>
 struct LuStep64 {
-  static __device__ __forceinline__ void run(T (&a)[64], int col, LuState &S) {
-    const bool done = (S.used >> col) & 1ull;
+  static __device__ __forceinline__ void run(T (&a)[64], int col, T &pivs, uint64_t &used,
+                                             int &parity) {
+    // pivot key: |a| rounded to float with its 6 low mantissa bits replaced
+    // by 63 - lane (largest magnitude to 2^-17, then the lowest lane); NaN /
+    // Inf rank as 0
     const T mag = fabs(a[P]);
     const float fm = (mag <= lu_big<T>()) ? (float)mag : 0.0f;
-    const uint32_t key = done ? 0u : ((__float_as_uint(fm) & ~63u) | (uint32_t)(63 - col));
+    const uint32_t key = (__float_as_uint(fm) & ~63u) | (uint32_t)(63 - col);
     const int q = 63 - (int)(lu_wave_max(key) & 63u);
     const T piv = lu_bcast(a[P], q);
-    if (!(piv != T(0)) || !(fabs(piv) <= lu_big<T>())) S.singular = true;
+    const bool mine = col == q;
+    pivs = mine ? piv : pivs;
     if (P + 1 < 64) {
-      const T f = (piv != T(0)) ? a[P] / piv : T(0);
+      const T f = mine ? T(1) : lu_mult(a[P], piv);
+      // blocks of 8 pivot-row entries kept apart by scheduling barriers, so
+      // that the readlanes of one block are consumed before the next block's
+      // are issued (hoisting all of them overflows the scalar registers)
 #pragma unroll
-      for (int j = P + 1; j < 64; ++j) a[j] = fma(-f, lu_bcast(a[j], q), a[j]);
+      for (int j0 = P + 1; j0 < 64; j0 += 8) {
+        T r[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (j0 + k < 64) r[k] = lu_bcast(a[j0 + k], q);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (j0 + k < 64) a[j0 + k] = fma(-f, r[k], a[j0 + k]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
-    if (piv != T(0)) {
-      int e;
-      S.mprod *= frexp(fabs((double)piv), &e);
-      S.esum += e;
-      S.mprod = frexp(S.mprod, &e);
-      S.esum += e;
-    }
-    S.parity += (piv < T(0)) ? 1 : 0;
-    S.parity += __popcll(S.used & ~((2ull << q) - 1ull));
-    S.used |= 1ull << q;
-    LuStep64<P + 1, T>::run(a, col, S);
+    // row exchanges: lanes chosen earlier with a higher index than q
+    parity += __popcll(used & ~((2ull << q) - 1ull));
+    used |= 1ull << q;
+    LuStep64<P + 1, T>::run(a, col, pivs, used, parity);
   }
 };
 
 template <typename T>
 struct LuStep64<64, T> {
-  static __device__ __forceinline__ void run(T (&)[64], int, LuState &) {}
+  static __device__ __forceinline__ void run(T (&)[64], int, T &, uint64_t &, int &) {}
 };
 
+// sign and log|det| from the 64 pivots (lane l: the pivot taken from lane
+// l), summed in FP64 over the wave; a zero, non-finite or missing pivot
+// (a lane never chosen keeps 0) makes the matrix singular
+template <typename T>
+__device__ __forceinline__ void lu64_finish(T pivs, uint64_t used, int parity, double &logdet,
+                                            int &par, bool &singular) {
+  const bool bad = !(pivs != T(0)) || !(fabs(pivs) <= lu_big<T>());
+  singular = (__ballot(bad) != 0ull) || (used != ~0ull);
+  par = parity + __popcll(__ballot(pivs < T(0)));
+  double l = bad ? 0.0 : log(fabs((double)pivs));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
+  logdet = l;
+}
+
 // T = double, or float (the mixed-precision amplitude: FP32 elimination,
-// |det| still accumulated as an FP64 mantissa product and exponent sum)
+// log|det| still accumulated in FP64: as a sum of the pivots' logarithms at
+// NFP = 64, as a mantissa product and an exponent sum otherwise)
 template <int NFP, typename T>
 __device__ __forceinline__ GJResult lu_logdet(T (&a)[NFP], int col, int gbase, T *prow) {
-  LuState S{1.0, 0, 0, 0ull, false};
-  if constexpr (NFP == 64)
-    LuStep64<0, T>::run(a, col, S);
-  else
+  if constexpr (NFP == 64) {
+    T pivs = T(0);
+    uint64_t used = 0ull;
+    int parity = 0;
+    LuStep64<0, T>::run(a, col, pivs, used, parity);
+    GJResult r{0.0, 0, false, -1};
+    lu64_finish(pivs, used, parity, r.logdet, r.parity, r.singular);
+    return r;
+  } else {
+    LuState S{1.0, 0, 0, 0ull, false};
     LuStep<NFP, 0, T>::run(a, col, gbase, prow, S);
-  return {log(S.mprod) + (double)S.esum * 0.69314718055994530942, S.parity, S.singular, -1};
+    return {log(S.mprod) + (double)S.esum * 0.69314718055994530942, S.parity, S.singular, -1};
+  }
 }
 
 }  // namespace vmcx

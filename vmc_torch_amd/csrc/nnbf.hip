@@ -374,10 +374,10 @@ __global__ void __launch_bounds__(NG_T)
 
 // Determinant from the compact Docc ([nf][nf] FP32 per configuration,
 // written by the NG_SCAT epilogue): lane col of the group holds column col,
-// a[j] = M[orb_j][col] + s Docc[j][col] in FP64 for orb_j the j-th occupied
-// orbital (ascending); padding as nnbf_rows.
+// a[j] = M[orb_j][col] + s Docc[j][col] for orb_j the j-th occupied orbital
+// (ascending); padding as nnbf_rows.
 // TL: the elimination's type (double; float for the FP32 determinant of the
-// mixed-precision amplitude: the entries are formed in FP64 and rounded)
+// mixed-precision amplitude, whose entries are formed by one FP32 fma)
 template <int NFP, typename TL>
 __global__ void __launch_bounds__(256)
     nnbf_detA_kernel(int64_t Bc, int N, int nf, const int64_t *__restrict__ cfg,
@@ -393,9 +393,19 @@ __global__ void __launch_bounds__(256)
   if (w >= Bc) return;
   TL *prow = reinterpret_cast<TL *>(smem) + wl * NFP;
   uint64_t up, dn;
-  group_masks<NFP>(cfg + w * N, N, col, up, dn);
+  if constexpr (NFP == 64) {
+    // one matrix per wave (N <= 64): the occupation masks by two ballots,
+    // so that they and the orbital of every row stay in scalar registers
+    const int64_t x = col < N ? cfg[w * N + col] : 0;
+    up = __ballot((x & 2) != 0);
+    dn = __ballot((x & 1) != 0);
+  } else {
+    group_masks<NFP>(cfg + w * N, N, col, up, dn);
+  }
   const bool count_ok = (__popcll(up) + __popcll(dn)) == nf;
   const float *Aw = A + w * (int64_t)nf * nf;
+  // FP32 elimination: the entries as fmaf(s, Docc, M), one rounding
+  const float sf = (float)s;
   TL a[NFP];
   uint64_t mu = up, md = dn;
 #pragma unroll
@@ -408,12 +418,16 @@ __global__ void __launch_bounds__(256)
       orb = N + __ffsll((long long)md) - 1;
       md &= md - 1;
     }
-    if (j < nf && col < nf)
-      a[j] = orb >= 0 ? (TL)((double)M[(int64_t)orb * nf + col] +
-                             s * (double)Aw[(int64_t)j * nf + col])
-                      : TL(0);
-    else
+    if (j < nf && col < nf) {
+      if (orb < 0)
+        a[j] = TL(0);
+      else if constexpr (sizeof(TL) == 4)
+        a[j] = fmaf(sf, Aw[(int64_t)j * nf + col], M[(int64_t)orb * nf + col]);
+      else
+        a[j] = (double)M[(int64_t)orb * nf + col] + s * (double)Aw[(int64_t)j * nf + col];
+    } else {
       a[j] = (j == col) ? TL(1) : TL(0);
+    }
   }
   const bool nonfinite = group_nonfinite<NFP>(a, grp * NFP);
   const GJResult r = lu_logdet<NFP>(a, col, grp * NFP, prow);
