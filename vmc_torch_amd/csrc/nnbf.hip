@@ -244,6 +244,133 @@ __host__ __device__ inline size_t ng_lds_bytes(int K) {
   return (size_t)NG_RB * (K + 8) * 2 + (size_t)NG_RB * 128;
 }
 
+// Epilogue of one 32-row tile n0 of W against two 32-configuration tiles
+// (acc0: configurations r0 + 0..31, acc1: r0 + 32..63); the slot table rows
+// of these configurations start at srow.
+// bias rows of a lane's accumulator registers: bv[q] = b[n0 + 8 (q / 4) +
+// 4 kh + q % 4] (0 past n_rows), loaded before the tile's products so that
+// the epilogue does not wait for them
+__device__ __forceinline__ void ng_bias(const float *__restrict__ bias, int n0, int n_rows,
+                                        int kh, float (&bv)[16]) {
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int nb = n0 + 8 * g + 4 * kh;
+    if (nb + 3 < n_rows && (n_rows & 3) == 0) {
+      const float4 v = *reinterpret_cast<const float4 *>(bias + nb);
+      bv[4 * g + 0] = v.x;
+      bv[4 * g + 1] = v.y;
+      bv[4 * g + 2] = v.z;
+      bv[4 * g + 3] = v.w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) bv[4 * g + e] = (nb + e < n_rows) ? bias[nb + e] : 0.0f;
+    }
+  }
+}
+
+template <int EPI>
+__device__ __forceinline__ void ng_epilogue(int n0, const ng_f16v &acc0, const ng_f16v &acc1,
+                                            int64_t r0, int srow, int lane, int64_t R,
+                                            int n_rows, const float (&bv)[16], int act,
+                                            float *z, __hip_bfloat16 *hout,
+                                            const uint8_t *slot, int nf, float *A) {
+  const int kh = lane >> 5;
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const int rl = half * 32 + (lane & 31);
+    const int64_t r = r0 + rl;
+    if (r >= R) continue;
+    const ng_f16v &acc = half ? acc1 : acc0;
+    if (EPI == NG_ACT) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int n = n0 + 8 * (q >> 2) + 4 * kh + (q & 3);
+        if (n >= n_rows) continue;
+        const float v = acc[q] + bv[q];
+        if (z) z[r * n_rows + n] = v;
+        hout[r * n_rows + n] = __float2bfloat16(act == 0 ? fmaxf(v, 0.0f) : tanhf(v));
+      }
+    } else {
+      float *Ar = A + r * (int64_t)nf * nf;
+      const uint8_t *srow_p = slot + (srow + rl) * 128;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int nb = n0 + 8 * g + 4 * kh;       // rows nb .. nb + 3
+        if (nb >= n_rows) continue;
+        const int o = nb / nf, j = nb - o * nf;
+        const int sl = srow_p[o];
+        if (sl == 0xFF) {
+          // an orbital boundary inside the group (nf not a multiple of 4):
+          // the other rows may belong to an occupied orbital
+          if ((nf & 3) == 0 || j + 3 < nf) continue;
+        }
+        if ((nf & 3) == 0) {
+          float4 v;
+          v.x = acc[4 * g + 0] + bv[4 * g + 0];
+          v.y = acc[4 * g + 1] + bv[4 * g + 1];
+          v.z = acc[4 * g + 2] + bv[4 * g + 2];
+          v.w = acc[4 * g + 3] + bv[4 * g + 3];
+          *reinterpret_cast<float4 *>(Ar + (int64_t)sl * nf + j) = v;
+        } else {
+          for (int e = 0; e < 4; ++e) {
+            const int n = nb + e;
+            if (n >= n_rows) break;
+            const int oo = n / nf, jj = n - oo * nf;
+            const int ss = srow_p[oo];
+            if (ss == 0xFF) continue;
+            Ar[(int64_t)ss * nf + jj] = acc[4 * g + e] + bv[4 * g + e];
+          }
+        }
+      }
+    }
+  }
+}
+
+// slot table rows srow .. srow + nr - 1 for configurations c0 .. c0 + nr - 1
+// (one wave): slot[(srow + r) * 128 + o] = rank of orbital o among the
+// occupied orbitals of configuration c0 + r (0xFF: empty, or past nf)
+__device__ __forceinline__ void ng_slot_row(int64_t x, int N, int nf, int lane, uint8_t *row) {
+  const uint64_t up = __ballot((x & 2) != 0), dn = __ballot((x & 1) != 0);
+  const int nup = __popcll(up);
+  // entry e of the row: orbital e (up orbitals 0 .. N-1, down N .. 2N-1)
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int e = h * 64 + lane;
+    uint8_t v = 0xFF;
+    if (e < 2 * N) {
+      const bool is_dn = e >= N;
+      const int site = is_dn ? e - N : e;
+      const uint64_t m = is_dn ? dn : up;
+      const uint64_t below = (site == 0) ? 0ull : ((1ull << site) - 1ull);
+      const int sl = (is_dn ? nup : 0) + __popcll(m & below);
+      // a slot past nf (more occupied orbitals than nf) is dropped: the
+      // determinant kernel reports such a configuration as NaN
+      if (((m >> site) & 1ull) && sl < nf) v = (uint8_t)sl;
+    }
+    row[e] = v;
+  }
+}
+
+__device__ __forceinline__ void ng_slots(const int64_t *__restrict__ cfg, int64_t c0, int nr,
+                                         int64_t R, int N, int nf, int lane, uint8_t *slot,
+                                         int srow) {
+  // eight configurations' loads in flight at a time
+  int r = 0;
+  for (; r + 8 <= nr; r += 8) {
+    int64_t x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      x[u] = (c0 + r + u < R && lane < N) ? cfg[(c0 + r + u) * N + lane] : 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) ng_slot_row(x[u], N, nf, lane, slot + (srow + r + u) * 128);
+  }
+  for (; r < nr; ++r) {
+    int64_t x = 0;
+    if (c0 + r < R && lane < N) x = cfg[(c0 + r) * N + lane];
+    ng_slot_row(x, N, nf, lane, slot + (srow + r) * 128);
+  }
+}
+
 // KFIX > 0: the input width known at compile time (the k loop fully unrolled,
 // all weight-fragment loads of a tile in flight at once); 0: K at run time.
 template <int EPI, int KFIX>
@@ -267,33 +394,8 @@ __global__ void __launch_bounds__(NG_T)
     if (r0 + r < R) v = reinterpret_cast<const uint4 *>(hin + (r0 + r) * K)[c];
     *reinterpret_cast<uint4 *>(hs + r * ldh + c * 8) = v;
   }
-  if (EPI == NG_SCAT) {
-    // slot of every orbital in each configuration (0xFF: empty), by a wave
-    // ballot over the sites and popcount prefixes
-    for (int r = wave; r < NG_RB; r += NG_T / 64) {
-      int64_t x = 0;
-      if (r0 + r < R && lane < N) x = cfg[(r0 + r) * N + lane];
-      const uint64_t up = __ballot((x & 2) != 0), dn = __ballot((x & 1) != 0);
-      const int nup = __popcll(up);
-      // entry e of the row: orbital e (up orbitals 0 .. N-1, down N .. 2N-1)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int e = h * 64 + lane;
-        uint8_t v = 0xFF;
-        if (e < 2 * N) {
-          const bool is_dn = e >= N;
-          const int site = is_dn ? e - N : e;
-          const uint64_t m = is_dn ? dn : up;
-          const uint64_t below = (site == 0) ? 0ull : ((1ull << site) - 1ull);
-          const int sl = (is_dn ? nup : 0) + __popcll(m & below);
-          // a slot past nf (more occupied orbitals than nf) is dropped: the
-          // determinant kernel reports such a configuration as NaN
-          if (((m >> site) & 1ull) && sl < nf) v = (uint8_t)sl;
-        }
-        slot[r * 128 + e] = v;
-      }
-    }
-  }
+  if (EPI == NG_SCAT)
+    for (int r = wave; r < NG_RB; r += NG_T / 64) ng_slots(cfg, r0 + r, 1, R, N, nf, lane, slot, r);
   __syncthreads();
   // blockIdx.y: a contiguous slice of the 32-row tiles of W (small batches
   // spread over more workgroups)
@@ -311,6 +413,8 @@ __global__ void __launch_bounds__(NG_T)
       acc0[q] = 0.0f;
       acc1[q] = 0.0f;
     }
+    float bv[16];
+    ng_bias(bias, n0, n_rows, kh, bv);
     const __bf16 *b0p = hs + (lane & 31) * ldh + 8 * kh;
     const __bf16 *b1p = hs + (32 + (lane & 31)) * ldh + 8 * kh;
 #pragma unroll (KFIX > 0 ? KFIX / 16 : 4)
@@ -321,54 +425,104 @@ __global__ void __launch_bounds__(NG_T)
       acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc0, 0, 0, 0);
       acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, acc1, 0, 0, 0);
     }
+    ng_epilogue<EPI>(n0, acc0, acc1, r0, 0, lane, R, n_rows, bv, act, z, hout, slot, nf, A);
+  }
+}
+
+// The same layers for input width 256 with the operands swapped in roles:
+// a wave keeps the B fragments of its 64 configurations (h rows, all 16
+// k-steps: 128 registers) for the whole launch, and the workgroup's four
+// waves stream W's 32-row tiles through LDS, double-buffered: while a tile
+// is multiplied, the next one (16 KiB, contiguous in the row-major W) is
+// loaded by all 256 threads into registers and written to the other buffer
+// after the tile's epilogue.  Each W fragment read from LDS feeds two MFMAs
+// (the wave's two configuration tiles), each W byte fetched from L2 serves
+// 256 configurations.  W tile rows are padded to 264 BF16 in LDS (16-byte
+// fragment reads of 16 lanes fall in distinct banks).  Epilogues as
+// nnbf_gemm_mfma_kernel; grid: (configurations / 256, slices of W's tiles).
+constexpr int NG2_T = 256, NG2_CPW = 64, NG2_CPB = NG2_CPW * (NG2_T / 64);
+constexpr int NG2_K = 256, NG2_LDW = NG2_K + 8;
+
+template <int EPI>
+__global__ void __launch_bounds__(NG2_T)
+    nnbf_ng2_kernel(int64_t R, int n_rows, const __hip_bfloat16 *__restrict__ W,
+                    const __hip_bfloat16 *__restrict__ hin, const float *__restrict__ bias,
+                    int act, float *z, __hip_bfloat16 *hout, const int64_t *__restrict__ cfg,
+                    int N, int nf, float *A) {
+  __shared__ __align__(16) __bf16 Ws[2 * 32 * NG2_LDW];                  // [2][32][264]
+  __shared__ __align__(16) uint8_t slot[EPI == NG_SCAT ? NG2_CPB * 128 : 16];  // [256][128]
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, kh = lane >> 5;
+  const int64_t rw = (int64_t)blockIdx.x * NG2_CPB + (int64_t)wave * NG2_CPW;
+  const __bf16 *Wb = reinterpret_cast<const __bf16 *>(W);
+  const __bf16 *hb = reinterpret_cast<const __bf16 *>(hin);
+  // the wave's B fragments: configuration rw + 32 c + lane % 32, k = 16 ks + 8 kh .. + 7
+  ng_bf8 b[2][16];
 #pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      const int rl = half * 32 + (lane & 31);
-      const int64_t r = r0 + rl;
-      if (r >= R) continue;
-      const ng_f16v &acc = half ? acc1 : acc0;
-      if (EPI == NG_ACT) {
+  for (int c = 0; c < 2; ++c) {
+    const int64_t row = rw + 32 * c + (lane & 31);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const int n = n0 + 8 * (q >> 2) + 4 * kh + (q & 3);
-          if (n >= n_rows) continue;
-          const float v = acc[q] + bias[n];
-          if (z) z[r * n_rows + n] = v;
-          hout[r * n_rows + n] = __float2bfloat16(act == 0 ? fmaxf(v, 0.0f) : tanhf(v));
-        }
-      } else {
-        float *Ar = A + r * (int64_t)nf * nf;
+    for (int ks = 0; ks < 16; ++ks) {
+      ng_bf8 v;
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const int nb = n0 + 8 * g + 4 * kh;       // rows nb .. nb + 3
-          if (nb >= n_rows) continue;
-          const int o = nb / nf, j = nb - o * nf;
-          const int sl = slot[rl * 128 + o];
-          if (sl == 0xFF) {
-            // an orbital boundary inside the group (nf not a multiple of 4):
-            // the other rows may belong to an occupied orbital
-            if ((nf & 3) == 0 || j + 3 < nf) continue;
-          }
-          if ((nf & 3) == 0) {
-            float4 v;
-            v.x = acc[4 * g + 0] + bias[nb + 0];
-            v.y = acc[4 * g + 1] + bias[nb + 1];
-            v.z = acc[4 * g + 2] + bias[nb + 2];
-            v.w = acc[4 * g + 3] + bias[nb + 3];
-            *reinterpret_cast<float4 *>(Ar + (int64_t)sl * nf + j) = v;
-          } else {
-            for (int e = 0; e < 4; ++e) {
-              const int n = nb + e;
-              if (n >= n_rows) break;
-              const int oo = n / nf, jj = n - oo * nf;
-              const int ss = slot[rl * 128 + oo];
-              if (ss == 0xFF) continue;
-              Ar[(int64_t)ss * nf + jj] = acc[4 * g + e] + bias[n];
-            }
-          }
-        }
-      }
+      for (int e = 0; e < 8; ++e) v[e] = (__bf16)0.0f;
+      if (row < R) v = *reinterpret_cast<const ng_bf8 *>(hb + row * NG2_K + 16 * ks + 8 * kh);
+      b[c][ks] = v;
     }
+  }
+  if (EPI == NG_SCAT) ng_slots(cfg, rw, NG2_CPW, R, N, nf, lane, slot, wave * NG2_CPW);
+  // this workgroup's slice of W's tiles
+  const int n_tiles = (n_rows + 31) / 32;
+  const int tps = (n_tiles + (int)gridDim.y - 1) / (int)gridDim.y;
+  const int t0 = (int)blockIdx.y * tps;
+  const int t1 = min(n_tiles, t0 + tps);
+  // tile staging: 16-byte chunk c = t + 256 i (i < 4) of the 16 KiB tile is
+  // row c / 32, columns 8 (c % 32) .. + 7
+  uint4 g[4];
+  auto load_tile = [&](int tile) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c = t + NG2_T * i;
+      const int row = tile * 32 + (c >> 5);
+      g[i] = row < n_rows
+                 ? *reinterpret_cast<const uint4 *>(Wb + (int64_t)row * NG2_K + 8 * (c & 31))
+                 : make_uint4(0u, 0u, 0u, 0u);
+    }
+  };
+  auto store_tile = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c = t + NG2_T * i;
+      *reinterpret_cast<uint4 *>(Ws + (buf * 32 + (c >> 5)) * NG2_LDW + 8 * (c & 31)) = g[i];
+    }
+  };
+  if (t0 < t1) {
+    load_tile(t0);
+    store_tile(0);
+  }
+  __syncthreads();
+  for (int tile = t0; tile < t1; ++tile) {
+    const int buf = (tile - t0) & 1;
+    const bool more = tile + 1 < t1;
+    if (more) load_tile(tile + 1);
+    ng_f16v acc0, acc1;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      acc0[q] = 0.0f;
+      acc1[q] = 0.0f;
+    }
+    float bv[16];
+    ng_bias(bias, tile * 32, n_rows, kh, bv);
+    const __bf16 *ap = Ws + (buf * 32 + (lane & 31)) * NG2_LDW + 8 * kh;
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+      const ng_bf8 a = *reinterpret_cast<const ng_bf8 *>(ap + 16 * ks);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[0][ks], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[1][ks], acc1, 0, 0, 0);
+    }
+    ng_epilogue<EPI>(tile * 32, acc0, acc1, rw, wave * NG2_CPW, lane, R, n_rows, bv, act, z,
+                     hout, slot, nf, A);
+    if (more) store_tile(buf ^ 1);
+    __syncthreads();
   }
 }
 
@@ -1094,8 +1248,21 @@ static int launch_ng(int64_t R, int n_rows, int K, const void *W, const void *hi
   // a batch of few 64-configuration blocks (the proposals of one step)
   // splits W's tiles over blockIdx.y, keeping >= 4 tiles (one per wave) per
   // workgroup, until about 1024 workgroups are in flight
-  const int64_t rb = (R + NG_RB - 1) / NG_RB;
   const int n_tiles = (n_rows + 31) / 32;
+  if (EPI == NG_SCAT && K == NG2_K) {
+    // 256 configurations per workgroup; W's tiles split over blockIdx.y
+    // (at least 4 tiles per slice) until about 512 workgroups are in flight
+    const int64_t rb = (R + NG2_CPB - 1) / NG2_CPB;
+    int S = 1;
+    while (rb * S < 512 && S * 2 * 4 <= n_tiles) S *= 2;
+    hipLaunchKernelGGL((nnbf_ng2_kernel<EPI>), dim3((unsigned)rb, (unsigned)S), dim3(NG2_T), 0,
+                       st, R, n_rows, (const __hip_bfloat16 *)W,
+                       (const __hip_bfloat16 *)hin, bias, act, z, (__hip_bfloat16 *)hout, cfg, N,
+                       nf, A);
+    VMCX_LAUNCHED();
+    return 0;
+  }
+  const int64_t rb = (R + NG_RB - 1) / NG_RB;
   int S = 1;
   while (rb * S < 1024 && S * 2 * 4 <= n_tiles) S *= 2;
   const dim3 g((unsigned)rb, (unsigned)S);
