@@ -248,15 +248,20 @@ struct LuStep<NFP, NFP, T> {
 
 // One 64 x 64 matrix per wave: the pivot is the same for every lane, so the
 // arg-max is a DPP max-scan (row_shr 1, 2, 4, 8, row_bcast 15, 31; the wave
-// maximum ends in lane 63) and the pivot row is read with readlane into
-// scalar registers, no LDS.  Every lane applies the update, the pivot lane
-// with multiplier exactly 1: its registers past P become exact zeros, so a
-// lane once chosen keys at most 63 (the lane bits alone) at every later step
-// and loses to any unchosen lane with a nonzero entry; no per-lane "done"
-// test is needed.  (If every unchosen entry is zero the matrix is singular
-// and whichever lane wins carries a zero pivot.)  The pivot of each step is
-// written into the chosen lane of `pivs`; sign and log|det| are formed from
-// them once, after the last step (lu64_finish).
+// maximum ends in lane 63).  The pivot row reaches the other lanes two ways
+// at once: its head by readlane into scalar registers (two VALU
+// instructions per entry with the update), its tail through the wave's
+// 64-entry LDS buffer (written by the pivot lane as 16-byte vectors, read
+// back by every lane as broadcasts: one VALU instruction per entry); the
+// tail takes 4/6 of the entries in FP32, 3/6 in FP64, the split that keeps
+// both the VALU and the LDS pipes busy.  Every lane applies the update, the
+// pivot lane with multiplier exactly 1: its registers past P become exact
+// zeros, so a lane once chosen keys at most 63 (the lane bits alone) at
+// every later step and loses to any unchosen lane with a nonzero entry; no
+// per-lane "done" test is needed.  (If every unchosen entry is zero the
+// matrix is singular and whichever lane wins carries a zero pivot.)  The
+// pivot of each step is kept in the chosen lane of `pivs`; sign and
+// log|det| are formed from them once, after the last step (lu64_finish).
 // multiplier a / piv: FP32 by the hardware reciprocal (1 ulp), FP64 by the
 // correctly rounded division
 __device__ __forceinline__ float lu_mult(float a, float piv) {
@@ -264,10 +269,28 @@ __device__ __forceinline__ float lu_mult(float a, float piv) {
 }
 __device__ __forceinline__ double lu_mult(double a, double piv) { return a / piv; }
 
+// 16-byte vectors of the LDS part
+template <typename T>
+struct LuVec;
+template <>
+struct LuVec<float> {
+  typedef float4 t;
+  static constexpr int n = 4, tail6 = 4;
+  static __device__ __forceinline__ t make(const float *x) {
+    return make_float4(x[0], x[1], x[2], x[3]);
+  }
+};
+template <>
+struct LuVec<double> {
+  typedef double2 t;
+  static constexpr int n = 2, tail6 = 3;
+  static __device__ __forceinline__ t make(const double *x) { return make_double2(x[0], x[1]); }
+};
+
 template <int P, typename T>
 struct LuStep64 {
   static __device__ __forceinline__ void run(T (&a)[64], int col, T &pivs, uint64_t &used,
-                                             int &parity) {
+                                             int &parity, T *buf) {
     // pivot key: |a| rounded to float with its 6 low mantissa bits replaced
     // by 63 - lane (largest magnitude to 2^-17, then the lowest lane); NaN /
     // Inf rank as 0
@@ -280,31 +303,56 @@ struct LuStep64 {
     pivs = mine ? piv : pivs;
     if (P + 1 < 64) {
       const T f = mine ? T(1) : lu_mult(a[P], piv);
-      // blocks of 8 pivot-row entries kept apart by scheduling barriers, so
+      typedef LuVec<T> V;
+      // entries [P + 1, js) by readlane, [js, 64) through LDS (js aligned)
+      constexpr int n_tail = ((63 - P) * V::tail6) / 6;
+      constexpr int js0 = (64 - n_tail + V::n - 1) & ~(V::n - 1);
+      constexpr int js = js0 < P + 1 ? ((P + 1 + V::n - 1) & ~(V::n - 1)) : js0;
+      if (js < 64 && mine) {
+#pragma unroll
+        for (int j = js; j < 64; j += V::n)
+          *reinterpret_cast<typename V::t *>(buf + j) = V::make(&a[j]);
+      }
+      // blocks of 8 readlane entries kept apart by scheduling barriers, so
       // that the readlanes of one block are consumed before the next block's
       // are issued (hoisting all of them overflows the scalar registers)
 #pragma unroll
-      for (int j0 = P + 1; j0 < 64; j0 += 8) {
+      for (int j0 = P + 1; j0 < js && j0 < 64; j0 += 8) {
         T r[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-          if (j0 + k < 64) r[k] = lu_bcast(a[j0 + k], q);
+          if (j0 + k < js && j0 + k < 64) r[k] = lu_bcast(a[j0 + k], q);
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-          if (j0 + k < 64) a[j0 + k] = fma(-f, r[k], a[j0 + k]);
+          if (j0 + k < js && j0 + k < 64) a[j0 + k] = fma(-f, r[k], a[j0 + k]);
         __builtin_amdgcn_sched_barrier(0);
+      }
+      if (js < 64) {
+        wave_lds_fence();
+#pragma unroll
+        for (int j = js; j < 64; j += V::n) {
+          const typename V::t v = *reinterpret_cast<const typename V::t *>(buf + j);
+          a[j] = fma(-f, v.x, a[j]);
+          a[j + 1] = fma(-f, v.y, a[j + 1]);
+          if constexpr (V::n == 4) {
+            a[j + 2] = fma(-f, v.z, a[j + 2]);
+            a[j + 3] = fma(-f, v.w, a[j + 3]);
+          }
+        }
+        // the buffer is rewritten by the next step's pivot lane
+        wave_lds_fence();
       }
     }
     // row exchanges: lanes chosen earlier with a higher index than q
     parity += __popcll(used & ~((2ull << q) - 1ull));
     used |= 1ull << q;
-    LuStep64<P + 1, T>::run(a, col, pivs, used, parity);
+    LuStep64<P + 1, T>::run(a, col, pivs, used, parity, buf);
   }
 };
 
 template <typename T>
 struct LuStep64<64, T> {
-  static __device__ __forceinline__ void run(T (&)[64], int, T &, uint64_t &, int &) {}
+  static __device__ __forceinline__ void run(T (&)[64], int, T &, uint64_t &, int &, T *) {}
 };
 
 // sign and log|det| from the 64 pivots (lane l: the pivot taken from lane
@@ -331,7 +379,7 @@ __device__ __forceinline__ GJResult lu_logdet(T (&a)[NFP], int col, int gbase, T
     T pivs = T(0);
     uint64_t used = 0ull;
     int parity = 0;
-    LuStep64<0, T>::run(a, col, pivs, used, parity);
+    LuStep64<0, T>::run(a, col, pivs, used, parity, prow);
     GJResult r{0.0, 0, false, -1};
     lu64_finish(pivs, used, parity, r.logdet, r.parity, r.singular);
     return r;
