@@ -389,6 +389,27 @@ int vmcx_nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *config
                       void *workspace, int64_t ws_bytes, int64_t chunk, double *sign,
                       double *logabs, void *z1_out, void *stream);
 
+/* (sign, ln|psi|) of B proposed configurations `proposed`, row b differing
+ * from configs[b] by a few moved electrons, with the first layer formed from
+ * z1 (configs' pre-activations [B][hidden], as vmcx_nnbf_forward's z1_out)
+ * plus, for each orbital whose occupation differs, +-W1[:, orbital] (+ when
+ * occupied in `proposed`); the rest of the forward as vmcx_nnbf_forward.
+ * z1_proposed receives the proposed configurations' pre-activations.
+ * Replaces: the model amplitude of the proposed configurations in a
+ * sampler step (GPU/sampler.py:216-388 with an NNBF model) in incremental
+ * form. */
+int vmcx_nnbf_forward_moved(const vmcx_nnbf_model *m, int64_t B, const int64_t *configs,
+                            const int64_t *proposed, const void *z1, void *workspace,
+                            int64_t ws_bytes, int64_t chunk, double *sign, double *logabs,
+                            void *z1_proposed, void *stream);
+
+/* dst rows (row_bytes each, a multiple of 4) replaced by src rows where
+ * mask[b] != 0: the walker state of the accepted proposals.
+ * Replaces: the accepted-row updates of a sampler step
+ * (GPU/sampler.py:353-366, `x[accept_mask] = proposed[accept_mask]`). */
+int vmcx_masked_rows(int64_t B, int64_t row_bytes, const uint8_t *mask, const void *src,
+                     void *dst, void *stream);
+
 /* Metropolis sweeps for NNBF walkers: per edge, all walkers propose (same
  * proposal table and draws as vmcx_slater_sweep), the dense layers and the
  * determinant are evaluated for the proposed configurations, and accepted

@@ -329,3 +329,77 @@ def test_nnbf_8x8_mixed_form_energy(device):
     print(f"  mixed form |dE/site| vs FP64 oracle over 1024 sampled walkers: {dE:.2e} "
           f"(per sample: mean {d.mean():.2e}, max {d.max():.2e})", flush=True)
     assert dE < 1e-4
+
+
+def _neighbour_swaps(x, Lx, Ly, rng):
+    """Each row with the contents of two neighbouring sites exchanged (a hop,
+    an up/down exchange, or no change when the two sites hold the same),
+    as the sampler's moves do."""
+    xp = x.copy()
+    for b in range(x.shape[0]):
+        s = int(rng.integers(Lx * Ly))
+        i, j = divmod(s, Ly)
+        nb = [(i + di) * Ly + (j + dj) for di, dj in ((1, 0), (0, 1), (-1, 0), (0, -1))
+              if 0 <= i + di < Lx and 0 <= j + dj < Ly]
+        t = nb[int(rng.integers(len(nb)))]
+        xp[b, s], xp[b, t] = x[b, t], x[b, s]
+    return xp
+
+
+MOVED_CASES = [("fp64", (4, 4, 5, 4, 32)), ("fp32", (6, 6, 16, 16, 64)),
+               ("bf16", (8, 8, 32, 32, 256)), ("bf16_fp32det", (8, 8, 32, 32, 256)),
+               ("bf16", (6, 6, 18, 17, 32))]
+
+
+@pytest.mark.parametrize("kind,case", MOVED_CASES)
+def test_nnbf_forward_moved(device, kind, case):
+    """Amplitudes of proposed configurations from the walkers' first-layer
+    pre-activations (forward_log_moved) against the full forward of the
+    proposed configurations.  FP64: ln|psi| within 1e-10, z1 within 1e-12;
+    FP32 network: z1 within 1e-5, ln|psi| within 1e-4; BF16 layers: the
+    first layer in FP32 differs in its last bits from the full sum, which
+    can move a BF16 rounding of h1 -- ln|psi| mean difference below 2e-5,
+    maximum below 2e-3 (the tolerance of the matrix-core layers against the
+    library GEMMs).  Signs identical; z1p = first_layer(proposed) as above;
+    rows without a change reproduce the walker's own amplitude."""
+    Lx, Ly, nu, nd, hid = case
+    gemm = "bf16" if kind.startswith("bf16") else kind
+    g = torch.Generator().manual_seed(3)
+    kw = dict(det="fp32") if kind == "bf16_fp32det" else {}
+    m = NNBF_GPU(Lx * Ly, nu + nd, hidden_dim=hid, activation="relu", bf_scale=0.05,
+                 gemm=gemm, init="xavier", m_std=0.3, generator=g, chunk=4096, **kw).to(device)
+    rng = np.random.default_rng(11)
+    x = random_configs(rng, 700, Lx * Ly, nu, nd)
+    xp = _neighbour_swaps(x, Lx, Ly, rng)
+    xd, xpd = torch.tensor(x, device=device), torch.tensor(xp, device=device)
+    s0, l0, z1 = m.first_layer(xd)
+    s_ref, l_ref = m.forward_log(xd)
+    assert torch.equal(s0, s_ref) and torch.equal(l0, l_ref)
+    sp, lp, z1p = m.forward_log_moved(xd, xpd, z1)
+    sf, lf, z1f = m.first_layer(xpd)
+    assert torch.equal(sp, sf)
+    dl = (lp - lf).abs().cpu().numpy()
+    dz = (z1p - z1f).abs().max().item()
+    same = np.all(x == xp, axis=1)
+    assert same.any() and (~same).any()
+    if kind == "fp64":
+        assert dl.max() < 1e-10 and dz < 1e-12
+    elif kind == "fp32":
+        assert dl.max() < 1e-4 and dz < 1e-5
+    else:
+        assert dz < 1e-5
+        assert dl.mean() < 2e-5 and dl.max() < 2e-3
+    # unchanged rows: the walker's own pre-activations and amplitude
+    sm = torch.tensor(same, device=device)
+    assert torch.equal(z1p[sm], z1[sm])
+    assert torch.equal(lp[sm], l0[sm]) and torch.equal(sp[sm], s0[sm])
+
+
+def test_masked_rows(device):
+    from vmc_torch_amd import kernels
+    src = torch.randn(37, 19, dtype=torch.float64, device=device)
+    dst = torch.randn(37, 19, dtype=torch.float64, device=device)
+    mask = (torch.arange(37, device=device) % 3 == 0).to(torch.uint8)
+    ref = torch.where(mask.bool()[:, None], src, dst)
+    kernels.masked_rows(mask, src, dst)
+    assert torch.equal(dst, ref)

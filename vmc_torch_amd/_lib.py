@@ -63,6 +63,8 @@ _SIGNATURES = {
     "vmcx_syrk_f64": (_I, [_I64, _I64, _P, _I64, _I, _D, _D, _P, _I64, _I, _I64, _P]),
     "vmcx_nnbf_workspace_bytes": (_I64, [_P, _I64, _I64, _I]),
     "vmcx_nnbf_forward": (_I, [_P, _I64, _P, _P, _I64, _I64, _P, _P, _P, _P]),
+    "vmcx_nnbf_forward_moved": (_I, [_P, _I64, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P]),
+    "vmcx_masked_rows": (_I, [_I64, _I64, _P, _P, _P, _P]),
     "vmcx_nnbf_sweep": (_I, [_P, _I64, _I, _P, _D, _I, _U64, _I64, _P, _P, _P, _P, _P,
                              _P, _P, _P, _P, _I64, _P]),
     "vmcx_nnbf_local_energy": (_I, [_P, _I64, _I, _P, _D, _D, _P, _P, _P, _P, _I64, _I64,

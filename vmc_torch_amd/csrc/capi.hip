@@ -60,7 +60,7 @@ static inline int status_of(int rc) {
 
 extern "C" {
 
-int vmcx_abi_version(void) { return 17; }
+int vmcx_abi_version(void) { return 18; }
 
 const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
 
@@ -258,6 +258,20 @@ int vmcx_nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *config
                       double *logabs, void *z1_out, void *stream) {
   return status3(vmcx::nnbf_forward(m, B, configs, workspace, ws_bytes, chunk, sign, logabs,
                                     z1_out, (hipStream_t)stream));
+}
+
+int vmcx_nnbf_forward_moved(const vmcx_nnbf_model *m, int64_t B, const int64_t *configs,
+                            const int64_t *proposed, const void *z1, void *workspace,
+                            int64_t ws_bytes, int64_t chunk, double *sign, double *logabs,
+                            void *z1_proposed, void *stream) {
+  return status3(vmcx::nnbf_forward_moved(m, B, configs, proposed, z1, workspace, ws_bytes,
+                                          chunk, sign, logabs, z1_proposed,
+                                          (hipStream_t)stream));
+}
+
+int vmcx_masked_rows(int64_t B, int64_t row_bytes, const uint8_t *mask, const void *src,
+                     void *dst, void *stream) {
+  return status3(vmcx::masked_rows(B, row_bytes, mask, src, dst, (hipStream_t)stream));
 }
 
 int vmcx_nnbf_sweep(const vmcx_nnbf_model *m, int64_t B, int n_edges,

@@ -196,6 +196,11 @@ int sym_images(int64_t B, int N, int G, const int64_t *cfg, const int32_t *perm,
                const int32_t *inv, int64_t *images, double *sign, hipStream_t st);
 
 int64_t nnbf_workspace_bytes(const vmcx_nnbf_model *m, int64_t B, int64_t chunk, int n_edges);
+int nnbf_forward_moved(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg,
+                       const int64_t *pcfg, const void *z1, void *ws, int64_t ws_bytes,
+                       int64_t chunk, double *sign, double *logabs, void *z1p, hipStream_t st);
+int masked_rows(int64_t B, int64_t row_bytes, const uint8_t *mask, const void *src, void *dst,
+                hipStream_t st);
 int nnbf_forward(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg, void *ws,
                  int64_t ws_bytes, int64_t chunk, double *sign, double *logabs, void *z1_out,
                  hipStream_t st);

@@ -715,6 +715,52 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// First layer of proposed configurations (one wave per configuration):
+// z1p[r] = z1[r] + sum over the orbitals o whose occupation differs between
+// cfg[r] and pcfg[r] of (+1 if o is occupied in pcfg, else -1) W1[:, o]
+// (orbitals in ascending order, up then down), h1[r] = act(z1p[r]).
+template <typename TW, typename TI>
+__global__ void __launch_bounds__(256)
+    nnbf_h1_moved_kernel(int64_t rows, int N, int H, const TW *__restrict__ W1T,
+                         const TW *__restrict__ z1, const int64_t *__restrict__ cfg,
+                         const int64_t *__restrict__ pcfg, int act, TW *z1p, TI *h1) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (r >= rows) return;
+  int64_t x = 0, y = 0;
+  if (lane < N) {
+    x = cfg[r * N + lane];
+    y = pcfg[r * N + lane];
+  }
+  const uint64_t dup = __ballot(((x ^ y) & 2) != 0), ddn = __ballot(((x ^ y) & 1) != 0);
+  const uint64_t nup = __ballot((y & 2) != 0), ndn = __ballot((y & 1) != 0);
+  for (int u = lane; u < H; u += 64) {
+    TW z = z1[r * H + u];
+    for (uint64_t m = dup; m; m &= m - 1) {
+      const int s = __ffsll((long long)m) - 1;
+      const TW w = W1T[(int64_t)s * H + u];
+      z = ((nup >> s) & 1ull) ? z + w : z - w;
+    }
+    for (uint64_t m = ddn; m; m &= m - 1) {
+      const int s = __ffsll((long long)m) - 1;
+      const TW w = W1T[(int64_t)(N + s) * H + u];
+      z = ((ndn >> s) & 1ull) ? z + w : z - w;
+    }
+    z1p[r * H + u] = z;
+    h1[r * H + u] = to_t<TI>(act_f(z, act));
+  }
+}
+
+// rows of dst (row_words 32-bit words each) replaced by those of src where
+// mask[r] != 0
+__global__ void masked_rows_kernel(int64_t B, int64_t row_words, const uint8_t *__restrict__ mask,
+                                   const uint32_t *__restrict__ src, uint32_t *dst) {
+  const int64_t n = B * row_words;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x)
+    if (mask[idx / row_words]) dst[idx] = src[idx];
+}
+
 // determinant of the proposed configuration, Metropolis test, commit
 template <int NFP, typename TW>
 __global__ void __launch_bounds__(256)
@@ -1383,6 +1429,71 @@ static int conn_layer1(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t r0, in
   hipLaunchKernelGGL((nnbf_h1_conn_kernel<TW, TI>), dim3(g), dim3(256), 0, st, rows, m.hidden,
                      (const TW *)L.w1t, (const TW *)L.z1w, L.cparent + r0, L.cmove + r0, m.act,
                      (TI *)L.h1);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+template <typename TW, typename TI>
+static int moved_layer1(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t rows,
+                        const int64_t *cfg, const int64_t *pcfg, const void *z1, void *z1p,
+                        hipStream_t st) {
+  hipLaunchKernelGGL((nnbf_h1_moved_kernel<TW, TI>), dim3((unsigned)((rows + 3) / 4)), dim3(256),
+                     0, st, rows, m.n_sites, m.hidden, (const TW *)L.w1t, (const TW *)z1, cfg,
+                     pcfg, m.act, (TW *)z1p, (TI *)L.h1);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+int nnbf_forward_moved(const vmcx_nnbf_model *m, int64_t B, const int64_t *cfg,
+                       const int64_t *pcfg, const void *z1, void *ws, int64_t ws_bytes,
+                       int64_t chunk, double *sign, double *logabs, void *z1p, hipStream_t st) {
+  if (check_model(m)) return 1;
+  VMCX_CHECK_ARG(B >= 0 && chunk >= 1, "bad sizes");
+  if (B == 0) return 0;
+  VMCX_CHECK_ARG(cfg && pcfg && z1 && z1p && sign && logabs && ws, "null pointer");
+  const NnbfWs L = nnbf_layout(*m, B, chunk, 0, (char *)ws);
+  VMCX_CHECK_ARG((int64_t)L.bytes <= ws_bytes, "workspace too small");
+  const int H = m->hidden, n_orb = 2 * m->n_sites, N = m->n_sites;
+  const size_t tw = m->gemm == 2 ? 8 : 4;
+  {
+    const int64_t n = (int64_t)H * n_orb;
+    if (m->gemm == 2)
+      hipLaunchKernelGGL((nnbf_w1t_kernel<double>), dim3((unsigned)((n + 255) / 256)), dim3(256),
+                         0, st, H, n_orb, weights_of<double>(*m).W1, (double *)L.w1t);
+    else
+      hipLaunchKernelGGL((nnbf_w1t_kernel<float>), dim3((unsigned)((n + 255) / 256)), dim3(256),
+                         0, st, H, n_orb, weights_of<float>(*m).W1, (float *)L.w1t);
+    VMCX_LAUNCHED();
+  }
+  for (int64_t r0 = 0; r0 < B; r0 += chunk) {
+    const int64_t rows = (B - r0 < chunk) ? B - r0 : chunk;
+    const char *zi = (const char *)z1 + (size_t)r0 * H * tw;
+    char *zo = (char *)z1p + (size_t)r0 * H * tw;
+    int rc;
+    if (m->gemm == 2)
+      rc = moved_layer1<double, double>(*m, L, rows, cfg + r0 * N, pcfg + r0 * N, zi, zo, st);
+    else if (m->gemm == 1)
+      rc = moved_layer1<float, __hip_bfloat16>(*m, L, rows, cfg + r0 * N, pcfg + r0 * N, zi, zo,
+                                               st);
+    else
+      rc = moved_layer1<float, float>(*m, L, rows, cfg + r0 * N, pcfg + r0 * N, zi, zo, st);
+    if (rc) return rc;
+    rc = forward_rows(*m, L, rows, pcfg + r0 * N, sign + r0, logabs + r0, nullptr, st, true);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int masked_rows(int64_t B, int64_t row_bytes, const uint8_t *mask, const void *src, void *dst,
+                hipStream_t st) {
+  VMCX_CHECK_ARG(B >= 0 && row_bytes >= 0 && row_bytes % 4 == 0,
+                 "row_bytes must be a multiple of 4");
+  if (B == 0 || row_bytes == 0) return 0;
+  VMCX_CHECK_ARG(mask && src && dst, "null pointer");
+  const int64_t n = B * (row_bytes / 4);
+  const unsigned g = (unsigned)((n + 255) / 256 < 16384 ? (n + 255) / 256 : 16384);
+  hipLaunchKernelGGL(masked_rows_kernel, dim3(g), dim3(256), 0, st, B, row_bytes / 4, mask,
+                     (const uint32_t *)src, (uint32_t *)dst);
   VMCX_LAUNCHED();
   return 0;
 }

@@ -99,8 +99,11 @@ class LearnedProposalGraph:
     instances -- the lattices of an uneven batch, each with its own model and
     walker batch and possibly one shared net -- captured as ONE graph: per
     instance and step the net's proposal (vmcx_lp_propose_given), the
-    model's amplitudes of the proposed configurations (its libvmcx forward)
-    and the Metropolis-Hastings test (vmcx_lp_accept, vmcx_lp_commit_logits).
+    model's amplitudes of the proposed configurations (its libvmcx forward;
+    for NNBF the moved-configuration forward from the walkers' first-layer
+    state, which each replay first recomputes from the configurations, as
+    an eager sweeps() call does) and the Metropolis-Hastings test
+    (vmcx_lp_accept, vmcx_lp_commit_logits).
     steps: proposals per replay for each instance (an int or one per
     instance).  The proposal counter of each instance is device-resident;
     replay k of instance i draws what LearnedProposalSamplerGPU.sweeps
@@ -131,27 +134,39 @@ class LearnedProposalGraph:
                      n_acc=torch.zeros(B, dtype=torch.int64, device=dev),
                      lx=torch.empty(B, M, dtype=torch.float64, device=dev),
                      acc=torch.empty(B, dtype=torch.uint8, device=dev),
+                     moved=hasattr(model, "forward_log_moved"),
                      bufs=(torch.empty(B, dtype=torch.int32, device=dev),
                            torch.empty(B, dtype=torch.float64, device=dev),
                            torch.empty(B, N, dtype=torch.int64, device=dev),
                            torch.empty(B, dtype=torch.float64, device=dev),
                            torch.empty(B, M, dtype=torch.float64, device=dev)))
+            if I["moved"]:
+                # first-layer state of the walkers and of the proposals
+                I["z1"] = torch.empty(B, model.hidden, dtype=model.z_dtype, device=dev)
+                I["mout"] = (torch.empty(B, dtype=torch.float64, device=dev),
+                             torch.empty(B, dtype=torch.float64, device=dev),
+                             torch.empty(B, model.hidden, dtype=model.z_dtype, device=dev))
             self.inst.append(I)
         self.device = self.inst[0]["fxs"].device
         self._warm()
         self.update()
 
-    def _steps(self, I, fxs, sign, logabs, lx, n_acc, counter):
+    def _steps(self, I, fxs, sign, logabs, lx, n_acc, counter, z1=None):
         s, m = I["sampler"], I["model"]
         move, lqf, prop, lqr, lp = I["bufs"]
         for _ in range(I["k"]):
             kernels.lp_propose_given(s.params, s.C, s.Lx, s.Ly, fxs, lx, s.seed, 0,
                                      precision=s.precision, step_dev=counter,
                                      out=(move, lqf, prop, lqr, lp))
-            ps, pl = m.forward_log(prop)
+            if I["moved"]:
+                ps, pl, z1p = m.forward_log_moved(fxs, prop, z1, out=I["mout"])
+            else:
+                ps, pl = m.forward_log(prop)
             kernels.lp_accept(fxs, sign, logabs, move, prop, ps, pl, lqf, lqr, s.seed, 0,
                               n_accept=n_acc, step_dev=counter, accepted=I["acc"])
             kernels.lp_commit_logits(I["acc"], lp, lx)
+            if I["moved"]:
+                kernels.masked_rows(I["acc"], z1p, z1)
             kernels.counter_add(counter, 1)
 
     def _warm(self):
@@ -159,10 +174,14 @@ class LearnedProposalGraph:
             m, s = I["model"], I["sampler"]
             x = I["fxs"].clone()
             with m.frozen():
-                sg, la = m.forward_log(x)
+                z1 = None
+                if I["moved"]:
+                    sg, la, z1 = m.first_layer(x)
+                else:
+                    sg, la = m.forward_log(x)
                 lx = kernels.lp_logits(s.params, s.C, s.Lx, s.Ly, x, precision=s.precision)
                 self._steps(I, x, sg.clone(), la.clone(), lx, torch.zeros_like(I["n_acc"]),
-                            I["counter"].clone())
+                            I["counter"].clone(), z1)
         torch.cuda.synchronize(self.device)
 
     def update(self):
@@ -178,8 +197,13 @@ class LearnedProposalGraph:
 
         def body():
             for I in self.inst:
+                if I["moved"]:
+                    # the first-layer state (and the amplitudes) of the walkers
+                    # from their configurations, as every eager sweeps() call
+                    # starts
+                    I["model"].first_layer(I["fxs"], out=(I["sign"], I["logabs"], I["z1"]))
                 self._steps(I, I["fxs"], I["sign"], I["logabs"], I["lx"], I["n_acc"],
-                            I["counter"])
+                            I["counter"], I.get("z1"))
         for I in self.inst:
             I["model"]._frozen = True
         try:

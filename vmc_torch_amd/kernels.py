@@ -661,6 +661,20 @@ def lp_commit_logits(accepted, logits_p, logits_x):
                                       stream_ptr(logits_x.device)))
 
 
+def masked_rows(mask, src, dst):
+    """dst[b] = src[b] for the rows b with mask[b] != 0 (uint8 mask [B];
+    src and dst of one shape and dtype, rows contiguous)."""
+    B = dst.shape[0]
+    require(mask, U8, "mask", (B,), dst.device)
+    if src.shape != dst.shape or src.dtype != dst.dtype or src.device != dst.device:
+        raise ValueError("masked_rows: src and dst differ in shape, dtype or device")
+    if not (src.is_contiguous() and dst.is_contiguous()):
+        raise ValueError("masked_rows: src and dst must be contiguous")
+    row_bytes = (dst.numel() // max(B, 1)) * dst.element_size()
+    check(lib().vmcx_masked_rows(B, row_bytes, ptr(mask), ptr(src), ptr(dst),
+                                 stream_ptr(dst.device)))
+
+
 def lp_accept(configs, sign, logabs, move, proposed, psign, plogabs, logq_f, logq_r, seed, step,
               u_acc=None, n_accept=None, step_dev=None, accepted=None):
     B, N = _check_configs(configs, configs.device)

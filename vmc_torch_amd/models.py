@@ -323,6 +323,53 @@ class NNBF_GPU(WavefunctionModel_GPU):
             chunk, kernels.ptr(sign), kernels.ptr(logabs), None, kernels.stream_ptr(x.device)))
         return sign, logabs
 
+    def first_layer(self, x, out=None):
+        """(sign, ln|psi|, z1) of configurations x, z1 = W1 n(x) + b1 ([B][hidden],
+        z_dtype): the state forward_log_moved starts from.  out: optional
+        (sign, logabs, z1) buffers."""
+        x = x.contiguous()
+        kernels.require(x, torch.int64, "configs")
+        self._sync_weights()
+        B = x.shape[0]
+        if out is None:
+            out = (torch.empty(B, dtype=torch.float64, device=x.device),
+                   torch.empty(B, dtype=torch.float64, device=x.device),
+                   torch.empty(B, self.hidden, dtype=self.z_dtype, device=x.device))
+        sign, logabs, z1 = out
+        chunk = min(self.chunk, max(B, 1))
+        ws = self._ws_for(B, chunk, 0, x.device)
+        kernels.check(kernels.lib().vmcx_nnbf_forward(
+            ctypes.byref(self._struct_obj), B, kernels.ptr(x), kernels.ptr(ws), ws.numel(),
+            chunk, kernels.ptr(sign), kernels.ptr(logabs), kernels.ptr(z1),
+            kernels.stream_ptr(x.device)))
+        return sign, logabs, z1
+
+    def forward_log_moved(self, x, xp, z1, out=None):
+        """(sign, ln|psi|, z1p) of proposed configurations xp, row b differing
+        from x[b] by a few moved electrons: the first layer is z1[b] (x[b]'s
+        pre-activations, first_layer) updated by the W1 columns of the
+        orbitals whose occupation changed, the rest of the forward as
+        forward_log.  z1p: xp's pre-activations (the walker state after an
+        accepted move).  out: optional (sign, logabs, z1p) buffers."""
+        x, xp = x.contiguous(), xp.contiguous()
+        kernels.require(x, torch.int64, "configs")
+        kernels.require(xp, torch.int64, "proposed", tuple(x.shape), x.device)
+        kernels.require(z1, self.z_dtype, "z1", (x.shape[0], self.hidden), x.device)
+        self._sync_weights()
+        B = x.shape[0]
+        if out is None:
+            out = (torch.empty(B, dtype=torch.float64, device=x.device),
+                   torch.empty(B, dtype=torch.float64, device=x.device),
+                   torch.empty(B, self.hidden, dtype=self.z_dtype, device=x.device))
+        sign, logabs, z1p = out
+        chunk = min(self.chunk, max(B, 1))
+        ws = self._ws_for(B, chunk, 0, x.device)
+        kernels.check(kernels.lib().vmcx_nnbf_forward_moved(
+            ctypes.byref(self._struct_obj), B, kernels.ptr(x), kernels.ptr(xp), kernels.ptr(z1),
+            kernels.ptr(ws), ws.numel(), chunk, kernels.ptr(sign), kernels.ptr(logabs),
+            kernels.ptr(z1p), kernels.stream_ptr(x.device)))
+        return sign, logabs, z1p
+
     # --------------------------------------------------------- walker state
     def walker_state(self, fxs, refresh=False):
         changed = self._sync_weights()
@@ -503,6 +550,53 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
             ctypes.byref(self._struct_obj), B, kernels.ptr(x), kernels.ptr(ws), ws.numel(), chunk,
             kernels.ptr(sign), kernels.ptr(logabs), kernels.stream_ptr(x.device)))
         return sign, logabs
+
+    def first_layer(self, x, out=None):
+        """(sign, ln|psi|, z1) of configurations x, z1 = W1 n(x) + b1 ([B][hidden],
+        z_dtype): the state forward_log_moved starts from.  out: optional
+        (sign, logabs, z1) buffers."""
+        x = x.contiguous()
+        kernels.require(x, torch.int64, "configs")
+        self._sync_weights()
+        B = x.shape[0]
+        if out is None:
+            out = (torch.empty(B, dtype=torch.float64, device=x.device),
+                   torch.empty(B, dtype=torch.float64, device=x.device),
+                   torch.empty(B, self.hidden, dtype=self.z_dtype, device=x.device))
+        sign, logabs, z1 = out
+        chunk = min(self.chunk, max(B, 1))
+        ws = self._ws_for(B, chunk, 0, x.device)
+        kernels.check(kernels.lib().vmcx_nnbf_forward(
+            ctypes.byref(self._struct_obj), B, kernels.ptr(x), kernels.ptr(ws), ws.numel(),
+            chunk, kernels.ptr(sign), kernels.ptr(logabs), kernels.ptr(z1),
+            kernels.stream_ptr(x.device)))
+        return sign, logabs, z1
+
+    def forward_log_moved(self, x, xp, z1, out=None):
+        """(sign, ln|psi|, z1p) of proposed configurations xp, row b differing
+        from x[b] by a few moved electrons: the first layer is z1[b] (x[b]'s
+        pre-activations, first_layer) updated by the W1 columns of the
+        orbitals whose occupation changed, the rest of the forward as
+        forward_log.  z1p: xp's pre-activations (the walker state after an
+        accepted move).  out: optional (sign, logabs, z1p) buffers."""
+        x, xp = x.contiguous(), xp.contiguous()
+        kernels.require(x, torch.int64, "configs")
+        kernels.require(xp, torch.int64, "proposed", tuple(x.shape), x.device)
+        kernels.require(z1, self.z_dtype, "z1", (x.shape[0], self.hidden), x.device)
+        self._sync_weights()
+        B = x.shape[0]
+        if out is None:
+            out = (torch.empty(B, dtype=torch.float64, device=x.device),
+                   torch.empty(B, dtype=torch.float64, device=x.device),
+                   torch.empty(B, self.hidden, dtype=self.z_dtype, device=x.device))
+        sign, logabs, z1p = out
+        chunk = min(self.chunk, max(B, 1))
+        ws = self._ws_for(B, chunk, 0, x.device)
+        kernels.check(kernels.lib().vmcx_nnbf_forward_moved(
+            ctypes.byref(self._struct_obj), B, kernels.ptr(x), kernels.ptr(xp), kernels.ptr(z1),
+            kernels.ptr(ws), ws.numel(), chunk, kernels.ptr(sign), kernels.ptr(logabs),
+            kernels.ptr(z1p), kernels.stream_ptr(x.device)))
+        return sign, logabs, z1p
 
     # --------------------------------------------------------- walker state
     def walker_state(self, fxs, refresh=False):

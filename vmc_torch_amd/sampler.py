@@ -160,9 +160,16 @@ class LearnedProposalSamplerGPU(SamplerGPU):
         # on entry instead of a value comparison (a host synchronisation) per
         # proposal
         frozen = model.frozen() if hasattr(model, "frozen") else contextlib.nullcontext()
+        # models with a first-layer state (NNBF) evaluate a proposed
+        # configuration from its walker's pre-activations and the moved
+        # electrons' weight columns
+        moved = hasattr(model, "forward_log_moved")
         with frozen:
-            sign, logabs = model.forward_log(fxs)
-            sign, logabs = sign.clone(), logabs.clone()
+            if moved:
+                sign, logabs, z1 = model.first_layer(fxs)
+            else:
+                sign, logabs = model.forward_log(fxs)
+                sign, logabs = sign.clone(), logabs.clone()
             n_acc = torch.zeros(fxs.shape[0], dtype=torch.int64, device=fxs.device)
             # logits at the current walkers: evaluated once, then carried over
             # from the proposed configurations of the accepted moves
@@ -175,10 +182,15 @@ class LearnedProposalSamplerGPU(SamplerGPU):
                 move, lqf, prop, lqr, lp = self.k.lp_propose_given(
                     self.params, self.C, self.Lx, self.Ly, fxs, lx, self.seed, self.step_counter,
                     uc, precision=self.precision)
-                ps, pl = model.forward_log(prop)
+                if moved:
+                    ps, pl, z1p = model.forward_log_moved(fxs, prop, z1)
+                else:
+                    ps, pl = model.forward_log(prop)
                 acc = self.k.lp_accept(fxs, sign, logabs, move, prop, ps, pl, lqf, lqr,
                                        self.seed, self.step_counter, ua, n_acc)
                 self.k.lp_commit_logits(acc, lp, lx)
+                if moved:
+                    self.k.masked_rows(acc, z1p, z1)
                 self.step_counter += 1
         if hasattr(model, "invalidate"):
             model.invalidate()      # the model's cached walker state predates the moves
