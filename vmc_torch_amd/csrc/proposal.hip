@@ -55,14 +55,16 @@ struct LpSh {
   int x[LP_NMAX];
 };
 
-__device__ void lp_edges(int Lx, int Ly, int *ei, int *ej) {
-  if (threadIdx.x == 0) {
-    int e = 0;
-    for (int s = 0; s < Lx * Ly; ++s) {
-      const int xx = s / Ly, yy = s - xx * Ly;
-      if (yy + 1 < Ly) { ei[e] = s; ej[e] = s + 1; ++e; }
-      if (xx + 1 < Lx) { ei[e] = s; ej[e] = s + Ly; ++e; }
-    }
+// the sorted edge list built by all threads of the block: site s = x Ly + y owns
+// the edges (s, s + 1) (y + 1 < Ly) then (s, s + Ly) (x + 1 < Lx), which
+// start at x (2 Ly - 1) + y (1 + [x + 1 < Lx]) in the sorted order
+__device__ void lp_edges_par(int Lx, int Ly, int *ei, int *ej) {
+  for (int s = threadIdx.x; s < Lx * Ly; s += blockDim.x) {
+    const int x = s / Ly, y = s - x * Ly;
+    const bool down = x + 1 < Lx;
+    int e = x * (2 * Ly - 1) + y * (down ? 2 : 1);
+    if (y + 1 < Ly) { ei[e] = s; ej[e] = s + 1; ++e; }
+    if (down) { ei[e] = s; ej[e] = s + Ly; }
   }
 }
 
@@ -104,7 +106,7 @@ __device__ void lp_net(const float *P, int C, int Lx, int Ly, const int64_t *cfg
     S.h[0][s] = (float)((v >> 1) & 1);
     S.h[0][N + s] = (float)(v & 1);
   }
-  lp_edges(Lx, Ly, S.ei, S.ej);
+  lp_edges_par(Lx, Ly, S.ei, S.ej);
   __syncthreads();
   lp_conv(Lx, Ly, 2, C, P + o.W1, P + o.b1, S.h[0], S.z[0], S.h[1]);
   lp_conv(Lx, Ly, C, C, P + o.W2, P + o.b2, S.h[1], S.z[1], S.h[2]);
@@ -170,7 +172,7 @@ __global__ void __launch_bounds__(LP_T)
     hb0[c * R + gg * (N + 1) + N] = 0.0f;
     hb1[c * R + gg * (N + 1) + N] = 0.0f;
   }
-  lp_edges(Lx, Ly, ei, ej);
+  lp_edges_par(Lx, Ly, ei, ej);
   int off[9];
   {
     const int x = s / Ly, y = s - (s / Ly) * Ly, base = g * (N + 1);
@@ -245,7 +247,8 @@ __global__ void __launch_bounds__(LP_T)
 // channels a lane contributes to a chunk are one 16-byte LDS read; the
 // layer's weights are staged BF16 in LDS in fragment order (chunk, lane, 8).
 // Bias and ReLU in FP32, the result rounded to BF16; the head accumulates in
-// FP64 from the BF16 activations.
+// FP64 from the BF16 activations, as two per-site projections per move type
+// (A[t, :C] . h3[i] and A[t, C:] . h3[j], the oracle's grouping).
 typedef __bf16 lp_bf8 __attribute__((ext_vector_type(8)));
 typedef float lp_f16v __attribute__((ext_vector_type(16)));
 
@@ -265,12 +268,34 @@ __global__ void __launch_bounds__(LPM_T)
     lp_logits_mfma_kernel(const float *__restrict__ P, int Lx, int Ly, int G, int64_t B,
                           const int64_t *__restrict__ cfg, double *__restrict__ logits) {
   __shared__ __attribute__((aligned(16))) __bf16 act[2][LPM_RMAX * LPM_CH];
-  __shared__ __attribute__((aligned(16))) __bf16 wf[LPM_KCH * 64 * 8];
+  __shared__ __attribute__((aligned(16))) __bf16 wf[2][LPM_KCH * 64 * 8];
   __shared__ int xs[LPM_RMAX];
   __shared__ int ei[LP_EMAX], ej[LP_EMAX];
+  __shared__ float w1s[LPM_CH * 2 * 9 + LPM_CH];       // layer 1 weights, then biases
+  static_assert(sizeof(double) * LPM_RMAX * 6 <= sizeof(__bf16) * LPM_RMAX * LPM_CH,
+                "the head projections alias a layer buffer");
+  // the head projections per slot, in act[1] once layer 3 has read it
+  double *proj = reinterpret_cast<double *>(act[1]);
   const int N = Lx * Ly, E = lp_n_edges(Lx, Ly), R = G * (N + 1), ncol = G * N;
   const int64_t b0 = (int64_t)blockIdx.x * G;
   const int t = threadIdx.x;
+  const LpOff o = lp_offsets(LPM_CH);
+  for (int idx = t; idx < LPM_CH * 2 * 9 + LPM_CH; idx += LPM_T)
+    w1s[idx] = idx < LPM_CH * 2 * 9 ? P[o.W1 + idx] : P[o.b1 + idx - LPM_CH * 2 * 9];
+  // layers 2 and 3 weights as BF16 matrix-core fragments, one 16-byte
+  // fragment per step: chunk = tap * 2 + (ci >> 4), lane ln = co + 32
+  // ((ci & 15) >> 3), the 8 input channels ci & 7
+  for (int f = t; f < 2 * LPM_KCH * 64; f += LPM_T) {
+    const int l = f / (LPM_KCH * 64), ff = f - l * (LPM_KCH * 64);
+    const float *W = P + (l == 0 ? o.W2 : o.W3);
+    const int chunk = ff >> 6, ln = ff & 63;
+    const int co = ln & 31, tap = chunk >> 1;
+    const int ci0 = 16 * (chunk & 1) + 8 * (ln >> 5);
+    lp_bf8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (__bf16)W[(co * LPM_CH + ci0 + e) * 9 + tap];
+    *reinterpret_cast<lp_bf8 *>(&wf[l][ff * 8]) = v;
+  }
   for (int idx = t; idx < R; idx += LPM_T) {
     const int gg = idx / (N + 1), ss = idx - gg * (N + 1);
     xs[idx] = (ss < N && b0 + gg < B) ? (int)cfg[(b0 + gg) * N + ss] : 0;
@@ -280,12 +305,12 @@ __global__ void __launch_bounds__(LPM_T)
     act[0][(gg * (N + 1) + N) * LPM_CH + c] = (__bf16)0.0f;
     act[1][(gg * (N + 1) + N) * LPM_CH + c] = (__bf16)0.0f;
   }
-  lp_edges(Lx, Ly, ei, ej);
+  lp_edges_par(Lx, Ly, ei, ej);
   __syncthreads();
-  const LpOff o = lp_offsets(LPM_CH);
-  // layer 1, FP32: thread = (column, half of the output channels)
+  // layer 1, FP32: thread = (column, half of the output channels), weights
+  // from LDS
   {
-    const float *W1 = P + o.W1, *b1 = P + o.b1;
+    const float *W1 = w1s, *b1 = w1s + LPM_CH * 2 * 9;
     for (int idx = t; idx < 2 * ncol; idx += LPM_T) {
       const int col = idx >> 1, half = idx & 1;
       const int g = col / N, s = col - g * N, x = s / Ly, y = s - (s / Ly) * Ly;
@@ -312,17 +337,11 @@ __global__ void __launch_bounds__(LPM_T)
   }
   const int wave = t >> 6, lane = t & 63, kh = lane >> 5;
   for (int layer = 1; layer < 3; ++layer) {
-    const float *W = P + (layer == 1 ? o.W2 : o.W3), *bias = P + (layer == 1 ? o.b2 : o.b3);
+    const float *bias = P + (layer == 1 ? o.b2 : o.b3);
     const __bf16 *in = act[layer == 1 ? 0 : 1];
     __bf16 *outb = act[layer == 1 ? 1 : 0];
-    __syncthreads();   // the previous layer's reads of wf and writes of `in` are done
-    for (int idx = t; idx < LPM_CH * LPM_CH * 9; idx += LPM_T) {
-      const int co = idx / (LPM_CH * 9), rem = idx - co * LPM_CH * 9;
-      const int ci = rem / 9, tap = rem - ci * 9;
-      const int chunk = tap * 2 + (ci >> 4), ln = co + 32 * ((ci & 15) >> 3);
-      wf[(chunk * 64 + ln) * 8 + (ci & 7)] = (__bf16)W[idx];
-    }
-    __syncthreads();
+    const __bf16 *wl = wf[layer - 1];
+    __syncthreads();   // the previous layer's writes of `in` are done
     for (int tile = wave; tile * 32 < ncol; tile += LPM_T / 64) {
       const int col = tile * 32 + (lane & 31);
       const bool ok = col < ncol;
@@ -341,7 +360,7 @@ __global__ void __launch_bounds__(LPM_T)
       for (int q = 0; q < 9; ++q)
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          const lp_bf8 a = *reinterpret_cast<const lp_bf8 *>(&wf[((q * 2 + h) * 64 + lane) * 8]);
+          const lp_bf8 a = *reinterpret_cast<const lp_bf8 *>(&wl[((q * 2 + h) * 64 + lane) * 8]);
           const lp_bf8 bv =
               *reinterpret_cast<const lp_bf8 *>(&in[nb[q] * LPM_CH + h * 16 + kh * 8]);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc, 0, 0, 0);
@@ -359,21 +378,30 @@ __global__ void __launch_bounds__(LPM_T)
     }
   }
   __syncthreads();
+  // head in FP64: per site and move type the two projections
+  // proj[slot][2 tt + side] = A[tt, side C .. side C + C - 1] . h3[slot], then
+  // logit(e, tt) = (bA[tt] + proj[i][2 tt]) + proj[j][2 tt + 1]
   const float *A = P + o.A, *bA = P + o.bA;
   const __bf16 *h3 = act[0];
+  for (int idx = t; idx < ncol * 6; idx += LPM_T) {
+    const int col = idx / 6, k = idx - col * 6;
+    const int g = col / N, slot = col + g;    // g (N + 1) + s
+    const float *a = A + (k >> 1) * 2 * LPM_CH + (k & 1) * LPM_CH;
+    const __bf16 *hs = h3 + slot * LPM_CH;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int c = 0; c < LPM_CH; ++c) acc += (double)a[c] * (double)(float)hs[c];
+    proj[slot * 6 + k] = acc;
+  }
+  __syncthreads();
   for (int idx = t; idx < G * 3 * E; idx += LPM_T) {
     const int gg = idx / (3 * E), m = idx - gg * 3 * E;
     if (b0 + gg >= B) break;
     const int e = m / 3, tt = m - 3 * e, base = gg * (N + 1);
     const int i = base + ei[e], j = base + ej[e];
     double v = -INFINITY;
-    if (lp_valid(tt, xs[i], xs[j])) {
-      double acc = bA[tt];
-      for (int c = 0; c < LPM_CH; ++c)
-        acc += (double)A[tt * 2 * LPM_CH + c] * (double)(float)h3[i * LPM_CH + c] +
-               (double)A[tt * 2 * LPM_CH + LPM_CH + c] * (double)(float)h3[j * LPM_CH + c];
-      v = acc;
-    }
+    if (lp_valid(tt, xs[i], xs[j]))
+      v = ((double)bA[tt] + proj[i * 6 + 2 * tt]) + proj[j * 6 + 2 * tt + 1];
     logits[(b0 + gg) * 3 * E + m] = v;
   }
 }
