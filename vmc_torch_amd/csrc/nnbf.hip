@@ -1349,7 +1349,9 @@ static int forward_rows_mfma(const vmcx_nnbf_model &m, const NnbfWs &L, int64_t 
   int rc = h1_ready ? 0 : layer1<float, __hip_bfloat16>(m, L, rows, cfg, st);
   if (rc) return rc;
   const int H = m.hidden, nD = 2 * m.n_sites * m.n_f;
-  rc = launch_ng<NG_ACT>(rows, H, H, m.W2_bf16, L.h1, m.b2, m.act, (float *)L.z2, L.h2, nullptr,
+  // (the pre-activations z2 are not kept: only the backward reads them, and
+  // it runs its own forward)
+  rc = launch_ng<NG_ACT>(rows, H, H, m.W2_bf16, L.h1, m.b2, m.act, nullptr, L.h2, nullptr,
                          m.n_sites, m.n_f, nullptr, st);
   if (rc) return rc;
   float *A = (float *)L.D;   // [rows][nf][nf] fits the [rows][2N nf] buffer
