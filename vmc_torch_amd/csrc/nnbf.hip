@@ -275,18 +275,29 @@ __device__ __forceinline__ void ng_epilogue(int n0, const ng_f16v &acc0, const n
                                             float *z, __hip_bfloat16 *hout,
                                             const uint8_t *slot, int nf, float *A) {
   const int kh = lane >> 5;
+  // the values are formed before any conditional store (a load first used
+  // inside a branch makes the compiler wait for every outstanding memory
+  // operation, earlier stores included, at each branch); the empty
+  // statements keep them there
+  float vs[2][16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    vs[0][q] = acc0[q] + bv[q];
+    vs[1][q] = acc1[q] + bv[q];
+    asm volatile("" : "+v"(vs[0][q]), "+v"(vs[1][q]));
+  }
 #pragma unroll
   for (int half = 0; half < 2; ++half) {
     const int rl = half * 32 + (lane & 31);
     const int64_t r = r0 + rl;
     if (r >= R) continue;
-    const ng_f16v &acc = half ? acc1 : acc0;
+    const float *vh = vs[half];
     if (EPI == NG_ACT) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int n = n0 + 8 * (q >> 2) + 4 * kh + (q & 3);
         if (n >= n_rows) continue;
-        const float v = acc[q] + bv[q];
+        const float v = vh[q];
         if (z) z[r * n_rows + n] = v;
         hout[r * n_rows + n] = __float2bfloat16(act == 0 ? fmaxf(v, 0.0f) : tanhf(v));
       }
@@ -306,10 +317,10 @@ __device__ __forceinline__ void ng_epilogue(int n0, const ng_f16v &acc0, const n
         }
         if ((nf & 3) == 0) {
           float4 v;
-          v.x = acc[4 * g + 0] + bv[4 * g + 0];
-          v.y = acc[4 * g + 1] + bv[4 * g + 1];
-          v.z = acc[4 * g + 2] + bv[4 * g + 2];
-          v.w = acc[4 * g + 3] + bv[4 * g + 3];
+          v.x = vh[4 * g + 0];
+          v.y = vh[4 * g + 1];
+          v.z = vh[4 * g + 2];
+          v.w = vh[4 * g + 3];
           *reinterpret_cast<float4 *>(Ar + (int64_t)sl * nf + j) = v;
         } else {
           for (int e = 0; e < 4; ++e) {
@@ -318,7 +329,7 @@ __device__ __forceinline__ void ng_epilogue(int n0, const ng_f16v &acc0, const n
             const int oo = n / nf, jj = n - oo * nf;
             const int ss = srow_p[oo];
             if (ss == 0xFF) continue;
-            Ar[(int64_t)ss * nf + jj] = acc[4 * g + e] + bv[4 * g + e];
+            Ar[(int64_t)ss * nf + jj] = vh[4 * g + e];
           }
         }
       }
@@ -429,33 +440,81 @@ __global__ void __launch_bounds__(NG_T)
   }
 }
 
-// The same layers for input width 256 with the operands swapped in roles:
-// a wave keeps the B fragments of its 64 configurations (h rows, all 16
-// k-steps: 128 registers) for the whole launch, and the workgroup's four
-// waves stream W's 32-row tiles through LDS, double-buffered: while a tile
-// is multiplied, the next one (16 KiB, contiguous in the row-major W) is
-// loaded by all 256 threads into registers and written to the other buffer
-// after the tile's epilogue.  Each W fragment read from LDS feeds two MFMAs
-// (the wave's two configuration tiles), each W byte fetched from L2 serves
-// 256 configurations.  W tile rows are padded to 264 BF16 in LDS (16-byte
-// fragment reads of 16 lanes fall in distinct banks).  Epilogues as
-// nnbf_gemm_mfma_kernel; grid: (configurations / 256, slices of W's tiles).
+// The output layer for input width 256 (NG_SCAT only), organised around
+// the configurations: a wave keeps the fragments of its 64 configurations
+// (h rows, all 16 k-steps: 128 registers) for the whole launch, and the
+// workgroup's four waves stream W's 32-row tiles through LDS,
+// double-buffered: while a tile is multiplied, the next one (16 KiB,
+// contiguous in the row-major W) is loaded by all 256 threads into
+// registers and written to the other buffer after the tile's epilogue.
+// Each W fragment read from LDS feeds two MFMAs (the wave's two
+// configuration tiles), each W byte fetched from L2 serves 256
+// configurations.  W tile rows are padded to 264 BF16 in LDS (16-byte
+// fragment reads of 16 lanes fall in distinct banks).
+// The configurations are the MFMA's A operand and W^T its B operand, so
+// that result register q of lane l holds configuration 8 (q / 4) + 4 (l /
+// 32) + q % 4 of the tile at output row n0 + l % 32: a store instruction
+// writes 32 consecutive outputs (one orbital row segment, 128 bytes when
+// nf is a multiple of 32) of two configurations.  The slot table is
+// orbital-major (slotT[o][configuration], rows of NG2_SLD bytes: the byte
+// writes of its construction fall in distinct banks, and the four slots
+// of configurations c .. c + 3 are one 32-bit read).
+// Grid: (configurations / 256, slices of W's tiles).
 constexpr int NG2_T = 256, NG2_CPW = 64, NG2_CPB = NG2_CPW * (NG2_T / 64);
-constexpr int NG2_K = 256, NG2_LDW = NG2_K + 8;
+constexpr int NG2_K = 256, NG2_LDW = NG2_K + 8, NG2_SLD = NG2_CPB + 4;
 
-template <int EPI>
+// slotT[o * NG2_SLD + srow + r] for configurations c0 .. c0 + nr - 1 (one
+// wave), eight configurations' loads in flight at a time
+__device__ __forceinline__ void ng2_slot_col(int64_t x, int N, int nf, int lane, uint8_t *col) {
+  const uint64_t up = __ballot((x & 2) != 0), dn = __ballot((x & 1) != 0);
+  const int nup = __popcll(up);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int e = h * 64 + lane;
+    uint8_t v = 0xFF;
+    if (e < 2 * N) {
+      const bool is_dn = e >= N;
+      const int site = is_dn ? e - N : e;
+      const uint64_t m = is_dn ? dn : up;
+      const uint64_t below = (site == 0) ? 0ull : ((1ull << site) - 1ull);
+      const int sl = (is_dn ? nup : 0) + __popcll(m & below);
+      if (((m >> site) & 1ull) && sl < nf) v = (uint8_t)sl;
+    }
+    if (e < 128) col[e * NG2_SLD] = v;
+  }
+}
+
+__device__ __forceinline__ void ng2_slots(const int64_t *__restrict__ cfg, int64_t c0, int nr,
+                                          int64_t R, int N, int nf, int lane, uint8_t *slotT,
+                                          int srow) {
+  int r = 0;
+  for (; r + 8 <= nr; r += 8) {
+    int64_t x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      x[u] = (c0 + r + u < R && lane < N) ? cfg[(c0 + r + u) * N + lane] : 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) ng2_slot_col(x[u], N, nf, lane, slotT + srow + r + u);
+  }
+  for (; r < nr; ++r) {
+    const int64_t x = (c0 + r < R && lane < N) ? cfg[(c0 + r) * N + lane] : 0;
+    ng2_slot_col(x, N, nf, lane, slotT + srow + r);
+  }
+}
+
 __global__ void __launch_bounds__(NG2_T)
     nnbf_ng2_kernel(int64_t R, int n_rows, const __hip_bfloat16 *__restrict__ W,
                     const __hip_bfloat16 *__restrict__ hin, const float *__restrict__ bias,
-                    int act, float *z, __hip_bfloat16 *hout, const int64_t *__restrict__ cfg,
-                    int N, int nf, float *A) {
-  __shared__ __align__(16) __bf16 Ws[2 * 32 * NG2_LDW];                  // [2][32][264]
-  __shared__ __align__(16) uint8_t slot[EPI == NG_SCAT ? NG2_CPB * 128 : 16];  // [256][128]
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, kh = lane >> 5;
+                    const int64_t *__restrict__ cfg, int N, int nf, float *A) {
+  __shared__ __align__(16) __bf16 Ws[2 * 32 * NG2_LDW];     // [2][32][264]
+  __shared__ __align__(16) uint8_t slotT[128 * NG2_SLD];     // [orbital][configuration]
+  const int t = threadIdx.x, lane = t & 63, kh = lane >> 5;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);   // wave-uniform
   const int64_t rw = (int64_t)blockIdx.x * NG2_CPB + (int64_t)wave * NG2_CPW;
   const __bf16 *Wb = reinterpret_cast<const __bf16 *>(W);
   const __bf16 *hb = reinterpret_cast<const __bf16 *>(hin);
-  // the wave's B fragments: configuration rw + 32 c + lane % 32, k = 16 ks + 8 kh .. + 7
+  // the wave's configuration fragments: configuration rw + 32 c + lane % 32,
+  // k = 16 ks + 8 kh .. + 7
   ng_bf8 b[2][16];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
@@ -469,7 +528,7 @@ __global__ void __launch_bounds__(NG2_T)
       b[c][ks] = v;
     }
   }
-  if (EPI == NG_SCAT) ng_slots(cfg, rw, NG2_CPW, R, N, nf, lane, slot, wave * NG2_CPW);
+  ng2_slots(cfg, rw, NG2_CPW, R, N, nf, lane, slotT, wave * NG2_CPW);
   // this workgroup's slice of W's tiles
   const int n_tiles = (n_rows + 31) / 32;
   const int tps = (n_tiles + (int)gridDim.y - 1) / (int)gridDim.y;
@@ -500,27 +559,71 @@ __global__ void __launch_bounds__(NG2_T)
     store_tile(0);
   }
   __syncthreads();
+  // configurations of this lane's accumulator rows: rw + 32 half + 8 gq + 4 kh
+  // + e; rows past R are not stored (lim: how many of the wave's rows from
+  // 4 kh on exist)
+  const int64_t lim64 = R - rw - 4 * kh;
+  const int lim = lim64 > NG2_CPW ? NG2_CPW : (int)lim64;
   for (int tile = t0; tile < t1; ++tile) {
     const int buf = (tile - t0) & 1;
     const bool more = tile + 1 < t1;
     if (more) load_tile(tile + 1);
+    // this lane's output row n of W: orbital o, column j of Docc
+    const int n = tile * 32 + (lane & 31);
+    const bool nok = n < n_rows;
+    const int o = nok ? n / nf : 0, j = n - o * nf;
+    const float bv = nok ? bias[n] : 0.0f;
     ng_f16v acc0, acc1;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       acc0[q] = 0.0f;
       acc1[q] = 0.0f;
     }
-    float bv[16];
-    ng_bias(bias, tile * 32, n_rows, kh, bv);
     const __bf16 *ap = Ws + (buf * 32 + (lane & 31)) * NG2_LDW + 8 * kh;
 #pragma unroll
     for (int ks = 0; ks < 16; ++ks) {
       const ng_bf8 a = *reinterpret_cast<const ng_bf8 *>(ap + 16 * ks);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[0][ks], acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[1][ks], acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[0][ks], a, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[1][ks], a, acc1, 0, 0, 0);
     }
-    ng_epilogue<EPI>(tile * 32, acc0, acc1, rw, wave * NG2_CPW, lane, R, n_rows, bv, act, z,
-                     hout, slot, nf, A);
+    // epilogue: Docc[r][slot_r(o)][j] = acc + b[n] for the configurations r
+    // with orbital o occupied.  The row strides are made opaque here so that
+    // the 32 row addresses are formed per store (scalar multiplies) instead
+    // of being hoisted out of the tile loop into 64 registers.
+    int64_t nf2 = (int64_t)nf * nf, rwt = rw;
+    asm volatile("" : "+s"(nf2), "+s"(rwt));
+    // The values and the slot words are formed before any conditional
+    // store: a load whose result is first used inside a branch makes the
+    // compiler wait for every outstanding memory operation (the stores
+    // before it included) at each branch.
+    float *Ab = A + (rwt + 4 * kh) * nf2 + j;
+    const uint8_t *scol = slotT + o * NG2_SLD + wave * NG2_CPW + 4 * kh;
+    float v[32];
+    uint32_t w4[8];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      v[q] = acc0[q] + bv;
+      v[16 + q] = acc1[q] + bv;
+    }
+#pragma unroll
+    for (int gq = 0; gq < 8; ++gq)
+      w4[gq] = *reinterpret_cast<const uint32_t *>(scol + 8 * gq);   // rows 8 gq + 4 kh ..
+    // (empty statements that take the values as register operands keep
+    // their computation, and its waits, ahead of the branches)
+#pragma unroll
+    for (int q = 0; q < 32; ++q) asm volatile("" : "+v"(v[q]));
+#pragma unroll
+    for (int gq = 0; gq < 8; ++gq) asm volatile("" : "+v"(w4[gq]));
+#pragma unroll
+    for (int gq = 0; gq < 8; ++gq) {
+      const int rl0 = 8 * gq;                           // + 4 kh + e
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int sl = (int)((w4[gq] >> (8 * e)) & 0xFFu);
+        if (nok && sl != 0xFF && rl0 + e < lim)
+          Ab[(int64_t)(rl0 + e) * nf2 + sl * nf] = v[4 * gq + e];
+      }
+    }
     if (more) store_tile(buf ^ 1);
     __syncthreads();
   }
@@ -1301,10 +1404,9 @@ static int launch_ng(int64_t R, int n_rows, int K, const void *W, const void *hi
     const int64_t rb = (R + NG2_CPB - 1) / NG2_CPB;
     int S = 1;
     while (rb * S < 512 && S * 2 * 4 <= n_tiles) S *= 2;
-    hipLaunchKernelGGL((nnbf_ng2_kernel<EPI>), dim3((unsigned)rb, (unsigned)S), dim3(NG2_T), 0,
-                       st, R, n_rows, (const __hip_bfloat16 *)W,
-                       (const __hip_bfloat16 *)hin, bias, act, z, (__hip_bfloat16 *)hout, cfg, N,
-                       nf, A);
+    hipLaunchKernelGGL(nnbf_ng2_kernel, dim3((unsigned)rb, (unsigned)S), dim3(NG2_T), 0, st, R,
+                       n_rows, (const __hip_bfloat16 *)W, (const __hip_bfloat16 *)hin, bias, cfg,
+                       N, nf, A);
     VMCX_LAUNCHED();
     return 0;
   }
