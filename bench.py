@@ -134,6 +134,9 @@ def parse():
     p.add_argument("--peps-reuse", action="store_true",
                    help="fPEPS: sweeps and local energies from the walkers' cached column "
                         "environments (fPEPS_Model_reuse_GPU)")
+    p.add_argument("--peps-amp", default="fp64", choices=["fp64", "fp32"],
+                   help="fPEPS / NN-fTNS: amplitudes contracted in FP64, or in FP32 on the "
+                        "matrix cores (amp='fp32'; the log-derivative stays FP64)")
     p.add_argument("--tau-sweeps", type=int, default=64)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -553,10 +556,11 @@ class PepsRun(NnbfRun):
         g = torch.Generator().manual_seed(0)
         if w["hidden"] > 0:
             self.model = NNfTNS_Model_GPU(w["Lx"], w["Ly"], w["D"], hidden_dim=w["hidden"],
-                                          nn_eta=w["eta"], generator=g).to(dev)
+                                          nn_eta=w["eta"], generator=g, amp=a.peps_amp).to(dev)
         else:
             cls = fPEPS_Model_reuse_GPU if getattr(a, "peps_reuse", False) else fPEPS_Model_GPU
-            self.model = cls(w["Lx"], w["Ly"], w["D"], generator=g).to(dev)
+            kw = {"amp": a.peps_amp} if a.peps_amp != "fp64" else {}
+            self.model = cls(w["Lx"], w["Ly"], w["D"], generator=g, **kw).to(dev)
         self.Np = self.model.n_params
         self.e_sweep = H.graph.edge_tensor(dev, "sweep")
         self.e_graph = H.graph.edge_tensor(dev, "graph")
@@ -839,7 +843,8 @@ def main():
                    "proposals_per_sample": getattr(run, "proposals_per_step", None),
                    "parallelism": "walkers on 1 GPU",
                    **({"environment_reuse": bool(a.peps_reuse)}
-                      if w["kind"] == "peps" and w.get("hidden", 0) == 0 else {})},
+                      if w["kind"] == "peps" and w.get("hidden", 0) == 0 else {}),
+                   **({"amplitudes": a.peps_amp} if w["kind"] == "peps" else {})},
         "sr_iters_per_s": sr_iters,
         "walker_sweeps_per_s": samples_per_s * spp,
         "sweep_only_walker_sweeps_per_s": B * spp / (kt["sweep"] * 1e-3),

@@ -498,7 +498,9 @@ typedef struct vmcx_peps_model {
   double eta;
   const double *W1, *b1, *W2, *b2, *Wo, *bo;
   int32_t mode;   /* 0 = additive theta + eta y, 1 = multiplicative theta (1 + eta y) */
-  int32_t reserved;
+  int32_t amp_fp32; /* 1: amplitudes (forward, sweep proposals, connected configurations of
+                       the local energy) contracted in FP32 on the matrix cores; the
+                       log-derivative and the environment-reuse entry points stay FP64 */
 } vmcx_peps_model;
 
 /* Number of dense tensor positions (length of dense_map); -1 on bad sizes. */

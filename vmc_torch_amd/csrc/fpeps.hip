@@ -288,6 +288,169 @@ __global__ void __launch_bounds__(PT)
   }
 }
 
+// ------------------------------------------------------------ FP32 amplitudes
+// The same contraction in FP32 on the matrix cores, one wave per
+// configuration (four per workgroup).  Top, Bot and the environment are
+// zero-padded to 16 x 4 x 16 and 16 x 16 so that both products of a column
+// are whole v_mfma_f32_16x16x4f32 tiles:
+//   X[b][v][a'] = sum_a env[a][b] Top[a][v][a']     (A = env^T, B = Top[., v, .]: 4 tiles x 4 k-steps)
+//   env'[a'][b'] = sum_{b,v} X[b][v][a'] Bot[b][v][b'] (A = X^T, B = Bot: 16 k-steps)
+// (operands, lane l: A[l % 16][l / 16], B[l / 16][l % 16]; result register
+// r of lane l: D[4 (l / 16) + r][l % 16]).  The slices are read from the FP64
+// parameters and rounded once; the environment is renormalised after every
+// column (its log scale carried in FP64), as in peps_amp_kernel.
+constexpr int P32_W = 4;
+typedef float p32f4 __attribute__((ext_vector_type(4)));
+
+// the wave's LDS writes visible to its own later reads
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct P32Sh {
+  float sl[PMAX_LX][SLICE];          // signed slices of the column; then X[b][v][a']
+  float top[PB * PMAX_D * PB];       // Top[a][v][a'], padded
+  float bot[PB * PMAX_D * PB];       // Bot[b][v][b'], padded
+  float env[PB * PB];                // env[a][b], padded
+  int p[PMAX_N], pr[PMAX_N], doff[PMAX_N];
+};
+
+__global__ void __launch_bounds__(64 * P32_W)
+    peps_amp32_kernel(PepsArgs A, int64_t rows, const int64_t *__restrict__ cfg,
+                      const int32_t *__restrict__ flag, double *sign, double *logabs) {
+  __shared__ P32Sh shm[P32_W];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t w = (int64_t)blockIdx.x * P32_W + wv;
+  if (w >= rows) return;
+  if (flag && !(flag[w] & 1)) return;
+  P32Sh &S = shm[wv];
+  const PepsGeo &g = A.g;
+  const int N = g.N, D = g.D, Ly = g.Ly;
+  const double *th = A.theta + w * A.theta_stride;
+  // the configuration: local states, P_right parities (singly occupied
+  // sites right of s in its row), dense offsets, the global sign
+  const int vl = lane < N ? (int)cfg[w * N + lane] : 0;
+  const uint64_t single = __ballot(lane < N && (vl == 1 || vl == 2));
+  const int P = __popcll(single);
+  if (lane < N) {
+    const int x = lane / Ly;
+    const int cnt = x * Ly + Ly - lane - 1;          // sites right of s in row x
+    const uint64_t m = (lane + 1 < 64 ? single >> (lane + 1) : 0ull) &
+                       (cnt >= 64 ? ~0ull : ((1ull << cnt) - 1ull));
+    S.p[lane] = vl;
+    S.pr[lane] = __popcll(m) & 1;
+    int off = 0;
+    for (int s2 = 0; s2 < lane; ++s2) {
+      int dl, du, dd, dr;
+      site_dims(g, s2 / Ly, s2 - (s2 / Ly) * Ly, dl, du, dd, dr);
+      off += 4 * dl * du * dd * dr;
+    }
+    S.doff[lane] = off;
+  }
+  const double gsign = ((P * (P - 1) / 2) & 1) ? -1.0 : 1.0;
+  for (int i = lane; i < PB * PB; i += 64) S.env[i] = i == 0 ? 1.0f : 0.0f;
+  wave_lds_fence();
+  const int tr = g.tr, br = g.Lx - tr, dv = D;
+  double lsc = 0.0;
+  for (int y = 0; y < Ly; ++y) {
+    // signed slices of the column
+    for (int x = 0; x < g.Lx; ++x) {
+      int dl, du, dd, dr;
+      site_dims(g, x, y, dl, du, dd, dr);
+      const int s = x * Ly + y, n = dl * du * dd * dr;
+      const int base = S.doff[s] + S.p[s] * n, prs = S.pr[s];
+      for (int idx = lane; idx < n; idx += 64) {
+        const int r = idx % dr, d = (idx / dr) % dd;
+        const int k = A.dmap[base + idx];
+        float v = k >= 0 ? (float)th[k] : 0.0f;
+        if ((qpar(d, dd) * (qpar(r, dr) + prs)) & 1) v = -v;
+        S.sl[x][idx] = v;
+      }
+    }
+    wave_lds_fence();
+    const int dlc = (y == 0) ? 1 : D, drc = (y == Ly - 1) ? 1 : D;
+    const int da = tr == 2 ? dlc * dlc : dlc, da2 = tr == 2 ? drc * drc : drc;
+    const int db = br == 2 ? dlc * dlc : dlc, db2 = br == 2 ? drc * drc : drc;
+    // Top[a][v][a'] and Bot[b][v][b'], zero outside (da or db) x dv x (da2 or db2)
+    for (int idx = lane; idx < PB * PMAX_D * PB; idx += 64) {
+      const int a = idx >> 6, v = (idx >> 4) & 3, a2 = idx & 15;
+      float t = 0.0f, bb = 0.0f;
+      if (v < dv) {
+        if (a < da && a2 < da2) {
+          if (tr == 1) {
+            t = S.sl[0][(a * dv + v) * drc + a2];
+          } else {
+            const int l0 = a / dlc, l1 = a % dlc, r0 = a2 / drc, r1 = a2 % drc;
+            for (int m = 0; m < D; ++m)
+              t = fmaf(S.sl[0][(l0 * D + m) * drc + r0],
+                       S.sl[1][((l1 * D + m) * dv + v) * drc + r1], t);
+          }
+        }
+        if (a < db && a2 < db2) {
+          const int xb = tr;
+          if (br == 1) {
+            bb = S.sl[xb][(a * dv + v) * drc + a2];
+          } else {
+            const int l2 = a / dlc, l3 = a % dlc, r2 = a2 / drc, r3 = a2 % drc;
+            for (int m = 0; m < D; ++m)
+              bb = fmaf(S.sl[xb][((l2 * dv + v) * D + m) * drc + r2],
+                        S.sl[xb + 1][(l3 * D + m) * drc + r3], bb);
+          }
+        }
+      }
+      S.top[idx] = t;
+      S.bot[idx] = bb;
+    }
+    wave_lds_fence();
+    // X[b][v][a'] = sum_a env[a][b] Top[a][v][a'] (into the slice buffer)
+    float *X = &S.sl[0][0];
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+      p32f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const float av = S.env[(4 * ks + lk) * PB + li];
+        const float bv = S.top[(4 * ks + lk) * 64 + nb * 16 + li];
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) X[((4 * lk + r) * 4 + nb) * 16 + li] = acc[r];
+    }
+    wave_lds_fence();
+    // env'[a'][b'] = sum_{b,v} X[b][v][a'] Bot[b][v][b']
+    p32f4 e = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+      const float av = X[(4 * ks + lk) * 16 + li];
+      const float bv = S.bot[(4 * ks + lk) * 16 + li];
+      e = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, e, 0, 0, 0);
+    }
+    // renormalise by the largest magnitude
+    float mx = fmaxf(fmaxf(fabsf(e[0]), fabsf(e[1])), fmaxf(fabsf(e[2]), fabsf(e[3])));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    const float inv = mx > 0.0f ? 1.0f / mx : 1.0f;
+    if (mx > 0.0f) lsc += log((double)mx);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) S.env[(4 * lk + r) * PB + li] = e[r] * inv;
+    wave_lds_fence();
+  }
+  if (lane == 0) {
+    const double v = (double)S.env[0] * gsign;
+    const double lo = log(1e-45);
+    if (v == 0.0 || !std::isfinite(v)) {
+      sign[w] = 0.0;
+      logabs[w] = lo;
+    } else {
+      sign[w] = v > 0.0 ? 1.0 : -1.0;
+      logabs[w] = fmax(log(fabs(v)) + lsc, lo);
+    }
+  }
+}
+
 // ------------------------------------------------------------ environment reuse
 // Per-walker cache of the column environments of the exact contraction:
 // L_y = columns 0 .. y-1 absorbed from the left, R_y = columns y .. Ly-1
@@ -1060,8 +1223,12 @@ int amp_rows(const vmcx_peps_model &m, const PepsWs &L, int64_t rows, const int6
     if (rc) return rc;
     A = args_of(m, L.theta_eff, m.n_params);
   }
-  hipLaunchKernelGGL(peps_amp_kernel, dim3((unsigned)rows), dim3(PT), 0, st, A, rows, cfg, flag,
-                     sign, logabs);
+  if (m.amp_fp32)
+    hipLaunchKernelGGL(peps_amp32_kernel, dim3((unsigned)((rows + P32_W - 1) / P32_W)),
+                       dim3(64 * P32_W), 0, st, A, rows, cfg, flag, sign, logabs);
+  else
+    hipLaunchKernelGGL(peps_amp_kernel, dim3((unsigned)rows), dim3(PT), 0, st, A, rows, cfg,
+                       flag, sign, logabs);
   VMCX_LAUNCHED();
   return 0;
 }

@@ -540,12 +540,13 @@ def peps_layout(Lx, Ly, D):
     return np.concatenate(maps), sizes
 
 
-def peps_struct(Lx, Ly, D, dense_map, theta, nn=None, act=0, eta=0.0, mode=0):
+def peps_struct(Lx, Ly, D, dense_map, theta, nn=None, act=0, eta=0.0, mode=0, amp_fp32=False):
     """vmcx_peps_model from device tensors: dense_map int32, theta float64,
     nn = dict W1, b1, W2, b2, Wo, bo (float64) or None; mode 0 = additive,
-    1 = multiplicative network correction."""
+    1 = multiplicative network correction; amp_fp32: amplitudes contracted
+    in FP32 on the matrix cores."""
     s = _lib.PepsModelStruct()
-    s.mode, s.reserved = int(mode), 0
+    s.mode, s.amp_fp32 = int(mode), 1 if amp_fp32 else 0
     require(dense_map, I32, "dense_map")
     require(theta, F64, "theta")
     s.Lx, s.Ly, s.D, s.n_params = int(Lx), int(Ly), int(D), int(theta.numel())

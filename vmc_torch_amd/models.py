@@ -484,12 +484,21 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
     S_s = T_s[x_s] (-1)^{q(d)(q(r) + P_right(s))} (see include/vmcx.h).
     theta: initial vector (numpy or tensor); default: entries N(0, 1) from
     `generator`, each site tensor scaled to unit norm.
+    amp: "fp64" (default) or "fp32": the amplitudes of forward_log, of the
+    sweep's proposals and of the local energy's connected configurations
+    contracted in FP32 on the matrix cores (the log-derivative stays FP64).
     """
 
+    AMPS = ("fp64", "fp32")
+
     def __init__(self, Lx, Ly, D, theta=None, generator=None, dtype=torch.float64,
-                 max_bond=-1, chunk=65536, _nn=None, act="relu", eta=0.0, _mode=0):
+                 max_bond=-1, chunk=65536, _nn=None, act="relu", eta=0.0, _mode=0,
+                 amp="fp64"):
         if dtype != torch.float64:
-            raise ValueError("the fPEPS kernels are FP64")
+            raise ValueError("the fPEPS parameters are FP64 (amp='fp32' selects FP32 "
+                             "contraction of the amplitudes)")
+        if amp not in self.AMPS:
+            raise ValueError(f"amp must be one of {self.AMPS}")
         if not (2 <= Lx <= 4 and 1 <= Ly <= 8 and 1 <= D <= 4):
             raise NotImplementedError("exact contraction needs 2 <= Lx <= 4, Ly <= 8, D <= 4")
         dmap, sizes = kernels.peps_layout(Lx, Ly, D)
@@ -514,6 +523,7 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
         self.act_name = act
         self.eta = float(eta)
         self.mode = int(_mode)
+        self.amp = amp
         self._struct_obj = None
         self._ws = {}
 
@@ -529,7 +539,8 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
                     p.data = p.data.contiguous()
             self._struct_obj = kernels.peps_struct(
                 self.Lx, self.Ly, self.D, self.dense_map, self.params[0].data,
-                self._nn_dict(), 0 if self.act_name == "relu" else 1, self.eta, self.mode)
+                self._nn_dict(), 0 if self.act_name == "relu" else 1, self.eta, self.mode,
+                amp_fp32=self.amp == "fp32")
             self._state_key = None
             return True
         return False
@@ -722,6 +733,8 @@ class fPEPS_Model_reuse_GPU(fPEPS_Model_GPU):
 
     def __init__(self, Lx, Ly, D, theta=None, generator=None, dtype=torch.float64,
                  max_bond=-1, chunk=65536, **kwargs):
+        if kwargs.get("amp", "fp64") != "fp64":
+            raise ValueError("the environment-reuse entry points contract in FP64")
         super().__init__(Lx, Ly, D, theta=theta, generator=generator, dtype=dtype,
                          max_bond=max_bond, chunk=chunk)
         self.radius = 0
@@ -790,7 +803,7 @@ class NNfTNS_Model_GPU(fPEPS_Model_GPU):
 
     def __init__(self, Lx, Ly, D, hidden_dim=128, nn_eta=1.0, activation="relu",
                  init_scale=1e-2, theta=None, generator=None, dtype=torch.float64, max_bond=-1,
-                 chunk=65536, correction="additive"):
+                 chunk=65536, correction="additive", amp="fp64"):
         if activation not in ("relu", "tanh"):
             raise ValueError("activation must be relu or tanh")
         if correction not in self.CORRECTIONS:
@@ -819,7 +832,7 @@ class NNfTNS_Model_GPU(fPEPS_Model_GPU):
                torch.zeros(n, dtype=torch.float64)]
         super().__init__(Lx, Ly, D, theta=th, generator=g, dtype=dtype, max_bond=max_bond,
                          chunk=chunk, _nn=nnp, act=activation, eta=nn_eta,
-                         _mode=self.CORRECTIONS[correction])
+                         _mode=self.CORRECTIONS[correction], amp=amp)
         self.hidden = hidden_dim
         self.nn_eta = float(nn_eta)
         self.correction = correction
