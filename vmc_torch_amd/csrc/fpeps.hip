@@ -317,17 +317,55 @@ struct P32Sh {
   int p[PMAX_N], pr[PMAX_N], doff[PMAX_N];
 };
 
+// Signed FP32 slices of a shared parameter vector (plain fPEPS), for every
+// site s, local state p and parity pr of P_right(s): tbl[((s 4 + p) 2 + pr)
+// 256 + idx] (zero past the slice's dl du dd dr entries)
+__global__ void __launch_bounds__(256)
+    peps_slice_table_kernel(PepsArgs A, float *tbl) {
+  const PepsGeo &g = A.g;
+  const int64_t n_all = (int64_t)g.N * 8 * 256;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_all;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int idx = (int)(e & 255), q = (int)(e >> 8);
+    const int prs = q & 1, pst = (q >> 1) & 3, s = q >> 3;
+    const int x = s / g.Ly, y = s - (s / g.Ly) * g.Ly;
+    int off = 0;
+    for (int s2 = 0; s2 < s; ++s2) {
+      int a, b, c, d;
+      site_dims(g, s2 / g.Ly, s2 - (s2 / g.Ly) * g.Ly, a, b, c, d);
+      off += 4 * a * b * c * d;
+    }
+    int dl, du, dd, dr;
+    site_dims(g, x, y, dl, du, dd, dr);
+    const int n = dl * du * dd * dr;
+    float v = 0.0f;
+    if (idx < n) {
+      const int r = idx % dr, d = (idx / dr) % dd;
+      const int k = A.dmap[off + pst * n + idx];
+      v = k >= 0 ? (float)A.theta[k] : 0.0f;
+      if ((qpar(d, dd) * (qpar(r, dr) + prs)) & 1) v = -v;
+    }
+    tbl[e] = v;
+  }
+}
+
+// DD: the bond dimension (1 .. 4, compile time); tbl: the slice table of a
+// shared parameter vector, or NULL (per-configuration parameters: slices
+// read through the dense map)
+template <int DD>
 __global__ void __launch_bounds__(64 * P32_W)
     peps_amp32_kernel(PepsArgs A, int64_t rows, const int64_t *__restrict__ cfg,
-                      const int32_t *__restrict__ flag, double *sign, double *logabs) {
-  __shared__ P32Sh shm[P32_W];
+                      const int32_t *__restrict__ flag, const float *__restrict__ tbl,
+                      double *sign, double *logabs) {
+  __shared__ __align__(16) P32Sh shm[P32_W];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int64_t w = (int64_t)blockIdx.x * P32_W + wv;
   if (w >= rows) return;
   if (flag && !(flag[w] & 1)) return;
   P32Sh &S = shm[wv];
   const PepsGeo &g = A.g;
-  const int N = g.N, D = g.D, Ly = g.Ly;
+  const int N = g.N, Ly = g.Ly;
+  constexpr int D = DD;
   const double *th = A.theta + w * A.theta_stride;
   // the configuration: local states, P_right parities (singly occupied
   // sites right of s in its row), dense offsets, the global sign
@@ -356,17 +394,27 @@ __global__ void __launch_bounds__(64 * P32_W)
   double lsc = 0.0;
   for (int y = 0; y < Ly; ++y) {
     // signed slices of the column
-    for (int x = 0; x < g.Lx; ++x) {
-      int dl, du, dd, dr;
-      site_dims(g, x, y, dl, du, dd, dr);
-      const int s = x * Ly + y, n = dl * du * dd * dr;
-      const int base = S.doff[s] + S.p[s] * n, prs = S.pr[s];
-      for (int idx = lane; idx < n; idx += 64) {
-        const int r = idx % dr, d = (idx / dr) % dd;
-        const int k = A.dmap[base + idx];
-        float v = k >= 0 ? (float)th[k] : 0.0f;
-        if ((qpar(d, dd) * (qpar(r, dr) + prs)) & 1) v = -v;
-        S.sl[x][idx] = v;
+    if (tbl) {
+      // one 16-byte piece of each site's 256-entry table row per lane
+      for (int x = 0; x < g.Lx; ++x) {
+        const int s = x * Ly + y;
+        const float4 v = reinterpret_cast<const float4 *>(
+            tbl + ((int64_t)(s * 4 + S.p[s]) * 2 + S.pr[s]) * 256)[lane];
+        reinterpret_cast<float4 *>(S.sl[x])[lane] = v;
+      }
+    } else {
+      for (int x = 0; x < g.Lx; ++x) {
+        int dl, du, dd, dr;
+        site_dims(g, x, y, dl, du, dd, dr);
+        const int s = x * Ly + y, n = dl * du * dd * dr;
+        const int base = S.doff[s] + S.p[s] * n, prs = S.pr[s];
+        for (int idx = lane; idx < n; idx += 64) {
+          const int r = idx % dr, d = (idx / dr) % dd;
+          const int k = A.dmap[base + idx];
+          float v = k >= 0 ? (float)th[k] : 0.0f;
+          if ((qpar(d, dd) * (qpar(r, dr) + prs)) & 1) v = -v;
+          S.sl[x][idx] = v;
+        }
       }
     }
     wave_lds_fence();
@@ -1084,6 +1132,8 @@ struct PepsWs {
   int64_t *pcfg;
   int64_t *counts, *offsets, *total, *etas;
   double *coeffs, *csign, *clog;
+  // FP32 amplitudes of a shared parameter vector: the signed slice table
+  float *tbl;
   // environment reuse: the walkers' column environments, and the walker and
   // hop of each connected configuration
   double *env;
@@ -1130,6 +1180,7 @@ PepsWs peps_layout(const vmcx_peps_model &m, int64_t B, int64_t chunk, int n_edg
   L.coeffs = (double *)take((size_t)T * 8);
   L.csign = (double *)take((size_t)T * 8);
   L.clog = (double *)take((size_t)T * 8);
+  L.tbl = (float *)take(m.amp_fp32 && H == 0 ? (size_t)N * 8 * 256 * 4 : 0);
   if (reuse) {
     L.env = (double *)take((size_t)B * env_stride(m.Ly) * 8);
     L.cparent = (int32_t *)take((size_t)T * 4);
@@ -1223,10 +1274,23 @@ int amp_rows(const vmcx_peps_model &m, const PepsWs &L, int64_t rows, const int6
     if (rc) return rc;
     A = args_of(m, L.theta_eff, m.n_params);
   }
-  if (m.amp_fp32)
-    hipLaunchKernelGGL(peps_amp32_kernel, dim3((unsigned)((rows + P32_W - 1) / P32_W)),
-                       dim3(64 * P32_W), 0, st, A, rows, cfg, flag, sign, logabs);
-  else
+  if (m.amp_fp32) {
+    const float *tbl = nullptr;
+    if (m.hidden == 0) {
+      const int64_t ne = (int64_t)A.g.N * 8 * 256;
+      hipLaunchKernelGGL(peps_slice_table_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256),
+                         0, st, A, L.tbl);
+      VMCX_LAUNCHED();
+      tbl = L.tbl;
+    }
+    const dim3 grid((unsigned)((rows + P32_W - 1) / P32_W)), blk(64 * P32_W);
+    switch (m.D) {
+      case 1: hipLaunchKernelGGL(peps_amp32_kernel<1>, grid, blk, 0, st, A, rows, cfg, flag, tbl, sign, logabs); break;
+      case 2: hipLaunchKernelGGL(peps_amp32_kernel<2>, grid, blk, 0, st, A, rows, cfg, flag, tbl, sign, logabs); break;
+      case 3: hipLaunchKernelGGL(peps_amp32_kernel<3>, grid, blk, 0, st, A, rows, cfg, flag, tbl, sign, logabs); break;
+      default: hipLaunchKernelGGL(peps_amp32_kernel<4>, grid, blk, 0, st, A, rows, cfg, flag, tbl, sign, logabs); break;
+    }
+  } else
     hipLaunchKernelGGL(peps_amp_kernel, dim3((unsigned)rows), dim3(PT), 0, st, A, rows, cfg,
                        flag, sign, logabs);
   VMCX_LAUNCHED();
