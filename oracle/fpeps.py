@@ -337,3 +337,117 @@ def local_energy(H, shape, theta, xs, nn=None, eta=0.0, act="relu", mode="add"):
     for e, c, b in zip(etas, coeffs, bids):
         E[b] += c * amp(e) / psi[b]
     return E
+
+
+# ------------------------------------------- chi-truncated boundary MPS
+# The reference contracts an fPEPS amplitude with quimb's boundary MPS
+# (GPU/models/pureTNS.py:81-111): rows 0 .. Lx/2-1 absorbed from xmin, rows
+# Lx-1 .. Lx/2 from xmax, each with max_bond=chi, cutoff=0, mode="mps",
+# canonize=True (GPU/run_scripts/vmc_run_fpeps.py:126-136), then the two
+# boundaries contracted exactly.  quimb is absent, so its sweep direction is
+# unpinned; this restatement fixes it: after each absorbed row whose bonds
+# exceed chi, canonize from the last column to the first (QR), then truncate
+# from the first column to the last by SVD, keeping the chi largest singular
+# values.  That is the sequential left-to-right Schmidt truncation of the
+# boundary state (a TT-SVD of the state with maximal rank chi), which the
+# tests check against that dense definition.
+
+def _bmps_init(row, down):
+    """The first row of a side as an MPS [a, v, b]: its open vertical legs
+    (down legs from the top, up legs from the bottom)."""
+    return [S[:, 0, :, :] if down else S[:, :, 0, :] for S in row]
+
+
+def _bmps_absorb(mps, row, down):
+    out = []
+    for B, S in zip(mps, row):
+        T = np.einsum("avb,lvdr->aldbr", B, S) if down else np.einsum("avb,luvr->alubr", B, S)
+        a, l, v, b, r = T.shape
+        out.append(T.reshape(a * l, v, b * r))
+    return out
+
+
+def _bmps_compress(mps, chi, gaps, canonize=True):
+    """QR canonicalisation right to left, SVD truncation left to right; the
+    relative gap (s[chi-1] - s[chi]) / s[0] of every truncation goes to
+    `gaps` (a small gap makes the truncated state ill-conditioned).
+    canonize=False exists for the tests only (the truncation without the
+    canonical form, which they must reject)."""
+    mps = list(mps)
+    for y in range(len(mps) - 1, 0, -1) if canonize else ():
+        a, v, b = mps[y].shape
+        Q, R = np.linalg.qr(mps[y].reshape(a, v * b).T)
+        mps[y] = Q.T.reshape(Q.shape[1], v, b)
+        mps[y - 1] = np.einsum("avb,kb->avk", mps[y - 1], R)
+    for y in range(len(mps) - 1):
+        a, v, b = mps[y].shape
+        U, s, Vt = np.linalg.svd(mps[y].reshape(a * v, b), full_matrices=False)
+        k = min(chi, s.size)
+        if k < s.size and s[0] > 0:
+            gaps.append((s[k - 1] - s[k]) / s[0])
+        mps[y] = U[:, :k].reshape(a, v, k)
+        mps[y + 1] = np.einsum("k,kb,bvc->kvc", s[:k], Vt[:k], mps[y + 1])
+    return mps
+
+
+def _bmps_normalise(mps):
+    lg = 0.0
+    out = []
+    for T in mps:
+        m = np.abs(T).max()
+        if m == 0.0:
+            return [T for T in mps], -np.inf
+        out.append(T / m)
+        lg += np.log(m)
+    return out, lg
+
+
+def bmps_side(shape, slices, chi, down, gaps=None):
+    """(MPS, log scale) of one side: rows 0 .. Lx/2-1 (down=True) or
+    Lx-1 .. Lx/2 (down=False), compressed to chi after every absorbed row
+    whose bonds exceed chi (chi <= 0: exact)."""
+    Lx, Ly = shape.Lx, shape.Ly
+    h = Lx // 2
+    rows = list(range(h)) if down else list(range(Lx - 1, h - 1, -1))
+    gaps = [] if gaps is None else gaps
+    mps, lg = _bmps_normalise(_bmps_init([slices[rows[0] * Ly + y] for y in range(Ly)], down))
+    for xx in rows[1:]:
+        mps = _bmps_absorb(mps, [slices[xx * Ly + y] for y in range(Ly)], down)
+        if chi > 0 and max(T.shape[2] for T in mps) > chi:
+            mps = _bmps_compress(mps, chi, gaps)
+        mps, l2 = _bmps_normalise(mps)
+        lg += l2
+    return mps, lg
+
+
+def bmps_contract(shape, slices, chi, gaps=None):
+    """(sign, ln|value|) of the sliced network by the two boundaries."""
+    top, lt = bmps_side(shape, slices, chi, True, gaps)
+    bot, lb = bmps_side(shape, slices, chi, False, gaps)
+    E = np.ones((1, 1))
+    for T, Bo in zip(top, bot):
+        E = np.einsum("ab,avc,bvd->cd", E, T, Bo)
+    v = float(E[0, 0])
+    if v == 0.0 or not np.isfinite(lt + lb):
+        return 0.0, -np.inf
+    return float(np.sign(v)), float(np.log(abs(v)) + lt + lb)
+
+
+def bmps_log_amplitude(shape, theta, x, chi, gaps=None):
+    """(sign, ln|psi|) of fPEPS_Model_GPU.log_amplitude with max_bond=chi
+    (pureTNS.py:114-146) in this module's sign convention."""
+    tensors = shape.unpack(theta)
+    s, l = bmps_contract(shape, signed_slices(shape, tensors, x), chi, gaps)
+    return s * global_sign(x), l
+
+
+def bmps_log_amplitudes(shape, theta, xs, chi):
+    """Batch form; also returns each configuration's smallest truncation gap
+    (inf when nothing was truncated)."""
+    out = []
+    for x in np.atleast_2d(xs):
+        g = []
+        s, l = bmps_log_amplitude(shape, theta, x, chi, g)
+        out.append((s, l, min(g) if g else np.inf))
+    a = np.array(out)
+    return a[:, 0], a[:, 1], a[:, 2]
