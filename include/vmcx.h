@@ -501,6 +501,15 @@ typedef struct vmcx_peps_model {
   int32_t amp_fp32; /* 1: amplitudes (forward, sweep proposals, connected configurations of
                        the local energy) contracted in FP32 on the matrix cores; the
                        log-derivative and the environment-reuse entry points stay FP64 */
+  int32_t chi;      /* max_bond of fPEPS_Model_GPU (pureTNS.py:81-146).  <= 0: exact
+                       contraction (Lx <= 4, Ly <= 8).  > 0 and below D^rows of a side (or a
+                       lattice beyond the exact kernels): chi-truncated boundary MPS, rows
+                       0 .. Lx/2-1 from the top and Lx-1 .. Lx/2 from the bottom, each
+                       truncated left to right onto the chi leading Schmidt vectors of every
+                       cut after each absorbed row; chi <= 32, max(chi, D) * D <= 64,
+                       Lx * Ly <= 64, Ly <= 16; FP64 amplitudes only (forward, sweep, local
+                       energy) */
+  int32_t reserved;
 } vmcx_peps_model;
 
 /* Number of dense tensor positions (length of dense_map); -1 on bad sizes. */

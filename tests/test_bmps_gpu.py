@@ -1,0 +1,180 @@
+"""GPU parity of the chi-truncated boundary-MPS fPEPS / NN-fTNS amplitudes
+(fPEPS_Model_GPU(max_bond=chi): bmps.hip) against the oracle's boundary MPS
+(oracle.fpeps.bmps_*, QR canonical form and SVD truncation; the device
+truncates through right Grams and a Jacobi eigensolver per Z2 block).
+
+Tolerances: FP64 both sides, ln|psi| within 1e-8 and signs equal for every
+configuration whose truncations all have a relative singular-value gap
+(s[chi-1] - s[chi]) / s[0] of at least 1e-6 (below that the truncated state
+is ill-conditioned and both sides are right to differ; the tests assert such
+configurations are rare); local energies 1e-8 relative to the batch's
+largest; sweeps with injected draws reproduce the oracle's configurations and
+signs exactly."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import random_configs, tie_free_draws
+from oracle import fpeps as fp
+from oracle import hubbard as ohub
+from vmc_torch_amd import NNfTNS_Model_GPU, fPEPS_Model_GPU, fPEPS_Model_reuse_GPU
+from vmc_torch_amd.hamiltonian import spinful_Fermi_Hubbard_square_lattice_torch
+
+pytestmark = pytest.mark.gpu
+
+GAP = 1e-6
+
+
+def _plain(Lx, Ly, D, chi, seed, device):
+    sh = fp.PEPSShape(Lx, Ly, D)
+    th = fp.random_theta(sh, np.random.default_rng(seed))
+    return fPEPS_Model_GPU(Lx, Ly, D, theta=th, max_bond=chi).to(device), sh, th
+
+
+def _oracle(sh, th, xs, chi, nn=None, eta=0.0, mode="add"):
+    out = []
+    for x in np.atleast_2d(xs):
+        t = th
+        if nn is not None:
+            y, _ = fp.mlp_forward(nn, x)
+            t = fp.theta_eff(th, y[0], eta, mode)
+        g = []
+        s, l = fp.bmps_log_amplitude(sh, t, x, chi, g)
+        out.append((s, l, min(g) if g else np.inf))
+    a = np.array(out)
+    return a[:, 0], a[:, 1], a[:, 2]
+
+
+def _check(s, l, so, lo, go, min_frac=0.9):
+    s, l = s.cpu().numpy(), l.cpu().numpy()
+    ok = go >= GAP
+    assert ok.mean() >= min_frac
+    assert np.array_equal(s[ok], so[ok])
+    nz = ok & (so != 0)
+    assert nz.sum() >= ok.sum() // 2
+    assert np.abs(l[nz] - lo[nz]).max() < 1e-8
+
+
+@pytest.mark.parametrize("Lx,Ly,D,chi", [(4, 4, 2, 3), (4, 5, 2, 3), (6, 4, 2, 3), (4, 4, 4, 8),
+                                         (5, 4, 3, 5), (6, 6, 4, 16), (3, 6, 3, 4),
+                                         (8, 8, 2, 6)])
+def test_bmps_forward_matches_oracle(device, Lx, Ly, D, chi):
+    m, sh, th = _plain(Lx, Ly, D, chi, 31 + Lx + Ly + D + chi, device)
+    n = 24 if sh.N > 30 else 48
+    x = random_configs(np.random.default_rng(3), n, sh.N, sh.N // 2, sh.N // 2)
+    s, l = m.forward_log(torch.tensor(x, device=device))
+    so, lo, go = _oracle(sh, th, x, chi)
+    assert np.isfinite(go).any()          # the case truncates
+    _check(s, l, so, lo, go)
+
+
+@pytest.mark.parametrize("Lx,Ly,D,chi", [(6, 3, 2, 8), (5, 2, 3, 9), (6, 4, 1, 1)])
+def test_bmps_untruncated_equals_exact_contraction(device, Lx, Ly, D, chi):
+    """Lattices beyond the exact kernels with chi at or above every bond:
+    the boundary MPS equals the exact zipper contraction."""
+    m, sh, th = _plain(Lx, Ly, D, chi, 5 + Lx + Ly, device)
+    rng = np.random.default_rng(4)
+    if D == 1:
+        # even bonds only: singly occupied sites vanish, so empty / doubly occupied ones
+        x = np.array([3 * (rng.permutation(sh.N) < sh.N // 2) for _ in range(32)])
+    else:
+        x = random_configs(rng, 32, sh.N, sh.N // 2, sh.N // 2)
+    s, l = m.forward_log(torch.tensor(x, device=device))
+    want = fp.amplitudes(sh, th, x)
+    s, l = s.cpu().numpy(), l.cpu().numpy()
+    nz = np.abs(want) > 1e-12 * np.abs(want).max()
+    assert nz.sum() >= 8
+    assert np.array_equal(s[nz], np.sign(want[nz]))
+    assert np.abs(l[nz] - np.log(np.abs(want[nz]))).max() < 1e-9
+
+
+@pytest.mark.parametrize("correction", ["additive", "multiplicative"])
+def test_bmps_nnftn_forward_matches_oracle(device, correction):
+    Lx, Ly, D, chi, H = 4, 5, 2, 3, 8
+    sh = fp.PEPSShape(Lx, Ly, D)
+    rng = np.random.default_rng(12)
+    th = fp.random_theta(sh, rng)
+    nn = fp.mlp_init(rng, 2 * sh.N, H, sh.n_params, 0.2)
+    m = NNfTNS_Model_GPU(Lx, Ly, D, hidden_dim=H, nn_eta=0.7, theta=th, max_bond=chi,
+                         correction=correction)
+    with torch.no_grad():
+        for p, v in zip(m.params[1:], nn):
+            p.copy_(torch.as_tensor(v))
+    m = m.to(device)
+    x = random_configs(np.random.default_rng(5), 32, sh.N, sh.N // 2, sh.N // 2)
+    s, l = m.forward_log(torch.tensor(x, device=device))
+    mode = "add" if correction == "additive" else "mul"
+    so, lo, go = _oracle(sh, th, x, chi, nn, m.nn_eta, mode)
+    _check(s, l, so, lo, go)
+
+
+def test_bmps_passes_agree(device):
+    """More configurations than one pass of the kernels' workspace (2048):
+    every pass gives the values of a small batch, and a sample of them the
+    oracle's."""
+    m, sh, th = _plain(4, 4, 2, 3, 77, device)
+    x = random_configs(np.random.default_rng(6), 2100, sh.N, 8, 8)
+    xd = torch.tensor(x, device=device)
+    s, l = m.forward_log(xd)
+    s2, l2 = m.forward_log(xd[2040:].clone())
+    assert torch.equal(s[2040:], s2) and torch.equal(l[2040:], l2)
+    so, lo, go = _oracle(sh, th, x[2040:], 3)
+    _check(s2, l2, so, lo, go)
+
+
+def test_bmps_local_energy_matches_oracle(device):
+    m, sh, th = _plain(4, 4, 2, 3, 41, device)
+    nu = nd = 8
+    H = spinful_Fermi_Hubbard_square_lattice_torch(4, 4, 1.0, 8.0, 16, n_fermions_per_spin=(8, 8))
+    x = random_configs(np.random.default_rng(8), 12, sh.N, nu, nd)
+    E = m.local_energy(torch.tensor(x, device=device), H).cpu().numpy()
+    Ho = ohub.Hubbard(4, 4, 1.0, 8.0, nu, nd)
+    s0, l0, g0 = _oracle(sh, th, x, 3)
+    etas, coeffs, bids = Ho.get_conn_batch(x)
+    se, le, ge = _oracle(sh, th, etas, 3)
+    Eo = np.zeros(x.shape[0])
+    good = g0 >= GAP
+    for e_s, e_l, e_g, c, b in zip(se, le, ge, coeffs, bids):
+        Eo[b] += c * e_s * s0[b] * np.exp(e_l - l0[b]) if e_s != 0 else 0.0
+        good[b] &= e_g >= GAP
+    assert good.sum() >= 8
+    assert np.allclose(E[good], Eo[good], rtol=1e-8, atol=1e-8 * np.abs(Eo).max())
+
+
+def test_bmps_sweep_matches_oracle(device):
+    m, sh, th = _plain(4, 4, 2, 3, 52, device)
+    gaps = []
+
+    def la(cs):
+        s, l, g = _oracle(sh, th, cs, 3)
+        gaps.extend(g.tolist())
+        return s, np.maximum(l, np.log(1e-45))
+    H = spinful_Fermi_Hubbard_square_lattice_torch(4, 4, 1.0, 8.0, 16, n_fermions_per_spin=(8, 8))
+    edges = H.graph.sweep_edges()
+    rng = np.random.default_rng(10)
+    x0 = random_configs(rng, 12, sh.N, 8, 8)
+    draws, xo, so, lo = tie_free_draws(la, x0, edges, 1, rng, margin=1e-6)
+    assert min(gaps) >= GAP           # no ill-conditioned truncation on the oracle's path
+    xd = torch.tensor(x0, device=device)
+    dd = tuple(torch.tensor(a, device=device) for a in draws)
+    sign, logabs, nacc = m.mcmc_sweep(xd, H.graph.edge_tensor(device, "sweep"), 1, 0, 0,
+                                      draws=dd)
+    assert np.array_equal(xd.cpu().numpy(), xo)
+    assert np.array_equal(sign.cpu().numpy(), so)
+    assert np.abs(logabs.cpu().numpy() - lo).max() < 1e-8
+
+
+def test_bmps_options(device):
+    with pytest.raises(ValueError):
+        fPEPS_Model_GPU(4, 4, 2, max_bond=3, amp="fp32")
+    with pytest.raises(NotImplementedError):
+        fPEPS_Model_reuse_GPU(4, 4, 2, max_bond=3)
+    with pytest.raises(NotImplementedError):
+        fPEPS_Model_GPU(6, 4, 4, max_bond=20)          # max(chi, D) * D > 64
+    # chi at or above D^2 on Lx <= 4 keeps the exact kernels
+    m = fPEPS_Model_GPU(4, 4, 2, max_bond=4)
+    assert not m._bmps
+    m = fPEPS_Model_GPU(4, 4, 2, max_bond=3).to(device)
+    x = torch.tensor(random_configs(np.random.default_rng(1), 4, 16, 8, 8), device=device)
+    with pytest.raises(RuntimeError, match="log-derivative"):
+        m.log_derivative(x)

@@ -18,7 +18,7 @@ def test_library_exports_header_symbols():
 
 def test_abi_version_and_nfp():
     lib = _lib.lib()
-    assert lib.vmcx_abi_version() == 18
+    assert lib.vmcx_abi_version() == 19
     assert [lib.vmcx_slater_nfp(n) for n in (1, 8, 9, 16, 17, 32, 33, 64, 65)] == \
         [8, 8, 16, 16, 32, 32, 64, 64, -1]
 

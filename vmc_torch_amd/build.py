@@ -14,7 +14,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvmcx.so")
-SOURCES = ["capi.hip", "slater.hip", "hubbard.hip", "sr.hip", "chol.hip", "minres.hip", "nnbf.hip", "fpeps.hip", "proposal.hip", "syrk.hip", "symmetry.hip", "noalloc.hip"]
+SOURCES = ["capi.hip", "slater.hip", "hubbard.hip", "sr.hip", "chol.hip", "minres.hip", "nnbf.hip", "fpeps.hip", "bmps.hip", "proposal.hip", "syrk.hip", "symmetry.hip", "noalloc.hip"]
 ARCH = os.environ.get("VMCX_OFFLOAD_ARCH", "gfx950")
 
 

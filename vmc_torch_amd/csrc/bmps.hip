@@ -1,0 +1,779 @@
+// Chi-truncated boundary-MPS contraction of fPEPS / NN-fTNS amplitudes on
+// gfx950: the reference's fPEPS_Model_GPU.amplitude with max_bond = chi
+// (GPU/models/pureTNS.py:81-146: quimb contract_boundary_from_xmin over rows
+// 0 .. Lx/2-1 and from_xmax over rows Lx-1 .. Lx/2, mode "mps", cutoff 0,
+// canonize, then the two boundaries contracted exactly).
+//
+// Signed slices S_s[l,u,d,r] as in fpeps.hip (the sign convention of that
+// file).  Each side is a boundary MPS B_y[a][v][b] along the row (a, b:
+// boundary bonds, v: the open vertical leg).  Absorbing a row multiplies
+// the bonds by the PEPS bond D:
+//   A_y[(a,l)][v][(b,r)] = sum_u B_y[a][u][b] S_y[l][u][v][r]
+// (u: the slice's leg facing the boundary, v: its open leg).  When a bond
+// then exceeds chi, the MPS is truncated left to right onto the chi
+// leading Schmidt vectors of each cut (the result of canonizing from the
+// right and truncating by SVD from the left) by the density-matrix form:
+//   right Grams  R_y = sum_v A_v R_{y+1} A_v^T   (y = Ly-1 .. 1, R_Ly = 1),
+//   then for y = 0 .. Ly-2, with P the projection carried from the left:
+//     C[(k,v)][(b,r)] = sum_{(a,l)} P[k][(a,l)] A_y[(a,l)][v][(b,r)],
+//     rho = C R_{y+1} C^T, its chi leading eigenvectors U,
+//     B'_y[k][v][j] = U[(k,v)][j],  P <- U^T C;
+//   and B'_{Ly-1} = C.
+// The tensors are Z2-graded, so rho is block diagonal in the parity of
+// (k, v) (par(k) from the sector each kept eigenvector came from); each
+// block is diagonalised by cyclic two-sided Jacobi in LDS (parallel
+// round-robin ordering, every rotation of a round applied at once as 2 x 2
+// block updates J^T rho J).  Norms are carried as FP64 log scales.
+//
+// One workgroup of 256 threads per (configuration, side); the Grams and the
+// boundary tensors live in a per-slot workspace in HBM, the 64 x 64 working
+// matrices in LDS.  A second kernel joins the two boundaries (one wave per
+// configuration).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "kernels.h"
+
+namespace vmcx {
+
+namespace {
+
+constexpr int BT = 256;         // threads per (configuration, side)
+constexpr int BN = 64;          // largest absorbed bond (chi * D)
+constexpr int BLD = 65;         // LDS row stride of the BN x BN buffers
+constexpr int BMAX_NS = 64;     // sites
+constexpr int BMAX_LY = 16;
+constexpr int BMAX_TS = 2048;   // largest stored tensor cb * D * cb
+constexpr int BMAX_CHI = 32;
+constexpr int BMAX_SWEEPS = 30;
+constexpr int BHDR = 32;        // int64 header words per slot: log scale, bond dims
+
+struct BmGeo {
+  int Lx, Ly, D, N, chi, cb;    // cb = max(chi, D): bound of a stored bond
+  const int32_t *dmap;
+  const double *theta;
+  int64_t theta_stride;
+  int64_t slot;                 // doubles per (configuration, side)
+  int64_t ts;                   // doubles per stored tensor, cb * D * cb
+};
+
+__device__ __forceinline__ int bqpar(int k, int d) { return d == 1 ? 0 : (k >= (d >> 1)); }
+__device__ __forceinline__ int hdim(const BmGeo &g, int y) {
+  return (y == 0 || y == g.Ly) ? 1 : g.D;
+}
+
+struct BmCfg {
+  int p[BMAX_NS], pr[BMAX_NS], doff[BMAX_NS];
+};
+
+__device__ void bm_load_cfg(const BmGeo &g, const int64_t *c, BmCfg &cs) {
+  const int tid = threadIdx.x;
+  if (tid < g.N) cs.p[tid] = (int)c[tid];
+  __syncthreads();
+  if (tid < g.N) {
+    const int x = tid / g.Ly, y = tid - x * g.Ly;
+    int pr = 0;
+    for (int yy = y + 1; yy < g.Ly; ++yy) {
+      const int v = cs.p[x * g.Ly + yy];
+      pr += (v == 1 || v == 2);
+    }
+    cs.pr[tid] = pr & 1;
+  }
+  if (tid == 0) {
+    int off = 0;
+    for (int s = 0; s < g.N; ++s) {
+      const int x = s / g.Ly, y = s - x * g.Ly;
+      const int dl = y == 0 ? 1 : g.D, du = x == 0 ? 1 : g.D;
+      const int dd = x == g.Lx - 1 ? 1 : g.D, dr = y == g.Ly - 1 ? 1 : g.D;
+      cs.doff[s] = off;
+      off += 4 * dl * du * dd * dr;
+    }
+  }
+  __syncthreads();
+}
+
+// signed slice of site (x, y) as dst[l][f][o][r]: f the leg facing the
+// boundary (up from the top side, down from the bottom), o the open leg
+__device__ void bm_slice(const BmGeo &g, const BmCfg &cs, const double *__restrict__ th, int x,
+                         int y, bool down, double *dst) {
+  const int dl = y == 0 ? 1 : g.D, du = x == 0 ? 1 : g.D;
+  const int dd = x == g.Lx - 1 ? 1 : g.D, dr = y == g.Ly - 1 ? 1 : g.D;
+  const int s = x * g.Ly + y;
+  const int n = dl * du * dd * dr;
+  const int base = cs.doff[s] + cs.p[s] * n;
+  const int pr = cs.pr[s];
+  for (int idx = threadIdx.x; idx < n; idx += BT) {
+    const int r = idx % dr, d = (idx / dr) % dd, u = (idx / (dr * dd)) % du;
+    const int l = idx / (dr * dd * du);
+    const int k = g.dmap[base + idx];
+    double v = k >= 0 ? th[k] : 0.0;
+    if ((bqpar(d, dd) * (bqpar(r, dr) + pr)) & 1) v = -v;
+    if (down)
+      dst[((l * du + u) * dd + d) * dr + r] = v;
+    else
+      dst[((l * dd + d) * du + u) * dr + r] = v;
+  }
+}
+
+__device__ double bm_max(double v, double *red) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double m = red[0];
+  for (int i = 1; i < BT / 64; ++i) m = fmax(m, red[i]);
+  __syncthreads();
+  return m;
+}
+
+struct BmSh {
+  double R[BN * BLD];       // right Gram; then rho
+  double A[BN * BLD];       // A_v; then C
+  double T[BN * BLD];       // A_v R; then W, G, V
+  double Bt[BMAX_TS];       // the boundary tensor of the current site
+  double Sl[256];           // its slice [l][f][o][r]
+  double P[BMAX_CHI * BN];  // projection carried from the left
+  double jc[BN / 2 + 1], js[BN / 2 + 1];
+  int jp[BN / 2 + 1], jq[BN / 2 + 1];
+  int perm[BN], pos[BN], sel[BN];
+  int kpar[BN];             // parity of the current left bond's indices
+  int bd[BMAX_LY + 1], nbd[BMAX_LY + 1];
+  uint64_t bpm[BMAX_LY + 1], npm[BMAX_LY + 1];   // bond parity masks
+  double red[BT / 64];
+  int sec[4];               // m0, m1, n0, n1
+  int rot;
+};
+
+// B_y (dims na x D x nb, stride cb) from the slot into Bt
+__device__ void bm_load_tensor(const BmGeo &g, const double *slot, int y, int na, int nb,
+                               double *Bt) {
+  const double *src = slot + (int64_t)y * g.ts;
+  const int n = na * g.D * nb;
+  for (int idx = threadIdx.x; idx < n; idx += BT) {
+    const int b = idx % nb, av = idx / nb;
+    Bt[idx] = src[av * g.cb + b];
+  }
+}
+
+// Bt-shaped (na x D x nb, dense) LDS tensor scaled by 1 / max into the slot;
+// returns ln max (-inf for a zero tensor)
+__device__ double bm_store_scaled(const BmGeo &g, double *slot, int y, int na, int nb,
+                                  const double *src, double *red) {
+  const int n = na * g.D * nb;
+  double m = 0.0;
+  for (int idx = threadIdx.x; idx < n; idx += BT) m = fmax(m, fabs(src[idx]));
+  m = bm_max(m, red);
+  if (!(m > 0.0) || !std::isfinite(m)) return -INFINITY;
+  const double inv = 1.0 / m;
+  double *dst = slot + (int64_t)y * g.ts;
+  for (int idx = threadIdx.x; idx < n; idx += BT) {
+    const int b = idx % nb, av = idx / nb;
+    dst[av * g.cb + b] = src[idx] * inv;
+  }
+  return log(m);
+}
+
+// out (n x m, stride BLD) = X (n x K, stride BLD) * Y (K x m, stride BLD); 4 x 4 tiles
+__device__ void bm_mm(int n, int m, int K, const double *X, const double *Y, double *out) {
+  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+  const int i0 = 4 * ti, j0 = 4 * tj;
+  if (i0 >= n || j0 >= m) return;
+  double acc[4][4] = {};
+  for (int k = 0; k < K; ++k) {
+    double xa[4], yb[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      xa[q] = (i0 + q < n) ? X[(i0 + q) * BLD + k] : 0.0;
+      yb[q] = (j0 + q < m) ? Y[k * BLD + j0 + q] : 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = fma(xa[a], yb[b], acc[a][b]);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (i0 + a < n && j0 + b < m) out[(i0 + a) * BLD + j0 + b] = acc[a][b];
+}
+
+// cyclic Jacobi on the two diagonal blocks [0, m0) and [m0, m0 + m1) of
+// S.R (sizes even; padding rows / columns zero), eigenvectors into S.T
+__device__ void bm_jacobi(BmSh &S, int m0, int m1) {
+  const int tid = threadIdx.x;
+  const int np0 = m0 >> 1, np1 = m1 >> 1, npr = np0 + np1;
+  const int mt = m0 + m1;
+  for (int idx = tid; idx < mt * mt; idx += BT) {
+    const int i = idx / mt, j = idx - i * mt;
+    S.T[i * BLD + j] = i == j ? 1.0 : 0.0;
+  }
+  const int rounds = (m0 > m1 ? m0 : m1) - 1;
+  if (rounds <= 0) {
+    __syncthreads();
+    return;
+  }
+  for (int sweep = 0; sweep < BMAX_SWEEPS; ++sweep) {
+    if (tid == 0) S.rot = 0;
+    __syncthreads();
+    for (int r = 0; r < rounds; ++r) {
+      if (tid < npr) {
+        const int b = tid >= np0;
+        const int i = b ? tid - np0 : tid;
+        const int mb = b ? m1 : m0, off = b ? m0 : 0;
+        int p = 0, q = 0;
+        double c = 1.0, s = 0.0;
+        if (mb >= 2) {
+          const int rr = r % (mb - 1);
+          if (i == 0) {
+            p = mb - 1;
+            q = rr;
+          } else {
+            p = (rr + i) % (mb - 1);
+            q = (rr - i + mb - 1) % (mb - 1);
+          }
+          p += off;
+          q += off;
+          const double app = S.R[p * BLD + p], aqq = S.R[q * BLD + q], apq = S.R[p * BLD + q];
+          if (fabs(apq) > fmax(1e-15 * sqrt(fabs(app * aqq)), 1e-30)) {
+            const double th = (aqq - app) / (2.0 * apq);
+            const double t = fabs(th) > 1e150 ? 0.5 / th
+                                              : copysign(1.0, th) / (fabs(th) + sqrt(th * th + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            s = t * c;
+            S.rot = 1;
+          }
+        }
+        S.jp[tid] = p;
+        S.jq[tid] = q;
+        S.jc[tid] = c;
+        S.js[tid] = s;
+      }
+      __syncthreads();
+      // rho <- J^T rho J, one 2 x 2 block (pair i1 rows, pair i2 columns) per item
+      const int nb0 = np0 * np0, nb1 = np1 * np1;
+      for (int wi = tid; wi < nb0 + nb1; wi += BT) {
+        int i1, i2;
+        if (wi < nb0) {
+          i1 = wi / np0;
+          i2 = wi - i1 * np0;
+        } else {
+          const int w2 = wi - nb0;
+          i1 = np0 + w2 / np1;
+          i2 = np0 + w2 % np1;
+        }
+        const double s1 = S.js[i1], s2 = S.js[i2];
+        if (s1 == 0.0 && s2 == 0.0) continue;
+        const double c1 = S.jc[i1], c2 = S.jc[i2];
+        const int p1 = S.jp[i1], q1 = S.jq[i1], p2 = S.jp[i2], q2 = S.jq[i2];
+        const double xpp = S.R[p1 * BLD + p2], xpq = S.R[p1 * BLD + q2];
+        const double xqp = S.R[q1 * BLD + p2], xqq = S.R[q1 * BLD + q2];
+        const double ypp = c1 * xpp - s1 * xqp, ypq = c1 * xpq - s1 * xqq;
+        const double yqp = s1 * xpp + c1 * xqp, yqq = s1 * xpq + c1 * xqq;
+        S.R[p1 * BLD + p2] = ypp * c2 - ypq * s2;
+        S.R[p1 * BLD + q2] = ypp * s2 + ypq * c2;
+        S.R[q1 * BLD + p2] = yqp * c2 - yqq * s2;
+        S.R[q1 * BLD + q2] = yqp * s2 + yqq * c2;
+      }
+      // V <- V J over the rows of each block
+      const int nv0 = m0 * np0, nv1 = m1 * np1;
+      for (int wi = tid; wi < nv0 + nv1; wi += BT) {
+        int row, pi;
+        if (wi < nv0) {
+          row = wi / np0;
+          pi = wi - row * np0;
+        } else {
+          const int w2 = wi - nv0;
+          row = m0 + w2 / np1;
+          pi = np0 + w2 % np1;
+        }
+        const double s = S.js[pi];
+        if (s == 0.0) continue;
+        const double c = S.jc[pi];
+        const int p = S.jp[pi], q = S.jq[pi];
+        const double vp = S.T[row * BLD + p], vq = S.T[row * BLD + q];
+        S.T[row * BLD + p] = c * vp - s * vq;
+        S.T[row * BLD + q] = s * vp + c * vq;
+      }
+      __syncthreads();
+    }
+    const int any = S.rot;
+    __syncthreads();
+    if (!any) break;
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ one side
+// grid (rows, 2): blockIdx.y = 0 the top side (rows 0 .. Lx/2-1, open down
+// legs), 1 the bottom side (rows Lx-1 .. Lx/2, open up legs).
+__global__ void __launch_bounds__(BT)
+    peps_bmps_side_kernel(BmGeo g, int64_t rows, const int64_t *__restrict__ cfg,
+                          const int32_t *__restrict__ flag, double *ws, int64_t *hdr) {
+  const int64_t w = blockIdx.x;
+  if (w >= rows) return;
+  if (flag && !(flag[w] & 1)) return;
+  const int side = blockIdx.y;
+  const bool down = side == 0;
+  const int tid = threadIdx.x;
+  __shared__ BmCfg cs;
+  __shared__ BmSh S;
+  bm_load_cfg(g, cfg + w * g.N, cs);
+  const double *th = g.theta + w * g.theta_stride;
+  double *slot = ws + (w * 2 + side) * g.slot;
+  double *Rg = slot + (int64_t)g.Ly * g.ts;
+  const int Ly = g.Ly, D = g.D, h = g.Lx / 2;
+  const int nrows = down ? h : g.Lx - h;
+  const uint64_t dmask = D == 1 ? 0ull : (((1ull << D) - 1) & ~((1ull << (D >> 1)) - 1));
+  if (tid <= Ly) {
+    S.bd[tid] = hdim(g, tid);
+    S.bpm[tid] = hdim(g, tid) == 1 ? 0ull : dmask;
+  }
+  double lg = 0.0;
+  bool dead = false;
+  // first row: B_y[l][v][r] = S[l][0][v][r]
+  {
+    const int x = down ? 0 : g.Lx - 1;
+    for (int y = 0; y < Ly && !dead; ++y) {
+      bm_slice(g, cs, th, x, y, down, S.Sl);
+      __syncthreads();
+      const int dl = hdim(g, y), dr = hdim(g, y + 1);
+      const double l2 = bm_store_scaled(g, slot, y, dl, dr, S.Sl, S.red);
+      __syncthreads();
+      if (std::isinf(l2)) dead = true;
+      lg += l2;
+    }
+  }
+  for (int ri = 1; ri < nrows && !dead; ++ri) {
+    const int x = down ? ri : g.Lx - 1 - ri;
+    if (tid <= Ly) {
+      const int hd = hdim(g, tid), b = S.bd[tid];
+      S.nbd[tid] = b * hd;
+      uint64_t m = 0;
+      for (int a = 0; a < b; ++a)
+        for (int l = 0; l < hd; ++l)
+          if ((((S.bpm[tid] >> a) & 1) ^ bqpar(l, hd))) m |= 1ull << (a * hd + l);
+      S.npm[tid] = m;
+    }
+    __syncthreads();
+    int mxb = 0;
+    for (int y = 0; y <= Ly; ++y) mxb = S.nbd[y] > mxb ? S.nbd[y] : mxb;
+    if (mxb <= g.chi) {
+      // absorb without truncation
+      for (int y = 0; y < Ly && !dead; ++y) {
+        const int na = S.bd[y], nb = S.bd[y + 1], dl = hdim(g, y), dr = hdim(g, y + 1);
+        bm_load_tensor(g, slot, y, na, nb, S.Bt);
+        bm_slice(g, cs, th, x, y, down, S.Sl);
+        __syncthreads();
+        const int ni = na * dl, nj = nb * dr, n = ni * D * nj;
+        for (int idx = tid; idx < n; idx += BT) {
+          const int j = idx % nj, v = (idx / nj) % D, i = idx / (nj * D);
+          const int a = i / dl, l = i - a * dl, b = j / dr, r = j - b * dr;
+          double s = 0.0;
+          for (int u = 0; u < D; ++u)
+            s = fma(S.Bt[(a * D + u) * nb + b], S.Sl[((l * D + u) * D + v) * dr + r], s);
+          S.A[idx] = s;
+        }
+        __syncthreads();
+        const double l2 = bm_store_scaled(g, slot, y, ni, nj, S.A, S.red);
+        __syncthreads();
+        if (std::isinf(l2)) dead = true;
+        lg += l2;
+      }
+      if (tid <= Ly) {
+        S.bd[tid] = S.nbd[tid];
+        S.bpm[tid] = S.npm[tid];
+      }
+      __syncthreads();
+      continue;
+    }
+    // ---- backward: right Grams R_y, y = Ly-1 .. 1 (R_Ly = 1)
+    if (tid == 0) S.R[0] = 1.0;
+    for (int y = Ly - 1; y >= 1; --y) {
+      const int na = S.bd[y], nb = S.bd[y + 1], dl = hdim(g, y), dr = hdim(g, y + 1);
+      const int n = na * dl, m = nb * dr;
+      bm_load_tensor(g, slot, y, na, nb, S.Bt);
+      bm_slice(g, cs, th, x, y, down, S.Sl);
+      __syncthreads();
+      const int ti = tid >> 4, tj = tid & 15, i0 = 4 * ti, j0 = 4 * tj;
+      double acc[4][4] = {};
+      for (int v = 0; v < D; ++v) {
+        for (int idx = tid; idx < n * m; idx += BT) {
+          const int i = idx / m, j = idx - i * m;
+          const int a = i / dl, l = i - a * dl, b = j / dr, r = j - b * dr;
+          double s = 0.0;
+          for (int u = 0; u < D; ++u)
+            s = fma(S.Bt[(a * D + u) * nb + b], S.Sl[((l * D + u) * D + v) * dr + r], s);
+          S.A[i * BLD + j] = s;
+        }
+        __syncthreads();
+        bm_mm(n, m, m, S.A, S.R, S.T);
+        __syncthreads();
+        if (i0 < n && j0 < n) {
+          for (int k = 0; k < m; ++k) {
+            double xa[4], yb[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              xa[q] = (i0 + q < n) ? S.T[(i0 + q) * BLD + k] : 0.0;
+              yb[q] = (j0 + q < n) ? S.A[(j0 + q) * BLD + k] : 0.0;
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+              for (int b = 0; b < 4; ++b) acc[a][b] = fma(xa[a], yb[b], acc[a][b]);
+          }
+        }
+        __syncthreads();
+      }
+      double mx = 0.0;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) mx = fmax(mx, fabs(acc[a][b]));
+      mx = bm_max(mx, S.red);
+      const double inv = mx > 0.0 ? 1.0 / mx : 0.0;
+      double *Rd = Rg + (int64_t)y * BN * BN;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (i0 + a < n && j0 + b < n) {
+            const double v = acc[a][b] * inv;
+            S.R[(i0 + a) * BLD + j0 + b] = v;
+            Rd[(i0 + a) * BN + j0 + b] = v;
+          }
+      __syncthreads();
+    }
+    // ---- forward: truncation left to right
+    if (tid == 0) {
+      S.P[0] = 1.0;
+      S.kpar[0] = 0;
+      S.nbd[0] = 1;
+      S.npm[0] = 0;
+    }
+    __syncthreads();
+    int k = 1;
+    for (int y = 0; y < Ly && !dead; ++y) {
+      const int na = S.bd[y], nb = S.bd[y + 1], dl = hdim(g, y), dr = hdim(g, y + 1);
+      const int m = nb * dr, nC = k * D;
+      bm_load_tensor(g, slot, y, na, nb, S.Bt);
+      bm_slice(g, cs, th, x, y, down, S.Sl);
+      __syncthreads();
+      // W[kk][l][u][b] = sum_a P[kk][(a,l)] B[a][u][b]  (into T)
+      const int nw = k * dl * D * nb;
+      for (int idx = tid; idx < nw; idx += BT) {
+        const int b = idx % nb, u = (idx / nb) % D, l = (idx / (nb * D)) % dl;
+        const int kk = idx / (nb * D * dl);
+        double s = 0.0;
+        for (int a = 0; a < na; ++a)
+          s = fma(S.P[kk * BN + a * dl + l], S.Bt[(a * D + u) * nb + b], s);
+        S.T[idx] = s;
+      }
+      __syncthreads();
+      // C[(kk,v)][(b,r)] = sum_{l,u} W[kk][l][u][b] S[l][u][v][r]  (into A)
+      for (int idx = tid; idx < nC * m; idx += BT) {
+        const int j = idx % m, i = idx / m;
+        const int kk = i / D, v = i - kk * D, b = j / dr, r = j - b * dr;
+        double s = 0.0;
+        for (int l = 0; l < dl; ++l)
+          for (int u = 0; u < D; ++u)
+            s = fma(S.T[((kk * dl + l) * D + u) * nb + b], S.Sl[((l * D + u) * D + v) * dr + r],
+                    s);
+        S.A[i * BLD + j] = s;
+      }
+      __syncthreads();
+      if (y == Ly - 1) {
+        // last site: B'[kk][v][0] = C[(kk,v)][0]
+        for (int idx = tid; idx < nC; idx += BT) S.T[idx] = S.A[idx * BLD];
+        __syncthreads();
+        const double l2 = bm_store_scaled(g, slot, y, k, 1, S.T, S.red);
+        __syncthreads();
+        if (std::isinf(l2)) dead = true;
+        lg += l2;
+        if (tid == 0) {
+          S.nbd[Ly] = 1;
+          S.npm[Ly] = 0;
+        }
+        break;
+      }
+      double *dst = slot + (int64_t)y * g.ts;
+      int kn;
+      if (nC <= g.chi) {
+        // no truncation at this bond: the identity basis, P = C
+        kn = nC;
+        for (int idx = tid; idx < k * D * kn; idx += BT) {
+          const int j = idx % kn, kv = idx / kn;
+          dst[kv * g.cb + j] = j == kv ? 1.0 : 0.0;
+        }
+        for (int idx = tid; idx < kn * m; idx += BT) {
+          const int j = idx % m, i = idx / m;
+          S.P[i * BN + j] = S.A[i * BLD + j];
+        }
+        __syncthreads();
+        if (tid < kn) S.sel[tid] = S.kpar[tid / D] ^ bqpar(tid % D, D);
+        __syncthreads();
+        if (tid < kn) S.kpar[tid] = S.sel[tid];
+      } else {
+        kn = g.chi;
+        // R_{y+1} from the slot
+        const double *Rs = Rg + (int64_t)(y + 1) * BN * BN;
+        for (int idx = tid; idx < m * m; idx += BT) {
+          const int i = idx / m, j = idx - i * m;
+          S.R[i * BLD + j] = Rs[i * BN + j];
+        }
+        if (tid == 0) {
+          int n0 = 0;
+          for (int i = 0; i < nC; ++i) n0 += !(S.kpar[i / D] ^ bqpar(i % D, D));
+          const int n1 = nC - n0, m0 = n0 + (n0 & 1), m1 = n1 + (n1 & 1);
+          int c0 = 0, c1 = m0;
+          for (int i = 0; i < nC; ++i) {
+            const int sct = S.kpar[i / D] ^ bqpar(i % D, D);
+            const int pi = sct ? c1++ : c0++;
+            S.perm[pi] = i;
+            S.pos[i] = pi;
+          }
+          if (n0 & 1) S.perm[m0 - 1] = -1;
+          if (n1 & 1) S.perm[m0 + m1 - 1] = -1;
+          S.sec[0] = m0;
+          S.sec[1] = m1;
+        }
+        __syncthreads();
+        const int m0 = S.sec[0], m1 = S.sec[1], mt = m0 + m1;
+        // G = C R (into T)
+        bm_mm(nC, m, m, S.A, S.R, S.T);
+        __syncthreads();
+        // rho (permuted, block diagonal) = G C^T (into R)
+        for (int idx = tid; idx < mt * mt; idx += BT) {
+          const int p1 = idx / mt, p2 = idx - p1 * mt;
+          const int i1 = S.perm[p1], i2 = S.perm[p2];
+          double s = 0.0;
+          if (i1 >= 0 && i2 >= 0 && ((p1 < m0) == (p2 < m0)))
+            for (int j = 0; j < m; ++j) s = fma(S.T[i1 * BLD + j], S.A[i2 * BLD + j], s);
+          S.R[p1 * BLD + p2] = s;
+        }
+        __syncthreads();
+        double dmx = 0.0;
+        for (int i = tid; i < mt; i += BT) dmx = fmax(dmx, fabs(S.R[i * BLD + i]));
+        dmx = bm_max(dmx, S.red);
+        if (!(dmx > 0.0)) {
+          dead = true;
+          break;
+        }
+        const double dinv = 1.0 / dmx;
+        for (int idx = tid; idx < mt * mt; idx += BT) {
+          const int p1 = idx / mt, p2 = idx - p1 * mt;
+          S.R[p1 * BLD + p2] *= dinv;
+        }
+        __syncthreads();
+        bm_jacobi(S, m0, m1);
+        // the chi leading eigenvalues (ties by position), padding excluded
+        if (tid < mt && S.perm[tid] >= 0) {
+          const double lam = S.R[tid * BLD + tid];
+          int rank = 0;
+          for (int q = 0; q < mt; ++q) {
+            if (S.perm[q] < 0) continue;
+            const double lq = S.R[q * BLD + q];
+            rank += (lq > lam) || (lq == lam && q < tid);
+          }
+          if (rank < kn) S.sel[rank] = tid;
+        }
+        __syncthreads();
+        // B'[kk][v][j] = V[pos(kk,v)][sel j]
+        for (int idx = tid; idx < nC * kn; idx += BT) {
+          const int j = idx % kn, kv = idx / kn;
+          dst[kv * g.cb + j] = S.T[S.pos[kv] * BLD + S.sel[j]];
+        }
+        // P'[j][(b,r)] = sum_pi V[pi][sel j] C[perm pi][(b,r)]
+        for (int idx = tid; idx < kn * m; idx += BT) {
+          const int jj = idx % m, j = idx / m;
+          const int sj = S.sel[j];
+          const int lo = sj < m0 ? 0 : m0, hi = sj < m0 ? m0 : mt;
+          double s = 0.0;
+          for (int pi = lo; pi < hi; ++pi) {
+            const int i = S.perm[pi];
+            if (i >= 0) s = fma(S.T[pi * BLD + sj], S.A[i * BLD + jj], s);
+          }
+          S.P[j * BN + jj] = s;
+        }
+        __syncthreads();
+        if (tid < kn) S.kpar[tid] = S.sel[tid] >= m0;
+      }
+      __syncthreads();
+      // renormalise P
+      double pm = 0.0;
+      for (int idx = tid; idx < kn * m; idx += BT) {
+        const int j = idx % m, i = idx / m;
+        pm = fmax(pm, fabs(S.P[i * BN + j]));
+      }
+      pm = bm_max(pm, S.red);
+      if (!(pm > 0.0) || !std::isfinite(pm)) {
+        dead = true;
+        break;
+      }
+      const double pinv = 1.0 / pm;
+      for (int idx = tid; idx < kn * m; idx += BT) {
+        const int j = idx % m, i = idx / m;
+        S.P[i * BN + j] *= pinv;
+      }
+      lg += log(pm);
+      if (tid == 0) {
+        S.nbd[y + 1] = kn;
+        uint64_t msk = 0;
+        for (int j = 0; j < kn; ++j) msk |= (uint64_t)S.kpar[j] << j;
+        S.npm[y + 1] = msk;
+      }
+      __syncthreads();
+      k = kn;
+    }
+    __syncthreads();
+    if (tid <= Ly) {
+      S.bd[tid] = S.nbd[tid];
+      S.bpm[tid] = S.npm[tid];
+    }
+    __syncthreads();
+  }
+  if (tid <= Ly) hdr[(w * 2 + side) * BHDR + 1 + tid] = S.bd[tid];
+  if (tid == 0) {
+    const double v = dead ? -INFINITY : lg;
+    hdr[(w * 2 + side) * BHDR] = __double_as_longlong(v);
+  }
+}
+
+// ------------------------------------------------------------ join
+__global__ void __launch_bounds__(64)
+    peps_bmps_join_kernel(BmGeo g, int64_t rows, const int64_t *__restrict__ cfg,
+                          const int32_t *__restrict__ flag, const double *__restrict__ ws,
+                          const int64_t *__restrict__ hdr, double *sign, double *logabs) {
+  const int64_t w = blockIdx.x;
+  if (w >= rows) return;
+  if (flag && !(flag[w] & 1)) return;
+  const int tid = threadIdx.x;
+  __shared__ double E[BMAX_CHI * BMAX_CHI], X[BMAX_TS];
+  const int64_t *ht = hdr + (w * 2) * BHDR, *hb = hdr + (w * 2 + 1) * BHDR;
+  const double lt = __longlong_as_double(ht[0]), lb = __longlong_as_double(hb[0]);
+  const double lo = log(1e-45);
+  int P = 0;
+  for (int s = 0; s < g.N; ++s) {
+    const int v = (int)cfg[w * g.N + s];
+    P += (v == 1 || v == 2);
+  }
+  if (std::isinf(lt) || std::isinf(lb)) {
+    if (tid == 0) {
+      sign[w] = 0.0;
+      logabs[w] = lo;
+    }
+    return;
+  }
+  const double *Tt = ws + (w * 2) * g.slot, *Tb = ws + (w * 2 + 1) * g.slot;
+  const int D = g.D, cb = g.cb;
+  if (tid == 0) E[0] = 1.0;
+  __syncthreads();
+  double lsc = 0.0;
+  for (int y = 0; y < g.Ly; ++y) {
+    const int ta = (int)ht[1 + y], tb = (int)ht[2 + y], ba = (int)hb[1 + y], bb = (int)hb[2 + y];
+    const double *T = Tt + (int64_t)y * g.ts, *Bo = Tb + (int64_t)y * g.ts;
+    // X[a'][v][b] = sum_a E[a][a'] T[a][v][b]
+    for (int idx = tid; idx < ba * D * tb; idx += 64) {
+      const int b = idx % tb, v = (idx / tb) % D, ap = idx / (tb * D);
+      double s = 0.0;
+      for (int a = 0; a < ta; ++a) s = fma(E[a * BMAX_CHI + ap], T[(a * D + v) * cb + b], s);
+      X[idx] = s;
+    }
+    __syncthreads();
+    // E'[b][b'] = sum_{a',v} X[a'][v][b] Bo[a'][v][b']
+    double mx = 0.0;
+    for (int idx = tid; idx < tb * bb; idx += 64) {
+      const int bp = idx % bb, b = idx / bb;
+      double s = 0.0;
+      for (int ap = 0; ap < ba; ++ap)
+        for (int v = 0; v < D; ++v) s = fma(X[(ap * D + v) * tb + b], Bo[(ap * D + v) * cb + bp], s);
+      E[b * BMAX_CHI + bp] = s;
+      mx = fmax(mx, fabs(s));
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    __syncthreads();
+    if (!(mx > 0.0) || !std::isfinite(mx)) {
+      if (tid == 0) {
+        sign[w] = 0.0;
+        logabs[w] = lo;
+      }
+      return;
+    }
+    const double inv = 1.0 / mx;
+    for (int idx = tid; idx < tb * bb; idx += 64) {
+      const int bp = idx % bb, b = idx / bb;
+      E[b * BMAX_CHI + bp] *= inv;
+    }
+    lsc += log(mx);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const double v = E[0] * (((P * (P - 1) / 2) & 1) ? -1.0 : 1.0);
+    if (v == 0.0 || !std::isfinite(v)) {
+      sign[w] = 0.0;
+      logabs[w] = lo;
+    } else {
+      sign[w] = v > 0.0 ? 1.0 : -1.0;
+      logabs[w] = fmax(log(fabs(v)) + lsc + lt + lb, lo);
+    }
+  }
+}
+
+// ------------------------------------------------------------ host side
+namespace {
+BmGeo bm_geo(int Lx, int Ly, int D, int chi) {
+  BmGeo g{};
+  g.Lx = Lx;
+  g.Ly = Ly;
+  g.D = D;
+  g.N = Lx * Ly;
+  g.chi = chi;
+  g.cb = chi > D ? chi : D;
+  g.ts = (int64_t)g.cb * D * g.cb;
+  g.slot = (int64_t)Ly * g.ts + (int64_t)Ly * BN * BN;
+  return g;
+}
+}  // namespace
+
+const char *bmps_check(int Lx, int Ly, int D, int chi) {
+  if (chi < 1 || chi > BMAX_CHI) return "max_bond (chi) must be 1..32";
+  if (Lx < 2 || Lx > 16 || Ly < 1 || Ly > BMAX_LY || Lx * Ly > BMAX_NS)
+    return "boundary MPS needs 2 <= Lx <= 16, 1 <= Ly <= 16, Lx * Ly <= 64";
+  if (D < 1 || D > 4) return "D must be 1..4";
+  const int cb = chi > D ? chi : D;
+  if (cb * D > BN || cb * D * cb > BMAX_TS) return "max(chi, D) * D must be <= 64";
+  return nullptr;
+}
+
+int64_t bmps_workspace_bytes(int Lx, int Ly, int D, int chi, int64_t rows) {
+  const BmGeo g = bm_geo(Lx, Ly, D, chi);
+  return rows * 2 * (g.slot * 8 + BHDR * 8);
+}
+
+int bmps_amplitudes(int Lx, int Ly, int D, int chi, const int32_t *dmap, const double *theta,
+                    int64_t theta_stride, int64_t rows, const int64_t *cfg, const int32_t *flag,
+                    double *sign, double *logabs, void *ws, int64_t rows_per_pass,
+                    hipStream_t st) {
+  BmGeo g = bm_geo(Lx, Ly, D, chi);
+  g.dmap = dmap;
+  double *slots = (double *)ws;
+  int64_t *hdr = (int64_t *)(slots + rows_per_pass * 2 * g.slot);
+  for (int64_t r0 = 0; r0 < rows; r0 += rows_per_pass) {
+    const int64_t n = rows - r0 < rows_per_pass ? rows - r0 : rows_per_pass;
+    g.theta = theta + r0 * theta_stride;
+    g.theta_stride = theta_stride;
+    const int64_t *c = cfg + r0 * g.N;
+    const int32_t *f = flag ? flag + r0 : nullptr;
+    hipLaunchKernelGGL(peps_bmps_side_kernel, dim3((unsigned)n, 2), dim3(BT), 0, st, g, n, c, f,
+                       slots, hdr);
+    VMCX_LAUNCHED();
+    hipLaunchKernelGGL(peps_bmps_join_kernel, dim3((unsigned)n), dim3(64), 0, st, g, n, c, f,
+                       slots, hdr, sign + r0, logabs + r0);
+    VMCX_LAUNCHED();
+  }
+  return 0;
+}
+
+}  // namespace vmcx

@@ -60,7 +60,7 @@ static inline int status_of(int rc) {
 
 extern "C" {
 
-int vmcx_abi_version(void) { return 18; }
+int vmcx_abi_version(void) { return 19; }
 
 const char *vmcx_last_error(void) { return vmcx::g_last_error.c_str(); }
 

@@ -37,6 +37,7 @@ namespace vmcx {
 namespace {
 
 constexpr int PMAX_LX = 4, PMAX_LY = 8, PMAX_D = 4, PMAX_N = PMAX_LX * PMAX_LY;
+constexpr int PMAX_NS = 64;         // sites of a boundary-MPS lattice (the MLP kernels)
 constexpr int PB = 16;              // max bond of a half (D^2)
 constexpr int PT = 256;             // threads per configuration
 constexpr int SLICE = PMAX_D * PMAX_D * PMAX_D * PMAX_D;
@@ -927,7 +928,7 @@ __global__ void __launch_bounds__(256)
                       double *z1, double *h1) {
   const int64_t w = blockIdx.x;
   if (w >= rows) return;
-  __shared__ int occ[2 * PMAX_N];
+  __shared__ int occ[2 * PMAX_NS];
   __shared__ int nocc;
   if (threadIdx.x == 0) {
     int k = 0;
@@ -1020,7 +1021,7 @@ __global__ void __launch_bounds__(256)
   if (w >= rows) return;
   extern __shared__ double dsm[];
   double *h1 = dsm, *h2 = h1 + H, *d1 = h2 + H, *d2 = d1 + H;
-  __shared__ float nvec[2 * PMAX_N];
+  __shared__ float nvec[2 * PMAX_NS];
   const int n_orb = 2 * N;
   const int64_t *c = cfg + w * N;
   for (int o = threadIdx.x; o < n_orb; o += blockDim.x)
@@ -1134,6 +1135,9 @@ struct PepsWs {
   double *coeffs, *csign, *clog;
   // FP32 amplitudes of a shared parameter vector: the signed slice table
   float *tbl;
+  // chi-truncated boundary MPS: slots of bm_pass configurations
+  void *bm;
+  int64_t bm_pass;
   // environment reuse: the walkers' column environments, and the walker and
   // hop of each connected configuration
   double *env;
@@ -1143,6 +1147,20 @@ struct PepsWs {
 };
 
 size_t alb(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// configurations per pass of the boundary-MPS kernels (their workspace)
+constexpr int64_t BM_PASS = 2048;
+
+// the chi-truncated boundary MPS instead of the exact column contraction:
+// chi > 0 and either the lattice is beyond the exact kernels or chi is below
+// the largest bond of a side (D^rows)
+bool use_bmps(const vmcx_peps_model &m) {
+  if (m.chi <= 0) return false;
+  if (m.Lx > PMAX_LX || m.Ly > PMAX_LY || m.Lx * m.Ly > PMAX_N) return true;
+  int64_t big = 1;
+  for (int i = 0; i < m.Lx - m.Lx / 2; ++i) big *= m.D;
+  return m.chi < big;
+}
 
 PepsWs peps_layout(const vmcx_peps_model &m, int64_t B, int64_t chunk, int n_edges,
                    char *base, bool reuse = false) {
@@ -1181,6 +1199,10 @@ PepsWs peps_layout(const vmcx_peps_model &m, int64_t B, int64_t chunk, int n_edg
   L.csign = (double *)take((size_t)T * 8);
   L.clog = (double *)take((size_t)T * 8);
   L.tbl = (float *)take(m.amp_fp32 && H == 0 ? (size_t)N * 8 * 256 * 4 : 0);
+  if (use_bmps(m)) {
+    L.bm_pass = rows < BM_PASS ? (rows > 0 ? rows : 1) : BM_PASS;
+    L.bm = take((size_t)bmps_workspace_bytes(m.Lx, m.Ly, m.D, m.chi, L.bm_pass));
+  }
   if (reuse) {
     L.env = (double *)take((size_t)B * env_stride(m.Ly) * 8);
     L.cparent = (int32_t *)take((size_t)T * 4);
@@ -1192,9 +1214,18 @@ PepsWs peps_layout(const vmcx_peps_model &m, int64_t B, int64_t chunk, int n_edg
 
 int check_peps(const vmcx_peps_model *m) {
   VMCX_CHECK_ARG(m != nullptr, "null model");
-  VMCX_CHECK_ARG(m->Lx >= 2 && m->Lx <= PMAX_LX, "Lx must be 2..4 (exact boundary contraction)");
-  VMCX_CHECK_ARG(m->Ly >= 1 && m->Ly <= PMAX_LY, "Ly must be 1..8");
   VMCX_CHECK_ARG(m->D >= 1 && m->D <= PMAX_D, "D must be 1..4");
+  if (use_bmps(*m)) {
+    const char *why = bmps_check(m->Lx, m->Ly, m->D, m->chi);
+    VMCX_CHECK_ARG(why == nullptr, why ? why : "");
+    VMCX_CHECK_ARG(!m->amp_fp32,
+                   "FP32 amplitudes need the exact contraction (Lx <= 4, chi <= 0 or chi >= "
+                   "D^rows)");
+  } else {
+    VMCX_CHECK_ARG(m->Lx >= 2 && m->Lx <= PMAX_LX,
+                   "Lx must be 2..4 for the exact contraction (chi > 0 selects the boundary MPS)");
+    VMCX_CHECK_ARG(m->Ly >= 1 && m->Ly <= PMAX_LY, "Ly must be 1..8 for the exact contraction");
+  }
   VMCX_CHECK_ARG(m->n_params > 0 && m->dense_map && m->theta, "missing parameters");
   VMCX_CHECK_ARG(m->hidden >= 0 && m->hidden <= 1024, "hidden must be 0..1024");
   VMCX_CHECK_ARG(m->act == 0 || m->act == 1, "act must be 0 (ReLU) or 1 (tanh)");
@@ -1274,6 +1305,9 @@ int amp_rows(const vmcx_peps_model &m, const PepsWs &L, int64_t rows, const int6
     if (rc) return rc;
     A = args_of(m, L.theta_eff, m.n_params);
   }
+  if (use_bmps(m))
+    return bmps_amplitudes(m.Lx, m.Ly, m.D, m.chi, A.dmap, A.theta, A.theta_stride, rows, cfg,
+                           flag, sign, logabs, L.bm, L.bm_pass, st);
   if (m.amp_fp32) {
     const float *tbl = nullptr;
     if (m.hidden == 0) {
@@ -1326,6 +1360,7 @@ namespace {
 
 int check_reuse(const vmcx_peps_model *m) {
   if (check_peps(m)) return 1;
+  VMCX_CHECK_ARG(!use_bmps(*m), "environment reuse needs the exact contraction (chi <= 0)");
   VMCX_CHECK_ARG(m->hidden == 0,
                  "environment reuse needs site tensors that do not depend on the "
                  "configuration (plain fPEPS, hidden = 0)");
@@ -1569,6 +1604,9 @@ static int peps_logderiv_impl(const vmcx_peps_model *m, int64_t B, const int64_t
                               int64_t ld, bool fact, double *sign, double *logabs,
                               hipStream_t st) {
   if (check_peps(m)) return 1;
+  VMCX_CHECK_ARG(!use_bmps(*m),
+                 "the log-derivative of the chi-truncated boundary MPS is not built (chi <= 0 "
+                 "or chi >= D^rows for the exact gradients)");
   VMCX_CHECK_ARG(B >= 0 && chunk >= 1, "bad sizes");
   VMCX_CHECK_ARG(o_dtype == 0 || o_dtype == 1, "o_dtype must be 0 (f64) or 1 (f32)");
   if (B == 0) return 0;

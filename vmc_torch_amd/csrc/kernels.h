@@ -164,6 +164,16 @@ int peps_logderiv_factors(const vmcx_peps_model *m, int64_t B, const int64_t *cf
                           int64_t ws_bytes, int64_t chunk, double *F, int64_t ld,
                           double *sign, double *logabs, hipStream_t st);
 
+// chi-truncated boundary MPS (bmps.hip): nullptr or the reason the shape is
+// not supported; workspace for rows_per_pass configurations; amplitudes of
+// `rows` configurations in passes of rows_per_pass
+const char *bmps_check(int Lx, int Ly, int D, int chi);
+int64_t bmps_workspace_bytes(int Lx, int Ly, int D, int chi, int64_t rows_per_pass);
+int bmps_amplitudes(int Lx, int Ly, int D, int chi, const int32_t *dmap, const double *theta,
+                    int64_t theta_stride, int64_t rows, const int64_t *cfg, const int32_t *flag,
+                    double *sign, double *logabs, void *ws, int64_t rows_per_pass,
+                    hipStream_t st);
+
 int lp_n_params(int C);
 int lp_n_moves(int Lx, int Ly);
 int lp_logits(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,

@@ -540,13 +540,15 @@ def peps_layout(Lx, Ly, D):
     return np.concatenate(maps), sizes
 
 
-def peps_struct(Lx, Ly, D, dense_map, theta, nn=None, act=0, eta=0.0, mode=0, amp_fp32=False):
+def peps_struct(Lx, Ly, D, dense_map, theta, nn=None, act=0, eta=0.0, mode=0, amp_fp32=False,
+                chi=0):
     """vmcx_peps_model from device tensors: dense_map int32, theta float64,
     nn = dict W1, b1, W2, b2, Wo, bo (float64) or None; mode 0 = additive,
     1 = multiplicative network correction; amp_fp32: amplitudes contracted
-    in FP32 on the matrix cores."""
+    in FP32 on the matrix cores; chi > 0: max_bond of the boundary MPS."""
     s = _lib.PepsModelStruct()
     s.mode, s.amp_fp32 = int(mode), 1 if amp_fp32 else 0
+    s.chi, s.reserved = max(int(chi), 0), 0
     require(dense_map, I32, "dense_map")
     require(theta, F64, "theta")
     s.Lx, s.Ly, s.D, s.n_params = int(Lx), int(Ly), int(D), int(theta.numel())

@@ -473,9 +473,27 @@ class NNBF_GPU(WavefunctionModel_GPU):
         return kernels.FactoredO(F, factors, blocks, self.n_params), (sign, logabs)
 
 
+def uses_bmps(Lx, Ly, D, max_bond):
+    """True when fPEPS amplitudes go through the chi-truncated boundary MPS:
+    max_bond > 0 and either the lattice is beyond the exact column
+    contraction (Lx <= 4, Ly <= 8) or max_bond is below the largest bond of
+    a side, D^(Lx - Lx // 2)."""
+    if max_bond <= 0:
+        return False
+    if Lx > 4 or Ly > 8:
+        return True
+    return max_bond < D ** (Lx - Lx // 2)
+
+
 class fPEPS_Model_GPU(WavefunctionModel_GPU):
-    """Z2-symmetric fermionic PEPS with exact boundary contraction
-    (GPU/models/pureTNS.py:24-156, chi <= 0 / chi >= D^2 on Lx <= 4).
+    """Z2-symmetric fermionic PEPS (GPU/models/pureTNS.py:24-156).
+    max_bond <= 0 (or >= D^2 on Lx <= 4): exact contraction, column by
+    column.  Otherwise the reference's boundary MPS with max_bond = chi
+    (pureTNS.py:81-146): rows 0 .. Lx/2-1 from the top, Lx-1 .. Lx/2 from the
+    bottom, each truncated left to right onto its chi leading Schmidt
+    vectors after every absorbed row, the two boundaries then contracted
+    exactly (chi <= 32, max(chi, D) * D <= 64, Lx * Ly <= 64, Ly <= 16;
+    FP64 amplitudes: forward, sweep, local energy).
 
     Parameters: one flat FP64 vector of the Z2-even entries of the site
     tensors T_s[p, l, u, d, r] (site-major, row-major), the layout of
@@ -499,8 +517,19 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
                              "contraction of the amplitudes)")
         if amp not in self.AMPS:
             raise ValueError(f"amp must be one of {self.AMPS}")
-        if not (2 <= Lx <= 4 and 1 <= Ly <= 8 and 1 <= D <= 4):
-            raise NotImplementedError("exact contraction needs 2 <= Lx <= 4, Ly <= 8, D <= 4")
+        self._bmps = uses_bmps(Lx, Ly, D, max_bond)
+        if self._bmps:
+            if amp != "fp64":
+                raise ValueError("the boundary-MPS contraction (max_bond > 0) is FP64")
+            cb = max(max_bond, D)
+            if not (2 <= Lx <= 16 and 1 <= Ly <= 16 and Lx * Ly <= 64 and 1 <= D <= 4
+                    and max_bond <= 32 and cb * D <= 64):
+                raise NotImplementedError(
+                    "boundary MPS needs 2 <= Lx <= 16, Ly <= 16, Lx * Ly <= 64, D <= 4, "
+                    "max_bond <= 32, max(max_bond, D) * D <= 64")
+        elif not (2 <= Lx <= 4 and 1 <= Ly <= 8 and 1 <= D <= 4):
+            raise NotImplementedError("exact contraction needs 2 <= Lx <= 4, Ly <= 8, D <= 4 "
+                                      "(max_bond > 0 selects the boundary MPS)")
         dmap, sizes = kernels.peps_layout(Lx, Ly, D)
         n = int(sum(sizes))
         if theta is None:
@@ -540,7 +569,7 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
             self._struct_obj = kernels.peps_struct(
                 self.Lx, self.Ly, self.D, self.dense_map, self.params[0].data,
                 self._nn_dict(), 0 if self.act_name == "relu" else 1, self.eta, self.mode,
-                amp_fp32=self.amp == "fp32")
+                amp_fp32=self.amp == "fp32", chi=self.chi if self._bmps else 0)
             self._state_key = None
             return True
         return False
@@ -735,6 +764,9 @@ class fPEPS_Model_reuse_GPU(fPEPS_Model_GPU):
                  max_bond=-1, chunk=65536, **kwargs):
         if kwargs.get("amp", "fp64") != "fp64":
             raise ValueError("the environment-reuse entry points contract in FP64")
+        if uses_bmps(Lx, Ly, D, max_bond):
+            raise NotImplementedError("environment reuse keeps exact environments "
+                                      "(max_bond <= 0 or >= D^2, Lx <= 4)")
         super().__init__(Lx, Ly, D, theta=theta, generator=generator, dtype=dtype,
                          max_bond=max_bond, chunk=chunk)
         self.radius = 0
