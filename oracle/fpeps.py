@@ -402,22 +402,29 @@ def _bmps_normalise(mps):
     return out, lg
 
 
-def bmps_side(shape, slices, chi, down, gaps=None):
-    """(MPS, log scale) of one side: rows 0 .. Lx/2-1 (down=True) or
-    Lx-1 .. Lx/2 (down=False), compressed to chi after every absorbed row
-    whose bonds exceed chi (chi <= 0: exact)."""
-    Lx, Ly = shape.Lx, shape.Ly
-    h = Lx // 2
-    rows = list(range(h)) if down else list(range(Lx - 1, h - 1, -1))
-    gaps = [] if gaps is None else gaps
+def _bmps_boundaries(shape, slices, chi, down, rows, gaps):
+    """(MPS, log scale) after each of `rows` absorbed in order, compressed to
+    chi after every absorbed row whose bonds exceed chi (chi <= 0: exact)."""
+    Ly = shape.Ly
     mps, lg = _bmps_normalise(_bmps_init([slices[rows[0] * Ly + y] for y in range(Ly)], down))
+    out = [(mps, lg)]
     for xx in rows[1:]:
         mps = _bmps_absorb(mps, [slices[xx * Ly + y] for y in range(Ly)], down)
         if chi > 0 and max(T.shape[2] for T in mps) > chi:
             mps = _bmps_compress(mps, chi, gaps)
         mps, l2 = _bmps_normalise(mps)
         lg += l2
-    return mps, lg
+        out.append((mps, lg))
+    return out
+
+
+def bmps_side(shape, slices, chi, down, gaps=None):
+    """(MPS, log scale) of one side: rows 0 .. Lx/2-1 (down=True) or
+    Lx-1 .. Lx/2 (down=False)."""
+    Lx = shape.Lx
+    h = Lx // 2
+    rows = list(range(h)) if down else list(range(Lx - 1, h - 1, -1))
+    return _bmps_boundaries(shape, slices, chi, down, rows, [] if gaps is None else gaps)[-1]
 
 
 def bmps_contract(shape, slices, chi, gaps=None):
@@ -451,3 +458,57 @@ def bmps_log_amplitudes(shape, theta, xs, chi):
         out.append((s, l, min(g) if g else np.inf))
     a = np.array(out)
     return a[:, 0], a[:, 1], a[:, 2]
+
+
+def bmps_strip_derivatives(top, row, bot):
+    """Z = <top| row |bot> of one row of slices [l, u, d, r] between the
+    boundary MPSs top [a, u, b] and bot [a', d, b'], and dZ/dS_y for every
+    site y, from left and right strip environments E[a, l, a']."""
+    Ly = len(row)
+    left = [np.ones((1, 1, 1))]
+    for y in range(Ly):
+        left.append(np.einsum("xly,xub,ludr,ydc->brc", left[-1], top[y], row[y], bot[y],
+                              optimize=True))
+    right = [np.ones((1, 1, 1))]
+    for y in range(Ly - 1, -1, -1):
+        right.append(np.einsum("xub,ludr,ydc,brc->xly", top[y], row[y], bot[y], right[-1],
+                               optimize=True))
+    right = right[::-1]
+    dS = [np.einsum("xly,xub,ydc,brc->ludr", left[y], top[y], bot[y], right[y + 1],
+                    optimize=True) for y in range(Ly)]
+    return float(left[-1].reshape(-1)[0]), dS
+
+
+def bmps_cheap_log_derivative(shape, theta, x, chi):
+    """d ln|psi| / d theta with the boundary environments held fixed, the
+    reference's cheap gradient (GPU/models/pureTNS.py:2230-2334): for every
+    row r the derivative of <env above r| row r |env below r> with respect to
+    the row's tensors (environments: the chi-truncated boundaries of the rows
+    above and below, quimb's compute_x_environments), summed over rows and
+    divided by psi of the two half boundaries (the amplitude).  Exact
+    (chi <= 0 or no truncation): the exact log-derivative."""
+    Lx, Ly, D = shape.Lx, shape.Ly, shape.D
+    x = np.asarray(x)
+    tensors = shape.unpack(theta)
+    S = signed_slices(shape, tensors, x)
+    s_psi, l_psi = bmps_contract(shape, S, chi)
+    g = np.zeros(shape.n_params)
+    if s_psi == 0.0:
+        return g, 0.0, -np.inf
+    gaps = []
+    tops = [None] + _bmps_boundaries(shape, S, chi, True, list(range(Lx - 1)), gaps)
+    bots = _bmps_boundaries(shape, S, chi, False, list(range(Lx - 1, 0, -1)), gaps)[::-1] + [None]
+    triv = ([np.ones((1, 1, 1))] * Ly, 0.0)
+    g0 = global_sign(x)
+    for r in range(Lx):
+        (tm, lt), (bm, lb) = tops[r] or triv, bots[r] or triv
+        _, dS = bmps_strip_derivatives(tm, [S[r * Ly + y] for y in range(Ly)], bm)
+        for y in range(Ly):
+            i = r * Ly + y
+            unit = np.ones_like(S[i])
+            sgn = signed_slices_single(shape, unit, x, i)
+            dT = np.zeros(shape.masks[i].shape)
+            # d psi / psi: the global sign cancels, 1 / sign(Z) = sign(Z)
+            dT[x[i]] = dS[y] * sgn * np.exp(lt + lb - l_psi) * s_psi
+            g[shape.offsets[i]:shape.offsets[i + 1]] = dT[shape.masks[i]]
+    return g, s_psi * g0, l_psi

@@ -144,3 +144,59 @@ def test_batch_gap_report():
     assert s.shape == l.shape == g.shape == (5,)
     s0, l0, g0 = fp.bmps_log_amplitudes(sh, th, xs, 0)
     assert np.all(np.isinf(g0))
+
+
+def _strip_einsum(top, row, bot, skip=None):
+    """<top| row |bot> (skip=None) or its derivative with respect to row[skip]
+    by one einsum over the whole strip."""
+    Ly = len(row)
+    L = iter("ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz")
+    ht, hs, hb = ([next(L) for _ in range(Ly + 1)] for _ in range(3))
+    vu, vd = ([next(L) for _ in range(Ly)] for _ in range(2))
+    terms, ops = [], []
+    for y in range(Ly):
+        terms += [ht[y] + vu[y] + ht[y + 1], hs[y] + vu[y] + vd[y] + hs[y + 1],
+                  hb[y] + vd[y] + hb[y + 1]]
+        ops += [top[y], row[y], bot[y]]
+    for e in (ht[0], ht[Ly], hs[0], hs[Ly], hb[0], hb[Ly]):
+        terms.append(e)
+        ops.append(np.ones(1))
+    out = ""
+    if skip is not None:
+        k = 3 * skip + 1
+        out = terms[k]
+        terms, ops = terms[:k] + terms[k + 1:], ops[:k] + ops[k + 1:]
+    return np.einsum(",".join(terms) + "->" + out, *ops)
+
+
+@pytest.mark.parametrize("Ly,D,chi", [(3, 2, 2), (2, 3, 3), (3, 2, 1)])
+def test_strip_derivatives_match_whole_strip_contraction(Ly, D, chi):
+    """The strip environments against one einsum per site, and Euler's
+    identity for a function linear in each site: sum_y <dZ/dS_y, S_y> = Ly Z."""
+    rng = np.random.default_rng(Ly * 10 + D)
+    top = [rng.normal(size=(1 if y == 0 else chi, D, 1 if y == Ly - 1 else chi))
+           for y in range(Ly)]
+    bot = [rng.normal(size=(1 if y == 0 else chi, D, 1 if y == Ly - 1 else chi))
+           for y in range(Ly)]
+    row = [rng.normal(size=(1 if y == 0 else D, D, D, 1 if y == Ly - 1 else D))
+           for y in range(Ly)]
+    Z, dS = fp.bmps_strip_derivatives(top, row, bot)
+    assert Z == pytest.approx(float(_strip_einsum(top, row, bot)), rel=1e-12)
+    for y in range(Ly):
+        assert np.allclose(dS[y], _strip_einsum(top, row, bot, y), rtol=1e-12, atol=1e-14)
+    assert sum(np.sum(d * s) for d, s in zip(dS, row)) == pytest.approx(Ly * Z, rel=1e-12)
+
+
+@pytest.mark.parametrize("Lx,Ly,D,chi", [(4, 3, 2, 0), (5, 3, 2, 0), (3, 3, 3, 0), (6, 3, 2, 8),
+                                         (2, 3, 2, 0)])
+def test_cheap_gradient_exact_limit(Lx, Ly, D, chi):
+    """Without truncation the environments are exact, so the reference's
+    cheap gradient is the exact log-derivative (zipper environments)."""
+    rng = np.random.default_rng(Lx + 10 * Ly + D)
+    sh = fp.PEPSShape(Lx, Ly, D)
+    th = fp.random_theta(sh, rng)
+    for x in _even_cfgs(rng, 3, sh.N):
+        g, s, l = fp.bmps_cheap_log_derivative(sh, th, x, chi)
+        ge = fp.log_derivative(sh, th, x)
+        assert np.abs(g - ge).max() < 1e-12 * np.abs(ge).max()
+        assert s * np.exp(l) == pytest.approx(fp.amplitude(sh, th, x), rel=1e-12)
