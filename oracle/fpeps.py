@@ -300,13 +300,15 @@ def nnftn_amplitude(shape, theta, nn, eta, x, act="relu", mode="add"):
     return amplitude(shape, theta_eff(theta, out[0], eta, mode), x)
 
 
-def nnftn_log_derivative(shape, theta, nn, eta, x, act="relu", mode="add"):
-    """[d/d theta, d/d (W1, b1, W2, b2, Wo, bo)] of ln|psi| for one config.
+def nnftn_log_derivative(shape, theta, nn, eta, x, act="relu", mode="add", chi=0):
+    """[d/d theta, d/d (W1, b1, W2, b2, Wo, bo)] of ln|psi| for one config
+    (chi > 0: with bmps_cheap_log_derivative's gradient of theta(x)).
     With g = d ln|psi| / d theta_eff: mode "add": d/d theta = g and
     d/d out = eta g; mode "mul": d/d theta = g (1 + eta out) and
     d/d out = eta g theta."""
     out, (n, z1, h1, z2, h2) = mlp_forward(nn, x, act)
-    g = log_derivative(shape, theta_eff(theta, out[0], eta, mode), x)
+    te = theta_eff(theta, out[0], eta, mode)
+    g = bmps_cheap_log_derivative(shape, te, x, chi)[0] if chi > 0 else log_derivative(shape, te, x)
     W1, b1, W2, b2, Wo, bo = nn
     if mode == "add":
         dout = eta * g
@@ -479,23 +481,24 @@ def bmps_strip_derivatives(top, row, bot):
     return float(left[-1].reshape(-1)[0]), dS
 
 
-def bmps_cheap_log_derivative(shape, theta, x, chi):
+def bmps_cheap_log_derivative(shape, theta, x, chi, gaps=None):
     """d ln|psi| / d theta with the boundary environments held fixed, the
     reference's cheap gradient (GPU/models/pureTNS.py:2230-2334): for every
     row r the derivative of <env above r| row r |env below r> with respect to
     the row's tensors (environments: the chi-truncated boundaries of the rows
     above and below, quimb's compute_x_environments), summed over rows and
     divided by psi of the two half boundaries (the amplitude).  Exact
-    (chi <= 0 or no truncation): the exact log-derivative."""
+    (chi <= 0 or no truncation): the exact log-derivative.  `gaps` collects
+    the relative gaps of every truncation (bmps_contract's)."""
     Lx, Ly, D = shape.Lx, shape.Ly, shape.D
     x = np.asarray(x)
     tensors = shape.unpack(theta)
     S = signed_slices(shape, tensors, x)
-    s_psi, l_psi = bmps_contract(shape, S, chi)
+    gaps = [] if gaps is None else gaps
+    s_psi, l_psi = bmps_contract(shape, S, chi, gaps)
     g = np.zeros(shape.n_params)
     if s_psi == 0.0:
         return g, 0.0, -np.inf
-    gaps = []
     tops = [None] + _bmps_boundaries(shape, S, chi, True, list(range(Lx - 1)), gaps)
     bots = _bmps_boundaries(shape, S, chi, False, list(range(Lx - 1, 0, -1)), gaps)[::-1] + [None]
     triv = ([np.ones((1, 1, 1))] * Ly, 0.0)

@@ -122,23 +122,124 @@ def test_bmps_passes_agree(device):
     _check(s2, l2, so, lo, go)
 
 
-def test_bmps_local_energy_matches_oracle(device):
-    m, sh, th = _plain(4, 4, 2, 3, 41, device)
-    nu = nd = 8
-    H = spinful_Fermi_Hubbard_square_lattice_torch(4, 4, 1.0, 8.0, 16, n_fermions_per_spin=(8, 8))
-    x = random_configs(np.random.default_rng(8), 12, sh.N, nu, nd)
-    E = m.local_energy(torch.tensor(x, device=device), H).cpu().numpy()
-    Ho = ohub.Hubbard(4, 4, 1.0, 8.0, nu, nd)
-    s0, l0, g0 = _oracle(sh, th, x, 3)
+def _oracle_eloc(sh, th, x, chi):
+    """E_loc from the oracle's boundary-MPS amplitudes, and which walkers
+    have every truncation (theirs and their connected configurations')
+    well conditioned."""
+    Ho = ohub.Hubbard(sh.Lx, sh.Ly, 1.0, 8.0, sh.N // 2, sh.N // 2)
+    s0, l0, g0 = _oracle(sh, th, x, chi)
     etas, coeffs, bids = Ho.get_conn_batch(x)
-    se, le, ge = _oracle(sh, th, etas, 3)
+    se, le, ge = _oracle(sh, th, etas, chi)
     Eo = np.zeros(x.shape[0])
     good = g0 >= GAP
     for e_s, e_l, e_g, c, b in zip(se, le, ge, coeffs, bids):
         Eo[b] += c * e_s * s0[b] * np.exp(e_l - l0[b]) if e_s != 0 else 0.0
         good[b] &= e_g >= GAP
+    return Eo, good
+
+
+def test_bmps_local_energy_matches_oracle(device):
+    m, sh, th = _plain(4, 4, 2, 3, 41, device)
+    H = spinful_Fermi_Hubbard_square_lattice_torch(4, 4, 1.0, 8.0, 16, n_fermions_per_spin=(8, 8))
+    x = random_configs(np.random.default_rng(8), 12, sh.N, 8, 8)
+    E = m.local_energy(torch.tensor(x, device=device), H).cpu().numpy()
+    Eo, good = _oracle_eloc(sh, th, x, 3)
     assert good.sum() >= 8
     assert np.allclose(E[good], Eo[good], rtol=1e-8, atol=1e-8 * np.abs(Eo).max())
+
+
+@pytest.mark.parametrize("Lx,Ly,D,chi", [(4, 5, 2, 3), (6, 4, 2, 3), (5, 4, 3, 5), (6, 6, 4, 16)])
+def test_bmps_log_derivative_matches_oracle(device, Lx, Ly, D, chi):
+    """The reference's cheap gradient (environments held fixed) against the
+    oracle's, entry by entry (1e-8 of the row's largest), dense and as
+    factors."""
+    m, sh, th = _plain(Lx, Ly, D, chi, 61 + Lx + D, device)
+    x = random_configs(np.random.default_rng(9), 6 if sh.N > 30 else 10, sh.N, sh.N // 2,
+                       sh.N // 2)
+    xd = torch.tensor(x, device=device)
+    O, (s, l) = m.log_derivative(xd, dtype=torch.float64)
+    FO, (sf, lf) = m.log_derivative_factored(xd)
+    assert torch.equal(s, sf) and torch.equal(l, lf)
+    Of = FO.materialize().cpu().numpy()
+    O = O.cpu().numpy()
+    n_ok = 0
+    for k in range(len(x)):
+        gaps = []
+        g, so, lo = fp.bmps_cheap_log_derivative(sh, th, x[k], chi, gaps)
+        if gaps and min(gaps) < GAP:
+            continue
+        n_ok += 1
+        sc = np.abs(g).max()
+        assert np.abs(O[k] - g).max() < 1e-8 * sc
+        assert np.abs(Of[k] - g).max() < 1e-8 * sc
+        assert float(s[k]) == so and abs(float(l[k]) - lo) < 1e-8
+    assert n_ok >= len(x) * 2 // 3
+
+
+def test_bmps_log_derivative_exact_limit(device):
+    """A 6-row lattice with chi at every bond: the exact log-derivative."""
+    m, sh, th = _plain(6, 3, 2, 8, 71, device)
+    x = random_configs(np.random.default_rng(11), 8, sh.N, 9, 9)
+    O, _ = m.log_derivative(torch.tensor(x, device=device), dtype=torch.float64)
+    O = O.cpu().numpy()
+    for k in range(len(x)):
+        g = fp.log_derivative(sh, th, x[k])
+        assert np.abs(O[k] - g).max() < 1e-9 * np.abs(g).max()
+
+
+def test_bmps_nnftn_log_derivative_matches_oracle(device):
+    Lx, Ly, D, chi, H = 4, 5, 2, 3, 8
+    sh = fp.PEPSShape(Lx, Ly, D)
+    rng = np.random.default_rng(13)
+    th = fp.random_theta(sh, rng)
+    nn = fp.mlp_init(rng, 2 * sh.N, H, sh.n_params, 0.2)
+    m = NNfTNS_Model_GPU(Lx, Ly, D, hidden_dim=H, nn_eta=0.7, theta=th, max_bond=chi)
+    with torch.no_grad():
+        for p, v in zip(m.params[1:], nn):
+            p.copy_(torch.as_tensor(v))
+    m = m.to(device)
+    x = random_configs(np.random.default_rng(14), 8, sh.N, 10, 10)
+    O, _ = m.log_derivative(torch.tensor(x, device=device), dtype=torch.float64)
+    O = O.cpu().numpy()
+    _, _, go = _oracle(sh, th, x, chi, nn, m.nn_eta)
+    n_ok = 0
+    for k in range(len(x)):
+        if go[k] < GAP:
+            continue
+        n_ok += 1
+        g = fp.nnftn_log_derivative(sh, th, nn, m.nn_eta, x[k], chi=chi)
+        assert np.abs(O[k] - g).max() < 1e-8 * np.abs(g).max()
+    assert n_ok >= 5
+
+
+def test_bmps_vmc_step_matches_oracle(device):
+    """One SR step of the reference-shaped driver with a chi-truncated fPEPS
+    equals the oracle's step on the same configurations (boundary-MPS local
+    energies, cheap gradients, minSR) within 1e-7 of the largest update."""
+    from oracle import sr as osr
+    from vmc_torch_amd import MetropolisExchangeSpinfulSamplerGPU
+    from vmc_torch_amd.optimizer import MinSRGPU, SGDGPU
+    from vmc_torch_amd.VMC import VMC_GPU, VMCLoopConfig
+    m, sh, th = _plain(4, 4, 2, 3, 81, device)
+    B, lr = 48, 0.05
+    Hg = spinful_Fermi_Hubbard_square_lattice_torch(4, 4, 1.0, 8.0, 16, n_fermions_per_spin=(8, 8))
+    Hg.precompute_hops_gpu(device)
+    fxs = torch.tensor(random_configs(np.random.default_rng(12), B, 16, 8, 8), device=device)
+    vmc = VMC_GPU(MetropolisExchangeSpinfulSamplerGPU(seed=4), MinSRGPU("fp64"), SGDGPU(lr))
+    cfg = VMCLoopConfig(vmc_steps=1, ns_per_rank=B, n_params=m.n_params, nsites=16,
+                        learning_rate=lr, diag_shift=1e-3, burn_in_steps=2,
+                        outlier_clip_factor=0.0, use_log_amp=True)
+    log = []
+    vmc.run_vmc_loop(fxs, m, Hg, Hg.graph, config=cfg,
+                     on_step_end=lambda d: log.append(d["fxs"].cpu().numpy().copy()))
+    x = log[0]
+    E, good = _oracle_eloc(sh, th, x, 3)
+    assert good.all()
+    O = np.array([fp.bmps_cheap_log_derivative(sh, th, c, 3)[0] for c in x])
+    dp = osr.minsr(O, E, E.mean(), 1e-3)
+    got = m.params[0].detach().cpu().numpy()
+    want = th - lr * dp
+    assert np.abs(got - want).max() <= 1e-7 * np.abs(lr * dp).max()
 
 
 def test_bmps_sweep_matches_oracle(device):
@@ -174,7 +275,4 @@ def test_bmps_options(device):
     # chi at or above D^2 on Lx <= 4 keeps the exact kernels
     m = fPEPS_Model_GPU(4, 4, 2, max_bond=4)
     assert not m._bmps
-    m = fPEPS_Model_GPU(4, 4, 2, max_bond=3).to(device)
-    x = torch.tensor(random_configs(np.random.default_rng(1), 4, 16, 8, 8), device=device)
-    with pytest.raises(RuntimeError, match="log-derivative"):
-        m.log_derivative(x)
+    assert fPEPS_Model_GPU(4, 4, 2, max_bond=3)._bmps

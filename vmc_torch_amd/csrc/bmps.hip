@@ -52,6 +52,7 @@ constexpr int BHDR = 32;        // int64 header words per slot: log scale, bond 
 
 struct BmGeo {
   int Lx, Ly, D, N, chi, cb;    // cb = max(chi, D): bound of a stored bond
+  int nbuf;                     // boundaries kept per side: 1 (amplitude) or Lx - 1
   const int32_t *dmap;
   const double *theta;
   int64_t theta_stride;
@@ -308,7 +309,11 @@ __device__ void bm_jacobi(BmSh &S, int m0, int m1) {
 
 // ------------------------------------------------------------ one side
 // grid (rows, 2): blockIdx.y = 0 the top side (rows 0 .. Lx/2-1, open down
-// legs), 1 the bottom side (rows Lx-1 .. Lx/2, open up legs).
+// legs), 1 the bottom side (rows Lx-1 .. Lx/2, open up legs).  With
+// g.nbuf = Lx - 1 every side runs over Lx - 1 rows and keeps the boundary
+// after each (buffer i: rows 0 .. i from the top, Lx-1 .. Lx-1-i from the
+// bottom): the environments of every row (quimb's compute_x_environments);
+// with g.nbuf = 1 it stops at the middle and works in place.
 __global__ void __launch_bounds__(BT)
     peps_bmps_side_kernel(BmGeo g, int64_t rows, const int64_t *__restrict__ cfg,
                           const int32_t *__restrict__ flag, double *ws, int64_t *hdr) {
@@ -323,9 +328,11 @@ __global__ void __launch_bounds__(BT)
   bm_load_cfg(g, cfg + w * g.N, cs);
   const double *th = g.theta + w * g.theta_stride;
   double *slot = ws + (w * 2 + side) * g.slot;
-  double *Rg = slot + (int64_t)g.Ly * g.ts;
+  const int64_t bstride = (int64_t)g.Ly * g.ts;
+  double *Rg = slot + g.nbuf * bstride;
+  int64_t *hd = hdr + (w * 2 + side) * g.nbuf * BHDR;
   const int Ly = g.Ly, D = g.D, h = g.Lx / 2;
-  const int nrows = down ? h : g.Lx - h;
+  const int nrows = g.nbuf > 1 ? g.Lx - 1 : (down ? h : g.Lx - h);
   const uint64_t dmask = D == 1 ? 0ull : (((1ull << D) - 1) & ~((1ull << (D >> 1)) - 1));
   if (tid <= Ly) {
     S.bd[tid] = hdim(g, tid);
@@ -345,9 +352,14 @@ __global__ void __launch_bounds__(BT)
       if (std::isinf(l2)) dead = true;
       lg += l2;
     }
+    if (tid <= Ly) hd[1 + tid] = S.bd[tid];
+    if (tid == 0) hd[0] = __double_as_longlong(dead ? -INFINITY : lg);
   }
   for (int ri = 1; ri < nrows && !dead; ++ri) {
     const int x = down ? ri : g.Lx - 1 - ri;
+    // boundary after row ri - 1 -> after row ri (the same buffer when nbuf = 1)
+    const double *src = slot + (g.nbuf > 1 ? (ri - 1) * bstride : 0);
+    double *dst = slot + (g.nbuf > 1 ? ri * bstride : 0);
     if (tid <= Ly) {
       const int hd = hdim(g, tid), b = S.bd[tid];
       S.nbd[tid] = b * hd;
@@ -364,7 +376,7 @@ __global__ void __launch_bounds__(BT)
       // absorb without truncation
       for (int y = 0; y < Ly && !dead; ++y) {
         const int na = S.bd[y], nb = S.bd[y + 1], dl = hdim(g, y), dr = hdim(g, y + 1);
-        bm_load_tensor(g, slot, y, na, nb, S.Bt);
+        bm_load_tensor(g, src, y, na, nb, S.Bt);
         bm_slice(g, cs, th, x, y, down, S.Sl);
         __syncthreads();
         const int ni = na * dl, nj = nb * dr, n = ni * D * nj;
@@ -377,7 +389,7 @@ __global__ void __launch_bounds__(BT)
           S.A[idx] = s;
         }
         __syncthreads();
-        const double l2 = bm_store_scaled(g, slot, y, ni, nj, S.A, S.red);
+        const double l2 = bm_store_scaled(g, dst, y, ni, nj, S.A, S.red);
         __syncthreads();
         if (std::isinf(l2)) dead = true;
         lg += l2;
@@ -387,6 +399,9 @@ __global__ void __launch_bounds__(BT)
         S.bpm[tid] = S.npm[tid];
       }
       __syncthreads();
+      if (tid <= Ly) hd[ri * (g.nbuf > 1) * BHDR + 1 + tid] = S.bd[tid];
+      if (tid == 0)
+        hd[ri * (g.nbuf > 1) * BHDR] = __double_as_longlong(dead ? -INFINITY : lg);
       continue;
     }
     // ---- backward: right Grams R_y, y = Ly-1 .. 1 (R_Ly = 1)
@@ -394,7 +409,7 @@ __global__ void __launch_bounds__(BT)
     for (int y = Ly - 1; y >= 1; --y) {
       const int na = S.bd[y], nb = S.bd[y + 1], dl = hdim(g, y), dr = hdim(g, y + 1);
       const int n = na * dl, m = nb * dr;
-      bm_load_tensor(g, slot, y, na, nb, S.Bt);
+      bm_load_tensor(g, src, y, na, nb, S.Bt);
       bm_slice(g, cs, th, x, y, down, S.Sl);
       __syncthreads();
       const int ti = tid >> 4, tj = tid & 15, i0 = 4 * ti, j0 = 4 * tj;
@@ -458,7 +473,7 @@ __global__ void __launch_bounds__(BT)
     for (int y = 0; y < Ly && !dead; ++y) {
       const int na = S.bd[y], nb = S.bd[y + 1], dl = hdim(g, y), dr = hdim(g, y + 1);
       const int m = nb * dr, nC = k * D;
-      bm_load_tensor(g, slot, y, na, nb, S.Bt);
+      bm_load_tensor(g, src, y, na, nb, S.Bt);
       bm_slice(g, cs, th, x, y, down, S.Sl);
       __syncthreads();
       // W[kk][l][u][b] = sum_a P[kk][(a,l)] B[a][u][b]  (into T)
@@ -488,7 +503,7 @@ __global__ void __launch_bounds__(BT)
         // last site: B'[kk][v][0] = C[(kk,v)][0]
         for (int idx = tid; idx < nC; idx += BT) S.T[idx] = S.A[idx * BLD];
         __syncthreads();
-        const double l2 = bm_store_scaled(g, slot, y, k, 1, S.T, S.red);
+        const double l2 = bm_store_scaled(g, dst, y, k, 1, S.T, S.red);
         __syncthreads();
         if (std::isinf(l2)) dead = true;
         lg += l2;
@@ -498,14 +513,14 @@ __global__ void __launch_bounds__(BT)
         }
         break;
       }
-      double *dst = slot + (int64_t)y * g.ts;
+      double *dsty = dst + (int64_t)y * g.ts;
       int kn;
       if (nC <= g.chi) {
         // no truncation at this bond: the identity basis, P = C
         kn = nC;
         for (int idx = tid; idx < k * D * kn; idx += BT) {
           const int j = idx % kn, kv = idx / kn;
-          dst[kv * g.cb + j] = j == kv ? 1.0 : 0.0;
+          dsty[kv * g.cb + j] = j == kv ? 1.0 : 0.0;
         }
         for (int idx = tid; idx < kn * m; idx += BT) {
           const int j = idx % m, i = idx / m;
@@ -583,7 +598,7 @@ __global__ void __launch_bounds__(BT)
         // B'[kk][v][j] = V[pos(kk,v)][sel j]
         for (int idx = tid; idx < nC * kn; idx += BT) {
           const int j = idx % kn, kv = idx / kn;
-          dst[kv * g.cb + j] = S.T[S.pos[kv] * BLD + S.sel[j]];
+          dsty[kv * g.cb + j] = S.T[S.pos[kv] * BLD + S.sel[j]];
         }
         // P'[j][(b,r)] = sum_pi V[pi][sel j] C[perm pi][(b,r)]
         for (int idx = tid; idx < kn * m; idx += BT) {
@@ -633,25 +648,28 @@ __global__ void __launch_bounds__(BT)
       S.bpm[tid] = S.npm[tid];
     }
     __syncthreads();
+    if (tid <= Ly) hd[ri * (g.nbuf > 1) * BHDR + 1 + tid] = S.bd[tid];
+    if (tid == 0) hd[ri * (g.nbuf > 1) * BHDR] = __double_as_longlong(dead ? -INFINITY : lg);
   }
-  if (tid <= Ly) hdr[(w * 2 + side) * BHDR + 1 + tid] = S.bd[tid];
-  if (tid == 0) {
-    const double v = dead ? -INFINITY : lg;
-    hdr[(w * 2 + side) * BHDR] = __double_as_longlong(v);
-  }
+  // buffers never reached (a vanishing boundary): mark them zero
+  if (dead && tid == 0)
+    for (int i = 0; i < g.nbuf; ++i) hd[i * BHDR] = __double_as_longlong(-INFINITY);
 }
 
 // ------------------------------------------------------------ join
+// the amplitude from the top boundary in buffer bt and the bottom one in bb
 __global__ void __launch_bounds__(64)
     peps_bmps_join_kernel(BmGeo g, int64_t rows, const int64_t *__restrict__ cfg,
                           const int32_t *__restrict__ flag, const double *__restrict__ ws,
-                          const int64_t *__restrict__ hdr, double *sign, double *logabs) {
+                          const int64_t *__restrict__ hdr, int bt, int bb, double *sign,
+                          double *logabs) {
   const int64_t w = blockIdx.x;
   if (w >= rows) return;
   if (flag && !(flag[w] & 1)) return;
   const int tid = threadIdx.x;
   __shared__ double E[BMAX_CHI * BMAX_CHI], X[BMAX_TS];
-  const int64_t *ht = hdr + (w * 2) * BHDR, *hb = hdr + (w * 2 + 1) * BHDR;
+  const int64_t *ht = hdr + ((w * 2) * g.nbuf + bt) * BHDR;
+  const int64_t *hb = hdr + ((w * 2 + 1) * g.nbuf + bb) * BHDR;
   const double lt = __longlong_as_double(ht[0]), lb = __longlong_as_double(hb[0]);
   const double lo = log(1e-45);
   int P = 0;
@@ -666,7 +684,9 @@ __global__ void __launch_bounds__(64)
     }
     return;
   }
-  const double *Tt = ws + (w * 2) * g.slot, *Tb = ws + (w * 2 + 1) * g.slot;
+  const int64_t bstride = (int64_t)g.Ly * g.ts;
+  const double *Tt = ws + (w * 2) * g.slot + bt * bstride;
+  const double *Tb = ws + (w * 2 + 1) * g.slot + bb * bstride;
   const int D = g.D, cb = g.cb;
   if (tid == 0) E[0] = 1.0;
   __syncthreads();
@@ -721,9 +741,231 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// ------------------------------------------------------------ cheap gradient
+// d ln|psi| / d theta of the sites of row r with the boundary environments of
+// the row held fixed (the reference's cheap gradient, pureTNS.py:2230-2334):
+// Z_r = <top_r| row r |bot_r>, top_r the boundary of rows < r (buffer r-1 of
+// the top side), bot_r of rows > r (buffer Lx-2-r of the bottom side); the
+// right strip environments R_y[b][r][b'] first (kept in renv), then a left
+// sweep with L_y[a][l][a'] that forms
+//   dZ/dS_y[l][u][d][r] = sum L_y[a][l][a'] T_y[a][u][b] Bo_y[a'][d][b'] R_{y+1}[b][r][b'],
+// divided by psi (sign, logabs of the join) and scattered to theta's entries
+// of the configuration's local states (G rows zeroed beforehand).
+struct GrSh {
+  double T[BMAX_TS], Bo[BMAX_TS], L[BMAX_TS], R[BMAX_TS];
+  double X[BN * BN], Y[BN * BN];
+  double Sl[256], dS[256];
+  double red[BT / 64];
+  int bt[BMAX_LY + 1], bb[BMAX_LY + 1];
+};
+
+__global__ void __launch_bounds__(BT)
+    peps_bmps_grad_kernel(BmGeo g, int64_t rows, const int64_t *__restrict__ cfg,
+                          const int32_t *__restrict__ flag, const double *__restrict__ ws,
+                          const int64_t *__restrict__ hdr, const double *__restrict__ sign,
+                          const double *__restrict__ logabs, double *renv, double *G,
+                          int64_t ldg) {
+  const int64_t w = blockIdx.x;
+  const int r = blockIdx.y;
+  if (w >= rows) return;
+  if (flag && !(flag[w] & 1)) return;
+  if (sign[w] == 0.0) return;
+  const int tid = threadIdx.x;
+  __shared__ BmCfg cs;
+  __shared__ GrSh S;
+  bm_load_cfg(g, cfg + w * g.N, cs);
+  const double *th = g.theta + w * g.theta_stride;
+  const int Lx = g.Lx, Ly = g.Ly, D = g.D;
+  const int64_t bstride = (int64_t)Ly * g.ts;
+  const bool ttriv = r == 0, btriv = r == Lx - 1;
+  const double *tsrc = ws + (w * 2) * g.slot + (ttriv ? 0 : (r - 1) * bstride);
+  const double *bsrc = ws + (w * 2 + 1) * g.slot + (btriv ? 0 : (Lx - 2 - r) * bstride);
+  const int64_t *th_ = hdr + ((w * 2) * g.nbuf + (ttriv ? 0 : r - 1)) * BHDR;
+  const int64_t *hb_ = hdr + ((w * 2 + 1) * g.nbuf + (btriv ? 0 : Lx - 2 - r)) * BHDR;
+  const double lt = ttriv ? 0.0 : __longlong_as_double(th_[0]);
+  const double lb = btriv ? 0.0 : __longlong_as_double(hb_[0]);
+  if (std::isinf(lt) || std::isinf(lb)) return;
+  if (tid <= Ly) {
+    S.bt[tid] = ttriv ? 1 : (int)th_[1 + tid];
+    S.bb[tid] = btriv ? 1 : (int)hb_[1 + tid];
+  }
+  int P = 0;
+  for (int q = 0; q < g.N; ++q) P += (cs.p[q] == 1 || cs.p[q] == 2);
+  const double gsign = ((P * (P - 1) / 2) & 1) ? -1.0 : 1.0;
+  const int du = r == 0 ? 1 : D, dd = r == Lx - 1 ? 1 : D;
+  double *rs = renv + (w * Lx + r) * ((int64_t)(Ly + 1) * g.ts + Ly + 1);
+  double *rlog = rs + (int64_t)(Ly + 1) * g.ts;
+  __syncthreads();
+  // load T_y (ta x du x tb) and Bo_y (ba x dd x bb) densely
+  auto load_env = [&](int y) {
+    const int ta = S.bt[y], tb = S.bt[y + 1], ba = S.bb[y], bb = S.bb[y + 1];
+    for (int idx = tid; idx < ta * du * tb; idx += BT) {
+      const int b = idx % tb, au = idx / tb;
+      S.T[idx] = ttriv ? 1.0 : tsrc[(int64_t)y * g.ts + au * g.cb + b];
+    }
+    for (int idx = tid; idx < ba * dd * bb; idx += BT) {
+      const int b = idx % bb, ad = idx / bb;
+      S.Bo[idx] = btriv ? 1.0 : bsrc[(int64_t)y * g.ts + ad * g.cb + b];
+    }
+  };
+  // ---- right strip environments R_y, y = Ly-1 .. 1 (R_Ly = 1)
+  if (tid == 0) {
+    S.R[0] = 1.0;
+    rlog[Ly] = 0.0;
+  }
+  double lr = 0.0;
+  for (int y = Ly - 1; y >= 1; --y) {
+    const int ta = S.bt[y], tb = S.bt[y + 1], ba = S.bb[y], bb = S.bb[y + 1];
+    const int dl = hdim(g, y), dr = hdim(g, y + 1);
+    load_env(y);
+    bm_slice(g, cs, th, r, y, true, S.Sl);
+    __syncthreads();
+    // Z1[a][u][rr][b'] = sum_b T[a][u][b] R[b][rr][b']
+    for (int idx = tid; idx < ta * du * dr * bb; idx += BT) {
+      const int bp = idx % bb, rr = (idx / bb) % dr, au = idx / (bb * dr);
+      double v = 0.0;
+      for (int b = 0; b < tb; ++b) v = fma(S.T[au * tb + b], S.R[(b * dr + rr) * bb + bp], v);
+      S.X[idx] = v;
+    }
+    __syncthreads();
+    // Z2[a][l][d][b'] = sum_{u,rr} Z1[a][u][rr][b'] S[l][u][d][rr]
+    for (int idx = tid; idx < ta * dl * dd * bb; idx += BT) {
+      const int bp = idx % bb, d = (idx / bb) % dd, l = (idx / (bb * dd)) % dl;
+      const int a = idx / (bb * dd * dl);
+      double v = 0.0;
+      for (int u = 0; u < du; ++u)
+        for (int rr = 0; rr < dr; ++rr)
+          v = fma(S.X[((a * du + u) * dr + rr) * bb + bp], S.Sl[((l * du + u) * dd + d) * dr + rr],
+                  v);
+      S.Y[idx] = v;
+    }
+    __syncthreads();
+    // R_y[a][l][a'] = sum_{d,b'} Z2[a][l][d][b'] Bo[a'][d][b']
+    double mx = 0.0;
+    const int nr = ta * dl * ba;
+    for (int idx = tid; idx < nr; idx += BT) {
+      const int ap = idx % ba, al = idx / ba;
+      double v = 0.0;
+      for (int d = 0; d < dd; ++d)
+        for (int bp = 0; bp < bb; ++bp)
+          v = fma(S.Y[(al * dd + d) * bb + bp], S.Bo[(ap * dd + d) * bb + bp], v);
+      S.L[idx] = v;
+      mx = fmax(mx, fabs(v));
+    }
+    mx = bm_max(mx, S.red);
+    if (!(mx > 0.0) || !std::isfinite(mx)) return;   // uniform: a vanishing strip
+    const double inv = 1.0 / mx;
+    lr += log(mx);
+    for (int idx = tid; idx < nr; idx += BT) {
+      const double v = S.L[idx] * inv;
+      S.R[idx] = v;
+      rs[(int64_t)y * g.ts + idx] = v;
+    }
+    if (tid == 0) rlog[y] = lr;
+    __syncthreads();
+  }
+  // ---- left sweep with the derivative of every site
+  if (tid == 0) S.L[0] = 1.0;
+  double ll = 0.0;
+  const double base = lt + lb - logabs[w];
+  const double sg = sign[w] * gsign;
+  for (int y = 0; y < Ly; ++y) {
+    const int ta = S.bt[y], tb = S.bt[y + 1], ba = S.bb[y], bb = S.bb[y + 1];
+    const int dl = hdim(g, y), dr = hdim(g, y + 1);
+    load_env(y);
+    bm_slice(g, cs, th, r, y, true, S.Sl);
+    const double lry = y == Ly - 1 ? 0.0 : rlog[y + 1];
+    if (y == Ly - 1) {
+      if (tid == 0) S.R[0] = 1.0;
+    } else {
+      for (int idx = tid; idx < tb * dr * bb; idx += BT)
+        S.R[idx] = rs[(int64_t)(y + 1) * g.ts + idx];
+    }
+    __syncthreads();
+    // X1[l][a'][u][b] = sum_a L[a][l][a'] T[a][u][b]
+    for (int idx = tid; idx < dl * ba * du * tb; idx += BT) {
+      const int b = idx % tb, u = (idx / tb) % du, ap = (idx / (tb * du)) % ba;
+      const int l = idx / (tb * du * ba);
+      double v = 0.0;
+      for (int a = 0; a < ta; ++a) v = fma(S.L[(a * dl + l) * ba + ap], S.T[(a * du + u) * tb + b], v);
+      S.X[idx] = v;
+    }
+    // Y2[a'][d][b][rr] = sum_b' Bo[a'][d][b'] R[b][rr][b']
+    for (int idx = tid; idx < ba * dd * tb * dr; idx += BT) {
+      const int rr = idx % dr, b = (idx / dr) % tb, ad = idx / (dr * tb);
+      double v = 0.0;
+      for (int bp = 0; bp < bb; ++bp) v = fma(S.Bo[ad * bb + bp], S.R[(b * dr + rr) * bb + bp], v);
+      S.Y[idx] = v;
+    }
+    __syncthreads();
+    // dS[l][u][d][rr] = sum_{a',b} X1[l][a'][u][b] Y2[a'][d][b][rr] -> G
+    {
+      const int n = dl * du * dd * dr;
+      const int s_ = r * Ly + y;
+      const int gbase = cs.doff[s_] + cs.p[s_] * n;
+      const int pr = cs.pr[s_];
+      const double coef = sg * exp(ll + lry + base);
+      for (int idx = tid; idx < n; idx += BT) {
+        const int rr = idx % dr, d = (idx / dr) % dd, u = (idx / (dr * dd)) % du;
+        const int l = idx / (dr * dd * du);
+        const int k = g.dmap[gbase + idx];
+        if (k < 0) continue;
+        double v = 0.0;
+        for (int ap = 0; ap < ba; ++ap)
+          for (int b = 0; b < tb; ++b)
+            v = fma(S.X[((l * ba + ap) * du + u) * tb + b], S.Y[((ap * dd + d) * tb + b) * dr + rr],
+                    v);
+        if ((bqpar(d, dd) * (bqpar(rr, dr) + pr)) & 1) v = -v;
+        G[w * ldg + k] = coef * v;
+      }
+    }
+    if (y == Ly - 1) break;
+    __syncthreads();
+    // X2[a'][b][d][rr] = sum_{l,u} X1[l][a'][u][b] S[l][u][d][rr]  (into Y)
+    for (int idx = tid; idx < ba * tb * dd * dr; idx += BT) {
+      const int rr = idx % dr, d = (idx / dr) % dd, b = (idx / (dr * dd)) % tb;
+      const int ap = idx / (dr * dd * tb);
+      double v = 0.0;
+      for (int l = 0; l < dl; ++l)
+        for (int u = 0; u < du; ++u)
+          v = fma(S.X[((l * ba + ap) * du + u) * tb + b], S.Sl[((l * du + u) * dd + d) * dr + rr],
+                  v);
+      S.Y[idx] = v;
+    }
+    __syncthreads();
+    // L_{y+1}[b][rr][b'] = sum_{a',d} X2[a'][b][d][rr] Bo[a'][d][b']
+    double mx = 0.0;
+    const int nl = tb * dr * bb;
+    for (int idx = tid; idx < nl; idx += BT) {
+      const int bp = idx % bb, rr = (idx / bb) % dr, b = idx / (bb * dr);
+      double v = 0.0;
+      for (int ap = 0; ap < ba; ++ap)
+        for (int d = 0; d < dd; ++d)
+          v = fma(S.Y[((ap * tb + b) * dd + d) * dr + rr], S.Bo[(ap * dd + d) * bb + bp], v);
+      S.X[idx] = v;
+      mx = fmax(mx, fabs(v));
+    }
+    mx = bm_max(mx, S.red);
+    if (!(mx > 0.0) || !std::isfinite(mx)) return;
+    const double inv = 1.0 / mx;
+    ll += log(mx);
+    for (int idx = tid; idx < nl; idx += BT) S.L[idx] = S.X[idx] * inv;
+    __syncthreads();
+  }
+}
+
+__global__ void bm_zero_rows_kernel(int64_t rows, int64_t P, double *G, int64_t ldg) {
+  const int64_t n = rows * P;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t w = i / P, k = i - w * P;
+    G[w * ldg + k] = 0.0;
+  }
+}
+
 // ------------------------------------------------------------ host side
 namespace {
-BmGeo bm_geo(int Lx, int Ly, int D, int chi) {
+BmGeo bm_geo(int Lx, int Ly, int D, int chi, int nbuf) {
   BmGeo g{};
   g.Lx = Lx;
   g.Ly = Ly;
@@ -731,8 +973,9 @@ BmGeo bm_geo(int Lx, int Ly, int D, int chi) {
   g.N = Lx * Ly;
   g.chi = chi;
   g.cb = chi > D ? chi : D;
+  g.nbuf = nbuf;
   g.ts = (int64_t)g.cb * D * g.cb;
-  g.slot = (int64_t)Ly * g.ts + (int64_t)Ly * BN * BN;
+  g.slot = (int64_t)nbuf * Ly * g.ts + (int64_t)Ly * BN * BN;
   return g;
 }
 }  // namespace
@@ -748,15 +991,21 @@ const char *bmps_check(int Lx, int Ly, int D, int chi) {
 }
 
 int64_t bmps_workspace_bytes(int Lx, int Ly, int D, int chi, int64_t rows) {
-  const BmGeo g = bm_geo(Lx, Ly, D, chi);
+  const BmGeo g = bm_geo(Lx, Ly, D, chi, 1);
   return rows * 2 * (g.slot * 8 + BHDR * 8);
+}
+
+int64_t bmps_logderiv_workspace_bytes(int Lx, int Ly, int D, int chi, int64_t rows) {
+  const BmGeo g = bm_geo(Lx, Ly, D, chi, Lx - 1);
+  return rows * 2 * (g.slot * 8 + g.nbuf * BHDR * 8) +
+         rows * Lx * ((int64_t)(Ly + 1) * g.ts + Ly + 1) * 8;
 }
 
 int bmps_amplitudes(int Lx, int Ly, int D, int chi, const int32_t *dmap, const double *theta,
                     int64_t theta_stride, int64_t rows, const int64_t *cfg, const int32_t *flag,
                     double *sign, double *logabs, void *ws, int64_t rows_per_pass,
                     hipStream_t st) {
-  BmGeo g = bm_geo(Lx, Ly, D, chi);
+  BmGeo g = bm_geo(Lx, Ly, D, chi, 1);
   g.dmap = dmap;
   double *slots = (double *)ws;
   int64_t *hdr = (int64_t *)(slots + rows_per_pass * 2 * g.slot);
@@ -770,7 +1019,40 @@ int bmps_amplitudes(int Lx, int Ly, int D, int chi, const int32_t *dmap, const d
                        slots, hdr);
     VMCX_LAUNCHED();
     hipLaunchKernelGGL(peps_bmps_join_kernel, dim3((unsigned)n), dim3(64), 0, st, g, n, c, f,
-                       slots, hdr, sign + r0, logabs + r0);
+                       slots, hdr, 0, 0, sign + r0, logabs + r0);
+    VMCX_LAUNCHED();
+  }
+  return 0;
+}
+
+int bmps_logderiv(int Lx, int Ly, int D, int chi, const int32_t *dmap, const double *theta,
+                  int64_t theta_stride, int64_t n_params, int64_t rows, const int64_t *cfg,
+                  double *G, int64_t ldg, double *sign, double *logabs, void *ws,
+                  int64_t rows_per_pass, hipStream_t st) {
+  BmGeo g = bm_geo(Lx, Ly, D, chi, Lx - 1);
+  g.dmap = dmap;
+  double *slots = (double *)ws;
+  int64_t *hdr = (int64_t *)(slots + rows_per_pass * 2 * g.slot);
+  double *renv = (double *)(hdr + rows_per_pass * 2 * g.nbuf * BHDR);
+  const int h = Lx / 2;
+  for (int64_t r0 = 0; r0 < rows; r0 += rows_per_pass) {
+    const int64_t n = rows - r0 < rows_per_pass ? rows - r0 : rows_per_pass;
+    g.theta = theta + r0 * theta_stride;
+    g.theta_stride = theta_stride;
+    const int64_t *c = cfg + r0 * g.N;
+    double *Gr = G + r0 * ldg;
+    hipLaunchKernelGGL(peps_bmps_side_kernel, dim3((unsigned)n, 2), dim3(BT), 0, st, g, n, c,
+                       nullptr, slots, hdr);
+    VMCX_LAUNCHED();
+    hipLaunchKernelGGL(peps_bmps_join_kernel, dim3((unsigned)n), dim3(64), 0, st, g, n, c,
+                       nullptr, slots, hdr, h - 1, Lx - h - 1, sign + r0, logabs + r0);
+    VMCX_LAUNCHED();
+    const int64_t nz = n * n_params;
+    const unsigned gz = (unsigned)((nz + 255) / 256 < 16384 ? (nz + 255) / 256 : 16384);
+    hipLaunchKernelGGL(bm_zero_rows_kernel, dim3(gz), dim3(256), 0, st, n, n_params, Gr, ldg);
+    VMCX_LAUNCHED();
+    hipLaunchKernelGGL(peps_bmps_grad_kernel, dim3((unsigned)n, (unsigned)Lx), dim3(BT), 0, st,
+                       g, n, c, nullptr, slots, hdr, sign + r0, logabs + r0, renv, Gr, ldg);
     VMCX_LAUNCHED();
   }
   return 0;

@@ -1135,9 +1135,10 @@ struct PepsWs {
   double *coeffs, *csign, *clog;
   // FP32 amplitudes of a shared parameter vector: the signed slice table
   float *tbl;
-  // chi-truncated boundary MPS: slots of bm_pass configurations
+  // chi-truncated boundary MPS: slots of bm_pass configurations (bm_gpass
+  // for the log-derivative's per-row environments)
   void *bm;
-  int64_t bm_pass;
+  int64_t bm_pass, bm_gpass;
   // environment reuse: the walkers' column environments, and the walker and
   // hop of each connected configuration
   double *env;
@@ -1149,7 +1150,7 @@ struct PepsWs {
 size_t alb(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // configurations per pass of the boundary-MPS kernels (their workspace)
-constexpr int64_t BM_PASS = 2048;
+constexpr int64_t BM_PASS = 2048, BM_GPASS = 1024;
 
 // the chi-truncated boundary MPS instead of the exact column contraction:
 // chi > 0 and either the lattice is beyond the exact kernels or chi is below
@@ -1201,7 +1202,10 @@ PepsWs peps_layout(const vmcx_peps_model &m, int64_t B, int64_t chunk, int n_edg
   L.tbl = (float *)take(m.amp_fp32 && H == 0 ? (size_t)N * 8 * 256 * 4 : 0);
   if (use_bmps(m)) {
     L.bm_pass = rows < BM_PASS ? (rows > 0 ? rows : 1) : BM_PASS;
-    L.bm = take((size_t)bmps_workspace_bytes(m.Lx, m.Ly, m.D, m.chi, L.bm_pass));
+    L.bm_gpass = chunk < BM_GPASS ? (chunk > 0 ? chunk : 1) : BM_GPASS;
+    const int64_t ba = bmps_workspace_bytes(m.Lx, m.Ly, m.D, m.chi, L.bm_pass);
+    const int64_t bg = bmps_logderiv_workspace_bytes(m.Lx, m.Ly, m.D, m.chi, L.bm_gpass);
+    L.bm = take((size_t)(ba > bg ? ba : bg));
   }
   if (reuse) {
     L.env = (double *)take((size_t)B * env_stride(m.Ly) * 8);
@@ -1604,9 +1608,6 @@ static int peps_logderiv_impl(const vmcx_peps_model *m, int64_t B, const int64_t
                               int64_t ld, bool fact, double *sign, double *logabs,
                               hipStream_t st) {
   if (check_peps(m)) return 1;
-  VMCX_CHECK_ARG(!use_bmps(*m),
-                 "the log-derivative of the chi-truncated boundary MPS is not built (chi <= 0 "
-                 "or chi >= D^rows for the exact gradients)");
   VMCX_CHECK_ARG(B >= 0 && chunk >= 1, "bad sizes");
   VMCX_CHECK_ARG(o_dtype == 0 || o_dtype == 1, "o_dtype must be 0 (f64) or 1 (f32)");
   if (B == 0) return 0;
@@ -1637,9 +1638,17 @@ static int peps_logderiv_impl(const vmcx_peps_model *m, int64_t B, const int64_t
     const bool direct = (H == 0 && o_dtype == 0);
     double *G = direct ? (double *)O + r0 * ld : L.G;
     const int64_t ldg = direct ? ld : P;
-    hipLaunchKernelGGL(peps_grad_kernel, dim3((unsigned)rows), dim3(PT), 0, st, A, rows, c, G,
-                       ldg, sign + r0, logabs + r0);
-    VMCX_LAUNCHED();
+    if (use_bmps(*m)) {
+      // the reference's cheap gradient on the chi-truncated environments
+      const int rc = bmps_logderiv(m->Lx, m->Ly, m->D, m->chi, A.dmap, A.theta, A.theta_stride,
+                                   P, rows, c, G, ldg, sign + r0, logabs + r0, L.bm, L.bm_gpass,
+                                   st);
+      if (rc) return rc;
+    } else {
+      hipLaunchKernelGGL(peps_grad_kernel, dim3((unsigned)rows), dim3(PT), 0, st, A, rows, c,
+                         G, ldg, sign + r0, logabs + r0);
+      VMCX_LAUNCHED();
+    }
     if (direct) continue;
     if (H == 0) {
       // FP32 O: plain copy of G

@@ -173,6 +173,13 @@ int bmps_amplitudes(int Lx, int Ly, int D, int chi, const int32_t *dmap, const d
                     int64_t theta_stride, int64_t rows, const int64_t *cfg, const int32_t *flag,
                     double *sign, double *logabs, void *ws, int64_t rows_per_pass,
                     hipStream_t st);
+// d ln|psi| / d theta with the boundary environments held fixed (G rows of
+// n_params at stride ldg, FP64), and (sign, logabs)
+int64_t bmps_logderiv_workspace_bytes(int Lx, int Ly, int D, int chi, int64_t rows_per_pass);
+int bmps_logderiv(int Lx, int Ly, int D, int chi, const int32_t *dmap, const double *theta,
+                  int64_t theta_stride, int64_t n_params, int64_t rows, const int64_t *cfg,
+                  double *G, int64_t ldg, double *sign, double *logabs, void *ws,
+                  int64_t rows_per_pass, hipStream_t st);
 
 int lp_n_params(int C);
 int lp_n_moves(int Lx, int Ly);
