@@ -48,6 +48,10 @@ constexpr int BMAX_LY = 16;
 constexpr int BMAX_TS = 2048;   // largest stored tensor cb * D * cb
 constexpr int BMAX_CHI = 32;
 constexpr int BMAX_SWEEPS = 30;
+// rho is scaled to a largest diagonal entry of 1; off-diagonal entries below
+// this floor are not rotated away (singular values below ~1e-9 of the
+// largest, which move an amplitude by about that much relative)
+constexpr double BM_ROT_FLOOR = 1e-18;
 constexpr int BHDR = 32;        // int64 header words per slot: log scale, bond dims
 
 struct BmGeo {
@@ -135,7 +139,7 @@ struct BmSh {
   double Bt[BMAX_TS];       // the boundary tensor of the current site
   double Sl[256];           // its slice [l][f][o][r]
   double P[BMAX_CHI * BN];  // projection carried from the left
-  double jc[BN / 2 + 1], js[BN / 2 + 1];
+  double jc[BN / 2 + 1], js[BN / 2 + 1], jt[BN / 2 + 1];
   int jp[BN / 2 + 1], jq[BN / 2 + 1];
   int perm[BN], pos[BN], sel[BN];
   int kpar[BN];             // parity of the current left bond's indices
@@ -200,8 +204,12 @@ __device__ void bm_mm(int n, int m, int K, const double *X, const double *Y, dou
       if (i0 + a < n && j0 + b < m) out[(i0 + a) * BLD + j0 + b] = acc[a][b];
 }
 
-// cyclic Jacobi on the two diagonal blocks [0, m0) and [m0, m0 + m1) of
-// S.R (sizes even; padding rows / columns zero), eigenvectors into S.T
+// cyclic Jacobi on the two diagonal blocks [0, m0) and [m0, m0 + m1) of the
+// symmetric S.R, kept as its upper triangle (entry (a, b) at
+// [min(a,b)][max(a,b)], so that rounding cannot make it asymmetric: a skew
+// residue would survive every rotation and keep the relative stopping test
+// from being met); block sizes even, padding rows / columns zero;
+// eigenvectors into S.T
 __device__ void bm_jacobi(BmSh &S, int m0, int m1) {
   const int tid = threadIdx.x;
   const int np0 = m0 >> 1, np1 = m1 >> 1, npr = np0 + np1;
@@ -224,7 +232,7 @@ __device__ void bm_jacobi(BmSh &S, int m0, int m1) {
         const int i = b ? tid - np0 : tid;
         const int mb = b ? m1 : m0, off = b ? m0 : 0;
         int p = 0, q = 0;
-        double c = 1.0, s = 0.0;
+        double c = 1.0, s = 0.0, t = 0.0;
         if (mb >= 2) {
           const int rr = r % (mb - 1);
           if (i == 0) {
@@ -236,11 +244,11 @@ __device__ void bm_jacobi(BmSh &S, int m0, int m1) {
           }
           p += off;
           q += off;
-          const double app = S.R[p * BLD + p], aqq = S.R[q * BLD + q], apq = S.R[p * BLD + q];
-          if (fabs(apq) > fmax(1e-15 * sqrt(fabs(app * aqq)), 1e-30)) {
+          const double app = S.R[p * BLD + p], aqq = S.R[q * BLD + q];
+          const double apq = S.R[(p < q ? p : q) * BLD + (p < q ? q : p)];
+          if (fabs(apq) > fmax(1e-15 * sqrt(fabs(app * aqq)), BM_ROT_FLOOR)) {
             const double th = (aqq - app) / (2.0 * apq);
-            const double t = fabs(th) > 1e150 ? 0.5 / th
-                                              : copysign(1.0, th) / (fabs(th) + sqrt(th * th + 1.0));
+            t = fabs(th) > 1e150 ? 0.5 / th : copysign(1.0, th) / (fabs(th) + sqrt(th * th + 1.0));
             c = 1.0 / sqrt(t * t + 1.0);
             s = t * c;
             S.rot = 1;
@@ -250,6 +258,7 @@ __device__ void bm_jacobi(BmSh &S, int m0, int m1) {
         S.jq[tid] = q;
         S.jc[tid] = c;
         S.js[tid] = s;
+        S.jt[tid] = t;
       }
       __syncthreads();
       // rho <- J^T rho J, one 2 x 2 block (pair i1 rows, pair i2 columns) per item
@@ -264,38 +273,73 @@ __device__ void bm_jacobi(BmSh &S, int m0, int m1) {
           i1 = np0 + w2 / np1;
           i2 = np0 + w2 % np1;
         }
+        if (i1 > i2) continue;                 // upper triangle of pair blocks
         const double s1 = S.js[i1], s2 = S.js[i2];
         if (s1 == 0.0 && s2 == 0.0) continue;
-        const double c1 = S.jc[i1], c2 = S.jc[i2];
         const int p1 = S.jp[i1], q1 = S.jq[i1], p2 = S.jp[i2], q2 = S.jq[i2];
-        const double xpp = S.R[p1 * BLD + p2], xpq = S.R[p1 * BLD + q2];
-        const double xqp = S.R[q1 * BLD + p2], xqq = S.R[q1 * BLD + q2];
+        if (i1 == i2) {
+          // the rotated pair itself: its off-diagonal entry becomes zero
+          const double t = S.jt[i1];
+          const int lo = p1 < q1 ? p1 : q1, hi = p1 < q1 ? q1 : p1;
+          const double apq = S.R[lo * BLD + hi];
+          S.R[p1 * BLD + p1] -= t * apq;
+          S.R[q1 * BLD + q1] += t * apq;
+          S.R[lo * BLD + hi] = 0.0;
+          continue;
+        }
+        const double c1 = S.jc[i1], c2 = S.jc[i2];
+        // entries of distinct pairs: never on the diagonal
+        const int e0 = p1 < p2 ? p1 * BLD + p2 : p2 * BLD + p1;
+        const int e1 = p1 < q2 ? p1 * BLD + q2 : q2 * BLD + p1;
+        const int e2 = q1 < p2 ? q1 * BLD + p2 : p2 * BLD + q1;
+        const int e3 = q1 < q2 ? q1 * BLD + q2 : q2 * BLD + q1;
+        const double xpp = S.R[e0], xpq = S.R[e1], xqp = S.R[e2], xqq = S.R[e3];
         const double ypp = c1 * xpp - s1 * xqp, ypq = c1 * xpq - s1 * xqq;
         const double yqp = s1 * xpp + c1 * xqp, yqq = s1 * xpq + c1 * xqq;
-        S.R[p1 * BLD + p2] = ypp * c2 - ypq * s2;
-        S.R[p1 * BLD + q2] = ypp * s2 + ypq * c2;
-        S.R[q1 * BLD + p2] = yqp * c2 - yqq * s2;
-        S.R[q1 * BLD + q2] = yqp * s2 + yqq * c2;
+        S.R[e0] = ypp * c2 - ypq * s2;
+        S.R[e1] = ypp * s2 + ypq * c2;
+        S.R[e2] = yqp * c2 - yqq * s2;
+        S.R[e3] = yqp * s2 + yqq * c2;
       }
-      // V <- V J over the rows of each block
+      // V <- V J over the rows of each block (up to four items per thread,
+      // every load issued before the stores)
       const int nv0 = m0 * np0, nv1 = m1 * np1;
-      for (int wi = tid; wi < nv0 + nv1; wi += BT) {
-        int row, pi;
-        if (wi < nv0) {
-          row = wi / np0;
-          pi = wi - row * np0;
-        } else {
-          const int w2 = wi - nv0;
-          row = m0 + w2 / np1;
-          pi = np0 + w2 % np1;
+      {
+        constexpr int NV = (BN * BN / 2) / BT;
+        int ip[NV], iq[NV];
+        double vp[NV], vq[NV], cc[NV], ss[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          const int wi = tid + k * BT;
+          ss[k] = 0.0;
+          ip[k] = iq[k] = 0;
+          if (wi < nv0 + nv1) {
+            int row, pi;
+            if (wi < nv0) {
+              row = wi / np0;
+              pi = wi - row * np0;
+            } else {
+              const int w2 = wi - nv0;
+              row = m0 + w2 / np1;
+              pi = np0 + w2 % np1;
+            }
+            ss[k] = S.js[pi];
+            cc[k] = S.jc[pi];
+            ip[k] = row * BLD + S.jp[pi];
+            iq[k] = row * BLD + S.jq[pi];
+          }
         }
-        const double s = S.js[pi];
-        if (s == 0.0) continue;
-        const double c = S.jc[pi];
-        const int p = S.jp[pi], q = S.jq[pi];
-        const double vp = S.T[row * BLD + p], vq = S.T[row * BLD + q];
-        S.T[row * BLD + p] = c * vp - s * vq;
-        S.T[row * BLD + q] = s * vp + c * vq;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          vp[k] = S.T[ip[k]];
+          vq[k] = S.T[iq[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+          if (ss[k] != 0.0) {
+            S.T[ip[k]] = cc[k] * vp[k] - ss[k] * vq[k];
+            S.T[iq[k]] = ss[k] * vp[k] + cc[k] * vq[k];
+          }
       }
       __syncthreads();
     }
@@ -564,9 +608,9 @@ __global__ void __launch_bounds__(BT)
           const int p1 = idx / mt, p2 = idx - p1 * mt;
           const int i1 = S.perm[p1], i2 = S.perm[p2];
           double s = 0.0;
-          if (i1 >= 0 && i2 >= 0 && ((p1 < m0) == (p2 < m0)))
+          if (p1 <= p2 && i1 >= 0 && i2 >= 0 && ((p1 < m0) == (p2 < m0)))
             for (int j = 0; j < m; ++j) s = fma(S.T[i1 * BLD + j], S.A[i2 * BLD + j], s);
-          S.R[p1 * BLD + p2] = s;
+          S.R[p1 * BLD + p2] = s;     // upper triangle only (the rest unused)
         }
         __syncthreads();
         double dmx = 0.0;
