@@ -80,6 +80,11 @@ WORKLOADS = {
         gram="fp64", D=4, hidden=0, eta=0.0,
         desc=("4x4 OBC Hubbard t=1 U=8 8u+8d, Z2 fPEPS D=4 (exact contraction, chi=16), "
               "3200 params, FP64, 2048 walkers")),
+    "hubbard6x6_fpeps_bmps": dict(
+        kind="peps", Lx=6, Ly=6, nu=14, nd=14, B=1024, burn=20, shift=5e-4, lr=0.1,
+        gram="fp64", D=4, chi=8, hidden=0, eta=0.0,
+        desc=("6x6 OBC Hubbard t=1 U=8 14u+14d (N-8 electrons, as the reference's fPEPS "
+              "script), Z2 fPEPS D=4, boundary MPS truncated to chi=8, FP64, 1024 walkers")),
     "hubbard_mixed_lp": dict(
         kind="mixed_lp", instances=[(4, 4, 8, 8, 8192), (6, 6, 16, 16, 4096), (8, 8, 32, 32, 4096)],
         hidden=256, bf_scale=0.01, mlp="bf16", burn=50, channels=32, train_iters=100,
@@ -219,7 +224,13 @@ def _cpu_worker(args):
         sh = ofp.PEPSShape(Lx, Ly, bf["D"])
         th, nn, eta = params[0], (params[1:] or None), bf["eta"]
 
+        chi = bf.get("chi", 0)
+
         def amp(c):
+            if chi > 0:
+                t = th if nn is None else ofp.theta_eff(th, ofp.mlp_forward(nn, c)[0][0], eta)
+                s_, l_ = ofp.bmps_log_amplitude(sh, t, c, chi)
+                return s_ * np.exp(l_)
             return (ofp.amplitude(sh, th, c) if nn is None
                     else ofp.nnftn_amplitude(sh, th, nn, eta, c))
 
@@ -229,12 +240,15 @@ def _cpu_worker(args):
                 return np.sign(a), np.maximum(np.log(np.abs(a)), np.log(1e-45))
 
         def work(xs, s, l):
-            ofp.local_energy(H, sh, th, xs, nn, eta)
+            ofp.local_energy(H, sh, th, xs, nn, eta, chi=chi)
             for c in xs:
                 if nn is None:
-                    ofp.log_derivative(sh, th, c)
+                    if chi > 0:
+                        ofp.bmps_cheap_log_derivative(sh, th, c, chi)
+                    else:
+                        ofp.log_derivative(sh, th, c)
                 else:
-                    ofp.nnftn_log_derivative(sh, th, nn, eta, c)
+                    ofp.nnftn_log_derivative(sh, th, nn, eta, c, chi=chi)
     done, t0 = 0, time.perf_counter()
     chunk = 4
     while time.perf_counter() - t0 < budget:
@@ -554,13 +568,16 @@ class PepsRun(NnbfRun):
         N, nf, B = w["Lx"] * w["Ly"], w["nu"] + w["nd"], w["B"]
         self.N, self.nf, self.B = N, nf, B
         g = torch.Generator().manual_seed(0)
+        chi = w.get("chi", -1)
         if w["hidden"] > 0:
             self.model = NNfTNS_Model_GPU(w["Lx"], w["Ly"], w["D"], hidden_dim=w["hidden"],
-                                          nn_eta=w["eta"], generator=g, amp=a.peps_amp).to(dev)
+                                          nn_eta=w["eta"], generator=g, amp=a.peps_amp,
+                                          max_bond=chi).to(dev)
         else:
             cls = fPEPS_Model_reuse_GPU if getattr(a, "peps_reuse", False) else fPEPS_Model_GPU
             kw = {"amp": a.peps_amp} if a.peps_amp != "fp64" else {}
-            self.model = cls(w["Lx"], w["Ly"], w["D"], generator=g, **kw).to(dev)
+            self.model = cls(w["Lx"], w["Ly"], w["D"], generator=g, max_bond=chi,
+                             **kw).to(dev)
         self.Np = self.model.n_params
         self.e_sweep = H.graph.edge_tensor(dev, "sweep")
         self.e_graph = H.graph.edge_tensor(dev, "graph")
@@ -581,15 +598,19 @@ class PepsRun(NnbfRun):
 
     def roofline_items(self):
         """Sweep and local energy: FP64 flops of every amplitude evaluated
-        (column-pair contraction; the MLP when present).  logderiv: the O write."""
+        (column-pair contraction, or the boundary MPS's dense products without
+        its eigensolver; the MLP when present).  logderiv: the O write."""
         D, Ly, Hd = self.w["D"], self.w["Ly"], self.w["hidden"]
-        contr = 0
-        for y in range(Ly):
-            dl = 1 if y == 0 else D
-            dr = 1 if y == Ly - 1 else D
-            da, da2 = dl * dl, dr * dr
-            contr += 2 * (da * D * da2 * D) + da2 * D * da2 * da + da2 * da2 * da2 * D
-        flops = 2.0 * contr
+        if getattr(self.model, "_bmps", False):
+            flops = bmps_amplitude_flops(self.w["Lx"], Ly, D, self.w["chi"])
+        else:
+            contr = 0
+            for y in range(Ly):
+                dl = 1 if y == 0 else D
+                dr = 1 if y == Ly - 1 else D
+                da, da2 = dl * dl, dr * dr
+                contr += 2 * (da * D * da2 * D) + da2 * D * da2 * da + da2 * da2 * da2 * D
+            flops = 2.0 * contr
         if Hd:
             flops += 2.0 * (2 * self.N * Hd + Hd * Hd + Hd * self.model.n_tn_params)
         n_prop = self.e_sweep.shape[0] * self.B
@@ -606,7 +627,45 @@ class PepsRun(NnbfRun):
 
     def oracle_params(self):
         ps = [p.detach().cpu().numpy() for p in self.model.params]
-        return ps, {"D": self.w["D"], "eta": self.w["eta"]}
+        chi = self.w.get("chi", 0) if getattr(self.model, "_bmps", False) else 0
+        return ps, {"D": self.w["D"], "eta": self.w["eta"], "chi": chi}
+
+
+def bmps_amplitude_flops(Lx, Ly, D, chi):
+    """FP64 flops of the dense products of one boundary-MPS amplitude (the
+    right Grams, C, C R, rho = C R C^T and the projection per site of every
+    truncated row, the absorbed tensors of the others, the join), bonds
+    taken at their bound min(chi, rank); the Jacobi eigensolver is not
+    counted."""
+    h = Lx // 2
+    hd = [1] + [D] * (Ly - 1) + [1]
+    total = 0
+    for nrows in (h, Lx - h):
+        bd = list(hd)
+        for _ in range(nrows - 1):
+            nbd = [b * x for b, x in zip(bd, hd)]
+            if max(nbd) <= chi:
+                total += sum(nbd[y] * D * nbd[y + 1] * D for y in range(Ly))
+                bd = nbd
+                continue
+            for y in range(1, Ly):      # Grams: A_v R A_v^T for every v
+                n, m = nbd[y], nbd[y + 1]
+                total += D * (n * m * D + n * m * m + n * n * m)
+            k = 1
+            new = [1]
+            for y in range(Ly):
+                m = nbd[y + 1]
+                total += k * hd[y] * D * bd[y + 1] * bd[y] + k * D * m * hd[y] * D
+                if y < Ly - 1:
+                    nC = k * D
+                    kn = min(chi, nC)
+                    if nC > chi:
+                        total += nC * m * m + nC * nC * m // 2 + kn * nC * m
+                    k = kn
+                new.append(k)
+            bd = new[:Ly] + [1]
+    total += Ly * chi * chi * D * chi * 2         # join
+    return 2.0 * total
 
 
 # --------------------------------------------------------------------- main
@@ -799,9 +858,10 @@ def main():
     single = {"sr.gram": 1, "sr.chol_diag": n_diag}
     if w["kind"] == "slater":
         single.update({"sweep": 1, "local_energy": 1, "logderiv": 1, "refresh": 1})
-    elif run.factored:
-        # with the Gram formed from the factors the SR step no longer sets
-        # the step: the sampling components (several kernels each) compete
+    elif run.factored or w["kind"] == "peps":
+        # with the Gram formed from the factors (or the small fPEPS O) the SR
+        # step no longer sets the step: the sampling components (several
+        # kernels each) compete
         single.update({"sweep": 1, "local_energy": 1, "logderiv": 1})
     for k, n in single.items():
         kern[k].setdefault("launches_per_step", n)
@@ -884,7 +944,7 @@ def main():
             Eo = onn.local_energy(Ho, params, xs, 2, bf)
         else:
             Eo = ofp.local_energy(Ho, ofp.PEPSShape(Lx, Ly, bf["D"]), params[0], xs,
-                                  params[1:] or None, bf["eta"])
+                                  params[1:] or None, bf["eta"], chi=bf.get("chi", 0))
         out["dE_per_site_vs_fp64_oracle"] = {
             "mean": float(abs(Eg.mean() - Eo.mean()) / N),
             "max_per_sample": float(np.abs(Eg - Eo).max() / N), "samples": int(len(xs))}

@@ -325,11 +325,18 @@ def nnftn_log_derivative(shape, theta, nn, eta, x, act="relu", mode="add", chi=0
     return np.concatenate([g, dW1.ravel(), dz1, dW2.ravel(), dz2, dWo.ravel(), dout])
 
 
-def local_energy(H, shape, theta, xs, nn=None, eta=0.0, act="relu", mode="add"):
-    """E_loc via the term-major get_conn and full amplitudes."""
+def local_energy(H, shape, theta, xs, nn=None, eta=0.0, act="relu", mode="add", chi=0):
+    """E_loc via the term-major get_conn and full amplitudes (chi > 0: the
+    chi-truncated boundary-MPS amplitudes)."""
     xs = np.atleast_2d(xs)
 
     def amp(c):
+        if chi > 0:
+            t = theta
+            if nn is not None:
+                t = theta_eff(theta, mlp_forward(nn, c, act)[0][0], eta, mode)
+            s, l = bmps_log_amplitude(shape, t, c, chi)
+            return s * np.exp(l)
         if nn is None:
             return amplitude(shape, theta, c)
         return nnftn_amplitude(shape, theta, nn, eta, c, act, mode)

@@ -132,17 +132,19 @@ __device__ double bm_max(double v, double *red) {
   return m;
 }
 
-struct BmSh {
-  double R[BN * BLD];       // right Gram; then rho
-  double A[BN * BLD];       // A_v; then C
-  double T[BN * BLD];       // A_v R; then W, G, V
-  double Bt[BMAX_TS];       // the boundary tensor of the current site
+template <int NB>
+struct BmShT {
+  static constexpr int LD = NB + 1;   // LDS row stride of the NB x NB buffers
+  double R[NB * LD];        // right Gram; then rho
+  double A[NB * LD];        // A_v; then C
+  double T[NB * LD];        // A_v R; then W, G, V
+  double Bt[NB * 32];       // the boundary tensor of the current site (cb * D * cb)
   double Sl[256];           // its slice [l][f][o][r]
-  double P[BMAX_CHI * BN];  // projection carried from the left
-  double jc[BN / 2 + 1], js[BN / 2 + 1], jt[BN / 2 + 1];
-  int jp[BN / 2 + 1], jq[BN / 2 + 1];
-  int perm[BN], pos[BN], sel[BN];
-  int kpar[BN];             // parity of the current left bond's indices
+  double P[32 * NB];        // projection carried from the left (chi <= 32 rows)
+  double jc[NB / 2 + 1], js[NB / 2 + 1], jt[NB / 2 + 1];
+  int jp[NB / 2 + 1], jq[NB / 2 + 1];
+  int perm[NB], pos[NB], sel[NB];
+  int kpar[NB];             // parity of the current left bond's indices
   int bd[BMAX_LY + 1], nbd[BMAX_LY + 1];
   uint64_t bpm[BMAX_LY + 1], npm[BMAX_LY + 1];   // bond parity masks
   double red[BT / 64];
@@ -179,7 +181,8 @@ __device__ double bm_store_scaled(const BmGeo &g, double *slot, int y, int na, i
   return log(m);
 }
 
-// out (n x m, stride BLD) = X (n x K, stride BLD) * Y (K x m, stride BLD); 4 x 4 tiles
+// out (n x m, stride LD) = X (n x K, stride LD) * Y (K x m, stride LD); 4 x 4 tiles
+template <int LD>
 __device__ void bm_mm(int n, int m, int K, const double *X, const double *Y, double *out) {
   const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
   const int i0 = 4 * ti, j0 = 4 * tj;
@@ -189,8 +192,8 @@ __device__ void bm_mm(int n, int m, int K, const double *X, const double *Y, dou
     double xa[4], yb[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      xa[q] = (i0 + q < n) ? X[(i0 + q) * BLD + k] : 0.0;
-      yb[q] = (j0 + q < m) ? Y[k * BLD + j0 + q] : 0.0;
+      xa[q] = (i0 + q < n) ? X[(i0 + q) * LD + k] : 0.0;
+      yb[q] = (j0 + q < m) ? Y[k * LD + j0 + q] : 0.0;
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -201,7 +204,7 @@ __device__ void bm_mm(int n, int m, int K, const double *X, const double *Y, dou
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b)
-      if (i0 + a < n && j0 + b < m) out[(i0 + a) * BLD + j0 + b] = acc[a][b];
+      if (i0 + a < n && j0 + b < m) out[(i0 + a) * LD + j0 + b] = acc[a][b];
 }
 
 // cyclic Jacobi on the two diagonal blocks [0, m0) and [m0, m0 + m1) of the
@@ -210,42 +213,88 @@ __device__ void bm_mm(int n, int m, int K, const double *X, const double *Y, dou
 // residue would survive every rotation and keep the relative stopping test
 // from being met); block sizes even, padding rows / columns zero;
 // eigenvectors into S.T
-__device__ void bm_jacobi(BmSh &S, int m0, int m1) {
+template <int NB>
+__device__ void bm_jacobi(BmShT<NB> &S, int m0, int m1) {
+  constexpr int LD = BmShT<NB>::LD;
   const int tid = threadIdx.x;
   const int np0 = m0 >> 1, np1 = m1 >> 1, npr = np0 + np1;
   const int mt = m0 + m1;
   for (int idx = tid; idx < mt * mt; idx += BT) {
     const int i = idx / mt, j = idx - i * mt;
-    S.T[i * BLD + j] = i == j ? 1.0 : 0.0;
+    S.T[i * LD + j] = i == j ? 1.0 : 0.0;
   }
   const int rounds = (m0 > m1 ? m0 : m1) - 1;
   if (rounds <= 0) {
     __syncthreads();
     return;
   }
+  // the work of every thread is the same in every round (only the pairing
+  // changes), so its items are decoded once here, not per round:
+  // phase 1: the pair this thread rotates (block b, index i, size mb)
+  const int pb = tid >= np0, pi_ = pb ? tid - np0 : tid;
+  const int pmb = pb ? m1 : m0, poff = pb ? m0 : 0;
+  // phase 2a: pair blocks (i1 <= i2) of the upper triangle, compacted
+  constexpr int NBK = ((NB / 2) * (NB / 2 + 1) / 2 + BT - 1) / BT;
+  const int nt0 = np0 * (np0 + 1) / 2, nt1 = np1 * (np1 + 1) / 2;
+  int bk1[NBK], bk2[NBK];
+#pragma unroll
+  for (int k = 0; k < NBK; ++k) {
+    int wi = tid + k * BT;
+    bk1[k] = -1;
+    bk2[k] = -1;
+    if (wi < nt0 + nt1) {
+      const int b = wi >= nt0;
+      const int np = b ? np1 : np0, base = b ? np0 : 0;
+      if (b) wi -= nt0;
+      int i1 = 0;
+      while (wi >= np - i1) {   // row i1 of the triangle holds np - i1 items
+        wi -= np - i1;
+        ++i1;
+      }
+      bk1[k] = base + i1;
+      bk2[k] = base + i1 + wi;
+    }
+  }
+  // phase 2b: rows of V and the pair whose columns they rotate
+  constexpr int NV = (NB * NB / 2 + BT - 1) / BT;
+  const int nv0 = m0 * np0, nv1 = m1 * np1;
+  int vrow[NV], vpi[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const int wi = tid + k * BT;
+    vrow[k] = -1;
+    vpi[k] = 0;
+    if (wi < nv0) {
+      vrow[k] = wi / np0;
+      vpi[k] = wi - vrow[k] * np0;
+    } else if (wi < nv0 + nv1) {
+      const int w2 = wi - nv0;
+      vrow[k] = m0 + w2 / np1;
+      vpi[k] = np0 + w2 % np1;
+    }
+  }
   for (int sweep = 0; sweep < BMAX_SWEEPS; ++sweep) {
     if (tid == 0) S.rot = 0;
     __syncthreads();
+    int rr = 0;   // the round within a block of size pmb: r mod (pmb - 1)
     for (int r = 0; r < rounds; ++r) {
       if (tid < npr) {
-        const int b = tid >= np0;
-        const int i = b ? tid - np0 : tid;
-        const int mb = b ? m1 : m0, off = b ? m0 : 0;
         int p = 0, q = 0;
         double c = 1.0, s = 0.0, t = 0.0;
-        if (mb >= 2) {
-          const int rr = r % (mb - 1);
-          if (i == 0) {
-            p = mb - 1;
+        if (pmb >= 2) {
+          if (pi_ == 0) {
+            p = pmb - 1;
             q = rr;
           } else {
-            p = (rr + i) % (mb - 1);
-            q = (rr - i + mb - 1) % (mb - 1);
+            p = rr + pi_;
+            if (p >= pmb - 1) p -= pmb - 1;
+            q = rr - pi_;
+            if (q < 0) q += pmb - 1;
           }
-          p += off;
-          q += off;
-          const double app = S.R[p * BLD + p], aqq = S.R[q * BLD + q];
-          const double apq = S.R[(p < q ? p : q) * BLD + (p < q ? q : p)];
+          p += poff;
+          q += poff;
+          const double app = S.R[p * LD + p], aqq = S.R[q * LD + q];
+          const double apq = S.R[(p < q ? p : q) * LD + (p < q ? q : p)];
           if (fabs(apq) > fmax(1e-15 * sqrt(fabs(app * aqq)), BM_ROT_FLOOR)) {
             const double th = (aqq - app) / (2.0 * apq);
             t = fabs(th) > 1e150 ? 0.5 / th : copysign(1.0, th) / (fabs(th) + sqrt(th * th + 1.0));
@@ -253,6 +302,7 @@ __device__ void bm_jacobi(BmSh &S, int m0, int m1) {
             s = t * c;
             S.rot = 1;
           }
+          if (++rr == pmb - 1) rr = 0;
         }
         S.jp[tid] = p;
         S.jq[tid] = q;
@@ -261,38 +311,31 @@ __device__ void bm_jacobi(BmSh &S, int m0, int m1) {
         S.jt[tid] = t;
       }
       __syncthreads();
-      // rho <- J^T rho J, one 2 x 2 block (pair i1 rows, pair i2 columns) per item
-      const int nb0 = np0 * np0, nb1 = np1 * np1;
-      for (int wi = tid; wi < nb0 + nb1; wi += BT) {
-        int i1, i2;
-        if (wi < nb0) {
-          i1 = wi / np0;
-          i2 = wi - i1 * np0;
-        } else {
-          const int w2 = wi - nb0;
-          i1 = np0 + w2 / np1;
-          i2 = np0 + w2 % np1;
-        }
-        if (i1 > i2) continue;                 // upper triangle of pair blocks
+      // rho <- J^T rho J on the upper triangle of pair blocks
+#pragma unroll
+      for (int k = 0; k < NBK; ++k) {
+        const int i1 = bk1[k], i2 = bk2[k];
+        if (i1 < 0) continue;
         const double s1 = S.js[i1], s2 = S.js[i2];
         if (s1 == 0.0 && s2 == 0.0) continue;
-        const int p1 = S.jp[i1], q1 = S.jq[i1], p2 = S.jp[i2], q2 = S.jq[i2];
+        const int p1 = S.jp[i1], q1 = S.jq[i1];
         if (i1 == i2) {
           // the rotated pair itself: its off-diagonal entry becomes zero
           const double t = S.jt[i1];
           const int lo = p1 < q1 ? p1 : q1, hi = p1 < q1 ? q1 : p1;
-          const double apq = S.R[lo * BLD + hi];
-          S.R[p1 * BLD + p1] -= t * apq;
-          S.R[q1 * BLD + q1] += t * apq;
-          S.R[lo * BLD + hi] = 0.0;
+          const double apq = S.R[lo * LD + hi];
+          S.R[p1 * LD + p1] -= t * apq;
+          S.R[q1 * LD + q1] += t * apq;
+          S.R[lo * LD + hi] = 0.0;
           continue;
         }
+        const int p2 = S.jp[i2], q2 = S.jq[i2];
         const double c1 = S.jc[i1], c2 = S.jc[i2];
         // entries of distinct pairs: never on the diagonal
-        const int e0 = p1 < p2 ? p1 * BLD + p2 : p2 * BLD + p1;
-        const int e1 = p1 < q2 ? p1 * BLD + q2 : q2 * BLD + p1;
-        const int e2 = q1 < p2 ? q1 * BLD + p2 : p2 * BLD + q1;
-        const int e3 = q1 < q2 ? q1 * BLD + q2 : q2 * BLD + q1;
+        const int e0 = p1 < p2 ? p1 * LD + p2 : p2 * LD + p1;
+        const int e1 = p1 < q2 ? p1 * LD + q2 : q2 * LD + p1;
+        const int e2 = q1 < p2 ? q1 * LD + p2 : p2 * LD + q1;
+        const int e3 = q1 < q2 ? q1 * LD + q2 : q2 * LD + q1;
         const double xpp = S.R[e0], xpq = S.R[e1], xqp = S.R[e2], xqq = S.R[e3];
         const double ypp = c1 * xpp - s1 * xqp, ypq = c1 * xpq - s1 * xqq;
         const double yqp = s1 * xpp + c1 * xqp, yqq = s1 * xpq + c1 * xqq;
@@ -301,33 +344,17 @@ __device__ void bm_jacobi(BmSh &S, int m0, int m1) {
         S.R[e2] = yqp * c2 - yqq * s2;
         S.R[e3] = yqp * s2 + yqq * c2;
       }
-      // V <- V J over the rows of each block (up to four items per thread,
-      // every load issued before the stores)
-      const int nv0 = m0 * np0, nv1 = m1 * np1;
+      // V <- V J over the rows of each block (loads issued before the stores)
       {
-        constexpr int NV = (BN * BN / 2) / BT;
         int ip[NV], iq[NV];
         double vp[NV], vq[NV], cc[NV], ss[NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
-          const int wi = tid + k * BT;
-          ss[k] = 0.0;
-          ip[k] = iq[k] = 0;
-          if (wi < nv0 + nv1) {
-            int row, pi;
-            if (wi < nv0) {
-              row = wi / np0;
-              pi = wi - row * np0;
-            } else {
-              const int w2 = wi - nv0;
-              row = m0 + w2 / np1;
-              pi = np0 + w2 % np1;
-            }
-            ss[k] = S.js[pi];
-            cc[k] = S.jc[pi];
-            ip[k] = row * BLD + S.jp[pi];
-            iq[k] = row * BLD + S.jq[pi];
-          }
+          const bool on = vrow[k] >= 0;
+          ss[k] = on ? S.js[vpi[k]] : 0.0;
+          cc[k] = on ? S.jc[vpi[k]] : 1.0;
+          ip[k] = on ? vrow[k] * LD + S.jp[vpi[k]] : 0;
+          iq[k] = on ? vrow[k] * LD + S.jq[vpi[k]] : 0;
         }
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
@@ -358,9 +385,11 @@ __device__ void bm_jacobi(BmSh &S, int m0, int m1) {
 // after each (buffer i: rows 0 .. i from the top, Lx-1 .. Lx-1-i from the
 // bottom): the environments of every row (quimb's compute_x_environments);
 // with g.nbuf = 1 it stops at the middle and works in place.
-__global__ void __launch_bounds__(BT)
+template <int NB>
+__global__ void __launch_bounds__(BT, NB == 32 ? 3 : 1)
     peps_bmps_side_kernel(BmGeo g, int64_t rows, const int64_t *__restrict__ cfg,
                           const int32_t *__restrict__ flag, double *ws, int64_t *hdr) {
+  constexpr int LD = BmShT<NB>::LD;
   const int64_t w = blockIdx.x;
   if (w >= rows) return;
   if (flag && !(flag[w] & 1)) return;
@@ -368,7 +397,7 @@ __global__ void __launch_bounds__(BT)
   const bool down = side == 0;
   const int tid = threadIdx.x;
   __shared__ BmCfg cs;
-  __shared__ BmSh S;
+  __shared__ BmShT<NB> S;
   bm_load_cfg(g, cfg + w * g.N, cs);
   const double *th = g.theta + w * g.theta_stride;
   double *slot = ws + (w * 2 + side) * g.slot;
@@ -465,18 +494,18 @@ __global__ void __launch_bounds__(BT)
           double s = 0.0;
           for (int u = 0; u < D; ++u)
             s = fma(S.Bt[(a * D + u) * nb + b], S.Sl[((l * D + u) * D + v) * dr + r], s);
-          S.A[i * BLD + j] = s;
+          S.A[i * LD + j] = s;
         }
         __syncthreads();
-        bm_mm(n, m, m, S.A, S.R, S.T);
+        bm_mm<LD>(n, m, m, S.A, S.R, S.T);
         __syncthreads();
         if (i0 < n && j0 < n) {
           for (int k = 0; k < m; ++k) {
             double xa[4], yb[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-              xa[q] = (i0 + q < n) ? S.T[(i0 + q) * BLD + k] : 0.0;
-              yb[q] = (j0 + q < n) ? S.A[(j0 + q) * BLD + k] : 0.0;
+              xa[q] = (i0 + q < n) ? S.T[(i0 + q) * LD + k] : 0.0;
+              yb[q] = (j0 + q < n) ? S.A[(j0 + q) * LD + k] : 0.0;
             }
 #pragma unroll
             for (int a = 0; a < 4; ++a)
@@ -500,7 +529,7 @@ __global__ void __launch_bounds__(BT)
         for (int b = 0; b < 4; ++b)
           if (i0 + a < n && j0 + b < n) {
             const double v = acc[a][b] * inv;
-            S.R[(i0 + a) * BLD + j0 + b] = v;
+            S.R[(i0 + a) * LD + j0 + b] = v;
             Rd[(i0 + a) * BN + j0 + b] = v;
           }
       __syncthreads();
@@ -527,7 +556,7 @@ __global__ void __launch_bounds__(BT)
         const int kk = idx / (nb * D * dl);
         double s = 0.0;
         for (int a = 0; a < na; ++a)
-          s = fma(S.P[kk * BN + a * dl + l], S.Bt[(a * D + u) * nb + b], s);
+          s = fma(S.P[kk * NB + a * dl + l], S.Bt[(a * D + u) * nb + b], s);
         S.T[idx] = s;
       }
       __syncthreads();
@@ -540,12 +569,12 @@ __global__ void __launch_bounds__(BT)
           for (int u = 0; u < D; ++u)
             s = fma(S.T[((kk * dl + l) * D + u) * nb + b], S.Sl[((l * D + u) * D + v) * dr + r],
                     s);
-        S.A[i * BLD + j] = s;
+        S.A[i * LD + j] = s;
       }
       __syncthreads();
       if (y == Ly - 1) {
         // last site: B'[kk][v][0] = C[(kk,v)][0]
-        for (int idx = tid; idx < nC; idx += BT) S.T[idx] = S.A[idx * BLD];
+        for (int idx = tid; idx < nC; idx += BT) S.T[idx] = S.A[idx * LD];
         __syncthreads();
         const double l2 = bm_store_scaled(g, dst, y, k, 1, S.T, S.red);
         __syncthreads();
@@ -568,7 +597,7 @@ __global__ void __launch_bounds__(BT)
         }
         for (int idx = tid; idx < kn * m; idx += BT) {
           const int j = idx % m, i = idx / m;
-          S.P[i * BN + j] = S.A[i * BLD + j];
+          S.P[i * NB + j] = S.A[i * LD + j];
         }
         __syncthreads();
         if (tid < kn) S.sel[tid] = S.kpar[tid / D] ^ bqpar(tid % D, D);
@@ -580,7 +609,7 @@ __global__ void __launch_bounds__(BT)
         const double *Rs = Rg + (int64_t)(y + 1) * BN * BN;
         for (int idx = tid; idx < m * m; idx += BT) {
           const int i = idx / m, j = idx - i * m;
-          S.R[i * BLD + j] = Rs[i * BN + j];
+          S.R[i * LD + j] = Rs[i * BN + j];
         }
         if (tid == 0) {
           int n0 = 0;
@@ -601,7 +630,7 @@ __global__ void __launch_bounds__(BT)
         __syncthreads();
         const int m0 = S.sec[0], m1 = S.sec[1], mt = m0 + m1;
         // G = C R (into T)
-        bm_mm(nC, m, m, S.A, S.R, S.T);
+        bm_mm<LD>(nC, m, m, S.A, S.R, S.T);
         __syncthreads();
         // rho (permuted, block diagonal) = G C^T (into R)
         for (int idx = tid; idx < mt * mt; idx += BT) {
@@ -609,12 +638,12 @@ __global__ void __launch_bounds__(BT)
           const int i1 = S.perm[p1], i2 = S.perm[p2];
           double s = 0.0;
           if (p1 <= p2 && i1 >= 0 && i2 >= 0 && ((p1 < m0) == (p2 < m0)))
-            for (int j = 0; j < m; ++j) s = fma(S.T[i1 * BLD + j], S.A[i2 * BLD + j], s);
-          S.R[p1 * BLD + p2] = s;     // upper triangle only (the rest unused)
+            for (int j = 0; j < m; ++j) s = fma(S.T[i1 * LD + j], S.A[i2 * LD + j], s);
+          S.R[p1 * LD + p2] = s;     // upper triangle only (the rest unused)
         }
         __syncthreads();
         double dmx = 0.0;
-        for (int i = tid; i < mt; i += BT) dmx = fmax(dmx, fabs(S.R[i * BLD + i]));
+        for (int i = tid; i < mt; i += BT) dmx = fmax(dmx, fabs(S.R[i * LD + i]));
         dmx = bm_max(dmx, S.red);
         if (!(dmx > 0.0)) {
           dead = true;
@@ -623,17 +652,17 @@ __global__ void __launch_bounds__(BT)
         const double dinv = 1.0 / dmx;
         for (int idx = tid; idx < mt * mt; idx += BT) {
           const int p1 = idx / mt, p2 = idx - p1 * mt;
-          S.R[p1 * BLD + p2] *= dinv;
+          S.R[p1 * LD + p2] *= dinv;
         }
         __syncthreads();
-        bm_jacobi(S, m0, m1);
+        bm_jacobi<NB>(S, m0, m1);
         // the chi leading eigenvalues (ties by position), padding excluded
         if (tid < mt && S.perm[tid] >= 0) {
-          const double lam = S.R[tid * BLD + tid];
+          const double lam = S.R[tid * LD + tid];
           int rank = 0;
           for (int q = 0; q < mt; ++q) {
             if (S.perm[q] < 0) continue;
-            const double lq = S.R[q * BLD + q];
+            const double lq = S.R[q * LD + q];
             rank += (lq > lam) || (lq == lam && q < tid);
           }
           if (rank < kn) S.sel[rank] = tid;
@@ -642,7 +671,7 @@ __global__ void __launch_bounds__(BT)
         // B'[kk][v][j] = V[pos(kk,v)][sel j]
         for (int idx = tid; idx < nC * kn; idx += BT) {
           const int j = idx % kn, kv = idx / kn;
-          dsty[kv * g.cb + j] = S.T[S.pos[kv] * BLD + S.sel[j]];
+          dsty[kv * g.cb + j] = S.T[S.pos[kv] * LD + S.sel[j]];
         }
         // P'[j][(b,r)] = sum_pi V[pi][sel j] C[perm pi][(b,r)]
         for (int idx = tid; idx < kn * m; idx += BT) {
@@ -652,9 +681,9 @@ __global__ void __launch_bounds__(BT)
           double s = 0.0;
           for (int pi = lo; pi < hi; ++pi) {
             const int i = S.perm[pi];
-            if (i >= 0) s = fma(S.T[pi * BLD + sj], S.A[i * BLD + jj], s);
+            if (i >= 0) s = fma(S.T[pi * LD + sj], S.A[i * LD + jj], s);
           }
-          S.P[j * BN + jj] = s;
+          S.P[j * NB + jj] = s;
         }
         __syncthreads();
         if (tid < kn) S.kpar[tid] = S.sel[tid] >= m0;
@@ -664,7 +693,7 @@ __global__ void __launch_bounds__(BT)
       double pm = 0.0;
       for (int idx = tid; idx < kn * m; idx += BT) {
         const int j = idx % m, i = idx / m;
-        pm = fmax(pm, fabs(S.P[i * BN + j]));
+        pm = fmax(pm, fabs(S.P[i * NB + j]));
       }
       pm = bm_max(pm, S.red);
       if (!(pm > 0.0) || !std::isfinite(pm)) {
@@ -674,7 +703,7 @@ __global__ void __launch_bounds__(BT)
       const double pinv = 1.0 / pm;
       for (int idx = tid; idx < kn * m; idx += BT) {
         const int j = idx % m, i = idx / m;
-        S.P[i * BN + j] *= pinv;
+        S.P[i * NB + j] *= pinv;
       }
       lg += log(pm);
       if (tid == 0) {
@@ -1022,6 +1051,17 @@ BmGeo bm_geo(int Lx, int Ly, int D, int chi, int nbuf) {
   g.slot = (int64_t)nbuf * Ly * g.ts + (int64_t)Ly * BN * BN;
   return g;
 }
+// the side kernel with 32 x 32 working matrices when chi * D allows (about
+// 45 KB of LDS, several workgroups per CU), else 64 x 64 (one per CU)
+void launch_side(const BmGeo &g, int64_t n, const int64_t *c, const int32_t *f, double *slots,
+                 int64_t *hdr, hipStream_t st) {
+  if (g.cb * g.D <= 32)
+    hipLaunchKernelGGL(peps_bmps_side_kernel<32>, dim3((unsigned)n, 2), dim3(BT), 0, st, g, n,
+                       c, f, slots, hdr);
+  else
+    hipLaunchKernelGGL(peps_bmps_side_kernel<64>, dim3((unsigned)n, 2), dim3(BT), 0, st, g, n,
+                       c, f, slots, hdr);
+}
 }  // namespace
 
 const char *bmps_check(int Lx, int Ly, int D, int chi) {
@@ -1059,8 +1099,7 @@ int bmps_amplitudes(int Lx, int Ly, int D, int chi, const int32_t *dmap, const d
     g.theta_stride = theta_stride;
     const int64_t *c = cfg + r0 * g.N;
     const int32_t *f = flag ? flag + r0 : nullptr;
-    hipLaunchKernelGGL(peps_bmps_side_kernel, dim3((unsigned)n, 2), dim3(BT), 0, st, g, n, c, f,
-                       slots, hdr);
+    launch_side(g, n, c, f, slots, hdr, st);
     VMCX_LAUNCHED();
     hipLaunchKernelGGL(peps_bmps_join_kernel, dim3((unsigned)n), dim3(64), 0, st, g, n, c, f,
                        slots, hdr, 0, 0, sign + r0, logabs + r0);
@@ -1085,8 +1124,7 @@ int bmps_logderiv(int Lx, int Ly, int D, int chi, const int32_t *dmap, const dou
     g.theta_stride = theta_stride;
     const int64_t *c = cfg + r0 * g.N;
     double *Gr = G + r0 * ldg;
-    hipLaunchKernelGGL(peps_bmps_side_kernel, dim3((unsigned)n, 2), dim3(BT), 0, st, g, n, c,
-                       nullptr, slots, hdr);
+    launch_side(g, n, c, nullptr, slots, hdr, st);
     VMCX_LAUNCHED();
     hipLaunchKernelGGL(peps_bmps_join_kernel, dim3((unsigned)n), dim3(64), 0, st, g, n, c,
                        nullptr, slots, hdr, h - 1, Lx - h - 1, sign + r0, logabs + r0);
