@@ -509,7 +509,10 @@ typedef struct vmcx_peps_model {
                        cut after each absorbed row; chi <= 32, max(chi, D) * D <= 64,
                        Lx * Ly <= 64, Ly <= 16; FP64 amplitudes only (forward, sweep, local
                        energy) */
-  int32_t reserved;
+  int32_t bmps_nocache; /* with chi-truncation (plain fPEPS): 0 = sweeps keep every walker's
+                           boundary after each row and recompute a proposal's boundaries
+                           from its first changed row on (bit-identical to the full
+                           contraction); 1 = recompute every proposal from scratch */
 } vmcx_peps_model;
 
 /* Number of dense tensor positions (length of dense_map); -1 on bad sizes. */

@@ -276,3 +276,51 @@ def test_bmps_options(device):
     m = fPEPS_Model_GPU(4, 4, 2, max_bond=4)
     assert not m._bmps
     assert fPEPS_Model_GPU(4, 4, 2, max_bond=3)._bmps
+
+
+@pytest.mark.parametrize("Lx,Ly,D,chi", [(6, 4, 2, 3), (4, 4, 4, 8), (5, 4, 3, 5), (3, 4, 2, 2)])
+def test_bmps_sweep_cache_bit_identical(device, Lx, Ly, D, chi):
+    """Sweeps that resume every proposal from the walkers' cached boundaries
+    give the full recontraction's walkers, signs, ln|psi| and acceptance
+    counts bit for bit (same Philox draws)."""
+    N = Lx * Ly
+    m, sh, th = _plain(Lx, Ly, D, chi, 91 + Lx + D, device)
+    H = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, 2 * (N // 2),
+                                                   n_fermions_per_spin=(N // 2, N // 2))
+    x0 = torch.tensor(random_configs(np.random.default_rng(21), 64, N, N // 2, N // 2),
+                      device=device)
+    out = []
+    for cache in (True, False):
+        m.bmps_cache = cache
+        m._struct_obj = None
+        m._state = None          # fresh walker state (acceptance counts from zero)
+        xd = x0.clone()
+        sign, logabs, nacc = m.mcmc_sweep(xd, H.graph.edge_tensor(device, "sweep"), 2, 7, 0)
+        out.append((xd, sign.clone(), logabs.clone(), nacc.clone()))
+    m.bmps_cache = True
+    m._struct_obj = None
+    assert int(out[0][3].sum()) > 0
+    for a, b in zip(out[0], out[1]):
+        assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("Lx,Ly,D,chi", [(6, 4, 2, 3), (5, 4, 3, 5)])
+def test_bmps_local_energy_cache_bit_identical(device, Lx, Ly, D, chi):
+    """Local energies with every connected configuration resumed from its
+    walker's cached boundaries equal the full recontraction's bit for bit."""
+    N = Lx * Ly
+    m, sh, th = _plain(Lx, Ly, D, chi, 95 + Lx + D, device)
+    H = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, 2 * (N // 2),
+                                                   n_fermions_per_spin=(N // 2, N // 2))
+    x = torch.tensor(random_configs(np.random.default_rng(22), 40, N, N // 2, N // 2),
+                     device=device)
+    out = []
+    for cache in (True, False):
+        m.bmps_cache = cache
+        m._struct_obj = None
+        m._state = None
+        out.append(m.local_energy(x, H).clone())
+    m.bmps_cache = True
+    m._struct_obj = None
+    assert torch.isfinite(out[0]).all()
+    assert torch.equal(out[0], out[1])

@@ -131,7 +131,7 @@ class PepsModelStruct(ctypes.Structure):
                 ("hidden", ctypes.c_int32), ("act", ctypes.c_int32), ("eta", ctypes.c_double),
                 ("W1", _P), ("b1", _P), ("W2", _P), ("b2", _P), ("Wo", _P), ("bo", _P),
                 ("mode", ctypes.c_int32), ("amp_fp32", ctypes.c_int32),
-                ("chi", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("chi", ctypes.c_int32), ("bmps_nocache", ctypes.c_int32)]
 
 
 _LIB = None

@@ -52,11 +52,15 @@ constexpr int BMAX_SWEEPS = 30;
 // this floor are not rotated away (singular values below ~1e-9 of the
 // largest, which move an amplitude by about that much relative)
 constexpr double BM_ROT_FLOOR = 1e-18;
-constexpr int BHDR = 32;        // int64 header words per slot: log scale, bond dims
+// int64 header words per boundary buffer: [0] log scale (double bits),
+// [1 .. Ly+1] bond dimensions, [Ly+2 .. 2Ly+2] bond parity masks
+constexpr int BHDR = 40;
 
 struct BmGeo {
   int Lx, Ly, D, N, chi, cb;    // cb = max(chi, D): bound of a stored bond
-  int nbuf;                     // boundaries kept per side: 1 (amplitude) or Lx - 1
+  int nbuf;                     // boundaries kept per side: 1 (amplitude), Lx - 1 (every
+                                // row's environments) or max(h, Lx - h) (half, the cache)
+  int half;                     // buffered sides stop at the middle (the sweep cache)
   const int32_t *dmap;
   const double *theta;
   int64_t theta_stride;
@@ -386,6 +390,19 @@ __device__ void bm_jacobi(BmShT<NB> &S, int m0, int m1) {
 
 }  // namespace
 
+// the side steps a move between sites sa and sb (spin offset N allowed)
+// changes: st >= the side's row count means unchanged
+__device__ __forceinline__ void bm_move_starts(const BmGeo &g, uint32_t mv, int &st_top,
+                                               int &st_bot) {
+  int sa = mv & 0xFF, sb = (mv >> 8) & 0xFF;
+  if (sa >= g.N) sa -= g.N;
+  if (sb >= g.N) sb -= g.N;
+  const int ra = sa / g.Ly, rb = sb / g.Ly, h = g.Lx / 2;
+  const int rmin = ra < rb ? ra : rb, rmax = ra < rb ? rb : ra;
+  st_top = rmin <= h - 1 ? rmin : h;
+  st_bot = rmax >= h ? g.Lx - 1 - rmax : g.Lx - h;
+}
+
 // ------------------------------------------------------------ one side
 // grid (rows, 2): blockIdx.y = 0 the top side (rows 0 .. Lx/2-1, open down
 // legs), 1 the bottom side (rows Lx-1 .. Lx/2, open up legs).  With
@@ -396,7 +413,10 @@ __device__ void bm_jacobi(BmShT<NB> &S, int m0, int m1) {
 template <int NB>
 __global__ void __launch_bounds__(BT, NB == 32 ? 3 : 1)
     peps_bmps_side_kernel(BmGeo g, int64_t rows, const int64_t *__restrict__ cfg,
-                          const int32_t *__restrict__ flag, double *ws, int64_t *hdr) {
+                          const int32_t *__restrict__ flag, double *ws, int64_t *hdr,
+                          int st_top, int st_bot, const double *__restrict__ cache,
+                          const int64_t *__restrict__ chdr, const int32_t *__restrict__ parent,
+                          const uint32_t *__restrict__ cmove) {
   constexpr int LD = BmShT<NB>::LD;
   const int64_t w = blockIdx.x;
   if (w >= rows) return;
@@ -404,6 +424,12 @@ __global__ void __launch_bounds__(BT, NB == 32 ? 3 : 1)
   const int side = blockIdx.y;
   const bool down = side == 0;
   const int tid = threadIdx.x;
+  // rows 0 .. start-1 of this side are those of the cached walker (parent[w],
+  // or w): resume from its boundary after them (start >= nrows: the side is
+  // unchanged); per configuration from its move when cmove is given
+  if (cache && cmove) bm_move_starts(g, cmove[w], st_top, st_bot);
+  const int start = cache ? (down ? st_top : st_bot) : 0;
+  const int64_t pw = parent ? (int64_t)parent[w] : w;
   __shared__ BmCfg cs;
   __shared__ BmShT<NB> S;
   bm_load_cfg(g, cfg + w * g.N, cs);
@@ -413,16 +439,29 @@ __global__ void __launch_bounds__(BT, NB == 32 ? 3 : 1)
   double *Rg = slot + g.nbuf * bstride;
   int64_t *hd = hdr + (w * 2 + side) * g.nbuf * BHDR;
   const int Ly = g.Ly, D = g.D, h = g.Lx / 2;
-  const int nrows = g.nbuf > 1 ? g.Lx - 1 : (down ? h : g.Lx - h);
+  const int nrows = (g.nbuf > 1 && !g.half) ? g.Lx - 1 : (down ? h : g.Lx - h);
+  if (start >= nrows) return;
+  const int bufd = g.nbuf > 1 ? 1 : 0;   // buffer index step per row
+  const double *cslot = cache ? cache + (pw * 2 + side) * g.slot : nullptr;
+  const int64_t *chd = chdr ? chdr + (pw * 2 + side) * g.nbuf * BHDR : nullptr;
   const uint64_t dmask = D == 1 ? 0ull : (((1ull << D) - 1) & ~((1ull << (D >> 1)) - 1));
-  if (tid <= Ly) {
+  double lg = 0.0;
+  bool dead = false;
+  if (start > 0) {
+    const int64_t *c = chd + (start - 1) * bufd * BHDR;
+    if (tid <= Ly) {
+      S.bd[tid] = (int)c[1 + tid];
+      S.bpm[tid] = (uint64_t)c[Ly + 2 + tid];
+    }
+    lg = __longlong_as_double(c[0]);
+    dead = std::isinf(lg);
+  } else if (tid <= Ly) {
     S.bd[tid] = hdim(g, tid);
     S.bpm[tid] = hdim(g, tid) == 1 ? 0ull : dmask;
   }
-  double lg = 0.0;
-  bool dead = false;
+  __syncthreads();
   // first row: B_y[l][v][r] = S[l][0][v][r]
-  {
+  if (start == 0) {
     const int x = down ? 0 : g.Lx - 1;
     for (int y = 0; y < Ly && !dead; ++y) {
       bm_slice(g, cs, th, x, y, down, S.Sl);
@@ -433,14 +472,19 @@ __global__ void __launch_bounds__(BT, NB == 32 ? 3 : 1)
       if (std::isinf(l2)) dead = true;
       lg += l2;
     }
-    if (tid <= Ly) hd[1 + tid] = S.bd[tid];
+    if (tid <= Ly) {
+      hd[1 + tid] = S.bd[tid];
+      hd[Ly + 2 + tid] = (int64_t)S.bpm[tid];
+    }
     if (tid == 0) hd[0] = __double_as_longlong(dead ? -INFINITY : lg);
   }
-  for (int ri = 1; ri < nrows && !dead; ++ri) {
+  for (int ri = start > 1 ? start : 1; ri < nrows && !dead; ++ri) {
     const int x = down ? ri : g.Lx - 1 - ri;
-    // boundary after row ri - 1 -> after row ri (the same buffer when nbuf = 1)
-    const double *src = slot + (g.nbuf > 1 ? (ri - 1) * bstride : 0);
-    double *dst = slot + (g.nbuf > 1 ? ri * bstride : 0);
+    // boundary after row ri - 1 (the cache's for the first resumed row) ->
+    // after row ri (the same buffer when nbuf = 1)
+    const double *src = (start > 0 && ri == start) ? cslot + (ri - 1) * bstride
+                                                   : slot + (ri - 1) * bufd * bstride;
+    double *dst = slot + ri * bufd * bstride;
     if (tid <= Ly) {
       const int hd = hdim(g, tid), b = S.bd[tid];
       S.nbd[tid] = b * hd;
@@ -480,9 +524,11 @@ __global__ void __launch_bounds__(BT, NB == 32 ? 3 : 1)
         S.bpm[tid] = S.npm[tid];
       }
       __syncthreads();
-      if (tid <= Ly) hd[ri * (g.nbuf > 1) * BHDR + 1 + tid] = S.bd[tid];
-      if (tid == 0)
-        hd[ri * (g.nbuf > 1) * BHDR] = __double_as_longlong(dead ? -INFINITY : lg);
+      if (tid <= Ly) {
+        hd[ri * bufd * BHDR + 1 + tid] = S.bd[tid];
+        hd[ri * bufd * BHDR + Ly + 2 + tid] = (int64_t)S.bpm[tid];
+      }
+      if (tid == 0) hd[ri * bufd * BHDR] = __double_as_longlong(dead ? -INFINITY : lg);
       continue;
     }
     // ---- backward: right Grams R_y, y = Ly-1 .. 1 (R_Ly = 1)
@@ -729,12 +775,15 @@ __global__ void __launch_bounds__(BT, NB == 32 ? 3 : 1)
       S.bpm[tid] = S.npm[tid];
     }
     __syncthreads();
-    if (tid <= Ly) hd[ri * (g.nbuf > 1) * BHDR + 1 + tid] = S.bd[tid];
-    if (tid == 0) hd[ri * (g.nbuf > 1) * BHDR] = __double_as_longlong(dead ? -INFINITY : lg);
+    if (tid <= Ly) {
+      hd[ri * bufd * BHDR + 1 + tid] = S.bd[tid];
+      hd[ri * bufd * BHDR + Ly + 2 + tid] = (int64_t)S.bpm[tid];
+    }
+    if (tid == 0) hd[ri * bufd * BHDR] = __double_as_longlong(dead ? -INFINITY : lg);
   }
   // buffers never reached (a vanishing boundary): mark them zero
   if (dead && tid == 0)
-    for (int i = 0; i < g.nbuf; ++i) hd[i * BHDR] = __double_as_longlong(-INFINITY);
+    for (int i = start; i < nrows; ++i) hd[i * bufd * BHDR] = __double_as_longlong(-INFINITY);
 }
 
 // ------------------------------------------------------------ join
@@ -743,14 +792,27 @@ __global__ void __launch_bounds__(64)
     peps_bmps_join_kernel(BmGeo g, int64_t rows, const int64_t *__restrict__ cfg,
                           const int32_t *__restrict__ flag, const double *__restrict__ ws,
                           const int64_t *__restrict__ hdr, int bt, int bb, double *sign,
-                          double *logabs) {
+                          double *logabs, int ct, int cb_, const double *__restrict__ cache,
+                          const int64_t *__restrict__ chdr, const int32_t *__restrict__ parent,
+                          const uint32_t *__restrict__ cmove) {
   const int64_t w = blockIdx.x;
   if (w >= rows) return;
   if (flag && !(flag[w] & 1)) return;
   const int tid = threadIdx.x;
   __shared__ double E[BMAX_CHI * BMAX_CHI], X[BMAX_TS];
-  const int64_t *ht = hdr + ((w * 2) * g.nbuf + bt) * BHDR;
-  const int64_t *hb = hdr + ((w * 2 + 1) * g.nbuf + bb) * BHDR;
+  // a side taken from the cache (ct / cb_ set: unchanged since the walker
+  // parent[w], or w; per configuration from its move when cmove is given)
+  if (cache && cmove) {
+    int s0, s1;
+    bm_move_starts(g, cmove[w], s0, s1);
+    ct = s0 >= g.Lx / 2;
+    cb_ = s1 >= g.Lx - g.Lx / 2;
+  }
+  const int64_t pw = parent ? (int64_t)parent[w] : w;
+  const int64_t *ht = ct ? chdr + ((pw * 2) * g.nbuf + bt) * BHDR
+                         : hdr + ((w * 2) * g.nbuf + bt) * BHDR;
+  const int64_t *hb = cb_ ? chdr + ((pw * 2 + 1) * g.nbuf + bb) * BHDR
+                          : hdr + ((w * 2 + 1) * g.nbuf + bb) * BHDR;
   const double lt = __longlong_as_double(ht[0]), lb = __longlong_as_double(hb[0]);
   const double lo = log(1e-45);
   int P = 0;
@@ -766,8 +828,10 @@ __global__ void __launch_bounds__(64)
     return;
   }
   const int64_t bstride = (int64_t)g.Ly * g.ts;
-  const double *Tt = ws + (w * 2) * g.slot + bt * bstride;
-  const double *Tb = ws + (w * 2 + 1) * g.slot + bb * bstride;
+  const double *Tt = ct ? cache + (pw * 2) * g.slot + bt * bstride
+                        : ws + (w * 2) * g.slot + bt * bstride;
+  const double *Tb = cb_ ? cache + (pw * 2 + 1) * g.slot + bb * bstride
+                         : ws + (w * 2 + 1) * g.slot + bb * bstride;
   const int D = g.D, cb = g.cb;
   if (tid == 0) E[0] = 1.0;
   __syncthreads();
@@ -1035,6 +1099,32 @@ __global__ void __launch_bounds__(BT)
   }
 }
 
+// the boundaries an accepted proposal recomputed (buffers start .. nrows-1
+// of each side, and their headers) into its walker's cache
+__global__ void __launch_bounds__(256)
+    bm_commit_kernel(BmGeo g, int64_t rows, const int32_t *__restrict__ acc, int st_top,
+                     int st_bot, const double *__restrict__ ws, const int64_t *__restrict__ hdr,
+                     double *cache, int64_t *chdr) {
+  const int64_t w = blockIdx.x;
+  if (w >= rows || !acc[w]) return;
+  const int side = blockIdx.y;
+  const bool down = side == 0;
+  const int h = g.Lx / 2, nrows = down ? h : g.Lx - h;
+  const int start = down ? st_top : st_bot;
+  if (start >= nrows) return;
+  const int bufd = g.nbuf > 1 ? 1 : 0;
+  const int64_t bstride = (int64_t)g.Ly * g.ts;
+  const double *src = ws + (w * 2 + side) * g.slot;
+  double *dst = cache + (w * 2 + side) * g.slot;
+  const int64_t *hs = hdr + (w * 2 + side) * g.nbuf * BHDR;
+  int64_t *hc = chdr + (w * 2 + side) * g.nbuf * BHDR;
+  for (int b = start; b < nrows; ++b) {
+    const int64_t o = b * bufd * bstride;
+    for (int64_t i = threadIdx.x; i < bstride; i += 256) dst[o + i] = src[o + i];
+    for (int i = threadIdx.x; i < BHDR; i += 256) hc[b * bufd * BHDR + i] = hs[b * bufd * BHDR + i];
+  }
+}
+
 __global__ void bm_zero_rows_kernel(int64_t rows, int64_t P, double *G, int64_t ldg) {
   const int64_t n = rows * P;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -1046,8 +1136,9 @@ __global__ void bm_zero_rows_kernel(int64_t rows, int64_t P, double *G, int64_t 
 
 // ------------------------------------------------------------ host side
 namespace {
-BmGeo bm_geo(int Lx, int Ly, int D, int chi, int nbuf) {
+BmGeo bm_geo(int Lx, int Ly, int D, int chi, int nbuf, int half = 0) {
   BmGeo g{};
+  g.half = half;
   g.Lx = Lx;
   g.Ly = Ly;
   g.D = D;
@@ -1062,13 +1153,15 @@ BmGeo bm_geo(int Lx, int Ly, int D, int chi, int nbuf) {
 // the side kernel with 32 x 32 working matrices when chi * D allows (about
 // 45 KB of LDS, several workgroups per CU), else 64 x 64 (one per CU)
 void launch_side(const BmGeo &g, int64_t n, const int64_t *c, const int32_t *f, double *slots,
-                 int64_t *hdr, hipStream_t st) {
+                 int64_t *hdr, hipStream_t st, int st_top = 0, int st_bot = 0,
+                 const double *cache = nullptr, const int64_t *chdr = nullptr,
+                 const int32_t *parent = nullptr, const uint32_t *cmove = nullptr) {
   if (g.cb * g.D <= 32)
     hipLaunchKernelGGL(peps_bmps_side_kernel<32>, dim3((unsigned)n, 2), dim3(BT), 0, st, g, n,
-                       c, f, slots, hdr);
+                       c, f, slots, hdr, st_top, st_bot, cache, chdr, parent, cmove);
   else
     hipLaunchKernelGGL(peps_bmps_side_kernel<64>, dim3((unsigned)n, 2), dim3(BT), 0, st, g, n,
-                       c, f, slots, hdr);
+                       c, f, slots, hdr, st_top, st_bot, cache, chdr, parent, cmove);
 }
 }  // namespace
 
@@ -1110,9 +1203,130 @@ int bmps_amplitudes(int Lx, int Ly, int D, int chi, const int32_t *dmap, const d
     launch_side(g, n, c, f, slots, hdr, st);
     VMCX_LAUNCHED();
     hipLaunchKernelGGL(peps_bmps_join_kernel, dim3((unsigned)n), dim3(64), 0, st, g, n, c, f,
-                       slots, hdr, 0, 0, sign + r0, logabs + r0);
+                       slots, hdr, 0, 0, sign + r0, logabs + r0, 0, 0, nullptr, nullptr, nullptr,
+                       nullptr);
     VMCX_LAUNCHED();
   }
+  return 0;
+}
+
+// ---- the sweep's boundary cache (plain fPEPS: the site tensors do not
+// depend on the configuration).  Every walker keeps the boundary of each
+// side after each of its rows (top: rows 0 .. h-1, bottom: Lx-1 .. h); a
+// proposal on an edge between rows rmin <= rmax recomputes a side only
+// from its first changed row on, starting from the walker's boundary before
+// it, and takes an unchanged side from the cache as it is.  The arithmetic
+// of a recomputed row is that of the full contraction on the same inputs, so
+// the amplitudes equal the full recomputation's bit for bit.
+namespace {
+BmGeo bm_cache_geo(int Lx, int Ly, int D, int chi) {
+  const int h = Lx / 2;
+  return bm_geo(Lx, Ly, D, chi, h > Lx - h ? h : Lx - h, 1);
+}
+struct BmCacheWs {
+  double *cache, *prop;
+  int64_t *chdr, *phdr;
+  int32_t *acc;
+};
+BmCacheWs bm_cache_ws(const BmGeo &g, int64_t B, void *ws) {
+  BmCacheWs c;
+  const int64_t sd = B * 2 * g.slot;
+  const int64_t hw = B * 2 * g.nbuf * BHDR;
+  c.cache = (double *)ws;
+  c.prop = c.cache + sd;
+  c.chdr = (int64_t *)(c.prop + sd);
+  c.phdr = c.chdr + hw;
+  c.acc = (int32_t *)(c.phdr + hw);
+  return c;
+}
+}  // namespace
+
+int64_t bmps_sweep_cache_bytes(int Lx, int Ly, int D, int chi, int64_t B) {
+  const BmGeo g = bm_cache_geo(Lx, Ly, D, chi);
+  return 2 * B * 2 * g.slot * 8 + 2 * B * 2 * g.nbuf * BHDR * 8 + B * 4;
+}
+
+int32_t *bmps_sweep_accept_flags(int Lx, int Ly, int D, int chi, int64_t B, void *ws) {
+  return bm_cache_ws(bm_cache_geo(Lx, Ly, D, chi), B, ws).acc;
+}
+
+int bmps_sweep_init(int Lx, int Ly, int D, int chi, const int32_t *dmap, const double *theta,
+                    int64_t B, const int64_t *cfg, void *ws, hipStream_t st) {
+  BmGeo g = bm_cache_geo(Lx, Ly, D, chi);
+  g.dmap = dmap;
+  g.theta = theta;
+  g.theta_stride = 0;
+  const BmCacheWs c = bm_cache_ws(g, B, ws);
+  launch_side(g, B, cfg, nullptr, c.cache, c.chdr, st);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+// the side steps an edge between rows r0 and r1 changes
+static void bm_edge_starts(int Lx, int r0, int r1, int &st_top, int &st_bot) {
+  const int h = Lx / 2;
+  const int rmin = r0 < r1 ? r0 : r1, rmax = r0 < r1 ? r1 : r0;
+  st_top = rmin <= h - 1 ? rmin : h;
+  st_bot = rmax >= h ? Lx - 1 - rmax : Lx - h;
+}
+
+int bmps_sweep_amplitudes(int Lx, int Ly, int D, int chi, const int32_t *dmap,
+                          const double *theta, int64_t B, const int64_t *pcfg,
+                          const int32_t *flag, int r0, int r1, void *ws, double *sign,
+                          double *logabs, hipStream_t st) {
+  BmGeo g = bm_cache_geo(Lx, Ly, D, chi);
+  g.dmap = dmap;
+  g.theta = theta;
+  g.theta_stride = 0;
+  const BmCacheWs c = bm_cache_ws(g, B, ws);
+  int st_top, st_bot;
+  bm_edge_starts(Lx, r0, r1, st_top, st_bot);
+  const int h = Lx / 2;
+  launch_side(g, B, pcfg, flag, c.prop, c.phdr, st, st_top, st_bot, c.cache, c.chdr);
+  VMCX_LAUNCHED();
+  hipLaunchKernelGGL(peps_bmps_join_kernel, dim3((unsigned)B), dim3(64), 0, st, g, B, pcfg, flag,
+                     c.prop, c.phdr, h - 1, Lx - h - 1, sign, logabs, st_top >= h ? 1 : 0,
+                     st_bot >= Lx - h ? 1 : 0, c.cache, c.chdr, nullptr, nullptr);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
+// amplitudes of T connected configurations, each one move (cmove) away from
+// its walker cparent, from the walkers' cached boundaries (bmps_sweep_init
+// of the walkers first), in passes of B through the proposal slots
+int bmps_conn_amplitudes(int Lx, int Ly, int D, int chi, const int32_t *dmap,
+                         const double *theta, int64_t B, int64_t T, const int64_t *etas,
+                         const int32_t *cparent, const uint32_t *cmove, void *ws, double *sign,
+                         double *logabs, hipStream_t st) {
+  BmGeo g = bm_cache_geo(Lx, Ly, D, chi);
+  g.dmap = dmap;
+  g.theta = theta;
+  g.theta_stride = 0;
+  const BmCacheWs c = bm_cache_ws(g, B, ws);
+  const int h = Lx / 2;
+  for (int64_t r0 = 0; r0 < T; r0 += B) {
+    const int64_t n = T - r0 < B ? T - r0 : B;
+    const int64_t *e = etas + r0 * g.N;
+    launch_side(g, n, e, nullptr, c.prop, c.phdr, st, 0, 0, c.cache, c.chdr, cparent + r0,
+                cmove + r0);
+    VMCX_LAUNCHED();
+    hipLaunchKernelGGL(peps_bmps_join_kernel, dim3((unsigned)n), dim3(64), 0, st, g, n, e,
+                       nullptr, c.prop, c.phdr, h - 1, Lx - h - 1, sign + r0, logabs + r0, 0, 0,
+                       c.cache, c.chdr, cparent + r0, cmove + r0);
+    VMCX_LAUNCHED();
+  }
+  return 0;
+}
+
+int bmps_sweep_commit(int Lx, int Ly, int D, int chi, int64_t B, int r0, int r1, void *ws,
+                      hipStream_t st) {
+  const BmGeo g = bm_cache_geo(Lx, Ly, D, chi);
+  const BmCacheWs c = bm_cache_ws(g, B, ws);
+  int st_top, st_bot;
+  bm_edge_starts(Lx, r0, r1, st_top, st_bot);
+  hipLaunchKernelGGL(bm_commit_kernel, dim3((unsigned)B, 2), dim3(256), 0, st, g, B, c.acc,
+                     st_top, st_bot, c.prop, c.phdr, c.cache, c.chdr);
+  VMCX_LAUNCHED();
   return 0;
 }
 
@@ -1135,7 +1349,8 @@ int bmps_logderiv(int Lx, int Ly, int D, int chi, const int32_t *dmap, const dou
     launch_side(g, n, c, nullptr, slots, hdr, st);
     VMCX_LAUNCHED();
     hipLaunchKernelGGL(peps_bmps_join_kernel, dim3((unsigned)n), dim3(64), 0, st, g, n, c,
-                       nullptr, slots, hdr, h - 1, Lx - h - 1, sign + r0, logabs + r0);
+                       nullptr, slots, hdr, h - 1, Lx - h - 1, sign + r0, logabs + r0, 0, 0,
+                       nullptr, nullptr, nullptr, nullptr);
     VMCX_LAUNCHED();
     const int64_t nz = n * n_params;
     const unsigned gz = (unsigned)((nz + 255) / 256 < 16384 ? (nz + 255) / 256 : 16384);

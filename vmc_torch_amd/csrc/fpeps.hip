@@ -1103,18 +1103,21 @@ __global__ void peps_prop_cfg_kernel(int64_t B, int N, int i, int j,
 }
 
 // Metropolis test with |psi'|^2 / |psi|^2 and commit
+// (acc, when given: 1 for the walkers that accepted, else 0)
 __global__ void peps_accept_kernel(int64_t B, int N, int i, int j, int64_t *cfg,
                                    const int32_t *__restrict__ prop,
                                    const double *__restrict__ u_acc,
                                    const double *__restrict__ psign,
                                    const double *__restrict__ plog, double *sign,
-                                   double *logabs, int64_t *n_accept) {
+                                   double *logabs, int64_t *n_accept, int32_t *acc) {
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= B) return;
+  if (acc) acc[w] = 0;
   const int p = prop[w];
   if (!(p & 1)) return;
   const double ratio = exp(2.0 * (plog[w] - logabs[w]));
   if (!(u_acc[w] < ratio)) return;
+  if (acc) acc[w] = 1;
   cfg[w * N + i] = (p >> 8) & 0xF;
   cfg[w * N + j] = (p >> 12) & 0xF;
   sign[w] = psign[w];
@@ -1136,8 +1139,9 @@ struct PepsWs {
   // FP32 amplitudes of a shared parameter vector: the signed slice table
   float *tbl;
   // chi-truncated boundary MPS: slots of bm_pass configurations (bm_gpass
-  // for the log-derivative's per-row environments)
-  void *bm;
+  // for the log-derivative's per-row environments); the sweep's boundary
+  // cache of the B walkers (plain fPEPS)
+  void *bm, *bmc;
   int64_t bm_pass, bm_gpass;
   // environment reuse: the walkers' column environments, and the walker and
   // hop of each connected configuration
@@ -1206,9 +1210,12 @@ PepsWs peps_layout(const vmcx_peps_model &m, int64_t B, int64_t chunk, int n_edg
     const int64_t ba = bmps_workspace_bytes(m.Lx, m.Ly, m.D, m.chi, L.bm_pass);
     const int64_t bg = bmps_logderiv_workspace_bytes(m.Lx, m.Ly, m.D, m.chi, L.bm_gpass);
     L.bm = take((size_t)(ba > bg ? ba : bg));
+    if (H == 0 && B > 0) L.bmc = take((size_t)bmps_sweep_cache_bytes(m.Lx, m.Ly, m.D, m.chi, B));
   }
-  if (reuse) {
-    L.env = (double *)take((size_t)B * env_stride(m.Ly) * 8);
+  if (reuse) L.env = (double *)take((size_t)B * env_stride(m.Ly) * 8);
+  if (reuse || (use_bmps(m) && H == 0 && B > 0)) {
+    // the walker and hop of each connected configuration (environment reuse,
+    // and the boundary cache of a chi-truncated plain fPEPS)
     L.cparent = (int32_t *)take((size_t)T * 4);
     L.cmove = (uint32_t *)take((size_t)T * 4);
   }
@@ -1424,6 +1431,16 @@ int peps_sweep(const vmcx_peps_model *m, int64_t B, int n_edges, const int32_t *
   }
   const unsigned gw = (unsigned)((B + 255) / 256);
   const unsigned gc = (unsigned)((B * N + 255) / 256);
+  // chi-truncated plain fPEPS: proposals resume from the walkers' cached
+  // boundaries (bit-identical to the full contraction)
+  const bool cached = use_bmps(*m) && m->hidden == 0 && !m->bmps_nocache;
+  int32_t *acc = nullptr;
+  if (cached) {
+    const int rc = bmps_sweep_init(m->Lx, m->Ly, m->D, m->chi, m->dense_map, m->theta, B, cfg,
+                                   L.bmc, st);
+    if (rc) return rc;
+    acc = bmps_sweep_accept_flags(m->Lx, m->Ly, m->D, m->chi, B, L.bmc);
+  }
   for (int sw = 0; sw < n_sweeps; ++sw) {
     for (int e = 0; e < n_edges; ++e) {
       const int i = edges_host[2 * e], j = edges_host[2 * e + 1];
@@ -1435,11 +1452,19 @@ int peps_sweep(const vmcx_peps_model *m, int64_t B, int n_edges, const int32_t *
       hipLaunchKernelGGL(peps_prop_cfg_kernel, dim3(gc), dim3(256), 0, st, B, N, i, j, cfg,
                          L.prop, L.pcfg);
       VMCX_LAUNCHED();
-      int rc = amp_rows(*m, L, B, L.pcfg, L.prop, L.psign, L.plog, st);
+      const int ri = i / m->Ly, rj = j / m->Ly;
+      int rc = cached ? bmps_sweep_amplitudes(m->Lx, m->Ly, m->D, m->chi, m->dense_map,
+                                              m->theta, B, L.pcfg, L.prop, ri, rj, L.bmc,
+                                              L.psign, L.plog, st)
+                      : amp_rows(*m, L, B, L.pcfg, L.prop, L.psign, L.plog, st);
       if (rc) return rc;
       hipLaunchKernelGGL(peps_accept_kernel, dim3(gw), dim3(256), 0, st, B, N, i, j, cfg,
-                         L.prop, L.uacc, L.psign, L.plog, sign, logabs, n_accept);
+                         L.prop, L.uacc, L.psign, L.plog, sign, logabs, n_accept, acc);
       VMCX_LAUNCHED();
+      if (cached) {
+        rc = bmps_sweep_commit(m->Lx, m->Ly, m->D, m->chi, B, ri, rj, L.bmc, st);
+        if (rc) return rc;
+      }
     }
   }
   return 0;
@@ -1463,19 +1488,30 @@ int peps_local_energy(const vmcx_peps_model *m, int64_t B, int n_edges, const in
   hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, st, B, L.counts, L.offsets,
                      L.total);
   VMCX_LAUNCHED();
+  const bool cached = use_bmps(*m) && m->hidden == 0 && !m->bmps_nocache;
   {
     const int rc = launch_conn_fill(B, N, n_edges, edges, t, cfg, L.offsets, L.etas, L.coeffs,
-                                    st);
+                                    st, cached ? L.cparent : nullptr, cached ? L.cmove : nullptr);
     if (rc) return rc;
   }
   int64_t T = 0;
   VMCX_HIP(hipMemcpyAsync(&T, L.total, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   VMCX_HIP(hipStreamSynchronize(st));
-  for (int64_t r0 = 0; r0 < T; r0 += chunk) {
-    const int64_t rows = (T - r0 < chunk) ? T - r0 : chunk;
-    const int rc = amp_rows(*m, L, rows, L.etas + r0 * N, nullptr, L.csign + r0, L.clog + r0,
-                            st);
+  if (cached) {
+    // every connected configuration from its walker's cached boundaries
+    int rc = bmps_sweep_init(m->Lx, m->Ly, m->D, m->chi, m->dense_map, m->theta, B, cfg, L.bmc,
+                             st);
     if (rc) return rc;
+    rc = bmps_conn_amplitudes(m->Lx, m->Ly, m->D, m->chi, m->dense_map, m->theta, B, T, L.etas,
+                              L.cparent, L.cmove, L.bmc, L.csign, L.clog, st);
+    if (rc) return rc;
+  } else {
+    for (int64_t r0 = 0; r0 < T; r0 += chunk) {
+      const int64_t rows = (T - r0 < chunk) ? T - r0 : chunk;
+      const int rc = amp_rows(*m, L, rows, L.etas + r0 * N, nullptr, L.csign + r0, L.clog + r0,
+                              st);
+      if (rc) return rc;
+    }
   }
   hipLaunchKernelGGL(eloc_segment_kernel, dim3(gw), dim3(256), 0, st, B, N, U, cfg, L.offsets,
                      L.counts, L.coeffs, L.csign, L.clog, sign, logabs, eloc);
@@ -1543,7 +1579,7 @@ int peps_sweep_reuse(const vmcx_peps_model *m, int64_t B, int n_edges, const int
       hipLaunchKernelGGL(peps_amp_env_kernel, dim3((unsigned)B), dim3(PT), 0, st, A, B, L.pcfg,
                          nullptr, nullptr, c0, c1, L.prop, L.env, L.psign, L.plog);
       hipLaunchKernelGGL(peps_accept_kernel, dim3(gw), dim3(256), 0, st, B, N, i, j, cfg,
-                         L.prop, L.uacc, L.psign, L.plog, sign, logabs, n_accept);
+                         L.prop, L.uacc, L.psign, L.plog, sign, logabs, n_accept, nullptr);
       VMCX_LAUNCHED();
       if (Lv > c0) Lv = c0;
       if (Rv < c1 + 1) Rv = c1 + 1;

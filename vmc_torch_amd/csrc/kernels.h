@@ -173,6 +173,24 @@ int bmps_amplitudes(int Lx, int Ly, int D, int chi, const int32_t *dmap, const d
                     int64_t theta_stride, int64_t rows, const int64_t *cfg, const int32_t *flag,
                     double *sign, double *logabs, void *ws, int64_t rows_per_pass,
                     hipStream_t st);
+// the sweep's boundary cache of plain fPEPS (B walkers): workspace, the
+// accept flags inside it, the walkers' boundaries, one edge's proposals
+// (rows r0, r1 changed; amplitudes bit-identical to the full contraction)
+// and the accepted proposals' boundaries into the cache
+int64_t bmps_sweep_cache_bytes(int Lx, int Ly, int D, int chi, int64_t B);
+int32_t *bmps_sweep_accept_flags(int Lx, int Ly, int D, int chi, int64_t B, void *ws);
+int bmps_sweep_init(int Lx, int Ly, int D, int chi, const int32_t *dmap, const double *theta,
+                    int64_t B, const int64_t *cfg, void *ws, hipStream_t st);
+int bmps_sweep_amplitudes(int Lx, int Ly, int D, int chi, const int32_t *dmap,
+                          const double *theta, int64_t B, const int64_t *pcfg,
+                          const int32_t *flag, int r0, int r1, void *ws, double *sign,
+                          double *logabs, hipStream_t st);
+int bmps_sweep_commit(int Lx, int Ly, int D, int chi, int64_t B, int r0, int r1, void *ws,
+                      hipStream_t st);
+int bmps_conn_amplitudes(int Lx, int Ly, int D, int chi, const int32_t *dmap,
+                         const double *theta, int64_t B, int64_t T, const int64_t *etas,
+                         const int32_t *cparent, const uint32_t *cmove, void *ws, double *sign,
+                         double *logabs, hipStream_t st);
 // d ln|psi| / d theta with the boundary environments held fixed (G rows of
 // n_params at stride ldg, FP64), and (sign, logabs)
 int64_t bmps_logderiv_workspace_bytes(int Lx, int Ly, int D, int chi, int64_t rows_per_pass);

@@ -541,14 +541,16 @@ def peps_layout(Lx, Ly, D):
 
 
 def peps_struct(Lx, Ly, D, dense_map, theta, nn=None, act=0, eta=0.0, mode=0, amp_fp32=False,
-                chi=0):
+                chi=0, bmps_cache=True):
     """vmcx_peps_model from device tensors: dense_map int32, theta float64,
     nn = dict W1, b1, W2, b2, Wo, bo (float64) or None; mode 0 = additive,
     1 = multiplicative network correction; amp_fp32: amplitudes contracted
-    in FP32 on the matrix cores; chi > 0: max_bond of the boundary MPS."""
+    in FP32 on the matrix cores; chi > 0: max_bond of the boundary MPS;
+    bmps_cache: sweeps of a truncated plain fPEPS resume proposals from the
+    walkers' cached boundaries (False: every proposal from scratch)."""
     s = _lib.PepsModelStruct()
     s.mode, s.amp_fp32 = int(mode), 1 if amp_fp32 else 0
-    s.chi, s.reserved = max(int(chi), 0), 0
+    s.chi, s.bmps_nocache = max(int(chi), 0), 0 if bmps_cache else 1
     require(dense_map, I32, "dense_map")
     require(theta, F64, "theta")
     s.Lx, s.Ly, s.D, s.n_params = int(Lx), int(Ly), int(D), int(theta.numel())

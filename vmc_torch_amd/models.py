@@ -553,6 +553,9 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
         self.eta = float(eta)
         self.mode = int(_mode)
         self.amp = amp
+        # sweeps of a truncated plain fPEPS resume each proposal from the
+        # walker's cached boundaries (same amplitudes, bit for bit)
+        self.bmps_cache = True
         self._struct_obj = None
         self._ws = {}
 
@@ -569,7 +572,8 @@ class fPEPS_Model_GPU(WavefunctionModel_GPU):
             self._struct_obj = kernels.peps_struct(
                 self.Lx, self.Ly, self.D, self.dense_map, self.params[0].data,
                 self._nn_dict(), 0 if self.act_name == "relu" else 1, self.eta, self.mode,
-                amp_fp32=self.amp == "fp32", chi=self.chi if self._bmps else 0)
+                amp_fp32=self.amp == "fp32", chi=self.chi if self._bmps else 0,
+                bmps_cache=self.bmps_cache)
             self._state_key = None
             return True
         return False
