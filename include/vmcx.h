@@ -658,6 +658,18 @@ int vmcx_lp_accept(int64_t B, int n_sites, uint64_t seed, int64_t step, const do
                    const double *plogabs, const double *logq_fwd, const double *logq_rev,
                    int64_t *configs, double *sign, double *logabs, int64_t *n_accept,
                    uint8_t *accepted, const int64_t *step_dev, void *stream);
+/* vmcx_lp_accept, vmcx_lp_commit_logits and (z1 != NULL) vmcx_masked_rows of
+ * the walkers' state rows (z1p -> z1, row_bytes a multiple of 4) in one
+ * launch, one wave per walker: the same decisions and results bit for bit.
+ * Replaces: the accept / commit step of the learned-proposal chain
+ * (GPU/sampler.py:216-388 with the MH rule of sampler.py:2214-2218). */
+int vmcx_lp_accept_commit(int64_t B, int n_sites, uint64_t seed, int64_t step,
+                          const double *u_acc, const int32_t *move, const int64_t *proposed,
+                          const double *psign, const double *plogabs, const double *logq_fwd,
+                          const double *logq_rev, int64_t *configs, double *sign, double *logabs,
+                          int64_t *n_accept, uint8_t *accepted, const int64_t *step_dev,
+                          int n_moves, const double *logits_p, double *logits_x,
+                          int64_t row_bytes, const void *z1p, void *z1, void *stream);
 /* grad[n_params] = sum_b d(sum_m dlogits[b][m] logits[b][m]) / d params
  * (FP32; per-configuration rows in `workspace`, summed in a fixed order). */
 int64_t vmcx_lp_backward_workspace_bytes(int C, int64_t B);

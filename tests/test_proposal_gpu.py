@@ -348,3 +348,34 @@ def test_cached_logits_chain_equals_full_evaluation(device, C, precision):
         ps, pl = model.forward_log(prop)
         kernels.lp_accept(b, sign, logabs, move, prop, ps, pl, lqf, lqr, 24, step)
     assert torch.equal(a, b)
+
+
+def test_accept_commit_fused_matches_separate(device):
+    """vmcx_lp_accept_commit equals vmcx_lp_accept + vmcx_lp_commit_logits +
+    vmcx_masked_rows bit for bit (Philox draws and injected uniforms)."""
+    from vmc_torch_amd import kernels
+    rng = np.random.default_rng(40)
+    B, N, M, H = 1000, 36, 180, 72
+    x = torch.tensor(rng.integers(0, 4, size=(B, N)), device=device)
+    prop = torch.tensor(rng.integers(0, 4, size=(B, N)), device=device)
+    move = torch.tensor(rng.integers(-1, M, size=B), dtype=torch.int32, device=device)
+    f64 = lambda *s: torch.tensor(rng.normal(size=s), device=device)   # noqa: E731
+    sign, logabs, ps, pl, lqf, lqr = (f64(B) for _ in range(6))
+    lx, lp = f64(B, M), f64(B, M)
+    z1, z1p = f64(B, H).float(), f64(B, H).float()
+    for u_acc in (None, torch.tensor(rng.random(B), device=device)):
+        a = [t.clone() for t in (x, sign, logabs, lx, z1)]
+        na = torch.zeros(B, dtype=torch.int64, device=device)
+        acc_a = kernels.lp_accept(a[0], a[1], a[2], move, prop, ps, pl, lqf, lqr, 5, 17,
+                                  u_acc=u_acc, n_accept=na)
+        kernels.lp_commit_logits(acc_a, lp, a[3])
+        kernels.masked_rows(acc_a, z1p, a[4])
+        b = [t.clone() for t in (x, sign, logabs, lx, z1)]
+        nb = torch.zeros(B, dtype=torch.int64, device=device)
+        acc_b = torch.empty(B, dtype=torch.uint8, device=device)
+        kernels.lp_accept_commit(b[0], b[1], b[2], move, prop, ps, pl, lqf, lqr, 5, 17, lp, b[3],
+                                 acc_b, n_accept=nb, u_acc=u_acc, z1p=z1p, z1=b[4])
+        assert 0 < int(acc_a.sum()) < B
+        assert torch.equal(acc_a, acc_b) and torch.equal(na, nb)
+        for ta, tb in zip(a, b):
+            assert torch.equal(ta, tb)

@@ -79,6 +79,8 @@ _SIGNATURES = {
                              _P]),
     "vmcx_lp_accept": (_I, [_I64, _I, _U64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                             _P, _P, _P]),
+    "vmcx_lp_accept_commit": (_I, [_I64, _I, _U64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                   _P, _P, _P, _I, _P, _P, _I64, _P, _P, _P]),
     "vmcx_lp_backward_workspace_bytes": (_I64, [_I, _I64]),
     "vmcx_lp_backward": (_I, [_P, _I, _I, _I, _I64, _P, _P, _P, _I64, _P, _P]),
     "vmcx_lp_candidates": (_I, [_I64, _I, _I, _P, _P, _P]),

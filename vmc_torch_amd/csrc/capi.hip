@@ -412,6 +412,19 @@ int vmcx_lp_propose_given(const float *params, int C, int Lx, int Ly, int64_t B,
                                           logq_rev, precision, (hipStream_t)stream, step_dev));
 }
 
+int vmcx_lp_accept_commit(int64_t B, int n_sites, uint64_t seed, int64_t step,
+                          const double *u_acc, const int32_t *move, const int64_t *proposed,
+                          const double *psign, const double *plogabs, const double *logq_fwd,
+                          const double *logq_rev, int64_t *configs, double *sign, double *logabs,
+                          int64_t *n_accept, uint8_t *accepted, const int64_t *step_dev,
+                          int n_moves, const double *logits_p, double *logits_x,
+                          int64_t row_bytes, const void *z1p, void *z1, void *stream) {
+  return status_of(vmcx::lp_accept_commit(B, n_sites, seed, step, u_acc, move, proposed, psign,
+                                          plogabs, logq_fwd, logq_rev, configs, sign, logabs,
+                                          n_accept, accepted, step_dev, n_moves, logits_p,
+                                          logits_x, row_bytes, z1p, z1, (hipStream_t)stream));
+}
+
 int vmcx_lp_commit_logits(int64_t B, int n_moves, const uint8_t *accepted,
                           const double *logits_p, double *logits_x, void *stream) {
   return status_of(vmcx::lp_commit_logits(B, n_moves, accepted, logits_p, logits_x,

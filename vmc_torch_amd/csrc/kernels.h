@@ -199,6 +199,13 @@ int bmps_logderiv(int Lx, int Ly, int D, int chi, const int32_t *dmap, const dou
                   double *G, int64_t ldg, double *sign, double *logabs, void *ws,
                   int64_t rows_per_pass, hipStream_t st);
 
+int lp_accept_commit(int64_t B, int N, uint64_t seed, int64_t step, const double *u_acc,
+                     const int32_t *move, const int64_t *pcfg, const double *psign,
+                     const double *plog, const double *logq_f, const double *logq_r,
+                     int64_t *cfg, double *sign, double *logabs, int64_t *n_accept,
+                     uint8_t *accepted, const int64_t *step_dev, int M, const double *logits_p,
+                     double *logits_x, int64_t row_bytes, const void *z1p, void *z1,
+                     hipStream_t st);
 int lp_n_params(int C);
 int lp_n_moves(int Lx, int Ly);
 int lp_logits(const float *P, int C, int Lx, int Ly, int64_t B, const int64_t *cfg,

@@ -543,6 +543,77 @@ __global__ void lp_accept_kernel(int64_t B, int N, uint64_t seed, int64_t step0,
   if (accepted) accepted[b] = 1;
 }
 
+// lp_accept_kernel, lp_commit_logits_kernel and a masked row copy in one
+// launch: one wave per walker, lane 0 takes the Metropolis-Hastings decision
+// (the same arithmetic and draw), then the wave copies the accepted walker's
+// configuration, logits row (logits_p -> logits_x) and state row (z1p -> z1,
+// row_bytes, a multiple of 4; z1 may be NULL)
+__global__ void __launch_bounds__(256)
+    lp_accept_commit_kernel(int64_t B, int N, uint64_t seed, int64_t step0,
+                            const double *__restrict__ u_acc, const int32_t *__restrict__ move,
+                            const int64_t *__restrict__ pcfg, const double *__restrict__ psign,
+                            const double *__restrict__ plog, const double *__restrict__ logq_f,
+                            const double *__restrict__ logq_r, int64_t *cfg, double *sign,
+                            double *logabs, int64_t *n_accept, uint8_t *accepted,
+                            const int64_t *__restrict__ step_dev, int M,
+                            const double *__restrict__ logits_p, double *logits_x,
+                            int64_t row_words, const uint32_t *__restrict__ z1p, uint32_t *z1) {
+  const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (b >= B) return;
+  int acc = 0;
+  if (lane == 0) {
+    const int64_t step = step0 + (step_dev ? step_dev[0] : 0);
+    if (move[b] >= 0) {
+      double u;
+      if (u_acc) {
+        u = u_acc[b];
+      } else {
+        const u32x4 r = philox4x32_10((uint32_t)step, (uint32_t)((uint64_t)step >> 32),
+                                      (uint32_t)b, 0x4C50u, (uint32_t)seed,
+                                      (uint32_t)(seed >> 32));
+        u = u53(r.z, r.w);
+      }
+      const double ratio = exp(2.0 * (plog[b] - logabs[b]) + logq_r[b] - logq_f[b]);
+      acc = u < ratio;
+    }
+    if (acc) {
+      sign[b] = psign[b];
+      logabs[b] = plog[b];
+      if (n_accept) n_accept[b] += 1;
+    }
+    if (accepted) accepted[b] = (uint8_t)acc;
+  }
+  acc = __shfl(acc, 0, 64);
+  if (!acc) return;
+  for (int s = lane; s < N; s += 64) cfg[b * N + s] = pcfg[b * N + s];
+  for (int m = lane; m < M; m += 64) logits_x[b * M + m] = logits_p[b * M + m];
+  if (z1)
+    for (int64_t k = lane; k < row_words; k += 64) z1[b * row_words + k] = z1p[b * row_words + k];
+}
+
+int lp_accept_commit(int64_t B, int N, uint64_t seed, int64_t step, const double *u_acc,
+                     const int32_t *move, const int64_t *pcfg, const double *psign,
+                     const double *plog, const double *logq_f, const double *logq_r,
+                     int64_t *cfg, double *sign, double *logabs, int64_t *n_accept,
+                     uint8_t *accepted, const int64_t *step_dev, int M, const double *logits_p,
+                     double *logits_x, int64_t row_bytes, const void *z1p, void *z1,
+                     hipStream_t st) {
+  VMCX_CHECK_ARG(B >= 0 && N >= 1 && N <= LP_NMAX && M >= 1, "bad sizes");
+  if (B == 0) return 0;
+  VMCX_CHECK_ARG(move && pcfg && psign && plog && logq_f && logq_r && cfg && sign && logabs &&
+                     logits_p && logits_x,
+                 "null pointer");
+  VMCX_CHECK_ARG(z1 == nullptr || (z1p != nullptr && row_bytes > 0 && row_bytes % 4 == 0),
+                 "state rows need z1p and a row size that is a multiple of 4 bytes");
+  hipLaunchKernelGGL(lp_accept_commit_kernel, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, st,
+                     B, N, seed, step, u_acc, move, pcfg, psign, plog, logq_f, logq_r, cfg, sign,
+                     logabs, n_accept, accepted, step_dev, M, logits_p, logits_x,
+                     z1 ? row_bytes / 4 : 0, (const uint32_t *)z1p, (uint32_t *)z1);
+  VMCX_LAUNCHED();
+  return 0;
+}
+
 // per-configuration gradient rows of sum_m dlogits[m] logits[m]
 __global__ void __launch_bounds__(LP_T)
     lp_backward_kernel(const float *__restrict__ P, int C, int Lx, int Ly, int64_t B,

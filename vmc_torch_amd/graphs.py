@@ -162,11 +162,10 @@ class LearnedProposalGraph:
                 ps, pl, z1p = m.forward_log_moved(fxs, prop, z1, out=I["mout"])
             else:
                 ps, pl = m.forward_log(prop)
-            kernels.lp_accept(fxs, sign, logabs, move, prop, ps, pl, lqf, lqr, s.seed, 0,
-                              n_accept=n_acc, step_dev=counter, accepted=I["acc"])
-            kernels.lp_commit_logits(I["acc"], lp, lx)
-            if I["moved"]:
-                kernels.masked_rows(I["acc"], z1p, z1)
+            kernels.lp_accept_commit(fxs, sign, logabs, move, prop, ps, pl, lqf, lqr, s.seed, 0,
+                                     lp, lx, I["acc"], n_accept=n_acc, step_dev=counter,
+                                     z1p=z1p if I["moved"] else None,
+                                     z1=z1 if I["moved"] else None)
             kernels.counter_add(counter, 1)
 
     def _warm(self):

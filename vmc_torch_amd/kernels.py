@@ -694,6 +694,33 @@ def lp_accept(configs, sign, logabs, move, proposed, psign, plogabs, logq_f, log
     return acc
 
 
+def lp_accept_commit(configs, sign, logabs, move, proposed, psign, plogabs, logq_f, logq_r,
+                     seed, step, logits_p, logits_x, accepted, n_accept=None, step_dev=None,
+                     u_acc=None, z1p=None, z1=None):
+    """lp_accept, lp_commit_logits(accepted, logits_p, logits_x) and, with z1
+    given, masked_rows(accepted, z1p, z1) in one launch (same results)."""
+    B, N = _check_configs(configs, configs.device)
+    dev = configs.device
+    M = logits_x.shape[1]
+    require(logits_p, F64, "logits_p", (B, M), dev)
+    require(accepted, U8, "accepted", (B,), dev)
+    if u_acc is not None:
+        require(u_acc, F64, "u_acc", (B,), dev)
+    row_bytes = 0
+    if z1 is not None:
+        if z1p is None or z1p.shape != z1.shape or z1p.dtype != z1.dtype or \
+                not (z1.is_contiguous() and z1p.is_contiguous()):
+            raise ValueError("lp_accept_commit: z1p and z1 must be contiguous and alike")
+        row_bytes = (z1.numel() // max(B, 1)) * z1.element_size()
+    check(lib().vmcx_lp_accept_commit(B, N, int(seed) & (2 ** 64 - 1), int(step), ptr(u_acc),
+                                      ptr(move), ptr(proposed), ptr(psign), ptr(plogabs),
+                                      ptr(logq_f), ptr(logq_r), ptr(configs), ptr(sign),
+                                      ptr(logabs), ptr(n_accept), ptr(accepted), ptr(step_dev),
+                                      int(M), ptr(logits_p), ptr(logits_x), int(row_bytes),
+                                      ptr(z1p), ptr(z1), stream_ptr(dev)))
+    return accepted
+
+
 def lp_backward(params, C, Lx, Ly, configs, dlogits):
     B, _ = _lp_check(params, C, configs, Lx, Ly)
     require(dlogits, F64, "dlogits", (B, lp_n_moves(Lx, Ly)), configs.device)

@@ -175,6 +175,8 @@ class LearnedProposalSamplerGPU(SamplerGPU):
             # from the proposed configurations of the accepted moves
             lx = self.k.lp_logits(self.params, self.C, self.Lx, self.Ly, fxs,
                                   precision=self.precision) if n_steps else None
+            acc = None
+            z1p = None
             for s in range(n_steps):
                 uc = ua = None
                 if draws is not None:
@@ -186,11 +188,12 @@ class LearnedProposalSamplerGPU(SamplerGPU):
                     ps, pl, z1p = model.forward_log_moved(fxs, prop, z1)
                 else:
                     ps, pl = model.forward_log(prop)
-                acc = self.k.lp_accept(fxs, sign, logabs, move, prop, ps, pl, lqf, lqr,
-                                       self.seed, self.step_counter, ua, n_acc)
-                self.k.lp_commit_logits(acc, lp, lx)
-                if moved:
-                    self.k.masked_rows(acc, z1p, z1)
+                if acc is None:
+                    acc = torch.empty(fxs.shape[0], dtype=torch.uint8, device=fxs.device)
+                self.k.lp_accept_commit(fxs, sign, logabs, move, prop, ps, pl, lqf, lqr,
+                                        self.seed, self.step_counter, lp, lx, acc,
+                                        n_accept=n_acc, u_acc=ua, z1p=z1p if moved else None,
+                                        z1=z1 if moved else None)
                 self.step_counter += 1
         if hasattr(model, "invalidate"):
             model.invalidate()      # the model's cached walker state predates the moves
