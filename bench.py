@@ -116,6 +116,8 @@ def parse():
                    help="NNBF: amplitudes per batched forward in the local energy")
     p.add_argument("--proposals-per-sample", type=int, default=0,
                    help="NNBF --sampler learned: proposals per sample (0 = |E| / 4)")
+    p.add_argument("--lp-moves", type=int, default=1, choices=[1, 2],
+                   help="NNBF --sampler learned: moves chained per proposal")
     p.add_argument("--sweeps-per-sample", type=int, default=1,
                    help="Slater: Metropolis sweeps between samples (1 = the reference's "
                         "one sample per sweep)")
@@ -456,7 +458,7 @@ class NnbfRun:
             from vmc_torch_amd.sampler import LearnedProposalSamplerGPU
             self.lp = LearnedProposalSamplerGPU(w["Lx"], w["Ly"], channels=32, seed=0x1EA,
                                                 generator=torch.Generator().manual_seed(0),
-                                                device=dev)
+                                                device=dev, moves_per_proposal=a.lp_moves)
             self.proposals_per_step = (a.proposals_per_sample if a.proposals_per_sample > 0
                                        else max(1, self.e_sweep.shape[0] // 4))
             self.n_steps = 0
@@ -900,6 +902,7 @@ def main():
                    "samples_per_sr_iter": Ns, "n_params": Np, "burn_in_sweeps": w["burn"],
                    "gram": gram, "diag_shift": w["shift"], "lr": w["lr"],
                    "sampler": a.sampler, "sweeps_per_sample": spp,
+                   **({"moves_per_proposal": a.lp_moves} if a.sampler == "learned" else {}),
                    "proposals_per_sample": getattr(run, "proposals_per_step", None),
                    "parallelism": "walkers on 1 GPU",
                    **({"environment_reuse": bool(a.peps_reuse)}

@@ -169,6 +169,37 @@ def test_chain_samples_psi_squared(device, C, precision):
     assert 0.5 * np.abs(emp - p).sum() < 0.02
 
 
+@pytest.mark.parametrize("C,precision", [(8, "fp32"), (32, "bf16")])
+def test_two_move_chain_samples_psi_squared(device, C, precision):
+    """moves_per_proposal = 2 (two chained moves, path acceptance): the 2x2
+    histogram equals |psi|^2 as with single moves; on 2x3 with two electrons
+    of each spin the composite proposals move more electrons at once."""
+    N = 4
+    model, M = _slater(N, 2, 11, device)
+    th, P = _net(C, 12, scale=0.3, head_bias=False)
+    s = LearnedProposalSamplerGPU(2, 2, channels=C, params=P.to(device), seed=6,
+                                  precision=precision, moves_per_proposal=2)
+    B = 4096
+    x0 = np.tile(np.array([[2, 1, 0, 0]], np.int64), (B, 1))
+    xd = torch.tensor(x0, device=device)
+    s.sweeps(xd, model, 200)                                 # burn-in
+    code = torch.tensor([64, 16, 4, 1], device=device)
+    hist = torch.zeros(256, dtype=torch.int64, device=device)
+    for _ in range(200):
+        s.sweeps(xd, model, 1)
+        hist += torch.bincount((xd * code).sum(1), minlength=256)
+    states = [np.array(c) for c in itertools.product(range(4), repeat=4)
+              if sum((v >> 1) & 1 for v in c) == 1 and sum(v & 1 for v in c) == 1]
+    p = np.array([np.exp(2 * osl.slogdet(M, c[None])[1][0]) for c in states])
+    p /= p.sum()
+    h = hist.cpu().numpy()
+    emp = np.array([h[int(c @ np.array([64, 16, 4, 1]))] for c in states], dtype=np.float64)
+    assert emp.sum() == 200 * B
+    emp /= emp.sum()
+    assert 0.5 * np.abs(emp - p).sum() < 0.02
+    assert (xd >= 0).all() and ((xd >> 1) & 1).sum(1).eq(1).all() and (xd & 1).sum(1).eq(1).all()
+
+
 def test_training_lowers_heat_bath_loss(device):
     Lx, Ly, N = 4, 4, 16
     model, _ = _slater(N, 8, 13, device)

@@ -119,6 +119,8 @@ class LearnedProposalGraph:
         if len(steps) != len(instances):
             raise ValueError("one step count per instance")
         for (sampler, model, fxs), k in zip(instances, steps):
+            if getattr(sampler, "moves_per_proposal", 1) != 1:
+                raise NotImplementedError("captured sweeps make one move per proposal")
             if not getattr(model, "graph_safe", False):
                 raise TypeError(f"{type(model).__name__} (this precision) launches library "
                                 "calls: not capturable")

@@ -113,10 +113,16 @@ class LearnedProposalSamplerGPU(SamplerGPU):
     exact in either precision: x and x' are scored by the same net.
     `step` (one sample, the sampler interface of the reference) makes
     `proposals_per_step` proposals; `burn_in(n)` makes n of them.
+    moves_per_proposal = 2: each proposal chains two moves, m1 from q(.|x)
+    and m2 from q(.|x1), so up to four electrons move at once; the
+    acceptance is that of the path, |psi(x2)|^2 q(m2'|x2) q(m1'|x1) /
+    (|psi(x)|^2 q(m1|x) q(m2|x1)) with m' the reverse moves (the reverse
+    path is the same two-move mechanism run from x2), which keeps |psi|^2
+    exact; the net runs at x1 and x2, the wavefunction only at x2.
     """
 
     def __init__(self, Lx, Ly, channels=32, seed=0, params=None, generator=None,
-                 device=None, precision=None, proposals_per_step=1):
+                 device=None, precision=None, proposals_per_step=1, moves_per_proposal=1):
         from . import kernels
         self.k = kernels
         self.Lx, self.Ly, self.C = int(Lx), int(Ly), int(channels)
@@ -128,6 +134,9 @@ class LearnedProposalSamplerGPU(SamplerGPU):
         self.proposals_per_step = int(proposals_per_step)
         if self.proposals_per_step < 1:
             raise ValueError("proposals_per_step must be >= 1")
+        self.moves_per_proposal = int(moves_per_proposal)
+        if self.moves_per_proposal not in (1, 2):
+            raise ValueError("moves_per_proposal must be 1 or 2")
         self.seed = int(seed)
         self.step_counter = 0
         self.last_acceptance = None
@@ -179,6 +188,33 @@ class LearnedProposalSamplerGPU(SamplerGPU):
             z1p = None
             for s in range(n_steps):
                 uc = ua = None
+                if self.moves_per_proposal == 2:
+                    # two chained moves: draws 2c (m1) and 2c + 1 (m2, and the
+                    # acceptance's other Philox words)
+                    uc2 = None
+                    if draws is not None:
+                        uc, uc2, ua = draws[s]
+                    c2 = 2 * self.step_counter
+                    m1, lqf1, x1, lqr1, lp1 = self.k.lp_propose_given(
+                        self.params, self.C, self.Lx, self.Ly, fxs, lx, self.seed, c2, uc,
+                        precision=self.precision)
+                    m2, lqf2, prop, lqr2, lp = self.k.lp_propose_given(
+                        self.params, self.C, self.Lx, self.Ly, x1, lp1, self.seed, c2 + 1, uc2,
+                        precision=self.precision)
+                    move = torch.where(m1 < 0, m1, m2)
+                    lqf, lqr = lqf1 + lqf2, lqr1 + lqr2
+                    if moved:
+                        ps, pl, z1p = model.forward_log_moved(fxs, prop, z1)
+                    else:
+                        ps, pl = model.forward_log(prop)
+                    if acc is None:
+                        acc = torch.empty(fxs.shape[0], dtype=torch.uint8, device=fxs.device)
+                    self.k.lp_accept_commit(fxs, sign, logabs, move, prop, ps, pl, lqf, lqr,
+                                            self.seed, c2 + 1, lp, lx, acc, n_accept=n_acc,
+                                            u_acc=ua, z1p=z1p if moved else None,
+                                            z1=z1 if moved else None)
+                    self.step_counter += 1
+                    continue
                 if draws is not None:
                     uc, ua = draws[s]
                 move, lqf, prop, lqr, lp = self.k.lp_propose_given(
