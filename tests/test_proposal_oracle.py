@@ -138,3 +138,56 @@ def test_bf16_forward_is_the_rounded_net():
     assert 0.0 < d < 2e-2 * max(1.0, np.abs(l32[v32]).max())
     for h in c16[2], c16[4], c16[6]:            # h1, h2, h3 are bfloat16 values
         assert np.array_equal(op.bf16_round(h), h)
+
+
+def _transition_matrix_two_moves(th, C, logamp, mistake=None):
+    """Two chained moves per proposal with the path acceptance
+    |psi(x2)|^2 q(m2|x2) q(m1|x1) / (|psi(x)|^2 q(m1|x) q(m2|x1)) (the moves
+    are their own reverses).  mistake="reverse_at_x2": both reverse terms
+    from the net at x2; "no_q": the bare |psi|^2 ratio."""
+    S = _states_2x2()
+    idx = {tuple(x): k for k, x in enumerate(S)}
+    P = np.zeros((len(S), len(S)))
+
+    def lq_at(x):
+        lf, vf, _ = op.forward(th, C, 2, 2, x)
+        return op.log_q(lf, vf), vf
+    for a, x in enumerate(S):
+        lq0, v0 = lq_at(x)
+        for m1 in np.nonzero(v0)[0]:
+            x1 = op.apply_move(x, 2, 2, m1)
+            lq1, v1 = lq_at(x1)
+            for m2 in np.nonzero(v1)[0]:
+                x2 = op.apply_move(x1, 2, 2, m2)
+                lq2, _ = lq_at(x2)
+                if mistake == "no_q":
+                    corr = 0.0
+                elif mistake == "reverse_at_x2":
+                    corr = lq2[m2] + lq2[m1] - lq0[m1] - lq1[m2]
+                else:
+                    corr = lq2[m2] + lq1[m1] - lq0[m1] - lq1[m2]
+                acc = min(1.0, np.exp(2 * (logamp(x2) - logamp(x)) + corr))
+                P[a, idx[tuple(x2)]] += np.exp(lq0[m1] + lq1[m2]) * acc
+        P[a, a] += 1.0 - P[a].sum()
+    return S, P
+
+
+def test_two_move_path_acceptance_leaves_psi_squared_stationary():
+    """The acceptance of the GPU sampler's moves_per_proposal = 2 leaves
+    |psi|^2 exactly stationary; the planted mistakes (reverse probabilities
+    at the wrong configuration, or none) do not."""
+    rng = np.random.default_rng(3)
+    M = rng.normal(size=(8, 2))
+    C = 4
+    th = op.init_params(rng, C, scale=3.0)
+
+    def logamp(x):
+        return osl.slogdet(M, np.asarray(x)[None])[1][0]
+    S, P = _transition_matrix_two_moves(th, C, logamp)
+    pi = np.array([np.exp(2 * logamp(x)) for x in S])
+    pi /= pi.sum()
+    assert np.abs(P.sum(1) - 1).max() < 1e-12
+    assert np.abs(pi @ P - pi).max() < 1e-12
+    for mistake in ("reverse_at_x2", "no_q"):
+        _, Pw = _transition_matrix_two_moves(th, C, logamp, mistake)
+        assert np.abs(pi @ Pw - pi).max() > 1e-4
