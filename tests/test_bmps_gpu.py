@@ -17,7 +17,10 @@ import torch
 from helpers import random_configs, tie_free_draws
 from oracle import fpeps as fp
 from oracle import hubbard as ohub
-from vmc_torch_amd import NNfTNS_Model_GPU, fPEPS_Model_GPU, fPEPS_Model_reuse_GPU
+from vmc_torch_amd import (MetropolisExchangeSpinfulSamplerGPU,
+                           MetropolisExchangeSpinfulSamplerReuse_GPU, NNfTNS_Model_GPU,
+                           evaluate_energy, evaluate_energy_reuse, fPEPS_Model_GPU,
+                           fPEPS_Model_reuse_GPU)
 from vmc_torch_amd.hamiltonian import spinful_Fermi_Hubbard_square_lattice_torch
 
 pytestmark = pytest.mark.gpu
@@ -268,8 +271,7 @@ def test_bmps_sweep_matches_oracle(device):
 def test_bmps_options(device):
     with pytest.raises(ValueError):
         fPEPS_Model_GPU(4, 4, 2, max_bond=3, amp="fp32")
-    with pytest.raises(NotImplementedError):
-        fPEPS_Model_reuse_GPU(4, 4, 2, max_bond=3)
+    assert fPEPS_Model_reuse_GPU(4, 4, 2, max_bond=3)._bmps   # boundary-cache reuse
     with pytest.raises(NotImplementedError):
         fPEPS_Model_GPU(6, 4, 4, max_bond=20)          # max(chi, D) * D > 64
     # chi at or above D^2 on Lx <= 4 keeps the exact kernels
@@ -324,3 +326,40 @@ def test_bmps_local_energy_cache_bit_identical(device, Lx, Ly, D, chi):
     m._struct_obj = None
     assert torch.isfinite(out[0]).all()
     assert torch.equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("Lx,Ly,D,chi", [(6, 4, 2, 3), (5, 4, 3, 5)])
+def test_bmps_reuse_model_equals_full_recontraction(device, Lx, Ly, D, chi):
+    """fPEPS_Model_reuse_GPU with a truncating max_bond: the reference-named
+    reuse sampler and evaluate_energy_reuse run from the walkers' cached
+    boundaries and give the plain model's full recontraction bit for bit
+    (walkers, signs, ln|psi|, acceptance counts, E_loc); the reuse model's
+    _full methods recontract every row and agree too."""
+    N = Lx * Ly
+    sh = fp.PEPSShape(Lx, Ly, D)
+    th = fp.random_theta(sh, np.random.default_rng(131 + Lx + D))
+    full = fPEPS_Model_GPU(Lx, Ly, D, theta=th, max_bond=chi).to(device)
+    full.bmps_cache = False
+    reuse = fPEPS_Model_reuse_GPU(Lx, Ly, D, theta=th, max_bond=chi).to(device)
+    assert reuse._bmps and reuse.bmps_cache
+    H = spinful_Fermi_Hubbard_square_lattice_torch(Lx, Ly, 1.0, 8.0, 2 * (N // 2),
+                                                   n_fermions_per_spin=(N // 2, N // 2))
+    x0 = torch.tensor(random_configs(np.random.default_rng(23), 48, N, N // 2, N // 2),
+                      device=device)
+    xa, xb = x0.clone(), x0.clone()
+    sa = MetropolisExchangeSpinfulSamplerGPU(seed=77)
+    sb = MetropolisExchangeSpinfulSamplerReuse_GPU(seed=77)
+    for _ in range(2):
+        _, (sga, la) = sa.step(xa, full, H.graph, use_log_amp=True)
+        _, (sgb, lb) = sb.step(xb, reuse, H.graph, use_log_amp=True)
+    assert torch.equal(xa, xb)
+    assert torch.equal(sga, sgb)
+    assert torch.equal(la, lb)
+    assert torch.equal(sa.last_acceptance[0], sb.last_acceptance[0])
+    assert int(sa.last_acceptance[0].sum()) > 0
+    Ea = evaluate_energy(xa, full, H)[1]
+    Eb = evaluate_energy_reuse(xb, reuse, H)[1]
+    assert torch.isfinite(Ea).all()
+    assert torch.equal(Ea, Eb)
+    assert torch.equal(reuse.local_energy_full(xb, H), Ea)
+    assert reuse.bmps_cache

@@ -756,21 +756,25 @@ class fPEPS_Model_reuse_GPU(fPEPS_Model_GPU):
     sweeps and local energies that evaluate each proposal or connected
     configuration from its walker's cached column environments, contracting
     only the one or two columns the move touches (vmcx_peps_sweep_reuse,
-    vmcx_peps_local_energy_reuse).  The reference caches boundary MPSs of a
-    chi-truncated contraction per row and column direction; here the
-    environments are the exact ones of the column-by-column contraction
-    (Lx <= 4, D <= 4), rebuilt incrementally as accepted moves change
-    columns.  `mcmc_sweep` and `local_energy` use the reuse path, so the
-    plain sampler and evaluate_energy get it too; the `_full` methods keep
-    the full contraction for comparison."""
+    vmcx_peps_local_energy_reuse).  In the exact regime (Lx <= 4, D <= 4,
+    max_bond <= 0 or >= D^2) the environments are the exact ones of the
+    column-by-column contraction, rebuilt incrementally as accepted moves
+    change columns.  With a truncating max_bond = chi the environments are
+    the walkers' cached boundary MPSs of both sides, row by row: a proposal
+    or connected configuration resumes each side from the boundary of the
+    rows before its first changed row (vmcx_peps_sweep / vmcx_peps_local_
+    energy with the boundary cache), which gives the full recontraction's
+    amplitude bit for bit; the reference's amplitude_reuse
+    (pureTNS.py:329-563) contracts the changed rows between a cached top and
+    bottom environment instead, a different truncation order.
+    `mcmc_sweep` and `local_energy` use the reuse path, so the plain sampler
+    and evaluate_energy get it too; the `_full` methods keep the full
+    contraction for comparison."""
 
     def __init__(self, Lx, Ly, D, theta=None, generator=None, dtype=torch.float64,
                  max_bond=-1, chunk=65536, **kwargs):
         if kwargs.get("amp", "fp64") != "fp64":
             raise ValueError("the environment-reuse entry points contract in FP64")
-        if uses_bmps(Lx, Ly, D, max_bond):
-            raise NotImplementedError("environment reuse keeps exact environments "
-                                      "(max_bond <= 0 or >= D^2, Lx <= 4)")
         super().__init__(Lx, Ly, D, theta=theta, generator=generator, dtype=dtype,
                          max_bond=max_bond, chunk=chunk)
         self.radius = 0
@@ -781,6 +785,9 @@ class fPEPS_Model_reuse_GPU(fPEPS_Model_GPU):
 
     def mcmc_sweep(self, fxs, edges, n_sweeps, seed, sweep_offset, hopping_rate=0.25,
                    refresh_every=64, draws=None):
+        if self._bmps:   # the walkers' cached boundaries
+            return fPEPS_Model_GPU.mcmc_sweep(self, fxs, edges, n_sweeps, seed, sweep_offset,
+                                              hopping_rate, refresh_every, draws)
         st = self.walker_state(fxs)
         B = fxs.shape[0]
         edges_host = edges.detach().to("cpu", torch.int32).contiguous()
@@ -798,6 +805,8 @@ class fPEPS_Model_reuse_GPU(fPEPS_Model_GPU):
         return st["sign"], st["logabs"], st["n_accept"]
 
     def local_energy(self, fxs, H):
+        if self._bmps:
+            return fPEPS_Model_GPU.local_energy(self, fxs, H)
         st = self.walker_state(fxs)
         B = fxs.shape[0]
         edges = H.graph.edge_tensor(fxs.device, "graph")
@@ -811,11 +820,23 @@ class fPEPS_Model_reuse_GPU(fPEPS_Model_GPU):
             kernels.stream_ptr(fxs.device)))
         return eloc
 
+    def _full(self, fn, *args, **kwargs):
+        if not self._bmps:
+            return fn(self, *args, **kwargs)
+        keep = self.bmps_cache
+        self.bmps_cache = False   # every row recontracted
+        self._struct_obj = None
+        try:
+            return fn(self, *args, **kwargs)
+        finally:
+            self.bmps_cache = keep
+            self._struct_obj = None
+
     def mcmc_sweep_full(self, *args, **kwargs):
-        return fPEPS_Model_GPU.mcmc_sweep(self, *args, **kwargs)
+        return self._full(fPEPS_Model_GPU.mcmc_sweep, *args, **kwargs)
 
     def local_energy_full(self, fxs, H):
-        return fPEPS_Model_GPU.local_energy(self, fxs, H)
+        return self._full(fPEPS_Model_GPU.local_energy, fxs, H)
 
     # the reference's names for the reuse path
     def forward_reuse_log(self, x, **kwargs):
