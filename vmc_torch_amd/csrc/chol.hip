@@ -232,10 +232,13 @@ __device__ __forceinline__ double leaf_inv(const LTile *L,
 
 __global__ void __launch_bounds__(DIAG_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4)))
     chol_diag_kernel(double *A, int64_t lda, int64_t k0, int nbk, int32_t *info,
-                     double *Dinv) {
+                     double *Dinv, int32_t *busy) {
   // the factorisation's critical path: its waves go first when the CU is
-  // shared with trailing-update workgroups
+  // shared with trailing-update workgroups, and `busy` names its CU to the
+  // trailing SYRKs, whose waves on this CU sleep until it is cleared
   __builtin_amdgcn_s_setprio(3);
+  if (busy && threadIdx.x == 0)
+    __hip_atomic_store(busy, cu_code(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __shared__ LTile L[NLT];                    // lower tiles of the block
   __shared__ double W[64];                    // leaf scratch
   __shared__ double rdiag_all[CHOL_NB];       // 1 / lt_at(L, i, i)
@@ -322,6 +325,7 @@ __global__ void __launch_bounds__(DIAG_THREADS) __attribute__((amdgpu_waves_per_
   }
   if (bad) {
     if (tid == 0 && info && info[0] == 0 && bad <= nbk) info[0] = (int32_t)(k0 + bad);
+    if (busy && tid == 0) __hip_atomic_store(busy, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
   // (L_kk itself is not stored: the solves and the panel use its inverse)
@@ -420,6 +424,7 @@ __global__ void __launch_bounds__(DIAG_THREADS) __attribute__((amdgpu_waves_per_
     }
     Dinv[idx] = xba;
   }
+  if (busy && tid == 0) __hip_atomic_store(busy, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // DinvT_k = Dinv_k^T for every block (grid: blocks x 64)
@@ -604,6 +609,7 @@ const double *chol_panels(int64_t n, const double *dinv) {
 struct LookAhead {
   hipStream_t side = nullptr;  // trailing SYRKs
   hipStream_t aux = nullptr;   // forward solve along the factorisation
+  int32_t *busy = nullptr;     // CU code of the running diagonal kernel, or 0
   std::vector<hipEvent_t> ev;
 };
 
@@ -618,6 +624,8 @@ static LookAhead *lookahead_for(hipStream_t caller, int nev) {
     LookAhead *n = new LookAhead();
     if (hipStreamCreateWithFlags(&n->side, hipStreamNonBlocking) != hipSuccess) return nullptr;
     if (hipStreamCreateWithFlags(&n->aux, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    if (hipMalloc((void **)&n->busy, 256) != hipSuccess) return nullptr;
+    if (hipMemset(n->busy, 0, 256) != hipSuccess) return nullptr;
     la = n;
   }
   while ((int)la->ev.size() < nev) {
@@ -683,7 +691,7 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
     double *Dk = Dinv + kb * CHOL_NB * CHOL_NB;
     if (diag_ms) VMCX_HIP(hipEventRecord(tev[2 * kb], st));
     hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(DIAG_THREADS), 0, st, A, lda, k0, nbk,
-                       info, Dk);
+                       info, Dk, la->busy);
     VMCX_LAUNCHED();
     if (diag_ms) VMCX_HIP(hipEventRecord(tev[2 * kb + 1], st));
     ++ndiag;
@@ -729,7 +737,8 @@ int chol_factor(void *handle, int64_t n, double *A, int64_t lda, double *dinv,
         VMCX_HIP(hipEventRecord(la->ev[1 + 2 * kb], st));
         VMCX_HIP(hipStreamWaitEvent(la->side, la->ev[1 + 2 * kb], 0));
         const int rc = syrk_f64(n - s0, G * CHOL_NB, W + (kb + 1 - G) * CHOL_NB * ldw + s0, ldw,
-                                0, -1.0, 1.0, A + s0 * lda + s0, lda, 1, 0, la->side);
+                                0, -1.0, 1.0, A + s0 * lda + s0, lda, 1, 0, la->side,
+                                nullptr, la->busy);
         if (rc) return rc;
         VMCX_HIP(hipEventRecord(la->ev[2 + 2 * kb], la->side));
         last_syrk = kb;

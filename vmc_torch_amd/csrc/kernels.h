@@ -79,9 +79,21 @@ int sr_minres(int64_t Ns, int64_t Np, const void *O, int o_dtype, int64_t ldo, c
               int64_t ws_bytes, double *dp, double *stats, int32_t *info, hipStream_t st);
 // splits >= 2: split 0 writes C, split s >= 1 writes Crest + (s - 1) cstride
 // (Crest == NULL: C + cstride, i.e. contiguous splits)
+// yield != NULL: a device word holding the CU code (cu_code below) of a running
+// diagonal factorisation kernel, or 0; waves of this SYRK on that CU sleep
+// while it runs (at most ~0.15 ms per chunk), leaving it the matrix pipes
 int syrk_f64(int64_t n, int64_t K, const double *X, int64_t ldx, int rowmajor, double alpha,
              double beta, double *C, int64_t ldc, int splits, int64_t cstride, hipStream_t st,
-             double *Crest = nullptr);
+             double *Crest = nullptr, const int32_t *yield = nullptr);
+#ifdef __HIPCC__
+// the calling wave's CU, never 0: XCC id (HW_REG_XCC_ID bits 0-3) and the
+// CU / SH / SE fields of HW_REG_HW_ID (bits 8-15)
+__device__ __forceinline__ int32_t cu_code() {
+  const int hw = __builtin_amdgcn_s_getreg((7 << 11) | (8 << 6) | 4);
+  const int xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);
+  return 1 + (((xcc & 15) << 8) | (hw & 255));
+}
+#endif
 int64_t chol_dinv_bytes(int64_t n);
 // the off-diagonal blocks of the factor (n x n, ld n) inside the dinv workspace
 const double *chol_panels(int64_t n, const double *dinv);

@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(SK_THREADS)
     __attribute__((amdgpu_waves_per_eu(SK_WC, SK_WC)))
     syrk_f64_kernel(int64_t n, int64_t K, const double *__restrict__ X, int64_t ldx,
                     int64_t kc, double alpha, double beta, double *C, int64_t ldc,
-                    double *Crest, int64_t cstride, int ntiles) {
+                    double *Crest, int64_t cstride, int ntiles, const int32_t *yield) {
   __shared__ double sR[2][SK_SLOTS];  // M rows of the tile's row block
   __shared__ double sC[2][SK_SLOTS];  // M rows of the tile's column block
   // XCD-aware order: consecutive workgroups go to different XCDs, so hand
@@ -163,6 +163,11 @@ __global__ void __launch_bounds__(SK_THREADS)
                            : 0.0;
       }
 
+  // yielding to a diagonal factorisation kernel on this CU: the flag is
+  // loaded beside each chunk's fetch and tested at the next chunk, so the
+  // test never waits on memory unless it sleeps
+  const int32_t mycu = yield ? cu_code() : 0;
+  int32_t yseen = 0;
   const int64_t nck = (ke > kb) ? (ke - kb + SK_KC - 1) / SK_KC : 0;
   SkStream<ROWMAJ> gr(X, ldx, n, r0, kb, tid), gc(X, ldx, n, c0, kb, tid);
   double vr[SK_PER], vc[SK_PER];
@@ -176,10 +181,17 @@ __global__ void __launch_bounds__(SK_THREADS)
   for (int64_t c = 0; c < nck; ++c) {
     const int buf = (int)(c & 1);
     const bool more = c + 1 < nck;
+    if (yield && yseen == mycu) {
+      for (int spin = 0; spin < 48; ++spin) {   // 48 x ~3.4 us at most
+        __builtin_amdgcn_s_sleep(127);
+        if (__hip_atomic_load(yield, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != mycu) break;
+      }
+    }
     if (more) {
       gr.fetch(ke, vr);
       if (!diag) gc.fetch(ke, vc);
     }
+    if (yield) yseen = __hip_atomic_load(yield, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (active) {
       const double *R = sR[buf];
       const double *Cm = diag ? sR[buf] : sC[buf];
@@ -224,7 +236,7 @@ __global__ void __launch_bounds__(SK_THREADS)
 
 int syrk_f64(int64_t n, int64_t K, const double *X, int64_t ldx, int rowmajor, double alpha,
              double beta, double *C, int64_t ldc, int splits, int64_t cstride, hipStream_t st,
-             double *Crest) {
+             double *Crest, const int32_t *yield) {
   VMCX_CHECK_ARG(n >= 0 && K >= 0, "n and K must be >= 0");
   if (n == 0) return 0;
   VMCX_CHECK_ARG(X != nullptr || K == 0, "X is NULL");
@@ -241,10 +253,11 @@ int syrk_f64(int64_t n, int64_t K, const double *X, int64_t ldx, int rowmajor, d
   const unsigned gx = (unsigned)(((ntiles + 7) / 8) * 8);
   if (rowmajor)
     hipLaunchKernelGGL(syrk_f64_kernel<true>, dim3(gx, (unsigned)splits), dim3(SK_THREADS), 0, st,
-                       n, K, X, ldx, kc, alpha, beta, C, ldc, Crest, cstride, ntiles);
+                       n, K, X, ldx, kc, alpha, beta, C, ldc, Crest, cstride, ntiles, yield);
   else
     hipLaunchKernelGGL(syrk_f64_kernel<false>, dim3(gx, (unsigned)splits), dim3(SK_THREADS), 0,
-                       st, n, K, X, ldx, kc, alpha, beta, C, ldc, Crest, cstride, ntiles);
+                       st, n, K, X, ldx, kc, alpha, beta, C, ldc, Crest, cstride, ntiles,
+                       yield);
   VMCX_LAUNCHED();
   return 0;
 }
