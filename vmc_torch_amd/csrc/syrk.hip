@@ -167,7 +167,8 @@ __global__ void __launch_bounds__(SK_THREADS)
   // loaded beside each chunk's fetch and tested at the next chunk, so the
   // test never waits on memory unless it sleeps
   const int32_t mycu = yield ? cu_code() : 0;
-  int32_t yseen = 0;
+  int32_t yseen =
+      yield ? __hip_atomic_load(yield, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
   const int64_t nck = (ke > kb) ? (ke - kb + SK_KC - 1) / SK_KC : 0;
   SkStream<ROWMAJ> gr(X, ldx, n, r0, kb, tid), gc(X, ldx, n, c0, kb, tid);
   double vr[SK_PER], vc[SK_PER];
