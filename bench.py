@@ -1040,7 +1040,7 @@ def slater_fp32_form(a, w, fxs0, dev, H, steps, warmup):
 
 
 def nnbf_target(a, dev, steps):
-    """The north-star comparison on 8x8 U=8 with the NNBF ansatz (128 -> 256
+    """The mixed-precision target comparison on 8x8 U=8 with the NNBF ansatz (128 -> 256
     -> 256 -> 8192, 2,212,352 parameters, 4096 walkers): the plain FP64 GPU
     form (FP64 network and determinant, plain sweep sampler) with the exact
     FP64 Gram formed from the factors of O, the same with O formed (FP64 Gram

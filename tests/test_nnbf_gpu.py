@@ -150,7 +150,7 @@ def test_nnbf_8x8_bf16_target_shape(device):
     1e-3 relative.  Against the FP64 oracle (the FP64 weights, no rounding)
     on 1024 walkers sampled from |psi|^2 (30 sweeps of the plain sampler
     from random configurations): the energy estimate per site within 1e-4
-    (the north-star bound |dE/site|); the per-sample differences are printed.
+    (the mixed-precision form's accuracy bound); the per-sample differences are printed.
     (On unsampled random configurations near a node of psi a single E_loc can
     move by O(1): the bound is on the estimate over sampled walkers.)"""
     Lx, Ly, nu, nd = 8, 8, 32, 32
