@@ -185,29 +185,30 @@ __device__ double bm_store_scaled(const BmGeo &g, double *slot, int y, int na, i
   return log(m);
 }
 
-// out (n x m, stride LD) = X (n x K, stride LD) * Y (K x m, stride LD); 4 x 4 tiles
-template <int LD>
+// out (n x m, stride LD) = X (n x K, stride LD) * Y (K x m, stride LD); TS x TS
+// tiles on a 16 x 16 thread grid (TS = 2 covers 32 x 32, TS = 4 covers 64 x 64)
+template <int LD, int TS>
 __device__ void bm_mm(int n, int m, int K, const double *X, const double *Y, double *out) {
   const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
-  const int i0 = 4 * ti, j0 = 4 * tj;
+  const int i0 = TS * ti, j0 = TS * tj;
   if (i0 >= n || j0 >= m) return;
-  double acc[4][4] = {};
+  double acc[TS][TS] = {};
   for (int k = 0; k < K; ++k) {
-    double xa[4], yb[4];
+    double xa[TS], yb[TS];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < TS; ++q) {
       xa[q] = (i0 + q < n) ? X[(i0 + q) * LD + k] : 0.0;
       yb[q] = (j0 + q < m) ? Y[k * LD + j0 + q] : 0.0;
     }
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < TS; ++a)
 #pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] = fma(xa[a], yb[b], acc[a][b]);
+      for (int b = 0; b < TS; ++b) acc[a][b] = fma(xa[a], yb[b], acc[a][b]);
   }
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < TS; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+    for (int b = 0; b < TS; ++b)
       if (i0 + a < n && j0 + b < m) out[(i0 + a) * LD + j0 + b] = acc[a][b];
 }
 
@@ -418,6 +419,7 @@ __global__ void __launch_bounds__(BT, NB == 32 ? 3 : 1)
                           const int64_t *__restrict__ chdr, const int32_t *__restrict__ parent,
                           const uint32_t *__restrict__ cmove) {
   constexpr int LD = BmShT<NB>::LD;
+  constexpr int TS = NB / 16;   // register tile of the dense products (16 x 16 threads)
   const int64_t w = blockIdx.x;
   if (w >= rows) return;
   if (flag && !(flag[w] & 1)) return;
@@ -539,8 +541,8 @@ __global__ void __launch_bounds__(BT, NB == 32 ? 3 : 1)
       bm_load_tensor(g, src, y, na, nb, S.Bt);
       bm_slice(g, cs, th, x, y, down, S.Sl);
       __syncthreads();
-      const int ti = tid >> 4, tj = tid & 15, i0 = 4 * ti, j0 = 4 * tj;
-      double acc[4][4] = {};
+      const int ti = tid >> 4, tj = tid & 15, i0 = TS * ti, j0 = TS * tj;
+      double acc[TS][TS] = {};
       for (int v = 0; v < D; ++v) {
         for (int idx = tid; idx < n * m; idx += BT) {
           const int i = idx / m, j = idx - i * m;
@@ -551,36 +553,36 @@ __global__ void __launch_bounds__(BT, NB == 32 ? 3 : 1)
           S.A[i * LD + j] = s;
         }
         __syncthreads();
-        bm_mm<LD>(n, m, m, S.A, S.R, S.T);
+        bm_mm<LD, TS>(n, m, m, S.A, S.R, S.T);
         __syncthreads();
         if (i0 < n && j0 < n) {
           for (int k = 0; k < m; ++k) {
-            double xa[4], yb[4];
+            double xa[TS], yb[TS];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < TS; ++q) {
               xa[q] = (i0 + q < n) ? S.T[(i0 + q) * LD + k] : 0.0;
               yb[q] = (j0 + q < n) ? S.A[(j0 + q) * LD + k] : 0.0;
             }
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int a = 0; a < TS; ++a)
 #pragma unroll
-              for (int b = 0; b < 4; ++b) acc[a][b] = fma(xa[a], yb[b], acc[a][b]);
+              for (int b = 0; b < TS; ++b) acc[a][b] = fma(xa[a], yb[b], acc[a][b]);
           }
         }
         __syncthreads();
       }
       double mx = 0.0;
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < TS; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) mx = fmax(mx, fabs(acc[a][b]));
+        for (int b = 0; b < TS; ++b) mx = fmax(mx, fabs(acc[a][b]));
       mx = bm_max(mx, S.red);
       const double inv = mx > 0.0 ? 1.0 / mx : 0.0;
       double *Rd = Rg + (int64_t)y * BN * BN;
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < TS; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int b = 0; b < TS; ++b)
           if (i0 + a < n && j0 + b < n) {
             const double v = acc[a][b] * inv;
             S.R[(i0 + a) * LD + j0 + b] = v;
@@ -684,7 +686,7 @@ __global__ void __launch_bounds__(BT, NB == 32 ? 3 : 1)
         __syncthreads();
         const int m0 = S.sec[0], m1 = S.sec[1], mt = m0 + m1;
         // G = C R (into T)
-        bm_mm<LD>(nC, m, m, S.A, S.R, S.T);
+        bm_mm<LD, TS>(nC, m, m, S.A, S.R, S.T);
         __syncthreads();
         // rho (permuted, block diagonal) = G C^T (into R)
         for (int idx = tid; idx < mt * mt; idx += BT) {
