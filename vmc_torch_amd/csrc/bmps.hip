@@ -142,9 +142,13 @@ struct BmShT {
   double R[NB * LD];        // right Gram; then rho
   double A[NB * LD];        // A_v; then C
   double T[NB * LD];        // A_v R; then W, G, V
-  double Bt[NB * 32];       // the boundary tensor of the current site (cb * D * cb)
+  // the boundary tensor of the current site (na x D x nb) and the projection
+  // carried from the left (kept-bond rows): bonds never exceed NB / 2 here
+  // (NB = 32 holds cb * D <= 32, so bonds <= 16 for D >= 2 and stay 1 for
+  // D = 1; NB = 64 holds chi <= 32)
+  double Bt[NB * (NB / 2)];
   double Sl[256];           // its slice [l][f][o][r]
-  double P[32 * NB];        // projection carried from the left (chi <= 32 rows)
+  double P[(NB / 2) * NB];
   double jc[NB / 2 + 1], js[NB / 2 + 1], jt[NB / 2 + 1];
   int jp[NB / 2 + 1], jq[NB / 2 + 1];
   int perm[NB], pos[NB], sel[NB];
@@ -412,7 +416,7 @@ __device__ __forceinline__ void bm_move_starts(const BmGeo &g, uint32_t mv, int 
 // bottom): the environments of every row (quimb's compute_x_environments);
 // with g.nbuf = 1 it stops at the middle and works in place.
 template <int NB>
-__global__ void __launch_bounds__(BT, NB == 32 ? 3 : 1)
+__global__ void __launch_bounds__(BT, NB == 32 ? 4 : 1)
     peps_bmps_side_kernel(BmGeo g, int64_t rows, const int64_t *__restrict__ cfg,
                           const int32_t *__restrict__ flag, double *ws, int64_t *hdr,
                           int st_top, int st_bot, const double *__restrict__ cache,
