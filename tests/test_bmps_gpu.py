@@ -71,10 +71,11 @@ def test_bmps_forward_matches_oracle(device, Lx, Ly, D, chi):
     _check(s, l, so, lo, go)
 
 
-@pytest.mark.parametrize("Lx,Ly,D,chi", [(6, 3, 2, 8), (5, 2, 3, 9), (6, 4, 1, 1)])
+@pytest.mark.parametrize("Lx,Ly,D,chi", [(6, 3, 2, 8), (5, 2, 3, 9), (6, 4, 1, 1), (6, 4, 1, 32)])
 def test_bmps_untruncated_equals_exact_contraction(device, Lx, Ly, D, chi):
     """Lattices beyond the exact kernels with chi at or above every bond:
-    the boundary MPS equals the exact zipper contraction."""
+    the boundary MPS equals the exact zipper contraction (D = 1, chi = 32:
+    buffers laid out for bonds of 32, bonds staying 1)."""
     m, sh, th = _plain(Lx, Ly, D, chi, 5 + Lx + Ly, device)
     rng = np.random.default_rng(4)
     if D == 1:
@@ -151,7 +152,10 @@ def test_bmps_local_energy_matches_oracle(device):
     assert np.allclose(E[good], Eo[good], rtol=1e-8, atol=1e-8 * np.abs(Eo).max())
 
 
-@pytest.mark.parametrize("Lx,Ly,D,chi", [(4, 5, 2, 3), (6, 4, 2, 3), (5, 4, 3, 5), (6, 6, 4, 16)])
+# (6, 8, 2, 16): every row environment of both sides, bonds up to 16 = the
+# largest boundary tensor and projection of the 32 x 32 working matrices
+@pytest.mark.parametrize("Lx,Ly,D,chi", [(4, 5, 2, 3), (6, 4, 2, 3), (5, 4, 3, 5), (6, 6, 4, 16),
+                                         (6, 8, 2, 16)])
 def test_bmps_log_derivative_matches_oracle(device, Lx, Ly, D, chi):
     """The reference's cheap gradient (environments held fixed) against the
     oracle's, entry by entry (1e-8 of the row's largest), dense and as
