@@ -304,7 +304,8 @@ __device__ void bm_jacobi(BmShT<NB> &S, int m0, int m1) {
           q += poff;
           const double app = S.R[p * LD + p], aqq = S.R[q * LD + q];
           const double apq = S.R[(p < q ? p : q) * LD + (p < q ? q : p)];
-          if (fabs(apq) > fmax(1e-15 * sqrt(fabs(app * aqq)), BM_ROT_FLOOR)) {
+          // |a_pq| > max(1e-15 sqrt|a_pp a_qq|, floor), compared in squares (no square root)
+          if (apq * apq > fmax(1e-30 * fabs(app * aqq), BM_ROT_FLOOR * BM_ROT_FLOOR)) {
             // t = sgn(theta) / (|theta| + sqrt(theta^2 + 1)), theta = tau / zeta, written
             // with one square root and one division (|t| <= 1); c = 1 / sqrt(1 + t^2)
             // by the hardware estimate and three Newton steps (the estimate's 2^-22
