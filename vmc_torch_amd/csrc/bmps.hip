@@ -149,8 +149,9 @@ struct BmShT {
   double Bt[NB * (NB / 2)];
   double Sl[256];           // its slice [l][f][o][r]
   double P[(NB / 2) * NB];
-  double jc[NB / 2 + 1], js[NB / 2 + 1], jt[NB / 2 + 1];
-  int jp[NB / 2 + 1], jq[NB / 2 + 1];
+  double2 jcs[NB / 2 + 1];  // rotation (c, s) of each pair slot: one 16-byte load
+  double jt[NB / 2 + 1];
+  int jpq[NB / 2 + 1];      // its rows p | q << 8
   int perm[NB], pos[NB], sel[NB];
   int kpar[NB];             // parity of the current left bond's indices
   int bd[BMAX_LY + 1], nbd[BMAX_LY + 1];
@@ -322,10 +323,8 @@ __device__ void bm_jacobi(BmShT<NB> &S, int m0, int m1) {
           }
           if (++rr == pmb - 1) rr = 0;
         }
-        S.jp[tid] = p;
-        S.jq[tid] = q;
-        S.jc[tid] = c;
-        S.js[tid] = s;
+        S.jpq[tid] = p | (q << 8);
+        S.jcs[tid] = make_double2(c, s);
         S.jt[tid] = t;
       }
       __syncthreads();
@@ -334,9 +333,11 @@ __device__ void bm_jacobi(BmShT<NB> &S, int m0, int m1) {
       for (int k = 0; k < NBK; ++k) {
         const int i1 = bk1[k], i2 = bk2[k];
         if (i1 < 0) continue;
-        const double s1 = S.js[i1], s2 = S.js[i2];
+        const double2 cs1 = S.jcs[i1], cs2 = S.jcs[i2];
+        const double s1 = cs1.y, s2 = cs2.y;
         if (s1 == 0.0 && s2 == 0.0) continue;
-        const int p1 = S.jp[i1], q1 = S.jq[i1];
+        const int pq1 = S.jpq[i1];
+        const int p1 = pq1 & 255, q1 = pq1 >> 8;
         if (i1 == i2) {
           // the rotated pair itself: its off-diagonal entry becomes zero
           const double t = S.jt[i1];
@@ -347,8 +348,9 @@ __device__ void bm_jacobi(BmShT<NB> &S, int m0, int m1) {
           S.R[lo * LD + hi] = 0.0;
           continue;
         }
-        const int p2 = S.jp[i2], q2 = S.jq[i2];
-        const double c1 = S.jc[i1], c2 = S.jc[i2];
+        const int pq2 = S.jpq[i2];
+        const int p2 = pq2 & 255, q2 = pq2 >> 8;
+        const double c1 = cs1.x, c2 = cs2.x;
         // entries of distinct pairs: never on the diagonal
         const int e0 = p1 < p2 ? p1 * LD + p2 : p2 * LD + p1;
         const int e1 = p1 < q2 ? p1 * LD + q2 : q2 * LD + p1;
@@ -369,10 +371,12 @@ __device__ void bm_jacobi(BmShT<NB> &S, int m0, int m1) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
           const bool on = vrow[k] >= 0;
-          ss[k] = on ? S.js[vpi[k]] : 0.0;
-          cc[k] = on ? S.jc[vpi[k]] : 1.0;
-          ip[k] = on ? vrow[k] * LD + S.jp[vpi[k]] : 0;
-          iq[k] = on ? vrow[k] * LD + S.jq[vpi[k]] : 0;
+          const double2 cs = on ? S.jcs[vpi[k]] : make_double2(1.0, 0.0);
+          const int pq = on ? S.jpq[vpi[k]] : 0;
+          ss[k] = cs.y;
+          cc[k] = cs.x;
+          ip[k] = on ? vrow[k] * LD + (pq & 255) : 0;
+          iq[k] = on ? vrow[k] * LD + (pq >> 8) : 0;
         }
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
